@@ -1,0 +1,682 @@
+// Synthetic parts for the exact signed-distance evaluator, written directly in
+// the evaluator's data model (include/icam_part.h): faces on exact surfaces with
+// their trim loops given as monotone polynomial cubics in the face's parameter
+// domain, edges with the faces that share them, vertices with angle-weighted
+// pseudo-normals.
+//
+// The reference builds such a solid from a STEP file (Solid3::from_step,
+// crates/icam-core/src/solid3.rs:597-851) and fits and certifies the trim loops
+// (trim2::build, crates/icam-core/src/trim2.rs:853-1090). Here every loop is
+// known in closed form: iso-parameter lines are exact cubics, a NURBS boundary
+// curve seen in a side plane is an exact polynomial cubic per knot span (unit
+// boundary weights, control points at the Greville abscissae, so the plane's
+// first coordinate is linear in the knot parameter), and a circle in a plane is
+// approximated by 16 cubic pieces a quadrant (radial deviation below 1e-10 of
+// the radius).
+//
+// Parts:
+//   "can"   solid cylinder r 3, z 0..5 (the reference's `can`, solid3.rs:2013-2040)
+//   "bump"  10 x 10 box with a biquadratic B-spline top (`bump_box`, solid3.rs:2094-2143)
+//   "c1"    box 100 x 60 x 40 minus a through hole r 10 along z with a torus
+//           fillet r 3 on the hole's top edge
+//   "c2"    120 x 120 block, top a rational bicubic NURBS, 16 x 16 net, clamped
+//           uniform knots, heights from seeded 3-octave value noise scaled to 5 mm,
+//           interior weights uniform in [0.8, 1.2]
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "../../../include/icam_part.h"
+
+namespace {
+
+struct V {
+  double x, y, z;
+};
+inline V operator+(V a, V b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V operator-(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V operator*(V a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V cross(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double len(V a) { return std::sqrt(dot(a, a)); }
+inline V unit(V a) { return a * (1.0 / len(a)); }
+
+// xorshift64 as the reference's checks seed it (crates/icam-check/src/exact_check.rs:46-53).
+struct Rng {
+  uint64_t s;
+  double next() {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    return double(s >> 11) / double(uint64_t(1) << 53);
+  }
+};
+
+struct Owned {
+  icam_part_desc desc;  // first member: the handle handed out is this
+  std::vector<icam_face_desc> faces;
+  std::vector<icam_edge_desc> edges;
+  std::vector<icam_vert_desc> verts;
+  std::deque<std::vector<double>> arrays;
+
+  const double* keep(std::vector<double> v) {
+    arrays.push_back(std::move(v));
+    return arrays.back().data();
+  }
+  void finish(V lo, V hi, double tol) {
+    std::memset(&desc, 0, sizeof(desc));
+    desc.n_faces = uint32_t(faces.size());
+    desc.n_edges = uint32_t(edges.size());
+    desc.n_verts = uint32_t(verts.size());
+    desc.faces = faces.data();
+    desc.edges = edges.data();
+    desc.verts = verts.data();
+    desc.lo[0] = lo.x, desc.lo[1] = lo.y, desc.lo[2] = lo.z;
+    desc.hi[0] = hi.x, desc.hi[1] = hi.y, desc.hi[2] = hi.z;
+    desc.tol = tol;
+  }
+};
+
+void put3(double* d, V v) { d[0] = v.x, d[1] = v.y, d[2] = v.z; }
+
+// ------------------------------------------------------------- trim pieces
+
+using Piece = std::array<double, 8>;
+
+Piece line_piece(double u0, double v0, double u1, double v1) {
+  return {u0, v0, u0 + (u1 - u0) / 3.0, v0 + (v1 - v0) / 3.0, u0 + 2.0 * (u1 - u0) / 3.0, v0 + 2.0 * (v1 - v0) / 3.0, u1, v1};
+}
+
+void add_polyline(std::vector<Piece>& out, const std::vector<std::array<double, 2>>& pts, bool closed) {
+  size_t n = pts.size();
+  for (size_t i = 0; i + 1 < n + (closed ? 1 : 0); ++i) {
+    auto a = pts[i], b = pts[(i + 1) % n];
+    out.push_back(line_piece(a[0], a[1], b[0], b[1]));
+  }
+}
+
+// A circle (cu, cv, r) in a plane's parameters, 16 cubic pieces a quadrant, each
+// monotone in both parameters (the quadrant ends are piece ends).
+void add_circle(std::vector<Piece>& out, double cu, double cv, double r) {
+  const int n = 64;
+  for (int i = 0; i < n; ++i) {
+    double a = 2.0 * M_PI * i / n, b = 2.0 * M_PI * (i + 1) / n;
+    double k = 4.0 / 3.0 * std::tan((b - a) / 4.0);
+    double p0u = cu + r * std::cos(a), p0v = cv + r * std::sin(a);
+    double p3u = cu + r * std::cos(b), p3v = cv + r * std::sin(b);
+    double p1u = p0u - k * r * std::sin(a), p1v = p0v + k * r * std::cos(a);
+    double p2u = p3u + k * r * std::sin(b), p2v = p3v - k * r * std::cos(b);
+    out.push_back({p0u, p0v, p1u, p1v, p2u, p2v, p3u, p3v});
+  }
+}
+
+// de Casteljau split of a 2-D cubic at t.
+void split2(const Piece& c, double t, Piece& l, Piece& r) {
+  double b[4][2];
+  for (int i = 0; i < 4; ++i) b[i][0] = c[2 * i], b[i][1] = c[2 * i + 1];
+  double L[4][2], R[4][2];
+  L[0][0] = b[0][0], L[0][1] = b[0][1];
+  R[3][0] = b[3][0], R[3][1] = b[3][1];
+  for (int lev = 1; lev <= 3; ++lev) {
+    for (int i = 0; i <= 3 - lev; ++i)
+      for (int k = 0; k < 2; ++k) b[i][k] = (1 - t) * b[i][k] + t * b[i + 1][k];
+    L[lev][0] = b[0][0], L[lev][1] = b[0][1];
+    R[3 - lev][0] = b[3 - lev][0], R[3 - lev][1] = b[3 - lev][1];
+  }
+  for (int i = 0; i < 4; ++i) l[2 * i] = L[i][0], l[2 * i + 1] = L[i][1], r[2 * i] = R[i][0], r[2 * i + 1] = R[i][1];
+}
+
+// Roots in (0, 1) of the derivative of one coordinate of a cubic.
+void deriv_roots(const Piece& c, int k, std::vector<double>& ts) {
+  double p0 = c[k], p1 = c[2 + k], p2 = c[4 + k], p3 = c[6 + k];
+  // derivative / 3 = (p1-p0)(1-t)^2 + 2(p2-p1)t(1-t) + (p3-p2)t^2 = A t^2 + B t + C
+  double d0 = p1 - p0, d1 = p2 - p1, d2 = p3 - p2;
+  double A = d0 - 2 * d1 + d2, B = 2 * (d1 - d0), C = d0;
+  auto keep = [&](double t) {
+    if (t > 1e-12 && t < 1 - 1e-12) ts.push_back(t);
+  };
+  if (std::fabs(A) < 1e-14 * (std::fabs(B) + std::fabs(C) + 1e-300)) {
+    if (B != 0) keep(-C / B);
+    return;
+  }
+  double disc = B * B - 4 * A * C;
+  if (disc < 0) return;
+  double s = std::sqrt(disc);
+  double q = -0.5 * (B + (B >= 0 ? s : -s));
+  if (q != 0) keep(C / q);
+  keep(q / A);
+}
+
+// A cubic cut into pieces monotone in both coordinates.
+void add_monotone(std::vector<Piece>& out, const Piece& c) {
+  std::vector<double> ts;
+  deriv_roots(c, 0, ts);
+  deriv_roots(c, 1, ts);
+  std::sort(ts.begin(), ts.end());
+  Piece rest = c;
+  double done = 0.0;
+  for (double t : ts) {
+    if (t - done < 1e-9) continue;
+    double local = (t - done) / (1.0 - done);
+    Piece l, r;
+    split2(rest, local, l, r);
+    out.push_back(l);
+    rest = r;
+    done = t;
+  }
+  out.push_back(rest);
+}
+
+// ------------------------------------------------------------- B-spline helpers
+
+// Bezier pieces of a clamped B-spline of degree p (points of dimension N, flat),
+// by inserting every interior knot until it has multiplicity p.
+std::vector<std::vector<double>> decompose(int p, std::vector<double> kn, std::vector<double> pts, int N) {
+  auto count = [&](double u) {
+    int c = 0;
+    for (double k : kn) c += (k == u);
+    return c;
+  };
+  std::vector<double> interior;
+  for (size_t i = p + 1; i + p + 1 < kn.size(); ++i)
+    if (interior.empty() || interior.back() != kn[i]) interior.push_back(kn[i]);
+  for (double u : interior) {
+    while (count(u) < p) {
+      // The NURBS Book algorithm 5.1, one insertion.
+      int k = 0;
+      for (int i = int(kn.size()) - 2; i >= 0; --i)
+        if (kn[i] <= u && kn[i + 1] > u) {
+          k = i;
+          break;
+        }
+      int s = 0;
+      for (int i = k; i >= 0 && kn[i] == u; --i) ++s;
+      int np = int(pts.size()) / N;
+      std::vector<double> q;
+      for (int i = 0; i <= k - p; ++i)
+        for (int c = 0; c < N; ++c) q.push_back(pts[i * N + c]);
+      for (int i = k - p + 1; i <= k - s; ++i) {
+        double a = (u - kn[i]) / (kn[i + p] - kn[i]);
+        for (int c = 0; c < N; ++c) q.push_back(a * pts[i * N + c] + (1 - a) * pts[(i - 1) * N + c]);
+      }
+      for (int i = k - s; i < np; ++i)
+        for (int c = 0; c < N; ++c) q.push_back(pts[i * N + c]);
+      kn.insert(kn.begin() + k + 1, u);
+      pts = q;
+    }
+  }
+  std::vector<std::vector<double>> out;
+  int np = int(pts.size()) / N;
+  for (int s = 0; s + p < np; s += p) out.emplace_back(pts.begin() + s * N, pts.begin() + (s + p + 1) * N);
+  return out;
+}
+
+// Degree elevation of a 2-D Bezier piece to a cubic (degree 1, 2 or 3).
+Piece to_cubic2(const std::vector<double>& c, int deg) {
+  if (deg == 3) return {c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7]};
+  if (deg == 1) return line_piece(c[0], c[1], c[2], c[3]);
+  // quadratic
+  return {c[0], c[1], c[0] / 3 + 2 * c[2] / 3, c[1] / 3 + 2 * c[3] / 3, 2 * c[2] / 3 + c[4] / 3, 2 * c[3] / 3 + c[5] / 3, c[4], c[5]};
+}
+
+// ------------------------------------------------------------- builders
+
+struct Builder {
+  Owned* o;
+
+  uint32_t plane(V org, V z, V x, double sense, double lo0, double lo1, double hi0, double hi1, const std::vector<Piece>& pcs, double r0, double r1,
+                 bool rin) {
+    icam_face_desc f;
+    std::memset(&f, 0, sizeof(f));
+    f.kind = ICAM_PLANE;
+    f.sense = sense;
+    put3(f.o, org), put3(f.x, x), put3(f.z, z);
+    set_trim(f, lo0, lo1, hi0, hi1, 0, 0, pcs, r0, r1, rin);
+    o->faces.push_back(f);
+    return uint32_t(o->faces.size() - 1);
+  }
+  uint32_t analytic(uint32_t kind, V org, V z, V x, double c0, double c1, double sense, double lo0, double lo1, double hi0, double hi1, double per0,
+                    double per1, const std::vector<Piece>& pcs, double r0, double r1, bool rin) {
+    icam_face_desc f;
+    std::memset(&f, 0, sizeof(f));
+    f.kind = kind;
+    f.sense = sense;
+    put3(f.o, org), put3(f.x, x), put3(f.z, z);
+    f.c[0] = c0, f.c[1] = c1;
+    set_trim(f, lo0, lo1, hi0, hi1, per0, per1, pcs, r0, r1, rin);
+    o->faces.push_back(f);
+    return uint32_t(o->faces.size() - 1);
+  }
+  void set_trim(icam_face_desc& f, double lo0, double lo1, double hi0, double hi1, double per0, double per1, const std::vector<Piece>& pcs, double r0,
+                double r1, bool rin) {
+    f.lo[0] = lo0, f.lo[1] = lo1, f.hi[0] = hi0, f.hi[1] = hi1;
+    f.period[0] = per0, f.period[1] = per1;
+    std::vector<double> flat;
+    for (auto& p : pcs) flat.insert(flat.end(), p.begin(), p.end());
+    f.n_pieces = uint32_t(pcs.size());
+    f.pieces = pcs.empty() ? nullptr : o->keep(flat);
+    f.ref[0] = r0, f.ref[1] = r1;
+    f.ref_inside = rin ? 1u : 0u;
+  }
+  void line(V a, V b, uint32_t f0, uint32_t f1, const double* uvmap = nullptr) {
+    icam_edge_desc e;
+    std::memset(&e, 0, sizeof(e));
+    e.kind = ICAM_LINE;
+    put3(e.a, a), put3(e.b, b);
+    e.n_pieces = 1;
+    e.faces[0] = f0, e.faces[1] = f1;
+    e.uvmap = uvmap;
+    o->edges.push_back(e);
+  }
+  void circle(V org, V z, V x, double r, uint32_t f0, uint32_t f1) {
+    icam_edge_desc e;
+    std::memset(&e, 0, sizeof(e));
+    e.kind = ICAM_ARC;
+    put3(e.o, org), put3(e.x, x), put3(e.z, z);
+    e.r = r, e.t0 = 0.0, e.sweep = 2.0 * M_PI;
+    e.n_pieces = 1;
+    e.faces[0] = f0, e.faces[1] = f1;
+    o->edges.push_back(e);
+  }
+  void spline(uint32_t degree, const std::vector<std::vector<double>>& pieces, uint32_t f0, uint32_t f1, const std::vector<double>& uvmap) {
+    icam_edge_desc e;
+    std::memset(&e, 0, sizeof(e));
+    e.kind = ICAM_SPLINE;
+    e.degree = degree;
+    e.n_pieces = uint32_t(pieces.size());
+    std::vector<double> flat;
+    for (auto& p : pieces) flat.insert(flat.end(), p.begin(), p.end());
+    e.pw = o->keep(flat);
+    e.faces[0] = f0, e.faces[1] = f1;
+    e.uvmap = uvmap.empty() ? nullptr : o->keep(uvmap);
+    o->edges.push_back(e);
+  }
+  void vert(V p, V pn) {
+    icam_vert_desc v;
+    put3(v.p, p), put3(v.pn, pn);
+    o->verts.push_back(v);
+  }
+};
+
+const double NaN = std::nan("");
+
+// ------------------------------------------------------------- can
+
+Owned* make_can() {
+  Owned* o = new Owned();
+  Builder b{o};
+  V X{1, 0, 0}, Y{0, 1, 0}, Z{0, 0, 1};
+  (void)Y;
+  std::vector<Piece> disc;
+  add_circle(disc, 0, 0, 3.0);
+  // bottom: the plane's normal +z, outward -z
+  uint32_t bot = b.plane({0, 0, 0}, Z, X, -1.0, -4, -4, 4, 4, disc, -3.5, 0.0123, false);
+  uint32_t top = b.plane({0, 0, 5}, Z, X, 1.0, -4, -4, 4, 4, disc, -3.5, 0.0123, false);
+  std::vector<Piece> band = {line_piece(0, 0, 4, 0), line_piece(0, 5, 4, 5)};
+  uint32_t cyl = b.analytic(ICAM_CYLINDER, {0, 0, 0}, Z, X, 3.0, 0, 1.0, 0, -1, 4, 6, 4, 0, band, 2.0, 2.5, true);
+  b.circle({0, 0, 0}, Z, X, 3.0, bot, cyl);
+  b.circle({0, 0, 5}, Z, X, 3.0, top, cyl);
+  o->finish({-3, -3, 0}, {3, 3, 5}, 1e-9);
+  return o;
+}
+
+// ------------------------------------------------------------- bump box
+
+Owned* make_bump() {
+  Owned* o = new Owned();
+  Builder b{o};
+  const double zc[3][3] = {{5, 5, 5}, {7, 9, 5}, {5, 6, 5}};  // zc[i][j], i along x
+  // the net, i along u (x), j along v (y)
+  std::vector<double> pw;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) pw.insert(pw.end(), {5.0 * i, 5.0 * j, zc[i][j], 1.0});
+  icam_face_desc top;
+  std::memset(&top, 0, sizeof(top));
+  top.kind = ICAM_NURBS;
+  top.sense = 1.0;
+  top.du = top.dv = 2;
+  top.nu = top.nv = 3;
+  top.pw = o->keep(pw);
+  top.ku = o->keep({0, 0, 0, 1, 1, 1});
+  top.kv = o->keep({0, 0, 0, 1, 1, 1});
+  b.set_trim(top, 0, 0, 1, 1, 0, 0, {}, 0.5, 0.5, true);
+  o->faces.push_back(top);
+  uint32_t ft = 0;
+  V X{1, 0, 0}, Y{0, 1, 0}, Z{0, 0, 1};
+  V b00{0, 0, 0}, b10{10, 0, 0}, b11{10, 10, 0}, b01{0, 10, 0};
+  V t00{0, 0, 5}, t10{10, 0, 5}, t11{10, 10, 5}, t01{0, 10, 5};
+  // bottom: normal -z, frame x = X, y = (-z) x X = -Y: params (x, -y)
+  uint32_t fb = b.plane(b00, Z * -1.0, X, 1.0, -1, -11, 11, 1, [] {
+    std::vector<Piece> p;
+    add_polyline(p, {{0, 0}, {10, 0}, {10, -10}, {0, -10}}, true);
+    return p;
+  }(), -0.5, 0.0123, false);
+  // y = 0 side: normal -y, frame x = X, y axis = (-Y) x X = +Z: params (x, z)
+  std::vector<Piece> s0;
+  add_polyline(s0, {{0, 5}, {0, 0}, {10, 0}, {10, 5}}, false);
+  for (auto& pc : std::vector<Piece>{to_cubic2({0, 5, 5, 7, 10, 5}, 2)}) add_monotone(s0, pc);
+  uint32_t fy0 = b.plane(b00, Y * -1.0, X, 1.0, -1, -1, 11, 11, s0, 5.0, -0.5, false);
+  // x = 10 side: normal +x, frame x = Y, y axis = X x Y = Z: params (y, z)
+  std::vector<Piece> s1;
+  add_polyline(s1, {{0, 5}, {0, 0}, {10, 0}, {10, 5}}, false);
+  add_monotone(s1, to_cubic2({0, 5, 5, 6, 10, 5}, 2));
+  uint32_t fx1 = b.plane(b10, X, Y, 1.0, -1, -1, 11, 11, s1, 5.0, -0.5, false);
+  // y = 10 side: normal +y, frame x = X, y axis = Y x X = -Z: params (x, -z)
+  std::vector<Piece> s2;
+  add_polyline(s2, {{0, 0}, {10, 0}, {10, -5}, {0, -5}}, true);
+  uint32_t fy1 = b.plane(b01, Y, X, 1.0, -1, -11, 11, 1, s2, 5.0, 0.5, false);
+  // x = 0 side: normal -x, frame x = Y, y axis = (-X) x Y = -Z: params (y, -z)
+  std::vector<Piece> s3;
+  add_polyline(s3, {{0, 0}, {10, 0}, {10, -5}, {0, -5}}, true);
+  uint32_t fx0 = b.plane(b00, X * -1.0, Y, 1.0, -1, -11, 11, 1, s3, 5.0, 0.5, false);
+  // edges: bottom square, vertical, top
+  b.line(b00, b10, fb, fy0);
+  b.line(b10, b11, fb, fx1);
+  b.line(b11, b01, fb, fy1);
+  b.line(b01, b00, fb, fx0);
+  b.line(b00, t00, fy0, fx0);
+  b.line(b10, t10, fy0, fx1);
+  b.line(b11, t11, fx1, fy1);
+  b.line(b01, t01, fy1, fx0);
+  auto nan4 = std::vector<double>{NaN, NaN, NaN, NaN};
+  auto cat = [](std::vector<double> a, const std::vector<double>& c) {
+    a.insert(a.end(), c.begin(), c.end());
+    return a;
+  };
+  // top boundary y = 0 (v = 0, u 0 -> 1) and x = 10 (u = 1, v 0 -> 1): quadratic B-splines
+  b.spline(2, {{0, 0, 5, 1, 5, 0, 7, 1, 10, 0, 5, 1}}, ft, fy0, cat({0, 0, 1, 0}, nan4));
+  b.spline(2, {{10, 0, 5, 1, 10, 5, 6, 1, 10, 10, 5, 1}}, ft, fx1, cat({1, 0, 1, 1}, nan4));
+  // straight top boundaries y = 10 (v = 1, u 1 -> 0) and x = 0 (u = 0, v 1 -> 0)
+  b.line(t11, t01, ft, fy1, o->keep(cat({1, 1, 0, 1}, nan4)));
+  b.line(t01, t00, ft, fx0, o->keep(cat({0, 1, 0, 0}, nan4)));
+  // vertices: bottom corners (three right angles), top corners from the faces' normals
+  double q = M_PI / 2;
+  b.vert(b00, V{-1, -1, -1} * q);
+  b.vert(b10, V{1, -1, -1} * q);
+  b.vert(b11, V{1, 1, -1} * q);
+  b.vert(b01, V{-1, 1, -1} * q);
+  // top corners: the surface's tangents at its corners (quadratic: 2 (P1 - P0))
+  auto corner = [&](V p, V su, V sv, V side_a_n, V side_b_n, V down) {
+    // the face angles: top between the two boundary tangents, each side between
+    // its boundary tangent and the vertical edge
+    auto ang = [](V a, V c) { return std::atan2(len(cross(a, c)), dot(a, c)); };
+    V n = unit(cross(su, sv));
+    if (n.z < 0) n = n * -1.0;
+    V pn = n * ang(su, sv) + side_a_n * ang(su, down) + side_b_n * ang(sv, down);
+    b.vert(p, pn);
+  };
+  // corner t00: tangents toward the surface's inside: +Su along y = 0, +Sv along x = 0
+  corner(t00, V{5, 0, 2}, V{0, 5, 0}, Y * -1.0, X * -1.0, Z * -1.0);
+  corner(t10, V{-5, 0, 2}, V{0, 5, 1}, Y * -1.0, X, Z * -1.0);
+  corner(t11, V{-5, 0, 0}, V{0, -5, 1}, Y, X, Z * -1.0);
+  corner(t01, V{5, 0, 0}, V{0, -5, 0}, Y, X * -1.0, Z * -1.0);
+  o->finish({0, 0, 0}, {10, 10, 9}, 1e-9);
+  return o;
+}
+
+// ------------------------------------------------------------- C1
+
+Owned* make_c1() {
+  Owned* o = new Owned();
+  Builder b{o};
+  V X{1, 0, 0}, Y{0, 1, 0}, Z{0, 0, 1};
+  const double ax = 50, ay = 30, az = 20, rh = 10, rf = 3;
+  const double R = rh + rf, zc = az - rf;  // fillet tube centre circle
+  auto rect = [](double u0, double v0, double u1, double v1) {
+    std::vector<Piece> p;
+    add_polyline(p, {{u0, v0}, {u1, v0}, {u1, v1}, {u0, v1}}, true);
+    return p;
+  };
+  // F0 top z = 20 (params x, y), minus the fillet's disc r 13
+  std::vector<Piece> top = rect(-ax, -ay, ax, ay);
+  add_circle(top, 0, 0, R);
+  uint32_t f_top = b.plane({0, 0, az}, Z, X, 1.0, -ax - 1, -ay - 1, ax + 1, ay + 1, top, -ax - 0.5, 0.0123, false);
+  // F1 bottom z = -20, normal -z (params x, -y), minus the hole's disc r 10
+  std::vector<Piece> bot = rect(-ax, -ay, ax, ay);
+  add_circle(bot, 0, 0, rh);
+  uint32_t f_bot = b.plane({0, 0, -az}, Z * -1.0, X, 1.0, -ax - 1, -ay - 1, ax + 1, ay + 1, bot, -ax - 0.5, 0.0123, false);
+  // F2 x = 50: frame x = Y, y axis = X x Y = Z: params (y, z)
+  uint32_t f_xp = b.plane({ax, 0, 0}, X, Y, 1.0, -ay - 1, -az - 1, ay + 1, az + 1, rect(-ay, -az, ay, az), -ay - 0.5, 0.0123, false);
+  // F3 x = -50: frame x = Y, y axis = (-X) x Y = -Z: params (y, -z)
+  uint32_t f_xm = b.plane({-ax, 0, 0}, X * -1.0, Y, 1.0, -ay - 1, -az - 1, ay + 1, az + 1, rect(-ay, -az, ay, az), -ay - 0.5, 0.0123, false);
+  // F4 y = 30: frame x = X, y axis = Y x X = -Z: params (x, -z)
+  uint32_t f_yp = b.plane({0, ay, 0}, Y, X, 1.0, -ax - 1, -az - 1, ax + 1, az + 1, rect(-ax, -az, ax, az), -ax - 0.5, 0.0123, false);
+  // F5 y = -30: frame x = X, y axis = (-Y) x X = Z: params (x, z)
+  uint32_t f_ym = b.plane({0, -ay, 0}, Y * -1.0, X, 1.0, -ax - 1, -az - 1, ax + 1, az + 1, rect(-ax, -az, ax, az), -ax - 0.5, 0.0123, false);
+  // F6 the hole: cylinder r 10 about z, material outside it (sense -1), z from -20 to 17
+  std::vector<Piece> wall = {line_piece(0, -az, 4, -az), line_piece(0, zc, 4, zc)};
+  uint32_t f_cyl = b.analytic(ICAM_CYLINDER, {0, 0, 0}, Z, X, rh, 0, -1.0, 0, -az - 1, 4, zc + 1, 4, 0, wall, 2.0, 0.0123, true);
+  // F7 the fillet: torus major 13 minor 3 about z at z 17, the quarter of its tube from
+  // its top (psi = 90 degrees, arc parameter 1) to its inner equator (180, 2)
+  std::vector<Piece> fil = {line_piece(0, 1, 4, 1), line_piece(0, 2, 4, 2)};
+  uint32_t f_tor = b.analytic(ICAM_TORUS, {0, 0, zc}, Z, X, R, rf, 1.0, 0, 0, 4, 4, 4, 4, fil, 2.0, 1.5, true);
+  // edges
+  V c[2][2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) c[i][j][k] = V{i ? ax : -ax, j ? ay : -ay, k ? az : -az};
+  for (int k = 0; k < 2; ++k) {
+    uint32_t f = k ? f_top : f_bot;
+    b.line(c[0][0][k], c[1][0][k], f, f_ym);
+    b.line(c[1][0][k], c[1][1][k], f, f_xp);
+    b.line(c[1][1][k], c[0][1][k], f, f_yp);
+    b.line(c[0][1][k], c[0][0][k], f, f_xm);
+  }
+  b.line(c[0][0][0], c[0][0][1], f_xm, f_ym);
+  b.line(c[1][0][0], c[1][0][1], f_xp, f_ym);
+  b.line(c[1][1][0], c[1][1][1], f_xp, f_yp);
+  b.line(c[0][1][0], c[0][1][1], f_xm, f_yp);
+  b.circle({0, 0, az}, Z, X, R, f_top, f_tor);
+  b.circle({0, 0, zc}, Z, X, rh, f_cyl, f_tor);
+  b.circle({0, 0, -az}, Z, X, rh, f_cyl, f_bot);
+  double q = M_PI / 2;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) b.vert(c[i][j][k], V{i ? 1.0 : -1.0, j ? 1.0 : -1.0, k ? 1.0 : -1.0} * q);
+  (void)Y;
+  o->finish({-ax, -ay, -az}, {ax, ay, az}, 1e-9);
+  return o;
+}
+
+// ------------------------------------------------------------- C2
+
+double hash01(uint64_t seed, int64_t ix, int64_t iy, int oct) {
+  uint64_t h = seed ^ (uint64_t(ix) * 0x9E3779B97F4A7C15ull) ^ (uint64_t(iy) * 0xC2B2AE3D27D4EB4Full) ^ (uint64_t(oct + 1) * 0x165667B19E3779F9ull);
+  Rng r{h ? h : 1};
+  r.next();
+  r.next();
+  return r.next();
+}
+
+// Seeded value noise in [-1, 1]: lattice values smoothly interpolated, octaves of
+// periods 80, 40 and 20 mm with amplitudes 1, 1/2, 1/4, normalized by their sum.
+double value_noise(uint64_t seed, double x, double y) {
+  double sum = 0, amp = 1, norm = 0, period = 80;
+  for (int oct = 0; oct < 3; ++oct) {
+    double fx = x / period, fy = y / period;
+    int64_t ix = int64_t(std::floor(fx)), iy = int64_t(std::floor(fy));
+    double tx = fx - ix, ty = fy - iy;
+    auto s = [](double t) { return t * t * (3 - 2 * t); };
+    double sx = s(tx), sy = s(ty);
+    double v00 = 2 * hash01(seed, ix, iy, oct) - 1, v10 = 2 * hash01(seed, ix + 1, iy, oct) - 1;
+    double v01 = 2 * hash01(seed, ix, iy + 1, oct) - 1, v11 = 2 * hash01(seed, ix + 1, iy + 1, oct) - 1;
+    double v = (v00 * (1 - sx) + v10 * sx) * (1 - sy) + (v01 * (1 - sx) + v11 * sx) * sy;
+    sum += amp * v;
+    norm += amp;
+    amp *= 0.5;
+    period *= 0.5;
+  }
+  return sum / norm;
+}
+
+Owned* make_c2(uint64_t seed) {
+  Owned* o = new Owned();
+  Builder b{o};
+  const int n = 16, p = 3, spans = n - p;
+  const double half = 60, zb = -20, ztop = 20, amp = 5;
+  std::vector<double> kn;
+  for (int i = 0; i <= p; ++i) kn.push_back(0);
+  for (int i = 1; i < spans; ++i) kn.push_back(double(i) / spans);
+  for (int i = 0; i <= p; ++i) kn.push_back(1);
+  // Greville abscissae: the net's x (and y) are then linear in u (and v)
+  std::vector<double> gx(n);
+  for (int i = 0; i < n; ++i) gx[i] = -half + 2 * half * (kn[i + 1] + kn[i + 2] + kn[i + 3]) / 3.0;
+  Rng rng{seed ? seed : 1};
+  std::vector<double> pw(n * n * 4);
+  std::vector<double> zz(n * n), ww(n * n);
+  // the noise scaled so its largest height over the net is the amplitude
+  std::vector<double> raw(n * n);
+  double peak = 0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) raw[i * n + j] = value_noise(seed, gx[i], gx[j]), peak = std::max(peak, std::fabs(raw[i * n + j]));
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      double z = ztop + amp * raw[i * n + j] / peak;
+      double w = (i == 0 || j == 0 || i == n - 1 || j == n - 1) ? 1.0 : 0.8 + 0.4 * rng.next();
+      zz[i * n + j] = z, ww[i * n + j] = w;
+      double* q = &pw[(i * n + j) * 4];
+      q[0] = gx[i] * w, q[1] = gx[j] * w, q[2] = z * w, q[3] = w;
+    }
+  icam_face_desc top;
+  std::memset(&top, 0, sizeof(top));
+  top.kind = ICAM_NURBS;
+  top.sense = 1.0;
+  top.du = top.dv = p;
+  top.nu = top.nv = n;
+  top.pw = o->keep(pw);
+  top.ku = o->keep(kn);
+  top.kv = o->keep(kn);
+  b.set_trim(top, 0, 0, 1, 1, 0, 0, {}, 0.5, 0.5, true);
+  o->faces.push_back(top);
+  const uint32_t ft = 0;
+  // the four boundary curves as 3-D cubic Bezier pieces (unit weights), each with
+  // its knot intervals: side 0 v = 0 (y = -60), 1 u = 1 (x = 60), 2 v = 1 (y = 60),
+  // 3 u = 0 (x = -60), each in the direction of increasing knot parameter
+  auto boundary = [&](int side) {
+    std::vector<double> pts;
+    for (int k = 0; k < n; ++k) {
+      int i = 0, j = 0;
+      if (side == 0) i = k, j = 0;
+      if (side == 1) i = n - 1, j = k;
+      if (side == 2) i = k, j = n - 1;
+      if (side == 3) i = 0, j = k;
+      pts.insert(pts.end(), {gx[i], gx[j], zz[i * n + j], 1.0});
+    }
+    return decompose(p, kn, pts, 4);
+  };
+  V X{1, 0, 0}, Y{0, 1, 0}, Z{0, 0, 1};
+  double top_max = ztop + amp + 1;
+  uint32_t side_face[4];
+  for (int side = 0; side < 4; ++side) {
+    auto pcs = boundary(side);
+    // the side plane's frame: outward normal, x axis, and which world axis is its
+    // first parameter; the second parameter is the height z
+    V nrm, xa;
+    if (side == 0) nrm = Y * -1.0, xa = X;  // y axis = (-Y) x X = Z
+    if (side == 1) nrm = X, xa = Y;         // X x Y = Z
+    if (side == 2) nrm = Y, xa = X * -1.0;  // Y x (-X) = Z
+    if (side == 3) nrm = X * -1.0, xa = Y * -1.0;  // (-X) x (-Y) = Z
+    V org = nrm * half;
+    auto param = [&](V w) { return std::array<double, 2>{dot(w - org, xa), w.z}; };
+    std::vector<Piece> loop;
+    for (auto& pc : pcs) {
+      std::vector<double> c2;
+      for (int k = 0; k <= p; ++k) {
+        auto q = param(V{pc[4 * k], pc[4 * k + 1], pc[4 * k + 2]});
+        c2.push_back(q[0]), c2.push_back(q[1]);
+      }
+      add_monotone(loop, to_cubic2(c2, p));
+    }
+    V a{pcs.front()[0], pcs.front()[1], pcs.front()[2]};
+    V e{pcs.back()[4 * p], pcs.back()[4 * p + 1], pcs.back()[4 * p + 2]};
+    auto pa = param(a), pe = param(e);
+    auto pab = param(V{a.x, a.y, zb}), peb = param(V{e.x, e.y, zb});
+    add_polyline(loop, {{pe[0], pe[1]}, {peb[0], peb[1]}, {pab[0], pab[1]}, {pa[0], pa[1]}}, false);
+    side_face[side] = b.plane(org, nrm, xa, 1.0, -half - 1, zb - 1, half + 1, top_max, loop, 0.0123, zb - 0.5, false);
+  }
+  // bottom: normal -z, frame x = X, y axis = -Y: params (x, -y)
+  std::vector<Piece> bot;
+  add_polyline(bot, {{-half, -half}, {half, -half}, {half, half}, {-half, half}}, true);
+  uint32_t fb = b.plane({0, 0, zb}, Z * -1.0, X, 1.0, -half - 1, -half - 1, half + 1, half + 1, bot, -half - 0.5, 0.0123, false);
+  // top spline edges with their pieces' knot intervals on the NURBS face
+  for (int side = 0; side < 4; ++side) {
+    auto pcs = boundary(side);
+    std::vector<double> uv;
+    for (int k = 0; k < int(pcs.size()); ++k) {
+      double a = double(k) / spans, c = double(k + 1) / spans;
+      if (side == 0) uv.insert(uv.end(), {a, 0, c, 0});
+      if (side == 1) uv.insert(uv.end(), {1, a, 1, c});
+      if (side == 2) uv.insert(uv.end(), {a, 1, c, 1});
+      if (side == 3) uv.insert(uv.end(), {0, a, 0, c});
+      uv.insert(uv.end(), {NaN, NaN, NaN, NaN});
+    }
+    b.spline(p, pcs, ft, side_face[side], uv);
+  }
+  // corners
+  V bc[4] = {{-half, -half, zb}, {half, -half, zb}, {half, half, zb}, {-half, half, zb}};
+  auto zat = [&](int i, int j) { return zz[i * n + j]; };
+  V tc[4] = {{-half, -half, zat(0, 0)}, {half, -half, zat(n - 1, 0)}, {half, half, zat(n - 1, n - 1)}, {-half, half, zat(0, n - 1)}};
+  // side faces around: corner k sits between side (k+3)%4 and side k
+  for (int k = 0; k < 4; ++k) {
+    int s0 = (k + 3) % 4, s1 = k;
+    b.line(bc[k], bc[(k + 1) % 4], fb, side_face[k]);
+    b.line(bc[k], tc[k], side_face[s0], side_face[s1]);
+  }
+  // vertices: bottom corners, and top corners from the three faces' normals and angles
+  double q = M_PI / 2;
+  for (int k = 0; k < 4; ++k) {
+    V d = bc[k] * (1.0 / half);
+    b.vert(bc[k], V{d.x > 0 ? 1.0 : -1.0, d.y > 0 ? 1.0 : -1.0, -1.0} * q);
+  }
+  auto ang = [](V a, V c) { return std::atan2(len(cross(a, c)), dot(a, c)); };
+  for (int k = 0; k < 4; ++k) {
+    int ci = (k == 1 || k == 2) ? n - 1 : 0, cj = (k >= 2) ? n - 1 : 0;
+    int di = ci == 0 ? 1 : -1, dj = cj == 0 ? 1 : -1;
+    // tangents into the surface along its two boundaries (unit weights there)
+    auto P = [&](int i, int j) { return V{gx[i], gx[j], zat(i, j)}; };
+    V tu = P(ci + di, cj) - P(ci, cj), tv = P(ci, cj + dj) - P(ci, cj);
+    V n0 = unit(cross(tu, tv));
+    if (n0.z < 0) n0 = n0 * -1.0;
+    V down{0, 0, -1};
+    V nu_side = (cj == 0) ? Y * -1.0 : Y, nv_side = (ci == 0) ? X * -1.0 : X;
+    V pn = n0 * ang(tu, tv) + nu_side * ang(tu, down) + nv_side * ang(tv, down);
+    b.vert(tc[k], pn);
+  }
+  double zmax = -1e300;
+  for (double z : zz) zmax = std::max(zmax, z);
+  o->finish({-half, -half, zb}, {half, half, zmax}, 1e-9);
+  return o;
+}
+
+}  // namespace
+
+extern "C" {
+
+// A synthetic part by name ("can", "bump", "c1", "c2"); NULL for an unknown name.
+// Free with icam_gen_free.
+const icam_part_desc* icam_gen_part(const char* name, uint64_t seed) {
+  std::string s(name ? name : "");
+  Owned* o = nullptr;
+  if (s == "can") o = make_can();
+  if (s == "bump") o = make_bump();
+  if (s == "c1") o = make_c1();
+  if (s == "c2") o = make_c2(seed);
+  return o ? &o->desc : nullptr;
+}
+
+void icam_gen_free(const icam_part_desc* p) { delete reinterpret_cast<const Owned*>(p); }
+
+// n points uniform in [lo, hi] from the reference checks' xorshift64 (x, y, z
+// drawn in that order per point).
+void icam_gen_uniform(uint64_t seed, size_t n, const double lo[3], const double hi[3], double* xyz) {
+  Rng r{seed ? seed : 1};
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) xyz[3 * i + k] = lo[k] + (hi[k] - lo[k]) * r.next();
+}
+
+}  // extern "C"
