@@ -22,6 +22,8 @@
 //   "c2"    120 x 120 block, top a rational bicubic NURBS, 16 x 16 net, clamped
 //           uniform knots, heights from seeded 3-octave value noise scaled to 5 mm,
 //           interior weights uniform in [0.8, 1.2]
+//   "c2flat" the same with every control point at the top's base height: its top
+//           is the plane z = 20 and the part the box [-60, 60]^2 x [-20, 20]
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -514,7 +516,7 @@ double value_noise(uint64_t seed, double x, double y) {
   return sum / norm;
 }
 
-Owned* make_c2(uint64_t seed) {
+Owned* make_c2(uint64_t seed, bool flat) {
   Owned* o = new Owned();
   Builder b{o};
   const int n = 16, p = 3, spans = n - p;
@@ -536,7 +538,7 @@ Owned* make_c2(uint64_t seed) {
     for (int j = 0; j < n; ++j) raw[i * n + j] = value_noise(seed, gx[i], gx[j]), peak = std::max(peak, std::fabs(raw[i * n + j]));
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
-      double z = ztop + amp * raw[i * n + j] / peak;
+      double z = flat ? ztop : ztop + amp * raw[i * n + j] / peak;
       double w = (i == 0 || j == 0 || i == n - 1 || j == n - 1) ? 1.0 : 0.8 + 0.4 * rng.next();
       zz[i * n + j] = z, ww[i * n + j] = w;
       double* q = &pw[(i * n + j) * 4];
@@ -665,7 +667,8 @@ const icam_part_desc* icam_gen_part(const char* name, uint64_t seed) {
   if (s == "can") o = make_can();
   if (s == "bump") o = make_bump();
   if (s == "c1") o = make_c1();
-  if (s == "c2") o = make_c2(seed);
+  if (s == "c2") o = make_c2(seed, false);
+  if (s == "c2flat") o = make_c2(seed, true);
   return o ? &o->desc : nullptr;
 }
 

@@ -1,0 +1,894 @@
+// Host packer: the part as record tables for the kernels (layout.hpp).
+//
+// What it builds, each after the reference's pack (crates/icam-core/src/solid3/gpu.rs):
+//  - analytic faces with their exact rational forms' parameter constants
+//    (trim2.rs:1807-1832) and their trim cell trees (trim2.rs:1262-1347);
+//  - B-spline faces cut into knot-span Bezier patches (surface3.rs:454-467),
+//    raised to bicubic (gpu.rs:149-171), each with a tree of sub-patches split
+//    breadth first while its radius of convexity is short of its size and the
+//    need (gpu.rs:110-112, 292-350), radii from interval bounds on the
+//    derivatives (surface3.rs:819-926);
+//  - spline edges cut into cubic pieces, each with a binary tree of parts under
+//    the same rule (gpu.rs:501-577), radii |C'|^2 / |C''| (surface3.rs:931-970);
+//  - line and arc edges, vertices with their pseudo-normals;
+//  - the flat primitive list with float32 boxes rounded outward.
+#include "pack.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace icam {
+namespace {
+
+const double INF = std::numeric_limits<double>::infinity();
+const double PI = 3.14159265358979323846;
+const double TAU = 2 * PI;
+
+struct V {
+  double x = 0, y = 0, z = 0;
+};
+inline V mk(double a, double b, double c) { return V{a, b, c}; }
+inline V operator+(V a, V b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V operator-(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V operator*(V a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V cross(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double len(V a) { return std::sqrt(dot(a, a)); }
+inline V of(const double* d) { return {d[0], d[1], d[2]}; }
+
+struct Box {
+  V lo{INF, INF, INF}, hi{-INF, -INF, -INF};
+  void add(V p) {
+    lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+    hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+  }
+  void add(const Box& b) { add(b.lo), add(b.hi); }
+  double diag() const { return len(hi - lo); }
+};
+
+// ------------------------------------------------------------------ trim cell trees
+// trim2.rs:104-136, 1150-1190: Bernstein evaluation and the half-open crossing.
+double beval3(const double* c, double t) {
+  if (c[0] == c[1] && c[1] == c[2] && c[2] == c[3]) return c[0];
+  double b[4] = {c[0], c[1], c[2], c[3]};
+  for (int l = 1; l <= 3; ++l)
+    for (int i = 0; i <= 3 - l; ++i) b[i] = (1 - t) * b[i] + t * b[i + 1];
+  return b[0];
+}
+double beval2(const double* c, double t) {
+  if (c[0] == c[1] && c[1] == c[2]) return c[0];
+  double b[3] = {c[0], c[1], c[2]};
+  for (int l = 1; l <= 2; ++l)
+    for (int i = 0; i <= 2 - l; ++i) b[i] = (1 - t) * b[i] + t * b[i + 1];
+  return b[0];
+}
+bool cross_at(const double a[4], double x, double& out) {
+  double p0 = a[0], p1 = a[3];
+  if (!(std::min(p0, p1) <= x && x < std::max(p0, p1))) return false;
+  double lo = 0, hi = 1, flo = beval3(a, 0) - x;
+  if (flo == 0) return out = 0, true;
+  double t = std::min(std::max((x - p0) / (p1 - p0), 0.0), 1.0);
+  double d3[3] = {a[1] - a[0], a[2] - a[1], a[3] - a[2]};
+  for (int it = 0; it < 60; ++it) {
+    double ft = beval3(a, t) - x;
+    if (ft == 0) return out = t, true;
+    if ((ft < 0) == (flo < 0))
+      lo = t, flo = ft;
+    else
+      hi = t;
+    double d = 3 * beval2(d3, t);
+    double nt = d != 0 ? t - ft / d : 0.5 * (lo + hi);
+    if (!(nt > lo && nt < hi)) nt = 0.5 * (lo + hi);
+    if (std::fabs(nt - t) <= 1e-16) return out = nt, true;
+    t = nt;
+  }
+  return out = t, true;
+}
+
+struct TrimBuild {
+  double lo[2], hi[2], per[2];
+  std::vector<std::array<double, 8>> inst;  // shifted copies
+  std::vector<uint32_t> inst_piece;
+  std::vector<std::array<double, 2>> inst_shift;
+  struct Cell {
+    double lo[2], hi[2], r[2];
+    uint32_t inside, kind, a, b;
+  };
+  std::vector<Cell> cells;
+  std::vector<uint32_t> among;
+
+  // trim2.rs:1205-1260 parity of the two-leg path with the symbolic perturbation
+  bool parity(const double a[2], const double b[2], const std::vector<uint32_t>& ids) const {
+    unsigned n = 0;
+    bool up = a[1] < b[1], right = a[0] < b[0];
+    for (uint32_t i : ids) {
+      const auto& c = inst[i];
+      double cu[4] = {c[0], c[2], c[4], c[6]}, cv[4] = {c[1], c[3], c[5], c[7]}, t;
+      if (a[1] != b[1] && cross_at(cu, a[0], t)) {
+        double y = beval3(cv, t);
+        if (up ? (y > a[1] && y <= b[1]) : (y > b[1] && y <= a[1])) ++n;
+      }
+      if (a[0] != b[0] && cross_at(cv, b[1], t)) {
+        double x = beval3(cu, t);
+        auto slope_pos = [&]() {
+          double du3[3] = {cu[1] - cu[0], cu[2] - cu[1], cu[3] - cu[2]}, dv3[3] = {cv[1] - cv[0], cv[2] - cv[1], cv[3] - cv[2]};
+          double du = beval2(du3, t), dv = beval2(dv3, t);
+          if (dv == 0) dv = cv[3] - cv[0];
+          return du * dv > 0;
+        };
+        bool counted = x == b[0] ? (!slope_pos()) == right : x == a[0] ? slope_pos() == right : right ? (x > a[0] && x < b[0]) : (x > b[0] && x < a[0]);
+        if (counted) ++n;
+      }
+    }
+    return n % 2 == 1;
+  }
+
+  static void bbox(const std::array<double, 8>& c, double blo[2], double bhi[2]) {
+    blo[0] = blo[1] = INF, bhi[0] = bhi[1] = -INF;
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 2; ++k) blo[k] = std::min(blo[k], c[2 * i + k]), bhi[k] = std::max(bhi[k], c[2 * i + k]);
+  }
+
+  // trim2.rs:1262-1347 place and build_cell, with the root answered from the given
+  // reference point instead of an offset reference
+  void build(const icam_face_desc& f) {
+    for (int k = 0; k < 2; ++k) lo[k] = f.lo[k], hi[k] = f.hi[k], per[k] = f.period[k];
+    double rlo[2] = {lo[0], lo[1]}, rhi[2] = {hi[0], hi[1]};
+    for (int k = 0; k < 2; ++k)
+      if (per[k] > 0) rlo[k] -= per[k], rhi[k] += per[k];
+    for (uint32_t p = 0; p < f.n_pieces; ++p) {
+      std::array<double, 8> c;
+      std::copy(f.pieces + 8 * p, f.pieces + 8 * p + 8, c.begin());
+      double plo[2], phi[2];
+      bbox(c, plo, phi);
+      long r0[2], r1[2];
+      for (int k = 0; k < 2; ++k) {
+        r0[k] = per[k] > 0 ? long(std::ceil((rlo[k] - phi[k]) / per[k])) : 0;
+        r1[k] = per[k] > 0 ? long(std::floor((rhi[k] - plo[k]) / per[k])) : 0;
+      }
+      for (long a = r0[0]; a <= r1[0]; ++a)
+        for (long b = r0[1]; b <= r1[1]; ++b) {
+          auto q = c;
+          for (int i = 0; i < 4; ++i) q[2 * i] += a * per[0], q[2 * i + 1] += b * per[1];
+          inst.push_back(q);
+          inst_piece.push_back(p);
+          inst_shift.push_back({a * per[0], b * per[1]});
+        }
+    }
+    const double REF[2] = {0.5137, 0.4871};  // trim2.rs:101
+    std::vector<uint32_t> all(inst.size());
+    for (size_t i = 0; i < all.size(); ++i) all[i] = uint32_t(i);
+    Cell root;
+    for (int k = 0; k < 2; ++k) root.lo[k] = lo[k], root.hi[k] = hi[k], root.r[k] = lo[k] + (hi[k] - lo[k]) * REF[k];
+    double ref[2] = {f.ref[0], f.ref[1]};
+    root.inside = (f.n_pieces == 0) ? 1u : uint32_t((f.ref_inside != 0) ^ parity(ref, root.r, all));
+    root.kind = 0, root.a = root.b = 0;
+    cells.push_back(root);
+    if (f.n_pieces) build_cell(0, all, 0, REF);
+  }
+
+  void build_cell(uint32_t id, const std::vector<uint32_t>& cand, int depth, const double REF[2]) {
+    Cell c = cells[id];
+    std::vector<uint32_t> touching;
+    for (uint32_t i : cand) {
+      double blo[2], bhi[2];
+      bbox(inst[i], blo, bhi);
+      if (c.lo[0] <= bhi[0] && blo[0] <= c.hi[0] && c.lo[1] <= bhi[1] && blo[1] <= c.hi[1]) touching.push_back(i);
+    }
+    if (touching.empty()) return;
+    if (depth >= 10 || touching.size() <= 4) {
+      cells[id].kind = 1, cells[id].a = uint32_t(among.size()), cells[id].b = uint32_t(touching.size());
+      among.insert(among.end(), touching.begin(), touching.end());
+      return;
+    }
+    double m[2] = {0.5 * (c.lo[0] + c.hi[0]), 0.5 * (c.lo[1] + c.hi[1])};
+    uint32_t first = uint32_t(cells.size());
+    cells[id].kind = 2, cells[id].a = first;
+    for (int q = 0; q < 4; ++q) {
+      Cell k;
+      k.lo[0] = (q & 1) ? m[0] : c.lo[0], k.hi[0] = (q & 1) ? c.hi[0] : m[0];
+      k.lo[1] = (q & 2) ? m[1] : c.lo[1], k.hi[1] = (q & 2) ? c.hi[1] : m[1];
+      for (int a = 0; a < 2; ++a) k.r[a] = k.lo[a] + (k.hi[a] - k.lo[a]) * REF[a];
+      k.inside = uint32_t((c.inside != 0) ^ parity(c.r, k.r, touching));
+      k.kind = 0, k.a = k.b = 0;
+      cells.push_back(k);
+    }
+    for (int q = 0; q < 4; ++q) build_cell(first + q, touching, depth + 1, REF);
+  }
+
+  void reduce(double& u, double& v) const {
+    double q[2] = {u, v};
+    for (int k = 0; k < 2; ++k)
+      q[k] = per[k] > 0 ? lo[k] + (q[k] - lo[k] - per[k] * std::floor((q[k] - lo[k]) / per[k])) : std::min(std::max(q[k], lo[k]), hi[k]);
+    u = q[0], v = q[1];
+  }
+  bool inside(double u, double v) const {
+    if (inst.empty() && cells.size() == 1 && cells[0].kind == 0) return cells[0].inside != 0;
+    reduce(u, v);
+    uint32_t id = 0;
+    for (;;) {
+      const Cell& c = cells[id];
+      if (c.kind == 0) return c.inside != 0;
+      if (c.kind == 1) {
+        std::vector<uint32_t> ids(among.begin() + c.a, among.begin() + c.a + c.b);
+        double q[2] = {u, v};
+        return (c.inside != 0) ^ parity(c.r, q, ids);
+      }
+      double m[2] = {0.5 * (c.lo[0] + c.hi[0]), 0.5 * (c.lo[1] + c.hi[1])};
+      id = c.a + uint32_t(u >= m[0]) + 2 * uint32_t(v >= m[1]);
+    }
+  }
+  // trim2.rs:1115-1132 meets_box
+  bool meets_box(const std::array<double, 8>& c, const double blo[2], const double bhi[2]) const {
+    double b0[2], b1[2];
+    bbox(c, b0, b1);
+    for (int k = 0; k < 2; ++k)
+      if (!(b0[k] <= bhi[k] && blo[k] <= b1[k])) return false;
+    double cu[4] = {c[0], c[2], c[4], c[6]}, cv[4] = {c[1], c[3], c[5], c[7]};
+    auto param_at = [&](double x) {
+      if (x == cu[0]) return 0.0;
+      if (x == cu[3]) return 1.0;
+      double t;
+      if (cross_at(cu, x, t)) return t;
+      return std::fabs(x - cu[0]) < std::fabs(x - cu[3]) ? 0.0 : 1.0;
+    };
+    double ta = 0, tb = 1;
+    if (cu[0] != cu[3]) {
+      double a = std::max(blo[0], std::min(cu[0], cu[3])), b = std::min(bhi[0], std::max(cu[0], cu[3]));
+      ta = param_at(a), tb = param_at(b);
+    }
+    double va = beval3(cv, ta), vb = beval3(cv, tb);
+    return std::min(va, vb) <= bhi[1] && std::max(va, vb) >= blo[1];
+  }
+  // trim2.rs:1570-1610 box_class, answered from the loops directly: 1 in, 0 out, -1 straddles
+  int box_class(const double blo[2], const double bhi[2]) const {
+    if (inst.empty()) return cells[0].inside ? 1 : 0;
+    double g = 1e-12 * (std::fabs(hi[0] - lo[0]) + std::fabs(hi[1] - lo[1]));
+    double glo[2] = {blo[0] - g, blo[1] - g}, ghi[2] = {bhi[0] + g, bhi[1] + g};
+    for (auto& c : inst)
+      if (meets_box(c, glo, ghi)) return -1;
+    return inside(0.5 * (blo[0] + bhi[0]), 0.5 * (blo[1] + bhi[1])) ? 1 : 0;
+  }
+};
+
+// ------------------------------------------------------------------ Bezier helpers
+using H = std::array<double, 4>;
+
+// Bezier decomposition of a clamped B-spline by knot insertion (The NURBS Book
+// algorithm 5.1), flat points of dimension N; returns pieces and their knot ranges.
+void decompose(int p, std::vector<double> kn, std::vector<double> pts, int N, std::vector<std::vector<double>>& pieces,
+               std::vector<std::pair<double, double>>& ranges) {
+  auto count = [&](double u) {
+    int c = 0;
+    for (double k : kn) c += (k == u);
+    return c;
+  };
+  std::vector<double> interior;
+  for (size_t i = p + 1; i + p + 1 < kn.size(); ++i)
+    if (kn[i] > kn[p] && kn[i] < kn[kn.size() - 1 - p] && (interior.empty() || interior.back() != kn[i])) interior.push_back(kn[i]);
+  for (double u : interior)
+    while (count(u) < p) {
+      int k = 0;
+      for (int i = int(kn.size()) - 2; i >= 0; --i)
+        if (kn[i] <= u && kn[i + 1] > u) {
+          k = i;
+          break;
+        }
+      int s = 0;
+      for (int i = k; i >= 0 && kn[i] == u; --i) ++s;
+      int np = int(pts.size()) / N;
+      std::vector<double> q;
+      for (int i = 0; i <= k - p; ++i)
+        for (int c = 0; c < N; ++c) q.push_back(pts[i * N + c]);
+      for (int i = k - p + 1; i <= k - s; ++i) {
+        double a = (u - kn[i]) / (kn[i + p] - kn[i]);
+        for (int c = 0; c < N; ++c) q.push_back(a * pts[i * N + c] + (1 - a) * pts[(i - 1) * N + c]);
+      }
+      for (int i = k - s; i < np; ++i)
+        for (int c = 0; c < N; ++c) q.push_back(pts[i * N + c]);
+      kn.insert(kn.begin() + k + 1, u);
+      pts = q;
+    }
+  std::vector<double> brk;
+  for (size_t i = p; i + p < kn.size(); ++i)
+    if (brk.empty() || brk.back() != kn[i]) brk.push_back(kn[i]);
+  int np = int(pts.size()) / N, sp = 0;
+  for (int s = 0; s + p < np; s += p, ++sp) {
+    pieces.emplace_back(pts.begin() + s * N, pts.begin() + (s + p + 1) * N);
+    ranges.push_back({brk[sp], brk[sp + 1]});
+  }
+}
+
+// gpu.rs:128-140 a Bezier curve raised one degree
+std::vector<H> raise(const std::vector<H>& p) {
+  size_t n = p.size();
+  std::vector<H> r(n + 1);
+  for (size_t i = 0; i <= n; ++i) {
+    double a = double(i) / n;
+    for (int k = 0; k < 4; ++k) r[i][k] = a * (i > 0 ? p[i - 1][k] : 0.0) + (1 - a) * (i < n ? p[i][k] : 0.0);
+  }
+  return r;
+}
+
+struct Patch4 {  // bicubic, homogeneous, i along u
+  H p[4][4];
+};
+struct Piece4 {
+  H p[4];
+};
+
+void split_curve(const H* in, double s, H* l, H* r) {  // 4 points
+  H b[4];
+  for (int i = 0; i < 4; ++i) b[i] = in[i];
+  l[0] = b[0], r[3] = b[3];
+  for (int lev = 1; lev <= 3; ++lev) {
+    for (int i = 0; i <= 3 - lev; ++i)
+      for (int k = 0; k < 4; ++k) b[i][k] = (1 - s) * b[i][k] + s * b[i + 1][k];
+    l[lev] = b[0], r[3 - lev] = b[3 - lev];
+  }
+}
+void quarter(const Patch4& pa, Patch4 q[4]) {  // q index = (s half) + 2 (t half)
+  Patch4 a, b;  // u halves
+  for (int j = 0; j < 4; ++j) {
+    H col[4], l[4], r[4];
+    for (int i = 0; i < 4; ++i) col[i] = pa.p[i][j];
+    split_curve(col, 0.5, l, r);
+    for (int i = 0; i < 4; ++i) a.p[i][j] = l[i], b.p[i][j] = r[i];
+  }
+  const Patch4* halves[2] = {&a, &b};
+  for (int hs = 0; hs < 2; ++hs)
+    for (int i = 0; i < 4; ++i) {
+      H l[4], r[4];
+      split_curve(halves[hs]->p[i], 0.5, l, r);
+      for (int j = 0; j < 4; ++j) q[hs + 0].p[i][j] = l[j], q[hs + 2].p[i][j] = r[j];
+    }
+}
+
+V point(const H& h) { return {h[0] / h[3], h[1] / h[3], h[2] / h[3]}; }
+
+// Intervals (surface3.rs:773-805)
+struct Iv {
+  double a, b;
+};
+Iv iv_of(std::initializer_list<double> xs) {
+  Iv r{INF, -INF};
+  for (double x : xs) r.a = std::min(r.a, x), r.b = std::max(r.b, x);
+  return r;
+}
+Iv add(Iv x, Iv y) { return {x.a + y.a, x.b + y.b}; }
+Iv sub(Iv x, Iv y) { return {x.a - y.b, x.b - y.a}; }
+Iv mul(Iv x, Iv y) {
+  double p[4] = {x.a * y.a, x.a * y.b, x.b * y.a, x.b * y.b};
+  return {*std::min_element(p, p + 4), *std::max_element(p, p + 4)};
+}
+Iv scale(Iv x, double k) { return k >= 0 ? Iv{x.a * k, x.b * k} : Iv{x.b * k, x.a * k}; }
+Iv divpos(Iv x, Iv w) { return mul(x, Iv{1.0 / w.b, 1.0 / w.a}); }
+double mig(Iv x) { return (x.a <= 0 && x.b >= 0) ? 0.0 : std::min(std::fabs(x.a), std::fabs(x.b)); }
+double mag(Iv x) { return std::max(std::fabs(x.a), std::fabs(x.b)); }
+
+// Bernstein evaluation of a bicubic with derivatives (point and first derivatives only)
+void eval_patch(const Patch4& pa, double s, double t, V& p, V& ps, V& pt) {
+  auto basis = [](double x, double b[4], double d[4]) {
+    double y = 1 - x;
+    b[0] = y * y * y, b[1] = 3 * x * y * y, b[2] = 3 * x * x * y, b[3] = x * x * x;
+    d[0] = -3 * y * y, d[1] = 3 * y * y - 6 * x * y, d[2] = 6 * x * y - 3 * x * x, d[3] = 3 * x * x;
+  };
+  double bu[4], du[4], bv[4], dv[4];
+  basis(s, bu, du), basis(t, bv, dv);
+  double a[3][4] = {{0}};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 4; ++k) {
+        a[0][k] += bu[i] * bv[j] * pa.p[i][j][k];
+        a[1][k] += du[i] * bv[j] * pa.p[i][j][k];
+        a[2][k] += bu[i] * dv[j] * pa.p[i][j][k];
+      }
+  double w = a[0][3];
+  p = mk(a[0][0], a[0][1], a[0][2]) * (1 / w);
+  ps = (mk(a[1][0], a[1][1], a[1][2]) - p * a[1][3]) * (1 / w);
+  pt = (mk(a[2][0], a[2][1], a[2][2]) - p * a[2][3]) * (1 / w);
+}
+
+// surface3.rs:819-830, 866-926: the radius of convexity of a rational bicubic, from
+// interval bounds on its derivatives over its Bernstein hulls (relative to the
+// points' middle), with lambda_min(G) bounded two ways
+double patch_radius(const Patch4& pw) {
+  const int du = 3, dv = 3;
+  V c{0, 0, 0};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) c = c + point(pw.p[i][j]);
+  c = c * (1.0 / 16);
+  auto q = [&](int i, int j, int k) {
+    const H& h = pw.p[i][j];
+    double cc[3] = {c.x, c.y, c.z};
+    return k < 3 ? h[k] - cc[k] * h[3] : h[3];
+  };
+  auto hull = [&](int which, int k) {
+    Iv r{INF, -INF};
+    for (int i = 0; i <= du; ++i)
+      for (int j = 0; j <= dv; ++j) {
+        double x;
+        if (which == 0) x = q(i, j, k);
+        else if (which == 1) { if (i >= du) continue; x = du * (q(i + 1, j, k) - q(i, j, k)); }
+        else if (which == 2) { if (j >= dv) continue; x = dv * (q(i, j + 1, k) - q(i, j, k)); }
+        else if (which == 3) { if (i + 1 >= du) continue; x = du * (du - 1.0) * (q(i + 2, j, k) - 2 * q(i + 1, j, k) + q(i, j, k)); }
+        else if (which == 4) { if (i >= du || j >= dv) continue; x = du * dv * (q(i + 1, j + 1, k) - q(i + 1, j, k) - q(i, j + 1, k) + q(i, j, k)); }
+        else { if (j + 1 >= dv) continue; x = dv * (dv - 1.0) * (q(i, j + 2, k) - 2 * q(i, j + 1, k) + q(i, j, k)); }
+        r.a = std::min(r.a, x), r.b = std::max(r.b, x);
+      }
+    return r;
+  };
+  Iv A[6][4];
+  for (int w = 0; w < 6; ++w)
+    for (int k = 0; k < 4; ++k) A[w][k] = hull(w, k);
+  Iv wt = A[0][3];
+  if (!(wt.a > 0)) return 0.0;
+  Iv s[3];
+  for (int k = 0; k < 3; ++k) {
+    Iv r{INF, -INF};
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        V pp = point(pw.p[i][j]) - c;
+        double x = k == 0 ? pp.x : k == 1 ? pp.y : pp.z;
+        r.a = std::min(r.a, x), r.b = std::max(r.b, x);
+      }
+    s[k] = r;
+  }
+  Iv ss[3], st[3], sss[3], sst[3], stt[3];
+  for (int k = 0; k < 3; ++k) {
+    ss[k] = divpos(sub(A[1][k], mul(s[k], A[1][3])), wt);
+    st[k] = divpos(sub(A[2][k], mul(s[k], A[2][3])), wt);
+  }
+  for (int k = 0; k < 3; ++k) {
+    sss[k] = divpos(sub(sub(A[3][k], scale(mul(ss[k], A[1][3]), 2.0)), mul(s[k], A[3][3])), wt);
+    sst[k] = divpos(sub(sub(sub(A[4][k], mul(ss[k], A[2][3])), mul(st[k], A[1][3])), mul(s[k], A[4][3])), wt);
+    stt[k] = divpos(sub(sub(A[5][k], scale(mul(st[k], A[2][3]), 2.0)), mul(s[k], A[5][3])), wt);
+  }
+  V p0, ps0, pt0;
+  eval_patch(pw, 0.5, 0.5, p0, ps0, pt0);
+  double sa = len(ps0), sb = len(pt0);
+  if (!(sa > 0)) sa = 1;
+  if (!(sb > 0)) sb = 1;
+  for (int k = 0; k < 3; ++k) {
+    ss[k] = scale(ss[k], 1 / sa), st[k] = scale(st[k], 1 / sb);
+    sss[k] = scale(sss[k], 1 / (sa * sa)), sst[k] = scale(sst[k], 1 / (sa * sb)), stt[k] = scale(stt[k], 1 / (sb * sb));
+  }
+  ps0 = ps0 * (1 / sa), pt0 = pt0 * (1 / sb);
+  auto mig2 = [](const Iv* a) { return mig(a[0]) * mig(a[0]) + mig(a[1]) * mig(a[1]) + mig(a[2]) * mig(a[2]); };
+  double ga = mig2(ss), gc = mig2(st);
+  double gb = mag(add(add(mul(ss[0], st[0]), mul(ss[1], st[1])), mul(ss[2], st[2])));
+  double by_entries = 0.5 * ((ga + gc) - std::sqrt((ga - gc) * (ga - gc) + 4 * gb * gb));
+  double a0 = dot(ps0, ps0), b0 = dot(ps0, pt0), c0 = dot(pt0, pt0);
+  double sigma0 = std::sqrt(std::max(0.5 * ((a0 + c0) - std::sqrt((a0 - c0) * (a0 - c0) + 4 * b0 * b0)), 0.0));
+  auto dev = [](const Iv* iv, V x) {
+    double xs[3] = {x.x, x.y, x.z}, s2 = 0;
+    for (int k = 0; k < 3; ++k) {
+      double m = std::max(std::fabs(iv[k].a - xs[k]), std::fabs(iv[k].b - xs[k]));
+      s2 += m * m;
+    }
+    return s2;
+  };
+  double spread = std::sqrt(dev(ss, ps0) + dev(st, pt0));
+  double by_middle = std::pow(std::max(sigma0 - spread, 0.0), 2);
+  double lambda = std::max(by_entries, by_middle);
+  // gpu.rs:87-98 rounding allowance on the bounds
+  lambda = lambda - std::fabs(lambda) * 1e-6 - 1e-30;
+  if (!(lambda > 0)) return 0.0;
+  auto mag2 = [](const Iv* a) { return mag(a[0]) * mag(a[0]) + mag(a[1]) * mag(a[1]) + mag(a[2]) * mag(a[2]); };
+  double hh = std::sqrt(mag2(sss) + 2 * mag2(sst) + mag2(stt)) * (1 + 1e-6);
+  return hh > 0 ? 0.99 * lambda / hh : INF;  // RHO_MARGIN, surface3.rs:807
+}
+
+// surface3.rs:931-970 a cubic piece's radius |C'|^2_min / |C''|_max
+double piece_radius(const H* pw) {
+  const int n = 3;
+  V c{0, 0, 0};
+  for (int i = 0; i < 4; ++i) c = c + point(pw[i]);
+  c = c * 0.25;
+  auto q = [&](int i, int k) {
+    double cc[3] = {c.x, c.y, c.z};
+    return k < 3 ? pw[i][k] - cc[k] * pw[i][3] : pw[i][3];
+  };
+  Iv a[4], a1[4], a2[4];
+  for (int k = 0; k < 4; ++k) {
+    a[k] = iv_of({q(0, k), q(1, k), q(2, k), q(3, k)});
+    a1[k] = iv_of({n * (q(1, k) - q(0, k)), n * (q(2, k) - q(1, k)), n * (q(3, k) - q(2, k))});
+    a2[k] = iv_of({n * (n - 1.0) * (q(2, k) - 2 * q(1, k) + q(0, k)), n * (n - 1.0) * (q(3, k) - 2 * q(2, k) + q(1, k))});
+  }
+  Iv w = a[3];
+  if (!(w.a > 0)) return 0.0;
+  Iv s[3], d1[3], d2[3];
+  for (int k = 0; k < 3; ++k) {
+    Iv r{INF, -INF};
+    for (int i = 0; i < 4; ++i) {
+      V pp = point(pw[i]) - c;
+      double x = k == 0 ? pp.x : k == 1 ? pp.y : pp.z;
+      r.a = std::min(r.a, x), r.b = std::max(r.b, x);
+    }
+    s[k] = r;
+    d1[k] = divpos(sub(a1[k], mul(s[k], a1[3])), w);
+  }
+  for (int k = 0; k < 3; ++k) d2[k] = divpos(sub(sub(a2[k], scale(mul(d1[k], a1[3]), 2.0)), mul(s[k], a2[3])), w);
+  // the middle's derivative (de Casteljau at 1/2)
+  H b[4];
+  for (int i = 0; i < 4; ++i) b[i] = pw[i];
+  H d1h;
+  for (int lev = 1; lev <= 3; ++lev) {
+    if (lev == 3)
+      for (int k = 0; k < 4; ++k) d1h[k] = 3 * (b[1][k] - b[0][k]);
+    for (int i = 0; i <= 3 - lev; ++i)
+      for (int k = 0; k < 4; ++k) b[i][k] = 0.5 * (b[i][k] + b[i + 1][k]);
+  }
+  V p = point(b[0]);
+  V e1 = (mk(d1h[0], d1h[1], d1h[2]) - p * d1h[3]) * (1 / b[0][3]);
+  double ex[3] = {e1.x, e1.y, e1.z}, spread = 0;
+  for (int k = 0; k < 3; ++k) {
+    double m = std::max(std::fabs(d1[k].a - ex[k]), std::fabs(d1[k].b - ex[k]));
+    spread += m * m;
+  }
+  spread = std::sqrt(spread);
+  double l = std::max(mig(d1[0]) * mig(d1[0]) + mig(d1[1]) * mig(d1[1]) + mig(d1[2]) * mig(d1[2]), std::pow(std::max(len(e1) - spread, 0.0), 2));
+  l = l - std::fabs(l) * 1e-6 - 1e-30;
+  if (!(l > 0)) return 0.0;
+  double h = std::sqrt(mag(d2[0]) * mag(d2[0]) + mag(d2[1]) * mag(d2[1]) + mag(d2[2]) * mag(d2[2])) * (1 + 1e-6);
+  return h > 0 ? 0.99 * l / h : INF;
+}
+
+// gpu.rs:110-112 the split rule
+bool split_rule(double rho, double diag, double need, int depth) { return depth < 8 && rho < 2.0 * diag + need && diag > 0.25 * need; }
+
+float down(double x) {
+  float f = float(x);
+  if (double(f) > x) f = std::nextafter(f, -INFINITY);
+  return f;
+}
+float up(double x) {
+  float f = float(x);
+  if (double(f) < x) f = std::nextafter(f, INFINITY);
+  return f;
+}
+
+}  // namespace
+
+std::vector<float> to_f32(const std::vector<double>& f) {
+  std::vector<float> r(f.size());
+  for (size_t i = 0; i < f.size(); ++i) r[i] = float(f[i]);
+  return r;
+}
+
+bool pack_part(const icam_part_desc* part, double need, Packed& out) {
+  Packed& P = out;
+  Layout& L = P.L;
+  V o = mk(0.5 * (part->lo[0] + part->hi[0]), 0.5 * (part->lo[1] + part->hi[1]), 0.5 * (part->lo[2] + part->hi[2]));
+  P.origin[0] = o.x, P.origin[1] = o.y, P.origin[2] = o.z;
+  auto rel = [&](const double* w) { return of(w) - o; };
+  auto relh = [&](const double* h) { return H{h[0] - o.x * h[3], h[1] - o.y * h[3], h[2] - o.z * h[3], h[3]}; };
+
+  // sections are built into separate vectors, then concatenated
+  std::vector<double> face_f, edge_f, vert_f, leaf_f, span_f, cleaf_f, cpiece_f, cell_f, inst_f, tpiece_f, uvmap_f;
+  std::vector<uint32_t> face_u, edge_u, leaf_u, span_u, cleaf_u, cpiece_u, cell_u, inst_u, among_u;
+  std::vector<Box> pbox;
+  std::vector<uint32_t> pcode;
+  std::vector<Box> face_box(part->n_faces);
+  std::vector<Box> edge_box(part->n_edges);
+
+  // ---- edges first (face boxes use them); their records and pieces
+  for (uint32_t e = 0; e < part->n_edges; ++e) {
+    const icam_edge_desc& ed = part->edges[e];
+    double rec[EDGE_F] = {0};
+    uint32_t ru[EDGE_U] = {0};
+    ru[0] = ed.kind, ru[1] = ed.faces[0], ru[2] = ed.faces[1], ru[3] = 0xffffffffu;
+    if (ed.uvmap) {
+      ru[3] = uint32_t(uvmap_f.size());
+      uint32_t np = std::max<uint32_t>(ed.n_pieces, 1);
+      uvmap_f.insert(uvmap_f.end(), ed.uvmap, ed.uvmap + 8 * np);
+    }
+    Box bx;
+    if (ed.kind == ICAM_LINE) {
+      V a = rel(ed.a), b = rel(ed.b);
+      rec[0] = a.x, rec[1] = a.y, rec[2] = a.z, rec[3] = b.x, rec[4] = b.y, rec[5] = b.z;
+      bx.add(a), bx.add(b);
+    } else if (ed.kind == ICAM_ARC) {
+      V c = rel(ed.o), x = of(ed.x), z = of(ed.z), y = cross(z, x);
+      double r[13] = {c.x, c.y, c.z, x.x, x.y, x.z, y.x, y.y, y.z, ed.r, ed.t0, ed.sweep, std::fabs(std::fabs(ed.sweep) - TAU) < 1e-12 ? 1.0 : 0.0};
+      std::copy(r, r + 13, rec);
+      V axes[3] = {mk(1, 0, 0), mk(0, 1, 0), mk(0, 0, 1)};
+      for (V ax : axes) {
+        double ex = std::sqrt(std::max(1 - dot(z, ax) * dot(z, ax), 0.0)) * ed.r;
+        bx.add(c + ax * ex), bx.add(c - ax * ex);
+      }
+    } else {
+      if (ed.degree > 3 || ed.degree < 1) {
+        P.error = "spline edge of degree " + std::to_string(ed.degree) + " (at most 3 are packed)";
+        return false;
+      }
+      ru[4] = uint32_t(cpiece_f.size() / CPIECE_F), ru[5] = ed.n_pieces;
+      for (uint32_t k = 0; k < ed.n_pieces; ++k) {
+        std::vector<H> pts;
+        for (uint32_t i = 0; i <= ed.degree; ++i) pts.push_back(relh(ed.pw + (k * (ed.degree + 1) + i) * 4));
+        while (pts.size() < 4) pts = raise(pts);
+        uint32_t cp = uint32_t(cpiece_f.size() / CPIECE_F);
+        for (auto& h : pts) cpiece_f.insert(cpiece_f.end(), h.begin(), h.end());
+        cpiece_u.insert(cpiece_u.end(), {e, k, ed.n_pieces, 0});
+        // the piece's tree of parts, breadth first; leaves are primitives
+        struct Part_ {
+          H p[4];
+          double t0, t1;
+          int depth;
+        };
+        std::vector<Part_> todo;
+        Part_ root;
+        for (int i = 0; i < 4; ++i) root.p[i] = pts[i];
+        root.t0 = 0, root.t1 = 1, root.depth = 0;
+        todo.push_back(root);
+        while (!todo.empty()) {
+          Part_ pp = todo.back();
+          todo.pop_back();
+          Box b;
+          for (int i = 0; i < 4; ++i) b.add(point(pp.p[i]));
+          double rho = piece_radius(pp.p);
+          ++P.nodes;
+          if (split_rule(rho, b.diag(), need, pp.depth)) {
+            Part_ l, r;
+            split_curve(pp.p, 0.5, l.p, r.p);
+            double tm = 0.5 * (pp.t0 + pp.t1);
+            l.t0 = pp.t0, l.t1 = tm, r.t0 = tm, r.t1 = pp.t1;
+            l.depth = r.depth = pp.depth + 1;
+            todo.push_back(r), todo.push_back(l);
+            continue;
+          }
+          uint32_t id = uint32_t(cleaf_f.size() / CLEAF_F);
+          cleaf_f.insert(cleaf_f.end(), {rho, pp.t0, pp.t1, 0.0});
+          cleaf_u.insert(cleaf_u.end(), {cp, e});
+          pbox.push_back(b);
+          pcode.push_back((P_CLEAF << 28) | id);
+        }
+        for (auto& h : pts) bx.add(point(h));
+      }
+    }
+    edge_f.insert(edge_f.end(), rec, rec + EDGE_F);
+    edge_u.insert(edge_u.end(), ru, ru + EDGE_U);
+    edge_box[e] = bx;
+  }
+
+  // ---- faces
+  for (uint32_t fi = 0; fi < part->n_faces; ++fi) {
+    const icam_face_desc& fd = part->faces[fi];
+    // trim cells
+    TrimBuild tb;
+    tb.build(fd);
+    uint32_t cell0 = uint32_t(cell_f.size() / CELL_F), inst0 = uint32_t(inst_f.size() / INST_F), piece0 = uint32_t(tpiece_f.size() / TPIECE_F),
+             among0 = uint32_t(among_u.size());
+    for (uint32_t p = 0; p < fd.n_pieces; ++p) tpiece_f.insert(tpiece_f.end(), fd.pieces + 8 * p, fd.pieces + 8 * p + 8);
+    for (size_t i = 0; i < tb.inst.size(); ++i) {
+      inst_f.insert(inst_f.end(), {tb.inst_shift[i][0], tb.inst_shift[i][1]});
+      inst_u.push_back(piece0 + tb.inst_piece[i]);
+    }
+    for (auto& c : tb.cells) {
+      cell_f.insert(cell_f.end(), {c.lo[0], c.lo[1], c.hi[0], c.hi[1], c.r[0], c.r[1]});
+      uint32_t a = c.kind == 1 ? among0 + c.a : c.kind == 2 ? cell0 + c.a : 0;
+      cell_u.insert(cell_u.end(), {c.inside, c.kind, a, c.b});
+    }
+    for (uint32_t x : tb.among) among_u.push_back(inst0 + x);
+    bool whole = fd.n_pieces == 0;
+
+    double rec[FACE_F] = {0};
+    uint32_t ru[FACE_U] = {0};
+    ru[0] = fd.kind, ru[1] = cell0, ru[2] = whole ? 1u : 0u;
+    rec[14] = fd.sense;
+    rec[17] = fd.lo[0], rec[18] = fd.lo[1], rec[19] = fd.hi[0], rec[20] = fd.hi[1], rec[21] = fd.period[0], rec[22] = fd.period[1];
+    Box fb;
+    if (fd.kind == ICAM_NURBS) {
+      if (fd.du > 3 || fd.dv > 3) {
+        P.error = "B-spline face of degree (" + std::to_string(fd.du) + ", " + std::to_string(fd.dv) + ") (at most bicubic is packed)";
+        return false;
+      }
+      // spans: along u per column, then along v per Bezier row (surface3.rs:1150-1165)
+      std::vector<double> ku(fd.ku, fd.ku + fd.nu + fd.du + 1), kv(fd.kv, fd.kv + fd.nv + fd.dv + 1);
+      std::vector<std::vector<std::vector<double>>> cols;
+      std::vector<std::pair<double, double>> ur, vr;
+      for (uint32_t j = 0; j < fd.nv; ++j) {
+        std::vector<double> pts;
+        for (uint32_t i = 0; i < fd.nu; ++i) pts.insert(pts.end(), fd.pw + (i * fd.nv + j) * 4, fd.pw + (i * fd.nv + j) * 4 + 4);
+        std::vector<std::vector<double>> pcs;
+        ur.clear();
+        decompose(int(fd.du), ku, pts, 4, pcs, ur);
+        cols.push_back(pcs);
+      }
+      int nsu = int(ur.size()), nsv = 0;
+      uint32_t span0 = uint32_t(span_f.size() / SPAN_F);
+      std::vector<std::vector<std::vector<std::vector<double>>>> rows(nsu);
+      for (int a = 0; a < nsu; ++a) {
+        rows[a].resize(fd.du + 1);
+        for (uint32_t i = 0; i <= fd.du; ++i) {
+          std::vector<double> pts;
+          for (uint32_t j = 0; j < fd.nv; ++j) pts.insert(pts.end(), cols[j][a].begin() + i * 4, cols[j][a].begin() + i * 4 + 4);
+          vr.clear();
+          decompose(int(fd.dv), kv, pts, 4, rows[a][i], vr);
+        }
+        nsv = int(vr.size());
+      }
+      for (int a = 0; a < nsu; ++a)
+        for (int b = 0; b < nsv; ++b) {
+          // the span's patch, raised to bicubic (gpu.rs:149-171)
+          std::vector<std::vector<H>> pw(fd.du + 1);
+          for (uint32_t i = 0; i <= fd.du; ++i)
+            for (uint32_t j = 0; j <= fd.dv; ++j) pw[i].push_back(relh(&rows[a][i][b][j * 4]));
+          for (auto& r : pw)
+            while (r.size() < 4) r = raise(r);
+          while (pw.size() < 4) {
+            std::vector<std::vector<H>> np(pw.size() + 1, std::vector<H>(4));
+            for (int j = 0; j < 4; ++j) {
+              std::vector<H> col;
+              for (auto& r : pw) col.push_back(r[j]);
+              col = raise(col);
+              for (size_t i = 0; i < col.size(); ++i) np[i][j] = col[i];
+            }
+            pw = np;
+          }
+          Patch4 pa;
+          for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) pa.p[i][j] = pw[i][j];
+          uint32_t sid = uint32_t(span_f.size() / SPAN_F);
+          for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) span_f.insert(span_f.end(), pa.p[i][j].begin(), pa.p[i][j].end());
+          span_f.insert(span_f.end(), {ur[a].first, ur[a].second, vr[b].first, vr[b].second, 0, 0, 0, 0});
+          span_u.insert(span_u.end(), {fi, whole ? 0u : 1u, cell0, 0});
+          // the span's tree of sub-patches, breadth first (gpu.rs:292-350)
+          struct Sub {
+            Patch4 p;
+            double s0, s1, t0, t1;
+            int depth;
+            int inside;
+          };
+          std::vector<Sub> queue;
+          double klo[2] = {ur[a].first, vr[b].first}, khi[2] = {ur[a].second, vr[b].second};
+          queue.push_back({pa, 0, 1, 0, 1, 0, whole ? 1 : tb.box_class(klo, khi)});
+          for (size_t qi = 0; qi < queue.size(); ++qi) {
+            Sub sb = queue[qi];
+            if (sb.inside == 0) continue;  // wholly outside the face's loops
+            Box bb;
+            for (int i = 0; i < 4; ++i)
+              for (int j = 0; j < 4; ++j) bb.add(point(sb.p.p[i][j]));
+            double rho = patch_radius(sb.p);
+            ++P.nodes;
+            if (split_rule(rho, bb.diag(), need, sb.depth)) {
+              Patch4 q[4];
+              quarter(sb.p, q);
+              double sm = 0.5 * (sb.s0 + sb.s1), tm = 0.5 * (sb.t0 + sb.t1);
+              for (int k = 0; k < 4; ++k) {
+                Sub c;
+                c.p = q[k];
+                c.s0 = (k & 1) ? sm : sb.s0, c.s1 = (k & 1) ? sb.s1 : sm;
+                c.t0 = (k & 2) ? tm : sb.t0, c.t1 = (k & 2) ? sb.t1 : tm;
+                c.depth = sb.depth + 1;
+                if (sb.inside == 1) {
+                  c.inside = 1;
+                } else {
+                  double du_ = ur[a].second - ur[a].first, dv_ = vr[b].second - vr[b].first;
+                  double clo[2] = {ur[a].first + c.s0 * du_, vr[b].first + c.t0 * dv_}, chi[2] = {ur[a].first + c.s1 * du_, vr[b].first + c.t1 * dv_};
+                  c.inside = tb.box_class(clo, chi);
+                }
+                queue.push_back(c);
+              }
+              continue;
+            }
+            uint32_t id = uint32_t(leaf_f.size() / LEAF_F);
+            leaf_f.insert(leaf_f.end(), {rho, sb.s0, sb.s1, sb.t0, sb.t1, 0, 0, 0});
+            leaf_u.insert(leaf_u.end(), {sid, fi, sb.inside == 1 ? 1u : 0u, 0});
+            pbox.push_back(bb);
+            pcode.push_back((P_LEAF << 28) | id);
+            fb.add(bb);
+          }
+        }
+      ru[3] = span0, ru[4] = uint32_t(nsu), ru[5] = uint32_t(nsv);
+    } else {
+      V c = rel(fd.o), x = of(fd.x), z = of(fd.z), y = cross(z, x);
+      double r[12] = {c.x, c.y, c.z, x.x, x.y, x.z, y.x, y.y, y.z, z.x, z.y, z.z};
+      std::copy(r, r + 12, rec);
+      rec[12] = fd.c[0], rec[13] = fd.c[1];
+      double psi0 = 0, span = TAU;
+      int narc = 4;
+      if (fd.kind == ICAM_TORUS && fd.c[0] < fd.c[1]) {
+        double pp = std::acos(std::min(std::max(-fd.c[0] / fd.c[1], -1.0), 1.0));
+        psi0 = -pp, span = 2 * pp;
+        narc = int(std::max(std::ceil(span / (0.5 * PI) - 1e-9), 1.0));
+      }
+      rec[15] = psi0, rec[16] = span, ru[3] = uint32_t(narc);
+      // solid3.rs:1211-1282 face_box
+      V axes[3] = {mk(1, 0, 0), mk(0, 1, 0), mk(0, 0, 1)};
+      auto circle = [&](V cc, V zz, double rad) {
+        for (V ax : axes) {
+          double ex = std::sqrt(std::max(1 - dot(zz, ax) * dot(zz, ax), 0.0)) * rad;
+          fb.add(cc + ax * ex), fb.add(cc - ax * ex);
+        }
+      };
+      if (fd.kind == ICAM_PLANE) {
+        for (uint32_t e = 0; e < part->n_edges; ++e)
+          if (part->edges[e].faces[0] == fi || part->edges[e].faces[1] == fi) fb.add(edge_box[e]);
+      } else if (fd.kind == ICAM_CYLINDER || fd.kind == ICAM_CONE) {
+        double h0 = INF, h1 = -INF;
+        for (uint32_t e = 0; e < part->n_edges; ++e)
+          if (part->edges[e].faces[0] == fi || part->edges[e].faces[1] == fi)
+            for (int k = 0; k < 8; ++k) {
+              const Box& eb = edge_box[e];
+              V q = mk((k & 1) ? eb.hi.x : eb.lo.x, (k & 2) ? eb.hi.y : eb.lo.y, (k & 4) ? eb.hi.z : eb.lo.z);
+              double h = dot(q - c, z);
+              h0 = std::min(h0, h), h1 = std::max(h1, h);
+            }
+        for (double h : {h0, h1}) circle(c + z * h, z, fd.kind == ICAM_CONE ? std::fabs(fd.c[0] + h * std::tan(fd.c[1])) : fd.c[0]);
+      } else if (fd.kind == ICAM_SPHERE) {
+        fb.add(c - mk(fd.c[0], fd.c[0], fd.c[0])), fb.add(c + mk(fd.c[0], fd.c[0], fd.c[0]));
+      } else {
+        for (double h : {-fd.c[1], fd.c[1]}) circle(c + z * h, z, fd.c[0] + fd.c[1]);
+      }
+      pbox.push_back(fb);
+      pcode.push_back((P_FACE << 28) | fi);
+    }
+    face_box[fi] = fb;
+    face_f.insert(face_f.end(), rec, rec + FACE_F);
+    face_u.insert(face_u.end(), ru, ru + FACE_U);
+  }
+
+  // ---- edge and vertex primitives (after the faces: their order in the list
+  // follows the reference's tree ids, faces, edges, vertices)
+  for (uint32_t e = 0; e < part->n_edges; ++e)
+    if (part->edges[e].kind != ICAM_SPLINE) {
+      pbox.push_back(edge_box[e]);
+      pcode.push_back((P_EDGE << 28) | e);
+    }
+  for (uint32_t v = 0; v < part->n_verts; ++v) {
+    V p = rel(part->verts[v].p), pn = of(part->verts[v].pn);
+    vert_f.insert(vert_f.end(), {p.x, p.y, p.z, pn.x, pn.y, pn.z, 0, 0});
+    Box b;
+    b.add(p);
+    pbox.push_back(b);
+    pcode.push_back((P_VERT << 28) | v);
+  }
+
+  // ---- concatenate
+  auto put = [](std::vector<double>& dst, const std::vector<double>& src) {
+    uint32_t at = uint32_t(dst.size());
+    dst.insert(dst.end(), src.begin(), src.end());
+    return at;
+  };
+  auto putu = [](std::vector<uint32_t>& dst, const std::vector<uint32_t>& src) {
+    uint32_t at = uint32_t(dst.size());
+    dst.insert(dst.end(), src.begin(), src.end());
+    return at;
+  };
+  P.F.clear(), P.U.clear();
+  L.face_f = put(P.F, face_f), L.edge_f = put(P.F, edge_f), L.vert_f = put(P.F, vert_f), L.leaf_f = put(P.F, leaf_f);
+  L.span_f = put(P.F, span_f), L.cleaf_f = put(P.F, cleaf_f), L.cpiece_f = put(P.F, cpiece_f), L.cell_f = put(P.F, cell_f);
+  L.inst_f = put(P.F, inst_f), L.tpiece_f = put(P.F, tpiece_f), L.uvmap_f = put(P.F, uvmap_f);
+  P.F.push_back(0);
+  L.face_u = putu(P.U, face_u), L.edge_u = putu(P.U, edge_u), L.leaf_u = putu(P.U, leaf_u), L.span_u = putu(P.U, span_u);
+  L.cleaf_u = putu(P.U, cleaf_u), L.cpiece_u = putu(P.U, cpiece_u), L.cell_u = putu(P.U, cell_u), L.inst_u = putu(P.U, inst_u);
+  L.among_u = putu(P.U, among_u);
+  P.U.push_back(0);
+  // uvmap offsets are relative to the uvmap section
+  L.n_faces = part->n_faces, L.n_edges = part->n_edges, L.n_verts = part->n_verts;
+  L.n_spans = uint32_t(span_u.size() / SPAN_U), L.n_leaves = uint32_t(leaf_u.size() / LEAF_U), L.n_cleaves = uint32_t(cleaf_u.size() / CLEAF_U);
+  L.n_cpieces = uint32_t(cpiece_u.size() / CPIECE_U);
+  L.n_prims = uint32_t(pcode.size());
+  P.pbox.resize(6 * pbox.size());
+  for (size_t i = 0; i < pbox.size(); ++i) {
+    P.pbox[6 * i + 0] = down(pbox[i].lo.x), P.pbox[6 * i + 1] = down(pbox[i].lo.y), P.pbox[6 * i + 2] = down(pbox[i].lo.z);
+    P.pbox[6 * i + 3] = up(pbox[i].hi.x), P.pbox[6 * i + 4] = up(pbox[i].hi.y), P.pbox[6 * i + 5] = up(pbox[i].hi.z);
+  }
+  P.pcode = pcode;
+  V plo = rel(part->lo), phi = rel(part->hi);
+  L.lo[0] = down(plo.x), L.lo[1] = down(plo.y), L.lo[2] = down(plo.z);
+  L.hi[0] = up(phi.x), L.hi[1] = up(phi.y), L.hi[2] = up(phi.z);
+  L.tol = float(part->tol);
+  L.diag = len(phi - plo);
+  double ext = std::max(0.5 * L.diag, 1.0);
+  L.eps32 = float(8.0 * 1.1920929e-7 * ext);  // partgpu.rs:139
+  L.eps64 = 8.0 * 2.220446049250313e-16 * ext;
+  return true;
+}
+
+}  // namespace icam

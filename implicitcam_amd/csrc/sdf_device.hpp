@@ -1,0 +1,749 @@
+// Device side of the exact signed distance: one lane per point, every feature's
+// closed form or certified Newton, in a compute type T (float or double).
+//
+// The rules are the reference evaluator's (crates/icam-core/src/solid3.rs:1750-1940)
+// and its kernel's (crates/icam-core/src/solid3/gpu.wgsl: face_d :415, edge_foot
+// :820, sub_min :679-756, curve_min_on :764-815, node_hit :1794-1906):
+//  - an analytic face answers with its closed-form foot when the foot lies inside
+//    the face by its trim loops (crossing parity in the cell the foot falls in);
+//  - a line or arc edge with its foot when interior; a vertex with itself;
+//  - a B-spline sub-patch or curve part is read by projected Newton only where its
+//    radius of convexity certifies one minimum (the point's farthest distance to its
+//    box under the radius); the minimum counts when it is a critical point inside the
+//    face; a part without that certificate leaves its box distance as a lower bound;
+//  - signs from the face normal, the two faces' normals summed at an edge point, the
+//    vertex pseudo-normal.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "layout.hpp"
+
+namespace icam {
+
+// ------------------------------------------------------------------ math by type
+__device__ inline float msqrt(float x) { return __builtin_sqrtf(x); }
+__device__ inline double msqrt(double x) { return __builtin_sqrt(x); }
+__device__ inline float matan2(float y, float x) { return atan2f(y, x); }
+__device__ inline double matan2(double y, double x) { return atan2(y, x); }
+__device__ inline float mtan(float x) { return tanf(x); }
+__device__ inline double mtan(double x) { return tan(x); }
+__device__ inline float msin(float x) { return sinf(x); }
+__device__ inline double msin(double x) { return sin(x); }
+__device__ inline float mcos(float x) { return cosf(x); }
+__device__ inline double mcos(double x) { return cos(x); }
+__device__ inline float mfloor(float x) { return floorf(x); }
+__device__ inline double mfloor(double x) { return floor(x); }
+__device__ inline float mabs(float x) { return fabsf(x); }
+__device__ inline double mabs(double x) { return fabs(x); }
+template <class T> __device__ inline T mmin(T a, T b) { return a < b ? a : b; }
+template <class T> __device__ inline T mmax(T a, T b) { return a > b ? a : b; }
+template <class T> __device__ inline T mclamp(T x, T a, T b) { return mmin(mmax(x, a), b); }
+
+template <class T> struct TypeEps;
+template <> struct TypeEps<float> {
+  static constexpr float ulp = 1.1920929e-7f;
+  static constexpr float big = 3.0e38f;
+};
+template <> struct TypeEps<double> {
+  static constexpr double ulp = 2.220446049250313e-16;
+  static constexpr double big = 1.0e300;
+};
+
+template <class T> struct v3 {
+  T x, y, z;
+};
+template <class T> __device__ inline v3<T> mk3(T a, T b, T c) { return v3<T>{a, b, c}; }
+template <class T> __device__ inline v3<T> operator+(v3<T> a, v3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> __device__ inline v3<T> operator-(v3<T> a, v3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> __device__ inline v3<T> operator*(v3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> __device__ inline T dot(v3<T> a, v3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> __device__ inline v3<T> cross(v3<T> a, v3<T> b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+template <class T> __device__ inline T len(v3<T> a) { return msqrt(dot(a, a)); }
+template <class T> __device__ inline v3<T> ld3(const T* p) { return {p[0], p[1], p[2]}; }
+template <class T, class S> __device__ inline v3<T> cv3(v3<S> a) { return {T(a.x), T(a.y), T(a.z)}; }
+
+// The tables of one compute type.
+template <class T> struct Tab {
+  const T* F;
+  const uint32_t* U;
+  const float* pbox;
+  const uint32_t* pcode;
+  Layout L;
+};
+
+template <class T> __device__ inline T box_dist(v3<T> p, const float* b) {
+  T dx = mmax(mmax(T(b[0]) - p.x, p.x - T(b[3])), T(0));
+  T dy = mmax(mmax(T(b[1]) - p.y, p.y - T(b[4])), T(0));
+  T dz = mmax(mmax(T(b[2]) - p.z, p.z - T(b[5])), T(0));
+  return msqrt(dx * dx + dy * dy + dz * dz);
+}
+// The distance to the farthest point of a box.
+template <class T> __device__ inline T far_dist(v3<T> p, const float* b) {
+  T dx = mmax(mabs(p.x - T(b[0])), mabs(p.x - T(b[3])));
+  T dy = mmax(mabs(p.y - T(b[1])), mabs(p.y - T(b[4])));
+  T dz = mmax(mabs(p.z - T(b[2])), mabs(p.z - T(b[5])));
+  return msqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// ------------------------------------------------------------------ trim loops
+// trim2.rs:104-136 a cubic coordinate and its derivative's quadratic
+template <class T> __device__ inline T beval3(T c0, T c1, T c2, T c3, T t) {
+  if (c0 == c1 && c1 == c2 && c2 == c3) return c0;
+  T s = T(1) - t;
+  T a0 = s * c0 + t * c1, a1 = s * c1 + t * c2, a2 = s * c2 + t * c3;
+  T b0 = s * a0 + t * a1, b1 = s * a1 + t * a2;
+  return s * b0 + t * b1;
+}
+template <class T> __device__ inline T beval2(T c0, T c1, T c2, T t) {
+  if (c0 == c1 && c1 == c2) return c0;
+  T s = T(1) - t;
+  T a0 = s * c0 + t * c1, a1 = s * c1 + t * c2;
+  return s * a0 + t * a1;
+}
+
+// trim2.rs:1150-1190 the parameter where a monotone cubic coordinate equals x, x in
+// its half-open range [min, max): safeguarded Newton. `near` is set when x lies
+// within m of an end of that range (the half-open rule decides there).
+template <class T> __device__ inline bool crossing(const T a[4], T x, T& out, T m, bool& near) {
+  T p0 = a[0], p1 = a[3];
+  T lo_ = mmin(p0, p1), hi_ = mmax(p0, p1);
+  if (m > T(0) && (mabs(x - lo_) <= m || mabs(x - hi_) <= m) && hi_ > lo_) near = true;
+  if (!(lo_ <= x && x < hi_)) return false;
+  T lo = 0, hi = 1;
+  T flo = beval3(a[0], a[1], a[2], a[3], T(0)) - x;
+  if (flo == T(0)) {
+    out = 0;
+    return true;
+  }
+  T t = mclamp((x - p0) / (p1 - p0), T(0), T(1));
+  const int iters = sizeof(T) == 4 ? 30 : 60;
+  for (int it = 0; it < iters; ++it) {
+    T ft = beval3(a[0], a[1], a[2], a[3], t) - x;
+    if (ft == T(0)) {
+      out = t;
+      return true;
+    }
+    if ((ft < T(0)) == (flo < T(0))) {
+      lo = t;
+      flo = ft;
+    } else {
+      hi = t;
+    }
+    T d = T(3) * beval2(a[1] - a[0], a[2] - a[1], a[3] - a[2], t);
+    T nt = d != T(0) ? t - ft / d : T(0.5) * (lo + hi);
+    if (!(nt > lo && nt < hi)) nt = T(0.5) * (lo + hi);
+    if (mabs(nt - t) <= (sizeof(T) == 4 ? T(1e-7) : T(1e-16))) {
+      out = nt;
+      return true;
+    }
+    t = nt;
+  }
+  out = t;
+  return true;
+}
+
+// trim2.rs:1205-1260 crossing parity of the path a -> (a.u, b.v) -> b, moved up by an
+// infinitesimal e and right by e^2. With m > 0, `near` is set when a decision was
+// taken within m (the caller then decides again in double precision).
+template <class T> __device__ bool parity(const Tab<T>& tb, const T a[2], const T b[2], uint32_t first, uint32_t count, T m, bool& near) {
+  unsigned n = 0;
+  bool up = a[1] < b[1], right = a[0] < b[0];
+  const Layout& L = tb.L;
+  for (uint32_t k = 0; k < count; ++k) {
+    uint32_t inst = tb.U[L.among_u + first + k];
+    uint32_t pc = tb.U[L.inst_u + inst];
+    const T* sh = tb.F + L.inst_f + INST_F * inst;
+    const T* c = tb.F + L.tpiece_f + TPIECE_F * pc;
+    T cu[4] = {c[0] + sh[0], c[2] + sh[0], c[4] + sh[0], c[6] + sh[0]};
+    T cv[4] = {c[1] + sh[1], c[3] + sh[1], c[5] + sh[1], c[7] + sh[1]};
+    T t;
+    if (a[1] != b[1] && crossing(cu, a[0], t, m, near)) {
+      T y = beval3(cv[0], cv[1], cv[2], cv[3], t);
+      if (m > T(0) && (mabs(y - a[1]) <= m || mabs(y - b[1]) <= m)) near = true;
+      if (up ? (y > a[1] && y <= b[1]) : (y > b[1] && y <= a[1])) ++n;
+    }
+    if (a[0] != b[0] && crossing(cv, b[1], t, m, near)) {
+      T x = beval3(cu[0], cu[1], cu[2], cu[3], t);
+      if (m > T(0) && (mabs(x - a[0]) <= m || mabs(x - b[0]) <= m)) near = true;
+      bool counted;
+      if (x == b[0] || x == a[0]) {
+        T du = beval2(cu[1] - cu[0], cu[2] - cu[1], cu[3] - cu[2], t);
+        T dv = beval2(cv[1] - cv[0], cv[2] - cv[1], cv[3] - cv[2], t);
+        if (dv == T(0)) dv = cv[3] - cv[0];
+        bool slope_pos = du * dv > T(0);
+        counted = x == b[0] ? (!slope_pos) == right : slope_pos == right;
+      } else if (right) {
+        counted = x > a[0] && x < b[0];
+      } else {
+        counted = x > b[0] && x < a[0];
+      }
+      if (counted) ++n;
+    }
+  }
+  return (n & 1u) != 0;
+}
+
+// trim2.rs:1433-1458 inside: the query reduced into the domain, the cell it falls in,
+// the answer there flipped by the parity from the cell's reference.
+// dom = face record words 17..22 (lo, hi, period), root = its first cell.
+template <class T> __device__ bool trim_inside(const Tab<T>& tb, const T* dom, uint32_t root, T u, T v, T m, bool& near) {
+  const Layout& L = tb.L;
+  T q[2] = {u, v};
+  for (int k = 0; k < 2; ++k) {
+    T lo = dom[k], hi = dom[2 + k], per = dom[4 + k];
+    if (per > T(0)) {
+      T x = q[k] - lo;
+      q[k] = lo + (x - per * mfloor(x / per));
+      if (q[k] >= lo + per) q[k] = lo;  // rounding of the reduction
+    } else {
+      q[k] = mclamp(q[k], lo, hi);
+    }
+  }
+  uint32_t id = root;
+  for (int depth = 0; depth < 64; ++depth) {
+    const uint32_t* cu = tb.U + L.cell_u + CELL_U * id;
+    const T* cf = tb.F + L.cell_f + CELL_F * id;
+    uint32_t kind = cu[1];
+    if (kind == 0) return cu[0] != 0;
+    if (kind == 1) {
+      T r[2] = {cf[4], cf[5]};
+      return (cu[0] != 0) ^ parity(tb, r, q, cu[2], cu[3], m, near);
+    }
+    T mu = T(0.5) * (cf[0] + cf[2]), mv = T(0.5) * (cf[1] + cf[3]);
+    if (m > T(0) && (mabs(q[0] - mu) <= m || mabs(q[1] - mv) <= m)) {
+      // the descent itself is exact whichever child is taken (each child's reference
+      // answer is certified); nothing to flag
+    }
+    id = cu[2] + uint32_t(q[0] >= mu) + 2u * uint32_t(q[1] >= mv);
+  }
+  near = true;
+  return false;
+}
+
+// ------------------------------------------------------------------ analytic faces
+template <class T> struct Axial {
+  T theta, rho, h;
+  v3<T> er;
+};
+// surface3.rs:36-53
+template <class T> __device__ inline Axial<T> axial(const T* f, v3<T> p) {
+  v3<T> o = ld3(f), x = ld3(f + 3), y = ld3(f + 6), z = ld3(f + 9);
+  v3<T> d = p - o;
+  T lx = dot(d, x), ly = dot(d, y), lz = dot(d, z);
+  T rho = msqrt(lx * lx + ly * ly);
+  Axial<T> a;
+  a.theta = rho > T(0) ? matan2(ly, lx) : T(0);
+  a.rho = rho;
+  a.h = lz;
+  if (rho > T(0)) {
+    a.er = x * (lx / rho) + y * (ly / rho);
+  } else {
+    a.er = x;
+  }
+  return a;
+}
+
+// surface3.rs:85-128 the foot of p on an analytic surface (kind from the face's U record)
+template <class T> __device__ v3<T> foot(uint32_t kind, const T* f, v3<T> p) {
+  v3<T> o = ld3(f), z = ld3(f + 9);
+  T c0 = f[12], c1 = f[13];
+  if (kind == ICAM_PLANE) return p - z * dot(p - o, z);
+  if (kind == ICAM_CYLINDER) {
+    Axial<T> ax = axial(f, p);
+    return o + ax.er * c0 + z * ax.h;
+  }
+  if (kind == ICAM_CONE) {
+    Axial<T> ax = axial(f, p);
+    T sa = msin(c1), ca = mcos(c1);
+    T ha = -c0 / mtan(c1);
+    T s = ax.rho * sa + (ax.h - ha) * ca;
+    if (s <= T(0)) return o + z * ha;
+    return o + ax.er * (s * sa) + z * (ha + s * ca);
+  }
+  if (kind == ICAM_SPHERE) {
+    v3<T> d = p - o;
+    T l = len(d);
+    return l > T(0) ? o + d * (c0 / l) : o + ld3(f + 3) * c0;
+  }
+  // torus
+  Axial<T> ax = axial(f, p);
+  v3<T> c = o + ax.er * c0;
+  v3<T> d = p - c;
+  T l = len(d);
+  v3<T> q = l > T(0) ? c + d * (c1 / l) : c + ax.er * c1;
+  if (dot(q - o, ax.er) >= T(0) || mabs(c0) > c1) return q;
+  T h = msqrt(mmax(c1 * c1 - c0 * c0, T(0)));
+  return ax.h >= T(0) ? o + z * h : o - z * h;
+}
+
+// surface3.rs:131-149 the surface's own unit normal at a point on it
+template <class T> __device__ v3<T> snormal(uint32_t kind, const T* f, v3<T> x) {
+  v3<T> o = ld3(f), z = ld3(f + 9);
+  if (kind == ICAM_PLANE) return z;
+  if (kind == ICAM_CYLINDER) return axial(f, x).er;
+  if (kind == ICAM_CONE) {
+    Axial<T> ax = axial(f, x);
+    return ax.er * mcos(f[13]) - z * msin(f[13]);
+  }
+  if (kind == ICAM_SPHERE) {
+    v3<T> d = x - o;
+    return d * (T(1) / len(d));
+  }
+  Axial<T> ax = axial(f, x);
+  v3<T> d = x - (o + ax.er * f[12]);
+  return d * (T(1) / len(d));
+}
+
+// trim2.rs:1789-1807 arc_param_over
+template <class T> __device__ inline T arc_param_over(T a, T a0, T span, T n) {
+  const T TAU = T(6.283185307179586);
+  T q = span / n;
+  T x = a - a0;
+  if (mabs(span - TAU) < T(1e-6)) {
+    x = x - TAU * mfloor(x / TAU);
+  } else {
+    T y = x - TAU * mfloor((x - T(0.5) * span) / TAU + T(0.5));
+    x = mclamp(y, T(0), span);
+  }
+  T k = mclamp(mfloor(x / q), T(0), n - T(1));
+  T alpha = x - q * k - T(0.5) * q;
+  T t = T(0.5) * (mtan(T(0.5) * alpha) / mtan(T(0.25) * q) + T(1));
+  return k + mclamp(t, T(0), T(1));
+}
+
+// trim2.rs:1907-1934 Analytic::uv
+template <class T> __device__ void face_uv(uint32_t kind, uint32_t narc, const T* f, v3<T> q, T& u, T& v) {
+  const T HALF_PI = T(1.5707963267948966), FULL = T(6.283185307179586);
+  if (kind == ICAM_PLANE) {
+    v3<T> d = q - ld3(f);
+    u = dot(d, ld3(f + 3)), v = dot(d, ld3(f + 6));
+    return;
+  }
+  Axial<T> ax = axial(f, q);
+  u = arc_param_over(ax.theta, T(0), FULL, T(4));
+  if (kind == ICAM_CYLINDER) {
+    v = ax.h;
+  } else if (kind == ICAM_CONE) {
+    T sa = msin(f[13]), ca = mcos(f[13]);
+    T ha = -f[12] / mtan(f[13]);
+    T s = mmax(ax.rho * sa + (ax.h - ha) * ca, T(0));
+    v = ha + s * ca;
+  } else if (kind == ICAM_SPHERE) {
+    v = arc_param_over(matan2(ax.h, ax.rho), -HALF_PI, T(2) * HALF_PI, T(2));
+  } else {
+    v = arc_param_over(matan2(ax.h, ax.rho - f[12]), f[15], f[16], T(narc));
+  }
+}
+
+// ------------------------------------------------------------------ rational bicubic
+template <class T> struct SEval {
+  v3<T> p, ps, pt, pss, pst, ptt;
+};
+
+template <class T> __device__ inline void bern3(T x, T b[4], T d[4], T e[4]) {
+  T y = T(1) - x;
+  b[0] = y * y * y, b[1] = T(3) * x * y * y, b[2] = T(3) * x * x * y, b[3] = x * x * x;
+  d[0] = T(-3) * y * y, d[1] = T(3) * y * (T(1) - T(3) * x), d[2] = T(3) * x * (T(2) - T(3) * x), d[3] = T(3) * x * x;
+  e[0] = T(6) * y, e[1] = T(18) * x - T(12), e[2] = T(6) - T(18) * x, e[3] = T(6) * x;
+}
+
+// surface3.rs:322-351 a rational bicubic's point and derivatives at (s, t) from its
+// homogeneous net (16 points, i along u)
+template <class T> __device__ SEval<T> peval(const T* pw, T s, T t) {
+  T bu[4], du[4], eu[4], bv[4], dv[4], ev[4];
+  bern3(s, bu, du, eu);
+  bern3(t, bv, dv, ev);
+  T a[6][4];
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[k][c] = T(0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    // the row's sums along v first
+    T r0[4], r1[4], r2[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) r0[c] = r1[c] = r2[c] = T(0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const T* h = pw + 16 * i + 4 * j;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        r0[c] += bv[j] * h[c];
+        r1[c] += dv[j] * h[c];
+        r2[c] += ev[j] * h[c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      a[0][c] += bu[i] * r0[c];
+      a[1][c] += du[i] * r0[c];
+      a[2][c] += bu[i] * r1[c];
+      a[3][c] += eu[i] * r0[c];
+      a[4][c] += du[i] * r1[c];
+      a[5][c] += bu[i] * r2[c];
+    }
+  }
+  T iw = T(1) / a[0][3];
+  SEval<T> e;
+  e.p = mk3(a[0][0], a[0][1], a[0][2]) * iw;
+  e.ps = (mk3(a[1][0], a[1][1], a[1][2]) - e.p * a[1][3]) * iw;
+  e.pt = (mk3(a[2][0], a[2][1], a[2][2]) - e.p * a[2][3]) * iw;
+  e.pss = (mk3(a[3][0], a[3][1], a[3][2]) - e.ps * (T(2) * a[1][3]) - e.p * a[3][3]) * iw;
+  e.pst = (mk3(a[4][0], a[4][1], a[4][2]) - e.ps * a[2][3] - e.pt * a[1][3] - e.p * a[4][3]) * iw;
+  e.ptt = (mk3(a[5][0], a[5][1], a[5][2]) - e.pt * (T(2) * a[2][3]) - e.p * a[5][3]) * iw;
+  return e;
+}
+
+// surface3.rs:401-452 projected Newton on |S - p|^2 over [s0, s1] x [t0, t1], a
+// parameter on its bound with its step pointing out held there
+template <class T> __device__ void newton2(const T* pw, v3<T> p, T& s, T& t, T s0, T s1, T t0, T t1, int iters, T stop) {
+  bool hs = false, ht = false;
+  for (int it = 0; it < iters; ++it) {
+    SEval<T> e = peval(pw, s, t);
+    v3<T> r = e.p - p;
+    T g0 = dot(e.ps, r), g1 = dot(e.pt, r);
+    T jss = dot(e.ps, e.ps), jst = dot(e.ps, e.pt), jtt = dot(e.pt, e.pt);
+    T h0 = jss + dot(e.pss, r), h1 = jst + dot(e.pst, r), h2 = jtt + dot(e.ptt, r);
+    T ds, dt;
+    if (hs && ht) break;
+    if (hs) {
+      ds = 0;
+      dt = -g1 / (h2 > T(0) ? h2 : mmax(jtt, TypeEps<T>::ulp));
+    } else if (ht) {
+      ds = -g0 / (h0 > T(0) ? h0 : mmax(jss, TypeEps<T>::ulp));
+      dt = 0;
+    } else {
+      if (!(h0 > T(0) && h0 * h2 - h1 * h1 > T(0))) h0 = jss, h1 = jst, h2 = jtt;
+      T det = h0 * h2 - h1 * h1;
+      if (!(mabs(det) > T(0))) break;
+      ds = -(h2 * g0 - h1 * g1) / det;
+      dt = -(h0 * g1 - h1 * g0) / det;
+    }
+    if (!hs && ((s <= s0 && ds < T(0)) || (s >= s1 && ds > T(0)))) {
+      hs = true;
+      continue;
+    }
+    if (!ht && ((t <= t0 && dt < T(0)) || (t >= t1 && dt > T(0)))) {
+      ht = true;
+      continue;
+    }
+    T ns = mclamp(s + ds, s0, s1), nt = mclamp(t + dt, t0, t1);
+    T moved = mabs(ns - s) + mabs(nt - t);
+    s = ns, t = nt;
+    if (moved < stop) break;
+  }
+}
+
+// A cubic rational curve piece's point and derivatives (curve3.rs:129-156).
+template <class T> struct CEval {
+  v3<T> p, d1, d2;
+};
+template <class T> __device__ CEval<T> ceval(const T* pw, T s) {
+  T b[4], d[4], e[4];
+  bern3(s, b, d, e);
+  T a[3][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    a[0][c] = b[0] * pw[c] + b[1] * pw[4 + c] + b[2] * pw[8 + c] + b[3] * pw[12 + c];
+    a[1][c] = d[0] * pw[c] + d[1] * pw[4 + c] + d[2] * pw[8 + c] + d[3] * pw[12 + c];
+    a[2][c] = e[0] * pw[c] + e[1] * pw[4 + c] + e[2] * pw[8 + c] + e[3] * pw[12 + c];
+  }
+  T iw = T(1) / a[0][3];
+  CEval<T> r;
+  r.p = mk3(a[0][0], a[0][1], a[0][2]) * iw;
+  r.d1 = (mk3(a[1][0], a[1][1], a[1][2]) - r.p * a[1][3]) * iw;
+  r.d2 = (mk3(a[2][0], a[2][1], a[2][2]) - r.d1 * (T(2) * a[1][3]) - r.p * a[2][3]) * iw;
+  return r;
+}
+
+// curve3.rs:192-213 Newton on |C - p|^2 over [t0, t1]
+template <class T> __device__ T newton1(const T* pw, v3<T> p, T s, T t0, T t1, int iters, T stop) {
+  for (int it = 0; it < iters; ++it) {
+    CEval<T> e = ceval(pw, s);
+    v3<T> r = e.p - p;
+    T f = dot(e.d1, r);
+    T fp = dot(e.d2, r) + dot(e.d1, e.d1);
+    if (fp <= T(0)) fp = dot(e.d1, e.d1);
+    if (!(fp > T(0))) break;
+    T ns = mclamp(s - f / fp, t0, t1);
+    if (mabs(ns - s) < stop) {
+      s = ns;
+      break;
+    }
+    s = ns;
+  }
+  return s;
+}
+
+// ------------------------------------------------------------------ the features
+
+// What a lane's search has so far.
+template <class T> struct Best {
+  T d;            // distance of the best answer (or the bound)
+  uint32_t code;  // kind << 28 | record, or ~0 for none
+  T a, b;         // its parameters (leaf: s, t; curve leaf: t; edge: t)
+  T sgn;          // its sign when known
+  T lmin;         // least lower bound among parts not decided
+  T d2;           // the runner-up's distance and sign (near-ties)
+  T sgn2;
+  bool unsure;    // a decision was taken within rounding (mixed precision redoes it)
+};
+
+// The outward normal of face fi at a point x on (or within rounding of) it. A
+// B-spline face takes its parameters from the edge's map (uvmap: u, v at the start
+// and end of the piece) at the piece parameter tt.
+template <class T> __device__ v3<T> face_normal_at(const Tab<T>& tb, uint32_t fi, v3<T> x, const T* uvm, T tt) {
+  const Layout& L = tb.L;
+  const T* f = tb.F + L.face_f + FACE_F * fi;
+  const uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
+  uint32_t kind = fu[0];
+  if (kind != ICAM_NURBS) return snormal(kind, f, x) * f[14];
+  if (!uvm) return mk3(T(0), T(0), T(0));
+  T u = uvm[0] + tt * (uvm[2] - uvm[0]), v = uvm[1] + tt * (uvm[3] - uvm[1]);
+  uint32_t span0 = fu[3], nsu = fu[4], nsv = fu[5];
+  uint32_t a = 0, b = 0;
+  for (uint32_t k = 1; k < nsu; ++k)
+    if (tb.F[L.span_f + SPAN_F * (span0 + k * nsv) + 64] <= u) a = k;
+  for (uint32_t k = 1; k < nsv; ++k)
+    if (tb.F[L.span_f + SPAN_F * (span0 + k) + 66] <= v) b = k;
+  const T* sp = tb.F + L.span_f + SPAN_F * (span0 + a * nsv + b);
+  T s = mclamp((u - sp[64]) / (sp[65] - sp[64]), T(0), T(1)), t = mclamp((v - sp[66]) / (sp[67] - sp[66]), T(0), T(1));
+  // the point is on the surface at these parameters; two Newton steps take up rounding
+  newton2(sp, x, s, t, T(0), T(1), T(0), T(1), 2, T(0));
+  SEval<T> e = peval(sp, s, t);
+  v3<T> n = cross(e.ps, e.pt);
+  T l = len(n);
+  return l > T(0) ? n * (f[14] / l) : n;
+}
+
+// An edge's sign vector at its point q (the two faces' outward normals summed).
+template <class T> __device__ v3<T> edge_pn(const Tab<T>& tb, uint32_t e, v3<T> q, uint32_t piece, T tt) {
+  const Layout& L = tb.L;
+  const uint32_t* eu = tb.U + L.edge_u + EDGE_U * e;
+  const T* uvm = eu[3] != 0xffffffffu ? tb.F + L.uvmap_f + eu[3] + 8 * piece : nullptr;
+  v3<T> n0 = face_normal_at(tb, eu[1], q, uvm, tt);
+  v3<T> n1 = face_normal_at(tb, eu[2], q, uvm ? uvm + 4 : nullptr, tt);
+  return n0 + n1;
+}
+
+template <class T> __device__ inline T sgn_of(T x) { return x < T(0) ? T(-1) : T(1); }
+
+template <class T> __device__ inline void offer(Best<T>& b, T d, uint32_t code, T pa, T pb, T s) {
+  if (d < b.d) {
+    b.d2 = b.d, b.sgn2 = b.sgn;
+    b.d = d, b.code = code, b.a = pa, b.b = pb, b.sgn = s;
+  } else if (d < b.d2) {
+    b.d2 = d, b.sgn2 = s;
+  }
+}
+
+// solid3.rs:1911-1925 / gpu.wgsl:415-440: an analytic face's foot inside the face
+template <class T> __device__ void try_face(const Tab<T>& tb, uint32_t fi, v3<T> p, Best<T>& b, T tm) {
+  const Layout& L = tb.L;
+  const T* f = tb.F + L.face_f + FACE_F * fi;
+  const uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
+  uint32_t kind = fu[0];
+  v3<T> q = foot(kind, f, p);
+  T d = len(p - q);
+  if (!(d < b.d)) return;
+  // a cone's apex or a lemon's tip is singular: the edges and vertices answer there
+  if (kind == ICAM_CONE) {
+    T ha = -f[12] / mtan(f[13]);
+    v3<T> apex = ld3(f) + ld3(f + 9) * ha;
+    if (len(apex - q) <= T(1e-6) * (T(1) + len(apex))) return;
+  }
+  bool inside = true;
+  if (!fu[2]) {
+    T u, v;
+    face_uv(kind, fu[3], f, q, u, v);
+    bool near = false;
+    inside = trim_inside(tb, f + 17, fu[1], u, v, tm, near);
+    if (near) b.unsure = true;
+  }
+  if (!inside) return;
+  v3<T> n = snormal(kind, f, q) * f[14];
+  offer(b, d, (P_FACE << 28) | fi, T(0), T(0), sgn_of(dot(p - q, n)));
+}
+
+// solid3.rs:1775-1782, curve3.rs:525-562: a line or arc edge's interior foot
+template <class T> __device__ void try_edge(const Tab<T>& tb, uint32_t e, v3<T> p, Best<T>& b) {
+  const Layout& L = tb.L;
+  const T* r = tb.F + L.edge_f + EDGE_F * e;
+  const uint32_t* eu = tb.U + L.edge_u + EDGE_U * e;
+  v3<T> q;
+  T tt;
+  if (eu[0] == ICAM_LINE) {
+    v3<T> a = ld3(r), ab = ld3(r + 3) - a;
+    T l2 = dot(ab, ab);
+    T t = l2 > T(0) ? dot(p - a, ab) / l2 : T(0);
+    if (!(t > T(0) && t < T(1))) return;
+    q = a + ab * t;
+    tt = t;
+  } else {
+    v3<T> o = ld3(r), x = ld3(r + 3), y = ld3(r + 6);
+    v3<T> dd = p - o;
+    T lx = dot(dd, x), ly = dot(dd, y);
+    if (lx == T(0) && ly == T(0)) return;
+    T t = matan2(ly, lx);
+    T t0 = r[10], sweep = r[11];
+    T frac = T(0);
+    if (r[12] == T(0)) {
+      const T TAU = T(6.283185307179586);
+      T dlt = sweep >= T(0) ? t - t0 : t0 - t;
+      dlt = dlt - TAU * mfloor(dlt / TAU);
+      if (!(dlt <= mabs(sweep))) return;
+      frac = dlt / mabs(sweep);
+    }
+    T rho = msqrt(lx * lx + ly * ly);
+    q = o + (x * lx + y * ly) * (r[9] / rho);
+    tt = frac;
+  }
+  T d = len(p - q);
+  if (!(d < b.d)) return;
+  v3<T> pn = edge_pn(tb, e, q, 0u, tt);
+  if (len(pn) < T(1e-6)) b.unsure = true;  // a cusp: the normals cancel
+  offer(b, d, (P_EDGE << 28) | e, tt, T(0), sgn_of(dot(p - q, pn)));
+}
+
+template <class T> __device__ void try_vert(const Tab<T>& tb, uint32_t vi, v3<T> p, Best<T>& b) {
+  const T* r = tb.F + tb.L.vert_f + VERT_F * vi;
+  v3<T> x = ld3(r);
+  T d = len(p - x);
+  if (!(d < b.d)) return;
+  offer(b, d, (P_VERT << 28) | vi, T(0), T(0), sgn_of(dot(p - x, ld3(r + 3))));
+}
+
+// The critical-point test at a B-spline foot (surface3.rs:564-570): the gradient of
+// the squared distance zero to within the precision's rounding.
+template <class T> __device__ inline bool is_critical(v3<T> r, v3<T> d, T tol) {
+  T l = len(d);
+  return mabs(dot(r, d)) <= tol * l;
+}
+
+// gpu.wgsl:679-756 / solid3.rs:1904-1910: a sub-patch read by Newton when its radius
+// certifies a single minimum for p; the minimum answers when it is a critical point
+// inside the face
+template <class T> __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps) {
+  const Layout& L = tb.L;
+  const T* lf = tb.F + L.leaf_f + LEAF_F * li;
+  const uint32_t* lu = tb.U + L.leaf_u + LEAF_U * li;
+  T rho = lf[0];
+  T fd = far_dist(p, box);
+  bool certified = fd < T(0.999) * rho - eps;
+  const T* sp = tb.F + L.span_f + SPAN_F * lu[0];
+  T s0 = lf[1], s1 = lf[2], t0 = lf[3], t1 = lf[4];
+  T s = T(0.5) * (s0 + s1), t = T(0.5) * (t0 + t1);
+  const bool single = sizeof(T) == 4;
+  newton2(sp, p, s, t, s0, s1, t0, t1, single ? 12 : 32, single ? T(1e-7) : T(1e-14));
+  SEval<T> e = peval(sp, s, t);
+  v3<T> r = e.p - p;
+  T d = len(r);
+  if (!certified) b.lmin = mmin(b.lmin, lb);
+  if (!(d < b.d)) return;
+  T tol = single ? T(1e-4) * d + eps : T(1e-9) * (T(1) + d);
+  if (!(is_critical(r, e.ps, tol) && is_critical(r, e.pt, tol))) return;
+  const uint32_t fi = lu[1];
+  const T* f = tb.F + L.face_f + FACE_F * fi;
+  if (!lu[2]) {
+    const uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
+    T u = sp[64] + s * (sp[65] - sp[64]), v = sp[66] + t * (sp[67] - sp[66]);
+    bool near = false;
+    bool in = trim_inside(tb, f + 17, fu[1], u, v, tm, near);
+    if (near) b.unsure = true;
+    if (!in) return;
+  }
+  v3<T> n = cross(e.ps, e.pt);
+  T nl = len(n);
+  if (!(nl > T(0))) return;
+  offer(b, d, (P_LEAF << 28) | li, s, t, sgn_of(dot(r, n) * -f[14]));
+}
+
+// gpu.wgsl:764-815: a part of a curve piece read by Newton when certified; the minimum
+// answers when it is critical and interior to its edge
+template <class T> __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const float* box, T lb, v3<T> p, Best<T>& b, T eps) {
+  const Layout& L = tb.L;
+  const T* cf = tb.F + L.cleaf_f + CLEAF_F * ci;
+  const uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * ci;
+  T rho = cf[0];
+  bool certified = far_dist(p, box) < T(0.999) * rho - eps;
+  const uint32_t cp = cu[0];
+  const T* pw = tb.F + L.cpiece_f + CPIECE_F * cp;
+  const uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cp;
+  T t0 = cf[1], t1 = cf[2];
+  const bool single = sizeof(T) == 4;
+  T s = newton1(pw, p, T(0.5) * (t0 + t1), t0, t1, single ? 16 : 40, single ? T(1e-7) : T(1e-15));
+  CEval<T> e = ceval(pw, s);
+  v3<T> r = e.p - p;
+  T d = len(r);
+  if (!certified) b.lmin = mmin(b.lmin, lb);
+  if (!(d < b.d)) return;
+  // an end of the edge is its vertex's
+  if ((pu[1] == 0 && s <= T(0)) || (pu[1] + 1 == pu[2] && s >= T(1))) return;
+  T tol = single ? T(1e-4) * d + eps : T(1e-9) * (T(1) + d);
+  if (!is_critical(r, e.d1, tol)) return;
+  v3<T> pn = edge_pn(tb, pu[0], e.p, pu[1], s);
+  if (len(pn) < T(1e-6)) b.unsure = true;
+  offer(b, d, (P_CLEAF << 28) | ci, s, T(0), sgn_of(dot(p - e.p, pn)));
+}
+
+// One candidate primitive for one lane.
+template <class T> __device__ inline void visit(const Tab<T>& tb, uint32_t code, const float* box, v3<T> p, Best<T>& b, T tm, T eps) {
+  T lb = box_dist(p, box);
+  if (!(lb < b.d)) return;
+  uint32_t kind = code >> 28, id = code & 0x0fffffffu;
+  switch (kind) {
+    case P_FACE: try_face(tb, id, p, b, tm); break;
+    case P_EDGE: try_edge(tb, id, p, b); break;
+    case P_VERT: try_vert(tb, id, p, b); break;
+    case P_LEAF: try_leaf(tb, id, box, lb, p, b, tm, eps); break;
+    default: try_cleaf(tb, id, box, lb, p, b, eps); break;
+  }
+}
+
+// The last correction in double precision of the answer a single-precision search
+// chose: the winner's foot and sign again from the float64 tables.
+__device__ inline void refine64(const Tab<double>& tb, uint32_t code, double pa, double pb, v3<double> p, double& d, double& sg) {
+  const Layout& L = tb.L;
+  uint32_t kind = code >> 28, id = code & 0x0fffffffu;
+  if (kind == P_FACE) {
+    const double* f = tb.F + L.face_f + FACE_F * id;
+    uint32_t k = tb.U[L.face_u + FACE_U * id];
+    v3<double> q = foot(k, f, p);
+    d = len(p - q);
+    sg = sgn_of(dot(p - q, snormal(k, f, q) * f[14]));
+  } else if (kind == P_EDGE) {
+    Best<double> b;
+    b.d = TypeEps<double>::big, b.d2 = b.d, b.code = ~0u, b.lmin = b.d, b.unsure = false, b.sgn = 1, b.sgn2 = 1, b.a = b.b = 0;
+    try_edge(tb, id, p, b);
+    if (b.code != ~0u) d = b.d, sg = b.sgn;
+  } else if (kind == P_VERT) {
+    const double* r = tb.F + L.vert_f + VERT_F * id;
+    v3<double> x = ld3(r);
+    d = len(p - x);
+    sg = sgn_of(dot(p - x, ld3(r + 3)));
+  } else if (kind == P_LEAF) {
+    const double* lf = tb.F + L.leaf_f + LEAF_F * id;
+    const uint32_t* lu = tb.U + L.leaf_u + LEAF_U * id;
+    const double* sp = tb.F + L.span_f + SPAN_F * lu[0];
+    double s = pa, t = pb;
+    newton2(sp, p, s, t, lf[1], lf[2], lf[3], lf[4], 4, 1e-15);
+    SEval<double> e = peval(sp, s, t);
+    v3<double> r = e.p - p;
+    d = len(r);
+    const double* f = tb.F + L.face_f + FACE_F * lu[1];
+    sg = sgn_of(dot(r, cross(e.ps, e.pt)) * -f[14]);
+  } else {
+    const double* cf = tb.F + L.cleaf_f + CLEAF_F * id;
+    const uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * id;
+    const double* pw = tb.F + L.cpiece_f + CPIECE_F * cu[0];
+    const uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cu[0];
+    double s = newton1(pw, p, pa, cf[1], cf[2], 4, 1e-16);
+    CEval<double> e = ceval(pw, s);
+    d = len(e.p - p);
+    sg = sgn_of(dot(p - e.p, edge_pn(tb, pu[0], e.p, pu[1], s)));
+  }
+}
+
+}  // namespace icam

@@ -1,0 +1,160 @@
+"""Host mirror of the reference's GPU runner `PartGpu` (crates/icam-sim/src/partgpu.rs)
+and of the device half of its brick cache `PartCache` (crates/icam-sim/src/partcache.rs),
+over the C ABI of libicam_hip.so (include/icam_hip.h).
+
+Method names and meanings follow `PartGpu`: `points` / `points_refined` take
+(point, clamp) records and give (value, flag) pairs, `bricks` takes (first node, h)
+records and gives 729 (value, flag) pairs each; flags 0 exact, 1 a bound past the
+clamp, 2 undecided (partgpu.rs:227-264, gpu.wgsl:1913-1922). Errors raise
+`RuntimeError` with the library's message, as `PartGpu` returns Err(String).
+`query` and `bake` are the additive entry points: unbounded signed distances, and
+a whole brick level (map and atlas in the layout of crates/icam-sweep/src/cache.rs:74-101).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+FP64 = 0
+MIXED = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class BakeStats(ctypes.Structure):
+    _fields_ = [("bricks", ctypes.c_int64), ("band", ctypes.c_int64), ("slots", ctypes.c_int64), ("undecided", ctypes.c_int64),
+                ("refined", ctypes.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = _native._load("libicam_hip.so")
+        L.icam_hip_open.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]
+        L.icam_hip_close.argtypes = [ctypes.c_void_p]
+        L.icam_hip_last_error.restype = ctypes.c_char_p
+        L.icam_hip_device_name.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+        L.icam_hip_warm.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.icam_hip_nodes.restype = ctypes.c_size_t
+        L.icam_hip_nodes.argtypes = [ctypes.c_void_p]
+        L.icam_hip_points.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, _dp]
+        L.icam_hip_bricks.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, _dp]
+        L.icam_hip_query.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_size_t, _dp, _u32p, _u32p]
+        L.icam_hip_query_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p]
+        L.icam_hip_bake.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, _u32p, _u32p, ctypes.c_size_t, ctypes.POINTER(BakeStats)]
+        L.icam_hip_bake_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                           ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(BakeStats), ctypes.c_void_p]
+        L.icam_hip_stage_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), _dp, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().icam_hip_last_error().decode())
+
+
+class PartGpu:
+    """The part's exact signed distance on the GPU (PartGpu::new, partgpu.rs:132-186).
+
+    `need`: the distance (mm) within which the B-spline sub-patch and curve-piece
+    trees certify a single minimum (the reference packs for 4 finest cells times
+    sqrt 3, crates/icam-sim/src/main.rs:644)."""
+
+    def __init__(self, part, need):
+        self.part = part
+        self.need = need
+        h = ctypes.c_void_p()
+        _check(lib().icam_hip_open(ctypes.cast(part.desc, ctypes.c_void_p), need, ctypes.byref(h)))
+        self.h = h
+        buf = ctypes.create_string_buffer(256)
+        _check(lib().icam_hip_device_name(self.h, buf, 256))
+        self.adapter = buf.value.decode()
+        self.nodes = lib().icam_hip_nodes(self.h)
+
+    def warm(self, secs):
+        _check(lib().icam_hip_warm(self.h, secs))
+
+    def points(self, pts, clamp, precision=FP64):
+        """(value, flag) per (point, clamp) record."""
+        pts = np.asarray(pts, dtype=np.float64).reshape(-1, 3)
+        rec = np.empty((len(pts), 4))
+        rec[:, :3] = pts
+        rec[:, 3] = np.broadcast_to(np.asarray(clamp, dtype=np.float64), (len(pts),))
+        rec = np.ascontiguousarray(rec)
+        out = np.empty((len(pts), 2))
+        _check(lib().icam_hip_points(self.h, precision, rec.ctypes.data, len(pts), out.ctypes.data_as(_dp)))
+        return out
+
+    def points_refined(self, pts, clamp):
+        return self.points(pts, clamp, MIXED)
+
+    def bricks(self, firsts, h, precision=FP64):
+        """729 (value, flag) pairs per brick whose first node is firsts[i], cell side h."""
+        firsts = np.asarray(firsts, dtype=np.float64).reshape(-1, 3)
+        rec = np.empty((len(firsts), 4))
+        rec[:, :3] = firsts
+        rec[:, 3] = np.broadcast_to(np.asarray(h, dtype=np.float64), (len(firsts),))
+        rec = np.ascontiguousarray(rec)
+        out = np.empty((len(firsts), 729, 2))
+        _check(lib().icam_hip_bricks(self.h, precision, rec.ctypes.data, len(firsts), out.ctypes.data_as(_dp)))
+        return out
+
+    def query(self, xyz, precision=FP64):
+        """Signed distances, flags and nearest features at world points (unbounded)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        n = len(xyz)
+        d = np.empty(n)
+        fl = np.empty(n, dtype=np.uint32)
+        ft = np.empty(n, dtype=np.uint32)
+        _check(lib().icam_hip_query(self.h, precision, xyz.ctypes.data_as(_dp), n, d.ctypes.data_as(_dp), fl.ctypes.data_as(_u32p),
+                                    ft.ctypes.data_as(_u32p)))
+        return d, fl, ft
+
+    def query_device(self, d_xyz, n, d_d, d_flags, d_feat=None, precision=FP64, stream=None):
+        """The same on device pointers (ints), enqueued on `stream` (a hipStream_t as int)."""
+        _check(lib().icam_hip_query_device(self.h, precision, d_xyz, n, d_d, d_flags, d_feat, stream))
+
+    def bake(self, level, precision=FP64):
+        """The level's brick map and atlas (PartCache::build on the device), and statistics."""
+        nb = level.nb
+        n = nb[0] * nb[1] * nb[2]
+        m = np.empty(n, dtype=np.uint32)
+        st = BakeStats()
+        cap = n * 365
+        atlas = np.empty(cap, dtype=np.uint32)
+        _check(lib().icam_hip_bake(self.h, precision, ctypes.byref(level), m.ctypes.data_as(_u32p), atlas.ctypes.data_as(_u32p), cap,
+                                   ctypes.byref(st)))
+        return m, atlas[: st.slots * 365].copy(), st
+
+    def bake_device(self, level, precision=FP64, stream=None, stats=False):
+        """The level baked into device memory; returns (map pointer, atlas pointer, stats or None)."""
+        dm, da = ctypes.c_void_p(), ctypes.c_void_p()
+        st = BakeStats() if stats else None
+        _check(lib().icam_hip_bake_device(self.h, precision, ctypes.byref(level), ctypes.byref(dm), ctypes.byref(da),
+                                          ctypes.byref(st) if st is not None else None, stream))
+        return dm.value, da.value, st
+
+    def stage_times(self):
+        """Device time (ms) of each kernel stage of the last bake or query."""
+        names = (ctypes.c_char_p * 32)()
+        ms = (ctypes.c_double * 32)()
+        k = lib().icam_hip_stage_times(self.h, names, ms, 32)
+        return {names[i].decode(): ms[i] for i in range(k)}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().icam_hip_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

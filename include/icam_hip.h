@@ -1,0 +1,132 @@
+/*
+ * icam_hip.h -- the C ABI of libicam_hip.so: the exact signed distance of a
+ * B-rep part and its sparse narrow-band brick baker on an MI355X (gfx950).
+ *
+ * It replaces the reference's GPU runner `PartGpu` (crates/icam-sim/src/partgpu.rs,
+ * binary-private) and the device half of its brick cache `PartCache`
+ * (crates/icam-sim/src/partcache.rs). A Rust host binds it through a thin
+ * `extern "C"` crate (see the binding in the repository's integration notes).
+ *
+ * Conventions
+ *  - Return 0 on success; otherwise a nonzero code, and icam_hip_last_error()
+ *    (thread-local) says what failed, as `PartGpu` returns Err(String).
+ *  - The caller owns every host array for the duration of the call; the
+ *    library owns device memory from icam_hip_open until icam_hip_close.
+ *  - A handle may be used from several threads: calls on one handle are
+ *    serialized by its own lock (PartCache builds two levels at once through
+ *    one runner, crates/icam-sim/src/play.rs:979-997).
+ *  - World coordinates in float64; the tables are packed relative to the part's
+ *    centre, in float32 and float64 copies.
+ *  - precision: ICAM_FP64 runs every step in double precision; ICAM_MIXED runs
+ *    the search in single precision and the last correction of the nearest
+ *    feature, the trim test near a trim curve and near-ties in double.
+ *
+ * Output flags per node or point (the reference kernel's, gpu.wgsl:1913-1922):
+ *   0  the value is the exact signed distance;
+ *   1  only a lower bound past the record's bound is known (the value is that
+ *      bound); the sign is not known;
+ *   2  undecided (a convexity certificate was missing within the bound).
+ */
+#ifndef ICAM_HIP_H
+#define ICAM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "icam_part.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct icam_hip icam_hip;
+
+enum icam_precision { ICAM_FP64 = 0, ICAM_MIXED = 1 };
+
+/* PartGpu::new(&Exact, need) (partgpu.rs:132-186) with Exact::gpu_pack(origin,
+ * need) (crates/icam-core/src/solid3/gpu.rs:259-701): packs the part relative to
+ * its centre, builds the trim cell trees, the B-spline sub-patch trees and the
+ * curve-piece trees, certified for points within `need` of each part (mm), and
+ * uploads them. Fails when no device is present or a surface is past degree 3. */
+int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out);
+
+/* Drop(PartGpu): frees the device tables. */
+void icam_hip_close(icam_hip* h);
+
+/* The last error on this thread (empty when none). */
+const char* icam_hip_last_error(void);
+
+/* PartGpu.adapter (partgpu.rs:84): the device's name. */
+int icam_hip_device_name(icam_hip* h, char* buf, size_t cap);
+
+/* PartGpu::warm(secs) (partgpu.rs:188-225): keeps the device busy for about
+ * `secs` seconds so its clocks are up before timing. */
+int icam_hip_warm(icam_hip* h, double secs);
+
+/* PartGpu.nodes (partgpu.rs:86): sub-patches and curve pieces packed. */
+size_t icam_hip_nodes(icam_hip* h);
+
+/* A point record (PartGpu::points, partgpu.rs:424-434; the reference's Brick in
+ * point mode, gpu.wgsl:11-18): the point, and the distance past which a lower
+ * bound will do (infinity: always exact). */
+typedef struct icam_point_rec {
+  double p[3];
+  double clamp;
+} icam_point_rec;
+
+/* PartGpu::points / points_refined (partgpu.rs:424-434): out[2 i] the signed
+ * distance (or bound), out[2 i + 1] the flag. */
+int icam_hip_points(icam_hip* h, int precision, const icam_point_rec* recs, size_t n, double* out);
+
+/* A brick record (PartGpu::bricks, partgpu.rs:227-233): the first node and the
+ * cell side; its 729 nodes are first + h (n % 9, n / 9 % 9, n / 81). */
+typedef struct icam_brick_rec {
+  double first[3];
+  double h;
+} icam_brick_rec;
+
+/* PartGpu::bricks (partgpu.rs:227-233): the 729 nodes of each brick, exact within
+ * the clamp h sqrt 3 (flag 0), else flagged 1 with the clamp as value. out holds
+ * 729 (value, flag) pairs per brick. */
+int icam_hip_bricks(icam_hip* h, int precision, const icam_brick_rec* recs, size_t n, double* out);
+
+/* Signed distances at n world points (unbounded), flags as above, and the nearest
+ * feature as kind << 28 | index (0 face, 1 edge, 2 vertex) when feat is not NULL.
+ * Host arrays. (Exact::sdf, solid3.rs:1750.) */
+int icam_hip_query(icam_hip* h, int precision, const double* xyz, size_t n, double* d, uint32_t* flags, uint32_t* feat);
+
+/* The same on device arrays already resident in HBM, enqueued on `stream`
+ * (a hipStream_t; NULL: the default stream); returns without waiting. */
+int icam_hip_query_device(icam_hip* h, int precision, const double* d_xyz, size_t n, double* d_d, uint32_t* d_flags, uint32_t* d_feat, void* stream);
+
+/* Statistics of a bake. */
+typedef struct icam_bake_stats {
+  int64_t bricks;      /* bricks in the level */
+  int64_t band;        /* bricks whose nodes were read (the decision's) */
+  int64_t slots;       /* bricks stored with words */
+  int64_t undecided;   /* nodes left undecided (bridged) */
+  int64_t refined;     /* nodes whose value took a double-precision step */
+} icam_bake_stats;
+
+/* PartCache::build (partcache.rs:458-486) on the device: the level's brick map
+ * (nb[0] nb[1] nb[2] entries, x fastest: ICAM_AIR, ICAM_SOLID or a slot) and its
+ * atlas (ICAM_WORDS words a slot, slots in map order), bit-compatible with
+ * crates/icam-sweep/src/cache.rs:74-101. Host arrays; atlas_cap in words.
+ * Returns nonzero when the atlas does not fit (stats->slots says how many). */
+int icam_hip_bake(icam_hip* h, int precision, const icam_level* lv, uint32_t* map, uint32_t* atlas, size_t atlas_cap, icam_bake_stats* stats);
+
+/* The same with the map and atlas left in device memory the library owns until
+ * the next bake or close; *d_map and *d_atlas receive their device addresses.
+ * Enqueued on `stream`; the statistics need the stream synchronized first, so
+ * they are written when stats is not NULL (which waits). */
+int icam_hip_bake_device(icam_hip* h, int precision, const icam_level* lv, uint32_t** d_map, uint32_t** d_atlas, icam_bake_stats* stats, void* stream);
+
+/* Average device time (ms) of each kernel stage of the last bake or query on
+ * this handle, measured with HIP events on the stream the kernels ran on:
+ * names[i] (static strings) and ms[i] for i < returned count (at most cap). */
+int icam_hip_stage_times(icam_hip* h, const char** names, double* ms, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,282 @@
+"""The FP64 oracle pinned against the reference's own tests and closed forms.
+
+Each check is independent of the oracle's code path: the reference's closed-form
+capped cylinder and its dense-sampling B-spline box tests (crates/icam-core/src/
+solid3.rs:2044-2210), per-face closed forms with a CSG inside test, a planar NURBS
+reduced to a box, brute-force sampling of curves and surfaces, and restated brick
+rules (crates/icam-sim/src/partcache.rs:911-915, 962-970).
+"""
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+from closed_forms import bump_samples, bump_top, c1_sdf, can_sdf, reference_points
+from implicitcam_amd.part import Part, level_for, uniform_points
+from oracle import Oracle
+
+SEED = 0x2545F4914F6CDD1D  # crates/icam-check/src/exact_check.rs:255
+
+
+@pytest.fixture(scope="module")
+def can(built):
+    p = Part("can")
+    return p, Oracle(p)
+
+
+@pytest.fixture(scope="module")
+def c1(built):
+    p = Part("c1")
+    return p, Oracle(p)
+
+
+@pytest.fixture(scope="module")
+def bump(built):
+    p = Part("bump")
+    return p, Oracle(p)
+
+
+@pytest.fixture(scope="module")
+def c2(built):
+    p = Part("c2")
+    return p, Oracle(p)
+
+
+def test_can_closed_form(can):
+    """solid3.rs:2044-2070: 2,000 points and 40 axis points to 1e-9."""
+    _, o = can
+    axis = np.stack([np.zeros(40), np.zeros(40), -2.0 + 9.0 * np.arange(40) / 39.0], 1)
+    pts = np.concatenate([reference_points(2000), axis])
+    d, _, flags = o.sdf(pts)
+    assert not (flags & 1).any()
+    assert np.abs(d - can_sdf(pts)).max() < 1e-9
+
+
+def test_c1_closed_form_uniform(c1):
+    part, o = c1
+    pts = uniform_points(SEED, 100_000, part.lo - 5, part.hi + 5)
+    d, _, flags = o.sdf(pts)
+    ref = c1_sdf(pts)
+    assert not (flags & 1).any()
+    assert np.abs(d - ref).max() < 1e-9
+    assert ((d < 0) == (ref < 0)).all()
+
+
+def test_c1_closed_form_near_fillet_and_corners(c1):
+    """Points crowded on the torus fillet, the hole, its edges and the box's corners."""
+    _, o = c1
+    rng = np.random.default_rng(3)
+    th = rng.uniform(0, 2 * np.pi, 40_000)
+    r = rng.uniform(8, 16, 40_000)
+    z = rng.uniform(14, 23, 40_000)
+    ring = np.stack([r * np.cos(th), r * np.sin(th), z], 1)
+    corner = np.array([50.0, 30.0, 20.0]) * rng.choice([-1, 1], size=(20_000, 3)) + rng.normal(0, 1.0, (20_000, 3))
+    pts = np.concatenate([ring, corner])
+    d, _, _ = o.sdf(pts)
+    ref = c1_sdf(pts)
+    # the planar loops approximate circles within 1e-10 of the radius
+    assert np.abs(d - ref).max() < 1e-8
+    clear = np.abs(ref) > 1e-8
+    assert ((d < 0) == (ref < 0))[clear].all()
+
+
+def test_c1_nearest_features(c1):
+    """A point over a face's interior is nearest its face; beside an edge, the edge; past
+    a corner, the vertex; on the hole's axis, the fillet (feature codes kind << 28 |
+    index)."""
+    _, o = c1
+    pts = np.array([[0.0, 20.0, 25.0], [50.5, 30.5, 0.0], [51.0, 31.0, 21.0], [0.0, 0.0, 25.0]])
+    _, feat, _ = o.sdf(pts)
+    assert feat[0] >> 28 == 0 and (feat[0] & 0xFFFFFFF) == 0  # the top face
+    assert feat[1] >> 28 == 1  # a vertical edge
+    assert feat[2] >> 28 == 2  # a corner
+    assert feat[3] == 7  # on the hole's axis: the fillet (every angle's foot alike)
+
+
+def test_bump_box_dense(bump):
+    """solid3.rs:2145-2210: the sign exact, |d| within [dense - h, dense + 1e-9]."""
+    _, o = bump
+    samples, h = bump_samples(300)
+    pts = reference_points(200, cx=5, cy=5, cz=4, r=8, rz=6)
+    d, _, flags = o.sdf(pts)
+    inside = (pts[:, 0] > 0) & (pts[:, 0] < 10) & (pts[:, 1] > 0) & (pts[:, 1] < 10) & (pts[:, 2] > 0) & (pts[:, 2] < bump_top(pts[:, 0], pts[:, 1]))
+    dense, _ = cKDTree(samples).query(pts)
+    assert not (flags & 1).any()
+    assert ((d < 0) == inside).all()
+    assert (np.abs(d) <= dense + 1e-9).all()
+    assert (np.abs(d) >= dense - h).all()
+
+
+def test_planar_nurbs_is_a_box(built):
+    """A rational NURBS top whose control points all lie at z = 20 is the plane z = 20:
+    the part is the box [-60, 60]^2 x [-20, 20] (textbook box distance)."""
+    part = Part("c2flat")
+    o = Oracle(part)
+    pts = uniform_points(SEED, 4000, part.lo - 5, part.hi + 5)
+    d, _, _ = o.sdf(pts)
+    half = np.array([60.0, 60.0, 20.0])
+    q = np.abs(pts) - half
+    box = np.linalg.norm(np.maximum(q, 0), axis=1) + np.minimum(np.max(q, axis=1), 0)
+    assert np.abs(d - box).max() < 1e-9
+
+
+def _c2_surface_samples(o, n=300):
+    us = np.arange(n + 1) / n
+    return np.array([o.nurbs_eval(0, u, v) for u in us for v in us])
+
+
+def test_c2_brute_force(c2):
+    """Upper bound: no sampled boundary point nearer than |d|; lower bound: |d| within the
+    sampling's reach of the samples; sign against the height field."""
+    part, o = c2
+    top = _c2_surface_samples(o, 240)
+    side = []
+    for t in np.linspace(0, 1, 241):
+        for q in (o.nurbs_eval(0, t, 0), o.nurbs_eval(0, t, 1), o.nurbs_eval(0, 0, t), o.nurbs_eval(0, 1, t)):
+            for f in np.linspace(0, 1, 61):
+                side.append([q[0], q[1], -20 + (q[2] + 20) * f])
+    g = np.linspace(-60, 60, 241)
+    gx, gy = np.meshgrid(g, g)
+    bottom = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, -20.0)], 1)
+    samples = np.concatenate([top, np.array(side), bottom])
+    spacing = 1.0  # at most this between neighbouring samples (mm)
+    pts = uniform_points(7, 2000, part.lo - 5, part.hi + 5)
+    d, _, flags = o.sdf(pts)
+    dense, _ = cKDTree(samples).query(pts)
+    assert not (flags & 1).any()
+    assert (np.abs(d) <= dense + 1e-9).all()
+    assert (np.abs(d) >= dense - spacing).all()
+    from scipy.interpolate import LinearNDInterpolator
+
+    height = LinearNDInterpolator(top[:, :2], top[:, 2])
+    inner = (np.abs(pts[:, 0]) < 59) & (np.abs(pts[:, 1]) < 59)
+    zt = height(pts[inner, 0], pts[inner, 1])
+    inside = (pts[inner, 2] > -20) & (pts[inner, 2] < zt)
+    clear = np.abs(pts[inner, 2] - zt) > 0.05
+    assert ((d[inner] < 0) == inside)[clear].all()
+
+
+def test_edge_closest_against_samples(c1, c2):
+    """curve3.rs:935-968: an edge's closest point is no farther than 20,001 samples'."""
+    _, o1 = c1
+    _, o2 = c2
+    rng = np.random.default_rng(5)
+    # C1's circles: edges 12..14; sample them directly
+    for e, (z, r) in zip(range(12, 15), [(20.0, 13.0), (17.0, 10.0), (-20.0, 10.0)]):
+        t = np.linspace(0, 2 * np.pi, 20001)
+        s = np.stack([r * np.cos(t), r * np.sin(t), np.full_like(t, z)], 1)
+        for p in rng.uniform(-20, 20, (20, 3)):
+            d, _, _ = o1.edge_closest(e, p)
+            assert d <= np.linalg.norm(s - p, axis=1).min() + 1e-12
+    # C2's spline edges: the NURBS boundary at v = 0 (edge 0)
+    s = np.array([o2.nurbs_eval(0, t, 0.0) for t in np.linspace(0, 1, 20001)])
+    for p in rng.uniform([-70, -70, 0], [70, -50, 30], (20, 3)):
+        d, q, _ = o2.edge_closest(0, p)
+        dense = np.linalg.norm(s - p, axis=1).min()
+        assert d <= dense + 1e-9
+        assert d >= dense - 0.02
+
+
+def test_trim_inside_closed_forms(c1, c2):
+    """Loops against closed forms: the top face is its rectangle less the fillet's disc;
+    the hole's band and the fillet's quarter tube cover every angle; a side of C2 lies
+    below the NURBS boundary curve."""
+    _, o1 = c1
+    rng = np.random.default_rng(9)
+    uv = rng.uniform([-52, -32], [52, 32], (20000, 2))
+    got = np.array([o1.face_inside(0, u, v) for u, v in uv])
+    r = np.hypot(uv[:, 0], uv[:, 1])
+    want = (np.abs(uv[:, 0]) < 50) & (np.abs(uv[:, 1]) < 30) & (r > 13)
+    clear = (np.abs(np.abs(uv[:, 0]) - 50) > 1e-6) & (np.abs(np.abs(uv[:, 1]) - 30) > 1e-6) & (np.abs(r - 13) > 1e-6)
+    assert (got == want)[clear].all()
+    uv = rng.uniform([-10, -21], [10, 18], (4000, 2))
+    got = np.array([o1.face_inside(6, u, v) for u, v in uv])
+    assert (got == ((uv[:, 1] > -20) & (uv[:, 1] < 17))).all()
+    uv = rng.uniform([-10, -10], [10, 10], (4000, 2))
+    got = np.array([o1.face_inside(7, u, v) for u, v in uv])
+    vr = np.mod(uv[:, 1], 4.0)
+    assert (got == ((vr > 1) & (vr < 2))).all()
+    _, o2 = c2
+    xs = rng.uniform(-61, 61, 3000)
+    zs = rng.uniform(-21, 26, 3000)
+    got = np.array([o2.face_inside(1, x, z) for x, z in zip(xs, zs)])
+    # face 1 is the y = -60 side: params (x, z); its top is the surface's v = 0 boundary
+    ztop = np.array([o2.nurbs_eval(0, np.clip((x + 60) / 120, 0, 1), 0.0)[2] for x in xs])
+    want = (np.abs(xs) < 60) & (zs > -20) & (zs < ztop)
+    clear = (np.abs(np.abs(xs) - 60) > 1e-6) & (np.abs(zs + 20) > 1e-6) & (np.abs(zs - ztop) > 1e-6)
+    assert (got == want)[clear].all()
+
+
+def test_arc_parameters(c1):
+    """trim2.rs:1770-1807: on a rational quadratic quarter arc (weights 1, cos 45, 1) the
+    arc parameter of its point at t is k + t; checked on the hole's cylinder."""
+    _, o = c1
+    w = np.cos(np.pi / 4)
+    for k in range(4):
+        a0 = k * np.pi / 2
+        p0 = np.array([np.cos(a0), np.sin(a0)])
+        p2 = np.array([np.cos(a0 + np.pi / 2), np.sin(a0 + np.pi / 2)])
+        p1 = (p0 + p2) / (2 * w * w) * 1.0
+        p1 = np.array([np.cos(a0 + np.pi / 4), np.sin(a0 + np.pi / 4)]) / w
+        for t in np.linspace(0.01, 0.99, 25):
+            num = (1 - t) ** 2 * p0 + 2 * t * (1 - t) * w * p1 + t * t * p2
+            den = (1 - t) ** 2 + 2 * t * (1 - t) * w + t * t
+            q = num / den * 10.0
+            uv = o.face_uv(6, np.array([q[0], q[1], 3.0]))
+            assert abs(uv[0] - (k + t)) < 1e-12
+            assert abs(uv[1] - 3.0) < 1e-12
+
+
+def test_brick_values_and_packing(c1):
+    """exact_brick's coarse-to-fine reading with Lipschitz skipping gives what reading
+    every node gives: q = round(clamp(d / clamp, -1, 1) 32767), two to a word, low half
+    first, x fastest; AIR when every q > 0, SOLID when every q <= 0."""
+    part, o = c1
+    h = 1.6
+    clamp = h * np.sqrt(3)
+    rng = np.random.default_rng(2)
+    firsts = [np.array([-6.0, -6.0, 14.0]), np.array([45.0, 25.0, 15.0]), np.array([0.0, 0.0, 0.0]) - 4 * h + rng.uniform(-1, 1, 3)]
+    for f in firsts:
+        cls, q = o.brick(f, h)
+        n = np.arange(729)
+        nodes = f + h * np.stack([n % 9, (n // 9) % 9, n // 81], 1)
+        d, _, _ = o.sdf(nodes)
+        want = np.round(np.clip(d / clamp, -1, 1) * 32767).astype(np.int16)
+        # f64::round rounds half away from zero, as numpy's rint does not: no half ties here
+        assert (q == want).all()
+        assert cls == (1 if (q > 0).all() else 2 if (q <= 0).all() else 0)
+
+
+def test_decide_against_brute_force(c1):
+    """partcache.rs:226-331: the coarse-to-fine decision keeps exactly the bricks whose
+    centre distance lies within the brick's reach (half the grown diagonal + clamp)."""
+    part, o = c1
+    lv = level_for(part, 1.6, 8)
+    kept = set(o.decide(lv).tolist())
+    h = lv.h
+    clamp = h * np.sqrt(3)
+    reach = 0.5 * np.sqrt(3) * (8 * h + 2 * clamp) + clamp
+    nb = list(lv.nb)
+    ids = np.arange(nb[0] * nb[1] * nb[2])
+    idx = np.stack([ids % nb[0], (ids // nb[0]) % nb[1], ids // (nb[0] * nb[1])], 1)
+    centres = np.array(lv.lo[:]) + 8 * h * idx + 4 * h
+    d, _, _ = o.sdf(centres)
+    want = set(ids[np.abs(d) <= reach].tolist())
+    assert kept == want
+
+
+def test_bake_map_and_atlas(c1):
+    """The level's map: AIR and SOLID bricks lie wholly outside / inside (centre sign),
+    slots number the stored bricks in map order, the atlas holds their words."""
+    part, o = c1
+    lv = level_for(part, 1.6, 8)
+    m, atlas, nread = o.bake(lv)
+    slots = m[(m != 0xFFFFFFFF) & (m != 0xFFFFFFFE)]
+    assert (slots == np.arange(len(slots))).all()
+    assert len(atlas) == 365 * len(slots)
+    nb = list(lv.nb)
+    ids = np.arange(len(m))
+    idx = np.stack([ids % nb[0], (ids // nb[0]) % nb[1], ids // (nb[0] * nb[1])], 1)
+    centres = np.array(lv.lo[:]) + 8 * lv.h * idx + 4 * lv.h
+    d, _, _ = o.sdf(centres)
+    assert (d[m == 0xFFFFFFFF] > 0).all()
+    assert (d[m == 0xFFFFFFFE] < 0).all()
