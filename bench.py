@@ -1,0 +1,182 @@
+"""Benchmark: the exact signed distance's narrow-band brick bake on one MI355X.
+
+Workload (a "step"): the freeform single solid -- a 120 x 120 mm block whose top is
+a rational bicubic NURBS with a 16 x 16 control net (seeded value-noise heights,
+5 mm, weights in [0.8, 1.2]) -- baked on a 256^3 node grid at 0.5 mm into 8^3-cell
+bricks (32^3 bricks) within the narrow band: the decision of every brick, the signs
+of the bricks kept whole, the 729 nodes of every band brick, flood fill, i16
+quantization and packing into the 365-word atlas, all on the device, in the
+FP32 + FP64-refine form. The plain FP64 form is timed beside it, and so are 2^20
+scattered SDF queries on the analytic box-with-hole part.
+
+Prints one JSON line (rank 0, N = 1). `value` is band bricks baked per second.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32 = 157.3e12  # MI355X dense fp32, FLOP/s
+PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12  # B/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the parity check against the oracle")
+    ap.add_argument("--profile", action="store_true", help="a short run for a profiler: bake steps only")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only: the path shards by brick ranges across GPUs (see the design notes); "
+                                   "run with --gpus 1"}))
+        sys.exit(2)
+
+    import torch
+
+    from implicitcam_amd import FP64, MIXED, Part, PartGpu, level_for, uniform_points
+
+    if not torch.cuda.is_available():
+        raise SystemExit("no GPU: the benchmark runs the HIP kernels on an MI355X")
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+
+    h = 0.5
+    part = Part("c2")
+    lv = level_for(part, h, 32)
+    lv.nb[:] = [32, 32, 32]  # the 256^3 node grid
+    reach_need = 12 * h
+    t0 = time.time()
+    gpu = PartGpu(part, reach_need)
+    log(f"packed c2: {gpu.nodes} sub-patches and curve parts in {time.time() - t0:.2f}s on {gpu.adapter}")
+
+    def bake_step(prec):
+        gpu.bake_device(lv, prec)
+
+    res = {}
+    for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+        for _ in range(args.warmup):
+            bake_step(prec)
+        torch.cuda.synchronize()
+        _, _, st = gpu.bake_device(lv, prec, stats=True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        stage_sum = {}
+        for _ in range(args.steps):
+            bake_step(prec)
+            for k, v in gpu.stage_times().items():
+                stage_sum[k] = stage_sum.get(k, 0.0) + v
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        res[name] = dict(ms=dt * 1e3, band=st.band, slots=st.slots, bricks=st.bricks, undecided=st.undecided, refined=st.refined,
+                         stages={k: v / args.steps for k, v in stage_sum.items()})
+        log(f"bake {name}: {dt * 1e3:.3f} ms/step, band {st.band}, slots {st.slots}, undecided {st.undecided}, refined nodes {st.refined}, "
+            f"stages {res[name]['stages']}")
+    if args.profile:
+        return
+
+    # 2^20 scattered queries on the analytic part (C1), inputs resident in HBM
+    c1 = Part("c1")
+    g1 = PartGpu(c1, 12 * 1.6)
+    n = 1 << 20
+    pts = uniform_points(0x2545F4914F6CDD1D, n, c1.lo - 5, c1.hi + 5)
+    d_pts = torch.from_numpy(pts).to(dev)
+    d_out = torch.empty(n, dtype=torch.float64, device=dev)
+    d_flag = torch.empty(n, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    q = {}
+    for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+        for _ in range(args.warmup):
+            g1.query_device(d_pts.data_ptr(), n, d_out.data_ptr(), d_flag.data_ptr(), None, prec, stream)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            g1.query_device(d_pts.data_ptr(), n, d_out.data_ptr(), d_flag.data_ptr(), None, prec, stream)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        q[name] = n / dt
+        log(f"c1 queries {name}: {n / dt / 1e6:.1f} M/s")
+
+    check = {}
+    cpu = None
+    if not args.no_check or not args.no_cpu:
+        from oracle import Oracle
+
+        o = Oracle(part)
+        threads = len(os.sched_getaffinity(0))
+        t = time.perf_counter()
+        m_ref, a_ref, nread = o.bake(lv, threads=threads)
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": nread / cpu_s, "unit": "bricks/s", "cores": threads, "kind": "port",
+               "sample": f"the same c2 level (all {nread} band bricks of the 32^3), FP64 oracle on {threads} host threads, {cpu_s:.1f} s"}
+        log(f"cpu oracle bake: {cpu_s:.1f}s, {nread / cpu_s:.1f} bricks/s on {threads} threads")
+        if not args.no_check:
+            for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+                m, a, st = gpu.bake(lv, prec)
+                qa = a.view(np.int16).astype(np.int64)
+                qr = a_ref.view(np.int16).astype(np.int64)
+                same_len = len(qa) == len(qr)
+                check[name] = {"map_equal": bool((m == m_ref).all()), "atlas_equal": bool(same_len and (qa == qr).all()),
+                               "max_dq": int(np.abs(qa - qr).max()) if same_len and len(qa) else None,
+                               "sign_mismatch": int(((qa > 0) != (qr > 0)).sum()) if same_len else None}
+            # query error against the oracle on a sample of the 2^20 points
+            o1 = Oracle(c1)
+            sample = pts[:: 64]
+            dref, _, _ = o1.sdf(sample, threads=threads)
+            for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+                dq, fl, _ = g1.query(sample, prec)
+                err = np.abs(dq - dref) / c1.diagonal
+                check["c1_" + name] = {"max_rel_err": float(err.max()), "p99_rel_err": float(np.percentile(err, 99)),
+                                       "sign_mismatch": int(((dq < 0) != (dref < 0)).sum()), "n": int(len(sample))}
+            log(f"check: {check}")
+
+    mixed, fp64 = res["mixed"], res["fp64"]
+    bricks_ms = mixed["stages"].get("bricks", float("nan"))
+    # HBM traffic of the dominant kernel: its algorithmic bytes are the staged words it
+    # writes (1460 B a band brick) and the tables it reads once (small); per launch
+    algo_bytes = mixed["band"] * 1460.0
+    out = {
+        "metric": "SDF queries/s + bricks/s, FP32+FP64-refine vs FP64 at <=1e-6 rel err, 1 GPU",
+        "value": mixed["band"] / (mixed["ms"] / 1e3),
+        "unit": "bricks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": "c2 freeform NURBS-top block, 256^3 nodes at 0.5 mm, 8^3-cell bricks, narrow band, FP32+FP64-refine",
+                   "bricks": mixed["bricks"], "band_bricks": mixed["band"], "stored_bricks": mixed["slots"]},
+        "fp64_bricks_per_s": fp64["band"] / (fp64["ms"] / 1e3),
+        "mixed_over_fp64": fp64["ms"] / mixed["ms"],
+        "c1_queries_per_s": {"mixed": q["mixed"], "fp64": q["fp64"]},
+        "stages_ms": {"mixed": mixed["stages"], "fp64": fp64["stages"]},
+        "undecided_nodes": mixed["undecided"],
+        "refined_nodes": mixed["refined"],
+        "roofline": {"bound": "hbm", "achieved": algo_bytes / (bricks_ms / 1e3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": algo_bytes / (bricks_ms / 1e3) / PEAK_HBM, "traffic": None},
+        "cpu_baseline": cpu,
+        "check": check,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

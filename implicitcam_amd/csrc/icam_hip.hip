@@ -74,10 +74,11 @@ __device__ inline float box_gap(const float* a, const float* b) {
 
 // The primitives whose box comes within R of the block's box gb (lo, hi), in list
 // order, into cand; returns how many (more than MAXC: the list did not fit).
+template <int NT = WG>
 __device__ int cull(const Dev& dv, const float* gb, float R, uint32_t* cand, int* wave_tot) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int base = 0;
-  for (uint32_t c0 = 0; c0 < dv.L.n_prims; c0 += WG) {
+  for (uint32_t c0 = 0; c0 < dv.L.n_prims; c0 += NT) {
     uint32_t i = c0 + tid;
     bool keep = false;
     if (i < dv.L.n_prims) keep = box_gap(dv.pbox + 6 * i, gb) <= R;
@@ -89,7 +90,7 @@ __device__ int cull(const Dev& dv, const float* gb, float R, uint32_t* cand, int
     for (int w = 0; w < wave; ++w) off += wave_tot[w];
     if (keep && off + before < MAXC) cand[off + before] = i;
     int tot = 0;
-    for (int w = 0; w < WG / 64; ++w) tot += wave_tot[w];
+    for (int w = 0; w < NT / 64; ++w) tot += wave_tot[w];
     __syncthreads();
     base += tot;
   }
@@ -97,12 +98,13 @@ __device__ int cull(const Dev& dv, const float* gb, float R, uint32_t* cand, int
 }
 
 // Block reduction of a box (lo, hi) and a radius.
+template <int NT = WG>
 __device__ void block_box(float lo[3], float hi[3], float& r, float* red) {
   const int tid = threadIdx.x;
   for (int k = 0; k < 3; ++k) {
     red[tid] = lo[k];
     __syncthreads();
-    for (int s = WG / 2; s > 0; s >>= 1) {
+    for (int s = NT / 2; s > 0; s >>= 1) {
       if (tid < s) red[tid] = fminf(red[tid], red[tid + s]);
       __syncthreads();
     }
@@ -110,7 +112,7 @@ __device__ void block_box(float lo[3], float hi[3], float& r, float* red) {
     __syncthreads();
     red[tid] = hi[k];
     __syncthreads();
-    for (int s = WG / 2; s > 0; s >>= 1) {
+    for (int s = NT / 2; s > 0; s >>= 1) {
       if (tid < s) red[tid] = fmaxf(red[tid], red[tid + s]);
       __syncthreads();
     }
@@ -119,7 +121,7 @@ __device__ void block_box(float lo[3], float hi[3], float& r, float* red) {
   }
   red[tid] = r;
   __syncthreads();
-  for (int s = WG / 2; s > 0; s >>= 1) {
+  for (int s = NT / 2; s > 0; s >>= 1) {
     if (tid < s) red[tid] = fmaxf(red[tid], red[tid + s]);
     __syncthreads();
   }
@@ -258,14 +260,20 @@ __device__ inline v3<double> brick_first(const Lv& lv, const double* origin, lon
 
 // Brick centres, searched within the brick's reach r + clamp (r the half diagonal of
 // the brick grown by the clamp): dc the signed distance or NaN past it; band 1 when
-// the decision keeps the brick (|dc| <= reach); sign +-1 when known, else 0.
+// the decision keeps the brick (|dc| <= reach); sign +-1 when known, else 0. One wave
+// a tile of 4 x 4 x 4 bricks, so a tile's candidates are those near the tile.
+constexpr int DT = 64;
 template <bool MIXED>
-__global__ __launch_bounds__(WG) void k_decide(Dev dv, Lv lv, long nbricks, double* dc, uint8_t* band, int8_t* sgn) {
+__global__ __launch_bounds__(DT) void k_decide(Dev dv, Lv lv, long nbricks, double* dc, uint8_t* band, int8_t* sgn) {
   __shared__ uint32_t cand[MAXC];
-  __shared__ float red[WG];
-  __shared__ int wave_tot[WG / 64];
-  long b = long(blockIdx.x) * WG + threadIdx.x;
-  bool live = b < nbricks;
+  __shared__ float red[DT];
+  __shared__ int wave_tot[1];
+  const int tx = (lv.nb[0] + 3) / 4, ty = (lv.nb[1] + 3) / 4;
+  long tile = blockIdx.x;
+  int lane = threadIdx.x;
+  long bx = 4 * (tile % tx) + (lane & 3), by = 4 * ((tile / tx) % ty) + ((lane >> 2) & 3), bz = 4 * (tile / (long(tx) * ty)) + (lane >> 4);
+  bool live = bx < lv.nb[0] && by < lv.nb[1] && bz < lv.nb[2];
+  long b = bx + lv.nb[0] * (by + long(lv.nb[1]) * bz);
   double h = lv.h, clamp = h * 1.7320508075688772;
   double half = 4.0 * h;
   double reach = 0.5 * sqrt(3.0) * (8.0 * h + 2.0 * clamp) + clamp;
@@ -274,9 +282,9 @@ __global__ __launch_bounds__(WG) void k_decide(Dev dv, Lv lv, long nbricks, doub
   float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
   float r = float(reach) * (1.0f + 1e-4f);
   if (live) lo[0] = hi[0] = float(c.x), lo[1] = hi[1] = float(c.y), lo[2] = hi[2] = float(c.z);
-  block_box(lo, hi, r, red);
+  block_box<DT>(lo, hi, r, red);
   float gb[6] = {lo[0] - 1e-4f, lo[1] - 1e-4f, lo[2] - 1e-4f, hi[0] + 1e-4f, hi[1] + 1e-4f, hi[2] + 1e-4f};
-  int nc = cull(dv, gb, r, cand, wave_tot);
+  int nc = cull<DT>(dv, gb, r, cand, wave_tot);
   if (!live) return;
   Answer a = answer<MIXED>(dv, c, reach * (1.0 + 1e-9), cand, nc);
   int8_t s = 0;
@@ -286,7 +294,8 @@ __global__ __launch_bounds__(WG) void k_decide(Dev dv, Lv lv, long nbricks, doub
     s = a.v < 0 ? -1 : 1;
   } else {
     dc[b] = __builtin_nan("");
-    band[b] = a.flag == 2 ? 1 : 0;  // an undecided centre keeps its brick
+    // an undecided centre keeps its brick unless its lower bound is past the reach
+    band[b] = (a.flag == 2 && a.v <= reach) ? 1 : 0;
     // a centre outside the part's box is outside the part
     const float* L0 = dv.L.lo;
     const float* L1 = dv.L.hi;
@@ -373,7 +382,8 @@ __device__ inline double nb26_len(int k) {
 
 template <bool MIXED, int MODE>
 __global__ __launch_bounds__(WG) void k_bricks(Dev dv, Lv lv, const uint32_t* list, const double* firsts, long nbricks, const double* dcs,
-                                               double* out_nodes, uint32_t* staging, uint32_t* cls, unsigned long long* counters) {
+                                               const int8_t* sgns, double* out_nodes, uint32_t* staging, uint32_t* cls,
+                                               unsigned long long* counters) {
   __shared__ uint32_t cand[MAXC];
   __shared__ float red[WG];
   __shared__ int wave_tot[WG / 64];
@@ -440,7 +450,10 @@ __global__ __launch_bounds__(WG) void k_bricks(Dev dv, Lv lv, const uint32_t* li
   };
   fill();
   // partcache.rs:856-871 far nodes within the empty ball of the centre share its sign
-  const double dc = dcs ? dcs[list ? list[bi] : bi] : __builtin_nan("");
+  // the centre's distance; when the decision left it undecided, its sign alone as carried
+  // along the grid (a brick no surface comes near hands its sign to its neighbours)
+  double dc = dcs ? dcs[list ? list[bi] : bi] : __builtin_nan("");
+  const int8_t csgn = sgns ? sgns[list ? list[bi] : bi] : int8_t(0);
   const double half = 4.0 * h;
   if (dc == dc && dc != 0.0) {
     double reach = 0.999 * fmax(clamp, fabs(dc));
@@ -462,7 +475,7 @@ __global__ __launch_bounds__(WG) void k_bricks(Dev dv, Lv lv, const uint32_t* li
       int nfar = 0;
       for (int n = 0; n < 729; ++n) nfar += st[n] == 1;
       if (nfar == 729) {
-        double s = (dc == dc && dc != 0.0) ? dc : 1.0;
+        double s = (dc == dc && dc != 0.0) ? dc : csgn < 0 ? -1.0 : 1.0;
         for (int n = 0; n < 729; ++n) val[n] = copysign(clamp, s), st[n] = 3;
       }
       bool exact[729];
@@ -781,12 +794,12 @@ int icam_hip_bricks(icam_hip* h, int precision, const icam_brick_rec* recs, size
   Lv lv{};
   if (precision == ICAM_MIXED)
     hipLaunchKernelGGL((k_bricks<true, MODE_NODES>), dim3(unsigned(n)), dim3(WG), 0, h->stream, h->dv, lv, (const uint32_t*)nullptr,
-                       h->firsts.as<double>(), long(n), (const double*)nullptr, h->nodes_out.as<double>(), (uint32_t*)nullptr, (uint32_t*)nullptr,
-                       (unsigned long long*)nullptr);
+                       h->firsts.as<double>(), long(n), (const double*)nullptr, (const int8_t*)nullptr, h->nodes_out.as<double>(), (uint32_t*)nullptr,
+                       (uint32_t*)nullptr, (unsigned long long*)nullptr);
   else
     hipLaunchKernelGGL((k_bricks<false, MODE_NODES>), dim3(unsigned(n)), dim3(WG), 0, h->stream, h->dv, lv, (const uint32_t*)nullptr,
-                       h->firsts.as<double>(), long(n), (const double*)nullptr, h->nodes_out.as<double>(), (uint32_t*)nullptr, (uint32_t*)nullptr,
-                       (unsigned long long*)nullptr);
+                       h->firsts.as<double>(), long(n), (const double*)nullptr, (const int8_t*)nullptr, h->nodes_out.as<double>(), (uint32_t*)nullptr,
+                       (uint32_t*)nullptr, (unsigned long long*)nullptr);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(out, h->nodes_out.p, n * 729 * 16, hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
@@ -815,10 +828,11 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   auto* N = timed ? &names : nullptr;
   bool mixed = precision == ICAM_MIXED;
   dim3 gb(unsigned((nb + WG - 1) / WG));
+  dim3 gt(unsigned(long((lv.nb[0] + 3) / 4) * ((lv.nb[1] + 3) / 4) * ((lv.nb[2] + 3) / 4)));
   if (mixed)
-    HIP_OK(launch(h, "decide", s, gb, dim3(WG), T, N, k_decide<true>, h->dv, lv, nb, h->dc.as<double>(), h->band.as<uint8_t>(), h->sgn.as<int8_t>()));
+    HIP_OK(launch(h, "decide", s, gt, dim3(DT), T, N, k_decide<true>, h->dv, lv, nb, h->dc.as<double>(), h->band.as<uint8_t>(), h->sgn.as<int8_t>()));
   else
-    HIP_OK(launch(h, "decide", s, gb, dim3(WG), T, N, k_decide<false>, h->dv, lv, nb, h->dc.as<double>(), h->band.as<uint8_t>(), h->sgn.as<int8_t>()));
+    HIP_OK(launch(h, "decide", s, gt, dim3(DT), T, N, k_decide<false>, h->dv, lv, nb, h->dc.as<double>(), h->band.as<uint8_t>(), h->sgn.as<int8_t>()));
   // signs of the bricks kept whole: along x, y, z lines, until nothing changes
   for (int round = 0; round < 8; ++round) {
     HIP_OK(hipMemsetAsync(h->changed.p, 0, 4, s));
@@ -847,11 +861,13 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   if (nband) {
     if (mixed)
       HIP_OK(launch(h, "bricks", s, dim3(nband), dim3(WG), T, N, k_bricks<true, MODE_BAKE>, h->dv, lv, (const uint32_t*)h->list.as<uint32_t>(),
-                    (const double*)nullptr, long(nband), (const double*)h->dc.as<double>(), (double*)nullptr, h->staging.as<uint32_t>(),
+                    (const double*)nullptr, long(nband), (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>(), (double*)nullptr,
+                    h->staging.as<uint32_t>(),
                     h->cls.as<uint32_t>(), h->counters.as<unsigned long long>()));
     else
       HIP_OK(launch(h, "bricks", s, dim3(nband), dim3(WG), T, N, k_bricks<false, MODE_BAKE>, h->dv, lv, (const uint32_t*)h->list.as<uint32_t>(),
-                    (const double*)nullptr, long(nband), (const double*)h->dc.as<double>(), (double*)nullptr, h->staging.as<uint32_t>(),
+                    (const double*)nullptr, long(nband), (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>(), (double*)nullptr,
+                    h->staging.as<uint32_t>(),
                     h->cls.as<uint32_t>(), h->counters.as<unsigned long long>()));
     HIP_OK(launch(h, "word_flags", s, dim3(unsigned((nband + 255) / 256)), dim3(256), T, N, k_flags_words, (const uint32_t*)h->cls.as<uint32_t>(),
                   long(nband), h->wflags.as<uint8_t>()));

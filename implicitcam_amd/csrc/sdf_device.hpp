@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/icam_part.h"
 #include "layout.hpp"
 
 namespace icam {
@@ -396,16 +397,21 @@ template <class T> __device__ SEval<T> peval(const T* pw, T s, T t) {
   return e;
 }
 
-// surface3.rs:401-452 projected Newton on |S - p|^2 over [s0, s1] x [t0, t1], a
-// parameter on its bound with its step pointing out held there
+// Projected Newton on |S - p|^2 over the rectangle [s0, s1] x [t0, t1] (after
+// surface3.rs:401-452, with the active set decided afresh each step): a parameter on
+// its bound is held there while the gradient points out of the rectangle, and Newton
+// runs in the other alone. Over a part its radius of convexity certifies, the
+// squared distance is strictly convex, so this reaches the part's one minimum,
+// inside it or on its border (where two parts share a border both reach it).
 template <class T> __device__ void newton2(const T* pw, v3<T> p, T& s, T& t, T s0, T s1, T t0, T t1, int iters, T stop) {
-  bool hs = false, ht = false;
   for (int it = 0; it < iters; ++it) {
     SEval<T> e = peval(pw, s, t);
     v3<T> r = e.p - p;
     T g0 = dot(e.ps, r), g1 = dot(e.pt, r);
     T jss = dot(e.ps, e.ps), jst = dot(e.ps, e.pt), jtt = dot(e.pt, e.pt);
     T h0 = jss + dot(e.pss, r), h1 = jst + dot(e.pst, r), h2 = jtt + dot(e.ptt, r);
+    bool hs = (s <= s0 && g0 > T(0)) || (s >= s1 && g0 < T(0));
+    bool ht = (t <= t0 && g1 > T(0)) || (t >= t1 && g1 < T(0));
     T ds, dt;
     if (hs && ht) break;
     if (hs) {
@@ -420,14 +426,6 @@ template <class T> __device__ void newton2(const T* pw, v3<T> p, T& s, T& t, T s
       if (!(mabs(det) > T(0))) break;
       ds = -(h2 * g0 - h1 * g1) / det;
       dt = -(h0 * g1 - h1 * g0) / det;
-    }
-    if (!hs && ((s <= s0 && ds < T(0)) || (s >= s1 && ds > T(0)))) {
-      hs = true;
-      continue;
-    }
-    if (!ht && ((t <= t0 && dt < T(0)) || (t >= t1 && dt > T(0)))) {
-      ht = true;
-      continue;
     }
     T ns = mclamp(s + ds, s0, s1), nt = mclamp(t + dt, t0, t1);
     T moved = mabs(ns - s) + mabs(nt - t);
