@@ -83,9 +83,9 @@ def main():
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / args.steps
         res[name] = dict(ms=dt * 1e3, band=st.band, slots=st.slots, bricks=st.bricks, undecided=st.undecided, refined=st.refined,
-                         stages={k: v / args.steps for k, v in stage_sum.items()})
-        log(f"bake {name}: {dt * 1e3:.3f} ms/step, band {st.band}, slots {st.slots}, undecided {st.undecided}, refined nodes {st.refined}, "
-            f"stages {res[name]['stages']}")
+                         unsure_trim=st.unsure_trim, unsure_tie=st.unsure_tie, stages={k: v / args.steps for k, v in stage_sum.items()})
+        log(f"bake {name}: {dt * 1e3:.3f} ms/step, band {st.band}, slots {st.slots}, undecided {st.undecided}, searched again in fp64 "
+            f"{st.refined} (trim {st.unsure_trim}, ties {st.unsure_tie}), stages {res[name]['stages']}")
     if args.profile:
         return
 
@@ -145,7 +145,7 @@ def main():
             log(f"check: {check}")
 
     mixed, fp64 = res["mixed"], res["fp64"]
-    bricks_ms = mixed["stages"].get("bricks", float("nan"))
+    bricks_ms = mixed["stages"].get("nodes_search32", float("nan"))
     # HBM traffic of the dominant kernel: its algorithmic bytes are the staged words it
     # writes (1460 B a band brick) and the tables it reads once (small); per launch
     algo_bytes = mixed["band"] * 1460.0

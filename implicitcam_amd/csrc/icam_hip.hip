@@ -1,19 +1,28 @@
 // libicam_hip: the exact signed distance and the narrow-band brick baker on gfx950.
 //
-// Kernels (one launch each, on the handle's stream):
-//  k_points  one lane a point; the block culls the primitive list against its
-//            points' box into LDS, then every lane reads the same candidates in
-//            the same order (their records come through the scalar cache);
-//  k_decide  one lane a brick centre, searched within the brick's reach
-//            (PartCache::decide's test, crates/icam-sim/src/partcache.rs:226-331);
-//  k_sign    signs of the bricks no surface comes near, carried along grid lines;
-//  k_bricks  one block a brick: its 729 nodes searched within the clamp, far nodes
-//            signed by flood fill, the rest bridged, quantized to i16 and packed two
-//            to a word (partcache.rs:759-918, 962-970);
-//  k_scan    exclusive prefix sums (band bricks, stored slots);
-//  k_store   the map and the atlas in map order (partcache.rs:128-146).
-// Each search runs in double precision (ICAM_FP64) or in single precision with a
-// last correction in double (ICAM_MIXED).
+// A query point comes from one of three sources: explicit points, the 729 nodes of
+// band bricks, or the centres of a level's bricks. Each is searched by one lane; a
+// block culls the primitive list against its points' box into LDS and sorts what it
+// keeps nearest first, then every lane reads the same candidates in the same order
+// (their records arrive through the scalar cache).
+//
+// Kernels, one launch each on the handle's stream:
+//  k_search<S, float>   the single-precision search: each point's winning feature,
+//                        its parameters, distance and sign, and whether a decision was
+//                        taken within single-precision rounding;
+//  k_refine              the last correction in double precision of each winner; the
+//                        points the search was unsure of are queued;
+//  k_fallback<S>         the queued points searched again in double precision;
+//  k_search<S, double>   the plain double-precision search (ICAM_FP64);
+//  k_decide_finish       the level's decision from its brick centres
+//                        (PartCache::decide's test, crates/icam-sim/src/partcache.rs:226-331);
+//  k_sign                signs of the bricks no surface comes near, along grid lines;
+//  k_bricks_finish       a brick's far nodes signed by flood fill, the rest bridged,
+//                        quantized to i16 and packed two to a word (partcache.rs:759-918,
+//                        962-970);
+//  k_scan, k_store*      prefix sums, the map and the atlas in map order (partcache.rs:128-146).
+// Splitting the precisions into their own kernels keeps each kernel's registers to
+// its own precision's needs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,6 +43,7 @@ namespace {
 
 constexpr int WG = 256;
 constexpr int MAXC = 4096;
+constexpr int SORTC = 1024;  // lists up to this long are sorted nearest first
 constexpr uint32_t NONE = 0xffffffffu;
 
 thread_local std::string g_error;
@@ -43,10 +53,10 @@ int fail(const std::string& s) {
   return 1;
 }
 
-#define HIP_OK(x)                                                                              \
-  do {                                                                                          \
-    hipError_t e_ = (x);                                                                        \
-    if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_));         \
+#define HIP_OK(x)                                                                      \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
 struct Dev {
@@ -63,6 +73,85 @@ template <class T> __device__ inline Tab<T> tab_of(const Dev& d);
 template <> __device__ inline Tab<float> tab_of<float>(const Dev& d) { return Tab<float>{d.F32, d.U, d.pbox, d.pcode, d.L}; }
 template <> __device__ inline Tab<double> tab_of<double>(const Dev& d) { return Tab<double>{d.F64, d.U, d.pbox, d.pcode, d.L}; }
 
+// The level's geometry.
+struct Lv {
+  double lo[3];
+  double h;
+  int nb[3];
+};
+
+__device__ inline v3<double> brick_first(const Lv& lv, const double* origin, long b) {
+  long x = b % lv.nb[0], y = (b / lv.nb[0]) % lv.nb[1], z = b / (long(lv.nb[0]) * lv.nb[1]);
+  double side = 8.0 * lv.h;
+  return mk3(lv.lo[0] - origin[0] + side * x, lv.lo[1] - origin[1] + side * y, lv.lo[2] - origin[2] + side * z);
+}
+
+// The reach of a brick: half the diagonal of the brick grown by the clamp, plus the
+// clamp (a centre farther than this from the surface decides the brick whole).
+__device__ inline double brick_reach(double h) {
+  double clamp = h * 1.7320508075688772;
+  return 0.5 * 1.7320508075688772 * (8.0 * h + 2.0 * clamp) + clamp;
+}
+
+// ------------------------------------------------------------------ point sources
+
+enum Src { SRC_POINTS = 0, SRC_NODES = 1, SRC_CENTRES = 2 };
+
+struct Job {
+  long n;                 // points
+  const double* xyz;      // SRC_POINTS: world points
+  const double* bounds;   // SRC_POINTS: per-point bounds (NULL: none)
+  const uint32_t* list;   // SRC_NODES: band bricks of the level (NULL: `firsts`)
+  const double* firsts;   // SRC_NODES: explicit bricks (first node xyz, h)
+  Lv lv;
+};
+
+// Point i of a job (relative to the origin) and the distance within which it must
+// be exact.
+template <int S> __device__ inline v3<double> job_point(const Job& j, const Dev& dv, long i, double& bound) {
+  if (S == SRC_POINTS) {
+    bound = j.bounds ? j.bounds[i] : 1e300;
+    return mk3(j.xyz[3 * i] - dv.origin[0], j.xyz[3 * i + 1] - dv.origin[1], j.xyz[3 * i + 2] - dv.origin[2]);
+  }
+  if (S == SRC_NODES) {
+    long bi = i / 729;
+    int n = int(i - 729 * bi);
+    v3<double> first;
+    double h;
+    if (j.list) {
+      first = brick_first(j.lv, dv.origin, long(j.list[bi]));
+      h = j.lv.h;
+    } else {
+      first = mk3(j.firsts[4 * bi] - dv.origin[0], j.firsts[4 * bi + 1] - dv.origin[1], j.firsts[4 * bi + 2] - dv.origin[2]);
+      h = j.firsts[4 * bi + 3];
+    }
+    bound = h * 1.7320508075688772;
+    return first + mk3(h * (n % 9), h * ((n / 9) % 9), h * (n / 81));
+  }
+  double h = j.lv.h;
+  bound = brick_reach(h) * (1.0 + 1e-9);
+  return brick_first(j.lv, dv.origin, i) + mk3(4.0 * h, 4.0 * h, 4.0 * h);
+}
+
+// Per-point results. The single-precision search fills code .. info; the final
+// answers are val (the signed distance, or the bound / lower bound), flag (0 exact,
+// 1 bound, 2 undecided, 3 waiting for the fallback) and feat (kind << 28 | index).
+struct Res {
+  uint32_t* code;
+  float* pa;
+  float* pb;
+  float* d32;
+  uint32_t* info;  // bit 0 found, 1 negative, 2 unsure, 3 undecided
+  double* val;
+  uint32_t* flag;
+  uint32_t* feat;
+  uint32_t* queue;    // blocks holding points the single-precision search was unsure of
+  uint32_t* qcount;
+  uint32_t* qmark;    // per block: queued already
+  unsigned long long* counters;  // [0] bridged nodes, [1] points searched again in double,
+                                 // [3] of them near a trim curve, [4] near-ties of opposite sign
+};
+
 // ------------------------------------------------------------------ block culling
 
 __device__ inline float box_gap(const float* a, const float* b) {
@@ -74,7 +163,7 @@ __device__ inline float box_gap(const float* a, const float* b) {
 
 // The primitives whose box comes within R of the block's box gb (lo, hi), in list
 // order, into cand; returns how many (more than MAXC: the list did not fit).
-template <int NT = WG>
+template <int NT>
 __device__ int cull(const Dev& dv, const float* gb, float R, uint32_t* cand, int* wave_tot) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int base = 0;
@@ -97,48 +186,51 @@ __device__ int cull(const Dev& dv, const float* gb, float R, uint32_t* cand, int
   return base;
 }
 
-// Block reduction of a box (lo, hi) and a radius.
-template <int NT = WG>
-__device__ void block_box(float lo[3], float hi[3], float& r, float* red) {
+// The kept candidates ordered by their box's distance from the block's box centre,
+// nearest first (ties by list order), so a lane's bound drops early. tmp and key
+// hold SORTC entries.
+template <int NT>
+__device__ void sort_cand(const Dev& dv, const float* gb, uint32_t* cand, uint32_t* tmp, float* key, int nc) {
+  if (nc <= 1 || nc > SORTC) return;
   const int tid = threadIdx.x;
-  for (int k = 0; k < 3; ++k) {
-    red[tid] = lo[k];
-    __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1) {
-      if (tid < s) red[tid] = fminf(red[tid], red[tid + s]);
-      __syncthreads();
-    }
-    lo[k] = red[0];
-    __syncthreads();
-    red[tid] = hi[k];
-    __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1) {
-      if (tid < s) red[tid] = fmaxf(red[tid], red[tid + s]);
-      __syncthreads();
-    }
-    hi[k] = red[0];
-    __syncthreads();
-  }
-  red[tid] = r;
+  float c[6] = {0.5f * (gb[0] + gb[3]), 0.5f * (gb[1] + gb[4]), 0.5f * (gb[2] + gb[5]), 0, 0, 0};
+  c[3] = c[0], c[4] = c[1], c[5] = c[2];
+  for (int k = tid; k < nc; k += NT) key[k] = box_gap(dv.pbox + 6 * cand[k], c), tmp[k] = cand[k];
   __syncthreads();
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (tid < s) red[tid] = fmaxf(red[tid], red[tid + s]);
-    __syncthreads();
+  for (int k = tid; k < nc; k += NT) {
+    float kk = key[k];
+    int rank = 0;
+    for (int j = 0; j < nc; ++j) rank += (key[j] < kk) || (key[j] == kk && j < k);
+    cand[rank] = tmp[k];
   }
-  r = red[0];
   __syncthreads();
 }
 
-// ------------------------------------------------------------------ one lane's search
+// Block reduction of a box (lo, hi) and a radius.
+template <int NT>
+__device__ void block_box(float lo[3], float hi[3], float& r, float* red) {
+  const int tid = threadIdx.x;
+  for (int k = 0; k < 7; ++k) {
+    bool is_min = k < 3;
+    float v = k < 3 ? lo[k] : k < 6 ? hi[k - 3] : r;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+      if (tid < s) red[tid] = is_min ? fminf(red[tid], red[tid + s]) : fmaxf(red[tid], red[tid + s]);
+      __syncthreads();
+    }
+    v = red[0];
+    __syncthreads();
+    if (k < 3)
+      lo[k] = v;
+    else if (k < 6)
+      hi[k - 3] = v;
+    else
+      r = v;
+  }
+}
 
-// The answer of one lane: value, flag (0 exact, 1 bound, 2 undecided), feature, and
-// whether a double-precision step was taken.
-struct Answer {
-  double v;
-  uint32_t flag;
-  uint32_t feat;
-  bool refined;
-};
+// ------------------------------------------------------------------ one lane's search
 
 template <class T>
 __device__ Best<T> search(const Tab<T>& tb, v3<T> p, T bound, const uint32_t* cand, int nc, T tm, T eps) {
@@ -153,150 +245,242 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, T bound, const uint32_t* ca
   return b;
 }
 
-__device__ inline uint32_t feat_of(const Tab<double>& tb, uint32_t code) {
+__device__ inline uint32_t feat_of(const Dev& dv, uint32_t code) {
   uint32_t kind = code >> 28, id = code & 0x0fffffffu;
   if (kind == P_FACE) return id;
   if (kind == P_EDGE) return (1u << 28) | id;
   if (kind == P_VERT) return (2u << 28) | id;
-  if (kind == P_LEAF) return tb.U[tb.L.leaf_u + LEAF_U * id + 1];
-  return (1u << 28) | tb.U[tb.L.cleaf_u + CLEAF_U * id + 1];
+  if (kind == P_LEAF) return dv.U[dv.L.leaf_u + LEAF_U * id + 1];
+  return (1u << 28) | dv.U[dv.L.cleaf_u + CLEAF_U * id + 1];
 }
 
-template <class T>
-__device__ Answer classify(const Best<T>& b, double d, double sg, double bound) {
-  Answer a;
-  a.refined = false;
-  a.feat = NONE;
-  if (b.code != NONE && !(b.lmin < b.d) && double(b.d) <= bound) {
-    a.v = d * sg, a.flag = 0;
-    return a;
+// A finished double-precision search as (value, flag): exact when a feature answered
+// and no undecided part had a lower bound under it; a bound when nothing came within
+// the bound; undecided (the least lower bound) otherwise.
+__device__ inline void classify64(const Best<double>& b, double bound, const Dev& dv, double& val, uint32_t& flag, uint32_t& feat) {
+  feat = NONE;
+  if (b.code != NONE && !(b.lmin < b.d) && b.d <= bound) {
+    val = b.d * b.sgn, flag = 0, feat = feat_of(dv, b.code);
+    return;
   }
-  T l = mmin(b.d, b.lmin);
-  if (double(l) >= bound || (b.code == NONE && !(b.lmin < b.d))) {
-    a.v = bound, a.flag = 1;
-    return a;
+  double l = fmin(b.d, b.lmin);
+  if (l >= bound || (b.code == NONE && !(b.lmin < b.d))) {
+    val = bound, flag = 1;
+    return;
   }
-  a.v = double(l), a.flag = 2;
-  return a;
+  val = l, flag = 2;
 }
 
-// One point: the search in the precision asked, with the last correction in double.
-template <bool MIXED>
-__device__ Answer answer(const Dev& dv, v3<double> p, double bound, const uint32_t* cand, int nc) {
-  Tab<double> t64 = tab_of<double>(dv);
-  if (MIXED) {
-    Tab<float> t32 = tab_of<float>(dv);
-    const float eps = dv.L.eps32;
-    float b32 = bound < 3.0e38 ? float(bound) * (1.0f + 1e-6f) + eps : 3.0e38f;
-    Best<float> b = search<float>(t32, cv3<float>(p), b32, cand, nc, 4e-5f * float(dv.L.diag) + eps, eps);
-    bool tie = b.d2 - b.d <= 8.0f * eps && b.sgn2 != b.sgn;
-    if (!b.unsure && !tie) {
-      double d = double(b.d), sg = double(b.sgn);
-      if (b.code != NONE) refine64(t64, b.code, double(b.a), double(b.b), p, d, sg);
-      Answer a = classify(b, d, sg, bound);
-      if (a.flag == 0) a.feat = feat_of(t64, b.code);
-      return a;
-    }
-  }
-  Best<double> b = search<double>(t64, p, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, cand, nc, 0.0, dv.L.eps64);
-  Answer a = classify(b, b.d, b.sgn, bound);
-  if (a.flag == 0) a.feat = feat_of(t64, b.code);
-  a.refined = MIXED;
-  return a;
+// Block shapes: explicit points 256 a block; a brick's 729 nodes a block of 256
+// lanes; a tile of 4 x 4 x 4 brick centres a block of 64.
+template <int S> struct Shape {
+  static constexpr int NT = S == SRC_CENTRES ? 64 : 256;
+};
+
+// The tile of 4 x 4 x 4 bricks of block `tile`: lane -> brick index, -1 past the level;
+// and the tile a brick lies in.
+__device__ inline long tile_of(const Lv& lv, long b) {
+  const long tx = (lv.nb[0] + 3) / 4, ty = (lv.nb[1] + 3) / 4;
+  long bx = b % lv.nb[0], by = (b / lv.nb[0]) % lv.nb[1], bz = b / (long(lv.nb[0]) * lv.nb[1]);
+  return bx / 4 + tx * (by / 4 + ty * (bz / 4));
+}
+__device__ inline long tile_point(const Lv& lv, long tile, int lane) {
+  const int tx = (lv.nb[0] + 3) / 4, ty = (lv.nb[1] + 3) / 4;
+  long bx = 4 * (tile % tx) + (lane & 3), by = 4 * ((tile / tx) % ty) + ((lane >> 2) & 3), bz = 4 * (tile / (long(tx) * ty)) + (lane >> 4);
+  if (bx >= lv.nb[0] || by >= lv.nb[1] || bz >= lv.nb[2]) return -1;
+  return bx + lv.nb[0] * (by + long(lv.nb[1]) * bz);
 }
 
-// ------------------------------------------------------------------ kernels
+// Block blk's m-th point for lane threadIdx.x (m counts the lane's rounds), or -1;
+// and the block a point belongs to.
+template <int S> __device__ inline long block_point(const Job& j, long blk, int m) {
+  constexpr int NT = Shape<S>::NT;
+  if (S == SRC_POINTS) {
+    if (m > 0) return -1;
+    long i = blk * NT + threadIdx.x;
+    return i < j.n ? i : -1;
+  }
+  if (S == SRC_NODES) {
+    int n = threadIdx.x + NT * m;
+    return n < 729 ? blk * 729 + n : -1;
+  }
+  return m > 0 ? -1 : tile_point(j.lv, blk, threadIdx.x);
+}
+template <int S> __device__ inline long block_of(const Job& j, long i) {
+  if (S == SRC_POINTS) return i / Shape<S>::NT;
+  if (S == SRC_NODES) return i / 729;
+  return tile_of(j.lv, i);
+}
+template <int S> constexpr int rounds() { return S == SRC_NODES ? 3 : 1; }
 
-// Points: xyz (world, double), optional per-point bounds.
-template <bool MIXED>
-__global__ __launch_bounds__(WG) void k_points(Dev dv, const double* xyz, const double* bounds, size_t n, double* out_d, uint32_t* out_flag,
-                                              uint32_t* out_feat) {
-  __shared__ uint32_t cand[MAXC];
-  __shared__ float red[WG];
-  __shared__ int wave_tot[WG / 64];
-  size_t i = size_t(blockIdx.x) * WG + threadIdx.x;
-  bool live = i < n;
-  v3<double> p = mk3(0.0, 0.0, 0.0);
-  double bound = 1e300;
-  if (live) {
-    p = mk3(xyz[3 * i] - dv.origin[0], xyz[3 * i + 1] - dv.origin[1], xyz[3 * i + 2] - dv.origin[2]);
-    if (bounds) bound = bounds[i];
-  }
-  // an upper bound on the distance: the nearest of the first vertices (points of the
-  // surface), or the bound
-  float ub = float(fmin(bound, 3.0e38));
-  uint32_t nv = dv.L.n_verts < 16 ? dv.L.n_verts : 16;
-  for (uint32_t k = 0; k < nv; ++k) {
-    const double* vr = dv.F64 + dv.L.vert_f + VERT_F * k;
-    ub = fminf(ub, float(len(p - ld3(vr))) * (1.0f + 1e-5f) + 1e-5f);
-  }
+// The block's candidates: the primitives within reach of its points' box, where the
+// reach covers every point's answer (its bound, or for unbounded points the nearest
+// of the first vertices, points of the surface).
+template <int S>
+__device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t* cand, uint32_t* tmp, float* key, float* red, int* wave_tot) {
+  constexpr int NT = Shape<S>::NT;
   float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
   float r = 0.f;
-  if (live) {
-    lo[0] = hi[0] = float(p.x), lo[1] = hi[1] = float(p.y), lo[2] = hi[2] = float(p.z);
-    r = ub;
+  for (int m = 0; m < rounds<S>(); ++m) {
+    long i = block_point<S>(j, blk, m);
+    if (i < 0) continue;
+    double bound;
+    v3<double> p = job_point<S>(j, dv, i, bound);
+    float ub = float(fmin(bound, 3.0e38));
+    if (S == SRC_POINTS) {
+      uint32_t nv = dv.L.n_verts < 16 ? dv.L.n_verts : 16;
+      for (uint32_t q = 0; q < nv; ++q) ub = fminf(ub, float(len(p - ld3(dv.F64 + dv.L.vert_f + VERT_F * q))) * (1.0f + 1e-5f) + 1e-5f);
+    }
+    lo[0] = fminf(lo[0], float(p.x)), lo[1] = fminf(lo[1], float(p.y)), lo[2] = fminf(lo[2], float(p.z));
+    hi[0] = fmaxf(hi[0], float(p.x)), hi[1] = fmaxf(hi[1], float(p.y)), hi[2] = fmaxf(hi[2], float(p.z));
+    r = fmaxf(r, ub * (1.0f + 1e-5f) + 1e-4f);
   }
-  block_box(lo, hi, r, red);
+  block_box<NT>(lo, hi, r, red);
   float gb[6] = {lo[0] - 1e-4f, lo[1] - 1e-4f, lo[2] - 1e-4f, hi[0] + 1e-4f, hi[1] + 1e-4f, hi[2] + 1e-4f};
-  int nc = cull(dv, gb, r, cand, wave_tot);
-  if (!live) return;
-  Answer a = answer<MIXED>(dv, p, bound, cand, nc);
-  out_d[i] = a.v;
-  if (out_flag) out_flag[i] = a.flag;
-  if (out_feat) out_feat[i] = a.feat;
+  int nc = cull<NT>(dv, gb, r, cand, wave_tot);
+  sort_cand<NT>(dv, gb, cand, tmp, key, nc);
+  return nc;
 }
 
-// The level's geometry.
-struct Lv {
-  double lo[3];
-  double h;
-  int nb[3];
-};
-__device__ inline v3<double> brick_first(const Lv& lv, const double* origin, long b) {
-  long x = b % lv.nb[0], y = (b / lv.nb[0]) % lv.nb[1], z = b / (long(lv.nb[0]) * lv.nb[1]);
-  double side = 8.0 * lv.h;
-  return mk3(lv.lo[0] - origin[0] + side * x, lv.lo[1] - origin[1] + side * y, lv.lo[2] - origin[2] + side * z);
-}
+// ------------------------------------------------------------------ search kernels
 
-// Brick centres, searched within the brick's reach r + clamp (r the half diagonal of
-// the brick grown by the clamp): dc the signed distance or NaN past it; band 1 when
-// the decision keeps the brick (|dc| <= reach); sign +-1 when known, else 0. One wave
-// a tile of 4 x 4 x 4 bricks, so a tile's candidates are those near the tile.
-constexpr int DT = 64;
-template <bool MIXED>
-__global__ __launch_bounds__(DT) void k_decide(Dev dv, Lv lv, long nbricks, double* dc, uint8_t* band, int8_t* sgn) {
+// The search, in single (T = float: records for k_refine) or double precision (T =
+// double: final answers).
+template <int S, class T>
+__global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res) {
+  constexpr int NT = Shape<S>::NT;
   __shared__ uint32_t cand[MAXC];
-  __shared__ float red[DT];
-  __shared__ int wave_tot[1];
-  const int tx = (lv.nb[0] + 3) / 4, ty = (lv.nb[1] + 3) / 4;
-  long tile = blockIdx.x;
-  int lane = threadIdx.x;
-  long bx = 4 * (tile % tx) + (lane & 3), by = 4 * ((tile / tx) % ty) + ((lane >> 2) & 3), bz = 4 * (tile / (long(tx) * ty)) + (lane >> 4);
-  bool live = bx < lv.nb[0] && by < lv.nb[1] && bz < lv.nb[2];
-  long b = bx + lv.nb[0] * (by + long(lv.nb[1]) * bz);
-  double h = lv.h, clamp = h * 1.7320508075688772;
-  double half = 4.0 * h;
-  double reach = 0.5 * sqrt(3.0) * (8.0 * h + 2.0 * clamp) + clamp;
-  v3<double> c = mk3(0.0, 0.0, 0.0);
-  if (live) c = brick_first(lv, dv.origin, b) + mk3(half, half, half);
-  float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
-  float r = float(reach) * (1.0f + 1e-4f);
-  if (live) lo[0] = hi[0] = float(c.x), lo[1] = hi[1] = float(c.y), lo[2] = hi[2] = float(c.z);
-  block_box<DT>(lo, hi, r, red);
-  float gb[6] = {lo[0] - 1e-4f, lo[1] - 1e-4f, lo[2] - 1e-4f, hi[0] + 1e-4f, hi[1] + 1e-4f, hi[2] + 1e-4f};
-  int nc = cull<DT>(dv, gb, r, cand, wave_tot);
-  if (!live) return;
-  Answer a = answer<MIXED>(dv, c, reach * (1.0 + 1e-9), cand, nc);
+  __shared__ uint32_t tmp[SORTC];
+  __shared__ float key[SORTC];
+  __shared__ float red[NT];
+  __shared__ int wave_tot[NT / 64];
+  const int nc = block_candidates<S>(dv, j, blockIdx.x, cand, tmp, key, red, wave_tot);
+  const Tab<T> tb = tab_of<T>(dv);
+  const T eps = sizeof(T) == 4 ? T(dv.L.eps32) : T(dv.L.eps64);
+  const T tm = sizeof(T) == 4 ? T(4e-5 * dv.L.diag) + eps : T(0);
+  for (int m = 0; m < rounds<S>(); ++m) {
+    long i = block_point<S>(j, blockIdx.x, m);
+    if (i < 0) continue;
+    double bound;
+    v3<double> p = job_point<S>(j, dv, i, bound);
+    if constexpr (sizeof(T) == 4) {
+      float b32 = bound < 3.0e38 ? float(bound) * (1.0f + 1e-6f) + float(eps) : 3.0e38f;
+      Best<T> b = search<T>(tb, cv3<T>(p), T(b32), cand, nc, tm, eps);
+      bool tie = b.d2 - b.d <= T(8) * eps && b.sgn2 != b.sgn;
+      uint32_t info = (b.code != NONE ? 1u : 0u) | (b.sgn < T(0) ? 2u : 0u) | ((b.unsure || tie) ? 4u : 0u) | ((b.lmin < b.d) ? 8u : 0u) |
+                      (b.unsure ? 16u : 0u) | (tie ? 32u : 0u);
+      res.code[i] = b.code;
+      res.pa[i] = float(b.a);
+      res.pb[i] = float(b.b);
+      res.d32[i] = float((b.lmin < b.d) ? b.lmin : b.d);
+      res.info[i] = info;
+    } else {
+      Best<T> b = search<T>(tb, cv3<T>(p), bound < 1e300 ? T(bound * (1.0 + 1e-12)) : T(bound), cand, nc, tm, eps);
+      double val;
+      uint32_t flag, feat;
+      classify64(b, bound, dv, val, flag, feat);
+      res.val[i] = val;
+      res.flag[i] = flag;
+      if (res.feat) res.feat[i] = feat;
+    }
+  }
+}
+
+// The last correction in double precision of each single-precision winner; points
+// the search was unsure of are marked (flag 3) and their blocks queued for k_fallback.
+template <int S>
+__global__ __launch_bounds__(256) void k_refine(Dev dv, Job j, Res res) {
+  long i = long(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= j.n) return;
+  uint32_t info = res.info[i];
+  double bound;
+  v3<double> p = job_point<S>(j, dv, i, bound);
+  if (info & 4u) {
+    res.flag[i] = 3;
+    long blk = block_of<S>(j, i);
+    if (atomicOr(&res.qmark[blk], 1u) == 0u) res.queue[atomicAdd(res.qcount, 1u)] = uint32_t(blk);
+    atomicAdd(&res.counters[1], 1ull);
+    if (info & 16u) atomicAdd(&res.counters[3], 1ull);
+    if (info & 32u) atomicAdd(&res.counters[4], 1ull);
+    return;
+  }
+  uint32_t flag = 0, feat = NONE;
+  double val;
+  bool found = info & 1u, undecided = info & 8u;
+  double d = res.d32[i];
+  if (found && !undecided) {
+    uint32_t code = res.code[i];
+    double sg = (info & 2u) ? -1.0 : 1.0;
+    refine64(tab_of<double>(dv), code, double(res.pa[i]), double(res.pb[i]), p, d, sg);
+    if (d <= bound) {
+      val = d * sg, feat = feat_of(dv, code);
+    } else {
+      val = bound, flag = 1;
+    }
+  } else if (!undecided) {
+    val = bound, flag = 1;
+  } else {
+    val = d >= bound ? bound : d;
+    flag = d >= bound ? 1 : 2;
+  }
+  res.val[i] = val;
+  res.flag[i] = flag;
+  if (res.feat) res.feat[i] = feat;
+}
+
+// The queued blocks: each block culls its candidates again and its marked points are
+// searched in double precision.
+template <int S>
+__global__ __launch_bounds__(Shape<S>::NT) void k_fallback(Dev dv, Job j, Res res) {
+  constexpr int NT = Shape<S>::NT;
+  __shared__ uint32_t cand[MAXC];
+  __shared__ uint32_t tmp[SORTC];
+  __shared__ float key[SORTC];
+  __shared__ float red[NT];
+  __shared__ int wave_tot[NT / 64];
+  const uint32_t total = *res.qcount;
+  const Tab<double> tb = tab_of<double>(dv);
+  for (uint32_t qb = blockIdx.x; qb < total; qb += gridDim.x) {
+    const long blk = res.queue[qb];
+    const int nc = block_candidates<S>(dv, j, blk, cand, tmp, key, red, wave_tot);
+    for (int m = 0; m < rounds<S>(); ++m) {
+      long i = block_point<S>(j, blk, m);
+      if (i < 0 || res.flag[i] != 3) continue;
+      double bound;
+      v3<double> p = job_point<S>(j, dv, i, bound);
+      Best<double> b = search<double>(tb, p, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, cand, nc, 0.0, dv.L.eps64);
+      double val;
+      uint32_t flag, feat;
+      classify64(b, bound, dv, val, flag, feat);
+      res.val[i] = val;
+      res.flag[i] = flag;
+      if (res.feat) res.feat[i] = feat;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ the level
+
+// The decision from the centres' answers: dc the signed distance or NaN; band 1 when
+// the brick is kept (|dc| <= reach, or undecided under the reach); sign +-1 when
+// known, else 0.
+__global__ void k_decide_finish(Dev dv, Lv lv, long nbricks, const double* val, const uint32_t* flag, double* dc, uint8_t* band, int8_t* sgn) {
+  long b = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nbricks) return;
+  double reach = brick_reach(lv.h);
+  double v = val[b];
+  uint32_t f = flag[b];
   int8_t s = 0;
-  if (a.flag == 0) {
-    dc[b] = a.v;
-    band[b] = fabs(a.v) <= reach ? 1 : 0;
-    s = a.v < 0 ? -1 : 1;
+  if (f == 0) {
+    dc[b] = v;
+    band[b] = fabs(v) <= reach ? 1 : 0;
+    s = v < 0 ? -1 : 1;
   } else {
     dc[b] = __builtin_nan("");
-    // an undecided centre keeps its brick unless its lower bound is past the reach
-    band[b] = (a.flag == 2 && a.v <= reach) ? 1 : 0;
+    band[b] = (f == 2 && v <= reach) ? 1 : 0;
     // a centre outside the part's box is outside the part
+    v3<double> c = brick_first(lv, dv.origin, b) + mk3(4.0 * lv.h, 4.0 * lv.h, 4.0 * lv.h);
     const float* L0 = dv.L.lo;
     const float* L1 = dv.L.hi;
     if (c.x < L0[0] || c.y < L0[1] || c.z < L0[2] || c.x > L1[0] || c.y > L1[1] || c.z > L1[2]) s = 1;
@@ -363,9 +547,6 @@ __global__ void k_band_list(const uint8_t* band, const uint32_t* pos, long n, ui
   if (i < n && band[i]) list[pos[i]] = uint32_t(i);
 }
 
-// Node values of a block's brick: 729 nodes, three a thread.
-enum { MODE_NODES = 0, MODE_BAKE = 1 };
-
 __device__ inline int nb26(int n, int k) {
   // the k-th of the 26 neighbours of node n (x fastest), or -1 outside the brick
   int d = k < 13 ? k : k + 1;  // skip the centre (13)
@@ -380,48 +561,25 @@ __device__ inline double nb26_len(int k) {
   return sqrt(double(di * di + dj * dj + dk * dk));
 }
 
-template <bool MIXED, int MODE>
-__global__ __launch_bounds__(WG) void k_bricks(Dev dv, Lv lv, const uint32_t* list, const double* firsts, long nbricks, const double* dcs,
-                                               const int8_t* sgns, double* out_nodes, uint32_t* staging, uint32_t* cls,
-                                               unsigned long long* counters) {
-  __shared__ uint32_t cand[MAXC];
-  __shared__ float red[WG];
-  __shared__ int wave_tot[WG / 64];
+// A band brick from its nodes' answers: far nodes signed by flood fill from the nodes
+// known, then from the centre's empty ball, what is left bridged; quantized, classed
+// and packed.
+__global__ __launch_bounds__(WG) void k_bricks_finish(Lv lv, const uint32_t* list, long nbricks, const double* dcs, const int8_t* sgns,
+                                                      const double* val_in, const uint32_t* flag_in, uint32_t* staging, uint32_t* cls,
+                                                      unsigned long long* counters) {
   __shared__ double val[729];
-  __shared__ uint8_t st[729];  // 0 exact, 1 far, 2 undecided, 3 filled from a neighbour
+  __shared__ uint8_t st[729];  // 0 exact, 1 far, 2 undecided, 3 filled, 4 bridged
   __shared__ int changed;
+  __shared__ int16_t q[730];
   const int tid = threadIdx.x;
   long bi = blockIdx.x;
   if (bi >= nbricks) return;
-  v3<double> first;
-  double h;
-  if (firsts) {
-    first = mk3(firsts[4 * bi] - dv.origin[0], firsts[4 * bi + 1] - dv.origin[1], firsts[4 * bi + 2] - dv.origin[2]);
-    h = firsts[4 * bi + 3];
-  } else {
-    first = brick_first(lv, dv.origin, long(list[bi]));
-    h = lv.h;
-  }
-  const double clamp = h * 1.7320508075688772;
-  float lo[3] = {float(first.x), float(first.y), float(first.z)};
-  float hi[3] = {float(first.x + 8 * h), float(first.y + 8 * h), float(first.z + 8 * h)};
-  float r = float(clamp) * (1.0f + 1e-4f) + 1e-4f;
-  float gb[6] = {lo[0] - 1e-4f, lo[1] - 1e-4f, lo[2] - 1e-4f, hi[0] + 1e-4f, hi[1] + 1e-4f, hi[2] + 1e-4f};
-  int nc = cull(dv, gb, r, cand, wave_tot);
-  unsigned long long refined = 0;
+  const double h = lv.h, clamp = h * 1.7320508075688772, half = 4.0 * h;
   for (int n = tid; n < 729; n += WG) {
-    v3<double> p = first + mk3(h * (n % 9), h * ((n / 9) % 9), h * (n / 81));
-    Answer a = answer<MIXED>(dv, p, clamp, cand, nc);
-    refined += a.refined ? 1 : 0;
-    if (MODE == MODE_NODES) {
-      out_nodes[2 * (729 * bi + n)] = a.v;
-      out_nodes[2 * (729 * bi + n) + 1] = double(a.flag);
-    } else {
-      val[n] = a.v;
-      st[n] = uint8_t(a.flag);
-    }
+    val[n] = val_in[729 * bi + n];
+    uint32_t f = flag_in[729 * bi + n];
+    st[n] = uint8_t(f > 2 ? 2 : f);
   }
-  if (MODE == MODE_NODES) return;
   __syncthreads();
   // partcache.rs:825-855 far nodes take the sign of any neighbour known, by flood fill
   auto fill = [&]() {
@@ -450,11 +608,9 @@ __global__ __launch_bounds__(WG) void k_bricks(Dev dv, Lv lv, const uint32_t* li
   };
   fill();
   // partcache.rs:856-871 far nodes within the empty ball of the centre share its sign
-  // the centre's distance; when the decision left it undecided, its sign alone as carried
-  // along the grid (a brick no surface comes near hands its sign to its neighbours)
-  double dc = dcs ? dcs[list ? list[bi] : bi] : __builtin_nan("");
-  const int8_t csgn = sgns ? sgns[list ? list[bi] : bi] : int8_t(0);
-  const double half = 4.0 * h;
+  const long b = list[bi];
+  const double dc = dcs[b];
+  const int8_t csgn = sgns[b];
   if (dc == dc && dc != 0.0) {
     double reach = 0.999 * fmax(clamp, fabs(dc));
     int any = 0;
@@ -466,7 +622,9 @@ __global__ __launch_bounds__(WG) void k_bricks(Dev dv, Lv lv, const uint32_t* li
     if (__syncthreads_or(any)) fill();
   }
   // partcache.rs:872-909 what is left is bridged (the mean of the known neighbours held
-  // within their Lipschitz brackets), one node after another
+  // within the exact nodes' Lipschitz brackets), one node after another; a brick all
+  // past the clamp takes its centre's sign (the centre's own, or as carried along the
+  // grid when the decision left it undecided)
   int open = 0;
   for (int n = tid; n < 729; n += WG) open |= (st[n] == 1 || st[n] == 2);
   if (__syncthreads_or(open)) {
@@ -478,8 +636,6 @@ __global__ __launch_bounds__(WG) void k_bricks(Dev dv, Lv lv, const uint32_t* li
         double s = (dc == dc && dc != 0.0) ? dc : csgn < 0 ? -1.0 : 1.0;
         for (int n = 0; n < 729; ++n) val[n] = copysign(clamp, s), st[n] = 3;
       }
-      bool exact[729];
-      for (int n = 0; n < 729; ++n) exact[n] = st[n] == 0;
       for (int pass = 0; pass < 729; ++pass) {
         bool left = false, moved = false;
         for (int n = 0; n < 729; ++n) {
@@ -490,7 +646,7 @@ __global__ __launch_bounds__(WG) void k_bricks(Dev dv, Lv lv, const uint32_t* li
             int m = nb26(n, k);
             if (m < 0) continue;
             if (st[m] == 0 || st[m] == 3 || st[m] == 4) sum += val[m], ++cnt;
-            if (exact[m]) lo_ = fmax(lo_, val[m] - nb26_len(k) * h), hi_ = fmin(hi_, val[m] + nb26_len(k) * h);
+            if (st[m] == 0) lo_ = fmax(lo_, val[m] - nb26_len(k) * h), hi_ = fmin(hi_, val[m] + nb26_len(k) * h);
           }
           if (!cnt) {
             left = true;
@@ -514,30 +670,20 @@ __global__ __launch_bounds__(WG) void k_bricks(Dev dv, Lv lv, const uint32_t* li
   }
   __syncthreads();
   // partcache.rs:911-915, 962-970 quantize, class, two values to a word (low half first)
-  __shared__ int16_t q[730];
-  int pos = 0, nonpos = 0;
+  int any_pos = 0, any_nonpos = 0;
   for (int n = tid; n < 729; n += WG) {
     double x = fmin(fmax(val[n] / clamp, -1.0), 1.0) * 32767.0;
     int16_t qq = int16_t(round(x));
     q[n] = qq;
-    pos |= qq <= 0, nonpos |= qq > 0;
+    any_nonpos |= qq <= 0, any_pos |= qq > 0;
   }
   if (tid == 0) q[729] = 0;
-  int any_nonpos = __syncthreads_or(pos), any_pos = __syncthreads_or(nonpos);
+  any_nonpos = __syncthreads_or(any_nonpos);
+  any_pos = __syncthreads_or(any_pos);
   uint32_t c = !any_nonpos ? 1u : !any_pos ? 2u : 0u;  // 1 AIR, 2 SOLID, 0 words
   if (tid == 0) cls[bi] = c;
   if (c == 0)
     for (int w = tid; w < ICAM_WORDS; w += WG) staging[size_t(bi) * ICAM_WORDS + w] = uint32_t(uint16_t(q[2 * w])) | (uint32_t(uint16_t(q[2 * w + 1])) << 16);
-  if (MIXED) {
-    // reduce the refined count (one atomic a block)
-    red[tid] = float(refined);
-    __syncthreads();
-    for (int s = WG / 2; s > 0; s >>= 1) {
-      if (tid < s) red[tid] += red[tid + s];
-      __syncthreads();
-    }
-    if (tid == 0) atomicAdd(&counters[1], (unsigned long long)red[0]);
-  }
 }
 
 __global__ void k_flags_words(const uint32_t* cls, long n, uint8_t* f) {
@@ -546,11 +692,15 @@ __global__ void k_flags_words(const uint32_t* cls, long n, uint8_t* f) {
 }
 
 // partcache.rs:128-146 assemble: the map (AIR, SOLID, or the slot) and the atlas in
-// map order (band bricks are listed in map order, so their slots are too).
-__global__ void k_store(long nbricks, const uint8_t* band, const int8_t* sgn, uint32_t* map) {
+// map order (band bricks are listed in map order, so their slots are too). A brick
+// kept whole whose sign was never found is counted (counters[2]).
+__global__ void k_store(long nbricks, const uint8_t* band, const int8_t* sgn, uint32_t* map, unsigned long long* counters) {
   long b = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= nbricks) return;
-  if (!band[b]) map[b] = sgn[b] < 0 ? ICAM_SOLID : ICAM_AIR;
+  if (!band[b]) {
+    map[b] = sgn[b] < 0 ? ICAM_SOLID : ICAM_AIR;
+    if (sgn[b] == 0) atomicAdd(&counters[2], 1ull);
+  }
 }
 __global__ void k_store_band(long nband, const uint32_t* list, const uint32_t* cls, const uint32_t* slot, const uint32_t* staging, uint32_t* map,
                              uint32_t* atlas) {
@@ -598,20 +748,25 @@ struct icam_hip {
   Packed pk;
   Dev dv{};
   Buf F32, F64, U, pbox, pcode;
-  // scratch of queries and bakes
+  // per-point results of the centres' and the nodes' searches, and host-call staging
+  struct ResBufs {
+    Buf code, pa, pb, d32, info, val, flag, queue, qcount, qmark;
+    void free() {
+      for (Buf* b : {&code, &pa, &pb, &d32, &info, &val, &flag, &queue, &qcount, &qmark}) b->free();
+    }
+  } centres, nodes;
   Buf q_xyz, q_bound, q_d, q_flag, q_feat;
-  Buf dc, band, sgn, pos, total, list, staging, cls, wflags, slot, stotal, map, atlas, counters, changed, firsts, nodes_out;
+  // the level
+  Buf dc, band, sgn, pos, total, list, staging, cls, wflags, slot, stotal, map, atlas, counters, changed, firsts;
   std::vector<std::pair<const char*, double>> stages;
   std::string name;
 };
 
 namespace {
 
-void stage_reset(icam_hip* h) { h->stages.clear(); }
-
 struct Timer {
   hipEvent_t a, b;
-  Timer(hipStream_t s) {
+  explicit Timer(hipStream_t s) {
     hipEventCreate(&a);
     hipEventCreate(&b);
     hipEventRecord(a, s);
@@ -627,17 +782,84 @@ struct Timer {
   }
 };
 
+// Per-stage HIP events, on the stream the kernels run on.
+struct Stages {
+  std::vector<Timer> tm;
+  std::vector<const char*> names;
+};
+
 template <class K, class... A>
-hipError_t launch(icam_hip* h, const char* name, hipStream_t s, dim3 g, dim3 b, std::vector<Timer>* tm, std::vector<const char*>* names, K k, A... args) {
-  if (tm) tm->emplace_back(s);
+hipError_t launch(Stages* st, const char* name, hipStream_t s, dim3 g, dim3 b, K k, A... args) {
+  if (st) st->tm.emplace_back(s);
   hipLaunchKernelGGL(k, g, b, 0, s, args...);
-  if (tm) tm->back().stop(s), names->push_back(name);
+  if (st) st->tm.back().stop(s), st->names.push_back(name);
   return hipGetLastError();
+}
+
+void collect(icam_hip* h, Stages& st) {
+  h->stages.clear();
+  for (size_t i = 0; i < st.tm.size(); ++i) {
+    double m = st.tm[i].ms();
+    bool found = false;
+    for (auto& x : h->stages)
+      if (std::strcmp(x.first, st.names[i]) == 0) x.second += m, found = true;
+    if (!found) h->stages.push_back({st.names[i], m});
+  }
 }
 
 int upload(Buf& b, const void* src, size_t bytes) {
   HIP_OK(b.need(std::max<size_t>(bytes, 16)));
   if (bytes) HIP_OK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Result buffers for n points from one of the handle's two sets (centres or nodes);
+// val, flag and feat may be the caller's.
+int results(icam_hip* h, bool nodes, long n, Res& r, double* val, uint32_t* flag, uint32_t* feat, bool mixed) {
+  size_t m = size_t(std::max<long>(n, 1));
+  icam_hip::ResBufs& B = nodes ? h->nodes : h->centres;
+  Buf &code = B.code, &pa = B.pa, &pb = B.pb, &d32 = B.d32, &info = B.info, &bval = B.val, &bflag = B.flag, &queue = B.queue, &qcount = B.qcount;
+  if (mixed) {
+    HIP_OK(code.need(m * 4));
+    HIP_OK(pa.need(m * 4));
+    HIP_OK(pb.need(m * 4));
+    HIP_OK(d32.need(m * 4));
+    HIP_OK(info.need(m * 4));
+    HIP_OK(queue.need(m * 4));
+    HIP_OK(B.qmark.need(m * 4));
+  }
+  HIP_OK(qcount.need(16));
+  HIP_OK(h->counters.need(64));
+  if (!val) HIP_OK(bval.need(m * 8));
+  if (!flag) HIP_OK(bflag.need(m * 4));
+  r.code = code.as<uint32_t>(), r.pa = pa.as<float>(), r.pb = pb.as<float>(), r.d32 = d32.as<float>();
+  r.info = info.as<uint32_t>(), r.queue = queue.as<uint32_t>(), r.qcount = qcount.as<uint32_t>(), r.qmark = B.qmark.as<uint32_t>();
+  r.val = val ? val : bval.as<double>();
+  r.flag = flag ? flag : bflag.as<uint32_t>();
+  r.feat = feat;
+  r.counters = h->counters.as<unsigned long long>();
+  return 0;
+}
+
+// Runs the search pipeline for a job of source S: single precision, the last
+// correction and the fallback (mixed), or double precision (fp64).
+template <int S>
+int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Stages* st, long blocks) {
+  if (j.n <= 0) return 0;
+  constexpr int NT = Shape<S>::NT;
+  static const char* names[3][4] = {{"points_search32", "points_refine64", "points_fallback64", "points_search64"},
+                                    {"nodes_search32", "nodes_refine64", "nodes_fallback64", "nodes_search64"},
+                                    {"centres_search32", "centres_refine64", "centres_fallback64", "centres_search64"}};
+  if (mixed) {
+    HIP_OK(hipMemsetAsync(r.qcount, 0, 4, s));
+    HIP_OK(hipMemsetAsync(r.qmark, 0, size_t(blocks) * 4, s));
+    HIP_OK(launch(st, names[S][0], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, float>, h->dv, j, r));
+    HIP_OK(launch(st, names[S][1], s, dim3(unsigned((j.n + 255) / 256)), dim3(256), k_refine<S>, h->dv, j, r));
+    unsigned fb = unsigned(std::min<long>(blocks, 2048));
+    HIP_OK(launch(st, names[S][2], s, dim3(fb), dim3(NT), k_fallback<S>, h->dv, j, r));
+  } else {
+    HIP_OK(launch(st, names[S][3], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, double>, h->dv, j, r));
+  }
   return 0;
 }
 
@@ -657,7 +879,7 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
   HIP_OK(hipGetDevice(&h->device));
   hipDeviceProp_t prop;
   HIP_OK(hipGetDeviceProperties(&prop, h->device));
-  h->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+  h->name = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ", " + std::to_string(prop.multiProcessorCount) + " CUs)";
   if (!pack_part(part, need, h->pk)) {
     std::string e = h->pk.error;
     delete h;
@@ -686,9 +908,10 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
 void icam_hip_close(icam_hip* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
-  for (Buf* b : {&h->F32, &h->F64, &h->U, &h->pbox, &h->pcode, &h->q_xyz, &h->q_bound, &h->q_d, &h->q_flag, &h->q_feat, &h->dc, &h->band, &h->sgn,
-                 &h->pos, &h->total, &h->list, &h->staging, &h->cls, &h->wflags, &h->slot, &h->stotal, &h->map, &h->atlas, &h->counters,
-                 &h->changed, &h->firsts, &h->nodes_out})
+  h->centres.free();
+  h->nodes.free();
+  for (Buf* b : {&h->F32, &h->F64, &h->U, &h->pbox, &h->pcode, &h->q_xyz, &h->q_bound, &h->q_d, &h->q_flag, &h->q_feat, &h->dc, &h->band, &h->sgn, &h->pos, &h->total,
+                 &h->list, &h->staging, &h->cls, &h->wflags, &h->slot, &h->stotal, &h->map, &h->atlas, &h->counters, &h->changed, &h->firsts})
     b->free();
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -717,17 +940,22 @@ int icam_hip_warm(icam_hip* h, double secs) {
   return 0;
 }
 
+static int query_impl(icam_hip* h, int precision, const double* d_xyz, const double* d_bounds, size_t n, double* d_d, uint32_t* d_flags,
+                      uint32_t* d_feat, hipStream_t s, Stages* st) {
+  if (!n) return 0;
+  if (precision != ICAM_FP64 && precision != ICAM_MIXED) return fail("unknown precision");
+  Res r{};
+  if (results(h, true, long(n), r, d_d, d_flags, d_feat, precision == ICAM_MIXED)) return 1;
+  Job j{};
+  j.n = long(n), j.xyz = d_xyz, j.bounds = d_bounds;
+  return run_search<SRC_POINTS>(h, precision == ICAM_MIXED, j, r, s, st, (long(n) + 255) / 256);
+}
+
 int icam_hip_query_device(icam_hip* h, int precision, const double* d_xyz, size_t n, double* d_d, uint32_t* d_flags, uint32_t* d_feat, void* stream) {
   if (!h) return fail("no handle");
-  if (!n) return 0;
+  std::lock_guard<std::mutex> g(h->mu);
   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-  dim3 g(unsigned((n + WG - 1) / WG));
-  if (precision == ICAM_MIXED)
-    hipLaunchKernelGGL(k_points<true>, g, dim3(WG), 0, s, h->dv, d_xyz, (const double*)nullptr, n, d_d, d_flags, d_feat);
-  else
-    hipLaunchKernelGGL(k_points<false>, g, dim3(WG), 0, s, h->dv, d_xyz, (const double*)nullptr, n, d_d, d_flags, d_feat);
-  HIP_OK(hipGetLastError());
-  return 0;
+  return query_impl(h, precision, d_xyz, nullptr, n, d_d, d_flags, d_feat, s, nullptr);
 }
 
 int icam_hip_query(icam_hip* h, int precision, const double* xyz, size_t n, double* d, uint32_t* flags, uint32_t* feat) {
@@ -739,16 +967,15 @@ int icam_hip_query(icam_hip* h, int precision, const double* xyz, size_t n, doub
   HIP_OK(h->q_flag.need(n * 4));
   HIP_OK(h->q_feat.need(n * 4));
   HIP_OK(hipMemcpyAsync(h->q_xyz.p, xyz, n * 24, hipMemcpyHostToDevice, h->stream));
-  Timer t(h->stream);
-  if (icam_hip_query_device(h, precision, h->q_xyz.as<double>(), n, h->q_d.as<double>(), h->q_flag.as<uint32_t>(), h->q_feat.as<uint32_t>(), h->stream))
+  Stages st;
+  if (query_impl(h, precision, h->q_xyz.as<double>(), nullptr, n, h->q_d.as<double>(), h->q_flag.as<uint32_t>(), h->q_feat.as<uint32_t>(), h->stream,
+                 &st))
     return 1;
-  t.stop(h->stream);
   HIP_OK(hipMemcpyAsync(d, h->q_d.p, n * 8, hipMemcpyDeviceToHost, h->stream));
   if (flags) HIP_OK(hipMemcpyAsync(flags, h->q_flag.p, n * 4, hipMemcpyDeviceToHost, h->stream));
   if (feat) HIP_OK(hipMemcpyAsync(feat, h->q_feat.p, n * 4, hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
-  stage_reset(h);
-  h->stages.push_back({"points", t.ms()});
+  collect(h, st);
   return 0;
 }
 
@@ -767,14 +994,9 @@ int icam_hip_points(icam_hip* h, int precision, const icam_point_rec* recs, size
   HIP_OK(h->q_flag.need(n * 4));
   HIP_OK(hipMemcpyAsync(h->q_xyz.p, xyz.data(), n * 24, hipMemcpyHostToDevice, h->stream));
   HIP_OK(hipMemcpyAsync(h->q_bound.p, bd.data(), n * 8, hipMemcpyHostToDevice, h->stream));
-  dim3 gr(unsigned((n + WG - 1) / WG));
-  if (precision == ICAM_MIXED)
-    hipLaunchKernelGGL(k_points<true>, gr, dim3(WG), 0, h->stream, h->dv, h->q_xyz.as<double>(), h->q_bound.as<double>(), n, h->q_d.as<double>(),
-                       h->q_flag.as<uint32_t>(), (uint32_t*)nullptr);
-  else
-    hipLaunchKernelGGL(k_points<false>, gr, dim3(WG), 0, h->stream, h->dv, h->q_xyz.as<double>(), h->q_bound.as<double>(), n, h->q_d.as<double>(),
-                       h->q_flag.as<uint32_t>(), (uint32_t*)nullptr);
-  HIP_OK(hipGetLastError());
+  if (query_impl(h, precision, h->q_xyz.as<double>(), h->q_bound.as<double>(), n, h->q_d.as<double>(), h->q_flag.as<uint32_t>(), nullptr,
+                 h->stream, nullptr))
+    return 1;
   std::vector<double> d(n);
   std::vector<uint32_t> f(n);
   HIP_OK(hipMemcpyAsync(d.data(), h->q_d.p, n * 8, hipMemcpyDeviceToHost, h->stream));
@@ -788,29 +1010,31 @@ int icam_hip_bricks(icam_hip* h, int precision, const icam_brick_rec* recs, size
   if (!h) return fail("no handle");
   std::lock_guard<std::mutex> g(h->mu);
   if (!n) return 0;
+  if (precision != ICAM_FP64 && precision != ICAM_MIXED) return fail("unknown precision");
   HIP_OK(h->firsts.need(n * 32));
-  HIP_OK(h->nodes_out.need(n * 729 * 16));
   HIP_OK(hipMemcpyAsync(h->firsts.p, recs, n * 32, hipMemcpyHostToDevice, h->stream));
-  Lv lv{};
-  if (precision == ICAM_MIXED)
-    hipLaunchKernelGGL((k_bricks<true, MODE_NODES>), dim3(unsigned(n)), dim3(WG), 0, h->stream, h->dv, lv, (const uint32_t*)nullptr,
-                       h->firsts.as<double>(), long(n), (const double*)nullptr, (const int8_t*)nullptr, h->nodes_out.as<double>(), (uint32_t*)nullptr,
-                       (uint32_t*)nullptr, (unsigned long long*)nullptr);
-  else
-    hipLaunchKernelGGL((k_bricks<false, MODE_NODES>), dim3(unsigned(n)), dim3(WG), 0, h->stream, h->dv, lv, (const uint32_t*)nullptr,
-                       h->firsts.as<double>(), long(n), (const double*)nullptr, (const int8_t*)nullptr, h->nodes_out.as<double>(), (uint32_t*)nullptr,
-                       (uint32_t*)nullptr, (unsigned long long*)nullptr);
-  HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpyAsync(out, h->nodes_out.p, n * 729 * 16, hipMemcpyDeviceToHost, h->stream));
+  Res r{};
+  long np = long(n) * 729;
+  if (results(h, true, np, r, nullptr, nullptr, nullptr, precision == ICAM_MIXED)) return 1;
+  Job j{};
+  j.n = np, j.firsts = h->firsts.as<double>();
+  if (run_search<SRC_NODES>(h, precision == ICAM_MIXED, j, r, h->stream, nullptr, long(n))) return 1;
+  std::vector<double> v(np);
+  std::vector<uint32_t> f(np);
+  HIP_OK(hipMemcpyAsync(v.data(), r.val, np * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipMemcpyAsync(f.data(), r.flag, np * 4, hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
+  for (long i = 0; i < np; ++i) out[2 * i] = v[i], out[2 * i + 1] = double(f[i]);
   return 0;
 }
 
-static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStream_t s, icam_bake_stats* stats, bool timed) {
+static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStream_t s, icam_bake_stats* stats) {
   Lv lv;
   for (int k = 0; k < 3; ++k) lv.lo[k] = lvp->lo[k], lv.nb[k] = lvp->nb[k];
   lv.h = lvp->h;
   if (!(lv.h > 0) || lv.nb[0] <= 0 || lv.nb[1] <= 0 || lv.nb[2] <= 0) return fail("bad level");
+  if (precision != ICAM_FP64 && precision != ICAM_MIXED) return fail("unknown precision");
+  const bool mixed = precision == ICAM_MIXED;
   long nb = long(lv.nb[0]) * lv.nb[1] * lv.nb[2];
   HIP_OK(h->dc.need(nb * 8));
   HIP_OK(h->band.need(nb));
@@ -821,24 +1045,24 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   HIP_OK(h->map.need(nb * 4));
   HIP_OK(h->counters.need(64));
   HIP_OK(h->changed.need(16));
-  HIP_OK(hipMemsetAsync(h->counters.p, 0, 64, s));
-  std::vector<Timer> tm;
-  std::vector<const char*> names;
-  std::vector<Timer>* T = timed ? &tm : nullptr;
-  auto* N = timed ? &names : nullptr;
-  bool mixed = precision == ICAM_MIXED;
+  Stages st;
   dim3 gb(unsigned((nb + WG - 1) / WG));
-  dim3 gt(unsigned(long((lv.nb[0] + 3) / 4) * ((lv.nb[1] + 3) / 4) * ((lv.nb[2] + 3) / 4)));
-  if (mixed)
-    HIP_OK(launch(h, "decide", s, gt, dim3(DT), T, N, k_decide<true>, h->dv, lv, nb, h->dc.as<double>(), h->band.as<uint8_t>(), h->sgn.as<int8_t>()));
-  else
-    HIP_OK(launch(h, "decide", s, gt, dim3(DT), T, N, k_decide<false>, h->dv, lv, nb, h->dc.as<double>(), h->band.as<uint8_t>(), h->sgn.as<int8_t>()));
+  // the brick centres
+  Res r{};
+  if (results(h, false, nb, r, nullptr, nullptr, nullptr, mixed)) return 1;
+  HIP_OK(hipMemsetAsync(h->counters.p, 0, 64, s));
+  Job jc{};
+  jc.n = nb, jc.lv = lv;
+  long tiles = long((lv.nb[0] + 3) / 4) * ((lv.nb[1] + 3) / 4) * ((lv.nb[2] + 3) / 4);
+  if (run_search<SRC_CENTRES>(h, mixed, jc, r, s, &st, tiles)) return 1;
+  HIP_OK(launch(&st, "decide", s, gb, dim3(WG), k_decide_finish, h->dv, lv, nb, (const double*)r.val, (const uint32_t*)r.flag, h->dc.as<double>(),
+                h->band.as<uint8_t>(), h->sgn.as<int8_t>()));
   // signs of the bricks kept whole: along x, y, z lines, until nothing changes
   for (int round = 0; round < 8; ++round) {
     HIP_OK(hipMemsetAsync(h->changed.p, 0, 4, s));
     for (int ax = 0; ax < 3; ++ax) {
       long lines = long(lv.nb[(ax + 1) % 3]) * lv.nb[(ax + 2) % 3];
-      HIP_OK(launch(h, "sign", s, dim3(unsigned((lines + 255) / 256)), dim3(256), T, N, k_sign, lv, ax, (const uint8_t*)h->band.as<uint8_t>(),
+      HIP_OK(launch(&st, "sign", s, dim3(unsigned((lines + 255) / 256)), dim3(256), k_sign, lv, ax, (const uint8_t*)h->band.as<uint8_t>(),
                     h->sgn.as<int8_t>(), h->changed.as<int>()));
     }
     int ch = 0;
@@ -846,9 +1070,9 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     HIP_OK(hipStreamSynchronize(s));
     if (!ch) break;
   }
-  HIP_OK(launch(h, "scan", s, dim3(1), dim3(1024), T, N, k_scan, (const uint8_t*)h->band.as<uint8_t>(), nb, h->pos.as<uint32_t>(),
+  HIP_OK(launch(&st, "scan", s, dim3(1), dim3(1024), k_scan, (const uint8_t*)h->band.as<uint8_t>(), nb, h->pos.as<uint32_t>(),
                 h->total.as<uint32_t>()));
-  HIP_OK(launch(h, "band_list", s, gb, dim3(WG), T, N, k_band_list, (const uint8_t*)h->band.as<uint8_t>(), (const uint32_t*)h->pos.as<uint32_t>(), nb,
+  HIP_OK(launch(&st, "band_list", s, gb, dim3(WG), k_band_list, (const uint8_t*)h->band.as<uint8_t>(), (const uint32_t*)h->pos.as<uint32_t>(), nb,
                 h->list.as<uint32_t>()));
   uint32_t nband = 0;
   HIP_OK(hipMemcpyAsync(&nband, h->total.p, 4, hipMemcpyDeviceToHost, s));
@@ -859,19 +1083,19 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   HIP_OK(h->slot.need(std::max<size_t>(size_t(nband) * 4, 16)));
   HIP_OK(h->stotal.need(16));
   if (nband) {
-    if (mixed)
-      HIP_OK(launch(h, "bricks", s, dim3(nband), dim3(WG), T, N, k_bricks<true, MODE_BAKE>, h->dv, lv, (const uint32_t*)h->list.as<uint32_t>(),
-                    (const double*)nullptr, long(nband), (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>(), (double*)nullptr,
-                    h->staging.as<uint32_t>(),
-                    h->cls.as<uint32_t>(), h->counters.as<unsigned long long>()));
-    else
-      HIP_OK(launch(h, "bricks", s, dim3(nband), dim3(WG), T, N, k_bricks<false, MODE_BAKE>, h->dv, lv, (const uint32_t*)h->list.as<uint32_t>(),
-                    (const double*)nullptr, long(nband), (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>(), (double*)nullptr,
-                    h->staging.as<uint32_t>(),
-                    h->cls.as<uint32_t>(), h->counters.as<unsigned long long>()));
-    HIP_OK(launch(h, "word_flags", s, dim3(unsigned((nband + 255) / 256)), dim3(256), T, N, k_flags_words, (const uint32_t*)h->cls.as<uint32_t>(),
+    // the band bricks' nodes
+    Res rn{};
+    long np = long(nband) * 729;
+    if (results(h, true, np, rn, nullptr, nullptr, nullptr, mixed)) return 1;
+    Job jn{};
+    jn.n = np, jn.list = h->list.as<uint32_t>(), jn.lv = lv;
+    if (run_search<SRC_NODES>(h, mixed, jn, rn, s, &st, long(nband))) return 1;
+    HIP_OK(launch(&st, "bricks_finish", s, dim3(nband), dim3(WG), k_bricks_finish, lv, (const uint32_t*)h->list.as<uint32_t>(), long(nband),
+                  (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>(), (const double*)rn.val, (const uint32_t*)rn.flag,
+                  h->staging.as<uint32_t>(), h->cls.as<uint32_t>(), h->counters.as<unsigned long long>()));
+    HIP_OK(launch(&st, "word_flags", s, dim3(unsigned((nband + 255) / 256)), dim3(256), k_flags_words, (const uint32_t*)h->cls.as<uint32_t>(),
                   long(nband), h->wflags.as<uint8_t>()));
-    HIP_OK(launch(h, "scan", s, dim3(1), dim3(1024), T, N, k_scan, (const uint8_t*)h->wflags.as<uint8_t>(), long(nband), h->slot.as<uint32_t>(),
+    HIP_OK(launch(&st, "scan", s, dim3(1), dim3(1024), k_scan, (const uint8_t*)h->wflags.as<uint8_t>(), long(nband), h->slot.as<uint32_t>(),
                   h->stotal.as<uint32_t>()));
   } else {
     HIP_OK(hipMemsetAsync(h->stotal.p, 0, 4, s));
@@ -880,30 +1104,21 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   HIP_OK(hipMemcpyAsync(&nslots, h->stotal.p, 4, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   HIP_OK(h->atlas.need(std::max<size_t>(size_t(nslots) * ICAM_WORDS * 4, 16)));
-  HIP_OK(launch(h, "store", s, gb, dim3(WG), T, N, k_store, nb, (const uint8_t*)h->band.as<uint8_t>(), (const int8_t*)h->sgn.as<int8_t>(),
-                h->map.as<uint32_t>()));
+  HIP_OK(launch(&st, "store", s, gb, dim3(WG), k_store, nb, (const uint8_t*)h->band.as<uint8_t>(), (const int8_t*)h->sgn.as<int8_t>(),
+                h->map.as<uint32_t>(), h->counters.as<unsigned long long>()));
   if (nband)
-    HIP_OK(launch(h, "store_band", s, dim3(nband), dim3(128), T, N, k_store_band, long(nband), (const uint32_t*)h->list.as<uint32_t>(),
+    HIP_OK(launch(&st, "store_band", s, dim3(nband), dim3(128), k_store_band, long(nband), (const uint32_t*)h->list.as<uint32_t>(),
                   (const uint32_t*)h->cls.as<uint32_t>(), (const uint32_t*)h->slot.as<uint32_t>(), (const uint32_t*)h->staging.as<uint32_t>(),
                   h->map.as<uint32_t>(), h->atlas.as<uint32_t>()));
-  if (stats || timed) {
-    unsigned long long cnt[2] = {0, 0};
-    HIP_OK(hipMemcpyAsync(cnt, h->counters.p, 16, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    if (stats) {
-      stats->bricks = nb, stats->band = nband, stats->slots = nslots;
-      stats->undecided = int64_t(cnt[0]), stats->refined = int64_t(cnt[1]);
-    }
-  }
-  if (timed) {
-    stage_reset(h);
-    for (size_t i = 0; i < tm.size(); ++i) {
-      double m = tm[i].ms();
-      bool found = false;
-      for (auto& st : h->stages)
-        if (std::strcmp(st.first, names[i]) == 0) st.second += m, found = true;
-      if (!found) h->stages.push_back({names[i], m});
-    }
+  unsigned long long cnt[5] = {0, 0, 0, 0, 0};
+  HIP_OK(hipMemcpyAsync(cnt, h->counters.p, 40, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  collect(h, st);
+  if (cnt[2]) return fail(std::to_string(cnt[2]) + " bricks kept whole with no sign found (no surface near them and none outside the part's box)");
+  if (stats) {
+    stats->bricks = nb, stats->band = nband, stats->slots = nslots;
+    stats->undecided = int64_t(cnt[0]), stats->refined = int64_t(cnt[1]);
+    stats->unsure_trim = int64_t(cnt[3]), stats->unsure_tie = int64_t(cnt[4]);
   }
   return 0;
 }
@@ -912,7 +1127,8 @@ int icam_hip_bake_device(icam_hip* h, int precision, const icam_level* lv, uint3
   if (!h || !lv) return fail("bad arguments");
   std::lock_guard<std::mutex> g(h->mu);
   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-  if (bake_impl(h, precision, lv, s, stats, true)) return 1;
+  icam_bake_stats tmp{};
+  if (bake_impl(h, precision, lv, s, stats ? stats : &tmp)) return 1;
   if (d_map) *d_map = h->map.as<uint32_t>();
   if (d_atlas) *d_atlas = h->atlas.as<uint32_t>();
   return 0;
@@ -922,7 +1138,7 @@ int icam_hip_bake(icam_hip* h, int precision, const icam_level* lv, uint32_t* ma
   if (!h || !lv) return fail("bad arguments");
   std::lock_guard<std::mutex> g(h->mu);
   icam_bake_stats st{};
-  if (bake_impl(h, precision, lv, h->stream, &st, true)) return 1;
+  if (bake_impl(h, precision, lv, h->stream, &st)) return 1;
   if (stats) *stats = st;
   long nb = long(lv->nb[0]) * lv->nb[1] * lv->nb[2];
   HIP_OK(hipMemcpyAsync(map, h->map.p, nb * 4, hipMemcpyDeviceToHost, h->stream));

@@ -105,7 +105,9 @@ typedef struct icam_bake_stats {
   int64_t band;        /* bricks whose nodes were read (the decision's) */
   int64_t slots;       /* bricks stored with words */
   int64_t undecided;   /* nodes left undecided (bridged) */
-  int64_t refined;     /* nodes whose value took a double-precision step */
+  int64_t refined;     /* points searched again in double precision (ICAM_MIXED) */
+  int64_t unsure_trim; /* of them, for a trim decision taken near a trim curve */
+  int64_t unsure_tie;  /* of them, for a near-tie of opposite signs */
 } icam_bake_stats;
 
 /* PartCache::build (partcache.rs:458-486) on the device: the level's brick map
