@@ -232,15 +232,49 @@ __device__ void block_box(float lo[3], float hi[3], float& r, float* red) {
 
 // ------------------------------------------------------------------ one lane's search
 
+// One lane's search over the block's candidates (or the whole list when they did not
+// fit): the closed-form features (faces, edges, vertices) in list order, then the
+// B-spline sub-patches and curve parts nearest first by their box distance, each
+// round the lane's own next one, until none can beat the best answer.
 template <class T>
 __device__ Best<T> search(const Tab<T>& tb, v3<T> p, T bound, const uint32_t* cand, int nc, T tm, T eps) {
   Best<T> b;
   b.d = bound, b.code = NONE, b.a = b.b = T(0), b.sgn = T(1), b.lmin = TypeEps<T>::big, b.d2 = TypeEps<T>::big, b.sgn2 = T(1), b.unsure = false;
-  const uint32_t n = nc <= MAXC ? uint32_t(nc) : tb.L.n_prims;
+  const bool all = nc > MAXC;
+  const uint32_t n = all ? tb.L.n_prims : uint32_t(nc);
+  bool parts = false;
   for (uint32_t k = 0; k < n; ++k) {
-    uint32_t i = nc <= MAXC ? cand[k] : k;
+    uint32_t i = all ? k : cand[k];
     i = __builtin_amdgcn_readfirstlane(i);
-    visit(tb, tb.pcode[i], tb.pbox + 6 * i, p, b, tm, eps);
+    const uint32_t code = tb.pcode[i];
+    if ((code >> 28) >= P_LEAF) {
+      parts = true;
+      continue;
+    }
+    visit(tb, code, tb.pbox + 6 * i, p, b, tm, eps);
+  }
+  if (!parts) return b;
+  T last_lb = T(-1);
+  uint32_t last_k = 0;
+  for (int round = 0; round < 4096; ++round) {
+    T next_lb = b.d;
+    uint32_t next_k = NONE;
+    for (uint32_t k = 0; k < n; ++k) {
+      uint32_t i = all ? k : cand[k];
+      i = __builtin_amdgcn_readfirstlane(i);
+      if ((tb.pcode[i] >> 28) < P_LEAF) continue;
+      T lb = box_dist(p, tb.pbox + 6 * i);
+      bool after = lb > last_lb || (lb == last_lb && k > last_k);
+      if (after && lb < next_lb) next_lb = lb, next_k = k;
+    }
+    if (next_k == NONE) break;
+    const uint32_t i = all ? next_k : cand[next_k];
+    const uint32_t code = tb.pcode[i], id = code & 0x0fffffffu;
+    if ((code >> 28) == P_LEAF)
+      try_leaf(tb, id, tb.pbox + 6 * i, next_lb, p, b, tm, eps);
+    else
+      try_cleaf(tb, id, tb.pbox + 6 * i, next_lb, p, b, eps);
+    last_lb = next_lb, last_k = next_k;
   }
   return b;
 }
