@@ -44,6 +44,7 @@ namespace {
 constexpr int WG = 256;
 constexpr int MAXC = 4096;
 constexpr int SORTC = 1024;  // lists up to this long are sorted nearest first
+constexpr int TCAP = 512;    // a tile's list kept for its bricks' nodes, at most
 constexpr uint32_t NONE = 0xffffffffu;
 
 thread_local std::string g_error;
@@ -104,6 +105,11 @@ struct Job {
   const uint32_t* list;   // SRC_NODES: band bricks of the level (NULL: `firsts`)
   const double* firsts;   // SRC_NODES: explicit bricks (first node xyz, h)
   Lv lv;
+  // the candidate list of each 4 x 4 x 4 tile of bricks: written by the centres'
+  // search (within the reach of the tile's centres), read by the nodes' search (a
+  // brick's nodes' candidates lie within that reach of its centre)
+  uint32_t* tile_cand;    // TCAP per tile
+  uint32_t* tile_count;   // per tile (more than TCAP: not kept)
 };
 
 // Point i of a job (relative to the origin) and the distance within which it must
@@ -164,13 +170,18 @@ __device__ inline float box_gap(const float* a, const float* b) {
 // The primitives whose box comes within R of the block's box gb (lo, hi), in list
 // order, into cand; returns how many (more than MAXC: the list did not fit).
 template <int NT>
-__device__ int cull(const Dev& dv, const float* gb, float R, uint32_t* cand, int* wave_tot) {
+__device__ int cull(const Dev& dv, const float* gb, float R, uint32_t* cand, int* wave_tot, const uint32_t* from = nullptr, uint32_t n_from = 0) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t total = from ? n_from : dv.L.n_prims;
   int base = 0;
-  for (uint32_t c0 = 0; c0 < dv.L.n_prims; c0 += NT) {
-    uint32_t i = c0 + tid;
+  for (uint32_t c0 = 0; c0 < total; c0 += NT) {
+    uint32_t k = c0 + tid;
+    uint32_t i = 0;
     bool keep = false;
-    if (i < dv.L.n_prims) keep = box_gap(dv.pbox + 6 * i, gb) <= R;
+    if (k < total) {
+      i = from ? from[k] : k;
+      keep = box_gap(dv.pbox + 6 * i, gb) <= R;
+    }
     unsigned long long m = __ballot(keep);
     int before = __popcll(m & ((1ull << lane) - 1ull));
     if (lane == 0) wave_tot[wave] = __popcll(m);
@@ -262,8 +273,10 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, T bound, const uint32_t* ca
     for (uint32_t k = 0; k < n; ++k) {
       uint32_t i = all ? k : cand[k];
       i = __builtin_amdgcn_readfirstlane(i);
-      if ((tb.pcode[i] >> 28) < P_LEAF) continue;
+      const uint32_t code = tb.pcode[i];
+      if ((code >> 28) < P_LEAF) continue;
       T lb = box_dist(p, tb.pbox + 6 * i);
+      if ((code >> 28) == P_LEAF) lb = mmax(lb, slab_dist(tb, code & 0x0fffffffu, p, eps));
       bool after = lb > last_lb || (lb == last_lb && k > last_k);
       if (after && lb < next_lb) next_lb = lb, next_k = k;
     }
@@ -371,8 +384,20 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
   }
   block_box<NT>(lo, hi, r, red);
   float gb[6] = {lo[0] - 1e-4f, lo[1] - 1e-4f, lo[2] - 1e-4f, hi[0] + 1e-4f, hi[1] + 1e-4f, hi[2] + 1e-4f};
-  int nc = cull<NT>(dv, gb, r, cand, wave_tot);
+  int nc;
+  if (S == SRC_NODES && j.tile_cand && j.list) {
+    // the brick's tile's list, when it was kept, instead of the whole primitive list
+    long tile = tile_of(j.lv, long(j.list[blk]));
+    uint32_t cnt = j.tile_count[tile];
+    nc = cnt <= uint32_t(TCAP) ? cull<NT>(dv, gb, r, cand, wave_tot, j.tile_cand + size_t(tile) * TCAP, cnt) : cull<NT>(dv, gb, r, cand, wave_tot);
+  } else {
+    nc = cull<NT>(dv, gb, r, cand, wave_tot);
+  }
   sort_cand<NT>(dv, gb, cand, tmp, key, nc);
+  if (S == SRC_CENTRES && j.tile_cand) {
+    for (int k = threadIdx.x; k < nc && k < TCAP; k += NT) j.tile_cand[size_t(blk) * TCAP + k] = cand[k];
+    if (threadIdx.x == 0) j.tile_count[blk] = uint32_t(nc);
+  }
   return nc;
 }
 
@@ -391,7 +416,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res)
   const int nc = block_candidates<S>(dv, j, blockIdx.x, cand, tmp, key, red, wave_tot);
   const Tab<T> tb = tab_of<T>(dv);
   const T eps = sizeof(T) == 4 ? T(dv.L.eps32) : T(dv.L.eps64);
-  const T tm = sizeof(T) == 4 ? T(4e-5 * dv.L.diag) + eps : T(0);
+  const T tm = sizeof(T) == 4 ? T(16) * eps : T(0);
   for (int m = 0; m < rounds<S>(); ++m) {
     long i = block_point<S>(j, blockIdx.x, m);
     if (i < 0) continue;
@@ -791,7 +816,7 @@ struct icam_hip {
   } centres, nodes;
   Buf q_xyz, q_bound, q_d, q_flag, q_feat;
   // the level
-  Buf dc, band, sgn, pos, total, list, staging, cls, wflags, slot, stotal, map, atlas, counters, changed, firsts;
+  Buf dc, band, sgn, pos, total, list, staging, cls, wflags, slot, stotal, map, atlas, counters, changed, firsts, tile_cand, tile_count;
   std::vector<std::pair<const char*, double>> stages;
   std::string name;
 };
@@ -945,7 +970,8 @@ void icam_hip_close(icam_hip* h) {
   h->centres.free();
   h->nodes.free();
   for (Buf* b : {&h->F32, &h->F64, &h->U, &h->pbox, &h->pcode, &h->q_xyz, &h->q_bound, &h->q_d, &h->q_flag, &h->q_feat, &h->dc, &h->band, &h->sgn, &h->pos, &h->total,
-                 &h->list, &h->staging, &h->cls, &h->wflags, &h->slot, &h->stotal, &h->map, &h->atlas, &h->counters, &h->changed, &h->firsts})
+                 &h->list, &h->staging, &h->cls, &h->wflags, &h->slot, &h->stotal, &h->map, &h->atlas, &h->counters, &h->changed, &h->firsts,
+                 &h->tile_cand, &h->tile_count})
     b->free();
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -1088,6 +1114,9 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   Job jc{};
   jc.n = nb, jc.lv = lv;
   long tiles = long((lv.nb[0] + 3) / 4) * ((lv.nb[1] + 3) / 4) * ((lv.nb[2] + 3) / 4);
+  HIP_OK(h->tile_cand.need(size_t(tiles) * TCAP * 4));
+  HIP_OK(h->tile_count.need(size_t(tiles) * 4));
+  jc.tile_cand = h->tile_cand.as<uint32_t>(), jc.tile_count = h->tile_count.as<uint32_t>();
   if (run_search<SRC_CENTRES>(h, mixed, jc, r, s, &st, tiles)) return 1;
   HIP_OK(launch(&st, "decide", s, gb, dim3(WG), k_decide_finish, h->dv, lv, nb, (const double*)r.val, (const uint32_t*)r.flag, h->dc.as<double>(),
                 h->band.as<uint8_t>(), h->sgn.as<int8_t>()));
@@ -1123,6 +1152,7 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     if (results(h, true, np, rn, nullptr, nullptr, nullptr, mixed)) return 1;
     Job jn{};
     jn.n = np, jn.list = h->list.as<uint32_t>(), jn.lv = lv;
+    jn.tile_cand = h->tile_cand.as<uint32_t>(), jn.tile_count = h->tile_count.as<uint32_t>();
     if (run_search<SRC_NODES>(h, mixed, jn, rn, s, &st, long(nband))) return 1;
     HIP_OK(launch(&st, "bricks_finish", s, dim3(nband), dim3(WG), k_bricks_finish, lv, (const uint32_t*)h->list.as<uint32_t>(), long(nband),
                   (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>(), (const double*)rn.val, (const uint32_t*)rn.flag,

@@ -29,8 +29,9 @@ enum : uint32_t {
 enum : uint32_t { EDGE_F = 16, EDGE_U = 8 };  // U: kind 0, face0 1, face1 2, uvmap offset 3 (F), first cpiece 4, n 5
 // VERT: p 0-2, pn 3-5
 enum : uint32_t { VERT_F = 8 };
-// LEAF (sub-patch of a span): rho 0, s0 1, s1 2, t0 3, t1 4; U: span 0, face 1, known inside 2
-enum : uint32_t { LEAF_F = 8, LEAF_U = 4 };
+// LEAF (sub-patch of a span): rho 0, s0 1, s1 2, t0 3, t1 4, slab normal 5-7, slab lo 8, hi 9
+// (the sub-patch lies between the planes n.x = lo and n.x = hi); U: span 0, face 1, known inside 2
+enum : uint32_t { LEAF_F = 12, LEAF_U = 4 };
 // SPAN: 16 homogeneous points (i along u, 4 words each) 0-63, u0 64, u1 65, v0 66, v1 67;
 // U: face 0, trimmed 1 (0 whole surface), root cell 2
 enum : uint32_t { SPAN_F = 72, SPAN_U = 4 };

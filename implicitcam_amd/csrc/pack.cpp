@@ -776,8 +776,28 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
               }
               continue;
             }
+            // the slab between two planes normal to the sub-patch's middle normal that
+            // holds its control points (the patch lies in their hull)
+            V p0, ps0, pt0;
+            eval_patch(sb.p, 0.5, 0.5, p0, ps0, pt0);
+            V nn = cross(ps0, pt0);
+            double nl = len(nn);
+            double slo = -INF, shi = INF;
+            if (nl > 0) {
+              nn = nn * (1 / nl);
+              slo = INF, shi = -INF;
+              for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) {
+                  double x = dot(point(sb.p.p[i][j]), nn);
+                  slo = std::min(slo, x), shi = std::max(shi, x);
+                }
+              double pad = 1e-9 * (std::fabs(slo) + std::fabs(shi) + bb.diag());
+              slo -= pad, shi += pad;
+            } else {
+              nn = mk(0, 0, 1);
+            }
             uint32_t id = uint32_t(leaf_f.size() / LEAF_F);
-            leaf_f.insert(leaf_f.end(), {rho, sb.s0, sb.s1, sb.t0, sb.t1, 0, 0, 0});
+            leaf_f.insert(leaf_f.end(), {rho, sb.s0, sb.s1, sb.t0, sb.t1, nn.x, nn.y, nn.z, slo, shi, 0, 0});
             leaf_u.insert(leaf_u.end(), {sid, fi, sb.inside == 1 ? 1u : 0u, 0});
             pbox.push_back(bb);
             pcode.push_back((P_LEAF << 28) | id);

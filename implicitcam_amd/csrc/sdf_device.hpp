@@ -634,7 +634,7 @@ template <class T> __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const
   T s0 = lf[1], s1 = lf[2], t0 = lf[3], t1 = lf[4];
   T s = T(0.5) * (s0 + s1), t = T(0.5) * (t0 + t1);
   const bool single = sizeof(T) == 4;
-  newton2(sp, p, s, t, s0, s1, t0, t1, single ? 12 : 32, single ? T(1e-7) : T(1e-14));
+  newton2(sp, p, s, t, s0, s1, t0, t1, single ? 8 : 32, single ? T(1e-5) : T(1e-14));
   SEval<T> e = peval(sp, s, t);
   v3<T> r = e.p - p;
   T d = len(r);
@@ -671,7 +671,7 @@ template <class T> __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, cons
   const uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cp;
   T t0 = cf[1], t1 = cf[2];
   const bool single = sizeof(T) == 4;
-  T s = newton1(pw, p, T(0.5) * (t0 + t1), t0, t1, single ? 16 : 40, single ? T(1e-7) : T(1e-15));
+  T s = newton1(pw, p, T(0.5) * (t0 + t1), t0, t1, single ? 10 : 40, single ? T(1e-5) : T(1e-15));
   CEval<T> e = ceval(pw, s);
   v3<T> r = e.p - p;
   T d = len(r);
@@ -684,6 +684,16 @@ template <class T> __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, cons
   v3<T> pn = edge_pn(tb, pu[0], e.p, pu[1], s);
   if (len(pn) < T(1e-6)) b.unsure = true;
   offer(b, d, (P_CLEAF << 28) | ci, s, T(0), sgn_of(dot(p - e.p, pn)));
+}
+
+// A lower bound on p's distance to a sub-patch from the slab between the two planes
+// normal to its middle normal that hold its control points (surface3.rs:973-975 takes
+// the same support-plane bound along the direction to a Newton point), less a
+// rounding allowance.
+template <class T> __device__ inline T slab_dist(const Tab<T>& tb, uint32_t li, v3<T> p, T eps) {
+  const T* lf = tb.F + tb.L.leaf_f + LEAF_F * li;
+  T x = dot(p, ld3(lf + 5));
+  return mmax(mmax(x - lf[9], lf[8] - x) - eps, T(0));
 }
 
 // One candidate primitive for one lane.
