@@ -71,8 +71,12 @@ struct Dev {
 };
 
 template <class T> __device__ inline Tab<T> tab_of(const Dev& d);
-template <> __device__ inline Tab<float> tab_of<float>(const Dev& d) { return Tab<float>{d.F32, d.U, d.pbox, d.pcode, d.L}; }
-template <> __device__ inline Tab<double> tab_of<double>(const Dev& d) { return Tab<double>{d.F64, d.U, d.pbox, d.pcode, d.L}; }
+template <> __device__ inline Tab<float> tab_of<float>(const Dev& d) {
+  return Tab<float>{(const ICAM_K float*)d.F32, (const ICAM_K uint32_t*)d.U, (const ICAM_K float*)d.pbox, (const ICAM_K uint32_t*)d.pcode, d.L};
+}
+template <> __device__ inline Tab<double> tab_of<double>(const Dev& d) {
+  return Tab<double>{(const ICAM_K double*)d.F64, (const ICAM_K uint32_t*)d.U, (const ICAM_K float*)d.pbox, (const ICAM_K uint32_t*)d.pcode, d.L};
+}
 
 // The level's geometry.
 struct Lv {

@@ -20,6 +20,11 @@
 #include "../../include/icam_part.h"
 #include "layout.hpp"
 
+// The part's tables are read through the constant address space: they do not change
+// while a kernel runs, so a read whose address is the same on every lane of a wave
+// (a block's candidate, its span's net) becomes a scalar load into SGPRs.
+#define ICAM_K __attribute__((address_space(4)))
+
 namespace icam {
 
 // ------------------------------------------------------------------ math by type
@@ -61,26 +66,27 @@ template <class T> __device__ inline v3<T> operator*(v3<T> a, T s) { return {a.x
 template <class T> __device__ inline T dot(v3<T> a, v3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <class T> __device__ inline v3<T> cross(v3<T> a, v3<T> b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 template <class T> __device__ inline T len(v3<T> a) { return msqrt(dot(a, a)); }
+template <class T> __device__ inline v3<T> ld3(const ICAM_K T* p) { return {p[0], p[1], p[2]}; }
 template <class T> __device__ inline v3<T> ld3(const T* p) { return {p[0], p[1], p[2]}; }
 template <class T, class S> __device__ inline v3<T> cv3(v3<S> a) { return {T(a.x), T(a.y), T(a.z)}; }
 
 // The tables of one compute type.
 template <class T> struct Tab {
-  const T* F;
-  const uint32_t* U;
-  const float* pbox;
-  const uint32_t* pcode;
+  const ICAM_K T* F;
+  const ICAM_K uint32_t* U;
+  const ICAM_K float* pbox;
+  const ICAM_K uint32_t* pcode;
   Layout L;
 };
 
-template <class T> __device__ inline T box_dist(v3<T> p, const float* b) {
+template <class T> __device__ inline T box_dist(v3<T> p, const ICAM_K float* b) {
   T dx = mmax(mmax(T(b[0]) - p.x, p.x - T(b[3])), T(0));
   T dy = mmax(mmax(T(b[1]) - p.y, p.y - T(b[4])), T(0));
   T dz = mmax(mmax(T(b[2]) - p.z, p.z - T(b[5])), T(0));
   return msqrt(dx * dx + dy * dy + dz * dz);
 }
 // The distance to the farthest point of a box.
-template <class T> __device__ inline T far_dist(v3<T> p, const float* b) {
+template <class T> __device__ inline T far_dist(v3<T> p, const ICAM_K float* b) {
   T dx = mmax(mabs(p.x - T(b[0])), mabs(p.x - T(b[3])));
   T dy = mmax(mabs(p.y - T(b[1])), mabs(p.y - T(b[4])));
   T dz = mmax(mabs(p.z - T(b[2])), mabs(p.z - T(b[5])));
@@ -154,8 +160,8 @@ template <class T> __device__ bool parity(const Tab<T>& tb, const T a[2], const 
   for (uint32_t k = 0; k < count; ++k) {
     uint32_t inst = tb.U[L.among_u + first + k];
     uint32_t pc = tb.U[L.inst_u + inst];
-    const T* sh = tb.F + L.inst_f + INST_F * inst;
-    const T* c = tb.F + L.tpiece_f + TPIECE_F * pc;
+    const ICAM_K T* sh = tb.F + L.inst_f + INST_F * inst;
+    const ICAM_K T* c = tb.F + L.tpiece_f + TPIECE_F * pc;
     T cu[4] = {c[0] + sh[0], c[2] + sh[0], c[4] + sh[0], c[6] + sh[0]};
     T cv[4] = {c[1] + sh[1], c[3] + sh[1], c[5] + sh[1], c[7] + sh[1]};
     T t;
@@ -188,7 +194,7 @@ template <class T> __device__ bool parity(const Tab<T>& tb, const T a[2], const 
 // trim2.rs:1433-1458 inside: the query reduced into the domain, the cell it falls in,
 // the answer there flipped by the parity from the cell's reference.
 // dom = face record words 17..22 (lo, hi, period), root = its first cell.
-template <class T> __device__ bool trim_inside(const Tab<T>& tb, const T* dom, uint32_t root, T u, T v, T m, bool& near) {
+template <class T> __device__ bool trim_inside(const Tab<T>& tb, const ICAM_K T* dom, uint32_t root, T u, T v, T m, bool& near) {
   const Layout& L = tb.L;
   T q[2] = {u, v};
   for (int k = 0; k < 2; ++k) {
@@ -203,8 +209,8 @@ template <class T> __device__ bool trim_inside(const Tab<T>& tb, const T* dom, u
   }
   uint32_t id = root;
   for (int depth = 0; depth < 64; ++depth) {
-    const uint32_t* cu = tb.U + L.cell_u + CELL_U * id;
-    const T* cf = tb.F + L.cell_f + CELL_F * id;
+    const ICAM_K uint32_t* cu = tb.U + L.cell_u + CELL_U * id;
+    const ICAM_K T* cf = tb.F + L.cell_f + CELL_F * id;
     uint32_t kind = cu[1];
     if (kind == 0) return cu[0] != 0;
     if (kind == 1) {
@@ -228,7 +234,7 @@ template <class T> struct Axial {
   v3<T> er;
 };
 // surface3.rs:36-53
-template <class T> __device__ inline Axial<T> axial(const T* f, v3<T> p) {
+template <class T> __device__ inline Axial<T> axial(const ICAM_K T* f, v3<T> p) {
   v3<T> o = ld3(f), x = ld3(f + 3), y = ld3(f + 6), z = ld3(f + 9);
   v3<T> d = p - o;
   T lx = dot(d, x), ly = dot(d, y), lz = dot(d, z);
@@ -246,7 +252,7 @@ template <class T> __device__ inline Axial<T> axial(const T* f, v3<T> p) {
 }
 
 // surface3.rs:85-128 the foot of p on an analytic surface (kind from the face's U record)
-template <class T> __device__ v3<T> foot(uint32_t kind, const T* f, v3<T> p) {
+template <class T> __device__ v3<T> foot(uint32_t kind, const ICAM_K T* f, v3<T> p) {
   v3<T> o = ld3(f), z = ld3(f + 9);
   T c0 = f[12], c1 = f[13];
   if (kind == ICAM_PLANE) return p - z * dot(p - o, z);
@@ -279,7 +285,7 @@ template <class T> __device__ v3<T> foot(uint32_t kind, const T* f, v3<T> p) {
 }
 
 // surface3.rs:131-149 the surface's own unit normal at a point on it
-template <class T> __device__ v3<T> snormal(uint32_t kind, const T* f, v3<T> x) {
+template <class T> __device__ v3<T> snormal(uint32_t kind, const ICAM_K T* f, v3<T> x) {
   v3<T> o = ld3(f), z = ld3(f + 9);
   if (kind == ICAM_PLANE) return z;
   if (kind == ICAM_CYLINDER) return axial(f, x).er;
@@ -314,7 +320,7 @@ template <class T> __device__ inline T arc_param_over(T a, T a0, T span, T n) {
 }
 
 // trim2.rs:1907-1934 Analytic::uv
-template <class T> __device__ void face_uv(uint32_t kind, uint32_t narc, const T* f, v3<T> q, T& u, T& v) {
+template <class T> __device__ void face_uv(uint32_t kind, uint32_t narc, const ICAM_K T* f, v3<T> q, T& u, T& v) {
   const T HALF_PI = T(1.5707963267948966), FULL = T(6.283185307179586);
   if (kind == ICAM_PLANE) {
     v3<T> d = q - ld3(f);
@@ -351,7 +357,7 @@ template <class T> __device__ inline void bern3(T x, T b[4], T d[4], T e[4]) {
 
 // surface3.rs:322-351 a rational bicubic's point and derivatives at (s, t) from its
 // homogeneous net (16 points, i along u)
-template <class T> __device__ SEval<T> peval(const T* pw, T s, T t) {
+template <class T> __device__ SEval<T> peval(const ICAM_K T* pw, T s, T t) {
   T bu[4], du[4], eu[4], bv[4], dv[4], ev[4];
   bern3(s, bu, du, eu);
   bern3(t, bv, dv, ev);
@@ -368,7 +374,7 @@ template <class T> __device__ SEval<T> peval(const T* pw, T s, T t) {
     for (int c = 0; c < 4; ++c) r0[c] = r1[c] = r2[c] = T(0);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const T* h = pw + 16 * i + 4 * j;
+      const ICAM_K T* h = pw + 16 * i + 4 * j;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         r0[c] += bv[j] * h[c];
@@ -403,7 +409,7 @@ template <class T> __device__ SEval<T> peval(const T* pw, T s, T t) {
 // runs in the other alone. Over a part its radius of convexity certifies, the
 // squared distance is strictly convex, so this reaches the part's one minimum,
 // inside it or on its border (where two parts share a border both reach it).
-template <class T> __device__ void newton2(const T* pw, v3<T> p, T& s, T& t, T s0, T s1, T t0, T t1, int iters, T stop) {
+template <class T> __device__ void newton2(const ICAM_K T* pw, v3<T> p, T& s, T& t, T s0, T s1, T t0, T t1, int iters, T stop) {
   for (int it = 0; it < iters; ++it) {
     SEval<T> e = peval(pw, s, t);
     v3<T> r = e.p - p;
@@ -438,7 +444,7 @@ template <class T> __device__ void newton2(const T* pw, v3<T> p, T& s, T& t, T s
 template <class T> struct CEval {
   v3<T> p, d1, d2;
 };
-template <class T> __device__ CEval<T> ceval(const T* pw, T s) {
+template <class T> __device__ CEval<T> ceval(const ICAM_K T* pw, T s) {
   T b[4], d[4], e[4];
   bern3(s, b, d, e);
   T a[3][4];
@@ -457,7 +463,7 @@ template <class T> __device__ CEval<T> ceval(const T* pw, T s) {
 }
 
 // curve3.rs:192-213 Newton on |C - p|^2 over [t0, t1]
-template <class T> __device__ T newton1(const T* pw, v3<T> p, T s, T t0, T t1, int iters, T stop) {
+template <class T> __device__ T newton1(const ICAM_K T* pw, v3<T> p, T s, T t0, T t1, int iters, T stop) {
   for (int it = 0; it < iters; ++it) {
     CEval<T> e = ceval(pw, s);
     v3<T> r = e.p - p;
@@ -492,10 +498,10 @@ template <class T> struct Best {
 // The outward normal of face fi at a point x on (or within rounding of) it. A
 // B-spline face takes its parameters from the edge's map (uvmap: u, v at the start
 // and end of the piece) at the piece parameter tt.
-template <class T> __device__ v3<T> face_normal_at(const Tab<T>& tb, uint32_t fi, v3<T> x, const T* uvm, T tt) {
+template <class T> __device__ v3<T> face_normal_at(const Tab<T>& tb, uint32_t fi, v3<T> x, const ICAM_K T* uvm, T tt) {
   const Layout& L = tb.L;
-  const T* f = tb.F + L.face_f + FACE_F * fi;
-  const uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
+  const ICAM_K T* f = tb.F + L.face_f + FACE_F * fi;
+  const ICAM_K uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
   uint32_t kind = fu[0];
   if (kind != ICAM_NURBS) return snormal(kind, f, x) * f[14];
   if (!uvm) return mk3(T(0), T(0), T(0));
@@ -506,7 +512,7 @@ template <class T> __device__ v3<T> face_normal_at(const Tab<T>& tb, uint32_t fi
     if (tb.F[L.span_f + SPAN_F * (span0 + k * nsv) + 64] <= u) a = k;
   for (uint32_t k = 1; k < nsv; ++k)
     if (tb.F[L.span_f + SPAN_F * (span0 + k) + 66] <= v) b = k;
-  const T* sp = tb.F + L.span_f + SPAN_F * (span0 + a * nsv + b);
+  const ICAM_K T* sp = tb.F + L.span_f + SPAN_F * (span0 + a * nsv + b);
   T s = mclamp((u - sp[64]) / (sp[65] - sp[64]), T(0), T(1)), t = mclamp((v - sp[66]) / (sp[67] - sp[66]), T(0), T(1));
   // the point is on the surface at these parameters; two Newton steps take up rounding
   newton2(sp, x, s, t, T(0), T(1), T(0), T(1), 2, T(0));
@@ -519,8 +525,8 @@ template <class T> __device__ v3<T> face_normal_at(const Tab<T>& tb, uint32_t fi
 // An edge's sign vector at its point q (the two faces' outward normals summed).
 template <class T> __device__ v3<T> edge_pn(const Tab<T>& tb, uint32_t e, v3<T> q, uint32_t piece, T tt) {
   const Layout& L = tb.L;
-  const uint32_t* eu = tb.U + L.edge_u + EDGE_U * e;
-  const T* uvm = eu[3] != 0xffffffffu ? tb.F + L.uvmap_f + eu[3] + 8 * piece : nullptr;
+  const ICAM_K uint32_t* eu = tb.U + L.edge_u + EDGE_U * e;
+  const ICAM_K T* uvm = eu[3] != 0xffffffffu ? tb.F + L.uvmap_f + eu[3] + 8 * piece : nullptr;
   v3<T> n0 = face_normal_at(tb, eu[1], q, uvm, tt);
   v3<T> n1 = face_normal_at(tb, eu[2], q, uvm ? uvm + 4 : nullptr, tt);
   return n0 + n1;
@@ -540,8 +546,8 @@ template <class T> __device__ inline void offer(Best<T>& b, T d, uint32_t code, 
 // solid3.rs:1911-1925 / gpu.wgsl:415-440: an analytic face's foot inside the face
 template <class T> __device__ void try_face(const Tab<T>& tb, uint32_t fi, v3<T> p, Best<T>& b, T tm) {
   const Layout& L = tb.L;
-  const T* f = tb.F + L.face_f + FACE_F * fi;
-  const uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
+  const ICAM_K T* f = tb.F + L.face_f + FACE_F * fi;
+  const ICAM_K uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
   uint32_t kind = fu[0];
   v3<T> q = foot(kind, f, p);
   T d = len(p - q);
@@ -568,8 +574,8 @@ template <class T> __device__ void try_face(const Tab<T>& tb, uint32_t fi, v3<T>
 // solid3.rs:1775-1782, curve3.rs:525-562: a line or arc edge's interior foot
 template <class T> __device__ void try_edge(const Tab<T>& tb, uint32_t e, v3<T> p, Best<T>& b) {
   const Layout& L = tb.L;
-  const T* r = tb.F + L.edge_f + EDGE_F * e;
-  const uint32_t* eu = tb.U + L.edge_u + EDGE_U * e;
+  const ICAM_K T* r = tb.F + L.edge_f + EDGE_F * e;
+  const ICAM_K uint32_t* eu = tb.U + L.edge_u + EDGE_U * e;
   v3<T> q;
   T tt;
   if (eu[0] == ICAM_LINE) {
@@ -606,7 +612,7 @@ template <class T> __device__ void try_edge(const Tab<T>& tb, uint32_t e, v3<T> 
 }
 
 template <class T> __device__ void try_vert(const Tab<T>& tb, uint32_t vi, v3<T> p, Best<T>& b) {
-  const T* r = tb.F + tb.L.vert_f + VERT_F * vi;
+  const ICAM_K T* r = tb.F + tb.L.vert_f + VERT_F * vi;
   v3<T> x = ld3(r);
   T d = len(p - x);
   if (!(d < b.d)) return;
@@ -623,14 +629,14 @@ template <class T> __device__ inline bool is_critical(v3<T> r, v3<T> d, T tol) {
 // gpu.wgsl:679-756 / solid3.rs:1904-1910: a sub-patch read by Newton when its radius
 // certifies a single minimum for p; the minimum answers when it is a critical point
 // inside the face
-template <class T> __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps) {
+template <class T> __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps) {
   const Layout& L = tb.L;
-  const T* lf = tb.F + L.leaf_f + LEAF_F * li;
-  const uint32_t* lu = tb.U + L.leaf_u + LEAF_U * li;
+  const ICAM_K T* lf = tb.F + L.leaf_f + LEAF_F * li;
+  const ICAM_K uint32_t* lu = tb.U + L.leaf_u + LEAF_U * li;
   T rho = lf[0];
   T fd = far_dist(p, box);
   bool certified = fd < T(0.999) * rho - eps;
-  const T* sp = tb.F + L.span_f + SPAN_F * lu[0];
+  const ICAM_K T* sp = tb.F + L.span_f + SPAN_F * lu[0];
   T s0 = lf[1], s1 = lf[2], t0 = lf[3], t1 = lf[4];
   T s = T(0.5) * (s0 + s1), t = T(0.5) * (t0 + t1);
   const bool single = sizeof(T) == 4;
@@ -643,9 +649,9 @@ template <class T> __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const
   T tol = single ? T(1e-4) * d + eps : T(1e-9) * (T(1) + d);
   if (!(is_critical(r, e.ps, tol) && is_critical(r, e.pt, tol))) return;
   const uint32_t fi = lu[1];
-  const T* f = tb.F + L.face_f + FACE_F * fi;
+  const ICAM_K T* f = tb.F + L.face_f + FACE_F * fi;
   if (!lu[2]) {
-    const uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
+    const ICAM_K uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
     T u = sp[64] + s * (sp[65] - sp[64]), v = sp[66] + t * (sp[67] - sp[66]);
     bool near = false;
     bool in = trim_inside(tb, f + 17, fu[1], u, v, tm, near);
@@ -660,15 +666,15 @@ template <class T> __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const
 
 // gpu.wgsl:764-815: a part of a curve piece read by Newton when certified; the minimum
 // answers when it is critical and interior to its edge
-template <class T> __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const float* box, T lb, v3<T> p, Best<T>& b, T eps) {
+template <class T> __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T eps) {
   const Layout& L = tb.L;
-  const T* cf = tb.F + L.cleaf_f + CLEAF_F * ci;
-  const uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * ci;
+  const ICAM_K T* cf = tb.F + L.cleaf_f + CLEAF_F * ci;
+  const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * ci;
   T rho = cf[0];
   bool certified = far_dist(p, box) < T(0.999) * rho - eps;
   const uint32_t cp = cu[0];
-  const T* pw = tb.F + L.cpiece_f + CPIECE_F * cp;
-  const uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cp;
+  const ICAM_K T* pw = tb.F + L.cpiece_f + CPIECE_F * cp;
+  const ICAM_K uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cp;
   T t0 = cf[1], t1 = cf[2];
   const bool single = sizeof(T) == 4;
   T s = newton1(pw, p, T(0.5) * (t0 + t1), t0, t1, single ? 10 : 40, single ? T(1e-5) : T(1e-15));
@@ -691,13 +697,13 @@ template <class T> __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, cons
 // the same support-plane bound along the direction to a Newton point), less a
 // rounding allowance.
 template <class T> __device__ inline T slab_dist(const Tab<T>& tb, uint32_t li, v3<T> p, T eps) {
-  const T* lf = tb.F + tb.L.leaf_f + LEAF_F * li;
+  const ICAM_K T* lf = tb.F + tb.L.leaf_f + LEAF_F * li;
   T x = dot(p, ld3(lf + 5));
   return mmax(mmax(x - lf[9], lf[8] - x) - eps, T(0));
 }
 
 // One candidate primitive for one lane.
-template <class T> __device__ inline void visit(const Tab<T>& tb, uint32_t code, const float* box, v3<T> p, Best<T>& b, T tm, T eps) {
+template <class T> __device__ inline void visit(const Tab<T>& tb, uint32_t code, const ICAM_K float* box, v3<T> p, Best<T>& b, T tm, T eps) {
   T lb = box_dist(p, box);
   if (!(lb < b.d)) return;
   uint32_t kind = code >> 28, id = code & 0x0fffffffu;
@@ -716,7 +722,7 @@ __device__ inline void refine64(const Tab<double>& tb, uint32_t code, double pa,
   const Layout& L = tb.L;
   uint32_t kind = code >> 28, id = code & 0x0fffffffu;
   if (kind == P_FACE) {
-    const double* f = tb.F + L.face_f + FACE_F * id;
+    const ICAM_K double* f = tb.F + L.face_f + FACE_F * id;
     uint32_t k = tb.U[L.face_u + FACE_U * id];
     v3<double> q = foot(k, f, p);
     d = len(p - q);
@@ -727,26 +733,26 @@ __device__ inline void refine64(const Tab<double>& tb, uint32_t code, double pa,
     try_edge(tb, id, p, b);
     if (b.code != ~0u) d = b.d, sg = b.sgn;
   } else if (kind == P_VERT) {
-    const double* r = tb.F + L.vert_f + VERT_F * id;
+    const ICAM_K double* r = tb.F + L.vert_f + VERT_F * id;
     v3<double> x = ld3(r);
     d = len(p - x);
     sg = sgn_of(dot(p - x, ld3(r + 3)));
   } else if (kind == P_LEAF) {
-    const double* lf = tb.F + L.leaf_f + LEAF_F * id;
-    const uint32_t* lu = tb.U + L.leaf_u + LEAF_U * id;
-    const double* sp = tb.F + L.span_f + SPAN_F * lu[0];
+    const ICAM_K double* lf = tb.F + L.leaf_f + LEAF_F * id;
+    const ICAM_K uint32_t* lu = tb.U + L.leaf_u + LEAF_U * id;
+    const ICAM_K double* sp = tb.F + L.span_f + SPAN_F * lu[0];
     double s = pa, t = pb;
     newton2(sp, p, s, t, lf[1], lf[2], lf[3], lf[4], 4, 1e-15);
     SEval<double> e = peval(sp, s, t);
     v3<double> r = e.p - p;
     d = len(r);
-    const double* f = tb.F + L.face_f + FACE_F * lu[1];
+    const ICAM_K double* f = tb.F + L.face_f + FACE_F * lu[1];
     sg = sgn_of(dot(r, cross(e.ps, e.pt)) * -f[14]);
   } else {
-    const double* cf = tb.F + L.cleaf_f + CLEAF_F * id;
-    const uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * id;
-    const double* pw = tb.F + L.cpiece_f + CPIECE_F * cu[0];
-    const uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cu[0];
+    const ICAM_K double* cf = tb.F + L.cleaf_f + CLEAF_F * id;
+    const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * id;
+    const ICAM_K double* pw = tb.F + L.cpiece_f + CPIECE_F * cu[0];
+    const ICAM_K uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cu[0];
     double s = newton1(pw, p, pa, cf[1], cf[2], 4, 1e-16);
     CEval<double> e = ceval(pw, s);
     d = len(e.p - p);
