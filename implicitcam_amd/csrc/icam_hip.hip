@@ -249,10 +249,28 @@ __device__ void block_box(float lo[3], float hi[3], float& r, float* red) {
 
 // One lane's search over the block's candidates (or the whole list when they did not
 // fit): the closed-form features (faces, edges, vertices) in list order, then the
-// B-spline sub-patches and curve parts nearest first by their box distance, each
-// round the lane's own next one, until none can beat the best answer.
+// B-spline sub-patches and curve parts nearest first by their lower bound (box and
+// slab), each round the lane's own next one, until none can beat the best answer.
+// The lane's parts under its bound go once, with their lower bounds, into its column
+// of the block's queue (Q entries of NT lanes in LDS), so a round reads the queue
+// instead of the tables; a lane with more than Q such parts scans the list each round.
+// Equal bounds are taken in list order either way.
 template <class T>
-__device__ Best<T> search(const Tab<T>& tb, v3<T> p, T bound, const uint32_t* cand, int nc, T tm, T eps) {
+__device__ inline T part_lb(const Tab<T>& tb, uint32_t i, uint32_t code, v3<T> p, T eps) {
+  T lb = box_dist(p, tb.pbox + 6 * i);
+  if ((code >> 28) == P_LEAF) lb = mmax(lb, slab_dist(tb, code & 0x0fffffffu, p, eps));
+  return lb;
+}
+template <class T>
+__device__ inline void try_part(const Tab<T>& tb, uint32_t i, T lb, v3<T> p, Best<T>& b, T tm, T eps) {
+  const uint32_t code = tb.pcode[i], id = code & 0x0fffffffu;
+  if ((code >> 28) == P_LEAF)
+    try_leaf(tb, id, tb.pbox + 6 * i, lb, p, b, tm, eps);
+  else
+    try_cleaf(tb, id, tb.pbox + 6 * i, lb, p, b, eps);
+}
+template <class T, int NT, int Q>
+__device__ Best<T> search(const Tab<T>& tb, v3<T> p, T bound, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk) {
   Best<T> b;
   b.d = bound, b.code = NONE, b.a = b.b = T(0), b.sgn = T(1), b.lmin = TypeEps<T>::big, b.d2 = TypeEps<T>::big, b.sgn2 = T(1), b.unsure = false;
   const bool all = nc > MAXC;
@@ -269,6 +287,34 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, T bound, const uint32_t* ca
     visit(tb, code, tb.pbox + 6 * i, p, b, tm, eps);
   }
   if (!parts) return b;
+  const int lane = threadIdx.x;
+  int m = 0;
+  if (!all && n <= 65535u) {
+    for (uint32_t k = 0; k < n; ++k) {
+      const uint32_t i = __builtin_amdgcn_readfirstlane(cand[k]);
+      const uint32_t code = tb.pcode[i];
+      if ((code >> 28) < P_LEAF) continue;
+      T lb = part_lb(tb, i, code, p, eps);
+      if (lb < b.d) {
+        if (m < Q) qlb[m * NT + lane] = lb, qk[m * NT + lane] = uint16_t(k);
+        ++m;
+      }
+    }
+    if (m <= Q) {
+      for (;;) {
+        T next_lb = b.d;
+        int next_j = -1;
+        for (int q = 0; q < m; ++q) {
+          T v = qlb[q * NT + lane];
+          if (v < next_lb) next_lb = v, next_j = q;
+        }
+        if (next_j < 0) break;
+        qlb[next_j * NT + lane] = TypeEps<T>::big;
+        try_part(tb, cand[qk[next_j * NT + lane]], next_lb, p, b, tm, eps);
+      }
+      return b;
+    }
+  }
   T last_lb = T(-1);
   uint32_t last_k = 0;
   for (int round = 0; round < 4096; ++round) {
@@ -279,18 +325,12 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, T bound, const uint32_t* ca
       i = __builtin_amdgcn_readfirstlane(i);
       const uint32_t code = tb.pcode[i];
       if ((code >> 28) < P_LEAF) continue;
-      T lb = box_dist(p, tb.pbox + 6 * i);
-      if ((code >> 28) == P_LEAF) lb = mmax(lb, slab_dist(tb, code & 0x0fffffffu, p, eps));
+      T lb = part_lb(tb, i, code, p, eps);
       bool after = lb > last_lb || (lb == last_lb && k > last_k);
       if (after && lb < next_lb) next_lb = lb, next_k = k;
     }
     if (next_k == NONE) break;
-    const uint32_t i = all ? next_k : cand[next_k];
-    const uint32_t code = tb.pcode[i], id = code & 0x0fffffffu;
-    if ((code >> 28) == P_LEAF)
-      try_leaf(tb, id, tb.pbox + 6 * i, next_lb, p, b, tm, eps);
-    else
-      try_cleaf(tb, id, tb.pbox + 6 * i, next_lb, p, b, eps);
+    try_part(tb, all ? next_k : cand[next_k], next_lb, p, b, tm, eps);
     last_lb = next_lb, last_k = next_k;
   }
   return b;
@@ -326,6 +366,7 @@ __device__ inline void classify64(const Best<double>& b, double bound, const Dev
 // lanes; a tile of 4 x 4 x 4 brick centres a block of 64.
 template <int S> struct Shape {
   static constexpr int NT = S == SRC_CENTRES ? 64 : 256;
+  static constexpr int Q = S == SRC_CENTRES ? 128 : 16;  // queue entries a lane
 };
 
 // The tile of 4 x 4 x 4 bricks of block `tile`: lane -> brick index, -1 past the level;
@@ -380,7 +421,7 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
     float ub = float(fmin(bound, 3.0e38));
     if (S == SRC_POINTS) {
       uint32_t nv = dv.L.n_verts < 16 ? dv.L.n_verts : 16;
-      for (uint32_t q = 0; q < nv; ++q) ub = fminf(ub, float(len(p - ld3(dv.F64 + dv.L.vert_f + VERT_F * q))) * (1.0f + 1e-5f) + 1e-5f);
+      for (uint32_t q = 0; q < nv; ++q) ub = fminf(ub, float(len(p - ld3((const ICAM_K double*)dv.F64 + dv.L.vert_f + VERT_F * q))) * (1.0f + 1e-5f) + 1e-5f);
     }
     lo[0] = fminf(lo[0], float(p.x)), lo[1] = fminf(lo[1], float(p.y)), lo[2] = fminf(lo[2], float(p.z));
     hi[0] = fmaxf(hi[0], float(p.x)), hi[1] = fmaxf(hi[1], float(p.y)), hi[2] = fmaxf(hi[2], float(p.z));
@@ -417,6 +458,8 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res)
   __shared__ float key[SORTC];
   __shared__ float red[NT];
   __shared__ int wave_tot[NT / 64];
+  __shared__ T qlb[Shape<S>::Q * NT];
+  __shared__ uint16_t qk[Shape<S>::Q * NT];
   const int nc = block_candidates<S>(dv, j, blockIdx.x, cand, tmp, key, red, wave_tot);
   const Tab<T> tb = tab_of<T>(dv);
   const T eps = sizeof(T) == 4 ? T(dv.L.eps32) : T(dv.L.eps64);
@@ -428,7 +471,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res)
     v3<double> p = job_point<S>(j, dv, i, bound);
     if constexpr (sizeof(T) == 4) {
       float b32 = bound < 3.0e38 ? float(bound) * (1.0f + 1e-6f) + float(eps) : 3.0e38f;
-      Best<T> b = search<T>(tb, cv3<T>(p), T(b32), cand, nc, tm, eps);
+      Best<T> b = search<T, NT, Shape<S>::Q>(tb, cv3<T>(p), T(b32), cand, nc, tm, eps, qlb, qk);
       bool tie = b.d2 - b.d <= T(8) * eps && b.sgn2 != b.sgn;
       uint32_t info = (b.code != NONE ? 1u : 0u) | (b.sgn < T(0) ? 2u : 0u) | ((b.unsure || tie) ? 4u : 0u) | ((b.lmin < b.d) ? 8u : 0u) |
                       (b.unsure ? 16u : 0u) | (tie ? 32u : 0u);
@@ -438,7 +481,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res)
       res.d32[i] = float((b.lmin < b.d) ? b.lmin : b.d);
       res.info[i] = info;
     } else {
-      Best<T> b = search<T>(tb, cv3<T>(p), bound < 1e300 ? T(bound * (1.0 + 1e-12)) : T(bound), cand, nc, tm, eps);
+      Best<T> b = search<T, NT, Shape<S>::Q>(tb, cv3<T>(p), bound < 1e300 ? T(bound * (1.0 + 1e-12)) : T(bound), cand, nc, tm, eps, qlb, qk);
       double val;
       uint32_t flag, feat;
       classify64(b, bound, dv, val, flag, feat);
@@ -501,6 +544,8 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_fallback(Dev dv, Job j, Res re
   __shared__ float key[SORTC];
   __shared__ float red[NT];
   __shared__ int wave_tot[NT / 64];
+  __shared__ double qlb[Shape<S>::Q * NT];
+  __shared__ uint16_t qk[Shape<S>::Q * NT];
   const uint32_t total = *res.qcount;
   const Tab<double> tb = tab_of<double>(dv);
   for (uint32_t qb = blockIdx.x; qb < total; qb += gridDim.x) {
@@ -511,7 +556,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_fallback(Dev dv, Job j, Res re
       if (i < 0 || res.flag[i] != 3) continue;
       double bound;
       v3<double> p = job_point<S>(j, dv, i, bound);
-      Best<double> b = search<double>(tb, p, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, cand, nc, 0.0, dv.L.eps64);
+      Best<double> b = search<double, NT, Shape<S>::Q>(tb, p, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, cand, nc, 0.0, dv.L.eps64, qlb, qk);
       double val;
       uint32_t flag, feat;
       classify64(b, bound, dv, val, flag, feat);

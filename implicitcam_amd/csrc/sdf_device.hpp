@@ -23,7 +23,9 @@
 // The part's tables are read through the constant address space: they do not change
 // while a kernel runs, so a read whose address is the same on every lane of a wave
 // (a block's candidate, its span's net) becomes a scalar load into SGPRs.
+#ifndef ICAM_K
 #define ICAM_K __attribute__((address_space(4)))
+#endif
 
 namespace icam {
 
@@ -67,7 +69,6 @@ template <class T> __device__ inline T dot(v3<T> a, v3<T> b) { return a.x * b.x 
 template <class T> __device__ inline v3<T> cross(v3<T> a, v3<T> b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 template <class T> __device__ inline T len(v3<T> a) { return msqrt(dot(a, a)); }
 template <class T> __device__ inline v3<T> ld3(const ICAM_K T* p) { return {p[0], p[1], p[2]}; }
-template <class T> __device__ inline v3<T> ld3(const T* p) { return {p[0], p[1], p[2]}; }
 template <class T, class S> __device__ inline v3<T> cv3(v3<S> a) { return {T(a.x), T(a.y), T(a.z)}; }
 
 // The tables of one compute type.
