@@ -87,6 +87,13 @@ def main():
         log(f"bake {name}: {dt * 1e3:.3f} ms/step, band {st.band}, slots {st.slots}, undecided {st.undecided}, searched again in fp64 "
             f"{st.refined} (trim {st.unsure_trim}, ties {st.unsure_tie}), stages {res[name]['stages']}")
     if args.profile:
+        # a plain copy of known size (256 MiB read, 256 MiB written): the scale for the
+        # profiler's HBM byte counters
+        x = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
         return
 
     # 2^20 scattered queries on the analytic part (C1), inputs resident in HBM

@@ -42,8 +42,8 @@ using namespace icam;
 namespace {
 
 constexpr int WG = 256;
-constexpr int MAXC = 4096;
 constexpr int SORTC = 1024;  // lists up to this long are sorted nearest first
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int TCAP = 512;    // a tile's list kept for its bricks' nodes, at most
 constexpr uint32_t NONE = 0xffffffffu;
 
@@ -109,9 +109,9 @@ struct Job {
   const uint32_t* list;   // SRC_NODES: band bricks of the level (NULL: `firsts`)
   const double* firsts;   // SRC_NODES: explicit bricks (first node xyz, h)
   Lv lv;
-  // the candidate list of each 4 x 4 x 4 tile of bricks: written by the centres'
-  // search (within the reach of the tile's centres), read by the nodes' search (a
-  // brick's nodes' candidates lie within that reach of its centre)
+  // the candidate list of each tile of bricks: written by the centres' search (within
+  // the reach of the tile's centres), read by the nodes' search (a brick's nodes'
+  // candidates lie within that reach of its centre)
   uint32_t* tile_cand;    // TCAP per tile
   uint32_t* tile_count;   // per tile (more than TCAP: not kept)
 };
@@ -172,8 +172,8 @@ __device__ inline float box_gap(const float* a, const float* b) {
 }
 
 // The primitives whose box comes within R of the block's box gb (lo, hi), in list
-// order, into cand; returns how many (more than MAXC: the list did not fit).
-template <int NT>
+// order, into cand (C entries); returns how many (more than C: the list did not fit).
+template <int NT, int C>
 __device__ int cull(const Dev& dv, const float* gb, float R, uint32_t* cand, int* wave_tot, const uint32_t* from = nullptr, uint32_t n_from = 0) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t total = from ? n_from : dv.L.n_prims;
@@ -192,7 +192,7 @@ __device__ int cull(const Dev& dv, const float* gb, float R, uint32_t* cand, int
     __syncthreads();
     int off = base;
     for (int w = 0; w < wave; ++w) off += wave_tot[w];
-    if (keep && off + before < MAXC) cand[off + before] = i;
+    if (keep && off + before < C) cand[off + before] = i;
     int tot = 0;
     for (int w = 0; w < NT / 64; ++w) tot += wave_tot[w];
     __syncthreads();
@@ -247,14 +247,23 @@ __device__ void block_box(float lo[3], float hi[3], float& r, float* red) {
 
 // ------------------------------------------------------------------ one lane's search
 
-// One lane's search over the block's candidates (or the whole list when they did not
-// fit): the closed-form features (faces, edges, vertices) in list order, then the
-// B-spline sub-patches and curve parts nearest first by their lower bound (box and
-// slab), each round the lane's own next one, until none can beat the best answer.
-// The lane's parts under its bound go once, with their lower bounds, into its column
-// of the block's queue (Q entries of NT lanes in LDS), so a round reads the queue
-// instead of the tables; a lane with more than Q such parts scans the list each round.
-// Equal bounds are taken in list order either way.
+// A point's search over the block's candidates (or the whole list when they did not
+// fit), shared by K consecutive lanes of a group: lane `sub` of the group takes the
+// candidates sub, sub + K, ... The closed-form features (faces, edges, vertices) go
+// first, in list order; then the B-spline sub-patches and curve parts nearest first
+// by their lower bound (box and slab), each round every lane its own next one under
+// the group's best answer so far, until none can beat it. A lane's parts under the
+// bound go once, with their lower bounds, into its column of the block's queue (Q
+// entries of NT lanes in LDS), so a round reads the queue and not the tables; a lane
+// with more than Q such parts scans its share of the list each round. Equal bounds
+// are taken in list order either way.
+//
+// Every part with a lower bound under the final answer is tried, as in one lane's
+// search; a group may try more (its best was still larger when it chose them), and
+// those change neither the answer nor its undecided bound. The group's answer is its
+// lanes' least distance (ties: the lowest lane); near-ties of opposite signs among all
+// its lanes' answers and runners-up are reported as a runner-up at the same distance.
+// Lanes of a group whose point is not `valid` take part with no work.
 template <class T>
 __device__ inline T part_lb(const Tab<T>& tb, uint32_t i, uint32_t code, v3<T> p, T eps) {
   T lb = box_dist(p, tb.pbox + 6 * i);
@@ -269,69 +278,99 @@ __device__ inline void try_part(const Tab<T>& tb, uint32_t i, T lb, v3<T> p, Bes
   else
     try_cleaf(tb, id, tb.pbox + 6 * i, lb, p, b, eps);
 }
-template <class T, int NT, int Q>
-__device__ Best<T> search(const Tab<T>& tb, v3<T> p, T bound, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk) {
+template <int K, class V> __device__ inline V group_min(V v) {
+  for (int o = 1; o < K; o <<= 1) v = mmin(v, V(__shfl_xor(v, o)));
+  return v;
+}
+template <int K> __device__ inline int group_or(int v) {
+  for (int o = 1; o < K; o <<= 1) v |= __shfl_xor(v, o);
+  return v;
+}
+template <class T, int NT, int Q, int K, int C>
+__device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk) {
   Best<T> b;
   b.d = bound, b.code = NONE, b.a = b.b = T(0), b.sgn = T(1), b.lmin = TypeEps<T>::big, b.d2 = TypeEps<T>::big, b.sgn2 = T(1), b.unsure = false;
-  const bool all = nc > MAXC;
+  const bool all = nc > C;
   const uint32_t n = all ? tb.L.n_prims : uint32_t(nc);
+  const int lane = threadIdx.x, sub = lane & (K - 1);
   bool parts = false;
-  for (uint32_t k = 0; k < n; ++k) {
-    uint32_t i = all ? k : cand[k];
-    i = __builtin_amdgcn_readfirstlane(i);
-    const uint32_t code = tb.pcode[i];
-    if ((code >> 28) >= P_LEAF) {
-      parts = true;
-      continue;
+  if (valid) {
+    for (uint32_t k = sub; k < n; k += K) {
+      uint32_t i = all ? k : cand[k];
+      if (K == 1) i = __builtin_amdgcn_readfirstlane(i);
+      const uint32_t code = tb.pcode[i];
+      if ((code >> 28) >= P_LEAF) {
+        parts = true;
+        continue;
+      }
+      visit(tb, code, tb.pbox + 6 * i, p, b, tm, eps);
     }
-    visit(tb, code, tb.pbox + 6 * i, p, b, tm, eps);
   }
-  if (!parts) return b;
-  const int lane = threadIdx.x;
+  if (K == 1 && !parts) return b;
   int m = 0;
-  if (!all && n <= 65535u) {
-    for (uint32_t k = 0; k < n; ++k) {
-      const uint32_t i = __builtin_amdgcn_readfirstlane(cand[k]);
+  if (parts && !all) {
+    const T bd = b.d;
+    for (uint32_t k = sub; k < n; k += K) {
+      uint32_t i = cand[k];
+      if (K == 1) i = __builtin_amdgcn_readfirstlane(i);
       const uint32_t code = tb.pcode[i];
       if ((code >> 28) < P_LEAF) continue;
       T lb = part_lb(tb, i, code, p, eps);
-      if (lb < b.d) {
+      if (lb < bd) {
         if (m < Q) qlb[m * NT + lane] = lb, qk[m * NT + lane] = uint16_t(k);
         ++m;
       }
     }
-    if (m <= Q) {
-      for (;;) {
-        T next_lb = b.d;
-        int next_j = -1;
-        for (int q = 0; q < m; ++q) {
-          T v = qlb[q * NT + lane];
-          if (v < next_lb) next_lb = v, next_j = q;
-        }
-        if (next_j < 0) break;
-        qlb[next_j * NT + lane] = TypeEps<T>::big;
-        try_part(tb, cand[qk[next_j * NT + lane]], next_lb, p, b, tm, eps);
-      }
-      return b;
-    }
   }
+  const bool scan = parts && (all || m > Q);
   T last_lb = T(-1);
   uint32_t last_k = 0;
   for (int round = 0; round < 4096; ++round) {
-    T next_lb = b.d;
-    uint32_t next_k = NONE;
-    for (uint32_t k = 0; k < n; ++k) {
-      uint32_t i = all ? k : cand[k];
-      i = __builtin_amdgcn_readfirstlane(i);
-      const uint32_t code = tb.pcode[i];
-      if ((code >> 28) < P_LEAF) continue;
-      T lb = part_lb(tb, i, code, p, eps);
-      bool after = lb > last_lb || (lb == last_lb && k > last_k);
-      if (after && lb < next_lb) next_lb = lb, next_k = k;
+    T next_lb = K > 1 ? group_min<K>(b.d) : b.d;
+    int next = -1;
+    if (parts && !scan) {
+      for (int q = 0; q < m; ++q) {
+        T v = qlb[q * NT + lane];
+        if (v < next_lb) next_lb = v, next = q;
+      }
+    } else if (scan) {
+      for (uint32_t k = sub; k < n; k += K) {
+        uint32_t i = all ? k : cand[k];
+        if (K == 1) i = __builtin_amdgcn_readfirstlane(i);
+        const uint32_t code = tb.pcode[i];
+        if ((code >> 28) < P_LEAF) continue;
+        T lb = part_lb(tb, i, code, p, eps);
+        bool after = lb > last_lb || (lb == last_lb && k > last_k);
+        if (after && lb < next_lb) next_lb = lb, next = int(k);
+      }
     }
-    if (next_k == NONE) break;
-    try_part(tb, all ? next_k : cand[next_k], next_lb, p, b, tm, eps);
-    last_lb = next_lb, last_k = next_k;
+    if (K > 1 ? !__any(next >= 0) : next < 0) break;
+    if (next >= 0) {
+      uint32_t k;
+      if (!scan) {
+        qlb[next * NT + lane] = TypeEps<T>::big;
+        k = qk[next * NT + lane];
+      } else {
+        k = uint32_t(next);
+        last_lb = next_lb, last_k = k;
+      }
+      try_part(tb, all ? k : cand[k], next_lb, p, b, tm, eps);
+    }
+  }
+  if (K > 1) {
+    const T dmin = group_min<K>(b.d);
+    const int wsub = group_min<K>(b.d == dmin ? sub : K);
+    const int src = (lane & ~(K - 1)) + wsub;
+    const T sw = __shfl(b.sgn, src);
+    // runners-up within 8 rounding steps of the answer on the other side
+    const T near = T(8) * eps;
+    int tie = (b.d2 - dmin <= near && b.sgn2 != sw) || (sub != wsub && b.code != NONE && b.d - dmin <= near && b.sgn != sw);
+    tie = group_or<K>(tie);
+    b.lmin = group_min<K>(b.lmin);
+    b.unsure = group_or<K>(b.unsure ? 1 : 0) != 0;
+    b.code = __shfl(b.code, src), b.a = __shfl(b.a, src), b.b = __shfl(b.b, src);
+    b.d = dmin, b.sgn = sw;
+    b.d2 = tie ? dmin : TypeEps<T>::big, b.sgn2 = tie ? -sw : sw;
   }
   return b;
 }
@@ -362,29 +401,39 @@ __device__ inline void classify64(const Best<double>& b, double bound, const Dev
   val = l, flag = 2;
 }
 
-// Block shapes: explicit points 256 a block; a brick's 729 nodes a block of 256
-// lanes; a tile of 4 x 4 x 4 brick centres a block of 64.
+// Block shapes: explicit points 256 a block, one lane each; a brick's 729 nodes a
+// block of 256 lanes; a tile of TILE_X x TILE_Y x TILE_Z brick centres a block of 64,
+// K = 4 lanes a centre (a centre's search reaches far, and the tiles holding the
+// surface are few).
 template <int S> struct Shape {
   static constexpr int NT = S == SRC_CENTRES ? 64 : 256;
-  static constexpr int Q = S == SRC_CENTRES ? 128 : 16;  // queue entries a lane
+  static constexpr int K = S == SRC_CENTRES ? 4 : 1;    // lanes a point
+  static constexpr int Q = S == SRC_CENTRES ? 32 : 16;  // queue entries a lane
+  static constexpr int C = S == SRC_POINTS ? 4096 : 1024;  // a block's candidates, at most
 };
+constexpr int TILE_X = 4, TILE_Y = 2, TILE_Z = 2;
+static_assert(TILE_X * TILE_Y * TILE_Z * Shape<SRC_CENTRES>::K == Shape<SRC_CENTRES>::NT, "a centres block covers its tile");
 
-// The tile of 4 x 4 x 4 bricks of block `tile`: lane -> brick index, -1 past the level;
-// and the tile a brick lies in.
-__device__ inline long tile_of(const Lv& lv, long b) {
-  const long tx = (lv.nb[0] + 3) / 4, ty = (lv.nb[1] + 3) / 4;
-  long bx = b % lv.nb[0], by = (b / lv.nb[0]) % lv.nb[1], bz = b / (long(lv.nb[0]) * lv.nb[1]);
-  return bx / 4 + tx * (by / 4 + ty * (bz / 4));
+// The tiles of a level, the tile a brick lies in, and the g-th brick of a tile (-1
+// past the level).
+__host__ __device__ inline long tiles_of(const int nb[3]) {
+  return long((nb[0] + TILE_X - 1) / TILE_X) * ((nb[1] + TILE_Y - 1) / TILE_Y) * ((nb[2] + TILE_Z - 1) / TILE_Z);
 }
-__device__ inline long tile_point(const Lv& lv, long tile, int lane) {
-  const int tx = (lv.nb[0] + 3) / 4, ty = (lv.nb[1] + 3) / 4;
-  long bx = 4 * (tile % tx) + (lane & 3), by = 4 * ((tile / tx) % ty) + ((lane >> 2) & 3), bz = 4 * (tile / (long(tx) * ty)) + (lane >> 4);
+__device__ inline long tile_of(const Lv& lv, long b) {
+  const long tx = (lv.nb[0] + TILE_X - 1) / TILE_X, ty = (lv.nb[1] + TILE_Y - 1) / TILE_Y;
+  long bx = b % lv.nb[0], by = (b / lv.nb[0]) % lv.nb[1], bz = b / (long(lv.nb[0]) * lv.nb[1]);
+  return bx / TILE_X + tx * (by / TILE_Y + ty * (bz / TILE_Z));
+}
+__device__ inline long tile_point(const Lv& lv, long tile, int g) {
+  const long tx = (lv.nb[0] + TILE_X - 1) / TILE_X, ty = (lv.nb[1] + TILE_Y - 1) / TILE_Y;
+  long bx = TILE_X * (tile % tx) + g % TILE_X, by = TILE_Y * ((tile / tx) % ty) + (g / TILE_X) % TILE_Y,
+       bz = TILE_Z * (tile / (tx * ty)) + g / (TILE_X * TILE_Y);
   if (bx >= lv.nb[0] || by >= lv.nb[1] || bz >= lv.nb[2]) return -1;
   return bx + lv.nb[0] * (by + long(lv.nb[1]) * bz);
 }
 
-// Block blk's m-th point for lane threadIdx.x (m counts the lane's rounds), or -1;
-// and the block a point belongs to.
+// Block blk's m-th point for lane threadIdx.x (m counts the lane's rounds; the K
+// lanes of a group share theirs), or -1; and the block a point belongs to.
 template <int S> __device__ inline long block_point(const Job& j, long blk, int m) {
   constexpr int NT = Shape<S>::NT;
   if (S == SRC_POINTS) {
@@ -396,7 +445,7 @@ template <int S> __device__ inline long block_point(const Job& j, long blk, int 
     int n = threadIdx.x + NT * m;
     return n < 729 ? blk * 729 + n : -1;
   }
-  return m > 0 ? -1 : tile_point(j.lv, blk, threadIdx.x);
+  return m > 0 ? -1 : tile_point(j.lv, blk, threadIdx.x / Shape<S>::K);
 }
 template <int S> __device__ inline long block_of(const Job& j, long i) {
   if (S == SRC_POINTS) return i / Shape<S>::NT;
@@ -434,9 +483,9 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
     // the brick's tile's list, when it was kept, instead of the whole primitive list
     long tile = tile_of(j.lv, long(j.list[blk]));
     uint32_t cnt = j.tile_count[tile];
-    nc = cnt <= uint32_t(TCAP) ? cull<NT>(dv, gb, r, cand, wave_tot, j.tile_cand + size_t(tile) * TCAP, cnt) : cull<NT>(dv, gb, r, cand, wave_tot);
+    nc = cnt <= uint32_t(TCAP) ? cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot, j.tile_cand + size_t(tile) * TCAP, cnt) : cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot);
   } else {
-    nc = cull<NT>(dv, gb, r, cand, wave_tot);
+    nc = cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot);
   }
   sort_cand<NT>(dv, gb, cand, tmp, key, nc);
   if (S == SRC_CENTRES && j.tile_cand) {
@@ -453,25 +502,32 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
 template <int S, class T>
 __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res) {
   constexpr int NT = Shape<S>::NT;
-  __shared__ uint32_t cand[MAXC];
-  __shared__ uint32_t tmp[SORTC];
-  __shared__ float key[SORTC];
+  constexpr int Q = Shape<S>::Q;
+  __shared__ uint32_t cand[Shape<S>::C];
   __shared__ float red[NT];
   __shared__ int wave_tot[NT / 64];
-  __shared__ T qlb[Shape<S>::Q * NT];
-  __shared__ uint16_t qk[Shape<S>::Q * NT];
+  // the sort's keys, then the lanes' queues
+  __shared__ __align__(16) unsigned char scratch[cmax(SORTC * 8, Q * NT * (int(sizeof(T)) + 2))];
+  uint32_t* tmp = (uint32_t*)scratch;
+  float* key = (float*)(scratch + SORTC * 4);
+  T* qlb = (T*)scratch;
+  uint16_t* qk = (uint16_t*)(scratch + Q * NT * sizeof(T));
   const int nc = block_candidates<S>(dv, j, blockIdx.x, cand, tmp, key, red, wave_tot);
   const Tab<T> tb = tab_of<T>(dv);
   const T eps = sizeof(T) == 4 ? T(dv.L.eps32) : T(dv.L.eps64);
   const T tm = sizeof(T) == 4 ? T(16) * eps : T(0);
+  constexpr int K = Shape<S>::K;
+  const bool owner = (threadIdx.x & (K - 1)) == 0;
   for (int m = 0; m < rounds<S>(); ++m) {
     long i = block_point<S>(j, blockIdx.x, m);
-    if (i < 0) continue;
-    double bound;
-    v3<double> p = job_point<S>(j, dv, i, bound);
+    if (K == 1 && i < 0) continue;
+    const bool valid = i >= 0;
+    double bound = 0.0;
+    v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
     if constexpr (sizeof(T) == 4) {
       float b32 = bound < 3.0e38 ? float(bound) * (1.0f + 1e-6f) + float(eps) : 3.0e38f;
-      Best<T> b = search<T, NT, Shape<S>::Q>(tb, cv3<T>(p), T(b32), cand, nc, tm, eps, qlb, qk);
+      Best<T> b = search<T, NT, Shape<S>::Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, T(b32), cand, nc, tm, eps, qlb, qk);
+      if (!valid || !owner) continue;
       bool tie = b.d2 - b.d <= T(8) * eps && b.sgn2 != b.sgn;
       uint32_t info = (b.code != NONE ? 1u : 0u) | (b.sgn < T(0) ? 2u : 0u) | ((b.unsure || tie) ? 4u : 0u) | ((b.lmin < b.d) ? 8u : 0u) |
                       (b.unsure ? 16u : 0u) | (tie ? 32u : 0u);
@@ -481,7 +537,8 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res)
       res.d32[i] = float((b.lmin < b.d) ? b.lmin : b.d);
       res.info[i] = info;
     } else {
-      Best<T> b = search<T, NT, Shape<S>::Q>(tb, cv3<T>(p), bound < 1e300 ? T(bound * (1.0 + 1e-12)) : T(bound), cand, nc, tm, eps, qlb, qk);
+      Best<T> b = search<T, NT, Shape<S>::Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, bound < 1e300 ? T(bound * (1.0 + 1e-12)) : T(bound), cand, nc, tm, eps, qlb, qk);
+      if (!valid || !owner) continue;
       double val;
       uint32_t flag, feat;
       classify64(b, bound, dv, val, flag, feat);
@@ -539,24 +596,30 @@ __global__ __launch_bounds__(256) void k_refine(Dev dv, Job j, Res res) {
 template <int S>
 __global__ __launch_bounds__(Shape<S>::NT) void k_fallback(Dev dv, Job j, Res res) {
   constexpr int NT = Shape<S>::NT;
-  __shared__ uint32_t cand[MAXC];
-  __shared__ uint32_t tmp[SORTC];
-  __shared__ float key[SORTC];
+  constexpr int Q = Shape<S>::Q;
+  __shared__ uint32_t cand[Shape<S>::C];
   __shared__ float red[NT];
   __shared__ int wave_tot[NT / 64];
-  __shared__ double qlb[Shape<S>::Q * NT];
-  __shared__ uint16_t qk[Shape<S>::Q * NT];
+  __shared__ __align__(16) unsigned char scratch[cmax(SORTC * 8, Q * NT * 10)];
+  uint32_t* tmp = (uint32_t*)scratch;
+  float* key = (float*)(scratch + SORTC * 4);
+  double* qlb = (double*)scratch;
+  uint16_t* qk = (uint16_t*)(scratch + Q * NT * 8);
   const uint32_t total = *res.qcount;
   const Tab<double> tb = tab_of<double>(dv);
+  constexpr int K = Shape<S>::K;
+  const bool owner = (threadIdx.x & (K - 1)) == 0;
   for (uint32_t qb = blockIdx.x; qb < total; qb += gridDim.x) {
     const long blk = res.queue[qb];
     const int nc = block_candidates<S>(dv, j, blk, cand, tmp, key, red, wave_tot);
     for (int m = 0; m < rounds<S>(); ++m) {
       long i = block_point<S>(j, blk, m);
-      if (i < 0 || res.flag[i] != 3) continue;
-      double bound;
-      v3<double> p = job_point<S>(j, dv, i, bound);
-      Best<double> b = search<double, NT, Shape<S>::Q>(tb, p, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, cand, nc, 0.0, dv.L.eps64, qlb, qk);
+      const bool valid = i >= 0 && res.flag[i] == 3;
+      if (K == 1 && !valid) continue;
+      double bound = 0.0;
+      v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
+      Best<double> b = search<double, NT, Shape<S>::Q, K, Shape<S>::C>(tb, p, valid, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, cand, nc, 0.0, dv.L.eps64, qlb, qk);
+      if (!valid || !owner) continue;
       double val;
       uint32_t flag, feat;
       classify64(b, bound, dv, val, flag, feat);
@@ -1162,7 +1225,7 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   HIP_OK(hipMemsetAsync(h->counters.p, 0, 64, s));
   Job jc{};
   jc.n = nb, jc.lv = lv;
-  long tiles = long((lv.nb[0] + 3) / 4) * ((lv.nb[1] + 3) / 4) * ((lv.nb[2] + 3) / 4);
+  long tiles = tiles_of(lv.nb);
   HIP_OK(h->tile_cand.need(size_t(tiles) * TCAP * 4));
   HIP_OK(h->tile_count.need(size_t(tiles) * 4));
   jc.tile_cand = h->tile_cand.as<uint32_t>(), jc.tile_count = h->tile_count.as<uint32_t>();
