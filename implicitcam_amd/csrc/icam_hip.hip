@@ -690,32 +690,110 @@ __global__ void k_sign(Lv lv, int ax, const uint8_t* band, int8_t* sgn, int* cha
 }
 
 // Exclusive prefix sum of flags (nonzero counts 1) over n items, one block of 1024.
-__global__ __launch_bounds__(1024) void k_scan(const uint8_t* flags, long n, uint32_t* out, uint32_t* total) {
-  __shared__ uint32_t s[1024];
-  const int tid = threadIdx.x;
-  uint32_t base = 0;
-  for (long c0 = 0; c0 < n; c0 += 1024) {
-    long i = c0 + tid;
-    uint32_t f = (i < n && flags[i]) ? 1u : 0u;
-    s[tid] = f;
+// The same sweeps, x then y then z, repeated until a round changes nothing (at most
+// SIGN_ROUNDS rounds), by one block with the level's bricks in LDS (one byte each:
+// the band bit 4, the sign + 1 in bits 0-1), for levels that fit SIGN_LDS bytes. Rows
+// along x are padded to an odd number of dwords, and the lines of a sweep are dealt
+// x fastest (or y, for the x sweep), so a wave's lanes read distinct LDS banks.
+constexpr int SIGN_T = 1024, SIGN_LDS = 96 * 1024, SIGN_ROUNDS = 8;
+__host__ __device__ inline long sign_pitch(int nb0) { return 4L * (2L * ((nb0 + 7) / 8) + 1); }
+__global__ __launch_bounds__(SIGN_T) void k_sign_block(Lv lv, const uint8_t* band, int8_t* sgn) {
+  __shared__ uint8_t g[SIGN_LDS];
+  __shared__ int changed;
+  const int nb[3] = {lv.nb[0], lv.nb[1], lv.nb[2]};
+  const int P = int(sign_pitch(lv.nb[0]));
+  const int rows = nb[1] * nb[2];  // row r = y + nb[1] z: nb[0] bricks along x
+  for (int r = threadIdx.x; r < rows; r += SIGN_T)
+    for (int x = 0; x < nb[0]; ++x) {
+      const long i = long(r) * nb[0] + x;
+      g[P * r + x] = uint8_t((band[i] ? 16 : 0) | (sgn[i] + 1));
+    }
+  const int stride[3] = {1, P, P * nb[1]};
+  for (int round = 0; round < SIGN_ROUNDS; ++round) {
+    if (threadIdx.x == 0) changed = 0;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      uint32_t v = tid >= off ? s[tid - off] : 0u;
-      __syncthreads();
-      s[tid] += v;
+    int any = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+      // the two other axes, the faster one first
+      const int f = ax == 0 ? 1 : 0, o = ax == 2 ? 1 : 2;
+      for (int line = threadIdx.x; line < nb[f] * nb[o]; line += SIGN_T) {
+        const int base = (line % nb[f]) * stride[f] + (line / nb[f]) * stride[o];
+        for (int pass = 0; pass < 2; ++pass) {
+          int carry = 0;
+          bool carry_ok = false;
+          const int step = pass == 0 ? stride[ax] : -stride[ax];
+          int id = pass == 0 ? base : base + (nb[ax] - 1) * stride[ax];
+          for (int k = 0; k < nb[ax]; ++k, id += step) {
+            const uint8_t v = g[id];
+            int sg = int(v & 3) - 1;
+            const bool nonband = !(v & 16);
+            if (sg == 0 && carry != 0 && (carry_ok || nonband)) {
+              sg = carry;
+              g[id] = uint8_t((v & 16) | (sg + 1));
+              any = 1;
+            }
+            carry = sg;
+            carry_ok = nonband;
+          }
+        }
+      }
       __syncthreads();
     }
-    if (i < n) out[i] = base + s[tid] - f;
-    uint32_t tot = s[1023];
+    if (any) changed = 1;
     __syncthreads();
-    base += tot;
+    const int c = changed;
+    __syncthreads();
+    if (!c) break;
   }
-  if (tid == 0) *total = base;
+  for (int r = threadIdx.x; r < rows; r += SIGN_T)
+    for (int x = 0; x < nb[0]; ++x) sgn[long(r) * nb[0] + x] = int8_t(int(g[P * r + x] & 3) - 1);
 }
 
-__global__ void k_band_list(const uint8_t* band, const uint32_t* pos, long n, uint32_t* list) {
-  long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n && band[i]) list[pos[i]] = uint32_t(i);
+// Positions of the bricks a predicate keeps, in order, by one block: each thread
+// counts a run of consecutive entries (a multiple of 16, read 16 bytes at a time), the
+// block scans the counts, and each thread writes its run's positions. Kept entries
+// get out[i] (their rank) and, when list is not NULL, list[rank] = i; total gets the
+// count. The predicate: band[i] != 0 (band), or cls[i] == 0 (cls: a band brick stored
+// with words).
+constexpr int SCAN_T = 1024;
+__global__ __launch_bounds__(SCAN_T) void k_scan(const uint8_t* band, const uint32_t* cls, long n, uint32_t* out, uint32_t* list, uint32_t* total) {
+  __shared__ uint32_t s[SCAN_T];
+  const int tid = threadIdx.x;
+  const long run = ((n + SCAN_T - 1) / SCAN_T + 15) / 16 * 16, i0 = tid * run, i1 = i0 + run < n ? i0 + run : n;
+  // the run's keep bits, 16 entries a word
+  auto bits16 = [&](long i) -> uint32_t {
+    uint32_t m = 0;
+    if (band && i + 16 <= n && (reinterpret_cast<uintptr_t>(band + i) & 15) == 0) {
+      const uint4 v = *reinterpret_cast<const uint4*>(band + i);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+      for (int q = 0; q < 16; ++q) m |= ((w[q >> 2] >> (8 * (q & 3))) & 0xffu) ? (1u << q) : 0u;
+      return m;
+    }
+    for (int q = 0; q < 16 && i + q < n; ++q) m |= (band ? band[i + q] != 0 : cls[i + q] == 0u) ? (1u << q) : 0u;
+    return m;
+  };
+  uint32_t c = 0;
+  for (long i = i0; i < i1; i += 16) c += __popc(bits16(i));
+  s[tid] = c;
+  __syncthreads();
+  for (int off = 1; off < SCAN_T; off <<= 1) {
+    uint32_t v = tid >= off ? s[tid - off] : 0u;
+    __syncthreads();
+    s[tid] += v;
+    __syncthreads();
+  }
+  uint32_t pos = s[tid] - c;
+  for (long i = i0; i < i1; i += 16) {
+    uint32_t m = bits16(i);
+    while (m) {
+      const int q = __ffs(m) - 1;
+      m &= m - 1;
+      if (out) out[i + q] = pos;
+      if (list) list[pos] = uint32_t(i + q);
+      ++pos;
+    }
+  }
+  if (tid == SCAN_T - 1) *total = s[SCAN_T - 1];
 }
 
 __device__ inline int nb26(int n, int k) {
@@ -857,10 +935,6 @@ __global__ __launch_bounds__(WG) void k_bricks_finish(Lv lv, const uint32_t* lis
     for (int w = tid; w < ICAM_WORDS; w += WG) staging[size_t(bi) * ICAM_WORDS + w] = uint32_t(uint16_t(q[2 * w])) | (uint32_t(uint16_t(q[2 * w + 1])) << 16);
 }
 
-__global__ void k_flags_words(const uint32_t* cls, long n, uint8_t* f) {
-  long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) f[i] = cls[i] == 0;
-}
 
 // partcache.rs:128-146 assemble: the map (AIR, SOLID, or the slot) and the atlas in
 // map order (band bricks are listed in map order, so their slots are too). A brick
@@ -928,7 +1002,7 @@ struct icam_hip {
   } centres, nodes;
   Buf q_xyz, q_bound, q_d, q_flag, q_feat;
   // the level
-  Buf dc, band, sgn, pos, total, list, staging, cls, wflags, slot, stotal, map, atlas, counters, changed, firsts, tile_cand, tile_count;
+  Buf dc, band, sgn, total, list, staging, cls, slot, stotal, map, atlas, counters, changed, firsts, tile_cand, tile_count;
   std::vector<std::pair<const char*, double>> stages;
   std::string name;
 };
@@ -1081,8 +1155,8 @@ void icam_hip_close(icam_hip* h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   h->centres.free();
   h->nodes.free();
-  for (Buf* b : {&h->F32, &h->F64, &h->U, &h->pbox, &h->pcode, &h->q_xyz, &h->q_bound, &h->q_d, &h->q_flag, &h->q_feat, &h->dc, &h->band, &h->sgn, &h->pos, &h->total,
-                 &h->list, &h->staging, &h->cls, &h->wflags, &h->slot, &h->stotal, &h->map, &h->atlas, &h->counters, &h->changed, &h->firsts,
+  for (Buf* b : {&h->F32, &h->F64, &h->U, &h->pbox, &h->pcode, &h->q_xyz, &h->q_bound, &h->q_d, &h->q_flag, &h->q_feat, &h->dc, &h->band, &h->sgn, &h->total,
+                 &h->list, &h->staging, &h->cls, &h->slot, &h->stotal, &h->map, &h->atlas, &h->counters, &h->changed, &h->firsts,
                  &h->tile_cand, &h->tile_count})
     b->free();
   if (h->stream) hipStreamDestroy(h->stream);
@@ -1211,7 +1285,6 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   HIP_OK(h->dc.need(nb * 8));
   HIP_OK(h->band.need(nb));
   HIP_OK(h->sgn.need(nb));
-  HIP_OK(h->pos.need(nb * 4));
   HIP_OK(h->total.need(16));
   HIP_OK(h->list.need(nb * 4));
   HIP_OK(h->map.need(nb * 4));
@@ -1233,7 +1306,9 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   HIP_OK(launch(&st, "decide", s, gb, dim3(WG), k_decide_finish, h->dv, lv, nb, (const double*)r.val, (const uint32_t*)r.flag, h->dc.as<double>(),
                 h->band.as<uint8_t>(), h->sgn.as<int8_t>()));
   // signs of the bricks kept whole: along x, y, z lines, until nothing changes
-  for (int round = 0; round < 8; ++round) {
+  if (sign_pitch(lv.nb[0]) * lv.nb[1] * lv.nb[2] <= SIGN_LDS) {
+    HIP_OK(launch(&st, "sign", s, dim3(1), dim3(SIGN_T), k_sign_block, lv, (const uint8_t*)h->band.as<uint8_t>(), h->sgn.as<int8_t>()));
+  } else for (int round = 0; round < SIGN_ROUNDS; ++round) {
     HIP_OK(hipMemsetAsync(h->changed.p, 0, 4, s));
     for (int ax = 0; ax < 3; ++ax) {
       long lines = long(lv.nb[(ax + 1) % 3]) * lv.nb[(ax + 2) % 3];
@@ -1245,16 +1320,13 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     HIP_OK(hipStreamSynchronize(s));
     if (!ch) break;
   }
-  HIP_OK(launch(&st, "scan", s, dim3(1), dim3(1024), k_scan, (const uint8_t*)h->band.as<uint8_t>(), nb, h->pos.as<uint32_t>(),
-                h->total.as<uint32_t>()));
-  HIP_OK(launch(&st, "band_list", s, gb, dim3(WG), k_band_list, (const uint8_t*)h->band.as<uint8_t>(), (const uint32_t*)h->pos.as<uint32_t>(), nb,
-                h->list.as<uint32_t>()));
+  HIP_OK(launch(&st, "band_list", s, dim3(1), dim3(SCAN_T), k_scan, (const uint8_t*)h->band.as<uint8_t>(), (const uint32_t*)nullptr, nb,
+                (uint32_t*)nullptr, h->list.as<uint32_t>(), h->total.as<uint32_t>()));
   uint32_t nband = 0;
   HIP_OK(hipMemcpyAsync(&nband, h->total.p, 4, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   HIP_OK(h->staging.need(std::max<size_t>(size_t(nband) * ICAM_WORDS * 4, 16)));
   HIP_OK(h->cls.need(std::max<size_t>(size_t(nband) * 4, 16)));
-  HIP_OK(h->wflags.need(std::max<size_t>(nband, 16)));
   HIP_OK(h->slot.need(std::max<size_t>(size_t(nband) * 4, 16)));
   HIP_OK(h->stotal.need(16));
   if (nband) {
@@ -1269,10 +1341,8 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     HIP_OK(launch(&st, "bricks_finish", s, dim3(nband), dim3(WG), k_bricks_finish, lv, (const uint32_t*)h->list.as<uint32_t>(), long(nband),
                   (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>(), (const double*)rn.val, (const uint32_t*)rn.flag,
                   h->staging.as<uint32_t>(), h->cls.as<uint32_t>(), h->counters.as<unsigned long long>()));
-    HIP_OK(launch(&st, "word_flags", s, dim3(unsigned((nband + 255) / 256)), dim3(256), k_flags_words, (const uint32_t*)h->cls.as<uint32_t>(),
-                  long(nband), h->wflags.as<uint8_t>()));
-    HIP_OK(launch(&st, "scan", s, dim3(1), dim3(1024), k_scan, (const uint8_t*)h->wflags.as<uint8_t>(), long(nband), h->slot.as<uint32_t>(),
-                  h->stotal.as<uint32_t>()));
+    HIP_OK(launch(&st, "slots", s, dim3(1), dim3(SCAN_T), k_scan, (const uint8_t*)nullptr, (const uint32_t*)h->cls.as<uint32_t>(), long(nband),
+                  h->slot.as<uint32_t>(), (uint32_t*)nullptr, h->stotal.as<uint32_t>()));
   } else {
     HIP_OK(hipMemsetAsync(h->stotal.p, 0, 4, s));
   }
