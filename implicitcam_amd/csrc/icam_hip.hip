@@ -85,8 +85,10 @@ struct Lv {
   int nb[3];
 };
 
+// (A level has fewer than 2^31 bricks: brick indices are taken apart in 32 bits.)
 __device__ inline v3<double> brick_first(const Lv& lv, const double* origin, long b) {
-  long x = b % lv.nb[0], y = (b / lv.nb[0]) % lv.nb[1], z = b / (long(lv.nb[0]) * lv.nb[1]);
+  const uint32_t ub = uint32_t(b), n0 = uint32_t(lv.nb[0]), n1 = uint32_t(lv.nb[1]);
+  const uint32_t yz = ub / n0, x = ub - yz * n0, z = yz / n1, y = yz - z * n1;
   double side = 8.0 * lv.h;
   return mk3(lv.lo[0] - origin[0] + side * x, lv.lo[1] - origin[1] + side * y, lv.lo[2] - origin[2] + side * z);
 }
@@ -116,6 +118,17 @@ struct Job {
   uint32_t* tile_count;   // per tile (more than TCAP: not kept)
 };
 
+// The first node and the cell side of brick bi of a nodes job, and its n-th node.
+__device__ inline v3<double> node_first(const Job& j, const Dev& dv, long bi, double& h) {
+  if (j.list) {
+    h = j.lv.h;
+    return brick_first(j.lv, dv.origin, long(j.list[bi]));
+  }
+  h = j.firsts[4 * bi + 3];
+  return mk3(j.firsts[4 * bi] - dv.origin[0], j.firsts[4 * bi + 1] - dv.origin[1], j.firsts[4 * bi + 2] - dv.origin[2]);
+}
+__device__ inline v3<double> node_at(v3<double> first, double h, int n) { return first + mk3(h * (n % 9), h * ((n / 9) % 9), h * (n / 81)); }
+
 // Point i of a job (relative to the origin) and the distance within which it must
 // be exact.
 template <int S> __device__ inline v3<double> job_point(const Job& j, const Dev& dv, long i, double& bound) {
@@ -124,19 +137,12 @@ template <int S> __device__ inline v3<double> job_point(const Job& j, const Dev&
     return mk3(j.xyz[3 * i] - dv.origin[0], j.xyz[3 * i + 1] - dv.origin[1], j.xyz[3 * i + 2] - dv.origin[2]);
   }
   if (S == SRC_NODES) {
-    long bi = i / 729;
-    int n = int(i - 729 * bi);
-    v3<double> first;
+    const long bi = long(uint64_t(i) / 729u);
+    const int n = int(i - 729 * bi);
     double h;
-    if (j.list) {
-      first = brick_first(j.lv, dv.origin, long(j.list[bi]));
-      h = j.lv.h;
-    } else {
-      first = mk3(j.firsts[4 * bi] - dv.origin[0], j.firsts[4 * bi + 1] - dv.origin[1], j.firsts[4 * bi + 2] - dv.origin[2]);
-      h = j.firsts[4 * bi + 3];
-    }
+    const v3<double> first = node_first(j, dv, bi, h);
     bound = h * 1.7320508075688772;
-    return first + mk3(h * (n % 9), h * ((n / 9) % 9), h * (n / 81));
+    return node_at(first, h, n);
   }
   double h = j.lv.h;
   bound = brick_reach(h) * (1.0 + 1e-9);
@@ -158,6 +164,9 @@ struct Res {
   uint32_t* queue;    // blocks holding points the single-precision search was unsure of
   uint32_t* qcount;
   uint32_t* qmark;    // per block: queued already
+  uint32_t* items;    // points whose parts wait for the dense pass (item_words words each)
+  uint32_t* icount;
+  uint32_t icap;
   unsigned long long* counters;  // [0] bridged nodes, [1] points searched again in double,
                                  // [3] of them near a trim curve, [4] near-ties of opposite sign
 };
@@ -286,8 +295,40 @@ template <int K> __device__ inline int group_or(int v) {
   for (int o = 1; o < K; o <<= 1) v |= __shfl_xor(v, o);
   return v;
 }
+// A point whose parts under its bound number 1 .. QI can leave its rounds to the
+// dense pass (k_leaves): its answer so far and its parts, sorted by lower bound (equal
+// bounds in list order), go to a work item, and its Best comes back with code DEFERRED.
+// Item words: 0 point, 1 parts, 2 code, 3 unsure; from 4 d, a, b, sgn, lmin, d2, sgn2;
+// then the parts' lower bounds (T) and primitives.
+constexpr int QI = 8;
+constexpr uint32_t DEFERRED = 0xfffffffeu;
+template <class T> struct Item {
+  static constexpr int W = sizeof(T) == 4 ? 32 : 48;                 // words an item
+  static constexpr int LB = 4 + 7 * int(sizeof(T) / 4);
+  static constexpr int PR = LB + QI * int(sizeof(T) / 4);
+  static_assert(PR + QI <= W, "an item holds its parts");
+  static_assert(sizeof(T) == 4 || LB % 2 == 0, "double words aligned");
+};
+template <class T> __device__ inline void item_put(uint32_t* w, int off, T v) { *reinterpret_cast<T*>(w + off) = v; }
+template <class T> __device__ inline T item_get(const uint32_t* w, int off) { return *reinterpret_cast<const T*>(w + off); }
+template <class T> __device__ inline void item_state(uint32_t* w, const Best<T>& b) {
+  constexpr int u = int(sizeof(T) / 4);
+  w[2] = b.code, w[3] = b.unsure ? 1u : 0u;
+  item_put(w, 4, b.d), item_put(w, 4 + u, b.a), item_put(w, 4 + 2 * u, b.b), item_put(w, 4 + 3 * u, b.sgn);
+  item_put(w, 4 + 4 * u, b.lmin), item_put(w, 4 + 5 * u, b.d2), item_put(w, 4 + 6 * u, b.sgn2);
+}
+template <class T> __device__ inline Best<T> item_best(const uint32_t* w) {
+  constexpr int u = int(sizeof(T) / 4);
+  Best<T> b;
+  b.code = w[2], b.unsure = w[3] != 0;
+  b.d = item_get<T>(w, 4), b.a = item_get<T>(w, 4 + u), b.b = item_get<T>(w, 4 + 2 * u), b.sgn = item_get<T>(w, 4 + 3 * u);
+  b.lmin = item_get<T>(w, 4 + 4 * u), b.d2 = item_get<T>(w, 4 + 5 * u), b.sgn2 = item_get<T>(w, 4 + 6 * u);
+  return b;
+}
+
 template <class T, int NT, int Q, int K, int C>
-__device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk) {
+__device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk,
+                          const Res* defer = nullptr, long point = 0) {
   Best<T> b;
   b.d = bound, b.code = NONE, b.a = b.b = T(0), b.sgn = T(1), b.lmin = TypeEps<T>::big, b.d2 = TypeEps<T>::big, b.sgn2 = T(1), b.unsure = false;
   const bool all = nc > C;
@@ -323,6 +364,25 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const 
     }
   }
   const bool scan = parts && (all || m > Q);
+  if (K == 1 && defer && !scan && m > 0 && m <= QI) {
+    uint32_t slot = atomicAdd(defer->icount, 1u);
+    if (slot < defer->icap) {
+      // the lane's parts by lower bound, equal bounds in list order
+      for (int q = 1; q < m; ++q) {
+        T v = qlb[q * NT + lane];
+        uint16_t kv = qk[q * NT + lane];
+        int r = q;
+        for (; r > 0 && qlb[(r - 1) * NT + lane] > v; --r) qlb[r * NT + lane] = qlb[(r - 1) * NT + lane], qk[r * NT + lane] = qk[(r - 1) * NT + lane];
+        qlb[r * NT + lane] = v, qk[r * NT + lane] = kv;
+      }
+      uint32_t* w = defer->items + size_t(slot) * Item<T>::W;
+      w[0] = uint32_t(point), w[1] = uint32_t(m);
+      item_state(w, b);
+      for (int q = 0; q < m; ++q) item_put(w, Item<T>::LB + q * int(sizeof(T) / 4), qlb[q * NT + lane]), w[Item<T>::PR + q] = cand[qk[q * NT + lane]];
+      b.code = DEFERRED;
+      return b;
+    }
+  }
   T last_lb = T(-1);
   uint32_t last_k = 0;
   for (int round = 0; round < 4096; ++round) {
@@ -462,6 +522,27 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
   constexpr int NT = Shape<S>::NT;
   float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
   float r = 0.f;
+  if (S != SRC_POINTS) {
+    // the box of a brick's nodes or of a tile's centres from its two extreme points
+    // (coordinates grow with the index), computed as each point's own
+    double bound;
+    v3<double> a, b;
+    if (S == SRC_NODES) {
+      double h;
+      const v3<double> first = node_first(j, dv, blk, h);
+      a = node_at(first, h, 0), b = node_at(first, h, 728);
+      bound = h * 1.7320508075688772;
+    } else {
+      const long tx = (j.lv.nb[0] + TILE_X - 1) / TILE_X, ty = (j.lv.nb[1] + TILE_Y - 1) / TILE_Y;
+      const long x0 = TILE_X * (blk % tx), y0 = TILE_Y * ((blk / tx) % ty), z0 = TILE_Z * (blk / (tx * ty));
+      const long x1 = min(x0 + TILE_X, long(j.lv.nb[0])) - 1, y1 = min(y0 + TILE_Y, long(j.lv.nb[1])) - 1, z1 = min(z0 + TILE_Z, long(j.lv.nb[2])) - 1;
+      a = job_point<S>(j, dv, x0 + j.lv.nb[0] * (y0 + long(j.lv.nb[1]) * z0), bound);
+      b = job_point<S>(j, dv, x1 + j.lv.nb[0] * (y1 + long(j.lv.nb[1]) * z1), bound);
+    }
+    lo[0] = float(a.x), lo[1] = float(a.y), lo[2] = float(a.z);
+    hi[0] = float(b.x), hi[1] = float(b.y), hi[2] = float(b.z);
+    r = float(fmin(bound, 3.0e38)) * (1.0f + 1e-5f) + 1e-4f;
+  } else {
   for (int m = 0; m < rounds<S>(); ++m) {
     long i = block_point<S>(j, blk, m);
     if (i < 0) continue;
@@ -477,6 +558,7 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
     r = fmaxf(r, ub * (1.0f + 1e-5f) + 1e-4f);
   }
   block_box<NT>(lo, hi, r, red);
+  }
   float gb[6] = {lo[0] - 1e-4f, lo[1] - 1e-4f, lo[2] - 1e-4f, hi[0] + 1e-4f, hi[1] + 1e-4f, hi[2] + 1e-4f};
   int nc;
   if (S == SRC_NODES && j.tile_cand && j.list) {
@@ -497,8 +579,35 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
 
 // ------------------------------------------------------------------ search kernels
 
+// A point's finished search: in single precision (T = float) the records k_refine
+// reads, in double precision the final answer.
+template <class T>
+__device__ inline void finish_point(const Dev& dv, const Res& res, long i, double bound, const Best<T>& b, T eps) {
+  if constexpr (sizeof(T) == 4) {
+    bool tie = b.d2 - b.d <= T(8) * eps && b.sgn2 != b.sgn;
+    uint32_t info = (b.code != NONE ? 1u : 0u) | (b.sgn < T(0) ? 2u : 0u) | ((b.unsure || tie) ? 4u : 0u) | ((b.lmin < b.d) ? 8u : 0u) |
+                    (b.unsure ? 16u : 0u) | (tie ? 32u : 0u);
+    res.code[i] = b.code;
+    res.pa[i] = float(b.a);
+    res.pb[i] = float(b.b);
+    res.d32[i] = float((b.lmin < b.d) ? b.lmin : b.d);
+    res.info[i] = info;
+  } else {
+    double val;
+    uint32_t flag, feat;
+    classify64(b, bound, dv, val, flag, feat);
+    res.val[i] = val;
+    res.flag[i] = flag;
+    if (res.feat) res.feat[i] = feat;
+  }
+}
+template <class T> __device__ inline T search_bound(double bound, T eps) {
+  if constexpr (sizeof(T) == 4) return bound < 3.0e38 ? float(bound) * (1.0f + 1e-6f) + float(eps) : 3.0e38f;
+  return bound < 1e300 ? bound * (1.0 + 1e-12) : bound;
+}
+
 // The search, in single (T = float: records for k_refine) or double precision (T =
-// double: final answers).
+// double: final answers). A brick's nodes leave their parts' rounds to k_leaves.
 template <int S, class T>
 __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res) {
   constexpr int NT = Shape<S>::NT;
@@ -518,34 +627,46 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res)
   const T tm = sizeof(T) == 4 ? T(16) * eps : T(0);
   constexpr int K = Shape<S>::K;
   const bool owner = (threadIdx.x & (K - 1)) == 0;
+  double nh = 0.0;
+  const v3<double> nfirst = S == SRC_NODES ? node_first(j, dv, blockIdx.x, nh) : mk3(0.0, 0.0, 0.0);
   for (int m = 0; m < rounds<S>(); ++m) {
     long i = block_point<S>(j, blockIdx.x, m);
     if (K == 1 && i < 0) continue;
     const bool valid = i >= 0;
     double bound = 0.0;
-    v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
-    if constexpr (sizeof(T) == 4) {
-      float b32 = bound < 3.0e38 ? float(bound) * (1.0f + 1e-6f) + float(eps) : 3.0e38f;
-      Best<T> b = search<T, NT, Shape<S>::Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, T(b32), cand, nc, tm, eps, qlb, qk);
-      if (!valid || !owner) continue;
-      bool tie = b.d2 - b.d <= T(8) * eps && b.sgn2 != b.sgn;
-      uint32_t info = (b.code != NONE ? 1u : 0u) | (b.sgn < T(0) ? 2u : 0u) | ((b.unsure || tie) ? 4u : 0u) | ((b.lmin < b.d) ? 8u : 0u) |
-                      (b.unsure ? 16u : 0u) | (tie ? 32u : 0u);
-      res.code[i] = b.code;
-      res.pa[i] = float(b.a);
-      res.pb[i] = float(b.b);
-      res.d32[i] = float((b.lmin < b.d) ? b.lmin : b.d);
-      res.info[i] = info;
-    } else {
-      Best<T> b = search<T, NT, Shape<S>::Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, bound < 1e300 ? T(bound * (1.0 + 1e-12)) : T(bound), cand, nc, tm, eps, qlb, qk);
-      if (!valid || !owner) continue;
-      double val;
-      uint32_t flag, feat;
-      classify64(b, bound, dv, val, flag, feat);
-      res.val[i] = val;
-      res.flag[i] = flag;
-      if (res.feat) res.feat[i] = feat;
+    v3<double> p;
+    if (S == SRC_NODES)
+      p = node_at(nfirst, nh, int(threadIdx.x) + NT * m), bound = nh * 1.7320508075688772;
+    else
+      p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
+    Best<T> b = search<T, NT, Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, search_bound<T>(bound, eps), cand, nc, tm, eps, qlb, qk,
+                                                 S == SRC_NODES && res.items ? &res : nullptr, i);
+    if (!valid || !owner || b.code == DEFERRED) continue;
+    finish_point<T>(dv, res, i, bound, b, eps);
+  }
+}
+
+// The dense pass over the deferred points: each lane one point, its parts in order of
+// their lower bounds while one can beat its answer.
+template <int S, class T>
+__global__ __launch_bounds__(256) void k_leaves(Dev dv, Job j, Res res) {
+  const uint32_t n = min(*res.icount, res.icap);
+  const Tab<T> tb = tab_of<T>(dv);
+  const T eps = sizeof(T) == 4 ? T(dv.L.eps32) : T(dv.L.eps64);
+  const T tm = sizeof(T) == 4 ? T(16) * eps : T(0);
+  for (uint32_t it = blockIdx.x * 256 + threadIdx.x; it < n; it += gridDim.x * 256) {
+    const uint32_t* w = res.items + size_t(it) * Item<T>::W;
+    const long i = long(w[0]);
+    const int m = int(w[1]);
+    Best<T> b = item_best<T>(w);
+    double bound;
+    const v3<T> p = cv3<T>(job_point<S>(j, dv, i, bound));
+    for (int q = 0; q < m; ++q) {
+      const T lb = item_get<T>(w, Item<T>::LB + q * int(sizeof(T) / 4));
+      if (!(lb < b.d)) break;
+      try_part(tb, w[Item<T>::PR + q], lb, p, b, tm, eps);
     }
+    finish_point<T>(dv, res, i, bound, b, eps);
   }
 }
 
@@ -995,9 +1116,9 @@ struct icam_hip {
   Buf F32, F64, U, pbox, pcode;
   // per-point results of the centres' and the nodes' searches, and host-call staging
   struct ResBufs {
-    Buf code, pa, pb, d32, info, val, flag, queue, qcount, qmark;
+    Buf code, pa, pb, d32, info, val, flag, queue, qcount, qmark, items, icount;
     void free() {
-      for (Buf* b : {&code, &pa, &pb, &d32, &info, &val, &flag, &queue, &qcount, &qmark}) b->free();
+      for (Buf* b : {&code, &pa, &pb, &d32, &info, &val, &flag, &queue, &qcount, &qmark, &items, &icount}) b->free();
     }
   } centres, nodes;
   Buf q_xyz, q_bound, q_d, q_flag, q_feat;
@@ -1060,7 +1181,7 @@ int upload(Buf& b, const void* src, size_t bytes) {
 
 // Result buffers for n points from one of the handle's two sets (centres or nodes);
 // val, flag and feat may be the caller's.
-int results(icam_hip* h, bool nodes, long n, Res& r, double* val, uint32_t* flag, uint32_t* feat, bool mixed) {
+int results(icam_hip* h, bool nodes, long n, Res& r, double* val, uint32_t* flag, uint32_t* feat, bool mixed, bool items = false) {
   size_t m = size_t(std::max<long>(n, 1));
   icam_hip::ResBufs& B = nodes ? h->nodes : h->centres;
   Buf &code = B.code, &pa = B.pa, &pb = B.pb, &d32 = B.d32, &info = B.info, &bval = B.val, &bflag = B.flag, &queue = B.queue, &qcount = B.qcount;
@@ -1083,6 +1204,14 @@ int results(icam_hip* h, bool nodes, long n, Res& r, double* val, uint32_t* flag
   r.flag = flag ? flag : bflag.as<uint32_t>();
   r.feat = feat;
   r.counters = h->counters.as<unsigned long long>();
+  r.items = nullptr, r.icount = nullptr, r.icap = 0;
+  if (items) {
+    // room for an eighth of the points; points past it run their rounds in place
+    const size_t cap = m / 8 + 1024;
+    HIP_OK(B.items.need(cap * Item<double>::W * 4));
+    HIP_OK(B.icount.need(16));
+    r.items = B.items.as<uint32_t>(), r.icount = B.icount.as<uint32_t>(), r.icap = uint32_t(cap);
+  }
   return 0;
 }
 
@@ -1092,18 +1221,23 @@ template <int S>
 int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Stages* st, long blocks) {
   if (j.n <= 0) return 0;
   constexpr int NT = Shape<S>::NT;
-  static const char* names[3][4] = {{"points_search32", "points_refine64", "points_fallback64", "points_search64"},
-                                    {"nodes_search32", "nodes_refine64", "nodes_fallback64", "nodes_search64"},
-                                    {"centres_search32", "centres_refine64", "centres_fallback64", "centres_search64"}};
+  static const char* names[3][6] = {{"points_search32", "points_refine64", "points_fallback64", "points_search64", "points_leaves32", "points_leaves64"},
+                                    {"nodes_search32", "nodes_refine64", "nodes_fallback64", "nodes_search64", "nodes_leaves32", "nodes_leaves64"},
+                                    {"centres_search32", "centres_refine64", "centres_fallback64", "centres_search64", "centres_leaves32",
+                                     "centres_leaves64"}};
+  const unsigned lb = r.items ? unsigned(std::min<long>((long(r.icap) + 255) / 256, 2048)) : 0;
+  if (r.items) HIP_OK(hipMemsetAsync(r.icount, 0, 4, s));
   if (mixed) {
     HIP_OK(hipMemsetAsync(r.qcount, 0, 4, s));
     HIP_OK(hipMemsetAsync(r.qmark, 0, size_t(blocks) * 4, s));
     HIP_OK(launch(st, names[S][0], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, float>, h->dv, j, r));
+    if (lb) HIP_OK(launch(st, names[S][4], s, dim3(lb), dim3(256), k_leaves<S, float>, h->dv, j, r));
     HIP_OK(launch(st, names[S][1], s, dim3(unsigned((j.n + 255) / 256)), dim3(256), k_refine<S>, h->dv, j, r));
     unsigned fb = unsigned(std::min<long>(blocks, 2048));
     HIP_OK(launch(st, names[S][2], s, dim3(fb), dim3(NT), k_fallback<S>, h->dv, j, r));
   } else {
     HIP_OK(launch(st, names[S][3], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, double>, h->dv, j, r));
+    if (lb) HIP_OK(launch(st, names[S][5], s, dim3(lb), dim3(256), k_leaves<S, double>, h->dv, j, r));
   }
   return 0;
 }
@@ -1261,7 +1395,7 @@ int icam_hip_bricks(icam_hip* h, int precision, const icam_brick_rec* recs, size
   HIP_OK(hipMemcpyAsync(h->firsts.p, recs, n * 32, hipMemcpyHostToDevice, h->stream));
   Res r{};
   long np = long(n) * 729;
-  if (results(h, true, np, r, nullptr, nullptr, nullptr, precision == ICAM_MIXED)) return 1;
+  if (results(h, true, np, r, nullptr, nullptr, nullptr, precision == ICAM_MIXED, true)) return 1;
   Job j{};
   j.n = np, j.firsts = h->firsts.as<double>();
   if (run_search<SRC_NODES>(h, precision == ICAM_MIXED, j, r, h->stream, nullptr, long(n))) return 1;
@@ -1279,6 +1413,7 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   for (int k = 0; k < 3; ++k) lv.lo[k] = lvp->lo[k], lv.nb[k] = lvp->nb[k];
   lv.h = lvp->h;
   if (!(lv.h > 0) || lv.nb[0] <= 0 || lv.nb[1] <= 0 || lv.nb[2] <= 0) return fail("bad level");
+  if (double(lv.nb[0]) * lv.nb[1] * lv.nb[2] >= 2147483648.0) return fail("a level holds fewer than 2^31 bricks");
   if (precision != ICAM_FP64 && precision != ICAM_MIXED) return fail("unknown precision");
   const bool mixed = precision == ICAM_MIXED;
   long nb = long(lv.nb[0]) * lv.nb[1] * lv.nb[2];
@@ -1333,7 +1468,7 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     // the band bricks' nodes
     Res rn{};
     long np = long(nband) * 729;
-    if (results(h, true, np, rn, nullptr, nullptr, nullptr, mixed)) return 1;
+    if (results(h, true, np, rn, nullptr, nullptr, nullptr, mixed, true)) return 1;
     Job jn{};
     jn.n = np, jn.list = h->list.as<uint32_t>(), jn.lv = lv;
     jn.tile_cand = h->tile_cand.as<uint32_t>(), jn.tile_count = h->tile_count.as<uint32_t>();
