@@ -181,3 +181,35 @@ def test_points_and_bricks_api(c1, prec):
         assert (nodes[b, inside, 1] == 0).all()
         assert np.abs(nodes[b, inside, 0] - dr[inside]).max() <= 1e-9
         assert (nodes[b, np.abs(dr) > clamp + 1e-9, 1] == 1).all()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_golden_vectors(prec):
+    """The committed golden vectors (tests/golden/oracle_vectors.npz): distances, signs
+    and nearest features at the seeded points, and the c1 level's map and atlas."""
+    import os
+
+    from implicitcam_amd import _native
+
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_vectors.npz"))
+    for name, need in (("can", 2.0), ("c1", 12 * 1.6), ("bump", 4.0), ("c2", 6.0)):
+        part = Part(name)
+        g = PartGpu(part, need)
+        pts, dref = G[f"{name}_xyz"], G[f"{name}_d"]
+        d, flags, feat = g.query(pts, prec)
+        ex = flags == 0
+        # the B-spline parts certify within their need; farther points may be left
+        # undecided with a lower bound
+        near = np.abs(dref) < need - 1.0
+        assert ex[near].mean() > (0.99 if name in ("bump", "c2") else 1.0 - 1e-12), name
+        assert (d[~ex] <= np.abs(dref[~ex]) + 1e-9).all(), name
+        assert np.abs(d[ex] - dref[ex]).max() <= tol(prec, part), name
+        assert ((d[ex] < 0) == (dref[ex] < 0)).all(), name
+        if name in ("can", "c1"):
+            assert (feat == G[f"{name}_feat"]).all(), name
+    lv = _native.Level()
+    lv.lo[:] = list(G["c1_level"][:3])
+    lv.h = float(G["c1_level"][3])
+    lv.nb[:] = [int(x) for x in G["c1_level"][4:]]
+    m, a, _ = PartGpu(Part("c1"), 12 * 1.6).bake(lv, prec)
+    assert (m == G["c1_map"]).all() and (a == G["c1_atlas"]).all()
