@@ -25,6 +25,7 @@
 //   "c2flat" the same with every control point at the top's base height: its top
 //           is the plane z = 20 and the part the box [-60, 60]^2 x [-20, 20]
 #include <algorithm>
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -57,6 +58,10 @@ struct Rng {
     s ^= s << 17;
     return double(s >> 11) / double(uint64_t(1) << 53);
   }
+  uint64_t next_u64() {
+    next();
+    return s;
+  }
 };
 
 struct Owned {
@@ -65,6 +70,8 @@ struct Owned {
   std::vector<icam_edge_desc> edges;
   std::vector<icam_vert_desc> verts;
   std::deque<std::vector<double>> arrays;
+  // what the part was made from, for closed forms (icam_gen_describe)
+  std::vector<double> info;
 
   const double* keep(std::vector<double> v) {
     arrays.push_back(std::move(v));
@@ -323,6 +330,10 @@ Owned* make_can() {
   uint32_t cyl = b.analytic(ICAM_CYLINDER, {0, 0, 0}, Z, X, 3.0, 0, 1.0, 0, -1, 4, 6, 4, 0, band, 2.0, 2.5, true);
   b.circle({0, 0, 0}, Z, X, 3.0, bot, cyl);
   b.circle({0, 0, 5}, Z, X, 3.0, top, cyl);
+  // the seam vertices of the two circles (the fixture's vertices 1 and 2), each
+  // between a disc and the cylinder at angle pi in both
+  b.vert({3, 0, 0}, V{1, 0, -1} * M_PI);
+  b.vert({3, 0, 5}, V{1, 0, 1} * M_PI);
   o->finish({-3, -3, 0}, {3, 3, 5}, 1e-9);
   return o;
 }
@@ -476,6 +487,11 @@ Owned* make_c1() {
   b.circle({0, 0, az}, Z, X, R, f_top, f_tor);
   b.circle({0, 0, zc}, Z, X, rh, f_cyl, f_tor);
   b.circle({0, 0, -az}, Z, X, rh, f_cyl, f_bot);
+  // the circles' seam vertices (at angle 0): the fillet meets the top and the hole
+  // tangentially, the hole meets the bottom at a right angle
+  b.vert({R, 0, az}, V{0, 0, 2 * M_PI});
+  b.vert({rh, 0, zc}, V{-2 * M_PI, 0, 0});
+  b.vert({rh, 0, -az}, V{-1, 0, -1} * M_PI);
   double q = M_PI / 2;
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j)
@@ -516,11 +532,11 @@ double value_noise(uint64_t seed, double x, double y) {
   return sum / norm;
 }
 
-Owned* make_c2(uint64_t seed, bool flat) {
+Owned* make_c2(uint64_t seed, bool flat, int n = 16, double half = 60, double amp = 5) {
   Owned* o = new Owned();
   Builder b{o};
-  const int n = 16, p = 3, spans = n - p;
-  const double half = 60, zb = -20, ztop = 20, amp = 5;
+  const int p = 3, spans = n - p;
+  const double zb = -half / 3, ztop = half / 3;
   std::vector<double> kn;
   for (int i = 0; i <= p; ++i) kn.push_back(0);
   for (int i = 1; i < spans; ++i) kn.push_back(double(i) / spans);
@@ -655,24 +671,412 @@ Owned* make_c2(uint64_t seed, bool flat) {
   return o;
 }
 
+// ------------------------------------------------------------- solids of revolution
+
+// The parameter of the exact rational arc form along an angle (the trim domain's
+// coordinate on spheres and tori, trim2.rs:1789-1807): n quarter-or-less arcs over
+// [a0, a0 + span], each mapped to [k, k + 1] by the half-angle tangent.
+double arc_param_over(double a, double a0, double span, int n) {
+  const double TAU = 2.0 * M_PI;
+  double q = span / n, x = a - a0;
+  if (std::fabs(span - TAU) < 1e-12) {
+    x = x - TAU * std::floor(x / TAU);
+  } else {
+    double y = x - TAU * std::round((x - 0.5 * span) / TAU);
+    x = std::min(std::max(y, 0.0), span);
+  }
+  long k = std::min(std::max(long(std::floor(x / q)), 0L), long(n - 1));
+  double alpha = x - q * k - 0.5 * q;
+  double t = 0.5 * (std::tan(0.5 * alpha) / std::tan(0.25 * q) + 1.0);
+  return k + std::min(std::max(t, 0.0), 1.0);
+}
+
+// One element of a profile in the (rho, z) half-plane: a segment, or a circular arc
+// about (rc, zc) of radius rad from angle psi0 to psi1 (angles from the +rho
+// direction towards +z). A profile is a closed loop, counterclockwise with rho as the
+// first coordinate, so the region it bounds is on its left. `face`: the element is
+// a face of the solid (true), or not (an axis segment, or a hole's end cap lying in
+// one of the plate's planes: then `ext` names that plane's face, or is NONE).
+struct Prof {
+  bool arc = false;
+  double r0 = 0, z0 = 0, r1 = 0, z1 = 0;
+  double rc = 0, zc = 0, rad = 0, psi0 = 0, psi1 = 0;
+  bool face = true;
+  uint32_t ext = 0xffffffffu;
+  double ext_nz = 0;  // an end cap's plane: its outward normal's z (+-1)
+};
+Prof seg(double r0, double z0, double r1, double z1) {
+  Prof p;
+  p.r0 = r0, p.z0 = z0, p.r1 = r1, p.z1 = z1;
+  p.face = !(r0 == 0 && r1 == 0);
+  return p;
+}
+Prof arc(double rc, double zc, double rad, double psi0, double psi1) {
+  Prof p;
+  p.arc = true;
+  p.rc = rc, p.zc = zc, p.rad = rad, p.psi0 = psi0, p.psi1 = psi1;
+  p.r0 = rc + rad * std::cos(psi0), p.z0 = zc + rad * std::sin(psi0);
+  p.r1 = rc + rad * std::cos(psi1), p.z1 = zc + rad * std::sin(psi1);
+  if (std::fabs(p.r0) < 1e-12) p.r0 = 0;
+  if (std::fabs(p.r1) < 1e-12) p.r1 = 0;
+  return p;
+}
+
+// The faces and circle edges of a profile revolved about the vertical axis through
+// (cx, cy). hole: the region the profile bounds is empty (a hole in a plate), so the
+// solid's outward normal points into it. The faces each element makes:
+//   horizontal segment: a plane (disc or annulus), frame z the outward normal;
+//   vertical segment: a cylinder;  slanted segment: a cone, its frame's z along the
+//   axis in the direction its radius grows (semi-angle in (0, 90) degrees);
+//   arc about a point of the axis: a sphere;  arc about a point off it: a ring torus.
+// Analytic trim domains as in the part model: (quarter-arc parameter of the angle
+// about the axis, height | latitude arc parameter | tube arc parameter).
+void revolve(Builder& b, double cx, double cy, std::vector<Prof> prof, bool hole) {
+  V X{1, 0, 0}, Z{0, 0, 1};
+  const size_t n = prof.size();
+  std::vector<uint32_t> fid(n, 0xffffffffu);
+  const double flip = hole ? -1.0 : 1.0;
+  for (size_t i = 0; i < n; ++i) {
+    Prof& e = prof[i];
+    if (!e.face) {
+      fid[i] = e.ext;
+      continue;
+    }
+    V o{cx, cy, 0};
+    if (e.arc) {
+      const bool ccw = e.psi1 > e.psi0;
+      double sense = (ccw ? 1.0 : -1.0) * flip;
+      if (e.rc == 0) {
+        // sphere: latitude arc parameter v in [0, 2]
+        double v0 = arc_param_over(e.psi0, -0.5 * M_PI, M_PI, 2), v1 = arc_param_over(e.psi1, -0.5 * M_PI, M_PI, 2);
+        double lo = std::min(v0, v1), hi = std::max(v0, v1);
+        std::vector<Piece> pcs;
+        if (lo > 1e-12) pcs.push_back(line_piece(0, lo, 4, lo));
+        if (hi < 2 - 1e-12) pcs.push_back(line_piece(0, hi, 4, hi));
+        fid[i] = b.analytic(ICAM_SPHERE, V{cx, cy, e.zc}, Z, X, e.rad, 0, sense, 0, -1, 4, 3, 4, 0, pcs, 2.0, 0.5 * (lo + hi), true);
+      } else {
+        // ring torus: tube arc parameter over the whole meridian circle, periodic
+        double pm = 0.5 * (e.psi0 + e.psi1);
+        double v0 = arc_param_over(e.psi0, 0, 2 * M_PI, 4), v1 = arc_param_over(e.psi1, 0, 2 * M_PI, 4);
+        double vm = arc_param_over(pm, 0, 2 * M_PI, 4);
+        std::vector<Piece> pcs = {line_piece(0, v0, 4, v0), line_piece(0, v1, 4, v1)};
+        fid[i] = b.analytic(ICAM_TORUS, V{cx, cy, e.zc}, Z, X, e.rc, e.rad, sense, 0, 0, 4, 4, 4, 4, pcs, 2.0, vm, true);
+      }
+      continue;
+    }
+    const double dr = e.r1 - e.r0, dz = e.z1 - e.z0;
+    // outward normal of the region (its right side) in (rho, z)
+    const double on_r = dz, on_z = -dr;
+    if (dz == 0) {
+      // plane at z0: frame z the solid's outward normal
+      V zn = Z * ((on_z > 0 ? 1.0 : -1.0) * flip);
+      double ra = std::min(e.r0, e.r1), rb = std::max(e.r0, e.r1);
+      std::vector<Piece> pcs;
+      add_circle(pcs, 0, 0, rb);
+      if (ra > 0) add_circle(pcs, 0, 0, ra);
+      fid[i] = b.plane(V{cx, cy, e.z0}, zn, X, 1.0, -rb - 1, -rb - 1, rb + 1, rb + 1, pcs, 0.5 * (ra + rb), 0.0123, true);
+    } else if (dr == 0) {
+      double lo = std::min(e.z0, e.z1), hi = std::max(e.z0, e.z1);
+      std::vector<Piece> pcs = {line_piece(0, lo, 4, lo), line_piece(0, hi, 4, hi)};
+      double sense = (on_r > 0 ? 1.0 : -1.0) * flip;
+      fid[i] = b.analytic(ICAM_CYLINDER, o, Z, X, e.r0, 0, sense, 0, lo - 1, 4, hi + 1, 4, 0, pcs, 2.0, 0.5 * (lo + hi), true);
+    } else {
+      // cone from its narrow end A along zdir
+      const bool a_is_0 = e.r0 < e.r1;
+      const double ra = a_is_0 ? e.r0 : e.r1, za = a_is_0 ? e.z0 : e.z1, zb = a_is_0 ? e.z1 : e.z0;
+      const double zdir = zb > za ? 1.0 : -1.0, len_h = std::fabs(zb - za);
+      const double ang = std::atan(std::fabs(dr) / std::fabs(dz));
+      // the surface's own normal in (rho, z): (cos a, -zdir sin a)
+      const double own_r = std::cos(ang), own_z = -zdir * std::sin(ang);
+      double sense = ((own_r * on_r + own_z * on_z) > 0 ? 1.0 : -1.0) * flip;
+      std::vector<Piece> pcs;
+      if (ra > 0) pcs.push_back(line_piece(0, 0, 4, 0));
+      pcs.push_back(line_piece(0, len_h, 4, len_h));
+      fid[i] = b.analytic(ICAM_CONE, V{cx, cy, za}, Z * zdir, X, ra, ang, sense, 0, -1, 4, len_h + 1, 4, 0, pcs, 2.0, 0.5 * len_h, true);
+      if (ra == 0) {
+        // the apex: a vertex loop of the face, as STEP writes it, and a vertex whose
+        // pseudo-normal is the cone's axis weighted by the full turn (solid3.rs:838-846)
+        V apex{cx, cy, za};
+        icam_face_desc& f = b.o->faces.back();
+        f.n_points = 1;
+        f.points = b.o->keep({apex.x, apex.y, apex.z});
+        b.vert(apex, Z * (zdir * -std::sin(ang) * sense * 2.0 * M_PI));
+      }
+    }
+  }
+  // the solid's outward normal (rho, z) of element e at its start (at_end false) or end
+  auto normal = [&](const Prof& e, bool at_end) {
+    double nr, nz;
+    if (!e.face) return std::array<double, 2>{0.0, e.ext_nz};
+    if (e.arc) {
+      double a = at_end ? e.psi1 : e.psi0, s = e.psi1 > e.psi0 ? 1.0 : -1.0;
+      nr = s * std::cos(a), nz = s * std::sin(a);
+    } else {
+      double dr = e.r1 - e.r0, dz = e.z1 - e.z0, l = std::hypot(dr, dz);
+      nr = dz / l, nz = -dr / l;
+    }
+    return std::array<double, 2>{nr * flip, nz * flip};
+  };
+  // circle edges where consecutive elements with faces meet off the axis, each with
+  // the seam vertex STEP gives a closed edge (at angle 0), its pseudo-normal the
+  // faces' normals weighted by their angle there, pi each (solid3.rs:797-842)
+  for (size_t i = 0; i < n; ++i) {
+    const Prof &e = prof[i], &f = prof[(i + 1) % n];
+    uint32_t a = fid[i], c = fid[(i + 1) % n];
+    if (a == 0xffffffffu || c == 0xffffffffu || (!e.face && !f.face)) continue;
+    if (e.r1 <= 0) continue;
+    b.circle(V{cx, cy, e.z1}, Z, X, e.r1, a, c);
+    auto na = normal(e, true), nc = normal(f, false);
+    b.vert(V{cx + e.r1, cy, e.z1}, V{na[0] + nc[0], 0, na[1] + nc[1]} * M_PI);
+  }
+}
+
+// A solid of revolution about the z axis from its profile; the box from the profile's
+// extremes.
+void describe(std::vector<double>& info, const std::vector<Prof>& prof) {
+  info.push_back(double(prof.size()));
+  for (auto& e : prof) info.insert(info.end(), {e.arc ? 1.0 : 0.0, e.r0, e.z0, e.r1, e.z1, e.rc, e.zc, e.rad, e.psi0, e.psi1, e.face ? 1.0 : 0.0});
+}
+
+Owned* make_revolved(const std::vector<Prof>& prof) {
+  Owned* o = new Owned();
+  Builder b{o};
+  revolve(b, 0, 0, prof, false);
+  o->info = {1.0};
+  describe(o->info, prof);
+  double rmax = 0, zlo = 1e300, zhi = -1e300;
+  for (auto& e : prof) {
+    rmax = std::max({rmax, e.r0, e.r1});
+    zlo = std::min({zlo, e.z0, e.z1}), zhi = std::max({zhi, e.z0, e.z1});
+    if (e.arc) {
+      // extremes the arc passes through
+      for (int k = -4; k <= 4; ++k) {
+        double a = 0.5 * M_PI * k;
+        double lo = std::min(e.psi0, e.psi1), hi = std::max(e.psi0, e.psi1);
+        if (a > lo && a < hi) {
+          rmax = std::max(rmax, e.rc + e.rad * std::cos(a));
+          zlo = std::min(zlo, e.zc + e.rad * std::sin(a)), zhi = std::max(zhi, e.zc + e.rad * std::sin(a));
+        }
+      }
+    }
+  }
+  o->finish({-rmax, -rmax, zlo}, {rmax, rmax, zhi}, 1e-9);
+  return o;
+}
+
+// Profiles of turned parts, each with g grooves (concave tori) cut into its
+// cylinder; sizes in mm from the seed.
+//   0 capsule: hemisphere, cylinder, hemisphere (sphere poles on the axis)
+//   1 rod: disc, cylinder, cone narrowing upward, hemisphere cap
+//   2 top: cone from its apex on the axis (a cone tip), cylinder, convex torus
+//     round, top disc
+//   3 washer: annulus, outer cylinder with rounded top edge, inner cylinder
+std::vector<Prof> turned_profile(int type, int grooves, Rng& rng) {
+  const double R = 6 + 14 * rng.next(), L = 2 * R + 40 * rng.next();
+  std::vector<Prof> p;
+  // the outer cylinder from z0 to z1 at radius R with grooves, upward
+  auto wall = [&](double z0, double z1) {
+    double len_ = z1 - z0;
+    int g = grooves;
+    while (g > 0 && len_ / (g + 1) < 4.0) --g;
+    double step = len_ / (g + 1), gr = std::min(0.3 * step, 0.25 * R);
+    double z = z0;
+    for (int k = 1; k <= g; ++k) {
+      double zg = z0 + step * k;
+      p.push_back(seg(R, z, R, zg - gr));
+      // a groove: the torus about (R, zg) of radius gr, cut into the material (clockwise)
+      p.push_back(arc(R, zg, gr, 1.5 * M_PI, 0.5 * M_PI));
+      z = zg + gr;
+    }
+    p.push_back(seg(R, z, R, z1));
+  };
+  if (type == 0) {
+    p.push_back(arc(0, 0, R, -0.5 * M_PI, 0));
+    wall(0, L);
+    p.push_back(arc(0, L, R, 0, 0.5 * M_PI));
+    p.push_back(seg(0, L + R, 0, -R));
+  } else if (type == 1) {
+    double h1 = 0.6 * L, r2 = (0.35 + 0.3 * rng.next()) * R, h2 = h1 + (R - r2) * (0.8 + rng.next());
+    p.push_back(seg(0, 0, R, 0));
+    wall(0, h1);
+    p.push_back(seg(R, h1, r2, h2));
+    p.push_back(arc(0, h2, r2, 0, 0.5 * M_PI));
+    p.push_back(seg(0, h2 + r2, 0, 0));
+  } else if (type == 2) {
+    double hc = R * (0.6 + 0.8 * rng.next()), f = (0.15 + 0.2 * rng.next()) * R;
+    p.push_back(seg(0, 0, R, hc));
+    wall(hc, hc + L * 0.5);
+    double zt = hc + L * 0.5;
+    p.push_back(arc(R - f, zt, f, 0, 0.5 * M_PI));
+    p.push_back(seg(R - f, zt + f, 0, zt + f));
+    p.push_back(seg(0, zt + f, 0, 0));
+  } else {
+    double ri = (0.3 + 0.3 * rng.next()) * R, H = 0.3 * L + 4, f = std::min(0.3 * (R - ri), 0.3 * H);
+    p.push_back(seg(ri, 0, R, 0));
+    wall(0, H - f);
+    p.push_back(arc(R - f, H - f, f, 0, 0.5 * M_PI));
+    p.push_back(seg(R - f, H, ri, H));
+    p.push_back(seg(ri, H, ri, 0));
+  }
+  return p;
+}
+
+// A plate W x D x T (centred at the origin) with k vertical through holes on a jittered
+// grid. A hole of radius r has a plain top edge, a torus fillet of radius f (the
+// fillet's circle on the top face at r + f), or a 45-degree countersink of depth c (a
+// cone from r at T/2 - c to r + c at T/2); every bottom edge is plain. treat[i]: 0
+// plain, 1 fillet, 2 countersink.
+struct Hole {
+  double x, y, r, e;
+  int treat;
+};
+Owned* make_plate(double W, double D, double T, const std::vector<Hole>& holes) {
+  Owned* o = new Owned();
+  Builder b{o};
+  V X{1, 0, 0}, Y{0, 1, 0}, Z{0, 0, 1};
+  const double ax = 0.5 * W, ay = 0.5 * D, az = 0.5 * T;
+  auto rect = [](double u0, double v0, double u1, double v1) {
+    std::vector<Piece> p;
+    add_polyline(p, {{u0, v0}, {u1, v0}, {u1, v1}, {u0, v1}}, true);
+    return p;
+  };
+  std::vector<Piece> top = rect(-ax, -ay, ax, ay), bot = rect(-ax, -ay, ax, ay);
+  for (auto& h : holes) add_circle(top, h.x, h.y, h.r + h.e), add_circle(bot, h.x, -h.y, h.r);
+  uint32_t f_top = b.plane({0, 0, az}, Z, X, 1.0, -ax - 1, -ay - 1, ax + 1, ay + 1, top, -ax - 0.5, 0.0123, false);
+  uint32_t f_bot = b.plane({0, 0, -az}, Z * -1.0, X, 1.0, -ax - 1, -ay - 1, ax + 1, ay + 1, bot, -ax - 0.5, 0.0123, false);
+  uint32_t f_xp = b.plane({ax, 0, 0}, X, Y, 1.0, -ay - 1, -az - 1, ay + 1, az + 1, rect(-ay, -az, ay, az), -ay - 0.5, 0.0123, false);
+  uint32_t f_xm = b.plane({-ax, 0, 0}, X * -1.0, Y, 1.0, -ay - 1, -az - 1, ay + 1, az + 1, rect(-ay, -az, ay, az), -ay - 0.5, 0.0123, false);
+  uint32_t f_yp = b.plane({0, ay, 0}, Y, X, 1.0, -ax - 1, -az - 1, ax + 1, az + 1, rect(-ax, -az, ax, az), -ax - 0.5, 0.0123, false);
+  uint32_t f_ym = b.plane({0, -ay, 0}, Y * -1.0, X, 1.0, -ax - 1, -az - 1, ax + 1, az + 1, rect(-ax, -az, ax, az), -ax - 0.5, 0.0123, false);
+  V c[2][2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) c[i][j][k] = V{i ? ax : -ax, j ? ay : -ay, k ? az : -az};
+  for (int k = 0; k < 2; ++k) {
+    uint32_t f = k ? f_top : f_bot;
+    b.line(c[0][0][k], c[1][0][k], f, f_ym);
+    b.line(c[1][0][k], c[1][1][k], f, f_xp);
+    b.line(c[1][1][k], c[0][1][k], f, f_yp);
+    b.line(c[0][1][k], c[0][0][k], f, f_xm);
+  }
+  b.line(c[0][0][0], c[0][0][1], f_xm, f_ym);
+  b.line(c[1][0][0], c[1][0][1], f_xp, f_ym);
+  b.line(c[1][1][0], c[1][1][1], f_xp, f_yp);
+  b.line(c[0][1][0], c[0][1][1], f_xm, f_yp);
+  double q = M_PI / 2;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) b.vert(c[i][j][k], V{i ? 1.0 : -1.0, j ? 1.0 : -1.0, k ? 1.0 : -1.0} * q);
+  o->info = {2.0, W, D, T, double(holes.size())};
+  // each hole's region in its meridian half-plane, counterclockwise: bottom cap, wall,
+  // the top treatment, top cap, down the axis
+  for (auto& h : holes) {
+    std::vector<Prof> p;
+    Prof cap_b = seg(0, -az, h.r, -az);
+    cap_b.face = false, cap_b.ext = f_bot, cap_b.ext_nz = -1;
+    p.push_back(cap_b);
+    double zw = az - (h.treat ? h.e : 0.0);
+    p.push_back(seg(h.r, -az, h.r, zw));
+    if (h.treat == 1) p.push_back(arc(h.r + h.e, zw, h.e, M_PI, 0.5 * M_PI));
+    if (h.treat == 2) p.push_back(seg(h.r, zw, h.r + h.e, az));
+    Prof cap_t = seg(h.r + h.e, az, 0, az);
+    cap_t.face = false, cap_t.ext = f_top, cap_t.ext_nz = 1;
+    p.push_back(cap_t);
+    Prof axis = seg(0, az, 0, -az);
+    p.push_back(axis);
+    o->info.push_back(h.x), o->info.push_back(h.y);
+    describe(o->info, p);
+    revolve(b, h.x, h.y, p, true);
+  }
+  o->finish({-ax, -ay, -az}, {ax, ay, az}, 1e-9);
+  return o;
+}
+
+// k holes on a jittered grid over the plate, radii and treatments from the seed, each
+// hole's top circle clear of the plate's sides and of its neighbours.
+std::vector<Hole> plate_holes(double W, double D, double T, int k, Rng& rng) {
+  int nx = std::max(1, int(std::ceil(std::sqrt(k * W / D)))), ny = (k + nx - 1) / nx;
+  double cw = W / nx, cd = D / ny, cell = std::min(cw, cd);
+  std::vector<Hole> out;
+  for (int i = 0; i < k; ++i) {
+    int gx = i % nx, gy = i / nx;
+    Hole h;
+    h.treat = int(rng.next() * 3.0) % 3;
+    h.r = cell * (0.12 + 0.12 * rng.next());
+    h.e = h.treat ? std::min(cell * (0.04 + 0.06 * rng.next()), 0.3 * T) : 0.0;
+    double room = 0.5 * cell - (h.r + h.e) - 0.05 * cell;
+    h.x = -0.5 * W + cw * (gx + 0.5) + room * (2 * rng.next() - 1) * 0.5;
+    h.y = -0.5 * D + cd * (gy + 0.5) + room * (2 * rng.next() - 1) * 0.5;
+    out.push_back(h);
+  }
+  return out;
+}
+
+// The batched workload's parts: part i of 64 from the seed. Face counts log-uniform in
+// 6..400: turned parts (3..14 faces) and NURBS-top blocks (6 faces, nets 4 x 4 to
+// 32 x 32) below 15, plates with holes above.
+Owned* make_c4(uint64_t seed, int index) {
+  Rng rng{(seed ^ (0x9E3779B97F4A7C15ull * uint64_t(index + 1))) | 1};
+  for (int k = 0; k < 4; ++k) rng.next();
+  const int faces = int(std::lround(std::exp(std::log(6.0) + (std::log(400.0) - std::log(6.0)) * rng.next())));
+  if (faces <= 14) {
+    if (rng.next() < 0.35) {
+      int n = 4 + int(rng.next() * 29.0) % 29;
+      double half = 15 + 45 * rng.next();
+      return make_c2(rng.next_u64(), false, n, half, 0.04 * half + 1.0);
+    }
+    int type = int(rng.next() * 4.0) % 4;
+    int base = type == 0 ? 3 : type == 1 ? 4 : type == 2 ? 4 : 5;
+    int grooves = std::max(0, faces - base);
+    return make_revolved(turned_profile(type, std::min(grooves, 8), rng));
+  }
+  // plates: 6 + k + (treated holes) faces; about two thirds of holes are treated
+  int k = std::max(1, int(std::lround((faces - 6) / 1.67)));
+  double W = 60 + 140 * rng.next(), D = W * (0.5 + 0.5 * rng.next()), T = 8 + 17 * rng.next();
+  return make_plate(W, D, T, plate_holes(W, D, T, k, rng));
+}
+
 }  // namespace
 
 extern "C" {
 
-// A synthetic part by name ("can", "bump", "c1", "c2"); NULL for an unknown name.
-// Free with icam_gen_free.
+// A synthetic part by name: "can", "bump", "c1", "c2", "c2flat", "c4:<i>" (part i of
+// the batched workload), "turned:<type>:<grooves>", "plate:<holes>",
+// "nurbs:<net>"; NULL for an unknown name. Free with icam_gen_free.
 const icam_part_desc* icam_gen_part(const char* name, uint64_t seed) {
   std::string s(name ? name : "");
   Owned* o = nullptr;
+  auto arg = [&](size_t at) { return std::atoi(s.c_str() + at); };
   if (s == "can") o = make_can();
   if (s == "bump") o = make_bump();
   if (s == "c1") o = make_c1();
   if (s == "c2") o = make_c2(seed, false);
   if (s == "c2flat") o = make_c2(seed, true);
+  if (s.rfind("c4:", 0) == 0) o = make_c4(seed, arg(3));
+  if (s.rfind("turned:", 0) == 0) {
+    Rng rng{seed | 1};
+    size_t c = s.find(':', 7);
+    o = make_revolved(turned_profile(arg(7) % 4, c == std::string::npos ? 0 : arg(c + 1), rng));
+  }
+  if (s.rfind("plate:", 0) == 0) {
+    Rng rng{seed | 1};
+    o = make_plate(120, 80, 16, plate_holes(120, 80, 16, std::max(1, arg(6)), rng));
+  }
+  if (s.rfind("nurbs:", 0) == 0) o = make_c2(seed, false, std::min(std::max(arg(6), 4), 32), 40, 3);
   return o ? &o->desc : nullptr;
 }
 
 void icam_gen_free(const icam_part_desc* p) { delete reinterpret_cast<const Owned*>(p); }
+
+// What a part was made from, for closed-form checks: [1, n, n elements] for a solid of
+// revolution about z, [2, W, D, T, k, then per hole x, y, n, n elements] for a plate,
+// [] otherwise; an element is (arc, r0, z0, r1, z1, rc, zc, rad, psi0, psi1, face).
+// Writes at most cap values and returns how many there are.
+size_t icam_gen_describe(const icam_part_desc* p, double* out, size_t cap) {
+  const Owned* o = reinterpret_cast<const Owned*>(p);
+  for (size_t i = 0; i < o->info.size() && i < cap; ++i) out[i] = o->info[i];
+  return o->info.size();
+}
 
 // n points uniform in [lo, hi] from the reference checks' xorshift64 (x, y, z
 // drawn in that order per point).

@@ -439,7 +439,11 @@ __device__ inline uint32_t feat_of(const Dev& dv, uint32_t code) {
   uint32_t kind = code >> 28, id = code & 0x0fffffffu;
   if (kind == P_FACE) return id;
   if (kind == P_EDGE) return (1u << 28) | id;
-  if (kind == P_VERT) return (2u << 28) | id;
+  if (kind == P_VERT) {
+    // a cone's tip reports its face
+    const double tip = dv.F64[dv.L.vert_f + VERT_F * id + 6];
+    return tip > 0.0 ? uint32_t(tip) - 1u : (2u << 28) | id;
+  }
   if (kind == P_LEAF) return dv.U[dv.L.leaf_u + LEAF_U * id + 1];
   return (1u << 28) | dv.U[dv.L.cleaf_u + CLEAF_U * id + 1];
 }

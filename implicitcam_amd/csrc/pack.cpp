@@ -826,11 +826,17 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
           fb.add(cc + ax * ex), fb.add(cc - ax * ex);
         }
       };
+      // the face's edges and its vertex loops' points (a cone's apex)
       if (fd.kind == ICAM_PLANE) {
         for (uint32_t e = 0; e < part->n_edges; ++e)
           if (part->edges[e].faces[0] == fi || part->edges[e].faces[1] == fi) fb.add(edge_box[e]);
+        for (uint32_t k = 0; k < fd.n_points; ++k) fb.add(rel(fd.points + 3 * k));
       } else if (fd.kind == ICAM_CYLINDER || fd.kind == ICAM_CONE) {
         double h0 = INF, h1 = -INF;
+        for (uint32_t k = 0; k < fd.n_points; ++k) {
+          double h = dot(rel(fd.points + 3 * k) - c, z);
+          h0 = std::min(h0, h), h1 = std::max(h1, h);
+        }
         for (uint32_t e = 0; e < part->n_edges; ++e)
           if (part->edges[e].faces[0] == fi || part->edges[e].faces[1] == fi)
             for (int k = 0; k < 8; ++k) {
@@ -868,6 +874,31 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
     pbox.push_back(b);
     pcode.push_back((P_VERT << 28) | v);
   }
+  // solid3.rs:1628-1642 tips: a cone's apex inside its face and away from the
+  // vertices answers as a point with the pseudo-normal z (-sin a) sense
+  // (solid3.rs:1991-2000); it is packed after the vertices as one more point record,
+  // word 6 holding its face + 1 (the feature it reports)
+  uint32_t n_tips = 0;
+  for (uint32_t fi = 0; fi < part->n_faces; ++fi) {
+    const icam_face_desc& fd = part->faces[fi];
+    if (fd.kind != ICAM_CONE) continue;
+    V z = of(fd.z);
+    const double ha = -fd.c[0] / std::tan(fd.c[1]);
+    V apex = of(fd.o) + z * ha;
+    bool near_vertex = false;
+    for (uint32_t v = 0; v < part->n_verts; ++v) near_vertex |= len(of(part->verts[v].p) - apex) <= part->tol;
+    // the apex's parameters: angle 0 about the axis (arc parameter 0), height ha
+    TrimBuild tb;
+    tb.build(fd);
+    if (near_vertex || !tb.inside(0.0, ha)) continue;
+    V p = apex - o, pn = z * (-std::sin(fd.c[1]) * fd.sense);
+    vert_f.insert(vert_f.end(), {p.x, p.y, p.z, pn.x, pn.y, pn.z, double(fi + 1), 0});
+    Box b;
+    b.add(p);
+    pbox.push_back(b);
+    pcode.push_back((P_VERT << 28) | (part->n_verts + n_tips));
+    ++n_tips;
+  }
 
   // ---- concatenate
   auto put = [](std::vector<double>& dst, const std::vector<double>& src) {
@@ -890,7 +921,7 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
   L.among_u = putu(P.U, among_u);
   P.U.push_back(0);
   // uvmap offsets are relative to the uvmap section
-  L.n_faces = part->n_faces, L.n_edges = part->n_edges, L.n_verts = part->n_verts;
+  L.n_faces = part->n_faces, L.n_edges = part->n_edges, L.n_verts = part->n_verts + n_tips;
   L.n_spans = uint32_t(span_u.size() / SPAN_U), L.n_leaves = uint32_t(leaf_u.size() / LEAF_U), L.n_cleaves = uint32_t(cleaf_u.size() / CLEAF_U);
   L.n_cpieces = uint32_t(cpiece_u.size() / CPIECE_U);
   L.n_prims = uint32_t(pcode.size());

@@ -553,11 +553,12 @@ template <class T> __device__ void try_face(const Tab<T>& tb, uint32_t fi, v3<T>
   v3<T> q = foot(kind, f, p);
   T d = len(p - q);
   if (!(d < b.d)) return;
-  // a cone's apex or a lemon's tip is singular: the edges and vertices answer there
+  // a cone's apex is singular: the edges, the vertices and the packed tip (a point
+  // record) answer there (solid3.rs:1915-1918)
   if (kind == ICAM_CONE) {
     T ha = -f[12] / mtan(f[13]);
     v3<T> apex = ld3(f) + ld3(f + 9) * ha;
-    if (len(apex - q) <= T(1e-6) * (T(1) + len(apex))) return;
+    if (len(apex - q) <= (sizeof(T) == 4 ? T(1e-6) : T(1e-12)) * (T(1) + len(apex))) return;
   }
   bool inside = true;
   if (!fu[2]) {

@@ -5,7 +5,12 @@ Parts are made by the native generator (csrc/gen/partgen.cpp):
 - ``can``: the reference's solid cylinder r 3, z 0..5 (crates/icam-core/src/solid3.rs:2013-2040);
 - ``bump``: its 10 x 10 box with a biquadratic B-spline top (solid3.rs:2094-2143);
 - ``c1``: box 100 x 60 x 40 minus a through hole r 10 with a torus fillet r 3;
-- ``c2``: 120 x 120 block whose top is a rational bicubic NURBS (16 x 16 net).
+- ``c2``: 120 x 120 block whose top is a rational bicubic NURBS (16 x 16 net);
+- ``turned:<type>:<grooves>``: solids of revolution (planes, cylinders, cones with
+  their apex on the axis, spheres with their poles on it, ring tori);
+- ``plate:<k>``: a plate with k holes, plain, filleted (torus) or countersunk (cone);
+- ``nurbs:<n>``: a block with an n x n rational bicubic top;
+- ``c4:<i>``: part i of the 64-part batched workload (face counts log-uniform 6-400).
 """
 import ctypes
 
@@ -28,6 +33,13 @@ class Part:
         self.lo = np.array(d.lo[:])
         self.hi = np.array(d.hi[:])
         self.n_faces, self.n_edges, self.n_verts = d.n_faces, d.n_edges, d.n_verts
+
+    def describe(self):
+        """What the part was made from (the generator's icam_gen_describe), as float64."""
+        n = self._lib.icam_gen_describe(self.desc, None, 0)
+        out = np.empty(max(n, 1))
+        self._lib.icam_gen_describe(self.desc, out.ctypes.data_as(_native.c_double_p), n)
+        return out[:n]
 
     @property
     def diagonal(self):

@@ -72,6 +72,11 @@ typedef struct icam_face_desc {
   const double* pieces; /* n_pieces * 8 */
   double ref[2];
   uint32_t ref_inside;
+  /* vertex loops: points of the face bounded by a lone vertex (a cone's apex),
+   * n_points * 3; they lie in the face's box (Face3.points, solid3.rs:1226-1231).
+   * The vertex itself is one of the part's vertices. */
+  uint32_t n_points;
+  const double* points;
 } icam_face_desc;
 
 typedef struct icam_edge_desc {

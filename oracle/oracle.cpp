@@ -1369,13 +1369,20 @@ struct Exact {
     // solid3.rs:1211-1282 face boxes
     for (uint32_t i = 0; i < faces.size(); ++i) {
       Face& f = faces[i];
+      // solid3.rs:1211-1282 face_box: the face's edges and its vertex loops' points
+      const icam_face_desc& fdesc = d->faces[i];
       if (f.kind == ICAM_NURBS) {
         f.box = f.ns->box;
       } else if (f.kind == ICAM_PLANE) {
         for (auto& e : edges)
           if (e.faces[0] == int(i) || e.faces[1] == int(i)) f.box.add(e.box);
+        for (uint32_t k = 0; k < fdesc.n_points; ++k) f.box.add(V3::of(fdesc.points + 3 * k));
       } else if (f.kind == ICAM_CYLINDER || f.kind == ICAM_CONE) {
         double h0 = INF, h1 = -INF;
+        for (uint32_t k = 0; k < fdesc.n_points; ++k) {
+          double h = dot(V3::of(fdesc.points + 3 * k) - f.an.f.o, f.an.f.z);
+          h0 = std::min(h0, h), h1 = std::max(h1, h);
+        }
         for (auto& e : edges)
           if (e.faces[0] == int(i) || e.faces[1] == int(i))
             for (int k = 0; k < 8; ++k) {

@@ -77,3 +77,123 @@ def reference_points(k, cx=0.0, cy=0.0, cz=2.5, r=6.0, rz=5.0):
     """The reference tests' deterministic points (solid3.rs:2060-2065, 2167)."""
     t = np.arange(k, dtype=np.float64)
     return np.stack([cx + r * np.sin(t * 0.37), cy + r * np.cos(t * 0.71), cz + rz * np.sin(t * 1.3)], 1)
+
+
+# ---------------------------------------------------------------- solids of revolution
+#
+# A profile is a closed counterclockwise loop in the (rho, z) half-plane of segments
+# and circular arcs (rows of the generator's description: arc, r0, z0, r1, z1, rc, zc,
+# rad, psi0, psi1, face). Revolved about an axis, its faces are the elements marked
+# `face`; the nearest point of a surface of revolution to p lies in p's meridian
+# half-plane, so the distance is the planar distance from (rho, z) to those elements,
+# and the sign is the planar inside test of the whole loop.
+
+
+def _elements(info, at):
+    n = int(info[at])
+    return info[at + 1: at + 1 + 11 * n].reshape(n, 11), at + 1 + 11 * n
+
+
+def _seg_dist(r, z, a, b):
+    ab = b - a
+    t = np.clip(((r - a[0]) * ab[0] + (z - a[1]) * ab[1]) / max(ab @ ab, 1e-300), 0, 1)
+    return np.hypot(r - a[0] - t * ab[0], z - a[1] - t * ab[1])
+
+
+def _arc_dist(r, z, e):
+    rc, zc, rad, p0, p1 = e[5], e[6], e[7], e[8], e[9]
+    lo, hi = min(p0, p1), max(p0, p1)
+    phi = np.arctan2(z - zc, r - rc)
+    phi = lo + np.mod(phi - lo, 2 * np.pi)
+    on = phi <= hi
+    radial = np.abs(np.hypot(r - rc, z - zc) - rad)
+    ends = np.minimum(np.hypot(r - e[1], z - e[2]), np.hypot(r - e[3], z - e[4]))
+    return np.where(on, radial, ends)
+
+
+def profile_dist(r, z, els):
+    d = np.full(np.shape(r), np.inf)
+    for e in els:
+        if e[10] == 0:
+            continue
+        if e[0]:
+            d = np.minimum(d, _arc_dist(r, z, e))
+        else:
+            d = np.minimum(d, _seg_dist(r, z, np.array([e[1], e[2]]), np.array([e[3], e[4]])))
+    return d
+
+
+def profile_inside(r, z, els):
+    """Even-odd test against the loop with its arcs replaced by chords, flipped inside
+    each circular segment between an arc and its chord (at most a half circle)."""
+    inside = np.zeros(np.shape(r), dtype=bool)
+    for e in els:
+        a, b = (e[1], e[2]), (e[3], e[4])
+        crosses = ((a[1] > z) != (b[1] > z))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            x = a[0] + (z - a[1]) * (b[0] - a[0]) / (b[1] - a[1])
+        inside ^= crosses & (r < x)
+        if e[0]:
+            rc, zc, rad = e[5], e[6], e[7]
+            pm = 0.5 * (e[8] + e[9])
+            mid = (rc + rad * np.cos(pm), zc + rad * np.sin(pm))
+            side = lambda pr, pz: (b[0] - a[0]) * (pz - a[1]) - (b[1] - a[1]) * (pr - a[0])  # noqa: E731
+            seg = (np.hypot(r - rc, z - zc) < rad) & (np.sign(side(r, z)) == np.sign(side(*mid)))
+            inside ^= seg
+    return inside
+
+
+def turned_sdf(p, info):
+    els, _ = _elements(info, 1)
+    p = np.asarray(p, dtype=np.float64)
+    r, z = np.hypot(p[:, 0], p[:, 1]), p[:, 2]
+    d = profile_dist(r, z, els)
+    return np.where(profile_inside(r, z, els), -d, d)
+
+
+def plate_sdf(p, info):
+    """A plate with holes: the least of its faces' distances (top and bottom planar
+    regions with the holes' discs removed, the four sides, each hole's surfaces of
+    revolution), negative inside the box and outside every hole's region."""
+    W, D, T, k = info[1], info[2], info[3], int(info[4])
+    p = np.asarray(p, dtype=np.float64)
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    ax, ay, az = W / 2, D / 2, T / 2
+    at = 5
+    holes = []
+    for _ in range(k):
+        hx, hy = info[at], info[at + 1]
+        els, at = _elements(info, at + 2)
+        holes.append((hx, hy, els))
+
+    def rect(u, v, hu, hv):
+        return np.hypot(np.maximum(np.abs(u) - hu, 0), np.maximum(np.abs(v) - hv, 0))
+
+    def region(level):
+        # in-plane distance from (x, y) to the rectangle minus the holes' discs where
+        # the holes meet the plane z = level
+        d = rect(x, y, ax, ay)
+        for hx, hy, els in holes:
+            rr = max(max(e[1], e[3]) for e in els if abs(e[2] - level) < 1e-12 and abs(e[4] - level) < 1e-12)
+            rho = np.hypot(x - hx, y - hy)
+            d = np.where(rho < rr, rr - rho, d)
+        return d
+
+    ds = [np.hypot(region(az), z - az), np.hypot(region(-az), z + az), np.hypot(rect(y, z, ay, az), x - ax),
+          np.hypot(rect(y, z, ay, az), x + ax), np.hypot(rect(x, z, ax, az), y - ay), np.hypot(rect(x, z, ax, az), y + ay)]
+    removed = np.zeros(len(p), dtype=bool)
+    for hx, hy, els in holes:
+        rho = np.hypot(x - hx, y - hy)
+        ds.append(profile_dist(rho, z, els))
+        removed |= profile_inside(rho, z, els)
+    d = np.min(np.array(ds), axis=0)
+    in_box = (np.abs(x) < ax) & (np.abs(y) < ay) & (np.abs(z) < az)
+    return np.where(in_box & ~removed, -d, d)
+
+
+def described_sdf(part, p):
+    """The closed form of a generated turned part or plate (None for other parts)."""
+    info = part.describe()
+    if len(info) == 0:
+        return None
+    return turned_sdf(p, info) if info[0] == 1 else plate_sdf(p, info)

@@ -213,3 +213,50 @@ def test_golden_vectors(prec):
     lv.nb[:] = [int(x) for x in G["c1_level"][4:]]
     m, a, _ = PartGpu(Part("c1"), 12 * 1.6).bake(lv, prec)
     assert (m == G["c1_map"]).all() and (a == G["c1_atlas"]).all()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("name", ["turned:0:2", "turned:1:1", "turned:2:1", "turned:3:2", "plate:12"])
+def test_revolved_and_plates(name, prec):
+    """Cone tips, sphere poles, concave grooves, convex rounds, fillets and countersinks:
+    queries against the closed form (uniform points and points on and around the axis)
+    and the oracle's nearest features, and a small level against the oracle's bake."""
+    from closed_forms import described_sdf
+
+    part = Part(name)
+    o = Oracle(part)
+    g = PartGpu(part, 4.0)
+    pts = uniform_points(SEED, 1 << 14, part.lo - 3, part.hi + 3)
+    rng = np.random.default_rng(2)
+    axis = np.stack([rng.normal(0, 0.02, 512), rng.normal(0, 0.02, 512), rng.uniform(part.lo[2] - 3, part.hi[2] + 3, 512)], 1)
+    pts = np.concatenate([pts, axis, [[0.0, 0.0, part.lo[2] - 1.0], [0.0, 0.0, part.hi[2] + 1.0]]])
+    d, flags, feat = g.query(pts, prec)
+    ref = described_sdf(part, pts)
+    assert (flags == 0).all()
+    assert np.abs(d - ref).max() <= max(tol(prec, part), 1e-8)
+    assert ((d < 0) == (ref < 0))[np.abs(ref) > 1e-8].all()
+    dref, fref, _ = o.sdf(pts)
+    assert np.abs(d - dref).max() <= tol(prec, part)
+    assert (feat == fref).mean() > 0.999
+    lv = level_for(part, part.diagonal / 48, 8)
+    m_ref, a_ref, _ = o.bake(lv)
+    m, a, st = g.bake(lv, prec)
+    assert (m == m_ref).all() and (a == a_ref).all()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_c4_parts_bake_like_oracle(prec):
+    """Every part of the batched workload (turned parts, NURBS blocks with 4 x 4 to
+    32 x 32 nets, plates with up to ~200 holes) baked on a 64^3 node grid: map and
+    atlas bit-identical to the oracle's."""
+    for i in range(64):
+        part = Part(f"c4:{i}")
+        h = float(np.max(part.hi - part.lo) + 2.0) / 62.0
+        lv = level_for(part, h, 8)
+        g = PartGpu(part, 12 * h)
+        m_ref, a_ref, _ = Oracle(part).bake(lv)
+        m, a, st = g.bake(lv, prec)
+        assert (m == m_ref).all(), i
+        qa, qr = a.view(np.int16).astype(int), a_ref.view(np.int16).astype(int)
+        assert len(qa) == len(qr), i
+        assert np.abs(qa - qr).max(initial=0) <= 1 and ((qa > 0) == (qr > 0)).all(), i

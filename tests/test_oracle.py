@@ -280,3 +280,68 @@ def test_bake_map_and_atlas(c1):
     d, _, _ = o.sdf(centres)
     assert (d[m == 0xFFFFFFFF] > 0).all()
     assert (d[m == 0xFFFFFFFE] < 0).all()
+
+
+@pytest.mark.parametrize("name", ["turned:0:0", "turned:0:3", "turned:1:2", "turned:2:0", "turned:2:2", "turned:3:1"])
+def test_turned_closed_form(built, name):
+    """Solids of revolution (planes, cylinders, cones with their apex on the axis, spheres
+    with their poles on it, ring tori as convex rounds and concave grooves) against the
+    planar distance to their profile in the meridian half-plane, on uniform points and
+    points crowded on the axis (the tips and poles)."""
+    from closed_forms import described_sdf
+
+    part = Part(name)
+    o = Oracle(part)
+    pts = uniform_points(SEED, 20000, part.lo - 3, part.hi + 3)
+    rng = np.random.default_rng(11)
+    axis = np.stack([rng.normal(0, 0.05, 2000), rng.normal(0, 0.05, 2000), rng.uniform(part.lo[2] - 3, part.hi[2] + 3, 2000)], 1)
+    pts = np.concatenate([pts, axis, [[0.0, 0.0, part.lo[2] - 1.0], [0.0, 0.0, part.hi[2] + 1.0]]])
+    d, _, flags = o.sdf(pts)
+    ref = described_sdf(part, pts)
+    assert not (flags & 1).any()
+    assert np.abs(d - ref).max() < 1e-9
+    assert ((d < 0) == (ref < 0))[np.abs(ref) > 1e-9].all()
+    # a search bounded by 2 mm finds every feature within it (the face boxes hold
+    # the faces: a cone's box reaches its apex)
+    db, _, _ = o.sdf(pts, bound=np.full(len(pts), 2.0))
+    near = np.abs(ref) < 2.0 - 1e-9
+    assert np.abs(db[near] - ref[near]).max() < 1e-9
+
+
+@pytest.mark.parametrize("holes", [1, 12, 40])
+def test_plate_closed_form(built, holes):
+    """Plates with plain, filleted (torus) and countersunk (cone) holes against per-face
+    closed forms and a constructive inside test."""
+    from closed_forms import described_sdf
+
+    part = Part(f"plate:{holes}")
+    o = Oracle(part)
+    pts = uniform_points(SEED + holes, 20000, part.lo - 3, part.hi + 3)
+    d, _, flags = o.sdf(pts)
+    ref = described_sdf(part, pts)
+    assert not (flags & 1).any()
+    # the planar loops approximate circles within 1e-10 of the radius
+    assert np.abs(d - ref).max() < 1e-8
+    assert ((d < 0) == (ref < 0))[np.abs(ref) > 1e-8].all()
+
+
+def test_c4_parts(built):
+    """The 64 parts of the batched workload: face counts log-uniform over 6..400 (turned
+    parts and NURBS blocks at the low end), and every turned part or plate against its
+    closed form on a small sample."""
+    from closed_forms import described_sdf
+
+    counts = []
+    for i in range(64):
+        part = Part(f"c4:{i}")
+        counts.append(part.n_faces)
+        ref_part = described_sdf(part, np.zeros((1, 3)))
+        if ref_part is None:
+            continue
+        pts = uniform_points(SEED + i, 400, part.lo - 2, part.hi + 2)
+        d, _, _ = Oracle(part).sdf(pts)
+        ref = described_sdf(part, pts)
+        assert np.abs(d - ref).max() < 1e-8, i
+    counts = np.array(counts)
+    assert counts.min() >= 3 and counts.max() <= 410
+    assert (counts > 100).sum() >= 10 and (counts <= 14).sum() >= 10
