@@ -167,8 +167,12 @@ struct Res {
   uint32_t* items;    // points whose parts wait for the dense pass (item_words words each)
   uint32_t* icount;
   uint32_t icap;
+  uint32_t* squeue;   // blocks holding points left undecided, for k_settle
+  uint32_t* scount;
+  uint32_t* smark;
   unsigned long long* counters;  // [0] bridged nodes, [1] points searched again in double,
-                                 // [3] of them near a trim curve, [4] near-ties of opposite sign
+                                 // [3] of them near a trim curve, [4] near-ties of opposite
+                                 // sign, [5] points settled by the certified search
 };
 
 // ------------------------------------------------------------------ block culling
@@ -280,12 +284,12 @@ __device__ inline T part_lb(const Tab<T>& tb, uint32_t i, uint32_t code, v3<T> p
   return lb;
 }
 template <class T>
-__device__ inline void try_part(const Tab<T>& tb, uint32_t i, T lb, v3<T> p, Best<T>& b, T tm, T eps) {
+__device__ inline void try_part(const Tab<T>& tb, uint32_t i, T lb, v3<T> p, Best<T>& b, T tm, T eps, bool settle = false) {
   const uint32_t code = tb.pcode[i], id = code & 0x0fffffffu;
   if ((code >> 28) == P_LEAF)
-    try_leaf(tb, id, tb.pbox + 6 * i, lb, p, b, tm, eps);
+    try_leaf(tb, id, tb.pbox + 6 * i, lb, p, b, tm, eps, settle);
   else
-    try_cleaf(tb, id, tb.pbox + 6 * i, lb, p, b, eps);
+    try_cleaf(tb, id, tb.pbox + 6 * i, lb, p, b, eps, settle);
 }
 template <int K, class V> __device__ inline V group_min(V v) {
   for (int o = 1; o < K; o <<= 1) v = mmin(v, V(__shfl_xor(v, o)));
@@ -328,7 +332,7 @@ template <class T> __device__ inline Best<T> item_best(const uint32_t* w) {
 
 template <class T, int NT, int Q, int K, int C>
 __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk,
-                          const Res* defer = nullptr, long point = 0) {
+                          const Res* defer = nullptr, long point = 0, bool settle = false) {
   Best<T> b;
   b.d = bound, b.code = NONE, b.a = b.b = T(0), b.sgn = T(1), b.lmin = TypeEps<T>::big, b.d2 = TypeEps<T>::big, b.sgn2 = T(1), b.unsure = false;
   const bool all = nc > C;
@@ -414,7 +418,7 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const 
         k = uint32_t(next);
         last_lb = next_lb, last_k = k;
       }
-      try_part(tb, all ? k : cand[k], next_lb, p, b, tm, eps);
+      try_part(tb, all ? k : cand[k], next_lb, p, b, tm, eps, settle);
     }
   }
   if (K > 1) {
@@ -744,6 +748,61 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_fallback(Dev dv, Job j, Res re
       double bound = 0.0;
       v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
       Best<double> b = search<double, NT, Shape<S>::Q, K, Shape<S>::C>(tb, p, valid, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, cand, nc, 0.0, dv.L.eps64, qlb, qk);
+      if (!valid || !owner) continue;
+      double val;
+      uint32_t flag, feat;
+      classify64(b, bound, dv, val, flag, feat);
+      res.val[i] = val;
+      res.flag[i] = flag;
+      if (res.feat) res.feat[i] = feat;
+    }
+    __syncthreads();
+  }
+}
+
+// Points left undecided (flag 2) are marked (flag 4) and their blocks queued for
+// k_settle.
+template <int S>
+__global__ __launch_bounds__(256) void k_mark_settle(Job j, Res res) {
+  long i = long(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= j.n || res.flag[i] != 2) return;
+  res.flag[i] = 4;
+  long blk = block_of<S>(j, i);
+  if (atomicOr(&res.smark[blk], 1u) == 0u) res.squeue[atomicAdd(res.scount, 1u)] = uint32_t(blk);
+  atomicAdd(&res.counters[5], 1ull);
+}
+
+// The queued blocks: each block culls its candidates again and its marked points are
+// searched in double precision with the parts past their convexity certificate
+// settled by the certified search (the reference settles them on the CPU,
+// crates/icam-sim/src/partcache.rs:563-751).
+template <int S>
+__global__ __launch_bounds__(Shape<S>::NT) void k_settle(Dev dv, Job j, Res res) {
+  constexpr int NT = Shape<S>::NT;
+  constexpr int Q = Shape<S>::Q;
+  __shared__ uint32_t cand[Shape<S>::C];
+  __shared__ float red[NT];
+  __shared__ int wave_tot[NT / 64];
+  __shared__ __align__(16) unsigned char scratch[cmax(SORTC * 8, Q * NT * 10)];
+  uint32_t* tmp = (uint32_t*)scratch;
+  float* key = (float*)(scratch + SORTC * 4);
+  double* qlb = (double*)scratch;
+  uint16_t* qk = (uint16_t*)(scratch + Q * NT * 8);
+  const uint32_t total = *res.scount;
+  const Tab<double> tb = tab_of<double>(dv);
+  constexpr int K = Shape<S>::K;
+  const bool owner = (threadIdx.x & (K - 1)) == 0;
+  for (uint32_t qb = blockIdx.x; qb < total; qb += gridDim.x) {
+    const long blk = res.squeue[qb];
+    const int nc = block_candidates<S>(dv, j, blk, cand, tmp, key, red, wave_tot);
+    for (int m = 0; m < rounds<S>(); ++m) {
+      long i = block_point<S>(j, blk, m);
+      const bool valid = i >= 0 && res.flag[i] == 4;
+      if (K == 1 && !valid) continue;
+      double bound = 0.0;
+      v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
+      Best<double> b = search<double, NT, Shape<S>::Q, K, Shape<S>::C>(tb, p, valid, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, cand, nc, 0.0,
+                                                                       dv.L.eps64, qlb, qk, nullptr, 0, true);
       if (!valid || !owner) continue;
       double val;
       uint32_t flag, feat;
@@ -1120,9 +1179,9 @@ struct icam_hip {
   Buf F32, F64, U, pbox, pcode;
   // per-point results of the centres' and the nodes' searches, and host-call staging
   struct ResBufs {
-    Buf code, pa, pb, d32, info, val, flag, queue, qcount, qmark, items, icount;
+    Buf code, pa, pb, d32, info, val, flag, queue, qcount, qmark, items, icount, squeue, scount, smark;
     void free() {
-      for (Buf* b : {&code, &pa, &pb, &d32, &info, &val, &flag, &queue, &qcount, &qmark, &items, &icount}) b->free();
+      for (Buf* b : {&code, &pa, &pb, &d32, &info, &val, &flag, &queue, &qcount, &qmark, &items, &icount, &squeue, &scount, &smark}) b->free();
     }
   } centres, nodes;
   Buf q_xyz, q_bound, q_d, q_flag, q_feat;
@@ -1199,6 +1258,9 @@ int results(icam_hip* h, bool nodes, long n, Res& r, double* val, uint32_t* flag
     HIP_OK(B.qmark.need(m * 4));
   }
   HIP_OK(qcount.need(16));
+  HIP_OK(B.squeue.need(m * 4));
+  HIP_OK(B.smark.need(m * 4));
+  HIP_OK(B.scount.need(16));
   HIP_OK(h->counters.need(64));
   if (!val) HIP_OK(bval.need(m * 8));
   if (!flag) HIP_OK(bflag.need(m * 4));
@@ -1208,6 +1270,7 @@ int results(icam_hip* h, bool nodes, long n, Res& r, double* val, uint32_t* flag
   r.flag = flag ? flag : bflag.as<uint32_t>();
   r.feat = feat;
   r.counters = h->counters.as<unsigned long long>();
+  r.squeue = B.squeue.as<uint32_t>(), r.scount = B.scount.as<uint32_t>(), r.smark = B.smark.as<uint32_t>();
   r.items = nullptr, r.icount = nullptr, r.icap = 0;
   if (items) {
     // room for an eighth of the points; points past it run their rounds in place
@@ -1225,10 +1288,12 @@ template <int S>
 int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Stages* st, long blocks) {
   if (j.n <= 0) return 0;
   constexpr int NT = Shape<S>::NT;
-  static const char* names[3][6] = {{"points_search32", "points_refine64", "points_fallback64", "points_search64", "points_leaves32", "points_leaves64"},
-                                    {"nodes_search32", "nodes_refine64", "nodes_fallback64", "nodes_search64", "nodes_leaves32", "nodes_leaves64"},
+  static const char* names[3][8] = {{"points_search32", "points_refine64", "points_fallback64", "points_search64", "points_leaves32", "points_leaves64",
+                                     "points_mark", "points_settle64"},
+                                    {"nodes_search32", "nodes_refine64", "nodes_fallback64", "nodes_search64", "nodes_leaves32", "nodes_leaves64",
+                                     "nodes_mark", "nodes_settle64"},
                                     {"centres_search32", "centres_refine64", "centres_fallback64", "centres_search64", "centres_leaves32",
-                                     "centres_leaves64"}};
+                                     "centres_leaves64", "centres_mark", "centres_settle64"}};
   const unsigned lb = r.items ? unsigned(std::min<long>((long(r.icap) + 255) / 256, 2048)) : 0;
   if (r.items) HIP_OK(hipMemsetAsync(r.icount, 0, 4, s));
   if (mixed) {
@@ -1243,6 +1308,11 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
     HIP_OK(launch(st, names[S][3], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, double>, h->dv, j, r));
     if (lb) HIP_OK(launch(st, names[S][5], s, dim3(lb), dim3(256), k_leaves<S, double>, h->dv, j, r));
   }
+  // what is still undecided: settled by the certified search
+  HIP_OK(hipMemsetAsync(r.scount, 0, 4, s));
+  HIP_OK(hipMemsetAsync(r.smark, 0, size_t(blocks) * 4, s));
+  HIP_OK(launch(st, names[S][6], s, dim3(unsigned((j.n + 255) / 256)), dim3(256), k_mark_settle<S>, j, r));
+  HIP_OK(launch(st, names[S][7], s, dim3(unsigned(std::min<long>(blocks, 1024))), dim3(NT), k_settle<S>, h->dv, j, r));
   return 0;
 }
 
@@ -1495,15 +1565,15 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     HIP_OK(launch(&st, "store_band", s, dim3(nband), dim3(128), k_store_band, long(nband), (const uint32_t*)h->list.as<uint32_t>(),
                   (const uint32_t*)h->cls.as<uint32_t>(), (const uint32_t*)h->slot.as<uint32_t>(), (const uint32_t*)h->staging.as<uint32_t>(),
                   h->map.as<uint32_t>(), h->atlas.as<uint32_t>()));
-  unsigned long long cnt[5] = {0, 0, 0, 0, 0};
-  HIP_OK(hipMemcpyAsync(cnt, h->counters.p, 40, hipMemcpyDeviceToHost, s));
+  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
+  HIP_OK(hipMemcpyAsync(cnt, h->counters.p, 48, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   collect(h, st);
   if (cnt[2]) return fail(std::to_string(cnt[2]) + " bricks kept whole with no sign found (no surface near them and none outside the part's box)");
   if (stats) {
     stats->bricks = nb, stats->band = nband, stats->slots = nslots;
     stats->undecided = int64_t(cnt[0]), stats->refined = int64_t(cnt[1]);
-    stats->unsure_trim = int64_t(cnt[3]), stats->unsure_tie = int64_t(cnt[4]);
+    stats->unsure_trim = int64_t(cnt[3]), stats->unsure_tie = int64_t(cnt[4]), stats->settled = int64_t(cnt[5]);
   }
   return 0;
 }

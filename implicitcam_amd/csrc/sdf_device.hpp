@@ -628,16 +628,243 @@ template <class T> __device__ inline bool is_critical(v3<T> r, v3<T> d, T tol) {
   return mabs(dot(r, d)) <= tol * l;
 }
 
+// ------------------------------------------------------------------ settling
+// A point past a part's convexity certificate is settled, in double precision, by the
+// certified search of the reference's CPU evaluator restated for the part
+// (NurbsSurf::closest_trimmed and polish, crates/icam-core/src/surface3.rs:1214-1365;
+// a curve piece's search, curve3.rs:408-476): best first over quarters (halves) of
+// the part, each read by SETTLE_NEWTON Newton steps from its middle; a critical
+// point inside the face answers, a non-critical one is carried across spans by Newton
+// first; any point of the face found lowers the part's pruning bound; a quarter is
+// dropped when its control points' box, or their slab along the direction from its
+// Newton point, keeps it at or past that bound, or at the depth limit. The reference's
+// critical-point clearing (surface3.rs:694-731) only prunes sooner and is not
+// restated. A part whose queue overflows, or that takes more than SETTLE_POPS reads,
+// is left undecided with its lower bound.
+constexpr int SETTLE_Q = 40;
+constexpr int SETTLE_DEPTH = 40;     // surface3.rs:1291
+constexpr int SETTLE_NEWTON = 12;    // surface3.rs:552
+constexpr int SETTLE_POPS = 4096;
+constexpr double SETTLE_EPS = 1e-9;  // surface3.rs:555
+
+// The homogeneous cubic c over [0, 1] replaced by its piece over [a, b] (de Casteljau:
+// the left part at b, then the right part of that at a / b).
+__device__ inline void sub_cubic4(double c[4][4], double a, double b) {
+  if (b < 1.0) {
+    double q[4][4], l[4][4];
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 4; ++k) q[i][k] = c[i][k];
+    for (int k = 0; k < 4; ++k) l[0][k] = q[0][k];
+    for (int lev = 1; lev <= 3; ++lev) {
+      for (int i = 0; i <= 3 - lev; ++i)
+        for (int k = 0; k < 4; ++k) q[i][k] = (1.0 - b) * q[i][k] + b * q[i + 1][k];
+      for (int k = 0; k < 4; ++k) l[lev][k] = q[0][k];
+    }
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 4; ++k) c[i][k] = l[i][k];
+  }
+  if (a > 0.0) {
+    const double t = a / b;
+    double q[4][4], r[4][4];
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 4; ++k) q[i][k] = c[i][k];
+    for (int k = 0; k < 4; ++k) r[3][k] = q[3][k];
+    for (int lev = 1; lev <= 3; ++lev) {
+      for (int i = 0; i <= 3 - lev; ++i)
+        for (int k = 0; k < 4; ++k) q[i][k] = (1.0 - t) * q[i][k] + t * q[i + 1][k];
+      for (int k = 0; k < 4; ++k) r[3 - lev][k] = q[3 - lev][k];
+    }
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 4; ++k) c[i][k] = r[i][k];
+  }
+}
+
+// The homogeneous net of a span over [s0, s1] x [t0, t1] (i along s, j along t).
+__device__ inline void sub_net(const ICAM_K double* sp, double s0, double s1, double t0, double t1, double n[4][4][4]) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 4; ++k) n[i][j][k] = sp[16 * i + 4 * j + k];
+  for (int i = 0; i < 4; ++i) sub_cubic4(n[i], t0, t1);
+  for (int j = 0; j < 4; ++j) {
+    double c[4][4];
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 4; ++k) c[i][k] = n[i][j][k];
+    sub_cubic4(c, s0, s1);
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 4; ++k) n[i][j][k] = c[i][k];
+  }
+}
+
+// p's distance to the box of the control points, and their least height over p along
+// dir (the slab bound, surface3.rs:973-975), of n points of homogeneous net c.
+__device__ inline void hull_bounds(const double (*c)[4], int n, v3<double> p, v3<double> dir, double& box, double& slab) {
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  slab = 1e300;
+  for (int k = 0; k < n; ++k) {
+    const double iw = 1.0 / c[k][3];
+    const double x[3] = {c[k][0] * iw, c[k][1] * iw, c[k][2] * iw};
+    for (int a = 0; a < 3; ++a) lo[a] = fmin(lo[a], x[a]), hi[a] = fmax(hi[a], x[a]);
+    slab = fmin(slab, (p.x - x[0]) * dir.x + (p.y - x[1]) * dir.y + (p.z - x[2]) * dir.z);
+  }
+  const double dx = fmax(fmax(lo[0] - p.x, p.x - hi[0]), 0.0), dy = fmax(fmax(lo[1] - p.y, p.y - hi[1]), 0.0),
+               dz = fmax(fmax(lo[2] - p.z, p.z - hi[2]), 0.0);
+  box = sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// surface3.rs:564-570 critical: the gradient of the squared distance zero to within
+// the rounding of the coordinates involved
+__device__ inline bool critical64(v3<double> p, v3<double> q, v3<double> r, v3<double> d) {
+  auto big = [](v3<double> x) { return fmax(fabs(x.x), fmax(fabs(x.y), fabs(x.z))); };
+  const double round = 1024.0 * 2.220446049250313e-16 * (big(q) + big(p) + len(r));
+  const double l = len(d);
+  return fabs(dot(r, d)) <= (round + 1e-12 * l) * l;
+}
+
+// surface3.rs:1323-1365 polish: Newton on whole spans of face fi from (u, v), carried
+// into the neighbouring span while the gradient points over a border, up to 16 hops.
+__device__ inline bool polish64(const Tab<double>& tb, uint32_t fi, v3<double> p, double u, double v, double& dist, double& ou, double& ov,
+                                double& side) {
+  const Layout& L = tb.L;
+  const ICAM_K uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
+  const uint32_t span0 = fu[3], nsu = fu[4], nsv = fu[5];
+  uint32_t a = 0, b = 0;
+  for (uint32_t k = 1; k < nsu; ++k)
+    if (tb.F[L.span_f + SPAN_F * (span0 + k * nsv) + 64] <= u) a = k;
+  for (uint32_t k = 1; k < nsv; ++k)
+    if (tb.F[L.span_f + SPAN_F * (span0 + k) + 66] <= v) b = k;
+  const ICAM_K double* sp = tb.F + L.span_f + SPAN_F * (span0 + a * nsv + b);
+  double s = (u - sp[64]) / (sp[65] - sp[64]), t = (v - sp[66]) / (sp[67] - sp[66]);
+  for (int hop = 0; hop < 16; ++hop) {
+    double ns = mclamp(s, 0.0, 1.0), nt = mclamp(t, 0.0, 1.0);
+    newton2(sp, p, ns, nt, 0.0, 1.0, 0.0, 1.0, 40, 1e-13);
+    SEval<double> e = peval(sp, ns, nt);
+    v3<double> r = e.p - p;
+    if (critical64(p, e.p, r, e.ps) && critical64(p, e.p, r, e.pt)) {
+      dist = len(r), ou = sp[64] + ns * (sp[65] - sp[64]), ov = sp[66] + nt * (sp[67] - sp[66]);
+      const v3<double> n = cross(e.ps, e.pt);
+      if (!(len(n) > 0.0)) return false;
+      side = sgn_of(-dot(r, n));
+      return true;
+    }
+    const double gs = dot(r, e.ps), gt = dot(r, e.pt);
+    bool moved = false;
+    uint32_t na = a, nb = b;
+    if (ns >= 1.0 && gs < 0 && a + 1 < nsu) {
+      na = a + 1, s = 0.0, moved = true;
+    } else if (ns <= 0.0 && gs > 0 && a > 0) {
+      na = a - 1, s = 1.0, moved = true;
+    } else {
+      s = ns;
+    }
+    if (nt >= 1.0 && gt < 0 && b + 1 < nsv) {
+      nb = b + 1, t = 0.0, moved = true;
+    } else if (nt <= 0.0 && gt > 0 && b > 0) {
+      nb = b - 1, t = 1.0, moved = true;
+    } else {
+      t = nt;
+    }
+    if (!moved) return false;
+    a = na, b = nb;
+    sp = tb.F + L.span_f + SPAN_F * (span0 + a * nsv + b);
+  }
+  return false;
+}
+
+struct SettleItem {
+  double a0, a1, b0, b1, lb;
+  int depth;
+};
+
+// Sub-patch li settled for p (its lower bound lb); false when left incomplete.
+__device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb, v3<double> p, Best<double>& b) {
+  const Layout& L = tb.L;
+  const ICAM_K double* lf = tb.F + L.leaf_f + LEAF_F * li;
+  const ICAM_K uint32_t* lu = tb.U + L.leaf_u + LEAF_U * li;
+  const ICAM_K double* sp = tb.F + L.span_f + SPAN_F * lu[0];
+  const uint32_t fi = lu[1];
+  const ICAM_K double* f = tb.F + L.face_f + FACE_F * fi;
+  const ICAM_K uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
+  const bool known_in = lu[2] != 0;
+  SettleItem q[SETTLE_Q];
+  int nq = 1, pops = 0;
+  q[0] = SettleItem{lf[1], lf[2], lf[3], lf[4], lb, 0};
+  double bd = b.d;
+  bool complete = true;
+  while (nq > 0) {
+    int k = 0;
+    for (int i = 1; i < nq; ++i)
+      if (q[i].lb < q[k].lb) k = i;
+    const SettleItem it = q[k];
+    q[k] = q[--nq];
+    if (it.lb >= bd - SETTLE_EPS || bd <= SETTLE_EPS) break;
+    if (++pops > SETTLE_POPS) {
+      complete = false;
+      break;
+    }
+    double s = 0.5 * (it.a0 + it.a1), t = 0.5 * (it.b0 + it.b1);
+    newton2(sp, p, s, t, it.a0, it.a1, it.b0, it.b1, SETTLE_NEWTON, 1e-13 * (it.a1 - it.a0));
+    SEval<double> e = peval(sp, s, t);
+    v3<double> r = e.p - p;
+    const double d = len(r);
+    v3<double> nrm = cross(e.ps, e.pt);
+    const double u = sp[64] + s * (sp[65] - sp[64]), v = sp[66] + t * (sp[67] - sp[66]);
+    bool near = false;
+    auto inside = [&](double uu, double vv) { return known_in || fu[2] != 0 || trim_inside(tb, f + 17, fu[1], uu, vv, 0.0, near); };
+    if (d < bd && inside(u, v)) {
+      bd = d;
+      if (critical64(p, e.p, r, e.ps) && critical64(p, e.p, r, e.pt)) {
+        if (len(nrm) > 0.0) offer(b, d, (P_LEAF << 28) | li, s, t, sgn_of(dot(r, nrm) * -f[14]));
+      } else {
+        double hd, hu, hv, hs;
+        if (polish64(tb, fi, p, u, v, hd, hu, hv, hs) && hd <= d && inside(hu, hv)) {
+          bd = fmin(bd, hd);
+          offer(b, hd, (P_LEAF << 28) | li, s, t, hs * f[14]);
+        }
+      }
+    }
+    double net[4][4][4];
+    sub_net(sp, it.a0, it.a1, it.b0, it.b1, net);
+    const v3<double> dir = d > 1e-12 ? (p - e.p) * (1.0 / d) : (len(nrm) > 0.0 ? nrm * (1.0 / len(nrm)) : mk3(0.0, 0.0, 1.0));
+    double box, slab;
+    hull_bounds(&net[0][0], 16, p, dir, box, slab);
+    if (slab >= bd - SETTLE_EPS || it.depth >= SETTLE_DEPTH) continue;
+    const double am = 0.5 * (it.a0 + it.a1), bm = 0.5 * (it.b0 + it.b1);
+    for (int c = 0; c < 4; ++c) {
+      SettleItem ch{(c & 1) ? am : it.a0, (c & 1) ? it.a1 : am, (c & 2) ? bm : it.b0, (c & 2) ? it.b1 : bm, 0.0, it.depth + 1};
+      double cn[4][4][4], cb, cs;
+      sub_net(sp, ch.a0, ch.a1, ch.b0, ch.b1, cn);
+      hull_bounds(&cn[0][0], 16, p, dir, cb, cs);
+      ch.lb = fmax(cb, slab);
+      if (ch.lb < bd - SETTLE_EPS) {
+        if (nq == SETTLE_Q) {
+          complete = false;
+          continue;
+        }
+        q[nq++] = ch;
+      }
+    }
+  }
+  return complete;
+}
+
 // gpu.wgsl:679-756 / solid3.rs:1904-1910: a sub-patch read by Newton when its radius
 // certifies a single minimum for p; the minimum answers when it is a critical point
-// inside the face
-template <class T> __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps) {
+// inside the face. With `settle` (double precision), a sub-patch past its certificate
+// is settled by the certified search instead of leaving its lower bound.
+template <class T> __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps,
+                                            bool settle = false) {
   const Layout& L = tb.L;
   const ICAM_K T* lf = tb.F + L.leaf_f + LEAF_F * li;
   const ICAM_K uint32_t* lu = tb.U + L.leaf_u + LEAF_U * li;
   T rho = lf[0];
   T fd = far_dist(p, box);
   bool certified = fd < T(0.999) * rho - eps;
+  if constexpr (sizeof(T) == 8) {
+    if (settle && !certified) {
+      if (!settle_leaf(tb, li, lb, p, b)) b.lmin = mmin(b.lmin, lb);
+      return;
+    }
+  }
   const ICAM_K T* sp = tb.F + L.span_f + SPAN_F * lu[0];
   T s0 = lf[1], s1 = lf[2], t0 = lf[3], t1 = lf[4];
   T s = T(0.5) * (s0 + s1), t = T(0.5) * (t0 + t1);
@@ -666,14 +893,85 @@ template <class T> __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const
   offer(b, d, (P_LEAF << 28) | li, s, t, sgn_of(dot(r, n) * -f[14]));
 }
 
+// Curve part ci settled for p (its lower bound lb): best first over halves of the
+// part, each read by Newton from its middle; a critical point interior to its edge
+// answers; false when left incomplete.
+__device__ inline bool settle_cleaf(const Tab<double>& tb, uint32_t ci, double lb, v3<double> p, Best<double>& b) {
+  const Layout& L = tb.L;
+  const ICAM_K double* cf = tb.F + L.cleaf_f + CLEAF_F * ci;
+  const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * ci;
+  const uint32_t cp = cu[0];
+  const ICAM_K double* pw = tb.F + L.cpiece_f + CPIECE_F * cp;
+  const ICAM_K uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cp;
+  SettleItem q[SETTLE_Q];
+  int nq = 1, pops = 0;
+  q[0] = SettleItem{cf[1], cf[2], 0.0, 0.0, lb, 0};
+  double bd = b.d;
+  bool complete = true;
+  while (nq > 0) {
+    int k = 0;
+    for (int i = 1; i < nq; ++i)
+      if (q[i].lb < q[k].lb) k = i;
+    const SettleItem it = q[k];
+    q[k] = q[--nq];
+    if (it.lb >= bd - SETTLE_EPS || bd <= SETTLE_EPS) break;
+    if (++pops > SETTLE_POPS) {
+      complete = false;
+      break;
+    }
+    const double s = newton1(pw, p, 0.5 * (it.a0 + it.a1), it.a0, it.a1, 40, 1e-15 * (it.a1 - it.a0));
+    CEval<double> e = ceval(pw, s);
+    v3<double> r = e.p - p;
+    const double d = len(r);
+    if (d < bd) {
+      bd = d;
+      const bool end = (pu[1] == 0 && s <= 0.0) || (pu[1] + 1 == pu[2] && s >= 1.0);
+      if (!end && critical64(p, e.p, r, e.d1)) {
+        v3<double> pn = edge_pn(tb, pu[0], e.p, pu[1], s);
+        if (len(pn) < 1e-6) b.unsure = true;
+        offer(b, d, (P_CLEAF << 28) | ci, s, 0.0, sgn_of(dot(p - e.p, pn)));
+      }
+    }
+    if (it.depth >= SETTLE_DEPTH) continue;
+    const double m = 0.5 * (it.a0 + it.a1);
+    const v3<double> dir = d > 1e-12 ? (p - e.p) * (1.0 / d) : mk3(0.0, 0.0, 1.0);
+    for (int c = 0; c < 2; ++c) {
+      SettleItem ch{c ? m : it.a0, c ? it.a1 : m, 0.0, 0.0, 0.0, it.depth + 1};
+      double cc[4][4];
+      for (int i = 0; i < 4; ++i)
+        for (int kk = 0; kk < 4; ++kk) cc[i][kk] = pw[4 * i + kk];
+      sub_cubic4(cc, ch.a0, ch.a1);
+      double cb, cs;
+      hull_bounds(cc, 4, p, dir, cb, cs);
+      ch.lb = cb;
+      if (ch.lb < bd - SETTLE_EPS) {
+        if (nq == SETTLE_Q) {
+          complete = false;
+          continue;
+        }
+        q[nq++] = ch;
+      }
+    }
+  }
+  return complete;
+}
+
 // gpu.wgsl:764-815: a part of a curve piece read by Newton when certified; the minimum
-// answers when it is critical and interior to its edge
-template <class T> __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T eps) {
+// answers when it is critical and interior to its edge. With `settle` (double
+// precision), a part past its certificate is settled by the certified search.
+template <class T> __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T eps,
+                                             bool settle = false) {
   const Layout& L = tb.L;
   const ICAM_K T* cf = tb.F + L.cleaf_f + CLEAF_F * ci;
   const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * ci;
   T rho = cf[0];
   bool certified = far_dist(p, box) < T(0.999) * rho - eps;
+  if constexpr (sizeof(T) == 8) {
+    if (settle && !certified) {
+      if (!settle_cleaf(tb, ci, lb, p, b)) b.lmin = mmin(b.lmin, lb);
+      return;
+    }
+  }
   const uint32_t cp = cu[0];
   const ICAM_K T* pw = tb.F + L.cpiece_f + CPIECE_F * cp;
   const ICAM_K uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cp;

@@ -108,6 +108,8 @@ typedef struct icam_bake_stats {
   int64_t refined;     /* points searched again in double precision (ICAM_MIXED) */
   int64_t unsure_trim; /* of them, for a trim decision taken near a trim curve */
   int64_t unsure_tie;  /* of them, for a near-tie of opposite signs */
+  int64_t settled;     /* points past a convexity certificate settled by the certified
+                          search (left undecided, then bridged, when it does not finish) */
 } icam_bake_stats;
 
 /* PartCache::build (partcache.rs:458-486) on the device: the level's brick map

@@ -260,3 +260,20 @@ def test_c4_parts_bake_like_oracle(prec):
         qa, qr = a.view(np.int16).astype(int), a_ref.view(np.int16).astype(int)
         assert len(qa) == len(qr), i
         assert np.abs(qa - qr).max(initial=0) <= 1 and ((qa > 0) == (qr > 0)).all(), i
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("name", ["c2", "bump", "nurbs:32"])
+def test_settle_past_certificates(name, prec):
+    """Sub-patch and curve trees packed for a need of 0.5 mm leave most points past
+    their convexity certificates: the certified search settles them, exact against the
+    oracle, with no point left undecided."""
+    part = Part(name)
+    o = Oracle(part)
+    g = PartGpu(part, 0.5)
+    pts = uniform_points(SEED + 3, 4096, part.lo - 3, part.hi + 3)
+    dref, _, _ = o.sdf(pts)
+    d, flags, _ = g.query(pts, prec)
+    assert (flags == 0).all()
+    assert np.abs(d - dref).max() <= tol(prec, part)
+    assert ((d < 0) == (dref < 0)).all()
