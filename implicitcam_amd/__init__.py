@@ -3,5 +3,5 @@
 A from-scratch MI355X implementation of ImplicitCAM's `solid3::Exact` evaluator and
 `PartCache` brick baker: HIP kernels for gfx950 behind the C ABI of include/icam_hip.h.
 """
-from .exact import FP64, MIXED, PartGpu  # noqa: F401
-from .part import Part, level_for, uniform_points  # noqa: F401
+from .exact import FP64, MIXED, Batch, PartGpu  # noqa: F401
+from .part import Part, c4_grid, c4_level, level_for, uniform_points  # noqa: F401

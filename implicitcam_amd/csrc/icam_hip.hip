@@ -29,6 +29,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -95,7 +96,7 @@ __device__ inline v3<double> brick_first(const Lv& lv, const double* origin, lon
 
 // The reach of a brick: half the diagonal of the brick grown by the clamp, plus the
 // clamp (a centre farther than this from the surface decides the brick whole).
-__device__ inline double brick_reach(double h) {
+__host__ __device__ inline double brick_reach(double h) {
   double clamp = h * 1.7320508075688772;
   return 0.5 * 1.7320508075688772 * (8.0 * h + 2.0 * clamp) + clamp;
 }
@@ -170,6 +171,7 @@ struct Res {
   uint32_t* squeue;   // blocks holding points left undecided, for k_settle
   uint32_t* scount;
   uint32_t* smark;
+  uint32_t part;      // the part's number in a batch (0 alone), in queue entries' top byte
   unsigned long long* counters;  // [0] bridged nodes, [1] points searched again in double,
                                  // [3] of them near a trim curve, [4] near-ties of opposite
                                  // sign, [5] points settled by the certified search
@@ -380,7 +382,7 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const 
         qlb[r * NT + lane] = v, qk[r * NT + lane] = kv;
       }
       uint32_t* w = defer->items + size_t(slot) * Item<T>::W;
-      w[0] = uint32_t(point), w[1] = uint32_t(m);
+      w[0] = uint32_t(point), w[1] = uint32_t(m) | (defer->part << 8);
       item_state(w, b);
       for (int q = 0; q < m; ++q) item_put(w, Item<T>::LB + q * int(sizeof(T) / 4), qlb[q * NT + lane]), w[Item<T>::PR + q] = cand[qk[q * NT + lane]];
       b.code = DEFERRED;
@@ -614,31 +616,37 @@ template <class T> __device__ inline T search_bound(double bound, T eps) {
   return bound < 1e300 ? bound * (1.0 + 1e-12) : bound;
 }
 
-// The search, in single (T = float: records for k_refine) or double precision (T =
-// double: final answers). A brick's nodes leave their parts' rounds to k_leaves.
+// Shared memory of a search block: the block's candidates, a reduction buffer, and a
+// scratch area holding the candidate sort's keys, then the lanes' queues.
+template <int S, class T> struct SearchLds {
+  static constexpr int NT = Shape<S>::NT, Q = Shape<S>::Q;
+  uint32_t cand[Shape<S>::C];
+  float red[NT];
+  int wave_tot[NT / 64];
+  alignas(16) unsigned char scratch[cmax(SORTC * 8, Q * NT * (int(sizeof(T)) + 2))];
+  __device__ uint32_t* tmp() { return (uint32_t*)scratch; }
+  __device__ float* key() { return (float*)(scratch + SORTC * 4); }
+  __device__ T* qlb() { return (T*)scratch; }
+  __device__ uint16_t* qk() { return (uint16_t*)(scratch + Q * NT * sizeof(T)); }
+};
+
+// Block blk of a job searched, in single (T = float: records for k_refine) or double
+// precision (T = double: final answers). A brick's nodes leave their parts' rounds to
+// the dense pass.
 template <int S, class T>
-__global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res) {
+__device__ inline void search_block(const Dev& dv, const Job& j, const Res& res, long blk, SearchLds<S, T>& sm) {
   constexpr int NT = Shape<S>::NT;
   constexpr int Q = Shape<S>::Q;
-  __shared__ uint32_t cand[Shape<S>::C];
-  __shared__ float red[NT];
-  __shared__ int wave_tot[NT / 64];
-  // the sort's keys, then the lanes' queues
-  __shared__ __align__(16) unsigned char scratch[cmax(SORTC * 8, Q * NT * (int(sizeof(T)) + 2))];
-  uint32_t* tmp = (uint32_t*)scratch;
-  float* key = (float*)(scratch + SORTC * 4);
-  T* qlb = (T*)scratch;
-  uint16_t* qk = (uint16_t*)(scratch + Q * NT * sizeof(T));
-  const int nc = block_candidates<S>(dv, j, blockIdx.x, cand, tmp, key, red, wave_tot);
+  const int nc = block_candidates<S>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot);
   const Tab<T> tb = tab_of<T>(dv);
   const T eps = sizeof(T) == 4 ? T(dv.L.eps32) : T(dv.L.eps64);
   const T tm = sizeof(T) == 4 ? T(16) * eps : T(0);
   constexpr int K = Shape<S>::K;
   const bool owner = (threadIdx.x & (K - 1)) == 0;
   double nh = 0.0;
-  const v3<double> nfirst = S == SRC_NODES ? node_first(j, dv, blockIdx.x, nh) : mk3(0.0, 0.0, 0.0);
+  const v3<double> nfirst = S == SRC_NODES ? node_first(j, dv, blk, nh) : mk3(0.0, 0.0, 0.0);
   for (int m = 0; m < rounds<S>(); ++m) {
-    long i = block_point<S>(j, blockIdx.x, m);
+    long i = block_point<S>(j, blk, m);
     if (K == 1 && i < 0) continue;
     const bool valid = i >= 0;
     double bound = 0.0;
@@ -647,50 +655,57 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res)
       p = node_at(nfirst, nh, int(threadIdx.x) + NT * m), bound = nh * 1.7320508075688772;
     else
       p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
-    Best<T> b = search<T, NT, Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, search_bound<T>(bound, eps), cand, nc, tm, eps, qlb, qk,
+    Best<T> b = search<T, NT, Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, search_bound<T>(bound, eps), sm.cand, nc, tm, eps, sm.qlb(), sm.qk(),
                                                  S == SRC_NODES && res.items ? &res : nullptr, i);
     if (!valid || !owner || b.code == DEFERRED) continue;
     finish_point<T>(dv, res, i, bound, b, eps);
   }
 }
 
-// The dense pass over the deferred points: each lane one point, its parts in order of
-// their lower bounds while one can beat its answer.
 template <int S, class T>
-__global__ __launch_bounds__(256) void k_leaves(Dev dv, Job j, Res res) {
-  const uint32_t n = min(*res.icount, res.icap);
+__global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res) {
+  __shared__ SearchLds<S, T> sm;
+  search_block<S, T>(dv, j, res, blockIdx.x, sm);
+}
+
+// One deferred point of the dense pass: its parts in order of their lower bounds
+// while one can beat its answer.
+template <int S, class T>
+__device__ inline void leaves_item(const Dev& dv, const Job& j, const Res& res, const uint32_t* w) {
   const Tab<T> tb = tab_of<T>(dv);
   const T eps = sizeof(T) == 4 ? T(dv.L.eps32) : T(dv.L.eps64);
   const T tm = sizeof(T) == 4 ? T(16) * eps : T(0);
-  for (uint32_t it = blockIdx.x * 256 + threadIdx.x; it < n; it += gridDim.x * 256) {
-    const uint32_t* w = res.items + size_t(it) * Item<T>::W;
-    const long i = long(w[0]);
-    const int m = int(w[1]);
-    Best<T> b = item_best<T>(w);
-    double bound;
-    const v3<T> p = cv3<T>(job_point<S>(j, dv, i, bound));
-    for (int q = 0; q < m; ++q) {
-      const T lb = item_get<T>(w, Item<T>::LB + q * int(sizeof(T) / 4));
-      if (!(lb < b.d)) break;
-      try_part(tb, w[Item<T>::PR + q], lb, p, b, tm, eps);
-    }
-    finish_point<T>(dv, res, i, bound, b, eps);
+  const long i = long(w[0]);
+  const int m = int(w[1] & 0xffu);
+  Best<T> b = item_best<T>(w);
+  double bound;
+  const v3<T> p = cv3<T>(job_point<S>(j, dv, i, bound));
+  for (int q = 0; q < m; ++q) {
+    const T lb = item_get<T>(w, Item<T>::LB + q * int(sizeof(T) / 4));
+    if (!(lb < b.d)) break;
+    try_part(tb, w[Item<T>::PR + q], lb, p, b, tm, eps);
   }
+  finish_point<T>(dv, res, i, bound, b, eps);
 }
 
-// The last correction in double precision of each single-precision winner; points
-// the search was unsure of are marked (flag 3) and their blocks queued for k_fallback.
+template <int S, class T>
+__global__ __launch_bounds__(256) void k_leaves(Dev dv, Job j, Res res) {
+  const uint32_t n = min(*res.icount, res.icap);
+  for (uint32_t it = blockIdx.x * 256 + threadIdx.x; it < n; it += gridDim.x * 256) leaves_item<S, T>(dv, j, res, res.items + size_t(it) * Item<T>::W);
+}
+
+// The last correction in double precision of point i's single-precision winner; a
+// point the search was unsure of is marked (flag 3) and its block queued (entries are
+// part << 24 | block) for the fallback.
 template <int S>
-__global__ __launch_bounds__(256) void k_refine(Dev dv, Job j, Res res) {
-  long i = long(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= j.n) return;
+__device__ inline void refine_point(const Dev& dv, const Job& j, const Res& res, long i) {
   uint32_t info = res.info[i];
   double bound;
   v3<double> p = job_point<S>(j, dv, i, bound);
   if (info & 4u) {
     res.flag[i] = 3;
     long blk = block_of<S>(j, i);
-    if (atomicOr(&res.qmark[blk], 1u) == 0u) res.queue[atomicAdd(res.qcount, 1u)] = uint32_t(blk);
+    if (atomicOr(&res.qmark[blk], 1u) == 0u) res.queue[atomicAdd(res.qcount, 1u)] = (res.part << 24) | uint32_t(blk);
     atomicAdd(&res.counters[1], 1ull);
     if (info & 16u) atomicAdd(&res.counters[3], 1ull);
     if (info & 32u) atomicAdd(&res.counters[4], 1ull);
@@ -720,99 +735,107 @@ __global__ __launch_bounds__(256) void k_refine(Dev dv, Job j, Res res) {
   if (res.feat) res.feat[i] = feat;
 }
 
-// The queued blocks: each block culls its candidates again and its marked points are
-// searched in double precision.
 template <int S>
-__global__ __launch_bounds__(Shape<S>::NT) void k_fallback(Dev dv, Job j, Res res) {
+__global__ __launch_bounds__(256) void k_refine(Dev dv, Job j, Res res) {
+  long i = long(blockIdx.x) * 256 + threadIdx.x;
+  if (i < j.n) refine_point<S>(dv, j, res, i);
+}
+
+// A queued block searched again in double precision for its points flagged `want`:
+// flag 3, the fallback of the mixed form; flag 4, the settle (parts past their
+// convexity certificate settled by the certified search, as the reference settles
+// them on the CPU, crates/icam-sim/src/partcache.rs:563-751).
+template <int S>
+__device__ inline void redo_block(const Dev& dv, const Job& j, const Res& res, long blk, uint32_t want, SearchLds<S, double>& sm) {
   constexpr int NT = Shape<S>::NT;
-  constexpr int Q = Shape<S>::Q;
-  __shared__ uint32_t cand[Shape<S>::C];
-  __shared__ float red[NT];
-  __shared__ int wave_tot[NT / 64];
-  __shared__ __align__(16) unsigned char scratch[cmax(SORTC * 8, Q * NT * 10)];
-  uint32_t* tmp = (uint32_t*)scratch;
-  float* key = (float*)(scratch + SORTC * 4);
-  double* qlb = (double*)scratch;
-  uint16_t* qk = (uint16_t*)(scratch + Q * NT * 8);
-  const uint32_t total = *res.qcount;
+  const int nc = block_candidates<S>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot);
   const Tab<double> tb = tab_of<double>(dv);
   constexpr int K = Shape<S>::K;
   const bool owner = (threadIdx.x & (K - 1)) == 0;
+  for (int m = 0; m < rounds<S>(); ++m) {
+    long i = block_point<S>(j, blk, m);
+    const bool valid = i >= 0 && res.flag[i] == want;
+    if (K == 1 && !valid) continue;
+    double bound = 0.0;
+    v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
+    Best<double> b = search<double, NT, Shape<S>::Q, K, Shape<S>::C>(tb, p, valid, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, sm.cand, nc, 0.0,
+                                                                     dv.L.eps64, sm.qlb(), sm.qk(), nullptr, 0, want == 4);
+    if (!valid || !owner) continue;
+    double val;
+    uint32_t flag, feat;
+    classify64(b, bound, dv, val, flag, feat);
+    res.val[i] = val;
+    res.flag[i] = flag;
+    if (res.feat) res.feat[i] = feat;
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(Shape<S>::NT) void k_fallback(Dev dv, Job j, Res res) {
+  __shared__ SearchLds<S, double> sm;
+  const uint32_t total = *res.qcount;
   for (uint32_t qb = blockIdx.x; qb < total; qb += gridDim.x) {
-    const long blk = res.queue[qb];
-    const int nc = block_candidates<S>(dv, j, blk, cand, tmp, key, red, wave_tot);
-    for (int m = 0; m < rounds<S>(); ++m) {
-      long i = block_point<S>(j, blk, m);
-      const bool valid = i >= 0 && res.flag[i] == 3;
-      if (K == 1 && !valid) continue;
-      double bound = 0.0;
-      v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
-      Best<double> b = search<double, NT, Shape<S>::Q, K, Shape<S>::C>(tb, p, valid, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, cand, nc, 0.0, dv.L.eps64, qlb, qk);
-      if (!valid || !owner) continue;
-      double val;
-      uint32_t flag, feat;
-      classify64(b, bound, dv, val, flag, feat);
-      res.val[i] = val;
-      res.flag[i] = flag;
-      if (res.feat) res.feat[i] = feat;
-    }
+    redo_block<S>(dv, j, res, long(res.queue[qb] & 0xffffffu), 3u, sm);
     __syncthreads();
   }
 }
 
-// Points left undecided (flag 2) are marked (flag 4) and their blocks queued for
-// k_settle.
+// Point i left undecided (flag 2) is marked (flag 4) and its block queued for the
+// settle.
 template <int S>
-__global__ __launch_bounds__(256) void k_mark_settle(Job j, Res res) {
-  long i = long(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= j.n || res.flag[i] != 2) return;
+__device__ inline void mark_point(const Job& j, const Res& res, long i) {
+  if (res.flag[i] != 2) return;
   res.flag[i] = 4;
   long blk = block_of<S>(j, i);
-  if (atomicOr(&res.smark[blk], 1u) == 0u) res.squeue[atomicAdd(res.scount, 1u)] = uint32_t(blk);
+  if (atomicOr(&res.smark[blk], 1u) == 0u) res.squeue[atomicAdd(res.scount, 1u)] = (res.part << 24) | uint32_t(blk);
   atomicAdd(&res.counters[5], 1ull);
 }
 
-// The queued blocks: each block culls its candidates again and its marked points are
-// searched in double precision with the parts past their convexity certificate
-// settled by the certified search (the reference settles them on the CPU,
-// crates/icam-sim/src/partcache.rs:563-751).
+template <int S>
+__global__ __launch_bounds__(256) void k_mark_settle(Job j, Res res) {
+  long i = long(blockIdx.x) * 256 + threadIdx.x;
+  if (i < j.n) mark_point<S>(j, res, i);
+}
+
 template <int S>
 __global__ __launch_bounds__(Shape<S>::NT) void k_settle(Dev dv, Job j, Res res) {
-  constexpr int NT = Shape<S>::NT;
-  constexpr int Q = Shape<S>::Q;
-  __shared__ uint32_t cand[Shape<S>::C];
-  __shared__ float red[NT];
-  __shared__ int wave_tot[NT / 64];
-  __shared__ __align__(16) unsigned char scratch[cmax(SORTC * 8, Q * NT * 10)];
-  uint32_t* tmp = (uint32_t*)scratch;
-  float* key = (float*)(scratch + SORTC * 4);
-  double* qlb = (double*)scratch;
-  uint16_t* qk = (uint16_t*)(scratch + Q * NT * 8);
+  __shared__ SearchLds<S, double> sm;
   const uint32_t total = *res.scount;
-  const Tab<double> tb = tab_of<double>(dv);
-  constexpr int K = Shape<S>::K;
-  const bool owner = (threadIdx.x & (K - 1)) == 0;
   for (uint32_t qb = blockIdx.x; qb < total; qb += gridDim.x) {
-    const long blk = res.squeue[qb];
-    const int nc = block_candidates<S>(dv, j, blk, cand, tmp, key, red, wave_tot);
-    for (int m = 0; m < rounds<S>(); ++m) {
-      long i = block_point<S>(j, blk, m);
-      const bool valid = i >= 0 && res.flag[i] == 4;
-      if (K == 1 && !valid) continue;
-      double bound = 0.0;
-      v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
-      Best<double> b = search<double, NT, Shape<S>::Q, K, Shape<S>::C>(tb, p, valid, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, cand, nc, 0.0,
-                                                                       dv.L.eps64, qlb, qk, nullptr, 0, true);
-      if (!valid || !owner) continue;
-      double val;
-      uint32_t flag, feat;
-      classify64(b, bound, dv, val, flag, feat);
-      res.val[i] = val;
-      res.flag[i] = flag;
-      if (res.feat) res.feat[i] = feat;
-    }
+    redo_block<S>(dv, j, res, long(res.squeue[qb] & 0xffffffu), 4u, sm);
     __syncthreads();
   }
+}
+
+// A band brick whose nodes all lie past the clamp takes its centre's sign. When the
+// decision left that centre undecided and no sign came to it along the grid, the
+// centre is marked (flag 4) for the settle; centre_sign then takes its settled sign.
+__device__ inline void need_centre(const Job& jc, const Res& rc, const uint32_t* list, long bi, const uint32_t* nflag, const double* dc,
+                                   const int8_t* sgn) {
+  int far = 1;
+  for (int n = threadIdx.x; n < 729; n += blockDim.x) far &= nflag[729 * bi + n] == 1u;
+  far = __syncthreads_and(far);
+  if (threadIdx.x == 0 && far) {
+    const long b = long(list[bi]);
+    if (dc[b] != dc[b] && sgn[b] == 0) mark_point<SRC_CENTRES>(jc, rc, b);
+  }
+}
+__device__ inline void centre_sign(const Res& rc, const uint32_t* list, long bi, double* dc, int8_t* sgn) {
+  const long b = long(list[bi]);
+  if (dc[b] != dc[b] && sgn[b] == 0 && rc.flag[b] == 0) {
+    dc[b] = rc.val[b];
+    sgn[b] = rc.val[b] < 0.0 ? int8_t(-1) : int8_t(1);
+  }
+}
+__global__ __launch_bounds__(256) void k_need_centre(Job jc, Res rc, const uint32_t* list, long nband, const uint32_t* nflag, const double* dc,
+                                                      const int8_t* sgn) {
+  for (long bi = blockIdx.x; bi < nband; bi += gridDim.x) {
+    need_centre(jc, rc, list, bi, nflag, dc, sgn);
+    __syncthreads();
+  }
+}
+__global__ __launch_bounds__(256) void k_centre_sign(Res rc, const uint32_t* list, long nband, double* dc, int8_t* sgn) {
+  for (long bi = long(blockIdx.x) * 256 + threadIdx.x; bi < nband; bi += long(gridDim.x) * 256) centre_sign(rc, list, bi, dc, sgn);
 }
 
 // ------------------------------------------------------------------ the level
@@ -820,9 +843,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_settle(Dev dv, Job j, Res res)
 // The decision from the centres' answers: dc the signed distance or NaN; band 1 when
 // the brick is kept (|dc| <= reach, or undecided under the reach); sign +-1 when
 // known, else 0.
-__global__ void k_decide_finish(Dev dv, Lv lv, long nbricks, const double* val, const uint32_t* flag, double* dc, uint8_t* band, int8_t* sgn) {
-  long b = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b >= nbricks) return;
+__device__ inline void decide_brick(const Dev& dv, const Lv& lv, long b, const double* val, const uint32_t* flag, double* dc, uint8_t* band, int8_t* sgn) {
   double reach = brick_reach(lv.h);
   double v = val[b];
   uint32_t f = flag[b];
@@ -841,6 +862,10 @@ __global__ void k_decide_finish(Dev dv, Lv lv, long nbricks, const double* val, 
     if (c.x < L0[0] || c.y < L0[1] || c.z < L0[2] || c.x > L1[0] || c.y > L1[1] || c.z > L1[2]) s = 1;
   }
   sgn[b] = s;
+}
+__global__ void k_decide_finish(Dev dv, Lv lv, long nbricks, const double* val, const uint32_t* flag, double* dc, uint8_t* band, int8_t* sgn) {
+  long b = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b < nbricks) decide_brick(dv, lv, b, val, flag, dc, band, sgn);
 }
 
 // Signs along grid lines of axis `ax`: a brick no surface comes within its reach of
@@ -881,9 +906,7 @@ __global__ void k_sign(Lv lv, int ax, const uint8_t* band, int8_t* sgn, int* cha
 // x fastest (or y, for the x sweep), so a wave's lanes read distinct LDS banks.
 constexpr int SIGN_T = 1024, SIGN_LDS = 96 * 1024, SIGN_ROUNDS = 8;
 __host__ __device__ inline long sign_pitch(int nb0) { return 4L * (2L * ((nb0 + 7) / 8) + 1); }
-__global__ __launch_bounds__(SIGN_T) void k_sign_block(Lv lv, const uint8_t* band, int8_t* sgn) {
-  __shared__ uint8_t g[SIGN_LDS];
-  __shared__ int changed;
+__device__ inline void sign_block(const Lv& lv, const uint8_t* band, int8_t* sgn, uint8_t* g, int& changed) {
   const int nb[3] = {lv.nb[0], lv.nb[1], lv.nb[2]};
   const int P = int(sign_pitch(lv.nb[0]));
   const int rows = nb[1] * nb[2];  // row r = y + nb[1] z: nb[0] bricks along x
@@ -932,6 +955,11 @@ __global__ __launch_bounds__(SIGN_T) void k_sign_block(Lv lv, const uint8_t* ban
   for (int r = threadIdx.x; r < rows; r += SIGN_T)
     for (int x = 0; x < nb[0]; ++x) sgn[long(r) * nb[0] + x] = int8_t(int(g[P * r + x] & 3) - 1);
 }
+__global__ __launch_bounds__(SIGN_T) void k_sign_block(Lv lv, const uint8_t* band, int8_t* sgn) {
+  __shared__ uint8_t g[SIGN_LDS];
+  __shared__ int changed;
+  sign_block(lv, band, sgn, g, changed);
+}
 
 // Positions of the bricks a predicate keeps, in order, by one block: each thread
 // counts a run of consecutive entries (a multiple of 16, read 16 bytes at a time), the
@@ -940,8 +968,7 @@ __global__ __launch_bounds__(SIGN_T) void k_sign_block(Lv lv, const uint8_t* ban
 // count. The predicate: band[i] != 0 (band), or cls[i] == 0 (cls: a band brick stored
 // with words).
 constexpr int SCAN_T = 1024;
-__global__ __launch_bounds__(SCAN_T) void k_scan(const uint8_t* band, const uint32_t* cls, long n, uint32_t* out, uint32_t* list, uint32_t* total) {
-  __shared__ uint32_t s[SCAN_T];
+__device__ inline void scan_block(const uint8_t* band, const uint32_t* cls, long n, uint32_t* out, uint32_t* list, uint32_t* total, uint32_t* s) {
   const int tid = threadIdx.x;
   const long run = ((n + SCAN_T - 1) / SCAN_T + 15) / 16 * 16, i0 = tid * run, i1 = i0 + run < n ? i0 + run : n;
   // the run's keep bits, 16 entries a word
@@ -979,6 +1006,10 @@ __global__ __launch_bounds__(SCAN_T) void k_scan(const uint8_t* band, const uint
   }
   if (tid == SCAN_T - 1) *total = s[SCAN_T - 1];
 }
+__global__ __launch_bounds__(SCAN_T) void k_scan(const uint8_t* band, const uint32_t* cls, long n, uint32_t* out, uint32_t* list, uint32_t* total) {
+  __shared__ uint32_t s[SCAN_T];
+  scan_block(band, cls, n, out, list, total, s);
+}
 
 __device__ inline int nb26(int n, int k) {
   // the k-th of the 26 neighbours of node n (x fastest), or -1 outside the brick
@@ -997,16 +1028,19 @@ __device__ inline double nb26_len(int k) {
 // A band brick from its nodes' answers: far nodes signed by flood fill from the nodes
 // known, then from the centre's empty ball, what is left bridged; quantized, classed
 // and packed.
-__global__ __launch_bounds__(WG) void k_bricks_finish(Lv lv, const uint32_t* list, long nbricks, const double* dcs, const int8_t* sgns,
-                                                      const double* val_in, const uint32_t* flag_in, uint32_t* staging, uint32_t* cls,
-                                                      unsigned long long* counters) {
-  __shared__ double val[729];
-  __shared__ uint8_t st[729];  // 0 exact, 1 far, 2 undecided, 3 filled, 4 bridged
-  __shared__ int changed;
-  __shared__ int16_t q[730];
+struct FinishLds {
+  double val[729];
+  uint8_t st[729];  // 0 exact, 1 far, 2 undecided, 3 filled, 4 bridged
+  int changed;
+  int16_t q[730];
+};
+__device__ inline void finish_brick(const Lv& lv, const uint32_t* list, long bi, const double* dcs, const int8_t* sgns, const double* val_in,
+                                    const uint32_t* flag_in, uint32_t* staging, uint32_t* cls, unsigned long long* counters, FinishLds& sm) {
+  double* val = sm.val;
+  uint8_t* st = sm.st;
+  int& changed = sm.changed;
+  int16_t* q = sm.q;
   const int tid = threadIdx.x;
-  long bi = blockIdx.x;
-  if (bi >= nbricks) return;
   const double h = lv.h, clamp = h * 1.7320508075688772, half = 4.0 * h;
   for (int n = tid; n < 729; n += WG) {
     val[n] = val_in[729 * bi + n];
@@ -1118,33 +1152,345 @@ __global__ __launch_bounds__(WG) void k_bricks_finish(Lv lv, const uint32_t* lis
   if (c == 0)
     for (int w = tid; w < ICAM_WORDS; w += WG) staging[size_t(bi) * ICAM_WORDS + w] = uint32_t(uint16_t(q[2 * w])) | (uint32_t(uint16_t(q[2 * w + 1])) << 16);
 }
+__global__ __launch_bounds__(WG) void k_bricks_finish(Lv lv, const uint32_t* list, long nbricks, const double* dcs, const int8_t* sgns,
+                                                      const double* val_in, const uint32_t* flag_in, uint32_t* staging, uint32_t* cls,
+                                                      unsigned long long* counters) {
+  __shared__ FinishLds sm;
+  if (long(blockIdx.x) < nbricks) finish_brick(lv, list, blockIdx.x, dcs, sgns, val_in, flag_in, staging, cls, counters, sm);
+}
 
 
 // partcache.rs:128-146 assemble: the map (AIR, SOLID, or the slot) and the atlas in
 // map order (band bricks are listed in map order, so their slots are too). A brick
 // kept whole whose sign was never found is counted (counters[2]).
-__global__ void k_store(long nbricks, const uint8_t* band, const int8_t* sgn, uint32_t* map, unsigned long long* counters) {
-  long b = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b >= nbricks) return;
+__device__ inline void store_brick(long b, const uint8_t* band, const int8_t* sgn, uint32_t* map, unsigned long long* counters) {
   if (!band[b]) {
     map[b] = sgn[b] < 0 ? ICAM_SOLID : ICAM_AIR;
     if (sgn[b] == 0) atomicAdd(&counters[2], 1ull);
   }
 }
-__global__ void k_store_band(long nband, const uint32_t* list, const uint32_t* cls, const uint32_t* slot, const uint32_t* staging, uint32_t* map,
-                             uint32_t* atlas) {
-  long bi = blockIdx.x;
-  if (bi >= nband) return;
+__global__ void k_store(long nbricks, const uint8_t* band, const int8_t* sgn, uint32_t* map, unsigned long long* counters) {
+  long b = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b < nbricks) store_brick(b, band, sgn, map, counters);
+}
+// Band brick bi's map entry and atlas words (its slot's), by a block of any size.
+__device__ inline void store_band_brick(long bi, const uint32_t* list, const uint32_t* cls, const uint32_t* slot, const uint32_t* staging, uint32_t* map,
+                                        uint32_t* atlas) {
   uint32_t c = cls[bi];
   if (threadIdx.x == 0) map[list[bi]] = c == 1 ? ICAM_AIR : c == 2 ? ICAM_SOLID : slot[bi];
   if (c == 0)
     for (int w = threadIdx.x; w < ICAM_WORDS; w += blockDim.x) atlas[size_t(slot[bi]) * ICAM_WORDS + w] = staging[size_t(bi) * ICAM_WORDS + w];
+}
+__global__ void k_store_band(long nband, const uint32_t* list, const uint32_t* cls, const uint32_t* slot, const uint32_t* staging, uint32_t* map,
+                             uint32_t* atlas) {
+  if (long(blockIdx.x) < nband) store_band_brick(blockIdx.x, list, cls, slot, staging, map, atlas);
 }
 
 __global__ void k_spin(float* x, int iters) {
   float a = x[threadIdx.x];
   for (int i = 0; i < iters; ++i) a = a * 0.999f + 0.001f;
   x[threadIdx.x] = a;
+}
+
+// ------------------------------------------------------------------ batches
+//
+// Many parts' levels baked by one sequence of launches (captured once in a hipGraph):
+// each stage is one launch over every part's work. The block stages (the centres' and
+// the nodes' searches, the fallback and settle, the band bricks' finish and store) are
+// persistent work queues: a fixed grid whose blocks take the next work item with an
+// atomic counter until none is left, the item's part found from prefix offsets; the
+// point stages stride over all parts' points; the per-part scans take a block each.
+// Parts are ordered heaviest first by the host, so the queue serves the long bricks
+// early (longest-processing-time order).
+
+// A part of a batch: its tables, level, jobs and results, and its level's arrays.
+struct BPart {
+  Dev dv;
+  Lv lv;
+  Job jc, jn;  // the centres' and the band nodes' jobs
+  Res rc, rn;
+  long nb;     // bricks
+  uint32_t cap;  // band bricks it has room for
+  double* dc;
+  uint8_t* band;
+  int8_t* sgn;
+  uint32_t* list;
+  uint32_t* staging;
+  uint32_t* cls;
+  uint32_t* slot;
+  uint32_t* map;
+  uint32_t* atlas;
+  unsigned long long* counters;
+};
+
+// The control words of a batch (zeroed before each run).
+enum BCtl : uint32_t {
+  BC_NEXT = 0,      // work counters of the persistent stages (16)
+  BC_QC = 16,       // centres' fallback queue count
+  BC_SC = 17,       // centres' settle queue count
+  BC_QN = 18,       // nodes' fallback queue count
+  BC_SN = 19,       // nodes' settle queue count
+  BC_ITEMS = 20,    // deferred items
+  BC_OVER = 21,     // a part whose band outgrew its room
+  BC_WORDS = 32,
+};
+enum BStage : int { BS_CSEARCH = 0, BS_CFALL, BS_CSETTLE, BS_NSEARCH, BS_NFALL, BS_NSETTLE, BS_FINISH, BS_STORE };
+
+struct BatchArgs {
+  const BPart* parts;
+  int n;
+  const long* brick_off;  // n + 1: bricks of the parts before p
+  const long* tile_off;   // n + 1: tiles
+  uint32_t* band_off;     // n + 1: band bricks (written by kb_plan)
+  uint32_t* btotal;       // per part: band bricks its scan found
+  uint32_t* stotal;       // per part: slots
+  uint32_t* ctl;
+};
+
+// The part whose range [off[p], off[p + 1]) holds w.
+template <class O> __device__ inline int part_of(const O* off, int n, O w) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= w)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+__device__ inline const ICAM_K BPart& bpart(const BatchArgs& a, int p) { return ((const ICAM_K BPart*)a.parts)[p]; }
+// A record copied out of the constant address space, a dword at a time (the loads are
+// uniform, so they are scalar; fields the caller never reads are dropped).
+template <class X> __device__ inline X ldk(const ICAM_K X& src) {
+  static_assert(sizeof(X) % 4 == 0, "dword records");
+  X out;
+  const ICAM_K uint32_t* s = reinterpret_cast<const ICAM_K uint32_t*>(&src);
+  uint32_t* d = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (int i = 0; i < int(sizeof(X) / 4); ++i) d[i] = s[i];
+  return out;
+}
+
+// The next work item of stage st for the whole block (uniform), or `total` and past.
+__device__ inline uint32_t next_item(const BatchArgs& a, int st, uint32_t* slot) {
+  if (threadIdx.x == 0) *slot = atomicAdd(&a.ctl[BC_NEXT + st], 1u);
+  __syncthreads();
+  const uint32_t w = __builtin_amdgcn_readfirstlane(*slot);
+  __syncthreads();
+  return w;
+}
+
+__global__ void kb_fill(uint32_t* p, size_t n, uint32_t v) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+
+// The searches, persistent: centres over every part's tiles, nodes over every part's
+// band bricks.
+template <int S, class T>
+__global__ __launch_bounds__(Shape<S>::NT) void kb_search(BatchArgs a) {
+  __shared__ SearchLds<S, T> sm;
+  __shared__ uint32_t wslot;
+  const uint32_t total = S == SRC_CENTRES ? uint32_t(a.tile_off[a.n]) : a.band_off[a.n];
+  for (;;) {
+    const uint32_t w = next_item(a, S == SRC_CENTRES ? BS_CSEARCH : BS_NSEARCH, &wslot);
+    if (w >= total) break;
+    int p;
+    long blk;
+    if (S == SRC_CENTRES) {
+      p = part_of<long>(a.tile_off, a.n, long(w));
+      blk = long(w) - a.tile_off[p];
+    } else {
+      p = part_of<uint32_t>(a.band_off, a.n, w);
+      blk = long(w - a.band_off[p]);
+    }
+    const ICAM_K BPart& P = bpart(a, p);
+    const Dev dv = ldk(P.dv);
+    const Job j = S == SRC_CENTRES ? ldk(P.jc) : ldk(P.jn);
+    const Res r = S == SRC_CENTRES ? ldk(P.rc) : ldk(P.rn);
+    search_block<S, T>(dv, j, r, blk, sm);
+    __syncthreads();
+  }
+}
+
+// The points of every part: centres (static counts) or band nodes (from the plan).
+template <int S> __device__ inline long batch_points(const BatchArgs& a) { return S == SRC_CENTRES ? a.brick_off[a.n] : long(a.band_off[a.n]) * 729; }
+template <int S> __device__ inline int point_part(const BatchArgs& a, long g, long& i) {
+  if (S == SRC_CENTRES) {
+    int p = part_of<long>(a.brick_off, a.n, g);
+    i = g - a.brick_off[p];
+    return p;
+  }
+  const uint32_t bi = uint32_t(g / 729);
+  int p = part_of<uint32_t>(a.band_off, a.n, bi);
+  i = g - long(a.band_off[p]) * 729;
+  return p;
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void kb_refine(BatchArgs a) {
+  const long total = batch_points<S>(a);
+  for (long g = long(blockIdx.x) * 256 + threadIdx.x; g < total; g += long(gridDim.x) * 256) {
+    long i;
+    const int p = point_part<S>(a, g, i);
+    const ICAM_K BPart& P = bpart(a, p);
+    const Dev dv = ldk(P.dv);
+    const Job j = S == SRC_CENTRES ? ldk(P.jc) : ldk(P.jn);
+    const Res r = S == SRC_CENTRES ? ldk(P.rc) : ldk(P.rn);
+    refine_point<S>(dv, j, r, i);
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void kb_mark(BatchArgs a) {
+  const long total = batch_points<S>(a);
+  for (long g = long(blockIdx.x) * 256 + threadIdx.x; g < total; g += long(gridDim.x) * 256) {
+    long i;
+    const int p = point_part<S>(a, g, i);
+    const ICAM_K BPart& P = bpart(a, p);
+    const Job j = S == SRC_CENTRES ? ldk(P.jc) : ldk(P.jn);
+    const Res r = S == SRC_CENTRES ? ldk(P.rc) : ldk(P.rn);
+    mark_point<S>(j, r, i);
+  }
+}
+
+// The fallback (want 3) or the settle (want 4) over the batch's shared queue, whose
+// entries are part << 24 | block.
+template <int S>
+__global__ __launch_bounds__(Shape<S>::NT) void kb_redo(BatchArgs a, uint32_t want) {
+  __shared__ SearchLds<S, double> sm;
+  __shared__ uint32_t wslot;
+  const bool settle = want == 4;
+  const int st = S == SRC_CENTRES ? (settle ? BS_CSETTLE : BS_CFALL) : (settle ? BS_NSETTLE : BS_NFALL);
+  const uint32_t total = a.ctl[S == SRC_CENTRES ? (settle ? BC_SC : BC_QC) : (settle ? BC_SN : BC_QN)];
+  for (;;) {
+    const uint32_t w = next_item(a, st, &wslot);
+    if (w >= total) break;
+    const ICAM_K BPart& P0 = bpart(a, 0);
+    const Res r0 = S == SRC_CENTRES ? ldk(P0.rc) : ldk(P0.rn);
+    const uint32_t e = __builtin_amdgcn_readfirstlane((settle ? r0.squeue : r0.queue)[w]);
+    const ICAM_K BPart& P = bpart(a, int(e >> 24));
+    const Dev dv = ldk(P.dv);
+    const Job j = S == SRC_CENTRES ? ldk(P.jc) : ldk(P.jn);
+    const Res r = S == SRC_CENTRES ? ldk(P.rc) : ldk(P.rn);
+    redo_block<S>(dv, j, r, long(e & 0xffffffu), want, sm);
+    __syncthreads();
+  }
+}
+
+// The dense pass over every part's deferred points (the part in an item's word 1).
+template <class T>
+__global__ __launch_bounds__(256) void kb_leaves(BatchArgs a) {
+  const ICAM_K BPart& P0 = bpart(a, 0);
+  const uint32_t n = min(a.ctl[BC_ITEMS], P0.rn.icap);
+  const uint32_t* items = P0.rn.items;
+  for (uint32_t it = blockIdx.x * 256 + threadIdx.x; it < n; it += gridDim.x * 256) {
+    const uint32_t* w = items + size_t(it) * Item<T>::W;
+    const ICAM_K BPart& P = bpart(a, int(w[1] >> 8));
+    const Dev dv = ldk(P.dv);
+    const Job j = ldk(P.jn);
+    const Res r = ldk(P.rn);
+    leaves_item<SRC_NODES, T>(dv, j, r, w);
+  }
+}
+
+__global__ void kb_decide(BatchArgs a) {
+  const long total = a.brick_off[a.n];
+  for (long g = long(blockIdx.x) * blockDim.x + threadIdx.x; g < total; g += long(gridDim.x) * blockDim.x) {
+    const int p = part_of<long>(a.brick_off, a.n, g);
+    const ICAM_K BPart& P = bpart(a, p);
+    const Dev dv = ldk(P.dv);
+    const Lv lv = ldk(P.lv);
+    decide_brick(dv, lv, g - a.brick_off[p], P.rc.val, P.rc.flag, P.dc, P.band, P.sgn);
+  }
+}
+
+// One block a part: the signs of its bricks kept whole, then its band list.
+__global__ __launch_bounds__(SIGN_T) void kb_sign(BatchArgs a) {
+  __shared__ uint8_t g[SIGN_LDS];
+  __shared__ int changed;
+  const ICAM_K BPart& P = bpart(a, blockIdx.x);
+  const Lv lv = ldk(P.lv);
+  sign_block(lv, P.band, P.sgn, g, changed);
+}
+__global__ __launch_bounds__(SCAN_T) void kb_band_list(BatchArgs a) {
+  __shared__ uint32_t s[SCAN_T];
+  const ICAM_K BPart& P = bpart(a, blockIdx.x);
+  scan_block(P.band, nullptr, P.nb, nullptr, P.list, &a.btotal[blockIdx.x], s);
+}
+// The band bricks' offsets (each part's count held to its room; a part past it is
+// flagged).
+__global__ void kb_plan(BatchArgs a) {
+  if (threadIdx.x != 0) return;
+  uint32_t acc = 0;
+  for (int p = 0; p < a.n; ++p) {
+    a.band_off[p] = acc;
+    const uint32_t cap = bpart(a, p).cap, t = a.btotal[p];
+    if (t > cap) a.ctl[BC_OVER] = uint32_t(p) + 1u;
+    acc += t < cap ? t : cap;
+  }
+  a.band_off[a.n] = acc;
+}
+
+__global__ __launch_bounds__(256) void kb_need_centre(BatchArgs a) {
+  const uint32_t total = a.band_off[a.n];
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+    const int p = part_of<uint32_t>(a.band_off, a.n, w);
+    const ICAM_K BPart& P = bpart(a, p);
+    const Job jc = ldk(P.jc);
+    const Res rc = ldk(P.rc);
+    need_centre(jc, rc, P.list, long(w - a.band_off[p]), P.rn.flag, P.dc, P.sgn);
+    __syncthreads();
+  }
+}
+__global__ __launch_bounds__(256) void kb_centre_sign(BatchArgs a) {
+  const uint32_t total = a.band_off[a.n];
+  for (uint32_t w = blockIdx.x * 256 + threadIdx.x; w < total; w += gridDim.x * 256) {
+    const int p = part_of<uint32_t>(a.band_off, a.n, w);
+    const ICAM_K BPart& P = bpart(a, p);
+    const Res rc = ldk(P.rc);
+    centre_sign(rc, P.list, long(w - a.band_off[p]), P.dc, P.sgn);
+  }
+}
+
+__global__ __launch_bounds__(WG) void kb_finish(BatchArgs a) {
+  __shared__ FinishLds sm;
+  __shared__ uint32_t wslot;
+  const uint32_t total = a.band_off[a.n];
+  for (;;) {
+    const uint32_t w = next_item(a, BS_FINISH, &wslot);
+    if (w >= total) break;
+    const int p = part_of<uint32_t>(a.band_off, a.n, w);
+    const ICAM_K BPart& P = bpart(a, p);
+    const Lv lv = ldk(P.lv);
+    finish_brick(lv, P.list, long(w - a.band_off[p]), P.dc, P.sgn, P.rn.val, P.rn.flag, P.staging, P.cls, P.counters, sm);
+    __syncthreads();
+  }
+}
+__global__ __launch_bounds__(SCAN_T) void kb_slots(BatchArgs a) {
+  __shared__ uint32_t s[SCAN_T];
+  const ICAM_K BPart& P = bpart(a, blockIdx.x);
+  const uint32_t nb = min(a.btotal[blockIdx.x], P.cap);
+  scan_block(nullptr, P.cls, long(nb), P.slot, nullptr, &a.stotal[blockIdx.x], s);
+}
+__global__ void kb_store(BatchArgs a) {
+  const long total = a.brick_off[a.n];
+  for (long g = long(blockIdx.x) * blockDim.x + threadIdx.x; g < total; g += long(gridDim.x) * blockDim.x) {
+    const int p = part_of<long>(a.brick_off, a.n, g);
+    const ICAM_K BPart& P = bpart(a, p);
+    store_brick(g - a.brick_off[p], P.band, P.sgn, P.map, P.counters);
+  }
+}
+__global__ __launch_bounds__(128) void kb_store_band(BatchArgs a) {
+  __shared__ uint32_t wslot;
+  const uint32_t total = a.band_off[a.n];
+  for (;;) {
+    const uint32_t w = next_item(a, BS_STORE, &wslot);
+    if (w >= total) break;
+    const int p = part_of<uint32_t>(a.band_off, a.n, w);
+    const ICAM_K BPart& P = bpart(a, p);
+    store_band_brick(long(w - a.band_off[p]), P.list, P.cls, P.slot, P.staging, P.map, P.atlas);
+  }
 }
 
 // ------------------------------------------------------------------ host side
@@ -1308,7 +1654,9 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
     HIP_OK(launch(st, names[S][3], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, double>, h->dv, j, r));
     if (lb) HIP_OK(launch(st, names[S][5], s, dim3(lb), dim3(256), k_leaves<S, double>, h->dv, j, r));
   }
-  // what is still undecided: settled by the certified search
+  // what is still undecided: settled by the certified search (not the brick centres:
+  // an undecided centre under its reach keeps its brick, whose nodes are read)
+  if (S == SRC_CENTRES) return 0;
   HIP_OK(hipMemsetAsync(r.scount, 0, 4, s));
   HIP_OK(hipMemsetAsync(r.smark, 0, size_t(blocks) * 4, s));
   HIP_OK(launch(st, names[S][6], s, dim3(unsigned((j.n + 255) / 256)), dim3(256), k_mark_settle<S>, j, r));
@@ -1547,6 +1895,15 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     jn.n = np, jn.list = h->list.as<uint32_t>(), jn.lv = lv;
     jn.tile_cand = h->tile_cand.as<uint32_t>(), jn.tile_count = h->tile_count.as<uint32_t>();
     if (run_search<SRC_NODES>(h, mixed, jn, rn, s, &st, long(nband))) return 1;
+    // the centres of all-far band bricks with no sign: settled
+    HIP_OK(hipMemsetAsync(r.scount, 0, 4, s));
+    HIP_OK(hipMemsetAsync(r.smark, 0, size_t(tiles) * 4, s));
+    HIP_OK(launch(&st, "centres_need", s, dim3(unsigned(std::min<long>(nband, 4096))), dim3(256), k_need_centre, jc, r, (const uint32_t*)h->list.as<uint32_t>(),
+                  long(nband), (const uint32_t*)rn.flag, (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>()));
+    HIP_OK(launch(&st, "centres_settle64", s, dim3(unsigned(std::min<long>(tiles, 1024))), dim3(Shape<SRC_CENTRES>::NT), k_settle<SRC_CENTRES>, h->dv, jc,
+                  r));
+    HIP_OK(launch(&st, "centres_sign", s, dim3(unsigned(std::min<long>((long(nband) + 255) / 256, 4096))), dim3(256), k_centre_sign, r,
+                  (const uint32_t*)h->list.as<uint32_t>(), long(nband), h->dc.as<double>(), h->sgn.as<int8_t>()));
     HIP_OK(launch(&st, "bricks_finish", s, dim3(nband), dim3(WG), k_bricks_finish, lv, (const uint32_t*)h->list.as<uint32_t>(), long(nband),
                   (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>(), (const double*)rn.val, (const uint32_t*)rn.flag,
                   h->staging.as<uint32_t>(), h->cls.as<uint32_t>(), h->counters.as<unsigned long long>()));
@@ -1615,6 +1972,372 @@ int icam_hip_stage_times(icam_hip* h, const char** names, double* ms, int cap) {
     names[n] = s.first, ms[n] = s.second, ++n;
   }
   return n;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches (host)
+
+struct icam_batch {
+  int precision = ICAM_MIXED;
+  int n = 0;
+  std::vector<icam_hip*> parts;  // in batch order (heaviest first)
+  std::vector<int> caller;       // batch position -> the caller's index
+  std::vector<int> pos;          // the caller's index -> batch position
+  std::vector<BPart> host;       // the records uploaded
+  std::vector<long> brick_off, tile_off;
+  std::vector<size_t> cap_off;   // band room before each part
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  Buf bparts, brick_off_d, tile_off_d, band_off, btotal, stotal, ctl, counters;
+  Buf c_code, c_pa, c_pb, c_d32, c_info, c_val, c_flag, c_qmark, c_smark, c_queue, c_squeue;
+  Buf n_code, n_pa, n_pb, n_d32, n_info, n_val, n_flag, n_qmark, n_smark, n_queue, n_squeue, items;
+  Buf tile_cand, tile_count, dc, band, sgn, list, staging, cls, slot, map, atlas;
+  long total_bricks = 0, total_tiles = 0;
+  size_t total_cap = 0;
+  // the launch shapes: persistent grids of the centres' search, the nodes' search
+  // and the band bricks' finish (blocks)
+  int gc = 2304, gn = 1024, gf = 2048;
+  std::string tuning;
+  BatchArgs args{};
+  ~icam_batch() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    for (Buf* b : {&bparts, &brick_off_d, &tile_off_d, &band_off, &btotal, &stotal, &ctl, &counters, &c_code, &c_pa, &c_pb, &c_d32, &c_info, &c_val,
+                   &c_flag, &c_qmark, &c_smark, &c_queue, &c_squeue, &n_code, &n_pa, &n_pb, &n_d32, &n_info, &n_val, &n_flag, &n_qmark, &n_smark,
+                   &n_queue, &n_squeue, &items, &tile_cand, &tile_count, &dc, &band, &sgn, &list, &staging, &cls, &slot, &map, &atlas})
+      b->free();
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Room for a part's band bricks: the bricks whose centre lies within the reach of
+// some primitive's box (a band brick's centre is within its reach of the surface, or
+// of a part's lower bound, and every surface point and bound lies in a primitive's
+// box).
+uint32_t band_room(const icam_hip* h, const Lv& lv) {
+  const long nb0 = lv.nb[0], nb1 = lv.nb[1], nb2 = lv.nb[2];
+  std::vector<uint8_t> mark(size_t(nb0 * nb1 * nb2), 0);
+  const double reach = brick_reach(lv.h) * (1.0 + 1e-6) + 1e-6 * h->pk.L.diag, side = 8.0 * lv.h;
+  const long nbs[3] = {nb0, nb1, nb2};
+  const std::vector<float>& bx = h->pk.pbox;
+  for (size_t k = 0; k < bx.size() / 6; ++k) {
+    long a[3], b[3];
+    bool empty = false;
+    for (int ax = 0; ax < 3; ++ax) {
+      const double c0 = lv.lo[ax] + 4.0 * lv.h;  // the first centre
+      const double lo = double(bx[6 * k + ax]) + h->pk.origin[ax] - reach, hi = double(bx[6 * k + 3 + ax]) + h->pk.origin[ax] + reach;
+      a[ax] = std::max(long(std::ceil((lo - c0) / side)), 0L);
+      b[ax] = std::min(long(std::floor((hi - c0) / side)), nbs[ax] - 1);
+      empty |= a[ax] > b[ax];
+    }
+    if (empty) continue;
+    for (long z = a[2]; z <= b[2]; ++z)
+      for (long y = a[1]; y <= b[1]; ++y)
+        for (long x = a[0]; x <= b[0]; ++x) mark[size_t(x + nb0 * (y + nb1 * z))] = 1;
+  }
+  size_t c = 0;
+  for (uint8_t m : mark) c += m;
+  return uint32_t(c);
+}
+
+// The batch's launches on stream s.
+int batch_enqueue(icam_batch* B, hipStream_t s) {
+  const BatchArgs& a = B->args;
+  const bool mixed = B->precision == ICAM_MIXED;
+  auto fill = [&](Buf& b, size_t words) -> hipError_t {
+    if (!words) return hipSuccess;
+    hipLaunchKernelGGL(kb_fill, dim3(unsigned(std::min<size_t>((words + 255) / 256, 4096))), dim3(256), 0, s, b.as<uint32_t>(), words, 0u);
+    return hipGetLastError();
+  };
+  HIP_OK(fill(B->ctl, BC_WORDS));
+  HIP_OK(fill(B->counters, size_t(B->n) * 16));
+  HIP_OK(fill(B->c_qmark, size_t(B->total_tiles)));
+  HIP_OK(fill(B->c_smark, size_t(B->total_tiles)));
+  HIP_OK(fill(B->n_qmark, B->total_cap));
+  HIP_OK(fill(B->n_smark, B->total_cap));
+  const unsigned pts_c = unsigned(std::min<long>((B->total_bricks + 255) / 256, 8192));
+  const unsigned pts_n = unsigned(std::min<size_t>((B->total_cap * 729 + 255) / 256, 16384));
+  // the brick centres: the level's decision
+  if (mixed) {
+    hipLaunchKernelGGL((kb_search<SRC_CENTRES, float>), dim3(B->gc), dim3(Shape<SRC_CENTRES>::NT), 0, s, a);
+    hipLaunchKernelGGL((kb_refine<SRC_CENTRES>), dim3(pts_c), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((kb_redo<SRC_CENTRES>), dim3(B->gc), dim3(Shape<SRC_CENTRES>::NT), 0, s, a, 3u);
+  } else {
+    hipLaunchKernelGGL((kb_search<SRC_CENTRES, double>), dim3(B->gc), dim3(Shape<SRC_CENTRES>::NT), 0, s, a);
+  }
+  HIP_OK(hipGetLastError());
+  hipLaunchKernelGGL(kb_decide, dim3(pts_c), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(kb_sign, dim3(B->n), dim3(SIGN_T), 0, s, a);
+  hipLaunchKernelGGL(kb_band_list, dim3(B->n), dim3(SCAN_T), 0, s, a);
+  hipLaunchKernelGGL(kb_plan, dim3(1), dim3(64), 0, s, a);
+  HIP_OK(hipGetLastError());
+  // the band bricks' nodes
+  const unsigned gl = unsigned(std::min<size_t>((B->total_cap * 729 / 8 + 1024 + 255) / 256, 2048));
+  if (mixed) {
+    hipLaunchKernelGGL((kb_search<SRC_NODES, float>), dim3(B->gn), dim3(Shape<SRC_NODES>::NT), 0, s, a);
+    hipLaunchKernelGGL(kb_leaves<float>, dim3(gl), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((kb_refine<SRC_NODES>), dim3(pts_n), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((kb_redo<SRC_NODES>), dim3(B->gn), dim3(Shape<SRC_NODES>::NT), 0, s, a, 3u);
+  } else {
+    hipLaunchKernelGGL((kb_search<SRC_NODES, double>), dim3(B->gn), dim3(Shape<SRC_NODES>::NT), 0, s, a);
+    hipLaunchKernelGGL(kb_leaves<double>, dim3(gl), dim3(256), 0, s, a);
+  }
+  hipLaunchKernelGGL((kb_mark<SRC_NODES>), dim3(pts_n), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((kb_redo<SRC_NODES>), dim3(B->gn), dim3(Shape<SRC_NODES>::NT), 0, s, a, 4u);
+  // the centres of all-far band bricks with no sign: settled
+  hipLaunchKernelGGL(kb_need_centre, dim3(B->gf), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((kb_redo<SRC_CENTRES>), dim3(B->gc), dim3(Shape<SRC_CENTRES>::NT), 0, s, a, 4u);
+  hipLaunchKernelGGL(kb_centre_sign, dim3(unsigned(std::min<size_t>((B->total_cap + 255) / 256, 4096))), dim3(256), 0, s, a);
+  HIP_OK(hipGetLastError());
+  // fill, quantize, pack, number and store
+  hipLaunchKernelGGL(kb_finish, dim3(B->gf), dim3(WG), 0, s, a);
+  hipLaunchKernelGGL(kb_slots, dim3(B->n), dim3(SCAN_T), 0, s, a);
+  hipLaunchKernelGGL(kb_store, dim3(pts_c), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(kb_store_band, dim3(B->gf), dim3(128), 0, s, a);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int batch_time(icam_batch* B, double& ms) {
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  HIP_OK(hipEventRecord(e0, B->stream));
+  if (batch_enqueue(B, B->stream)) return 1;
+  HIP_OK(hipEventRecord(e1, B->stream));
+  HIP_OK(hipEventSynchronize(e1));
+  float m = 0;
+  HIP_OK(hipEventElapsedTime(&m, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  ms = m;
+  return 0;
+}
+
+// The autotuner: the whole batch timed with each candidate grid of one persistent
+// stage at a time (the others held), the fastest kept, stage after stage.
+int batch_tune(icam_batch* B, int cus) {
+  double ms;
+  if (batch_time(B, ms)) return 1;  // warm-up
+  char line[160];
+  std::snprintf(line, sizeof line, "start gc %d gn %d gf %d: %.3f ms;", B->gc, B->gn, B->gf, ms);
+  B->tuning = line;
+  int* knobs[3] = {&B->gc, &B->gn, &B->gf};
+  const char* names[3] = {"gc", "gn", "gf"};
+  const int per_cu[3][4] = {{2, 4, 9, 18}, {1, 2, 4, 8}, {2, 4, 8, 16}};
+  for (int k = 0; k < 3; ++k) {
+    int best = *knobs[k];
+    double best_ms = 1e300;
+    for (int c = 0; c < 4; ++c) {
+      *knobs[k] = per_cu[k][c] * cus;
+      double t0, t1;
+      if (batch_time(B, t0) || batch_time(B, t1)) return 1;
+      const double t = std::min(t0, t1);
+      std::snprintf(line, sizeof line, " %s %d: %.3f ms;", names[k], *knobs[k], t);
+      B->tuning += line;
+      if (t < best_ms) best_ms = t, best = *knobs[k];
+    }
+    *knobs[k] = best;
+  }
+  std::snprintf(line, sizeof line, " chosen gc %d gn %d gf %d", B->gc, B->gn, B->gf);
+  B->tuning += line;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t n, int precision, int tune, icam_batch** out) {
+  g_error.clear();
+  if (!out) return fail("bad arguments");
+  *out = nullptr;
+  if (!parts || !levels || n == 0) return fail("an empty batch");
+  if (n > 255) return fail("a batch holds at most 255 parts");
+  if (precision != ICAM_FP64 && precision != ICAM_MIXED) return fail("unknown precision");
+  std::unique_ptr<icam_batch> B(new icam_batch());
+  B->precision = precision;
+  B->n = int(n);
+  int dev = parts[0] ? parts[0]->device : 0;
+  // levels and room, then the order: heaviest first (band room times the part's
+  // primitives, a proxy for its bricks' work)
+  std::vector<Lv> lvs(n);
+  std::vector<uint32_t> caps(n);
+  std::vector<double> work(n);
+  for (size_t i = 0; i < n; ++i) {
+    if (!parts[i]) return fail("part " + std::to_string(i) + " is not open");
+    if (parts[i]->device != dev) return fail("the parts of a batch live on one device");
+    Lv& lv = lvs[i];
+    for (int k = 0; k < 3; ++k) lv.lo[k] = levels[i].lo[k], lv.nb[k] = levels[i].nb[k];
+    lv.h = levels[i].h;
+    if (!(lv.h > 0) || lv.nb[0] <= 0 || lv.nb[1] <= 0 || lv.nb[2] <= 0) return fail("bad level " + std::to_string(i));
+    if (double(lv.nb[0]) * lv.nb[1] * lv.nb[2] >= double(1 << 24)) return fail("level " + std::to_string(i) + " has 2^24 bricks or more");
+    if (sign_pitch(lv.nb[0]) * lv.nb[1] * lv.nb[2] > SIGN_LDS) return fail("level " + std::to_string(i) + " is too large for the batched sign pass");
+    caps[i] = band_room(parts[i], lv);
+    work[i] = double(caps[i]) * (1.0 + double(parts[i]->pk.L.n_prims));
+  }
+  B->caller.resize(n);
+  for (size_t i = 0; i < n; ++i) B->caller[i] = int(i);
+  std::stable_sort(B->caller.begin(), B->caller.end(), [&](int x, int y) { return work[x] > work[y]; });
+  B->pos.resize(n);
+  for (size_t k = 0; k < n; ++k) B->pos[B->caller[k]] = int(k);
+  // offsets
+  B->brick_off.assign(n + 1, 0), B->tile_off.assign(n + 1, 0), B->cap_off.assign(n + 1, 0);
+  for (size_t k = 0; k < n; ++k) {
+    const int i = B->caller[k];
+    B->parts.push_back(parts[i]);
+    const Lv& lv = lvs[i];
+    B->brick_off[k + 1] = B->brick_off[k] + long(lv.nb[0]) * lv.nb[1] * lv.nb[2];
+    B->tile_off[k + 1] = B->tile_off[k] + tiles_of(lv.nb);
+    B->cap_off[k + 1] = B->cap_off[k] + caps[i];
+  }
+  B->total_bricks = B->brick_off[n], B->total_tiles = B->tile_off[n], B->total_cap = B->cap_off[n];
+  const size_t nbk = size_t(std::max<long>(B->total_bricks, 1)), ntl = size_t(std::max<long>(B->total_tiles, 1));
+  const size_t ncap = std::max<size_t>(B->total_cap, 1), nnode = ncap * 729;
+  const bool mixed = precision == ICAM_MIXED;
+  const size_t icap = nnode / 8 + 1024;
+  HIP_OK(hipSetDevice(dev));
+  HIP_OK(hipStreamCreate(&B->stream));
+  // device memory
+  for (auto [buf, bytes] : std::initializer_list<std::pair<Buf*, size_t>>{
+           {&B->bparts, n * sizeof(BPart)}, {&B->brick_off_d, (n + 1) * 8}, {&B->tile_off_d, (n + 1) * 8}, {&B->band_off, (n + 1) * 4},
+           {&B->btotal, n * 4}, {&B->stotal, n * 4}, {&B->ctl, BC_WORDS * 4}, {&B->counters, n * 64},
+           {&B->c_val, nbk * 8}, {&B->c_flag, nbk * 4}, {&B->c_qmark, ntl * 4}, {&B->c_smark, ntl * 4}, {&B->c_queue, ntl * 4},
+           {&B->c_squeue, ntl * 4}, {&B->n_val, nnode * 8}, {&B->n_flag, nnode * 4}, {&B->n_qmark, ncap * 4}, {&B->n_smark, ncap * 4},
+           {&B->n_queue, ncap * 4}, {&B->n_squeue, ncap * 4}, {&B->items, icap * Item<double>::W * 4}, {&B->tile_cand, ntl * TCAP * 4},
+           {&B->tile_count, ntl * 4}, {&B->dc, nbk * 8}, {&B->band, nbk}, {&B->sgn, nbk}, {&B->list, nbk * 4}, {&B->staging, ncap * ICAM_WORDS * 4},
+           {&B->cls, ncap * 4}, {&B->slot, ncap * 4}, {&B->map, nbk * 4}, {&B->atlas, ncap * ICAM_WORDS * 4}})
+    HIP_OK(buf->need(std::max<size_t>(bytes, 16)));
+  if (mixed)
+    for (auto [buf, bytes] : std::initializer_list<std::pair<Buf*, size_t>>{{&B->c_code, nbk * 4}, {&B->c_pa, nbk * 4}, {&B->c_pb, nbk * 4},
+                                                                             {&B->c_d32, nbk * 4}, {&B->c_info, nbk * 4}, {&B->n_code, nnode * 4},
+                                                                             {&B->n_pa, nnode * 4}, {&B->n_pb, nnode * 4}, {&B->n_d32, nnode * 4},
+                                                                             {&B->n_info, nnode * 4}})
+      HIP_OK(buf->need(std::max<size_t>(bytes, 16)));
+  // the parts' records
+  uint32_t* ctl = B->ctl.as<uint32_t>();
+  for (size_t k = 0; k < n; ++k) {
+    const int i = B->caller[k];
+    icam_hip* h = parts[i];
+    BPart P{};
+    const long bo = B->brick_off[k], to = B->tile_off[k];
+    const size_t co = B->cap_off[k];
+    P.dv = h->dv, P.lv = lvs[i], P.nb = B->brick_off[k + 1] - bo, P.cap = caps[i];
+    P.dc = B->dc.as<double>() + bo, P.band = B->band.as<uint8_t>() + bo, P.sgn = B->sgn.as<int8_t>() + bo, P.list = B->list.as<uint32_t>() + bo;
+    P.staging = B->staging.as<uint32_t>() + co * ICAM_WORDS, P.cls = B->cls.as<uint32_t>() + co, P.slot = B->slot.as<uint32_t>() + co;
+    P.map = B->map.as<uint32_t>() + bo, P.atlas = B->atlas.as<uint32_t>() + co * ICAM_WORDS;
+    P.counters = B->counters.as<unsigned long long>() + 8 * k;
+    P.jc.n = P.nb, P.jc.lv = P.lv, P.jc.tile_cand = B->tile_cand.as<uint32_t>() + size_t(to) * TCAP, P.jc.tile_count = B->tile_count.as<uint32_t>() + to;
+    P.jn = P.jc;
+    P.jn.n = long(P.cap) * 729, P.jn.list = P.list;
+    Res& rc = P.rc;
+    if (mixed)
+      rc.code = B->c_code.as<uint32_t>() + bo, rc.pa = B->c_pa.as<float>() + bo, rc.pb = B->c_pb.as<float>() + bo, rc.d32 = B->c_d32.as<float>() + bo,
+      rc.info = B->c_info.as<uint32_t>() + bo;
+    rc.val = B->c_val.as<double>() + bo, rc.flag = B->c_flag.as<uint32_t>() + bo;
+    rc.queue = B->c_queue.as<uint32_t>(), rc.qcount = ctl + BC_QC, rc.qmark = B->c_qmark.as<uint32_t>() + to;
+    rc.squeue = B->c_squeue.as<uint32_t>(), rc.scount = ctl + BC_SC, rc.smark = B->c_smark.as<uint32_t>() + to;
+    rc.counters = P.counters, rc.part = uint32_t(k);
+    Res& rn = P.rn;
+    const size_t no = co * 729;
+    if (mixed)
+      rn.code = B->n_code.as<uint32_t>() + no, rn.pa = B->n_pa.as<float>() + no, rn.pb = B->n_pb.as<float>() + no, rn.d32 = B->n_d32.as<float>() + no,
+      rn.info = B->n_info.as<uint32_t>() + no;
+    rn.val = B->n_val.as<double>() + no, rn.flag = B->n_flag.as<uint32_t>() + no;
+    rn.queue = B->n_queue.as<uint32_t>(), rn.qcount = ctl + BC_QN, rn.qmark = B->n_qmark.as<uint32_t>() + co;
+    rn.squeue = B->n_squeue.as<uint32_t>(), rn.scount = ctl + BC_SN, rn.smark = B->n_smark.as<uint32_t>() + co;
+    rn.items = B->items.as<uint32_t>(), rn.icount = ctl + BC_ITEMS, rn.icap = uint32_t(icap);
+    rn.counters = P.counters, rn.part = uint32_t(k);
+    B->host.push_back(P);
+  }
+  HIP_OK(hipMemcpy(B->bparts.p, B->host.data(), n * sizeof(BPart), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(B->brick_off_d.p, B->brick_off.data(), (n + 1) * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(B->tile_off_d.p, B->tile_off.data(), (n + 1) * 8, hipMemcpyHostToDevice));
+  BatchArgs& a = B->args;
+  a.parts = B->bparts.as<BPart>(), a.n = int(n), a.brick_off = B->brick_off_d.as<long>(), a.tile_off = B->tile_off_d.as<long>();
+  a.band_off = B->band_off.as<uint32_t>(), a.btotal = B->btotal.as<uint32_t>(), a.stotal = B->stotal.as<uint32_t>(), a.ctl = ctl;
+  // launch shapes: defaults from the device's compute units, then the autotuner
+  hipDeviceProp_t prop;
+  HIP_OK(hipGetDeviceProperties(&prop, dev));
+  const int cus = std::max(prop.multiProcessorCount, 1);
+  B->gc = 9 * cus, B->gn = 4 * cus, B->gf = 8 * cus;
+  if (tune && batch_tune(B.get(), cus)) return 1;
+  // the captured graph
+  HIP_OK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeRelaxed));
+  if (batch_enqueue(B.get(), B->stream)) {
+    hipGraph_t g = nullptr;
+    hipStreamEndCapture(B->stream, &g);
+    if (g) hipGraphDestroy(g);
+    return 1;
+  }
+  HIP_OK(hipStreamEndCapture(B->stream, &B->graph));
+  HIP_OK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+  *out = B.release();
+  return 0;
+}
+
+int icam_hip_batch_run(icam_batch* b, void* stream) {
+  if (!b) return fail("no batch");
+  HIP_OK(hipGraphLaunch(b->exec, stream ? (hipStream_t)stream : b->stream));
+  return 0;
+}
+
+int icam_hip_batch_sync(icam_batch* b) {
+  if (!b) return fail("no batch");
+  HIP_OK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int icam_hip_batch_result(icam_batch* b, size_t i, const uint32_t** d_map, const uint32_t** d_atlas, icam_bake_stats* stats) {
+  if (!b || i >= size_t(b->n)) return fail("bad arguments");
+  const int k = b->pos[i];
+  const BPart& P = b->host[k];
+  if (d_map) *d_map = P.map;
+  if (d_atlas) *d_atlas = P.atlas;
+  if (stats) {
+    uint32_t over = 0, bt = 0, sl = 0;
+    unsigned long long cnt[8];
+    HIP_OK(hipMemcpy(&over, b->ctl.as<uint32_t>() + BC_OVER, 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(&bt, b->btotal.as<uint32_t>() + k, 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(&sl, b->stotal.as<uint32_t>() + k, 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(cnt, P.counters, 64, hipMemcpyDeviceToHost));
+    if (over) return fail("the band of the batch's part " + std::to_string(b->caller[over - 1]) + " outgrew its room");
+    if (cnt[2]) return fail(std::to_string(cnt[2]) + " bricks kept whole with no sign found (no surface near them and none outside the part's box)");
+    stats->bricks = P.nb, stats->band = bt, stats->slots = sl;
+    stats->undecided = int64_t(cnt[0]), stats->refined = int64_t(cnt[1]);
+    stats->unsure_trim = int64_t(cnt[3]), stats->unsure_tie = int64_t(cnt[4]), stats->settled = int64_t(cnt[5]);
+  }
+  return 0;
+}
+
+int icam_hip_batch_download(icam_batch* b, size_t i, uint32_t* map, uint32_t* atlas, size_t atlas_cap, icam_bake_stats* stats) {
+  if (!b || i >= size_t(b->n)) return fail("bad arguments");
+  HIP_OK(hipStreamSynchronize(b->stream));
+  icam_bake_stats st{};
+  const uint32_t *dm, *da;
+  if (icam_hip_batch_result(b, i, &dm, &da, &st)) return 1;
+  if (stats) *stats = st;
+  if (map) HIP_OK(hipMemcpy(map, dm, size_t(st.bricks) * 4, hipMemcpyDeviceToHost));
+  const size_t words = size_t(st.slots) * ICAM_WORDS;
+  if (words > atlas_cap) return fail("atlas needs " + std::to_string(words) + " words, the buffer holds " + std::to_string(atlas_cap));
+  if (atlas && words) HIP_OK(hipMemcpy(atlas, da, words * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int icam_hip_batch_tuning(icam_batch* b, char* buf, size_t cap) {
+  if (!b || !buf || !cap) return fail("bad arguments");
+  std::snprintf(buf, cap, "%s", b->tuning.c_str());
+  return 0;
+}
+
+void icam_hip_batch_close(icam_batch* b) {
+  if (!b) return;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  delete b;
 }
 
 }  // extern "C"

@@ -51,6 +51,15 @@ def lib():
         L.icam_hip_bake_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
                                            ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(BakeStats), ctypes.c_void_p]
         L.icam_hip_stage_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), _dp, ctypes.c_int]
+        L.icam_hip_batch_open.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_void_p)]
+        L.icam_hip_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.icam_hip_batch_sync.argtypes = [ctypes.c_void_p]
+        L.icam_hip_batch_result.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+                                            ctypes.POINTER(BakeStats)]
+        L.icam_hip_batch_download.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u32p, _u32p, ctypes.c_size_t, ctypes.POINTER(BakeStats)]
+        L.icam_hip_batch_tuning.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+        L.icam_hip_batch_close.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -152,6 +161,61 @@ class PartGpu:
         if getattr(self, "h", None):
             lib().icam_hip_close(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many parts' brick levels baked by one launch of a captured hipGraph (the
+    additive batch entry points of include/icam_hip.h): `parts` are PartGpu handles,
+    `levels` their icam_level records. `tune`: run the autotuner over the persistent
+    grid sizes of the queue stages first."""
+
+    def __init__(self, parts, levels, precision=MIXED, tune=True):
+        n = len(parts)
+        self.parts = list(parts)
+        self.levels = (_native.Level * n)(*levels)
+        hs = (ctypes.c_void_p * n)(*[p.h.value for p in parts])
+        b = ctypes.c_void_p()
+        _check(lib().icam_hip_batch_open(hs, ctypes.cast(self.levels, ctypes.c_void_p), n, precision, 1 if tune else 0, ctypes.byref(b)))
+        self.b = b
+        self.n = n
+
+    def run(self, stream=None):
+        _check(lib().icam_hip_batch_run(self.b, stream))
+
+    def sync(self):
+        _check(lib().icam_hip_batch_sync(self.b))
+
+    def stats(self, i):
+        st = BakeStats()
+        _check(lib().icam_hip_batch_result(self.b, i, None, None, ctypes.byref(st)))
+        return st
+
+    def download(self, i):
+        """Part i's map, atlas and statistics (host arrays)."""
+        lv = self.levels[i]
+        nb = lv.nb[0] * lv.nb[1] * lv.nb[2]
+        st = self.stats(i)
+        m = np.empty(nb, dtype=np.uint32)
+        atlas = np.empty(max(st.slots * 365, 1), dtype=np.uint32)
+        _check(lib().icam_hip_batch_download(self.b, i, m.ctypes.data_as(_u32p), atlas.ctypes.data_as(_u32p), len(atlas), ctypes.byref(st)))
+        return m, atlas[: st.slots * 365].copy(), st
+
+    @property
+    def tuning(self):
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib().icam_hip_batch_tuning(self.b, buf, 4096))
+        return buf.value.decode()
+
+    def close(self):
+        if getattr(self, "b", None):
+            lib().icam_hip_batch_close(self.b)
+            self.b = None
 
     def __del__(self):
         try:

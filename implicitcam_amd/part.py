@@ -86,3 +86,22 @@ def level_for(part, h, max_bricks=None):
     lv.h = h
     lv.nb[:] = nb
     return lv
+
+
+def c4_level(part, grid):
+    """The batched workload's level for a part: a cubic grid of `grid` nodes a side over
+    the part's largest extent and a cell of margin each way (part-scaled spacing),
+    grid / 8 bricks along the largest axis."""
+    h = float(np.max(part.hi - part.lo) + 2.0) / (grid - 2)
+    return level_for(part, h, grid // 8)
+
+
+def c4_grid(index, seed=0x2545F4914F6CDD1D):
+    """Part `index`'s grid in the batched workload: 32, 64, 128 or 256 nodes a side,
+    log-uniform from the reference checks' xorshift64."""
+    x = (seed ^ (0xD1B54A32D192ED03 * (index + 1))) & 0xFFFFFFFFFFFFFFFF or 1
+    for _ in range(3):
+        x ^= (x << 13) & 0xFFFFFFFFFFFFFFFF
+        x ^= x >> 7
+        x ^= (x << 17) & 0xFFFFFFFFFFFFFFFF
+    return 32 << ((x >> 11) % 4)

@@ -125,6 +125,41 @@ int icam_hip_bake(icam_hip* h, int precision, const icam_level* lv, uint32_t* ma
  * they are written when stats is not NULL (which waits). */
 int icam_hip_bake_device(icam_hip* h, int precision, const icam_level* lv, uint32_t** d_map, uint32_t** d_atlas, icam_bake_stats* stats, void* stream);
 
+/* Batches: many parts' levels baked by one launch of a captured hipGraph. The
+ * reference bakes one level a call (PartCache::build, partcache.rs:458-486; the
+ * viewer runs two at once, play.rs:979-997); these entry points are additive. Each
+ * stage of the bake is one launch over every part's work, the block stages being
+ * persistent work queues that serve the parts heaviest first.
+ *
+ * icam_hip_batch_open: part i (an open handle, kept open while the batch lives) at
+ * level i, n <= 255, on one device; levels of at most 32 x 32 x 32 bricks (more
+ * along one axis if fewer along the others: the sign pass keeps a level in LDS).
+ * With `tune` nonzero the autotuner times the batch with candidate persistent grid
+ * sizes of its queue stages and keeps the fastest (icam_hip_batch_tuning reports
+ * what it tried). Device memory for every part's band is sized from its primitives'
+ * boxes, once. */
+typedef struct icam_batch icam_batch;
+int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t n, int precision, int tune, icam_batch** out);
+
+/* One run: the graph launched on `stream` (NULL: the batch's own); no waiting. */
+int icam_hip_batch_run(icam_batch* b, void* stream);
+
+/* Waits for the batch's own stream. */
+int icam_hip_batch_sync(icam_batch* b);
+
+/* After the run has finished: part i's map and atlas in device memory (valid until
+ * the next run or close) and its statistics (stats may be NULL). Fails when a
+ * part's band outgrew its room or a brick kept whole found no sign. */
+int icam_hip_batch_result(icam_batch* b, size_t i, const uint32_t** d_map, const uint32_t** d_atlas, icam_bake_stats* stats);
+
+/* The same copied to host arrays (waits for the batch's own stream first). */
+int icam_hip_batch_download(icam_batch* b, size_t i, uint32_t* map, uint32_t* atlas, size_t atlas_cap, icam_bake_stats* stats);
+
+/* The autotuner's report: the candidates it timed and its choice. */
+int icam_hip_batch_tuning(icam_batch* b, char* buf, size_t cap);
+
+void icam_hip_batch_close(icam_batch* b);
+
 /* Average device time (ms) of each kernel stage of the last bake or query on
  * this handle, measured with HIP events on the stream the kernels ran on:
  * names[i] (static strings) and ms[i] for i < returned count (at most cap). */

@@ -277,3 +277,46 @@ def test_settle_past_certificates(name, prec):
     assert (flags == 0).all()
     assert np.abs(d - dref).max() <= tol(prec, part)
     assert ((d < 0) == (dref < 0)).all()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_batch_matches_single_bakes(prec):
+    """The 64 parts of the batched workload at 64^3 nodes in one batch (one graph
+    launch): every part's map and atlas identical to its own bake (which matches the
+    oracle, test_c4_parts_bake_like_oracle); a second replay of the graph agrees."""
+    from implicitcam_amd import Batch, c4_level
+
+    parts = [Part(f"c4:{i}") for i in range(64)]
+    levels = [c4_level(p, 64) for p in parts]
+    gpus = [PartGpu(p, 12 * lv.h) for p, lv in zip(parts, levels)]
+    b = Batch(gpus, levels, prec, tune=False)
+    singles = [g.bake(lv, prec) for g, lv in zip(gpus, levels)]
+    for rep in range(2):
+        b.run()
+        b.sync()
+        for i, (m1, a1, st1) in enumerate(singles):
+            m, a, st = b.download(i)
+            assert (m == m1).all(), (rep, i)
+            assert len(a) == len(a1) and (a == a1).all(), (rep, i)
+            assert st.band == st1.band and st.slots == st1.slots, (rep, i)
+    b.close()
+
+
+def test_batch_tuned_uneven_grids():
+    """Sixteen parts of the workload at their own grids (32^3 to 256^3 nodes), batched
+    with the autotuner on: the same maps and atlases as their single bakes."""
+    from implicitcam_amd import Batch, c4_grid, c4_level
+
+    idx = list(range(0, 64, 4))
+    parts = [Part(f"c4:{i}") for i in idx]
+    levels = [c4_level(p, c4_grid(i)) for p, i in zip(parts, idx)]
+    gpus = [PartGpu(p, 12 * lv.h) for p, lv in zip(parts, levels)]
+    b = Batch(gpus, levels, MIXED, tune=True)
+    assert "chosen" in b.tuning
+    b.run()
+    b.sync()
+    for i, (g, lv) in enumerate(zip(gpus, levels)):
+        m1, a1, st1 = g.bake(lv, MIXED)
+        m, a, st = b.download(i)
+        assert (m == m1).all() and len(a) == len(a1) and (a == a1).all(), i
+    b.close()
