@@ -32,6 +32,58 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def batched_workload(args, log):
+    """64 parts (face counts log-uniform 6..400; turned parts, NURBS blocks with 4 x 4 to
+    32 x 32 nets, plates with holes) at grids of 32^3 to 256^3 nodes, baked as one batch
+    (one graph launch, autotuned queue grids) in both precisions, against the same 64
+    levels baked one after another through the single-level entry point."""
+    import torch
+
+    from implicitcam_amd import FP64, MIXED, Batch, Part, PartGpu, c4_grid, c4_level
+
+    t0 = time.time()
+    parts = [Part(f"c4:{i}") for i in range(64)]
+    levels = [c4_level(p, c4_grid(i)) for i, p in enumerate(parts)]
+    gpus = [PartGpu(p, 12 * lv.h) for p, lv in zip(parts, levels)]
+    log(f"c4: 64 parts packed in {time.time() - t0:.1f}s, faces {sum(p.n_faces for p in parts)}, "
+        f"bricks {sum(lv.nb[0] * lv.nb[1] * lv.nb[2] for lv in levels)}")
+    out = {"parts": 64, "faces": [p.n_faces for p in parts], "grids": [c4_grid(i) for i in range(64)]}
+    for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+        t0 = time.time()
+        b = Batch(gpus, levels, prec, tune=True)
+        tune_s = time.time() - t0
+        for _ in range(max(args.warmup, 1)):
+            b.run()
+        b.sync()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            b.run()
+        b.sync()
+        ms = (time.perf_counter() - t) / args.steps * 1e3
+        band = sum(b.stats(i).band for i in range(64))
+        out[name] = {"ms": ms, "band_bricks": band, "bricks_per_s": band / (ms / 1e3), "tuning": b.tuning, "open_and_tune_s": tune_s}
+        log(f"c4 batch {name}: {ms:.3f} ms, {band} band bricks, {band / (ms / 1e3) / 1e6:.2f} M bricks/s; {b.tuning}")
+        b.close()
+        # the same levels one after another (the single-level entry point: one bake each)
+        for g, lv in zip(gpus, levels):
+            g.bake_device(lv, prec)
+        torch.cuda.synchronize()
+        per = []
+        t = time.perf_counter()
+        for g, lv in zip(gpus, levels):
+            t1 = time.perf_counter()
+            g.bake_device(lv, prec)
+            torch.cuda.synchronize()
+            per.append((time.perf_counter() - t1) * 1e3)
+        seq = (time.perf_counter() - t) * 1e3
+        out[name].update({"sequential_ms": seq, "speedup_over_sequential": seq / ms, "per_part_ms_p50": float(np.percentile(per, 50)),
+                          "per_part_ms_p99": float(np.percentile(per, 99))})
+        log(f"c4 sequential {name}: {seq:.3f} ms (per part p50 {np.percentile(per, 50):.3f} ms, p99 {np.percentile(per, 99):.3f} ms): "
+            f"batch {seq / ms:.2f}x faster")
+    out["mixed_over_fp64"] = out["fp64"]["ms"] / out["mixed"]["ms"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,6 +92,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the parity check against the oracle")
     ap.add_argument("--profile", action="store_true", help="a short run for a profiler: bake steps only")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-part batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the path shards by brick ranges across GPUs (see the design notes); "
@@ -118,6 +171,9 @@ def main():
         q[name] = n / dt
         log(f"c1 queries {name}: {n / dt / 1e6:.1f} M/s")
 
+    # the batched workload: 64 parts of uneven face counts and grids in one graph
+    c4 = None if args.no_batch else batched_workload(args, log)
+
     check = {}
     cpu = None
     if not args.no_check or not args.no_cpu:
@@ -181,6 +237,7 @@ def main():
                      "frac": algo_bytes / (bricks_ms / 1e3) / PEAK_HBM, "traffic": None},
         "cpu_baseline": cpu,
         "check": check,
+        "c4_batch": c4,
     }
     print(json.dumps(out))
 

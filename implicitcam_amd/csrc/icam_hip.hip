@@ -634,7 +634,7 @@ template <int S, class T> struct SearchLds {
 // precision (T = double: final answers). A brick's nodes leave their parts' rounds to
 // the dense pass.
 template <int S, class T>
-__device__ inline void search_block(const Dev& dv, const Job& j, const Res& res, long blk, SearchLds<S, T>& sm) {
+__device__ __forceinline__ void search_block(const Dev& dv, const Job& j, const Res& res, long blk, SearchLds<S, T>& sm) {
   constexpr int NT = Shape<S>::NT;
   constexpr int Q = Shape<S>::Q;
   const int nc = block_candidates<S>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot);
@@ -671,7 +671,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res)
 // One deferred point of the dense pass: its parts in order of their lower bounds
 // while one can beat its answer.
 template <int S, class T>
-__device__ inline void leaves_item(const Dev& dv, const Job& j, const Res& res, const uint32_t* w) {
+__device__ __forceinline__ void leaves_item(const Dev& dv, const Job& j, const Res& res, const uint32_t* w) {
   const Tab<T> tb = tab_of<T>(dv);
   const T eps = sizeof(T) == 4 ? T(dv.L.eps32) : T(dv.L.eps64);
   const T tm = sizeof(T) == 4 ? T(16) * eps : T(0);
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(256) void k_leaves(Dev dv, Job j, Res res) {
 // point the search was unsure of is marked (flag 3) and its block queued (entries are
 // part << 24 | block) for the fallback.
 template <int S>
-__device__ inline void refine_point(const Dev& dv, const Job& j, const Res& res, long i) {
+__device__ __forceinline__ void refine_point(const Dev& dv, const Job& j, const Res& res, long i) {
   uint32_t info = res.info[i];
   double bound;
   v3<double> p = job_point<S>(j, dv, i, bound);
@@ -746,7 +746,7 @@ __global__ __launch_bounds__(256) void k_refine(Dev dv, Job j, Res res) {
 // convexity certificate settled by the certified search, as the reference settles
 // them on the CPU, crates/icam-sim/src/partcache.rs:563-751).
 template <int S>
-__device__ inline void redo_block(const Dev& dv, const Job& j, const Res& res, long blk, uint32_t want, SearchLds<S, double>& sm) {
+__device__ __forceinline__ void redo_block(const Dev& dv, const Job& j, const Res& res, long blk, uint32_t want, SearchLds<S, double>& sm) {
   constexpr int NT = Shape<S>::NT;
   const int nc = block_candidates<S>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot);
   const Tab<double> tb = tab_of<double>(dv);
@@ -783,7 +783,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_fallback(Dev dv, Job j, Res re
 // Point i left undecided (flag 2) is marked (flag 4) and its block queued for the
 // settle.
 template <int S>
-__device__ inline void mark_point(const Job& j, const Res& res, long i) {
+__device__ __forceinline__ void mark_point(const Job& j, const Res& res, long i) {
   if (res.flag[i] != 2) return;
   res.flag[i] = 4;
   long blk = block_of<S>(j, i);
@@ -810,7 +810,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_settle(Dev dv, Job j, Res res)
 // A band brick whose nodes all lie past the clamp takes its centre's sign. When the
 // decision left that centre undecided and no sign came to it along the grid, the
 // centre is marked (flag 4) for the settle; centre_sign then takes its settled sign.
-__device__ inline void need_centre(const Job& jc, const Res& rc, const uint32_t* list, long bi, const uint32_t* nflag, const double* dc,
+__device__ __forceinline__ void need_centre(const Job& jc, const Res& rc, const uint32_t* list, long bi, const uint32_t* nflag, const double* dc,
                                    const int8_t* sgn) {
   int far = 1;
   for (int n = threadIdx.x; n < 729; n += blockDim.x) far &= nflag[729 * bi + n] == 1u;
@@ -820,7 +820,7 @@ __device__ inline void need_centre(const Job& jc, const Res& rc, const uint32_t*
     if (dc[b] != dc[b] && sgn[b] == 0) mark_point<SRC_CENTRES>(jc, rc, b);
   }
 }
-__device__ inline void centre_sign(const Res& rc, const uint32_t* list, long bi, double* dc, int8_t* sgn) {
+__device__ __forceinline__ void centre_sign(const Res& rc, const uint32_t* list, long bi, double* dc, int8_t* sgn) {
   const long b = long(list[bi]);
   if (dc[b] != dc[b] && sgn[b] == 0 && rc.flag[b] == 0) {
     dc[b] = rc.val[b];
@@ -843,7 +843,7 @@ __global__ __launch_bounds__(256) void k_centre_sign(Res rc, const uint32_t* lis
 // The decision from the centres' answers: dc the signed distance or NaN; band 1 when
 // the brick is kept (|dc| <= reach, or undecided under the reach); sign +-1 when
 // known, else 0.
-__device__ inline void decide_brick(const Dev& dv, const Lv& lv, long b, const double* val, const uint32_t* flag, double* dc, uint8_t* band, int8_t* sgn) {
+__device__ __forceinline__ void decide_brick(const Dev& dv, const Lv& lv, long b, const double* val, const uint32_t* flag, double* dc, uint8_t* band, int8_t* sgn) {
   double reach = brick_reach(lv.h);
   double v = val[b];
   uint32_t f = flag[b];
@@ -906,7 +906,7 @@ __global__ void k_sign(Lv lv, int ax, const uint8_t* band, int8_t* sgn, int* cha
 // x fastest (or y, for the x sweep), so a wave's lanes read distinct LDS banks.
 constexpr int SIGN_T = 1024, SIGN_LDS = 96 * 1024, SIGN_ROUNDS = 8;
 __host__ __device__ inline long sign_pitch(int nb0) { return 4L * (2L * ((nb0 + 7) / 8) + 1); }
-__device__ inline void sign_block(const Lv& lv, const uint8_t* band, int8_t* sgn, uint8_t* g, int& changed) {
+__device__ __forceinline__ void sign_block(const Lv& lv, const uint8_t* band, int8_t* sgn, uint8_t* g, int& changed) {
   const int nb[3] = {lv.nb[0], lv.nb[1], lv.nb[2]};
   const int P = int(sign_pitch(lv.nb[0]));
   const int rows = nb[1] * nb[2];  // row r = y + nb[1] z: nb[0] bricks along x
@@ -968,7 +968,7 @@ __global__ __launch_bounds__(SIGN_T) void k_sign_block(Lv lv, const uint8_t* ban
 // count. The predicate: band[i] != 0 (band), or cls[i] == 0 (cls: a band brick stored
 // with words).
 constexpr int SCAN_T = 1024;
-__device__ inline void scan_block(const uint8_t* band, const uint32_t* cls, long n, uint32_t* out, uint32_t* list, uint32_t* total, uint32_t* s) {
+__device__ __forceinline__ void scan_block(const uint8_t* band, const uint32_t* cls, long n, uint32_t* out, uint32_t* list, uint32_t* total, uint32_t* s) {
   const int tid = threadIdx.x;
   const long run = ((n + SCAN_T - 1) / SCAN_T + 15) / 16 * 16, i0 = tid * run, i1 = i0 + run < n ? i0 + run : n;
   // the run's keep bits, 16 entries a word
@@ -1034,7 +1034,7 @@ struct FinishLds {
   int changed;
   int16_t q[730];
 };
-__device__ inline void finish_brick(const Lv& lv, const uint32_t* list, long bi, const double* dcs, const int8_t* sgns, const double* val_in,
+__device__ __forceinline__ void finish_brick(const Lv& lv, const uint32_t* list, long bi, const double* dcs, const int8_t* sgns, const double* val_in,
                                     const uint32_t* flag_in, uint32_t* staging, uint32_t* cls, unsigned long long* counters, FinishLds& sm) {
   double* val = sm.val;
   uint8_t* st = sm.st;
@@ -1163,7 +1163,7 @@ __global__ __launch_bounds__(WG) void k_bricks_finish(Lv lv, const uint32_t* lis
 // partcache.rs:128-146 assemble: the map (AIR, SOLID, or the slot) and the atlas in
 // map order (band bricks are listed in map order, so their slots are too). A brick
 // kept whole whose sign was never found is counted (counters[2]).
-__device__ inline void store_brick(long b, const uint8_t* band, const int8_t* sgn, uint32_t* map, unsigned long long* counters) {
+__device__ __forceinline__ void store_brick(long b, const uint8_t* band, const int8_t* sgn, uint32_t* map, unsigned long long* counters) {
   if (!band[b]) {
     map[b] = sgn[b] < 0 ? ICAM_SOLID : ICAM_AIR;
     if (sgn[b] == 0) atomicAdd(&counters[2], 1ull);
@@ -1174,7 +1174,7 @@ __global__ void k_store(long nbricks, const uint8_t* band, const int8_t* sgn, ui
   if (b < nbricks) store_brick(b, band, sgn, map, counters);
 }
 // Band brick bi's map entry and atlas words (its slot's), by a block of any size.
-__device__ inline void store_band_brick(long bi, const uint32_t* list, const uint32_t* cls, const uint32_t* slot, const uint32_t* staging, uint32_t* map,
+__device__ __forceinline__ void store_band_brick(long bi, const uint32_t* list, const uint32_t* cls, const uint32_t* slot, const uint32_t* staging, uint32_t* map,
                                         uint32_t* atlas) {
   uint32_t c = cls[bi];
   if (threadIdx.x == 0) map[list[bi]] = c == 1 ? ICAM_AIR : c == 2 ? ICAM_SOLID : slot[bi];
