@@ -1012,6 +1012,222 @@ std::vector<Hole> plate_holes(double W, double D, double T, int k, Rng& rng) {
   return out;
 }
 
+// ------------------------------------------------------------- C3: the trimmed plate
+
+// A clamped cubic B-spline of 8 control points (5 spans, knots 0 0 0 0 .2 .4 .6 .8 1 1
+// 1 1): its 5 Bezier pieces (points of dimension N, flat).
+std::vector<std::vector<double>> five_spans(const std::vector<double>& ctrl, int N) {
+  std::vector<double> kn = {0, 0, 0, 0, 0.2, 0.4, 0.6, 0.8, 1, 1, 1, 1};
+  return decompose(3, kn, ctrl, N);
+}
+
+// A plate 200 x 150 x 25 (centred) with 24 pockets and 12 bosses on a jittered 6 x 6
+// grid. Each pocket or boss has an outline: a closed curve of F clamped cubic B-spline
+// segments of 5 spans each (20 to 60 spans in all), 8 control points a segment from a
+// radially perturbed rounded rectangle, consecutive segments meeting with equal
+// tangents (the shared control point halfway between its neighbours). The outline is
+// a trim loop of the plate's top face and of the pocket's floor or the boss's top;
+// each segment is the bottom and top boundary of a wall face: a rational bicubic
+// B-spline with an 8 x 8 net, the outline's points along u, heights along v at the
+// Greville abscissae, weights varying along v only (its end rows weight 1), so the
+// wall is the vertical ruled surface over the outline and its boundary curves are
+// the outline's polynomial pieces.
+Owned* make_c3(uint64_t seed) {
+  Owned* o = new Owned();
+  Builder b{o};
+  Rng rng{seed ^ 0x3C3C3C3C3C3C3C3Cull};
+  for (int k = 0; k < 4; ++k) rng.next();
+  V X{1, 0, 0}, Y{0, 1, 0}, Z{0, 0, 1};
+  const double ax = 100, ay = 75, az = 12.5;
+  // slots: 24 pockets and 12 bosses in random order
+  std::vector<int> kind(36);
+  for (int i = 0; i < 36; ++i) kind[i] = i < 24 ? 0 : 1;
+  for (int i = 35; i > 0; --i) std::swap(kind[i], kind[int(rng.next() * (i + 1)) % (i + 1)]);
+  struct Loop {
+    int kind;            // 0 pocket, 1 boss
+    double z0, z1;       // the wall's height range
+    std::vector<std::vector<double>> seg;  // F segments: 8 control points (x, y)
+    std::vector<Piece> pieces;             // the loop as monotone trim pieces
+    double lo[2], hi[2];
+  };
+  std::vector<Loop> loops;
+  const double cw = 2 * ax / 6, cd = 2 * ay / 6;
+  for (int s = 0; s < 36; ++s) {
+    Loop L;
+    L.kind = kind[s];
+    const double cx = -ax + cw * (s % 6 + 0.5) + (rng.next() - 0.5) * 3.0, cy = -ay + cd * (s / 6 + 0.5) + (rng.next() - 0.5) * 2.0;
+    const double ha = 0.28 * cw + rng.next() * 0.06 * cw, hb = 0.26 * cd + rng.next() * 0.06 * cd, rc = 2.0 + 2.0 * rng.next();
+    if (L.kind == 0) {
+      L.z1 = az, L.z0 = az - (4.0 + 8.0 * rng.next());
+    } else {
+      L.z0 = az, L.z1 = az + (3.0 + 5.0 * rng.next());
+    }
+    const int F = 4 + int(rng.next() * 9.0) % 9;  // 4..12 segments: 20..60 spans
+    const int M = 7 * F;
+    // the rounded rectangle by arc length, counterclockwise from its right side's middle
+    const double straight_x = 2 * (ha - rc), straight_y = 2 * (hb - rc), quarter = 0.5 * M_PI * rc;
+    const double per = 2 * straight_x + 2 * straight_y + 4 * quarter;
+    auto rr = [&](double t) {
+      double l = t * per;
+      // pieces: half right side up, top-right corner, top (leftward), top-left corner,
+      // left (down), bottom-left corner, bottom (rightward), bottom-right corner, half
+      // right side up
+      const double segs[9] = {0.5 * straight_y, quarter, straight_x, quarter, straight_y, quarter, straight_x, quarter, 0.5 * straight_y};
+      int k = 0;
+      while (k < 8 && l > segs[k]) l -= segs[k], ++k;
+      double x = 0, y = 0, nx = 1, ny = 0;
+      const double ex = ha - rc, ey = hb - rc;
+      switch (k) {
+        case 0: x = ha, y = l, nx = 1, ny = 0; break;
+        case 1: { double a = l / rc; x = ex + rc * std::cos(a), y = ey + rc * std::sin(a), nx = std::cos(a), ny = std::sin(a); break; }
+        case 2: x = ex - l, y = hb, nx = 0, ny = 1; break;
+        case 3: { double a = 0.5 * M_PI + l / rc; x = -ex + rc * std::cos(a), y = ey + rc * std::sin(a), nx = std::cos(a), ny = std::sin(a); break; }
+        case 4: x = -ha, y = ey - l, nx = -1, ny = 0; break;
+        case 5: { double a = M_PI + l / rc; x = -ex + rc * std::cos(a), y = -ey + rc * std::sin(a), nx = std::cos(a), ny = std::sin(a); break; }
+        case 6: x = -ex + l, y = -hb, nx = 0, ny = -1; break;
+        case 7: { double a = 1.5 * M_PI + l / rc; x = ex + rc * std::cos(a), y = -ey + rc * std::sin(a), nx = std::cos(a), ny = std::sin(a); break; }
+        default: x = ha, y = -ey + l, nx = 1, ny = 0; break;
+      }
+      return std::array<double, 4>{x, y, nx, ny};
+    };
+    const double a1 = 0.5 + 0.4 * rng.next(), f1 = double(2 + int(rng.next() * 3.0)), p1 = 2 * M_PI * rng.next();
+    const double a2 = 0.3 * rng.next(), f2 = double(5 + int(rng.next() * 4.0)), p2 = 2 * M_PI * rng.next();
+    std::vector<std::array<double, 2>> pts(M);
+    for (int i = 0; i < M; ++i) {
+      const double t = double(i) / M;
+      auto q = rr(t);
+      const double d = a1 * std::sin(2 * M_PI * f1 * t + p1) + a2 * std::sin(2 * M_PI * f2 * t + p2);
+      pts[i] = {cx + q[0] + d * q[2], cy + q[1] + d * q[3]};
+    }
+    // segment k: the joint, 6 points, the next joint; a joint halfway between its
+    // neighbours
+    auto joint = [&](int k) {
+      const auto& a = pts[(7 * k + M - 1) % M];
+      const auto& c = pts[(7 * k + 1) % M];
+      return std::array<double, 2>{0.5 * (a[0] + c[0]), 0.5 * (a[1] + c[1])};
+    };
+    L.lo[0] = L.lo[1] = 1e300, L.hi[0] = L.hi[1] = -1e300;
+    for (int k = 0; k < F; ++k) {
+      std::vector<double> c;
+      auto j0 = joint(k), j1 = joint((k + 1) % F);
+      c.insert(c.end(), {j0[0], j0[1]});
+      for (int m = 1; m <= 6; ++m) c.insert(c.end(), {pts[(7 * k + m) % M][0], pts[(7 * k + m) % M][1]});
+      c.insert(c.end(), {j1[0], j1[1]});
+      for (int m = 0; m < 8; ++m)
+        for (int a = 0; a < 2; ++a) L.lo[a] = std::min(L.lo[a], c[2 * m + a]), L.hi[a] = std::max(L.hi[a], c[2 * m + a]);
+      for (auto& pc : five_spans(c, 2)) add_monotone(L.pieces, to_cubic2(pc, 3));
+      L.seg.push_back(c);
+    }
+    loops.push_back(L);
+  }
+  auto rect = [](double u0, double v0, double u1, double v1) {
+    std::vector<Piece> p;
+    add_polyline(p, {{u0, v0}, {u1, v0}, {u1, v1}, {u0, v1}}, true);
+    return p;
+  };
+  // the plate
+  std::vector<Piece> top = rect(-ax, -ay, ax, ay);
+  for (auto& L : loops) top.insert(top.end(), L.pieces.begin(), L.pieces.end());
+  uint32_t f_top = b.plane({0, 0, az}, Z, X, 1.0, -ax - 1, -ay - 1, ax + 1, ay + 1, top, -ax - 0.5, 0.0123, false);
+  uint32_t f_bot = b.plane({0, 0, -az}, Z * -1.0, X, 1.0, -ax - 1, -ay - 1, ax + 1, ay + 1, rect(-ax, -ay, ax, ay), -ax - 0.5, 0.0123, false);
+  uint32_t f_xp = b.plane({ax, 0, 0}, X, Y, 1.0, -ay - 1, -az - 1, ay + 1, az + 1, rect(-ay, -az, ay, az), -ay - 0.5, 0.0123, false);
+  uint32_t f_xm = b.plane({-ax, 0, 0}, X * -1.0, Y, 1.0, -ay - 1, -az - 1, ay + 1, az + 1, rect(-ay, -az, ay, az), -ay - 0.5, 0.0123, false);
+  uint32_t f_yp = b.plane({0, ay, 0}, Y, X, 1.0, -ax - 1, -az - 1, ax + 1, az + 1, rect(-ax, -az, ax, az), -ax - 0.5, 0.0123, false);
+  uint32_t f_ym = b.plane({0, -ay, 0}, Y * -1.0, X, 1.0, -ax - 1, -az - 1, ax + 1, az + 1, rect(-ax, -az, ax, az), -ax - 0.5, 0.0123, false);
+  V c[2][2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) c[i][j][k] = V{i ? ax : -ax, j ? ay : -ay, k ? az : -az};
+  for (int k = 0; k < 2; ++k) {
+    uint32_t f = k ? f_top : f_bot;
+    b.line(c[0][0][k], c[1][0][k], f, f_ym);
+    b.line(c[1][0][k], c[1][1][k], f, f_xp);
+    b.line(c[1][1][k], c[0][1][k], f, f_yp);
+    b.line(c[0][1][k], c[0][0][k], f, f_xm);
+  }
+  b.line(c[0][0][0], c[0][0][1], f_xm, f_ym);
+  b.line(c[1][0][0], c[1][0][1], f_xp, f_ym);
+  b.line(c[1][1][0], c[1][1][1], f_xp, f_yp);
+  b.line(c[0][1][0], c[0][1][1], f_xm, f_yp);
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) b.vert(c[i][j][k], V{i ? 1.0 : -1.0, j ? 1.0 : -1.0, k ? 1.0 : -1.0} * (M_PI / 2));
+  // the v knots of a wall and their Greville abscissae
+  const std::vector<double> kv = {0, 0, 0, 0, 0.2, 0.4, 0.6, 0.8, 1, 1, 1, 1};
+  double gv[8];
+  for (int j = 0; j < 8; ++j) gv[j] = (kv[j + 1] + kv[j + 2] + kv[j + 3]) / 3.0;
+  o->info = {3.0, 2 * ax, 2 * ay, 2 * az, double(loops.size())};
+  double zmax = az;
+  const V nanv{NaN, NaN, NaN};
+  (void)nanv;
+  for (auto& L : loops) {
+    const int F = int(L.seg.size());
+    zmax = std::max(zmax, L.z1);
+    o->info.insert(o->info.end(), {double(L.kind), L.z0, L.z1, double(F)});
+    for (auto& sgm : L.seg) o->info.insert(o->info.end(), sgm.begin(), sgm.end());
+    // the floor (pocket) or the top (boss): a plane facing up, trimmed by the outline
+    const double zc = L.kind == 0 ? L.z0 : L.z1;
+    uint32_t f_cap = b.plane({0, 0, zc}, Z, X, 1.0, L.lo[0] - 1, L.lo[1] - 1, L.hi[0] + 1, L.hi[1] + 1, L.pieces, L.lo[0] - 0.5, 0.5 * (L.lo[1] + L.hi[1]) + 0.0123,
+                             false);
+    // the rim on the plate's top face, and the far rim (on the cap)
+    const uint32_t f_near = f_top;
+    // walls: u along the segment, v up; the wall's normal Su x Sv points out of the
+    // outline (counterclockwise), so a pocket's wall faces in (sense -1)
+    std::vector<uint32_t> walls;
+    for (int k = 0; k < F; ++k) {
+      const auto& sgm = L.seg[k];
+      std::vector<double> w(8);
+      for (int j = 0; j < 8; ++j) w[j] = (j == 0 || j == 7) ? 1.0 : 0.8 + 0.4 * rng.next();
+      std::vector<double> pw(8 * 8 * 4);
+      for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) {
+          double* q = &pw[(i * 8 + j) * 4];
+          const double z = L.z0 + (L.z1 - L.z0) * gv[j];
+          q[0] = sgm[2 * i] * w[j], q[1] = sgm[2 * i + 1] * w[j], q[2] = z * w[j], q[3] = w[j];
+        }
+      icam_face_desc f;
+      std::memset(&f, 0, sizeof(f));
+      f.kind = ICAM_NURBS;
+      f.sense = L.kind == 0 ? -1.0 : 1.0;
+      f.du = f.dv = 3;
+      f.nu = f.nv = 8;
+      f.pw = o->keep(pw);
+      f.ku = o->keep(kv);
+      f.kv = o->keep(kv);
+      b.set_trim(f, 0, 0, 1, 1, 0, 0, {}, 0.5, 0.5, true);
+      o->faces.push_back(f);
+      walls.push_back(uint32_t(o->faces.size() - 1));
+    }
+    for (int k = 0; k < F; ++k) {
+      const auto& sgm = L.seg[k];
+      for (int end = 0; end < 2; ++end) {
+        // end 0: the rim at z0 (v = 0), end 1: at z1 (v = 1)
+        const double z = end ? L.z1 : L.z0, v = end ? 1.0 : 0.0;
+        std::vector<double> c3;
+        for (int i = 0; i < 8; ++i) c3.insert(c3.end(), {sgm[2 * i], sgm[2 * i + 1], z, 1.0});
+        auto pcs = five_spans(c3, 4);
+        std::vector<double> uv;
+        for (int m = 0; m < 5; ++m) uv.insert(uv.end(), {0.2 * m, v, 0.2 * (m + 1), v, NaN, NaN, NaN, NaN});
+        const bool on_top = (L.kind == 0) ? end == 1 : end == 0;  // the rim on the plate's top face
+        b.spline(3, pcs, walls[k], on_top ? f_near : f_cap, uv);
+      }
+      // the vertical edge where segment k meets segment k + 1, and its two vertices
+      const int k1 = (k + 1) % F;
+      V pj{sgm[14], sgm[15], 0};
+      const double tx = sgm[14] - sgm[12], ty = sgm[15] - sgm[13], tl = std::hypot(tx, ty);
+      V nout{ty / tl, -tx / tl, 0};  // the tangent times z: out of the outline
+      V nwall = nout * (L.kind == 0 ? -1.0 : 1.0);
+      V a{pj.x, pj.y, L.z0}, e{pj.x, pj.y, L.z1};
+      b.line(a, e, walls[k], walls[k1], o->keep({1, 0, 1, 1, 0, 0, 0, 1}));
+      // each end: a plane at angle pi (the loop is smooth there), the two walls at pi / 2
+      b.vert(a, (Z + nwall) * M_PI);
+      b.vert(e, (Z + nwall) * M_PI);
+    }
+  }
+  o->finish({-ax, -ay, -az}, {ax, ay, zmax}, 1e-9);
+  return o;
+}
+
 // The batched workload's parts: part i of 64 from the seed. Face counts log-uniform in
 // 6..400: turned parts (3..14 faces) and NURBS-top blocks (6 faces, nets 4 x 4 to
 // 32 x 32) below 15, plates with holes above.
@@ -1052,6 +1268,7 @@ const icam_part_desc* icam_gen_part(const char* name, uint64_t seed) {
   if (s == "c1") o = make_c1();
   if (s == "c2") o = make_c2(seed, false);
   if (s == "c2flat") o = make_c2(seed, true);
+  if (s == "c3") o = make_c3(seed);
   if (s.rfind("c4:", 0) == 0) o = make_c4(seed, arg(3));
   if (s.rfind("turned:", 0) == 0) {
     Rng rng{seed | 1};
@@ -1070,7 +1287,9 @@ void icam_gen_free(const icam_part_desc* p) { delete reinterpret_cast<const Owne
 
 // What a part was made from, for closed-form checks: [1, n, n elements] for a solid of
 // revolution about z, [2, W, D, T, k, then per hole x, y, n, n elements] for a plate,
-// [] otherwise; an element is (arc, r0, z0, r1, z1, rc, zc, rad, psi0, psi1, face).
+// [3, W, D, T, loops, then per loop kind (0 pocket, 1 boss), z0, z1, F, then F x 16
+// control points (x, y)] for the trimmed plate, [] otherwise; an element is (arc, r0,
+// z0, r1, z1, rc, zc, rad, psi0, psi1, face).
 // Writes at most cap values and returns how many there are.
 size_t icam_gen_describe(const icam_part_desc* p, double* out, size_t cap) {
   const Owned* o = reinterpret_cast<const Owned*>(p);

@@ -43,6 +43,9 @@ using namespace icam;
 namespace {
 
 constexpr int WG = 256;
+// The settle kernels' grid: their lanes hold the certified search's queue in private
+// memory, so few blocks run at once (the scratch they need is per block in flight).
+constexpr int SETTLE_GRID = 256;
 constexpr int SORTC = 1024;  // lists up to this long are sorted nearest first
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int TCAP = 512;    // a tile's list kept for its bricks' nodes, at most
@@ -285,13 +288,13 @@ __device__ inline T part_lb(const Tab<T>& tb, uint32_t i, uint32_t code, v3<T> p
   if ((code >> 28) == P_LEAF) lb = mmax(lb, slab_dist(tb, code & 0x0fffffffu, p, eps));
   return lb;
 }
-template <class T>
-__device__ inline void try_part(const Tab<T>& tb, uint32_t i, T lb, v3<T> p, Best<T>& b, T tm, T eps, bool settle = false) {
+template <class T, bool SETTLE = false>
+__device__ inline void try_part(const Tab<T>& tb, uint32_t i, T lb, v3<T> p, Best<T>& b, T tm, T eps) {
   const uint32_t code = tb.pcode[i], id = code & 0x0fffffffu;
   if ((code >> 28) == P_LEAF)
-    try_leaf(tb, id, tb.pbox + 6 * i, lb, p, b, tm, eps, settle);
+    try_leaf<T, SETTLE>(tb, id, tb.pbox + 6 * i, lb, p, b, tm, eps);
   else
-    try_cleaf(tb, id, tb.pbox + 6 * i, lb, p, b, eps, settle);
+    try_cleaf<T, SETTLE>(tb, id, tb.pbox + 6 * i, lb, p, b, eps);
 }
 template <int K, class V> __device__ inline V group_min(V v) {
   for (int o = 1; o < K; o <<= 1) v = mmin(v, V(__shfl_xor(v, o)));
@@ -332,9 +335,9 @@ template <class T> __device__ inline Best<T> item_best(const uint32_t* w) {
   return b;
 }
 
-template <class T, int NT, int Q, int K, int C>
+template <class T, int NT, int Q, int K, int C, bool SETTLE = false>
 __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk,
-                          const Res* defer = nullptr, long point = 0, bool settle = false) {
+                          const Res* defer = nullptr, long point = 0) {
   Best<T> b;
   b.d = bound, b.code = NONE, b.a = b.b = T(0), b.sgn = T(1), b.lmin = TypeEps<T>::big, b.d2 = TypeEps<T>::big, b.sgn2 = T(1), b.unsure = false;
   const bool all = nc > C;
@@ -420,7 +423,7 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const 
         k = uint32_t(next);
         last_lb = next_lb, last_k = k;
       }
-      try_part(tb, all ? k : cand[k], next_lb, p, b, tm, eps, settle);
+      try_part<T, SETTLE>(tb, all ? k : cand[k], next_lb, p, b, tm, eps);
     }
   }
   if (K > 1) {
@@ -745,8 +748,9 @@ __global__ __launch_bounds__(256) void k_refine(Dev dv, Job j, Res res) {
 // flag 3, the fallback of the mixed form; flag 4, the settle (parts past their
 // convexity certificate settled by the certified search, as the reference settles
 // them on the CPU, crates/icam-sim/src/partcache.rs:563-751).
-template <int S>
-__device__ __forceinline__ void redo_block(const Dev& dv, const Job& j, const Res& res, long blk, uint32_t want, SearchLds<S, double>& sm) {
+template <int S, bool SETTLE>
+__device__ __forceinline__ void redo_block(const Dev& dv, const Job& j, const Res& res, long blk, SearchLds<S, double>& sm) {
+  const uint32_t want = SETTLE ? 4u : 3u;
   constexpr int NT = Shape<S>::NT;
   const int nc = block_candidates<S>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot);
   const Tab<double> tb = tab_of<double>(dv);
@@ -758,8 +762,8 @@ __device__ __forceinline__ void redo_block(const Dev& dv, const Job& j, const Re
     if (K == 1 && !valid) continue;
     double bound = 0.0;
     v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
-    Best<double> b = search<double, NT, Shape<S>::Q, K, Shape<S>::C>(tb, p, valid, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, sm.cand, nc, 0.0,
-                                                                     dv.L.eps64, sm.qlb(), sm.qk(), nullptr, 0, want == 4);
+    Best<double> b = search<double, NT, Shape<S>::Q, K, Shape<S>::C, SETTLE>(tb, p, valid, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, sm.cand,
+                                                                             nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk());
     if (!valid || !owner) continue;
     double val;
     uint32_t flag, feat;
@@ -775,7 +779,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_fallback(Dev dv, Job j, Res re
   __shared__ SearchLds<S, double> sm;
   const uint32_t total = *res.qcount;
   for (uint32_t qb = blockIdx.x; qb < total; qb += gridDim.x) {
-    redo_block<S>(dv, j, res, long(res.queue[qb] & 0xffffffu), 3u, sm);
+    redo_block<S, false>(dv, j, res, long(res.queue[qb] & 0xffffffu), sm);
     __syncthreads();
   }
 }
@@ -802,7 +806,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_settle(Dev dv, Job j, Res res)
   __shared__ SearchLds<S, double> sm;
   const uint32_t total = *res.scount;
   for (uint32_t qb = blockIdx.x; qb < total; qb += gridDim.x) {
-    redo_block<S>(dv, j, res, long(res.squeue[qb] & 0xffffffu), 4u, sm);
+    redo_block<S, true>(dv, j, res, long(res.squeue[qb] & 0xffffffu), sm);
     __syncthreads();
   }
 }
@@ -1356,11 +1360,11 @@ __global__ __launch_bounds__(256) void kb_mark(BatchArgs a) {
 
 // The fallback (want 3) or the settle (want 4) over the batch's shared queue, whose
 // entries are part << 24 | block.
-template <int S>
-__global__ __launch_bounds__(Shape<S>::NT) void kb_redo(BatchArgs a, uint32_t want) {
+template <int S, bool SETTLE>
+__global__ __launch_bounds__(Shape<S>::NT) void kb_redo(BatchArgs a) {
   __shared__ SearchLds<S, double> sm;
   __shared__ uint32_t wslot;
-  const bool settle = want == 4;
+  const bool settle = SETTLE;
   const int st = S == SRC_CENTRES ? (settle ? BS_CSETTLE : BS_CFALL) : (settle ? BS_NSETTLE : BS_NFALL);
   const uint32_t total = a.ctl[S == SRC_CENTRES ? (settle ? BC_SC : BC_QC) : (settle ? BC_SN : BC_QN)];
   for (;;) {
@@ -1373,7 +1377,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void kb_redo(BatchArgs a, uint32_t wa
     const Dev dv = ldk(P.dv);
     const Job j = S == SRC_CENTRES ? ldk(P.jc) : ldk(P.jn);
     const Res r = S == SRC_CENTRES ? ldk(P.rc) : ldk(P.rn);
-    redo_block<S>(dv, j, r, long(e & 0xffffffu), want, sm);
+    redo_block<S, SETTLE>(dv, j, r, long(e & 0xffffffu), sm);
     __syncthreads();
   }
 }
@@ -1660,7 +1664,7 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
   HIP_OK(hipMemsetAsync(r.scount, 0, 4, s));
   HIP_OK(hipMemsetAsync(r.smark, 0, size_t(blocks) * 4, s));
   HIP_OK(launch(st, names[S][6], s, dim3(unsigned((j.n + 255) / 256)), dim3(256), k_mark_settle<S>, j, r));
-  HIP_OK(launch(st, names[S][7], s, dim3(unsigned(std::min<long>(blocks, 1024))), dim3(NT), k_settle<S>, h->dv, j, r));
+  HIP_OK(launch(st, names[S][7], s, dim3(unsigned(std::min<long>(blocks, SETTLE_GRID))), dim3(NT), k_settle<S>, h->dv, j, r));
   return 0;
 }
 
@@ -1900,8 +1904,8 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     HIP_OK(hipMemsetAsync(r.smark, 0, size_t(tiles) * 4, s));
     HIP_OK(launch(&st, "centres_need", s, dim3(unsigned(std::min<long>(nband, 4096))), dim3(256), k_need_centre, jc, r, (const uint32_t*)h->list.as<uint32_t>(),
                   long(nband), (const uint32_t*)rn.flag, (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>()));
-    HIP_OK(launch(&st, "centres_settle64", s, dim3(unsigned(std::min<long>(tiles, 1024))), dim3(Shape<SRC_CENTRES>::NT), k_settle<SRC_CENTRES>, h->dv, jc,
-                  r));
+    HIP_OK(launch(&st, "centres_settle64", s, dim3(unsigned(std::min<long>(tiles, SETTLE_GRID))), dim3(Shape<SRC_CENTRES>::NT), k_settle<SRC_CENTRES>,
+                  h->dv, jc, r));
     HIP_OK(launch(&st, "centres_sign", s, dim3(unsigned(std::min<long>((long(nband) + 255) / 256, 4096))), dim3(256), k_centre_sign, r,
                   (const uint32_t*)h->list.as<uint32_t>(), long(nband), h->dc.as<double>(), h->sgn.as<int8_t>()));
     HIP_OK(launch(&st, "bricks_finish", s, dim3(nband), dim3(WG), k_bricks_finish, lv, (const uint32_t*)h->list.as<uint32_t>(), long(nband),
@@ -2044,61 +2048,55 @@ uint32_t band_room(const icam_hip* h, const Lv& lv) {
   return uint32_t(c);
 }
 
-// The batch's launches on stream s.
-int batch_enqueue(icam_batch* B, hipStream_t s) {
+// The batch's launches on stream s (with per-stage HIP events when st is not NULL).
+int batch_enqueue(icam_batch* B, hipStream_t s, Stages* st = nullptr) {
   const BatchArgs& a = B->args;
   const bool mixed = B->precision == ICAM_MIXED;
-  auto fill = [&](Buf& b, size_t words) -> hipError_t {
-    if (!words) return hipSuccess;
-    hipLaunchKernelGGL(kb_fill, dim3(unsigned(std::min<size_t>((words + 255) / 256, 4096))), dim3(256), 0, s, b.as<uint32_t>(), words, 0u);
-    return hipGetLastError();
+  auto fill = [&](Buf& b, size_t words) -> int {
+    if (!words) return 0;
+    HIP_OK(launch(st, "zero", s, dim3(unsigned(std::min<size_t>((words + 255) / 256, 4096))), dim3(256), kb_fill, b.as<uint32_t>(), words, 0u));
+    return 0;
   };
-  HIP_OK(fill(B->ctl, BC_WORDS));
-  HIP_OK(fill(B->counters, size_t(B->n) * 16));
-  HIP_OK(fill(B->c_qmark, size_t(B->total_tiles)));
-  HIP_OK(fill(B->c_smark, size_t(B->total_tiles)));
-  HIP_OK(fill(B->n_qmark, B->total_cap));
-  HIP_OK(fill(B->n_smark, B->total_cap));
+  if (fill(B->ctl, BC_WORDS) || fill(B->counters, size_t(B->n) * 16) || fill(B->c_qmark, size_t(B->total_tiles)) ||
+      fill(B->c_smark, size_t(B->total_tiles)) || fill(B->n_qmark, B->total_cap) || fill(B->n_smark, B->total_cap))
+    return 1;
   const unsigned pts_c = unsigned(std::min<long>((B->total_bricks + 255) / 256, 8192));
   const unsigned pts_n = unsigned(std::min<size_t>((B->total_cap * 729 + 255) / 256, 16384));
+  const dim3 ntc(Shape<SRC_CENTRES>::NT), ntn(Shape<SRC_NODES>::NT);
   // the brick centres: the level's decision
   if (mixed) {
-    hipLaunchKernelGGL((kb_search<SRC_CENTRES, float>), dim3(B->gc), dim3(Shape<SRC_CENTRES>::NT), 0, s, a);
-    hipLaunchKernelGGL((kb_refine<SRC_CENTRES>), dim3(pts_c), dim3(256), 0, s, a);
-    hipLaunchKernelGGL((kb_redo<SRC_CENTRES>), dim3(B->gc), dim3(Shape<SRC_CENTRES>::NT), 0, s, a, 3u);
+    HIP_OK(launch(st, "centres_search32", s, dim3(B->gc), ntc, kb_search<SRC_CENTRES, float>, a));
+    HIP_OK(launch(st, "centres_refine64", s, dim3(pts_c), dim3(256), kb_refine<SRC_CENTRES>, a));
+    HIP_OK(launch(st, "centres_fallback64", s, dim3(B->gc), ntc, kb_redo<SRC_CENTRES, false>, a));
   } else {
-    hipLaunchKernelGGL((kb_search<SRC_CENTRES, double>), dim3(B->gc), dim3(Shape<SRC_CENTRES>::NT), 0, s, a);
+    HIP_OK(launch(st, "centres_search64", s, dim3(B->gc), ntc, kb_search<SRC_CENTRES, double>, a));
   }
-  HIP_OK(hipGetLastError());
-  hipLaunchKernelGGL(kb_decide, dim3(pts_c), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(kb_sign, dim3(B->n), dim3(SIGN_T), 0, s, a);
-  hipLaunchKernelGGL(kb_band_list, dim3(B->n), dim3(SCAN_T), 0, s, a);
-  hipLaunchKernelGGL(kb_plan, dim3(1), dim3(64), 0, s, a);
-  HIP_OK(hipGetLastError());
+  HIP_OK(launch(st, "decide", s, dim3(pts_c), dim3(256), kb_decide, a));
+  HIP_OK(launch(st, "sign", s, dim3(B->n), dim3(SIGN_T), kb_sign, a));
+  HIP_OK(launch(st, "band_list", s, dim3(B->n), dim3(SCAN_T), kb_band_list, a));
+  HIP_OK(launch(st, "plan", s, dim3(1), dim3(64), kb_plan, a));
   // the band bricks' nodes
   const unsigned gl = unsigned(std::min<size_t>((B->total_cap * 729 / 8 + 1024 + 255) / 256, 2048));
   if (mixed) {
-    hipLaunchKernelGGL((kb_search<SRC_NODES, float>), dim3(B->gn), dim3(Shape<SRC_NODES>::NT), 0, s, a);
-    hipLaunchKernelGGL(kb_leaves<float>, dim3(gl), dim3(256), 0, s, a);
-    hipLaunchKernelGGL((kb_refine<SRC_NODES>), dim3(pts_n), dim3(256), 0, s, a);
-    hipLaunchKernelGGL((kb_redo<SRC_NODES>), dim3(B->gn), dim3(Shape<SRC_NODES>::NT), 0, s, a, 3u);
+    HIP_OK(launch(st, "nodes_search32", s, dim3(B->gn), ntn, kb_search<SRC_NODES, float>, a));
+    HIP_OK(launch(st, "nodes_leaves32", s, dim3(gl), dim3(256), kb_leaves<float>, a));
+    HIP_OK(launch(st, "nodes_refine64", s, dim3(pts_n), dim3(256), kb_refine<SRC_NODES>, a));
+    HIP_OK(launch(st, "nodes_fallback64", s, dim3(B->gn), ntn, kb_redo<SRC_NODES, false>, a));
   } else {
-    hipLaunchKernelGGL((kb_search<SRC_NODES, double>), dim3(B->gn), dim3(Shape<SRC_NODES>::NT), 0, s, a);
-    hipLaunchKernelGGL(kb_leaves<double>, dim3(gl), dim3(256), 0, s, a);
+    HIP_OK(launch(st, "nodes_search64", s, dim3(B->gn), ntn, kb_search<SRC_NODES, double>, a));
+    HIP_OK(launch(st, "nodes_leaves64", s, dim3(gl), dim3(256), kb_leaves<double>, a));
   }
-  hipLaunchKernelGGL((kb_mark<SRC_NODES>), dim3(pts_n), dim3(256), 0, s, a);
-  hipLaunchKernelGGL((kb_redo<SRC_NODES>), dim3(B->gn), dim3(Shape<SRC_NODES>::NT), 0, s, a, 4u);
+  HIP_OK(launch(st, "nodes_mark", s, dim3(pts_n), dim3(256), kb_mark<SRC_NODES>, a));
+  HIP_OK(launch(st, "nodes_settle64", s, dim3(SETTLE_GRID), ntn, kb_redo<SRC_NODES, true>, a));
   // the centres of all-far band bricks with no sign: settled
-  hipLaunchKernelGGL(kb_need_centre, dim3(B->gf), dim3(256), 0, s, a);
-  hipLaunchKernelGGL((kb_redo<SRC_CENTRES>), dim3(B->gc), dim3(Shape<SRC_CENTRES>::NT), 0, s, a, 4u);
-  hipLaunchKernelGGL(kb_centre_sign, dim3(unsigned(std::min<size_t>((B->total_cap + 255) / 256, 4096))), dim3(256), 0, s, a);
-  HIP_OK(hipGetLastError());
+  HIP_OK(launch(st, "centres_need", s, dim3(B->gf), dim3(256), kb_need_centre, a));
+  HIP_OK(launch(st, "centres_settle64", s, dim3(SETTLE_GRID), ntc, kb_redo<SRC_CENTRES, true>, a));
+  HIP_OK(launch(st, "centres_sign", s, dim3(unsigned(std::min<size_t>((B->total_cap + 255) / 256, 4096))), dim3(256), kb_centre_sign, a));
   // fill, quantize, pack, number and store
-  hipLaunchKernelGGL(kb_finish, dim3(B->gf), dim3(WG), 0, s, a);
-  hipLaunchKernelGGL(kb_slots, dim3(B->n), dim3(SCAN_T), 0, s, a);
-  hipLaunchKernelGGL(kb_store, dim3(pts_c), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(kb_store_band, dim3(B->gf), dim3(128), 0, s, a);
-  HIP_OK(hipGetLastError());
+  HIP_OK(launch(st, "bricks_finish", s, dim3(B->gf), dim3(WG), kb_finish, a));
+  HIP_OK(launch(st, "slots", s, dim3(B->n), dim3(SCAN_T), kb_slots, a));
+  HIP_OK(launch(st, "store", s, dim3(pts_c), dim3(256), kb_store, a));
+  HIP_OK(launch(st, "store_band", s, dim3(B->gf), dim3(128), kb_store_band, a));
   return 0;
 }
 
@@ -2326,6 +2324,27 @@ int icam_hip_batch_download(icam_batch* b, size_t i, uint32_t* map, uint32_t* at
   if (words > atlas_cap) return fail("atlas needs " + std::to_string(words) + " words, the buffer holds " + std::to_string(atlas_cap));
   if (atlas && words) HIP_OK(hipMemcpy(atlas, da, words * 4, hipMemcpyDeviceToHost));
   return 0;
+}
+
+int icam_hip_batch_stage_times(icam_batch* b, const char** names, double* ms, int cap) {
+  if (!b || !names || !ms) return fail("bad arguments");
+  Stages st;
+  if (batch_enqueue(b, b->stream, &st)) return 1;
+  HIP_OK(hipStreamSynchronize(b->stream));
+  std::vector<std::pair<const char*, double>> acc;
+  for (size_t i = 0; i < st.tm.size(); ++i) {
+    double m = st.tm[i].ms();
+    bool found = false;
+    for (auto& x : acc)
+      if (std::strcmp(x.first, st.names[i]) == 0) x.second += m, found = true;
+    if (!found) acc.push_back({st.names[i], m});
+  }
+  int n = 0;
+  for (auto& x : acc) {
+    if (n >= cap) break;
+    names[n] = x.first, ms[n] = x.second, ++n;
+  }
+  return n;
 }
 
 int icam_hip_batch_tuning(icam_batch* b, char* buf, size_t cap) {

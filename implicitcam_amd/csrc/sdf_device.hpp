@@ -637,14 +637,14 @@ template <class T> __device__ inline bool is_critical(v3<T> r, v3<T> d, T tol) {
 // point inside the face answers, a non-critical one is carried across spans by Newton
 // first; any point of the face found lowers the part's pruning bound; a quarter is
 // dropped when its control points' box, or their slab along the direction from its
-// Newton point, keeps it at or past that bound, or at the depth limit. The reference's
-// critical-point clearing (surface3.rs:694-731) only prunes sooner and is not
-// restated. A part whose queue overflows, or that takes more than SETTLE_POPS reads,
-// is left undecided with its lower bound.
-constexpr int SETTLE_Q = 40;
+// Newton point, keeps it at or past that bound, or at the depth limit, or when the
+// squared distance has no critical point on it (surface3.rs:694-731). A part whose
+// queue overflows, or that takes more than SETTLE_POPS reads, is left undecided with
+// its lower bound.
+constexpr int SETTLE_Q = 128;
 constexpr int SETTLE_DEPTH = 40;     // surface3.rs:1291
 constexpr int SETTLE_NEWTON = 12;    // surface3.rs:552
-constexpr int SETTLE_POPS = 4096;
+constexpr int SETTLE_POPS = 20000;  // surface3.rs:1230
 constexpr double SETTLE_EPS = 1e-9;  // surface3.rs:555
 
 // The homogeneous cubic c over [0, 1] replaced by its piece over [a, b] (de Casteljau:
@@ -775,6 +775,80 @@ struct SettleItem {
   int depth;
 };
 
+// out (om x on Bernstein coefficients) += s * a (am x an) * b (bm x bn), om = am + bm,
+// on = an + bn: the product of tensor Bernstein polynomials (surface3.rs:593-630).
+__device__ inline double binom(int n, int k) {
+  double r = 1.0;
+  for (int i = 1; i <= k; ++i) r = r * double(n - k + i) / double(i);
+  return r;
+}
+__device__ inline void bern_mul_add(const double* a, int am, int an, const double* b, int bm, int bn, double s, double* out) {
+  const int om = am + bm, on = an + bn;
+  for (int i = 0; i <= am; ++i)
+    for (int j = 0; j <= an; ++j) {
+      const double x = s * a[i * (an + 1) + j] * binom(am, i) * binom(an, j);
+      if (x == 0.0) continue;
+      for (int k = 0; k <= bm; ++k)
+        for (int l = 0; l <= bn; ++l)
+          out[(i + k) * (on + 1) + j + l] += x * b[k * (bn + 1) + l] * binom(bm, k) * binom(bn, l) / (binom(om, i + k) * binom(on, j + l));
+    }
+}
+
+// surface3.rs:694-731 no_critical_point: the squared distance from p has no critical
+// point on the closed patch (homogeneous bicubic net n) when the gradient's component
+// along (a, b), the gradient's direction at the patch's middle, has all its Bernstein
+// coefficients clear of zero: with Q = P - p w (homogeneous), the components are
+// w Q.Q_s - w_s Q.Q and w Q.Q_t - w_t Q.Q over w^3.
+__device__ inline bool no_critical64(const double n[4][4][4], v3<double> p, double a, double b) {
+  double q[3][16], w[16], qu[3][12], qv[3][12], wu[12], wv[12];
+  const double pc[3] = {p.x, p.y, p.z};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      w[i * 4 + j] = n[i][j][3];
+      if (!(n[i][j][3] > 0.0)) return false;
+      for (int k = 0; k < 3; ++k) q[k][i * 4 + j] = n[i][j][k] - pc[k] * n[i][j][3];
+    }
+  // derivatives: along s (2 x 3 degrees, i < 3), along t (3 x 2, j < 3)
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 4; ++j) {
+      wu[i * 4 + j] = 3.0 * (w[(i + 1) * 4 + j] - w[i * 4 + j]);
+      for (int k = 0; k < 3; ++k) qu[k][i * 4 + j] = 3.0 * (q[k][(i + 1) * 4 + j] - q[k][i * 4 + j]);
+    }
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 3; ++j) {
+      wv[i * 3 + j] = 3.0 * (w[i * 4 + j + 1] - w[i * 4 + j]);
+      for (int k = 0; k < 3; ++k) qv[k][i * 3 + j] = 3.0 * (q[k][i * 4 + j + 1] - q[k][i * 4 + j]);
+    }
+  double qq[49] = {0}, qqu[42] = {0}, qqv[42] = {0};
+  for (int k = 0; k < 3; ++k) {
+    bern_mul_add(q[k], 3, 3, q[k], 3, 3, 1.0, qq);
+    bern_mul_add(q[k], 3, 3, qu[k], 2, 3, 1.0, qqu);
+    bern_mul_add(q[k], 3, 3, qv[k], 3, 2, 1.0, qqv);
+  }
+  double gs[90] = {0}, gt[90] = {0};  // (8, 9) and (9, 8)
+  bern_mul_add(w, 3, 3, qqu, 5, 6, 1.0, gs);
+  bern_mul_add(wu, 2, 3, qq, 6, 6, -1.0, gs);
+  bern_mul_add(w, 3, 3, qqv, 6, 5, 1.0, gt);
+  bern_mul_add(wv, 3, 2, qq, 6, 6, -1.0, gt);
+  auto maxabs = [](const double* x, int c) {
+    double m = 0.0;
+    for (int i = 0; i < c; ++i) m = fmax(m, fabs(x[i]));
+    return m;
+  };
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+  for (int k = 0; k < 3; ++k) q0 = fmax(q0, maxabs(q[k], 16)), q1 = fmax(q1, maxabs(qu[k], 12)), q2 = fmax(q2, maxabs(qv[k], 12));
+  const double size = maxabs(w, 16) * 3.0 * q0 * (fabs(a) * q1 + fabs(b) * q2) + (fabs(a) * maxabs(wu, 12) + fabs(b) * maxabs(wv, 12)) * 3.0 * q0 * q0;
+  // f = a (gs raised along s) + b (gt raised along t), 10 x 10 coefficients
+  for (int i = 0; i <= 9; ++i)
+    for (int j = 0; j <= 9; ++j) {
+      const double al = double(i) / 9.0, be = double(j) / 9.0;
+      const double su = (i > 0 ? al * gs[(i - 1) * 10 + j] : 0.0) + (i < 9 ? (1.0 - al) * gs[i * 10 + j] : 0.0);
+      const double tv = (j > 0 ? be * gt[i * 9 + j - 1] : 0.0) + (j < 9 ? (1.0 - be) * gt[i * 9 + j] : 0.0);
+      if (!(a * su + b * tv > 1e-11 * size)) return false;
+    }
+  return true;
+}
+
 // Sub-patch li settled for p (its lower bound lb); false when left incomplete.
 __device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb, v3<double> p, Best<double>& b) {
   const Layout& L = tb.L;
@@ -829,6 +903,13 @@ __device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb
     hull_bounds(&net[0][0], 16, p, dir, box, slab);
     if (slab >= bd - SETTLE_EPS || it.depth >= SETTLE_DEPTH) continue;
     const double am = 0.5 * (it.a0 + it.a1), bm = 0.5 * (it.b0 + it.b1);
+    {
+      // no critical point on the quarter: no minimum of its own to find there
+      SEval<double> em = peval(sp, am, bm);
+      const v3<double> rm = em.p - p;
+      const double ga = dot(em.ps, rm) * (it.a1 - it.a0), gb = dot(em.pt, rm) * (it.b1 - it.b0), gl = sqrt(ga * ga + gb * gb);
+      if (gl > 0.0 && no_critical64(net, p, ga / gl, gb / gl)) continue;
+    }
     for (int c = 0; c < 4; ++c) {
       SettleItem ch{(c & 1) ? am : it.a0, (c & 1) ? it.a1 : am, (c & 2) ? bm : it.b0, (c & 2) ? it.b1 : bm, 0.0, it.depth + 1};
       double cn[4][4][4], cb, cs;
@@ -851,16 +932,16 @@ __device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb
 // certifies a single minimum for p; the minimum answers when it is a critical point
 // inside the face. With `settle` (double precision), a sub-patch past its certificate
 // is settled by the certified search instead of leaving its lower bound.
-template <class T> __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps,
-                                            bool settle = false) {
+template <class T, bool SETTLE = false>
+__device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps) {
   const Layout& L = tb.L;
   const ICAM_K T* lf = tb.F + L.leaf_f + LEAF_F * li;
   const ICAM_K uint32_t* lu = tb.U + L.leaf_u + LEAF_U * li;
   T rho = lf[0];
   T fd = far_dist(p, box);
   bool certified = fd < T(0.999) * rho - eps;
-  if constexpr (sizeof(T) == 8) {
-    if (settle && !certified) {
+  if constexpr (SETTLE && sizeof(T) == 8) {
+    if (!certified) {
       if (!settle_leaf(tb, li, lb, p, b)) b.lmin = mmin(b.lmin, lb);
       return;
     }
@@ -943,7 +1024,7 @@ __device__ inline bool settle_cleaf(const Tab<double>& tb, uint32_t ci, double l
       sub_cubic4(cc, ch.a0, ch.a1);
       double cb, cs;
       hull_bounds(cc, 4, p, dir, cb, cs);
-      ch.lb = cb;
+      ch.lb = fmax(cb, cs);
       if (ch.lb < bd - SETTLE_EPS) {
         if (nq == SETTLE_Q) {
           complete = false;
@@ -959,15 +1040,15 @@ __device__ inline bool settle_cleaf(const Tab<double>& tb, uint32_t ci, double l
 // gpu.wgsl:764-815: a part of a curve piece read by Newton when certified; the minimum
 // answers when it is critical and interior to its edge. With `settle` (double
 // precision), a part past its certificate is settled by the certified search.
-template <class T> __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T eps,
-                                             bool settle = false) {
+template <class T, bool SETTLE = false>
+__device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T eps) {
   const Layout& L = tb.L;
   const ICAM_K T* cf = tb.F + L.cleaf_f + CLEAF_F * ci;
   const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * ci;
   T rho = cf[0];
   bool certified = far_dist(p, box) < T(0.999) * rho - eps;
-  if constexpr (sizeof(T) == 8) {
-    if (settle && !certified) {
+  if constexpr (SETTLE && sizeof(T) == 8) {
+    if (!certified) {
       if (!settle_cleaf(tb, ci, lb, p, b)) b.lmin = mmin(b.lmin, lb);
       return;
     }

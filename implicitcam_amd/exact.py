@@ -59,6 +59,7 @@ def lib():
                                             ctypes.POINTER(BakeStats)]
         L.icam_hip_batch_download.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u32p, _u32p, ctypes.c_size_t, ctypes.POINTER(BakeStats)]
         L.icam_hip_batch_tuning.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+        L.icam_hip_batch_stage_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), _dp, ctypes.c_int]
         L.icam_hip_batch_close.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
@@ -205,6 +206,15 @@ class Batch:
         atlas = np.empty(max(st.slots * 365, 1), dtype=np.uint32)
         _check(lib().icam_hip_batch_download(self.b, i, m.ctypes.data_as(_u32p), atlas.ctypes.data_as(_u32p), len(atlas), ctypes.byref(st)))
         return m, atlas[: st.slots * 365].copy(), st
+
+    def stage_times(self):
+        """Device time (ms) of each stage of one run outside the graph (waits)."""
+        names = (ctypes.c_char_p * 48)()
+        ms = (ctypes.c_double * 48)()
+        k = lib().icam_hip_batch_stage_times(self.b, names, ms, 48)
+        if k < 0:
+            _check(1)
+        return {names[i].decode(): ms[i] for i in range(k)}
 
     @property
     def tuning(self):

@@ -155,6 +155,11 @@ int icam_hip_batch_result(icam_batch* b, size_t i, const uint32_t** d_map, const
 /* The same copied to host arrays (waits for the batch's own stream first). */
 int icam_hip_batch_download(icam_batch* b, size_t i, uint32_t* map, uint32_t* atlas, size_t atlas_cap, icam_bake_stats* stats);
 
+/* One run outside the graph with HIP events between its launches (waits): the
+ * device time (ms) of each stage, names[i] (static strings) and ms[i] for i below the
+ * returned count (at most cap). */
+int icam_hip_batch_stage_times(icam_batch* b, const char** names, double* ms, int cap);
+
 /* The autotuner's report: the candidates it timed and its choice. */
 int icam_hip_batch_tuning(icam_batch* b, char* buf, size_t cap);
 

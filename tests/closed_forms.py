@@ -197,3 +197,146 @@ def described_sdf(part, p):
     if len(info) == 0:
         return None
     return turned_sdf(p, info) if info[0] == 1 else plate_sdf(p, info)
+
+
+# ---------------------------------------------------------------- the trimmed plate (C3)
+#
+# Its surfaces are planes and vertical walls over closed outlines (curves of the
+# plane), so every distance is a function of the planar distance to an outline and of
+# heights: the nearest point of a wall over outline C between heights z0 and z1 is
+# (c, clamp(z, z0, z1)) with c the outline's nearest point to (x, y). Outlines are
+# projected exactly: nearest of dense samples, then Newton on that cubic piece and its
+# neighbours.
+
+_KN = np.array([0, 0, 0, 0, 0.2, 0.4, 0.6, 0.8, 1, 1, 1, 1.0])
+
+
+def _bezier_pieces_of_segment(ctrl):
+    """The 5 cubic Bezier pieces (4 x 2 each) of a clamped cubic B-spline with 8 control
+    points and knots 0 0 0 0 .2 .4 .6 .8 1 1 1 1 (knot insertion to multiplicity 3)."""
+    kn = list(_KN)
+    P = [np.array(c, dtype=np.float64) for c in ctrl]
+    for u in (0.2, 0.4, 0.6, 0.8):
+        while kn.count(u) < 3:
+            k = max(i for i in range(len(kn) - 1) if kn[i] <= u < kn[i + 1])
+            s = kn.count(u)
+            Q = P[: k - 3 + 1]
+            for i in range(k - 3 + 1, k - s + 1):
+                a = (u - kn[i]) / (kn[i + 3] - kn[i])
+                Q.append(a * P[i] + (1 - a) * P[i - 1])
+            Q += P[k - s:]
+            kn.insert(k + 1, u)
+            P = Q
+    return [np.array(P[3 * m: 3 * m + 4]) for m in range(5)]
+
+
+class Outline:
+    """A closed outline of F five-span segments (control points from the description)."""
+
+    def __init__(self, segs):
+        self.pieces = np.array([pc for sg in segs for pc in _bezier_pieces_of_segment(sg)])  # (n, 4, 2)
+        t = np.linspace(0, 1, 65)[:-1]
+        B = np.stack([(1 - t) ** 3, 3 * t * (1 - t) ** 2, 3 * t * t * (1 - t), t ** 3], 1)
+        self.samples = np.einsum("tk,nkd->ntd", B, self.pieces).reshape(-1, 2)
+        self.sample_piece = np.repeat(np.arange(len(self.pieces)), len(t))
+        self.sample_t = np.tile(t, len(self.pieces))
+        from scipy.spatial import cKDTree
+
+        self.tree = cKDTree(self.samples)
+
+    def _eval(self, pc, t):
+        s = 1 - t
+        b = np.stack([s ** 3, 3 * t * s * s, 3 * t * t * s, t ** 3], -1)
+        d = np.stack([-3 * s * s, 3 * s * s - 6 * t * s, 6 * t * s - 3 * t * t, 3 * t * t], -1)
+        e = np.stack([6 * s, -12 * s + 6 * t, 6 * s - 12 * t, 6 * t], -1)
+        return np.einsum("nk,nkd->nd", b, pc), np.einsum("nk,nkd->nd", d, pc), np.einsum("nk,nkd->nd", e, pc)
+
+    def project(self, xy):
+        """Distance to the outline, nearest point and whether xy is inside (left of the
+        counterclockwise outline)."""
+        _, k = self.tree.query(xy)
+        best_d = np.full(len(xy), np.inf)
+        best_q = np.zeros_like(xy)
+        best_tan = np.zeros_like(xy)
+        n = len(self.pieces)
+        for off in (-1, 0, 1):
+            pid = (self.sample_piece[k] + off) % n
+            t = np.clip(self.sample_t[k] if off == 0 else np.full(len(xy), 0.0 if off == 1 else 1.0), 0, 1)
+            pc = self.pieces[pid]
+            for _ in range(30):
+                q, d1, d2 = self._eval(pc, t)
+                r = q - xy
+                g = np.sum(r * d1, 1)
+                h = np.sum(d1 * d1, 1) + np.sum(r * d2, 1)
+                h = np.where(h > 0, h, np.sum(d1 * d1, 1))
+                t = np.clip(t - g / h, 0, 1)
+            q, d1, _ = self._eval(pc, t)
+            d = np.hypot(*(q - xy).T)
+            better = d < best_d
+            best_d, best_q[better], best_tan[better] = np.where(better, d, best_d), q[better], d1[better]
+        cross = best_tan[:, 0] * (xy[:, 1] - best_q[:, 1]) - best_tan[:, 1] * (xy[:, 0] - best_q[:, 0])
+        return best_d, best_q, cross > 0
+
+
+def c3_sdf(p, info):
+    W, D, T, nl = info[1], info[2], info[3], int(info[4])
+    ax, ay, az = W / 2, D / 2, T / 2
+    p = np.asarray(p, dtype=np.float64)
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    at = 5
+    loops = []
+    for _ in range(nl):
+        kind, z0, z1, F = info[at], info[at + 1], info[at + 2], int(info[at + 3])
+        segs = info[at + 4: at + 4 + 16 * F].reshape(F, 8, 2)
+        at += 4 + 16 * F
+        loops.append((int(kind), z0, z1, Outline(segs)))
+
+    def rect(u, v, hu, hv):
+        return np.hypot(np.maximum(np.abs(u) - hu, 0), np.maximum(np.abs(v) - hv, 0))
+
+    xy = p[:, :2]
+    top_plane = rect(x, y, ax, ay)  # in-plane distance to the top face's region
+    ds = [np.hypot(rect(x, y, ax, ay), z + az), np.hypot(rect(y, z, ay, az), x - ax), np.hypot(rect(y, z, ay, az), x + ax),
+          np.hypot(rect(x, z, ax, az), y - ay), np.hypot(rect(x, z, ax, az), y + ay)]
+    inside_box = (np.abs(x) < ax) & (np.abs(y) < ay) & (np.abs(z) < az)
+    solid = inside_box.copy()
+    for kind, z0, z1, ol in loops:
+        d2, _, inl = ol.project(xy)
+        top_plane = np.where(inl, d2, top_plane)
+        dz = np.maximum(np.maximum(z0 - z, z - z1), 0)
+        ds.append(np.hypot(d2, dz))  # the wall
+        zc = z0 if kind == 0 else z1  # the floor or the boss's top
+        ds.append(np.where(inl, np.abs(z - zc), np.hypot(d2, z - zc)))
+        if kind == 0:
+            solid &= ~(inl & (z > z0))
+        else:
+            solid |= inl & (z >= az) & (z < z1)
+    ds.append(np.hypot(top_plane, z - az))
+    d = np.min(np.array(ds), axis=0)
+    return np.where(solid, -d, d)
+
+
+def c3_near_trim_points(info, n, radius, seed):
+    """n points within `radius` of the outlines' rims on the plate's top face (the trim
+    curves of that face and of the walls' ends), seeded."""
+    rng = np.random.default_rng(seed)
+    W, D, T, nl = info[1], info[2], info[3], int(info[4])
+    at = 5
+    outs = []
+    for _ in range(nl):
+        F = int(info[at + 3])
+        outs.append(Outline(info[at + 4: at + 4 + 16 * F].reshape(F, 8, 2)))
+        at += 4 + 16 * F
+    li = rng.integers(0, nl, n)
+    out = np.empty((n, 3))
+    for k in range(nl):
+        m = li == k
+        ol = outs[k]
+        pid = rng.integers(0, len(ol.pieces), m.sum())
+        t = rng.random(m.sum())
+        q, _, _ = ol._eval(ol.pieces[pid], t)
+        v = rng.normal(size=(m.sum(), 3))
+        v *= (radius * rng.random(m.sum()) ** (1 / 3) / np.linalg.norm(v, axis=1))[:, None]
+        out[m, :2] = q + v[:, :2]
+        out[m, 2] = T / 2 + v[:, 2]
+    return out

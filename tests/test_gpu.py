@@ -320,3 +320,29 @@ def test_batch_tuned_uneven_grids():
         m, a, st = b.download(i)
         assert (m == m1).all() and len(a) == len(a1) and (a == a1).all(), i
     b.close()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_c3_trimmed_plate(prec):
+    """The trimmed plate: queries uniform over the part and within 0.05 mm of its trim
+    curves (the double-precision trim test's work in the mixed form) against the
+    oracle; the sign exact, the distance within the tolerance; a small level's bake
+    bit-identical to the oracle's."""
+    from closed_forms import c3_near_trim_points
+
+    part = Part("c3")
+    o = Oracle(part)
+    g = PartGpu(part, 4.0)
+    info = part.describe()
+    pts = np.concatenate([uniform_points(SEED, 1 << 14, part.lo - 2, part.hi + 2), c3_near_trim_points(info, 1 << 14, 0.05, 9)])
+    dref, fref, _ = o.sdf(pts)
+    d, flags, feat = g.query(pts, prec)
+    assert (flags == 0).all()
+    assert np.abs(d - dref).max() <= tol(prec, part)
+    assert ((d < 0) == (dref < 0)).all()
+    lv = level_for(part, 2.0, 16)
+    m_ref, a_ref, _ = o.bake(lv)
+    m, a, st = g.bake(lv, prec)
+    assert (m == m_ref).all()
+    qa, qr = a.view(np.int16).astype(int), a_ref.view(np.int16).astype(int)
+    assert len(qa) == len(qr) and np.abs(qa - qr).max(initial=0) <= 1 and ((qa > 0) == (qr > 0)).all()

@@ -345,3 +345,20 @@ def test_c4_parts(built):
     counts = np.array(counts)
     assert counts.min() >= 3 and counts.max() <= 410
     assert (counts > 100).sum() >= 10 and (counts <= 14).sum() >= 10
+
+
+def test_c3_closed_form(built):
+    """The trimmed plate (24 pockets, 12 bosses, B-spline outlines of 20-60 spans, walls
+    of 8 x 8 rational bicubic patches) against its closed form in the outlines' planar
+    distance, on uniform points and on points within 0.05 mm of the trim curves."""
+    from closed_forms import c3_near_trim_points, c3_sdf
+
+    part = Part("c3")
+    o = Oracle(part)
+    info = part.describe()
+    for pts in (uniform_points(SEED, 3000, part.lo - 2, part.hi + 2), c3_near_trim_points(info, 3000, 0.05, 7)):
+        d, _, flags = o.sdf(pts)
+        ref = c3_sdf(pts, info)
+        assert not (flags & 1).any()
+        assert np.abs(d - ref).max() < 1e-9
+        assert ((d < 0) == (ref < 0))[np.abs(ref) > 1e-9].all()
