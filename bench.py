@@ -32,6 +32,60 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def trimmed_plate_queries(args, log, threads):
+    """The trimmed plate: 2^20 points, half uniform over its box (2 mm margin) and half
+    within 0.05 mm of its planar faces' trim curves (where the mixed form takes its trim
+    decisions in double precision), inputs resident in HBM, in both precisions; the two
+    precisions' signs compared on every point and both against the FP64 oracle on a
+    sample of 2^14. Scattered points are searched in the order given (a block's
+    points may span the part), and points past their sub-patches' convexity
+    certificates are settled in double precision: this leg is far from the bake's
+    rates."""
+    import torch
+
+    from implicitcam_amd import FP64, MIXED, Part, PartGpu, near_trim_points, uniform_points
+
+    part = Part("c3")
+    g = PartGpu(part, 20.0)
+    n = 1 << 20
+    pts = np.concatenate([uniform_points(0x2545F4914F6CDD1D, n // 2, part.lo - 2, part.hi + 2), near_trim_points(part, 0x9E3779B97F4A7C15, n // 2, 0.05)])
+    dev = torch.device("cuda:0")
+    d_pts = torch.from_numpy(pts).to(dev)
+    outs = {}
+    res = {"points": n, "near_trim": n // 2, "faces": part.n_faces}
+    for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+        d_out = torch.empty(n, dtype=torch.float64, device=dev)
+        d_flag = torch.empty(n, dtype=torch.int32, device=dev)
+        for _ in range(max(args.warmup, 1)):
+            g.query_device(d_pts.data_ptr(), n, d_out.data_ptr(), d_flag.data_ptr(), None, prec, None)
+        torch.cuda.synchronize()
+        st = g.query_stats()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            g.query_device(d_pts.data_ptr(), n, d_out.data_ptr(), d_flag.data_ptr(), None, prec, None)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        outs[name] = (d_out.cpu().numpy(), d_flag.cpu().numpy())
+        res[name] = {"queries_per_s": n / dt, "ms": dt * 1e3, "searched_again_fp64": st.refined, "near_trim_curve": st.unsure_trim,
+                     "near_tie": st.unsure_tie, "settled": st.settled, "undecided": int((outs[name][1] != 0).sum())}
+        log(f"c3 queries {name}: {n / dt / 1e6:.1f} M/s, {res[name]}")
+    dm, dd = outs["mixed"][0], outs["fp64"][0]
+    res["mixed_vs_fp64"] = {"sign_mismatch": int(((dm < 0) != (dd < 0)).sum()), "max_rel_err": float(np.abs(dm - dd).max() / part.diagonal),
+                            "p99_rel_err": float(np.percentile(np.abs(dm - dd), 99) / part.diagonal)}
+    res["mixed_over_fp64"] = res["mixed"]["queries_per_s"] / res["fp64"]["queries_per_s"]
+    if not args.no_check:
+        from oracle import Oracle
+
+        idx = np.concatenate([np.arange(0, n // 2, max((n // 2) >> 13, 1)), np.arange(n // 2, n, max((n // 2) >> 13, 1))])
+        dref, _, _ = Oracle(part).sdf(pts[idx], threads=threads)
+        for name in ("mixed", "fp64"):
+            err = np.abs(outs[name][0][idx] - dref) / part.diagonal
+            res[name]["vs_oracle"] = {"n": int(len(idx)), "max_rel_err": float(err.max()), "p99_rel_err": float(np.percentile(err, 99)),
+                                      "p50_rel_err": float(np.percentile(err, 50)), "sign_mismatch": int(((outs[name][0][idx] < 0) != (dref < 0)).sum())}
+    log(f"c3 check: {res['mixed_vs_fp64']} {res.get('mixed', {}).get('vs_oracle')}")
+    return res
+
+
 def batched_workload(args, log):
     """64 parts (face counts log-uniform 6..400; turned parts, NURBS blocks with 4 x 4 to
     32 x 32 nets, plates with holes) at grids of 32^3 to 256^3 nodes, baked as one batch
@@ -61,8 +115,10 @@ def batched_workload(args, log):
         b.sync()
         ms = (time.perf_counter() - t) / args.steps * 1e3
         band = sum(b.stats(i).band for i in range(64))
-        out[name] = {"ms": ms, "band_bricks": band, "bricks_per_s": band / (ms / 1e3), "tuning": b.tuning, "open_and_tune_s": tune_s}
-        log(f"c4 batch {name}: {ms:.3f} ms, {band} band bricks, {band / (ms / 1e3) / 1e6:.2f} M bricks/s; {b.tuning}")
+        stages = b.stage_times()
+        out[name] = {"ms": ms, "band_bricks": band, "bricks_per_s": band / (ms / 1e3), "tuning": b.tuning, "open_and_tune_s": tune_s,
+                     "stages_ms": stages}
+        log(f"c4 batch {name}: {ms:.3f} ms, {band} band bricks, {band / (ms / 1e3) / 1e6:.2f} M bricks/s; {b.tuning}; stages {stages}")
         b.close()
         # the same levels one after another (the single-level entry point: one bake each)
         for g, lv in zip(gpus, levels):
@@ -93,6 +149,7 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the parity check against the oracle")
     ap.add_argument("--profile", action="store_true", help="a short run for a profiler: bake steps only")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-part batched workload")
+    ap.add_argument("--c3", action="store_true", help="add 2^20 scattered queries on the trimmed plate (half at its trim curves)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the path shards by brick ranges across GPUs (see the design notes); "
@@ -171,6 +228,8 @@ def main():
         q[name] = n / dt
         log(f"c1 queries {name}: {n / dt / 1e6:.1f} M/s")
 
+    # the trimmed plate's queries, half of them at its trim curves (on request)
+    c3 = trimmed_plate_queries(args, log, len(os.sched_getaffinity(0))) if args.c3 else None
     # the batched workload: 64 parts of uneven face counts and grids in one graph
     c4 = None if args.no_batch else batched_workload(args, log)
 
@@ -237,6 +296,7 @@ def main():
                      "frac": algo_bytes / (bricks_ms / 1e3) / PEAK_HBM, "traffic": None},
         "cpu_baseline": cpu,
         "check": check,
+        "c3_queries": c3,
         "c4_batch": c4,
     }
     print(json.dumps(out))

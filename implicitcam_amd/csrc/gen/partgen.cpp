@@ -1297,6 +1297,38 @@ size_t icam_gen_describe(const icam_part_desc* p, double* out, size_t cap) {
   return o->info.size();
 }
 
+// n points within `radius` of the trim curves of the part's planar faces (a planar
+// face drawn in proportion to its trim pieces, a piece and its parameter uniformly, the
+// offset uniform in the ball), from the reference checks' xorshift64.
+void icam_gen_near_trim(const icam_part_desc* p, uint64_t seed, size_t n, double radius, double* xyz) {
+  Rng r{seed ? seed : 1};
+  std::vector<std::pair<uint32_t, uint32_t>> pieces;  // (face, piece)
+  for (uint32_t f = 0; f < p->n_faces; ++f)
+    if (p->faces[f].kind == ICAM_PLANE)
+      for (uint32_t k = 0; k < p->faces[f].n_pieces; ++k) pieces.push_back({f, k});
+  for (size_t i = 0; i < n; ++i) {
+    double* out = xyz + 3 * i;
+    if (pieces.empty()) {
+      out[0] = out[1] = out[2] = 0;
+      continue;
+    }
+    auto fp = pieces[size_t(r.next() * pieces.size()) % pieces.size()];
+    const icam_face_desc& f = p->faces[fp.first];
+    const double* c = f.pieces + 8 * fp.second;
+    const double t = r.next(), s = 1 - t;
+    const double b[4] = {s * s * s, 3 * t * s * s, 3 * t * t * s, t * t * t};
+    const double u = b[0] * c[0] + b[1] * c[2] + b[2] * c[4] + b[3] * c[6], v = b[0] * c[1] + b[1] * c[3] + b[2] * c[5] + b[3] * c[7];
+    V x{f.x[0], f.x[1], f.x[2]}, z{f.z[0], f.z[1], f.z[2]}, y = cross(z, x);
+    V q = V{f.o[0], f.o[1], f.o[2]} + x * u + y * v;
+    V d;
+    do {
+      d = V{2 * r.next() - 1, 2 * r.next() - 1, 2 * r.next() - 1};
+    } while (dot(d, d) > 1.0);
+    q = q + d * radius;
+    out[0] = q.x, out[1] = q.y, out[2] = q.z;
+  }
+}
+
 // n points uniform in [lo, hi] from the reference checks' xorshift64 (x, y, z
 // drawn in that order per point).
 void icam_gen_uniform(uint64_t seed, size_t n, const double lo[3], const double hi[3], double* xyz) {

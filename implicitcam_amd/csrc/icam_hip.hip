@@ -1752,6 +1752,7 @@ static int query_impl(icam_hip* h, int precision, const double* d_xyz, const dou
   if (precision != ICAM_FP64 && precision != ICAM_MIXED) return fail("unknown precision");
   Res r{};
   if (results(h, true, long(n), r, d_d, d_flags, d_feat, precision == ICAM_MIXED)) return 1;
+  HIP_OK(hipMemsetAsync(h->counters.p, 0, 64, s));
   Job j{};
   j.n = long(n), j.xyz = d_xyz, j.bounds = d_bounds;
   return run_search<SRC_POINTS>(h, precision == ICAM_MIXED, j, r, s, st, (long(n) + 255) / 256);
@@ -1965,6 +1966,17 @@ int icam_hip_bake(icam_hip* h, int precision, const icam_level* lv, uint32_t* ma
   }
   if (words) HIP_OK(hipMemcpyAsync(atlas, h->atlas.p, words * 4, hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int icam_hip_query_stats(icam_hip* h, icam_bake_stats* stats) {
+  if (!h || !stats) return fail("bad arguments");
+  std::lock_guard<std::mutex> g(h->mu);
+  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
+  HIP_OK(hipDeviceSynchronize());
+  if (h->counters.p) HIP_OK(hipMemcpy(cnt, h->counters.p, 48, hipMemcpyDeviceToHost));
+  *stats = icam_bake_stats{};
+  stats->refined = int64_t(cnt[1]), stats->unsure_trim = int64_t(cnt[3]), stats->unsure_tie = int64_t(cnt[4]), stats->settled = int64_t(cnt[5]);
   return 0;
 }
 

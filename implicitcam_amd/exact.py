@@ -51,6 +51,7 @@ def lib():
         L.icam_hip_bake_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
                                            ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(BakeStats), ctypes.c_void_p]
         L.icam_hip_stage_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), _dp, ctypes.c_int]
+        L.icam_hip_query_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(BakeStats)]
         L.icam_hip_batch_open.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_void_p)]
         L.icam_hip_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -130,6 +131,13 @@ class PartGpu:
     def query_device(self, d_xyz, n, d_d, d_flags, d_feat=None, precision=FP64, stream=None):
         """The same on device pointers (ints), enqueued on `stream` (a hipStream_t as int)."""
         _check(lib().icam_hip_query_device(self.h, precision, d_xyz, n, d_d, d_flags, d_feat, stream))
+
+    def query_stats(self):
+        """Statistics of the last query: points searched again in double precision (for
+        a trim decision near a curve, a near-tie) and points settled (waits)."""
+        st = BakeStats()
+        _check(lib().icam_hip_query_stats(self.h, ctypes.byref(st)))
+        return st
 
     def bake(self, level, precision=FP64):
         """The level's brick map and atlas (PartCache::build on the device), and statistics."""

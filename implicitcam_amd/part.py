@@ -72,6 +72,14 @@ def uniform_points(seed, n, lo, hi):
     return out
 
 
+def near_trim_points(part, seed, n, radius):
+    """n points within `radius` (mm) of the trim curves of the part's planar faces,
+    float64 (n, 3), from the reference checks' xorshift64."""
+    out = np.empty((n, 3), dtype=np.float64)
+    part._lib.icam_gen_near_trim(part.desc, seed, n, radius, out.ctypes.data_as(_native.c_double_p))
+    return out
+
+
 def level_for(part, h, max_bricks=None):
     """The brick level of cell side h over the part, placed as the reference's cache
     places a level (crates/icam-sweep/src/cache.rs:505-521): first node a cell below the

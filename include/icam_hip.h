@@ -99,7 +99,7 @@ int icam_hip_query(icam_hip* h, int precision, const double* xyz, size_t n, doub
  * (a hipStream_t; NULL: the default stream); returns without waiting. */
 int icam_hip_query_device(icam_hip* h, int precision, const double* d_xyz, size_t n, double* d_d, uint32_t* d_flags, uint32_t* d_feat, void* stream);
 
-/* Statistics of a bake. */
+/* Statistics of a bake (or of a query: icam_hip_query_stats). */
 typedef struct icam_bake_stats {
   int64_t bricks;      /* bricks in the level */
   int64_t band;        /* bricks whose nodes were read (the decision's) */
@@ -111,6 +111,11 @@ typedef struct icam_bake_stats {
   int64_t settled;     /* points past a convexity certificate settled by the certified
                           search (left undecided, then bridged, when it does not finish) */
 } icam_bake_stats;
+
+/* Statistics of the last query on this handle (waits for the device): the points
+ * searched again in double precision (refined, unsure_trim, unsure_tie) and the points
+ * settled by the certified search; the other fields are zero. */
+int icam_hip_query_stats(icam_hip* h, icam_bake_stats* stats);
 
 /* PartCache::build (partcache.rs:458-486) on the device: the level's brick map
  * (nb[0] nb[1] nb[2] entries, x fastest: ICAM_AIR, ICAM_SOLID or a slot) and its
