@@ -32,6 +32,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def pmc_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` from a committed counter profile (its table row's
+    scaled column), or None."""
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.split("|")]
+            if len(cells) > 5 and cells[1] == f"`{kernel}`":
+                return float(cells[5].replace(",", ""))
+    except OSError:
+        return None
+    return None
+
+
 def trimmed_plate_queries(args, log, threads):
     """The trimmed plate: 2^20 points, half uniform over its box (2 mm margin) and half
     within 0.05 mm of its planar faces' trim curves (where the mixed form takes its trim
@@ -268,9 +281,13 @@ def main():
 
     mixed, fp64 = res["mixed"], res["fp64"]
     bricks_ms = mixed["stages"].get("nodes_search32", float("nan"))
-    # HBM traffic of the dominant kernel: its algorithmic bytes are the staged words it
-    # writes (1460 B a band brick) and the tables it reads once (small); per launch
-    algo_bytes = mixed["band"] * 1460.0
+    # the dominant kernel (the mixed form's node search): its algorithmic bytes per launch
+    # are the single-precision node records it must write (winning feature, two
+    # parameters, distance, flags: 20 B for each of the band bricks' 729 nodes) and the
+    # packed tables it reads (under 2.3 MB: left out); its measured HBM traffic comes from
+    # the committed FETCH_SIZE / WRITE_SIZE profile of the same kernel and workload
+    algo_bytes = mixed["band"] * 729 * 20.0
+    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "bake_c2_256_pmc.md"), "k_search<1, float>")
     out = {
         "metric": "SDF queries/s + bricks/s, FP32+FP64-refine vs FP64 at <=1e-6 rel err, 1 GPU",
         "value": mixed["band"] / (mixed["ms"] / 1e3),
@@ -293,7 +310,8 @@ def main():
         "undecided_nodes": mixed["undecided"],
         "refined_nodes": mixed["refined"],
         "roofline": {"bound": "hbm", "achieved": algo_bytes / (bricks_ms / 1e3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                     "frac": algo_bytes / (bricks_ms / 1e3) / PEAK_HBM, "traffic": None},
+                     "frac": algo_bytes / (bricks_ms / 1e3) / PEAK_HBM, "traffic": traffic, "kernel": "k_search<1, float> (nodes_search32)",
+                     "kernel_ms": bricks_ms, "algorithmic_bytes": algo_bytes},
         "cpu_baseline": cpu,
         "check": check,
         "c3_queries": c3,

@@ -304,6 +304,25 @@ template <int K> __device__ inline int group_or(int v) {
   for (int o = 1; o < K; o <<= 1) v |= __shfl_xor(v, o);
   return v;
 }
+
+// Counters and slots shared by the whole grid, taken one atomic a wave: the active
+// lanes that ask are counted by a ballot and the lowest of them adds the count. (A
+// per-lane atomic on one address is serialized at the cache, 64 times a wave; the
+// compiler merges them only when it can prove the pointer global.)
+__device__ inline void wave_count(unsigned long long* p, bool want) {
+  const unsigned long long mask = __ballot(want);
+  if (mask && __lane_id() == __ffsll(mask) - 1) atomicAdd(p, (unsigned long long)__popcll(mask));
+}
+// For each asking lane, its own slot of counter *p (the others get 0).
+__device__ inline uint32_t wave_slot(uint32_t* p, bool want) {
+  const unsigned long long mask = __ballot(want);
+  if (!mask) return 0;
+  const int leader = __ffsll(mask) - 1;
+  uint32_t base = 0;
+  if (__lane_id() == leader) base = atomicAdd(p, uint32_t(__popcll(mask)));
+  base = uint32_t(__shfl(int(base), leader));
+  return base + uint32_t(__popcll(mask & __lanemask_lt()));
+}
 // A point whose parts under its bound number 1 .. QI can leave its rounds to the
 // dense pass (k_leaves): its answer so far and its parts, sorted by lower bound (equal
 // bounds in list order), go to a work item, and its Best comes back with code DEFERRED.
@@ -373,8 +392,10 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const 
     }
   }
   const bool scan = parts && (all || m > Q);
-  if (K == 1 && defer && !scan && m > 0 && m <= QI) {
-    uint32_t slot = atomicAdd(defer->icount, 1u);
+  const bool leave = K == 1 && defer && !scan && m > 0 && m <= QI;
+  uint32_t slot = 0;
+  if (K == 1 && defer) slot = wave_slot(defer->icount, leave);
+  if (leave) {
     if (slot < defer->icap) {
       // the lane's parts by lower bound, equal bounds in list order
       for (int q = 1; q < m; ++q) {
@@ -705,13 +726,14 @@ __device__ __forceinline__ void refine_point(const Dev& dv, const Job& j, const 
   uint32_t info = res.info[i];
   double bound;
   v3<double> p = job_point<S>(j, dv, i, bound);
-  if (info & 4u) {
+  const bool unsure = info & 4u;
+  wave_count(&res.counters[1], unsure);
+  wave_count(&res.counters[3], unsure && (info & 16u));
+  wave_count(&res.counters[4], unsure && (info & 32u));
+  if (unsure) {
     res.flag[i] = 3;
     long blk = block_of<S>(j, i);
-    if (atomicOr(&res.qmark[blk], 1u) == 0u) res.queue[atomicAdd(res.qcount, 1u)] = (res.part << 24) | uint32_t(blk);
-    atomicAdd(&res.counters[1], 1ull);
-    if (info & 16u) atomicAdd(&res.counters[3], 1ull);
-    if (info & 32u) atomicAdd(&res.counters[4], 1ull);
+    if (res.qmark[blk] == 0u && atomicOr(&res.qmark[blk], 1u) == 0u) res.queue[atomicAdd(res.qcount, 1u)] = (res.part << 24) | uint32_t(blk);
     return;
   }
   uint32_t flag = 0, feat = NONE;
@@ -788,11 +810,12 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_fallback(Dev dv, Job j, Res re
 // settle.
 template <int S>
 __device__ __forceinline__ void mark_point(const Job& j, const Res& res, long i) {
-  if (res.flag[i] != 2) return;
+  const bool undecided = res.flag[i] == 2;
+  wave_count(&res.counters[5], undecided);
+  if (!undecided) return;
   res.flag[i] = 4;
   long blk = block_of<S>(j, i);
-  if (atomicOr(&res.smark[blk], 1u) == 0u) res.squeue[atomicAdd(res.scount, 1u)] = (res.part << 24) | uint32_t(blk);
-  atomicAdd(&res.counters[5], 1ull);
+  if (res.smark[blk] == 0u && atomicOr(&res.smark[blk], 1u) == 0u) res.squeue[atomicAdd(res.scount, 1u)] = (res.part << 24) | uint32_t(blk);
 }
 
 template <int S>
@@ -1301,12 +1324,14 @@ __global__ __launch_bounds__(Shape<S>::NT) void kb_search(BatchArgs a) {
     if (w >= total) break;
     int p;
     long blk;
+    // the part and the block are the same on every lane: said so, so that the part's
+    // records and tables are read by scalar loads
     if (S == SRC_CENTRES) {
-      p = part_of<long>(a.tile_off, a.n, long(w));
-      blk = long(w) - a.tile_off[p];
+      p = __builtin_amdgcn_readfirstlane(part_of<long>(a.tile_off, a.n, long(w)));
+      blk = long(w) - long(__builtin_amdgcn_readfirstlane(uint32_t(a.tile_off[p])));
     } else {
-      p = part_of<uint32_t>(a.band_off, a.n, w);
-      blk = long(w - a.band_off[p]);
+      p = __builtin_amdgcn_readfirstlane(part_of<uint32_t>(a.band_off, a.n, w));
+      blk = long(w - __builtin_amdgcn_readfirstlane(a.band_off[p]));
     }
     const ICAM_K BPart& P = bpart(a, p);
     const Dev dv = ldk(P.dv);
@@ -1373,7 +1398,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void kb_redo(BatchArgs a) {
     const ICAM_K BPart& P0 = bpart(a, 0);
     const Res r0 = S == SRC_CENTRES ? ldk(P0.rc) : ldk(P0.rn);
     const uint32_t e = __builtin_amdgcn_readfirstlane((settle ? r0.squeue : r0.queue)[w]);
-    const ICAM_K BPart& P = bpart(a, int(e >> 24));
+    const ICAM_K BPart& P = bpart(a, __builtin_amdgcn_readfirstlane(int(e >> 24)));
     const Dev dv = ldk(P.dv);
     const Job j = S == SRC_CENTRES ? ldk(P.jc) : ldk(P.jn);
     const Res r = S == SRC_CENTRES ? ldk(P.rc) : ldk(P.rn);
@@ -1439,11 +1464,11 @@ __global__ void kb_plan(BatchArgs a) {
 __global__ __launch_bounds__(256) void kb_need_centre(BatchArgs a) {
   const uint32_t total = a.band_off[a.n];
   for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
-    const int p = part_of<uint32_t>(a.band_off, a.n, w);
+    const int p = __builtin_amdgcn_readfirstlane(part_of<uint32_t>(a.band_off, a.n, w));
     const ICAM_K BPart& P = bpart(a, p);
     const Job jc = ldk(P.jc);
     const Res rc = ldk(P.rc);
-    need_centre(jc, rc, P.list, long(w - a.band_off[p]), P.rn.flag, P.dc, P.sgn);
+    need_centre(jc, rc, P.list, long(w - __builtin_amdgcn_readfirstlane(a.band_off[p])), P.rn.flag, P.dc, P.sgn);
     __syncthreads();
   }
 }
@@ -1464,10 +1489,11 @@ __global__ __launch_bounds__(WG) void kb_finish(BatchArgs a) {
   for (;;) {
     const uint32_t w = next_item(a, BS_FINISH, &wslot);
     if (w >= total) break;
-    const int p = part_of<uint32_t>(a.band_off, a.n, w);
+    const int p = __builtin_amdgcn_readfirstlane(part_of<uint32_t>(a.band_off, a.n, w));
     const ICAM_K BPart& P = bpart(a, p);
     const Lv lv = ldk(P.lv);
-    finish_brick(lv, P.list, long(w - a.band_off[p]), P.dc, P.sgn, P.rn.val, P.rn.flag, P.staging, P.cls, P.counters, sm);
+    finish_brick(lv, P.list, long(w - __builtin_amdgcn_readfirstlane(a.band_off[p])), P.dc, P.sgn, P.rn.val, P.rn.flag, P.staging, P.cls, P.counters,
+                 sm);
     __syncthreads();
   }
 }
@@ -1491,9 +1517,9 @@ __global__ __launch_bounds__(128) void kb_store_band(BatchArgs a) {
   for (;;) {
     const uint32_t w = next_item(a, BS_STORE, &wslot);
     if (w >= total) break;
-    const int p = part_of<uint32_t>(a.band_off, a.n, w);
+    const int p = __builtin_amdgcn_readfirstlane(part_of<uint32_t>(a.band_off, a.n, w));
     const ICAM_K BPart& P = bpart(a, p);
-    store_band_brick(long(w - a.band_off[p]), P.list, P.cls, P.slot, P.staging, P.map, P.atlas);
+    store_band_brick(long(w - __builtin_amdgcn_readfirstlane(a.band_off[p])), P.list, P.cls, P.slot, P.staging, P.map, P.atlas);
   }
 }
 
