@@ -403,6 +403,45 @@ template <class T> __device__ SEval<T> peval(const ICAM_K T* pw, T s, T t) {
   e.ptt = (mk3(a[5][0], a[5][1], a[5][2]) - e.pt * (T(2) * a[2][3]) - e.p * a[5][3]) * iw;
   return e;
 }
+// The point and first derivatives alone, summed in peval's order (so equal to its p,
+// ps, pt bit for bit); the second derivatives are left unset.
+template <class T> __device__ SEval<T> peval1(const ICAM_K T* pw, T s, T t) {
+  T bu[4], du[4], eu[4], bv[4], dv[4], ev[4];
+  bern3(s, bu, du, eu);
+  bern3(t, bv, dv, ev);
+  T a[3][4];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[k][c] = T(0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    T r0[4], r1[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) r0[c] = r1[c] = T(0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const ICAM_K T* h = pw + 16 * i + 4 * j;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        r0[c] += bv[j] * h[c];
+        r1[c] += dv[j] * h[c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      a[0][c] += bu[i] * r0[c];
+      a[1][c] += du[i] * r0[c];
+      a[2][c] += bu[i] * r1[c];
+    }
+  }
+  T iw = T(1) / a[0][3];
+  SEval<T> e;
+  e.p = mk3(a[0][0], a[0][1], a[0][2]) * iw;
+  e.ps = (mk3(a[1][0], a[1][1], a[1][2]) - e.p * a[1][3]) * iw;
+  e.pt = (mk3(a[2][0], a[2][1], a[2][2]) - e.p * a[2][3]) * iw;
+  return e;
+}
 
 // Projected Newton on |S - p|^2 over the rectangle [s0, s1] x [t0, t1] (after
 // surface3.rs:401-452, with the active set decided afresh each step): a parameter on
@@ -777,21 +816,40 @@ struct SettleItem {
 
 // out (om x on Bernstein coefficients) += s * a (am x an) * b (bm x bn), om = am + bm,
 // on = an + bn: the product of tensor Bernstein polynomials (surface3.rs:593-630).
-__device__ inline double binom(int n, int k) {
+// C(n, k) for n <= 9: the products below run in scaled Bernstein form (a coefficient
+// times its binomials), where a product of two polynomials is a plain convolution.
+__device__ constexpr double binom(int n, int k) {
   double r = 1.0;
   for (int i = 1; i <= k; ++i) r = r * double(n - k + i) / double(i);
   return r;
 }
-__device__ inline void bern_mul_add(const double* a, int am, int an, const double* b, int bm, int bn, double s, double* out) {
-  const int om = am + bm, on = an + bn;
-  for (int i = 0; i <= am; ++i)
-    for (int j = 0; j <= an; ++j) {
-      const double x = s * a[i * (an + 1) + j] * binom(am, i) * binom(an, j);
-      if (x == 0.0) continue;
-      for (int k = 0; k <= bm; ++k)
-        for (int l = 0; l <= bn; ++l)
-          out[(i + k) * (on + 1) + j + l] += x * b[k * (bn + 1) + l] * binom(bm, k) * binom(bn, l) / (binom(om, i + k) * binom(on, j + l));
+// out (degrees AM + BM, AN + BN) += s a b, every polynomial in scaled form.
+template <int AM, int AN, int BM, int BN>
+__device__ inline void bern_mul_add(const double* a, const double* b, double s, double* out) {
+  constexpr int ON = AN + BN;
+#pragma unroll
+  for (int i = 0; i <= AM; ++i)
+#pragma unroll
+    for (int j = 0; j <= AN; ++j) {
+      const double x = s * a[i * (AN + 1) + j];
+#pragma unroll
+      for (int k = 0; k <= BM; ++k)
+#pragma unroll
+        for (int l = 0; l <= BN; ++l) out[(i + k) * (ON + 1) + j + l] += x * b[k * (BN + 1) + l];
     }
+}
+// Scaled form and back, for an (M, N) net.
+template <int M, int N> __device__ inline void bern_scale(double* x) {
+#pragma unroll
+  for (int i = 0; i <= M; ++i)
+#pragma unroll
+    for (int j = 0; j <= N; ++j) x[i * (N + 1) + j] *= binom(M, i) * binom(N, j);
+}
+template <int M, int N> __device__ inline void bern_unscale(double* x) {
+#pragma unroll
+  for (int i = 0; i <= M; ++i)
+#pragma unroll
+    for (int j = 0; j <= N; ++j) x[i * (N + 1) + j] *= 1.0 / (binom(M, i) * binom(N, j));
 }
 
 // surface3.rs:694-731 no_critical_point: the squared distance from p has no critical
@@ -819,17 +877,6 @@ __device__ inline bool no_critical64(const double n[4][4][4], v3<double> p, doub
       wv[i * 3 + j] = 3.0 * (w[i * 4 + j + 1] - w[i * 4 + j]);
       for (int k = 0; k < 3; ++k) qv[k][i * 3 + j] = 3.0 * (q[k][i * 4 + j + 1] - q[k][i * 4 + j]);
     }
-  double qq[49] = {0}, qqu[42] = {0}, qqv[42] = {0};
-  for (int k = 0; k < 3; ++k) {
-    bern_mul_add(q[k], 3, 3, q[k], 3, 3, 1.0, qq);
-    bern_mul_add(q[k], 3, 3, qu[k], 2, 3, 1.0, qqu);
-    bern_mul_add(q[k], 3, 3, qv[k], 3, 2, 1.0, qqv);
-  }
-  double gs[90] = {0}, gt[90] = {0};  // (8, 9) and (9, 8)
-  bern_mul_add(w, 3, 3, qqu, 5, 6, 1.0, gs);
-  bern_mul_add(wu, 2, 3, qq, 6, 6, -1.0, gs);
-  bern_mul_add(w, 3, 3, qqv, 6, 5, 1.0, gt);
-  bern_mul_add(wv, 3, 2, qq, 6, 6, -1.0, gt);
   auto maxabs = [](const double* x, int c) {
     double m = 0.0;
     for (int i = 0; i < c; ++i) m = fmax(m, fabs(x[i]));
@@ -838,6 +885,26 @@ __device__ inline bool no_critical64(const double n[4][4][4], v3<double> p, doub
   double q0 = 0.0, q1 = 0.0, q2 = 0.0;
   for (int k = 0; k < 3; ++k) q0 = fmax(q0, maxabs(q[k], 16)), q1 = fmax(q1, maxabs(qu[k], 12)), q2 = fmax(q2, maxabs(qv[k], 12));
   const double size = maxabs(w, 16) * 3.0 * q0 * (fabs(a) * q1 + fabs(b) * q2) + (fabs(a) * maxabs(wu, 12) + fabs(b) * maxabs(wv, 12)) * 3.0 * q0 * q0;
+  // the products in scaled form, then gs (8, 9) and gt (9, 8) back in Bernstein form
+  bern_scale<3, 3>(w);
+  bern_scale<2, 3>(wu);
+  bern_scale<3, 2>(wv);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) bern_scale<3, 3>(q[k]), bern_scale<2, 3>(qu[k]), bern_scale<3, 2>(qv[k]);
+  double qq[49] = {0}, qqu[42] = {0}, qqv[42] = {0};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    bern_mul_add<3, 3, 3, 3>(q[k], q[k], 1.0, qq);
+    bern_mul_add<3, 3, 2, 3>(q[k], qu[k], 1.0, qqu);
+    bern_mul_add<3, 3, 3, 2>(q[k], qv[k], 1.0, qqv);
+  }
+  double gs[90] = {0}, gt[90] = {0};
+  bern_mul_add<3, 3, 5, 6>(w, qqu, 1.0, gs);
+  bern_mul_add<2, 3, 6, 6>(wu, qq, -1.0, gs);
+  bern_mul_add<3, 3, 6, 5>(w, qqv, 1.0, gt);
+  bern_mul_add<3, 2, 6, 6>(wv, qq, -1.0, gt);
+  bern_unscale<8, 9>(gs);
+  bern_unscale<9, 8>(gt);
   // f = a (gs raised along s) + b (gt raised along t), 10 x 10 coefficients
   for (int i = 0; i <= 9; ++i)
     for (int j = 0; j <= 9; ++j) {
@@ -951,7 +1018,7 @@ __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box,
   T s = T(0.5) * (s0 + s1), t = T(0.5) * (t0 + t1);
   const bool single = sizeof(T) == 4;
   newton2(sp, p, s, t, s0, s1, t0, t1, single ? 8 : 32, single ? T(1e-5) : T(1e-14));
-  SEval<T> e = peval(sp, s, t);
+  SEval<T> e = peval1(sp, s, t);
   v3<T> r = e.p - p;
   T d = len(r);
   if (!certified) b.lmin = mmin(b.lmin, lb);
@@ -1122,9 +1189,13 @@ __device__ inline void refine64(const Tab<double>& tb, uint32_t code, double pa,
     const ICAM_K double* lf = tb.F + L.leaf_f + LEAF_F * id;
     const ICAM_K uint32_t* lu = tb.U + L.leaf_u + LEAF_U * id;
     const ICAM_K double* sp = tb.F + L.span_f + SPAN_F * lu[0];
-    double s = pa, t = pb;
-    newton2(sp, p, s, t, lf[1], lf[2], lf[3], lf[4], 4, 1e-15);
-    SEval<double> e = peval(sp, s, t);
+    // the single-precision winner's parameters are within rounding of the foot, where
+    // the distance is stationary: one Newton step (more only if it moved far) leaves a
+    // distance error of the order of the step squared; first put them inside the
+    // sub-patch's double-precision bounds (a foot on its edge then lies on it exactly)
+    double s = mclamp(pa, lf[1], lf[2]), t = mclamp(pb, lf[3], lf[4]);
+    newton2(sp, p, s, t, lf[1], lf[2], lf[3], lf[4], 4, 1e-6);
+    SEval<double> e = peval1(sp, s, t);
     v3<double> r = e.p - p;
     d = len(r);
     const ICAM_K double* f = tb.F + L.face_f + FACE_F * lu[1];
@@ -1134,7 +1205,7 @@ __device__ inline void refine64(const Tab<double>& tb, uint32_t code, double pa,
     const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * id;
     const ICAM_K double* pw = tb.F + L.cpiece_f + CPIECE_F * cu[0];
     const ICAM_K uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cu[0];
-    double s = newton1(pw, p, pa, cf[1], cf[2], 4, 1e-16);
+    double s = newton1(pw, p, mclamp(pa, cf[1], cf[2]), cf[1], cf[2], 4, 1e-7);
     CEval<double> e = ceval(pw, s);
     d = len(e.p - p);
     sg = sgn_of(dot(p - e.p, edge_pn(tb, pu[0], e.p, pu[1], s)));
