@@ -6,8 +6,11 @@ a rational bicubic NURBS with a 16 x 16 control net (seeded value-noise heights,
 bricks (32^3 bricks) within the narrow band: the decision of every brick, the signs
 of the bricks kept whole, the 729 nodes of every band brick, flood fill, i16
 quantization and packing into the 365-word atlas, all on the device, in the
-FP32 + FP64-refine form. The plain FP64 form is timed beside it, and so are 2^20
-scattered SDF queries on the analytic box-with-hole part.
+FP32 + FP64-refine form, run by the batched evaluator (the bake's launches captured
+in one hipGraph, persistent queue stages with autotuned grids; here one part). The
+plain FP64 form is timed beside it, so is the single-level entry point in both forms,
+and so are 2^20 scattered SDF queries on the analytic box-with-hole part and the
+64-part batched workload.
 
 Prints one JSON line (rank 0, N = 1). `value` is band bricks baked per second.
 """
@@ -40,6 +43,20 @@ def pmc_traffic(path, kernel):
             cells = [c.strip() for c in line.split("|")]
             if len(cells) > 5 and cells[1] == f"`{kernel}`":
                 return float(cells[5].replace(",", ""))
+    except OSError:
+        return None
+    return None
+
+
+def pmc_valu(path, kernel):
+    """FP32 and FP64 VALU rates (TFLOP/s) and lane use of `kernel` from a committed
+    counter profile, or None."""
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.split("|")]
+            if len(cells) > 8 and cells[1] == f"`{kernel}`":
+                return {"fp32_tflops": float(cells[4]), "fp64_tflops": float(cells[6]), "lane_use": float(cells[7]),
+                        "frac_fp32_peak": float(cells[4]) * 1e12 / PEAK_FP32}
     except OSError:
         return None
     return None
@@ -171,7 +188,7 @@ def main():
 
     import torch
 
-    from implicitcam_amd import FP64, MIXED, Part, PartGpu, level_for, uniform_points
+    from implicitcam_amd import FP64, MIXED, Batch, Part, PartGpu, level_for, uniform_points
 
     if not torch.cuda.is_available():
         raise SystemExit("no GPU: the benchmark runs the HIP kernels on an MI355X")
@@ -190,12 +207,35 @@ def main():
     def bake_step(prec):
         gpu.bake_device(lv, prec)
 
-    res = {}
+    # the level through the batched evaluator (one part: the captured graph of the
+    # bake's launches, persistent queue stages, autotuned grids) and through the
+    # single-level entry point (the same kernels' single-part forms, host-side sizing)
+    res, single, batches = {}, {}, {}
     for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+        t0 = time.time()
+        b = Batch([gpu], [lv], prec, tune=True)
+        open_s = time.time() - t0
+        for _ in range(max(args.warmup, 1)):
+            b.run()
+        b.sync()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            b.run()
+        b.sync()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        st = b.stats(0)
+        stages = b.stage_times()
+        res[name] = dict(ms=dt * 1e3, band=st.band, slots=st.slots, bricks=st.bricks, undecided=st.undecided, refined=st.refined,
+                         unsure_trim=st.unsure_trim, unsure_tie=st.unsure_tie, settled=st.settled, stages=stages, tuning=b.tuning,
+                         open_and_tune_s=open_s)
+        batches[name] = b
+        log(f"bake {name} (batched evaluator, one part): {dt * 1e3:.3f} ms/step, band {st.band}, slots {st.slots}, undecided "
+            f"{st.undecided}, searched again in fp64 {st.refined} (trim {st.unsure_trim}, ties {st.unsure_tie}), settled {st.settled}; "
+            f"{b.tuning}; stages {stages}")
         for _ in range(args.warmup):
             bake_step(prec)
-        torch.cuda.synchronize()
-        _, _, st = gpu.bake_device(lv, prec, stats=True)
         torch.cuda.synchronize()
         t = time.perf_counter()
         stage_sum = {}
@@ -205,10 +245,8 @@ def main():
                 stage_sum[k] = stage_sum.get(k, 0.0) + v
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / args.steps
-        res[name] = dict(ms=dt * 1e3, band=st.band, slots=st.slots, bricks=st.bricks, undecided=st.undecided, refined=st.refined,
-                         unsure_trim=st.unsure_trim, unsure_tie=st.unsure_tie, stages={k: v / args.steps for k, v in stage_sum.items()})
-        log(f"bake {name}: {dt * 1e3:.3f} ms/step, band {st.band}, slots {st.slots}, undecided {st.undecided}, searched again in fp64 "
-            f"{st.refined} (trim {st.unsure_trim}, ties {st.unsure_tie}), stages {res[name]['stages']}")
+        single[name] = dict(ms=dt * 1e3, bricks_per_s=st.band / dt, stages={k: v / args.steps for k, v in stage_sum.items()})
+        log(f"bake {name} (single-level entry point): {dt * 1e3:.3f} ms/step, stages {single[name]['stages']}")
     if args.profile:
         # a plain copy of known size (256 MiB read, 256 MiB written): the scale for the
         # profiler's HBM byte counters
@@ -261,13 +299,13 @@ def main():
         log(f"cpu oracle bake: {cpu_s:.1f}s, {nread / cpu_s:.1f} bricks/s on {threads} threads")
         if not args.no_check:
             for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
-                m, a, st = gpu.bake(lv, prec)
-                qa = a.view(np.int16).astype(np.int64)
-                qr = a_ref.view(np.int16).astype(np.int64)
-                same_len = len(qa) == len(qr)
-                check[name] = {"map_equal": bool((m == m_ref).all()), "atlas_equal": bool(same_len and (qa == qr).all()),
-                               "max_dq": int(np.abs(qa - qr).max()) if same_len and len(qa) else None,
-                               "sign_mismatch": int(((qa > 0) != (qr > 0)).sum()) if same_len else None}
+                for how, (m, a, st) in (("", batches[name].download(0)), ("single_", gpu.bake(lv, prec))):
+                    qa = a.view(np.int16).astype(np.int64)
+                    qr = a_ref.view(np.int16).astype(np.int64)
+                    same_len = len(qa) == len(qr)
+                    check[how + name] = {"map_equal": bool((m == m_ref).all()), "atlas_equal": bool(same_len and (qa == qr).all()),
+                                         "max_dq": int(np.abs(qa - qr).max()) if same_len and len(qa) else None,
+                                         "sign_mismatch": int(((qa > 0) != (qr > 0)).sum()) if same_len else None}
             # query error against the oracle on a sample of the 2^20 points
             o1 = Oracle(c1)
             sample = pts[:: 64]
@@ -279,6 +317,8 @@ def main():
                                        "sign_mismatch": int(((dq < 0) != (dref < 0)).sum()), "n": int(len(sample))}
             log(f"check: {check}")
 
+    for b in batches.values():
+        b.close()
     mixed, fp64 = res["mixed"], res["fp64"]
     bricks_ms = mixed["stages"].get("nodes_search32", float("nan"))
     # the dominant kernel (the mixed form's node search): its algorithmic bytes per launch
@@ -287,7 +327,7 @@ def main():
     # packed tables it reads (under 2.3 MB: left out); its measured HBM traffic comes from
     # the committed FETCH_SIZE / WRITE_SIZE profile of the same kernel and workload
     algo_bytes = mixed["band"] * 729 * 20.0
-    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "bake_c2_256_pmc.md"), "k_search<1, float>")
+    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "bake_c2_256_pmc.md"), "kb_search<1, float>")
     out = {
         "metric": "SDF queries/s + bricks/s, FP32+FP64-refine vs FP64 at <=1e-6 rel err, 1 GPU",
         "value": mixed["band"] / (mixed["ms"] / 1e3),
@@ -301,17 +341,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": "c2 freeform NURBS-top block, 256^3 nodes at 0.5 mm, 8^3-cell bricks, narrow band, FP32+FP64-refine",
+        "config": {"workload": "c2 freeform NURBS-top block, 256^3 nodes at 0.5 mm, 8^3-cell bricks, narrow band, FP32+FP64-refine, "
+                               "batched evaluator (one part, captured graph)",
                    "bricks": mixed["bricks"], "band_bricks": mixed["band"], "stored_bricks": mixed["slots"]},
         "fp64_bricks_per_s": fp64["band"] / (fp64["ms"] / 1e3),
         "mixed_over_fp64": fp64["ms"] / mixed["ms"],
+        "single_level_entry_point": {"mixed": single["mixed"], "fp64": single["fp64"]},
+        "tuning": {"mixed": mixed["tuning"], "fp64": fp64["tuning"]},
         "c1_queries_per_s": {"mixed": q["mixed"], "fp64": q["fp64"]},
         "stages_ms": {"mixed": mixed["stages"], "fp64": fp64["stages"]},
         "undecided_nodes": mixed["undecided"],
         "refined_nodes": mixed["refined"],
         "roofline": {"bound": "hbm", "achieved": algo_bytes / (bricks_ms / 1e3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                     "frac": algo_bytes / (bricks_ms / 1e3) / PEAK_HBM, "traffic": traffic, "kernel": "k_search<1, float> (nodes_search32)",
-                     "kernel_ms": bricks_ms, "algorithmic_bytes": algo_bytes},
+                     "frac": algo_bytes / (bricks_ms / 1e3) / PEAK_HBM, "traffic": traffic, "kernel": "kb_search<1, float> (nodes_search32)",
+                     "kernel_ms": bricks_ms, "algorithmic_bytes": algo_bytes,
+                     "valu": pmc_valu(os.path.join(ROOT, "profiles", "bake_c2_256_valu.md"), "kb_search<1, float>")},
         "cpu_baseline": cpu,
         "check": check,
         "c3_queries": c3,
