@@ -30,15 +30,19 @@ enum : uint32_t { EDGE_F = 16, EDGE_U = 8 };  // U: kind 0, face0 1, face1 2, uv
 // VERT: p 0-2, pn 3-5
 enum : uint32_t { VERT_F = 8 };
 // LEAF (sub-patch of a span): rho 0, s0 1, s1 2, t0 3, t1 4, slab normal 5-7, slab lo 8, hi 9
-// (the sub-patch lies between the planes n.x = lo and n.x = hi); U: span 0, face 1, known inside 2
-enum : uint32_t { LEAF_F = 12, LEAF_U = 4 };
+// (the sub-patch lies between the planes n.x = lo and n.x = hi); from LEAF_CERT its bounds
+// for the per-point convexity test: lambda_min(G) from below, then the intervals (x lo, x hi,
+// y lo, y hi, z lo, z hi) of S_s, S_t, S_ss, S_st, S_tt in the parameters scaled by the
+// middle's tangent lengths; U: span 0, face 1, known inside 2
+enum : uint32_t { LEAF_CERT = 12, LEAF_F = 44, LEAF_U = 4 };
 // SPAN: 16 homogeneous points (i along u, 4 words each) 0-63, u0 64, u1 65, v0 66, v1 67;
 // U: face 0, trimmed 1 (0 whole surface), root cell 2
 enum : uint32_t { SPAN_F = 72, SPAN_U = 4 };
 // NURBS face: sense 0; U: first span 0, spans along u 1, along v 2, root cell 3, whole 4
 // (it sits in the FACE section too, kind 5)
-// CLEAF (leaf of a curve piece's tree): rho 0, t0 1, t1 2; U: cpiece 0, edge 1
-enum : uint32_t { CLEAF_F = 4, CLEAF_U = 2 };
+// CLEAF (leaf of a curve piece's tree): rho 0, t0 1, t1 2; from CLEAF_CERT: |C'|^2 from
+// below, then the intervals of C' and C''; U: cpiece 0, edge 1
+enum : uint32_t { CLEAF_CERT = 4, CLEAF_F = 17, CLEAF_U = 2 };
 // CPIECE: 4 homogeneous points 0-15; U: edge 0, piece number in its edge 1, pieces 2
 enum : uint32_t { CPIECE_F = 16, CPIECE_U = 4 };
 // CELL (trim cell tree): lo 0-1, hi 2-3, ref 4-5; U: inside 0, kind 1 (0 leaf, 1 crossed,

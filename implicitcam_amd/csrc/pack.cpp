@@ -392,10 +392,31 @@ void eval_patch(const Patch4& pa, double s, double t, V& p, V& ps, V& pt) {
   pt = (mk(a[2][0], a[2][1], a[2][2]) - p * a[2][3]) * (1 / w);
 }
 
+// What a leaf keeps of its bounds for the per-point convexity test (gpu.wgsl:1581-1614
+// reads the same quantities): lambda_min(G) from below, then the intervals of S_s, S_t,
+// S_ss, S_st and S_tt (x lo, x hi, y lo, y hi, z lo, z hi each), in the parameters
+// scaled by the middle's tangent lengths. Unusable bounds are left wide open.
+constexpr int PATCH_CERT = 31;
+// The same for a curve leaf: |C'|^2 from below, then the intervals of C' and C''.
+constexpr int PIECE_CERT = 13;
+void open_cert(double* out, int n) {
+  out[0] = 0.0;
+  for (int k = 1; k < n; k += 2) out[k] = -1e30, out[k + 1] = 1e30;
+}
+void put_iv(double* out, const Iv* iv) {
+  for (int k = 0; k < 3; ++k) {
+    // widened by their own rounding (the hulls and quotients are summed in double)
+    double pad = 1e-13 * (std::fabs(iv[k].a) + std::fabs(iv[k].b)) + 1e-300;
+    out[2 * k] = iv[k].a - pad, out[2 * k + 1] = iv[k].b + pad;
+  }
+}
+
 // surface3.rs:819-830, 866-926: the radius of convexity of a rational bicubic, from
 // interval bounds on its derivatives over its Bernstein hulls (relative to the
-// points' middle), with lambda_min(G) bounded two ways
-double patch_radius(const Patch4& pw) {
+// points' middle), with lambda_min(G) bounded two ways; `cert` (when not NULL)
+// receives the bounds themselves (PATCH_CERT words)
+double patch_radius(const Patch4& pw, double* cert = nullptr) {
+  if (cert) open_cert(cert, PATCH_CERT);
   const int du = 3, dv = 3;
   V c{0, 0, 0};
   for (int i = 0; i < 4; ++i)
@@ -476,14 +497,20 @@ double patch_radius(const Patch4& pw) {
   double lambda = std::max(by_entries, by_middle);
   // gpu.rs:87-98 rounding allowance on the bounds
   lambda = lambda - std::fabs(lambda) * 1e-6 - 1e-30;
+  if (cert) {
+    cert[0] = lambda > 0 ? lambda : 0.0;
+    put_iv(cert + 1, ss), put_iv(cert + 7, st), put_iv(cert + 13, sss), put_iv(cert + 19, sst), put_iv(cert + 25, stt);
+  }
   if (!(lambda > 0)) return 0.0;
   auto mag2 = [](const Iv* a) { return mag(a[0]) * mag(a[0]) + mag(a[1]) * mag(a[1]) + mag(a[2]) * mag(a[2]); };
   double hh = std::sqrt(mag2(sss) + 2 * mag2(sst) + mag2(stt)) * (1 + 1e-6);
   return hh > 0 ? 0.99 * lambda / hh : INF;  // RHO_MARGIN, surface3.rs:807
 }
 
-// surface3.rs:931-970 a cubic piece's radius |C'|^2_min / |C''|_max
-double piece_radius(const H* pw) {
+// surface3.rs:931-970 a cubic piece's radius |C'|^2_min / |C''|_max; `cert` (when not
+// NULL) receives the bounds (PIECE_CERT words)
+double piece_radius(const H* pw, double* cert = nullptr) {
+  if (cert) open_cert(cert, PIECE_CERT);
   const int n = 3;
   V c{0, 0, 0};
   for (int i = 0; i < 4; ++i) c = c + point(pw[i]);
@@ -532,6 +559,7 @@ double piece_radius(const H* pw) {
   spread = std::sqrt(spread);
   double l = std::max(mig(d1[0]) * mig(d1[0]) + mig(d1[1]) * mig(d1[1]) + mig(d1[2]) * mig(d1[2]), std::pow(std::max(len(e1) - spread, 0.0), 2));
   l = l - std::fabs(l) * 1e-6 - 1e-30;
+  if (cert) cert[0] = l > 0 ? l : 0.0, put_iv(cert + 1, d1), put_iv(cert + 7, d2);
   if (!(l > 0)) return 0.0;
   double h = std::sqrt(mag(d2[0]) * mag(d2[0]) + mag(d2[1]) * mag(d2[1]) + mag(d2[2]) * mag(d2[2])) * (1 + 1e-6);
   return h > 0 ? 0.99 * l / h : INF;
@@ -642,6 +670,9 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
           }
           uint32_t id = uint32_t(cleaf_f.size() / CLEAF_F);
           cleaf_f.insert(cleaf_f.end(), {rho, pp.t0, pp.t1, 0.0});
+          double cert[PIECE_CERT];
+          piece_radius(pp.p, cert);
+          cleaf_f.insert(cleaf_f.end(), cert, cert + PIECE_CERT);
           cleaf_u.insert(cleaf_u.end(), {cp, e});
           pbox.push_back(b);
           pcode.push_back((P_CLEAF << 28) | id);
@@ -798,6 +829,9 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
             }
             uint32_t id = uint32_t(leaf_f.size() / LEAF_F);
             leaf_f.insert(leaf_f.end(), {rho, sb.s0, sb.s1, sb.t0, sb.t1, nn.x, nn.y, nn.z, slo, shi, 0, 0});
+            double cert[PATCH_CERT + 1] = {0};
+            patch_radius(sb.p, cert);
+            leaf_f.insert(leaf_f.end(), cert, cert + PATCH_CERT + 1);
             leaf_u.insert(leaf_u.end(), {sid, fi, sb.inside == 1 ? 1u : 0u, 0});
             pbox.push_back(bb);
             pcode.push_back((P_LEAF << 28) | id);

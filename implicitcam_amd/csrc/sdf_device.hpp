@@ -995,6 +995,73 @@ __device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb
   return complete;
 }
 
+// Interval tests of |S - p|^2 over a part for one point (gpu.wgsl:1508-1614), in the
+// precision of the search: the rounding allowance is the reference kernel's in single
+// precision and a much smaller one in double (the stored bounds are sums of a few
+// products, each off by an ulp or two).
+template <class T> struct IvTol;
+template <> struct IvTol<float> {
+  static constexpr float rel = 1e-5f, mag = 2e-6f, tiny = 1e-30f, guard = 1e-4f;
+};
+template <> struct IvTol<double> {
+  static constexpr double rel = 1e-12, mag = 1e-13, tiny = 1e-300, guard = 1e-9;
+};
+// The interval of (S - p).D with S - p in [el, eh] and D's components in d (x lo, x hi,
+// y lo, y hi, z lo, z hi).
+template <class T> __device__ inline void dot_iv(const T el[3], const T eh[3], const ICAM_K T* d, T& lo, T& hi) {
+  T rl = T(0), rh = T(0), mg = T(0);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const T q0 = el[k] * d[2 * k], q1 = el[k] * d[2 * k + 1], q2 = eh[k] * d[2 * k], q3 = eh[k] * d[2 * k + 1];
+    rl += mmin(mmin(q0, q1), mmin(q2, q3));
+    rh += mmax(mmax(q0, q1), mmax(q2, q3));
+    mg += mmax(mmax(mabs(q0), mabs(q1)), mmax(mabs(q2), mabs(q3)));
+  }
+  const T m = IvTol<T>::rel * (mabs(rl) + mabs(rh)) + IvTol<T>::mag * mg + IvTol<T>::tiny;
+  lo = rl - m, hi = rh + m;
+}
+// The greatest |D|^2 over a part.
+template <class T> __device__ inline T mag2_hi(const ICAM_K T* d) {
+  T m = T(0);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) m += mmax(d[2 * k] * d[2 * k], d[2 * k + 1] * d[2 * k + 1]);
+  return m * (T(1) + IvTol<T>::rel);
+}
+// What a sub-patch (bounds cb: lambda_min(G), then S_s, S_t, S_ss, S_st, S_tt) is for p,
+// its points within the float box: 1 no interior minimum (a part of the gradient keeps
+// one sign, or a diagonal entry of the Hessian is negative throughout: its least point is
+// on its border, which its neighbours and the face's edges answer); 2 |S - p|^2 strictly
+// convex on it (lambda_min(G) + lambda_min([(S - p).S_ij]) > 0); 0 neither
+// (gpu.wgsl:1581-1600).
+template <class T> __device__ inline int leaf_kind(const ICAM_K T* cb, const ICAM_K float* box, v3<T> p) {
+  const T el[3] = {T(box[0]) - p.x, T(box[1]) - p.y, T(box[2]) - p.z}, eh[3] = {T(box[3]) - p.x, T(box[4]) - p.y, T(box[5]) - p.z};
+  T lo, hi;
+  dot_iv(el, eh, cb + 1, lo, hi);
+  if (lo > T(0) || hi < T(0)) return 1;
+  dot_iv(el, eh, cb + 7, lo, hi);
+  if (lo > T(0) || hi < T(0)) return 1;
+  T al, ah, bl, bh, cl, ch;
+  dot_iv(el, eh, cb + 13, al, ah);
+  dot_iv(el, eh, cb + 19, bl, bh);
+  dot_iv(el, eh, cb + 25, cl, ch);
+  if (mag2_hi(cb + 1) + ah < T(0) || mag2_hi(cb + 7) + ch < T(0)) return 1;
+  const T lg = cb[0], bm = mmax(mabs(bl), mabs(bh));
+  const T lm = T(0.5) * ((al + cl) - sqrt((al - cl) * (al - cl) + T(4) * bm * bm));
+  return lg > T(0) && lg + lm > IvTol<T>::guard * lg ? 2 : 0;
+}
+// The same for a curve part (bounds cb: |C'|^2 from below, then C', C''; gpu.wgsl:1602-1614).
+template <class T> __device__ inline int cleaf_kind(const ICAM_K T* cb, const ICAM_K float* box, v3<T> p) {
+  const T el[3] = {T(box[0]) - p.x, T(box[1]) - p.y, T(box[2]) - p.z}, eh[3] = {T(box[3]) - p.x, T(box[4]) - p.y, T(box[5]) - p.z};
+  T lo, hi;
+  dot_iv(el, eh, cb + 1, lo, hi);
+  if (lo > T(0) || hi < T(0)) return 1;
+  T kl, kh;
+  dot_iv(el, eh, cb + 7, kl, kh);
+  if (mag2_hi(cb + 1) + kh < T(0)) return 1;
+  const T l = cb[0];
+  return l > T(0) && l + kl > IvTol<T>::guard * l ? 2 : 0;
+}
+
 // gpu.wgsl:679-756 / solid3.rs:1904-1910: a sub-patch read by Newton when its radius
 // certifies a single minimum for p; the minimum answers when it is a critical point
 // inside the face. With `settle` (double precision), a sub-patch past its certificate
@@ -1007,6 +1074,12 @@ __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box,
   T rho = lf[0];
   T fd = far_dist(p, box);
   bool certified = fd < T(0.999) * rho - eps;
+  if (!certified) {
+    // past its radius: the interval test for this point (gpu.wgsl:1677-1683)
+    const int kind = leaf_kind(lf + LEAF_CERT, box, p);
+    if (kind == 1) return;
+    certified = kind == 2;
+  }
   if constexpr (SETTLE && sizeof(T) == 8) {
     if (!certified) {
       if (!settle_leaf(tb, li, lb, p, b)) b.lmin = mmin(b.lmin, lb);
@@ -1114,6 +1187,11 @@ __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box
   const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * ci;
   T rho = cf[0];
   bool certified = far_dist(p, box) < T(0.999) * rho - eps;
+  if (!certified) {
+    const int kind = cleaf_kind(cf + CLEAF_CERT, box, p);
+    if (kind == 1) return;
+    certified = kind == 2;
+  }
   if constexpr (SETTLE && sizeof(T) == 8) {
     if (!certified) {
       if (!settle_cleaf(tb, ci, lb, p, b)) b.lmin = mmin(b.lmin, lb);
