@@ -1561,6 +1561,10 @@ struct icam_hip {
     }
   } centres, nodes;
   Buf q_xyz, q_bound, q_d, q_flag, q_feat;
+  // scattered queries taken in cell order: the part's box (world), the cells' counts,
+  // each point's cell and rank in it, the order, and the points and results in it
+  double box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};
+  Buf o_count, o_cell, o_rank, o_idx, o_xyz, o_d, o_flag, o_feat;
   // the level
   Buf dc, band, sgn, total, list, staging, cls, slot, stotal, map, atlas, counters, changed, firsts, tile_cand, tile_count;
   std::vector<std::pair<const char*, double>> stages;
@@ -1694,6 +1698,73 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
   return 0;
 }
 
+// ------------------------------------------------------------------ point order
+//
+// Scattered queries are searched in the Morton order of a 64^3 grid of cells over the
+// part's box (points outside it fall in its border cells): a block's 256 points then
+// lie together and cull to a short candidate list, where points in the caller's order
+// may each lie anywhere. A counting sort (cells and ranks by atomics, one exclusive
+// scan, placement); the order within a cell is the atomics' and does not matter. The
+// results go back to the caller's order.
+constexpr int ORDER_BITS = 6;
+constexpr uint32_t ORDER_CELLS = 1u << (3 * ORDER_BITS);
+constexpr size_t ORDER_MIN = 8192;  // fewer points are searched as given
+
+__global__ void k_order_cells(const double* xyz, long n, double lx, double ly, double lz, double sx, double sy, double sz, uint32_t* count,
+                              uint32_t* cell, uint32_t* rank) {
+  const long i = long(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double lo[3] = {lx, ly, lz}, sc[3] = {sx, sy, sz};
+  uint32_t code = 0;
+  for (int k = 0; k < 3; ++k) {
+    const double f = (xyz[3 * i + k] - lo[k]) * sc[k];
+    const uint32_t q = f >= 0.0 ? (f < double((1 << ORDER_BITS) - 1) ? uint32_t(f) : uint32_t((1 << ORDER_BITS) - 1)) : 0u;
+    for (int b = 0; b < ORDER_BITS; ++b) code |= ((q >> b) & 1u) << (3 * b + k);
+  }
+  cell[i] = code;
+  rank[i] = atomicAdd(&count[code], 1u);
+}
+
+// Exclusive scan of the cells' counts in place, by one block.
+__global__ __launch_bounds__(1024) void k_order_scan(uint32_t* count) {
+  __shared__ uint32_t part[1024];
+  constexpr uint32_t per = ORDER_CELLS / 1024;
+  uint32_t* c = count + size_t(threadIdx.x) * per;
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < per; ++k) sum += c[k];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const uint32_t v = int(threadIdx.x) >= o ? part[threadIdx.x - o] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t base = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+  for (uint32_t k = 0; k < per; ++k) {
+    const uint32_t v = c[k];
+    c[k] = base;
+    base += v;
+  }
+}
+
+__global__ void k_order_place(const double* xyz, long n, const uint32_t* cell, const uint32_t* rank, const uint32_t* start, uint32_t* idx, double* oxyz) {
+  const long i = long(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t at = start[cell[i]] + rank[i];
+  idx[at] = uint32_t(i);
+  for (int k = 0; k < 3; ++k) oxyz[3 * size_t(at) + k] = xyz[3 * i + k];
+}
+
+__global__ void k_order_back(long n, const uint32_t* idx, const double* od, const uint32_t* of, const uint32_t* oft, double* d, uint32_t* f, uint32_t* ft) {
+  const long k = long(blockIdx.x) * 256 + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = idx[k];
+  if (d) d[i] = od[k];
+  if (f) f[i] = of[k];
+  if (ft) ft[i] = oft[k];
+}
+
 }  // namespace
 
 extern "C" {
@@ -1732,6 +1803,12 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
   h->dv.pcode = h->pcode.as<uint32_t>();
   h->dv.L = h->pk.L;
   for (int k = 0; k < 3; ++k) h->dv.origin[k] = h->pk.origin[k];
+  for (int k = 0; k < 3; ++k) h->box_lo[k] = 1e300, h->box_hi[k] = -1e300;
+  for (size_t i = 0; i + 6 <= h->pk.pbox.size(); i += 6)
+    for (int k = 0; k < 3; ++k) {
+      h->box_lo[k] = std::min(h->box_lo[k], double(h->pk.pbox[i + k]) + h->pk.origin[k]);
+      h->box_hi[k] = std::max(h->box_hi[k], double(h->pk.pbox[i + 3 + k]) + h->pk.origin[k]);
+    }
   *out = h;
   return 0;
 }
@@ -1743,7 +1820,7 @@ void icam_hip_close(icam_hip* h) {
   h->nodes.free();
   for (Buf* b : {&h->F32, &h->F64, &h->U, &h->pbox, &h->pcode, &h->q_xyz, &h->q_bound, &h->q_d, &h->q_flag, &h->q_feat, &h->dc, &h->band, &h->sgn, &h->total,
                  &h->list, &h->staging, &h->cls, &h->slot, &h->stotal, &h->map, &h->atlas, &h->counters, &h->changed, &h->firsts,
-                 &h->tile_cand, &h->tile_count})
+                 &h->tile_cand, &h->tile_count, &h->o_count, &h->o_cell, &h->o_rank, &h->o_idx, &h->o_xyz, &h->o_d, &h->o_flag, &h->o_feat})
     b->free();
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -1776,12 +1853,45 @@ static int query_impl(icam_hip* h, int precision, const double* d_xyz, const dou
                       uint32_t* d_feat, hipStream_t s, Stages* st) {
   if (!n) return 0;
   if (precision != ICAM_FP64 && precision != ICAM_MIXED) return fail("unknown precision");
+  const bool mixed = precision == ICAM_MIXED;
+  const unsigned blocks = unsigned((n + 255) / 256);
   Res r{};
-  if (results(h, true, long(n), r, d_d, d_flags, d_feat, precision == ICAM_MIXED)) return 1;
-  HIP_OK(hipMemsetAsync(h->counters.p, 0, 64, s));
   Job j{};
   j.n = long(n), j.xyz = d_xyz, j.bounds = d_bounds;
-  return run_search<SRC_POINTS>(h, precision == ICAM_MIXED, j, r, s, st, (long(n) + 255) / 256);
+  // scattered points without bounds of their own: searched in cell order
+  const bool order = n >= ORDER_MIN && !d_bounds && n < (size_t(1) << 31);
+  if (order) {
+    HIP_OK(h->o_count.need(ORDER_CELLS * 4));
+    HIP_OK(h->o_cell.need(n * 4));
+    HIP_OK(h->o_rank.need(n * 4));
+    HIP_OK(h->o_idx.need(n * 4));
+    HIP_OK(h->o_xyz.need(n * 24));
+    HIP_OK(h->o_d.need(n * 8));
+    HIP_OK(h->o_flag.need(n * 4));
+    if (d_feat) HIP_OK(h->o_feat.need(n * 4));
+    double sc[3];
+    for (int k = 0; k < 3; ++k) {
+      const double side = h->box_hi[k] - h->box_lo[k];
+      sc[k] = side > 0.0 ? double(1 << ORDER_BITS) / side : 0.0;
+    }
+    uint32_t* count = h->o_count.as<uint32_t>();
+    HIP_OK(hipMemsetAsync(count, 0, ORDER_CELLS * 4, s));
+    HIP_OK(launch(st, "order", s, dim3(blocks), dim3(256), k_order_cells, d_xyz, long(n), h->box_lo[0], h->box_lo[1], h->box_lo[2], sc[0], sc[1], sc[2],
+                  count, h->o_cell.as<uint32_t>(), h->o_rank.as<uint32_t>()));
+    HIP_OK(launch(st, "order", s, dim3(1), dim3(1024), k_order_scan, count));
+    HIP_OK(launch(st, "order", s, dim3(blocks), dim3(256), k_order_place, d_xyz, long(n), (const uint32_t*)h->o_cell.as<uint32_t>(),
+                  (const uint32_t*)h->o_rank.as<uint32_t>(), (const uint32_t*)count, h->o_idx.as<uint32_t>(), h->o_xyz.as<double>()));
+    j.xyz = h->o_xyz.as<double>();
+    if (results(h, true, long(n), r, h->o_d.as<double>(), h->o_flag.as<uint32_t>(), d_feat ? h->o_feat.as<uint32_t>() : nullptr, mixed)) return 1;
+  } else if (results(h, true, long(n), r, d_d, d_flags, d_feat, mixed)) {
+    return 1;
+  }
+  HIP_OK(hipMemsetAsync(h->counters.p, 0, 64, s));
+  if (run_search<SRC_POINTS>(h, mixed, j, r, s, st, blocks)) return 1;
+  if (order)
+    HIP_OK(launch(st, "order", s, dim3(blocks), dim3(256), k_order_back, long(n), (const uint32_t*)h->o_idx.as<uint32_t>(), (const double*)h->o_d.as<double>(),
+                  (const uint32_t*)h->o_flag.as<uint32_t>(), (const uint32_t*)(d_feat ? h->o_feat.as<uint32_t>() : nullptr), d_d, d_flags, d_feat));
+  return 0;
 }
 
 int icam_hip_query_device(icam_hip* h, int precision, const double* d_xyz, size_t n, double* d_d, uint32_t* d_flags, uint32_t* d_feat, void* stream) {

@@ -85,6 +85,28 @@ def test_c1_full_size_closed_form(c1, prec):
 
 
 @pytest.mark.parametrize("prec", PRECS)
+def test_query_order_returns_callers_order(c2, prec):
+    """Queries of 8,192 points or more are searched in the Morton order of cells over the
+    part's box and their results put back: the same points as given, permuted, and split
+    into batches below that size (searched as given) must agree point for point."""
+    part, o, g = c2
+    rng = np.random.default_rng(7)
+    pts = uniform_points(SEED, 20000, part.lo - 3, part.hi + 3)
+    d1, f1, t1 = g.query(pts, prec)
+    perm = rng.permutation(len(pts))
+    d2, f2, t2 = g.query(pts[perm], prec)
+    parts = [g.query(pts[k:k + 4000], prec) for k in range(0, len(pts), 4000)]
+    d3 = np.concatenate([x[0] for x in parts])
+    f3 = np.concatenate([x[1] for x in parts])
+    assert (f1[perm] == f2).all() and (f1 == f3).all()
+    assert np.abs(d1[perm] - d2).max() <= tol(prec, part) and np.abs(d1 - d3).max() <= tol(prec, part)
+    assert ((d1[perm] < 0) == (d2 < 0)).all() and ((d1 < 0) == (d3 < 0)).all()
+    sample = np.arange(0, len(pts), 40)
+    dref, _, _ = o.sdf(pts[sample])
+    assert np.abs(d1[sample] - dref).max() <= tol(prec, part)
+
+
+@pytest.mark.parametrize("prec", PRECS)
 def test_c1_edge_cases(c1, prec):
     """Points on a face, on an edge, at a corner, on the hole's axis, on the fillet, and
     an empty batch."""
