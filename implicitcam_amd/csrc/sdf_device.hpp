@@ -809,10 +809,211 @@ __device__ inline bool polish64(const Tab<double>& tb, uint32_t fi, v3<double> p
   return false;
 }
 
+// Interval tests of |S - p|^2 over a part for one point (gpu.wgsl:1508-1614), in the
+// precision of the search: the rounding allowance is the reference kernel's in single
+// precision and a much smaller one in double (the stored bounds are sums of a few
+// products, each off by an ulp or two).
+template <class T> struct IvTol;
+template <> struct IvTol<float> {
+  static constexpr float rel = 1e-5f, mag = 2e-6f, tiny = 1e-30f, guard = 1e-4f;
+};
+template <> struct IvTol<double> {
+  static constexpr double rel = 1e-12, mag = 1e-13, tiny = 1e-300, guard = 1e-9;
+};
+// The interval of (S - p).D with S - p in [el, eh] and D's components in d (x lo, x hi,
+// y lo, y hi, z lo, z hi).
+template <class T, class P> __device__ inline void dot_iv(const T el[3], const T eh[3], P d, T& lo, T& hi) {
+  T rl = T(0), rh = T(0), mg = T(0);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const T q0 = el[k] * d[2 * k], q1 = el[k] * d[2 * k + 1], q2 = eh[k] * d[2 * k], q3 = eh[k] * d[2 * k + 1];
+    rl += mmin(mmin(q0, q1), mmin(q2, q3));
+    rh += mmax(mmax(q0, q1), mmax(q2, q3));
+    mg += mmax(mmax(mabs(q0), mabs(q1)), mmax(mabs(q2), mabs(q3)));
+  }
+  const T m = IvTol<T>::rel * (mabs(rl) + mabs(rh)) + IvTol<T>::mag * mg + IvTol<T>::tiny;
+  lo = rl - m, hi = rh + m;
+}
+// The greatest |D|^2 over a part.
+template <class T, class P> __device__ inline T mag2_hi(P d) {
+  T m = T(0);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) m += mmax(d[2 * k] * d[2 * k], d[2 * k + 1] * d[2 * k + 1]);
+  return m * (T(1) + IvTol<T>::rel);
+}
+// What a sub-patch (bounds cb: lambda_min(G), then S_s, S_t, S_ss, S_st, S_tt) is for p,
+// its points within the float box: 1 no interior minimum (a part of the gradient keeps
+// one sign, or a diagonal entry of the Hessian is negative throughout: its least point is
+// on its border, which its neighbours and the face's edges answer); 2 |S - p|^2 strictly
+// convex on it (lambda_min(G) + lambda_min([(S - p).S_ij]) > 0); 0 neither
+// (gpu.wgsl:1581-1600).
+template <class T, class P> __device__ inline int patch_kind(P cb, const T el[3], const T eh[3]) {
+  T lo, hi;
+  dot_iv(el, eh, cb + 1, lo, hi);
+  if (lo > T(0) || hi < T(0)) return 1;
+  dot_iv(el, eh, cb + 7, lo, hi);
+  if (lo > T(0) || hi < T(0)) return 1;
+  T al, ah, bl, bh, cl, ch;
+  dot_iv(el, eh, cb + 13, al, ah);
+  dot_iv(el, eh, cb + 19, bl, bh);
+  dot_iv(el, eh, cb + 25, cl, ch);
+  if (mag2_hi<T>(cb + 1) + ah < T(0) || mag2_hi<T>(cb + 7) + ch < T(0)) return 1;
+  const T lg = cb[0], bm = mmax(mabs(bl), mabs(bh));
+  const T lm = T(0.5) * ((al + cl) - sqrt((al - cl) * (al - cl) + T(4) * bm * bm));
+  return lg > T(0) && lg + lm > IvTol<T>::guard * lg ? 2 : 0;
+}
+template <class T> __device__ inline int leaf_kind(const ICAM_K T* cb, const ICAM_K float* box, v3<T> p) {
+  const T el[3] = {T(box[0]) - p.x, T(box[1]) - p.y, T(box[2]) - p.z}, eh[3] = {T(box[3]) - p.x, T(box[4]) - p.y, T(box[5]) - p.z};
+  return patch_kind<T>(cb, el, eh);
+}
+// The same for a curve part (bounds cb: |C'|^2 from below, then C', C''; gpu.wgsl:1602-1614).
+template <class T, class P> __device__ inline int piece_kind(P cb, const T el[3], const T eh[3]) {
+  T lo, hi;
+  dot_iv(el, eh, cb + 1, lo, hi);
+  if (lo > T(0) || hi < T(0)) return 1;
+  T kl, kh;
+  dot_iv(el, eh, cb + 7, kl, kh);
+  if (mag2_hi<T>(cb + 1) + kh < T(0)) return 1;
+  const T l = cb[0];
+  return l > T(0) && l + kl > IvTol<T>::guard * l ? 2 : 0;
+}
+template <class T> __device__ inline int cleaf_kind(const ICAM_K T* cb, const ICAM_K float* box, v3<T> p) {
+  const T el[3] = {T(box[0]) - p.x, T(box[1]) - p.y, T(box[2]) - p.z}, eh[3] = {T(box[3]) - p.x, T(box[4]) - p.y, T(box[5]) - p.z};
+  return piece_kind<T>(cb, el, eh);
+}
+
 struct SettleItem {
   double a0, a1, b0, b1, lb;
   int depth;
 };
+
+// Interval arithmetic for the settle's per-part tests (the bounds the packer keeps for
+// a leaf, made here for a quarter from its own net; surface3.rs:866-970).
+struct Iv {
+  double a, b;
+};
+__device__ inline Iv iv_hull(Iv r, double x) { return Iv{fmin(r.a, x), fmax(r.b, x)}; }
+__device__ inline Iv iv_sub(Iv x, Iv y) { return Iv{x.a - y.b, x.b - y.a}; }
+__device__ inline Iv iv_mul(Iv x, Iv y) {
+  const double p0 = x.a * y.a, p1 = x.a * y.b, p2 = x.b * y.a, p3 = x.b * y.b;
+  return Iv{fmin(fmin(p0, p1), fmin(p2, p3)), fmax(fmax(p0, p1), fmax(p2, p3))};
+}
+__device__ inline Iv iv_scale(Iv x, double c) { return c >= 0.0 ? Iv{x.a * c, x.b * c} : Iv{x.b * c, x.a * c}; }
+__device__ inline Iv iv_divpos(Iv x, Iv w) {  // w.a > 0
+  const double p0 = x.a / w.a, p1 = x.a / w.b, p2 = x.b / w.a, p3 = x.b / w.b;
+  return Iv{fmin(fmin(p0, p1), fmin(p2, p3)), fmax(fmax(p0, p1), fmax(p2, p3))};
+}
+__device__ inline double iv_mig(Iv x) { return x.a > 0.0 ? x.a : (x.b < 0.0 ? -x.b : 0.0); }
+__device__ inline double iv_mag(Iv x) { return fmax(fabs(x.a), fabs(x.b)); }
+// Bounds (PATCH_CERT layout: lambda_min(G), S_s, S_t, S_ss, S_st, S_tt) of a homogeneous
+// bicubic net, and the box of its points (relative to p: el, eh); false when a weight
+// is not positive.
+__device__ inline bool net_bounds(const double n[4][4][4], v3<double> p, double cb[31], double el[3], double eh[3]) {
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, c[3] = {0.0, 0.0, 0.0};
+  Iv wt{1e300, -1e300};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const double w = n[i][j][3];
+      if (!(w > 0.0)) return false;
+      wt = iv_hull(wt, w);
+      for (int k = 0; k < 3; ++k) {
+        const double x = n[i][j][k] / w;
+        lo[k] = fmin(lo[k], x), hi[k] = fmax(hi[k], x), c[k] += x * (1.0 / 16.0);
+      }
+    }
+  const double pc[3] = {p.x, p.y, p.z};
+  for (int k = 0; k < 3; ++k) el[k] = lo[k] - pc[k], eh[k] = hi[k] - pc[k];
+  // derivative hulls of the coordinates relative to the points' mean
+  Iv A[6][4];
+  for (int w = 0; w < 6; ++w)
+    for (int k = 0; k < 4; ++k) A[w][k] = Iv{1e300, -1e300};
+  auto q = [&](int i, int j, int k) { return k < 3 ? n[i][j][k] - c[k] * n[i][j][3] : n[i][j][3]; };
+  for (int k = 0; k < 4; ++k)
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        if (i < 3) A[1][k] = iv_hull(A[1][k], 3.0 * (q(i + 1, j, k) - q(i, j, k)));
+        if (j < 3) A[2][k] = iv_hull(A[2][k], 3.0 * (q(i, j + 1, k) - q(i, j, k)));
+        if (i < 2) A[3][k] = iv_hull(A[3][k], 6.0 * (q(i + 2, j, k) - 2.0 * q(i + 1, j, k) + q(i, j, k)));
+        if (i < 3 && j < 3) A[4][k] = iv_hull(A[4][k], 9.0 * (q(i + 1, j + 1, k) - q(i + 1, j, k) - q(i, j + 1, k) + q(i, j, k)));
+        if (j < 2) A[5][k] = iv_hull(A[5][k], 6.0 * (q(i, j + 2, k) - 2.0 * q(i, j + 1, k) + q(i, j, k)));
+      }
+  Iv sv[3], ss[3], st[3], sss[3], sst[3], stt[3];
+  for (int k = 0; k < 3; ++k) {
+    sv[k] = Iv{lo[k] - c[k], hi[k] - c[k]};
+    ss[k] = iv_divpos(iv_sub(A[1][k], iv_mul(sv[k], A[1][3])), wt);
+    st[k] = iv_divpos(iv_sub(A[2][k], iv_mul(sv[k], A[2][3])), wt);
+  }
+  for (int k = 0; k < 3; ++k) {
+    sss[k] = iv_divpos(iv_sub(iv_sub(A[3][k], iv_scale(iv_mul(ss[k], A[1][3]), 2.0)), iv_mul(sv[k], A[3][3])), wt);
+    sst[k] = iv_divpos(iv_sub(iv_sub(iv_sub(A[4][k], iv_mul(ss[k], A[2][3])), iv_mul(st[k], A[1][3])), iv_mul(sv[k], A[4][3])), wt);
+    stt[k] = iv_divpos(iv_sub(iv_sub(A[5][k], iv_scale(iv_mul(st[k], A[2][3]), 2.0)), iv_mul(sv[k], A[5][3])), wt);
+  }
+  // parameters scaled by the tangents' middle lengths (a congruence: definiteness is
+  // unchanged, the entrywise bound below is better conditioned)
+  double sa = 0.0, sb = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const double ma = 0.5 * (ss[k].a + ss[k].b), mb = 0.5 * (st[k].a + st[k].b);
+    sa += ma * ma, sb += mb * mb;
+  }
+  sa = sa > 0.0 ? 1.0 / sqrt(sa) : 1.0, sb = sb > 0.0 ? 1.0 / sqrt(sb) : 1.0;
+  double ga = 0.0, gc = 0.0;
+  Iv gb{0.0, 0.0};
+  for (int k = 0; k < 3; ++k) {
+    ss[k] = iv_scale(ss[k], sa), st[k] = iv_scale(st[k], sb);
+    sss[k] = iv_scale(sss[k], sa * sa), sst[k] = iv_scale(sst[k], sa * sb), stt[k] = iv_scale(stt[k], sb * sb);
+    ga += iv_mig(ss[k]) * iv_mig(ss[k]), gc += iv_mig(st[k]) * iv_mig(st[k]);
+    const Iv m = iv_mul(ss[k], st[k]);
+    gb.a += m.a, gb.b += m.b;
+  }
+  const double gbm = iv_mag(gb);
+  const double lg = 0.5 * ((ga + gc) - sqrt((ga - gc) * (ga - gc) + 4.0 * gbm * gbm));
+  cb[0] = lg - fabs(lg) * 1e-9 - 1e-300;
+  const Iv* vs[5] = {ss, st, sss, sst, stt};
+  for (int v = 0; v < 5; ++v)
+    for (int k = 0; k < 3; ++k) {
+      const double pad = 1e-13 * (fabs(vs[v][k].a) + fabs(vs[v][k].b)) + 1e-300;
+      cb[1 + 6 * v + 2 * k] = vs[v][k].a - pad, cb[2 + 6 * v + 2 * k] = vs[v][k].b + pad;
+    }
+  return true;
+}
+// The same for a homogeneous cubic's net (PIECE_CERT layout: |C'|^2, C', C'').
+__device__ inline bool cnet_bounds(const double n[4][4], v3<double> p, double cb[13], double el[3], double eh[3]) {
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, c[3] = {0.0, 0.0, 0.0};
+  Iv wt{1e300, -1e300};
+  for (int i = 0; i < 4; ++i) {
+    const double w = n[i][3];
+    if (!(w > 0.0)) return false;
+    wt = iv_hull(wt, w);
+    for (int k = 0; k < 3; ++k) {
+      const double x = n[i][k] / w;
+      lo[k] = fmin(lo[k], x), hi[k] = fmax(hi[k], x), c[k] += 0.25 * x;
+    }
+  }
+  const double pc[3] = {p.x, p.y, p.z};
+  for (int k = 0; k < 3; ++k) el[k] = lo[k] - pc[k], eh[k] = hi[k] - pc[k];
+  auto q = [&](int i, int k) { return k < 3 ? n[i][k] - c[k] * n[i][3] : n[i][3]; };
+  Iv a1[4], a2[4];
+  for (int k = 0; k < 4; ++k) {
+    a1[k] = Iv{1e300, -1e300}, a2[k] = Iv{1e300, -1e300};
+    for (int i = 0; i < 3; ++i) a1[k] = iv_hull(a1[k], 3.0 * (q(i + 1, k) - q(i, k)));
+    for (int i = 0; i < 2; ++i) a2[k] = iv_hull(a2[k], 6.0 * (q(i + 2, k) - 2.0 * q(i + 1, k) + q(i, k)));
+  }
+  Iv d1[3], d2[3];
+  double l = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const Iv sv{lo[k] - c[k], hi[k] - c[k]};
+    d1[k] = iv_divpos(iv_sub(a1[k], iv_mul(sv, a1[3])), wt);
+    d2[k] = iv_divpos(iv_sub(iv_sub(a2[k], iv_scale(iv_mul(d1[k], a1[3]), 2.0)), iv_mul(sv, a2[3])), wt);
+    l += iv_mig(d1[k]) * iv_mig(d1[k]);
+  }
+  cb[0] = l - fabs(l) * 1e-9 - 1e-300;
+  for (int k = 0; k < 3; ++k) {
+    const double p1 = 1e-13 * (fabs(d1[k].a) + fabs(d1[k].b)) + 1e-300, p2 = 1e-13 * (fabs(d2[k].a) + fabs(d2[k].b)) + 1e-300;
+    cb[1 + 2 * k] = d1[k].a - p1, cb[2 + 2 * k] = d1[k].b + p1;
+    cb[7 + 2 * k] = d2[k].a - p2, cb[8 + 2 * k] = d2[k].b + p2;
+  }
+  return true;
+}
 
 // out (om x on Bernstein coefficients) += s * a (am x an) * b (bm x bn), om = am + bm,
 // on = an + bn: the product of tensor Bernstein polynomials (surface3.rs:593-630).
@@ -969,6 +1170,12 @@ __device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb
     double box, slab;
     hull_bounds(&net[0][0], 16, p, dir, box, slab);
     if (slab >= bd - SETTLE_EPS || it.depth >= SETTLE_DEPTH) continue;
+    {
+      // the quarter's own interval test (gpu.wgsl:1581-1600 on its net): no interior
+      // minimum drops it; strictly convex for p, its minimum is the Newton point above
+      double cb[31], el[3], eh[3];
+      if (net_bounds(net, p, cb, el, eh) && patch_kind<double>((const double*)cb, el, eh) != 0) continue;
+    }
     const double am = 0.5 * (it.a0 + it.a1), bm = 0.5 * (it.b0 + it.b1);
     {
       // no critical point on the quarter: no minimum of its own to find there
@@ -993,73 +1200,6 @@ __device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb
     }
   }
   return complete;
-}
-
-// Interval tests of |S - p|^2 over a part for one point (gpu.wgsl:1508-1614), in the
-// precision of the search: the rounding allowance is the reference kernel's in single
-// precision and a much smaller one in double (the stored bounds are sums of a few
-// products, each off by an ulp or two).
-template <class T> struct IvTol;
-template <> struct IvTol<float> {
-  static constexpr float rel = 1e-5f, mag = 2e-6f, tiny = 1e-30f, guard = 1e-4f;
-};
-template <> struct IvTol<double> {
-  static constexpr double rel = 1e-12, mag = 1e-13, tiny = 1e-300, guard = 1e-9;
-};
-// The interval of (S - p).D with S - p in [el, eh] and D's components in d (x lo, x hi,
-// y lo, y hi, z lo, z hi).
-template <class T> __device__ inline void dot_iv(const T el[3], const T eh[3], const ICAM_K T* d, T& lo, T& hi) {
-  T rl = T(0), rh = T(0), mg = T(0);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const T q0 = el[k] * d[2 * k], q1 = el[k] * d[2 * k + 1], q2 = eh[k] * d[2 * k], q3 = eh[k] * d[2 * k + 1];
-    rl += mmin(mmin(q0, q1), mmin(q2, q3));
-    rh += mmax(mmax(q0, q1), mmax(q2, q3));
-    mg += mmax(mmax(mabs(q0), mabs(q1)), mmax(mabs(q2), mabs(q3)));
-  }
-  const T m = IvTol<T>::rel * (mabs(rl) + mabs(rh)) + IvTol<T>::mag * mg + IvTol<T>::tiny;
-  lo = rl - m, hi = rh + m;
-}
-// The greatest |D|^2 over a part.
-template <class T> __device__ inline T mag2_hi(const ICAM_K T* d) {
-  T m = T(0);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) m += mmax(d[2 * k] * d[2 * k], d[2 * k + 1] * d[2 * k + 1]);
-  return m * (T(1) + IvTol<T>::rel);
-}
-// What a sub-patch (bounds cb: lambda_min(G), then S_s, S_t, S_ss, S_st, S_tt) is for p,
-// its points within the float box: 1 no interior minimum (a part of the gradient keeps
-// one sign, or a diagonal entry of the Hessian is negative throughout: its least point is
-// on its border, which its neighbours and the face's edges answer); 2 |S - p|^2 strictly
-// convex on it (lambda_min(G) + lambda_min([(S - p).S_ij]) > 0); 0 neither
-// (gpu.wgsl:1581-1600).
-template <class T> __device__ inline int leaf_kind(const ICAM_K T* cb, const ICAM_K float* box, v3<T> p) {
-  const T el[3] = {T(box[0]) - p.x, T(box[1]) - p.y, T(box[2]) - p.z}, eh[3] = {T(box[3]) - p.x, T(box[4]) - p.y, T(box[5]) - p.z};
-  T lo, hi;
-  dot_iv(el, eh, cb + 1, lo, hi);
-  if (lo > T(0) || hi < T(0)) return 1;
-  dot_iv(el, eh, cb + 7, lo, hi);
-  if (lo > T(0) || hi < T(0)) return 1;
-  T al, ah, bl, bh, cl, ch;
-  dot_iv(el, eh, cb + 13, al, ah);
-  dot_iv(el, eh, cb + 19, bl, bh);
-  dot_iv(el, eh, cb + 25, cl, ch);
-  if (mag2_hi(cb + 1) + ah < T(0) || mag2_hi(cb + 7) + ch < T(0)) return 1;
-  const T lg = cb[0], bm = mmax(mabs(bl), mabs(bh));
-  const T lm = T(0.5) * ((al + cl) - sqrt((al - cl) * (al - cl) + T(4) * bm * bm));
-  return lg > T(0) && lg + lm > IvTol<T>::guard * lg ? 2 : 0;
-}
-// The same for a curve part (bounds cb: |C'|^2 from below, then C', C''; gpu.wgsl:1602-1614).
-template <class T> __device__ inline int cleaf_kind(const ICAM_K T* cb, const ICAM_K float* box, v3<T> p) {
-  const T el[3] = {T(box[0]) - p.x, T(box[1]) - p.y, T(box[2]) - p.z}, eh[3] = {T(box[3]) - p.x, T(box[4]) - p.y, T(box[5]) - p.z};
-  T lo, hi;
-  dot_iv(el, eh, cb + 1, lo, hi);
-  if (lo > T(0) || hi < T(0)) return 1;
-  T kl, kh;
-  dot_iv(el, eh, cb + 7, kl, kh);
-  if (mag2_hi(cb + 1) + kh < T(0)) return 1;
-  const T l = cb[0];
-  return l > T(0) && l + kl > IvTol<T>::guard * l ? 2 : 0;
 }
 
 // gpu.wgsl:679-756 / solid3.rs:1904-1910: a sub-patch read by Newton when its radius
@@ -1154,6 +1294,14 @@ __device__ inline bool settle_cleaf(const Tab<double>& tb, uint32_t ci, double l
       }
     }
     if (it.depth >= SETTLE_DEPTH) continue;
+    {
+      // the half's own interval test (gpu.wgsl:1602-1614 on its net)
+      double cc[4][4], cb[13], el[3], eh[3];
+      for (int i = 0; i < 4; ++i)
+        for (int kk = 0; kk < 4; ++kk) cc[i][kk] = pw[4 * i + kk];
+      sub_cubic4(cc, it.a0, it.a1);
+      if (cnet_bounds(cc, p, cb, el, eh) && piece_kind<double>((const double*)cb, el, eh) != 0) continue;
+    }
     const double m = 0.5 * (it.a0 + it.a1);
     const v3<double> dir = d > 1e-12 ? (p - e.p) * (1.0 / d) : mk3(0.0, 0.0, 1.0);
     for (int c = 0; c < 2; ++c) {
