@@ -1261,7 +1261,7 @@ enum BCtl : uint32_t {
   BC_OVER = 21,     // a part whose band outgrew its room
   BC_WORDS = 32,
 };
-enum BStage : int { BS_CSEARCH = 0, BS_CFALL, BS_CSETTLE, BS_NSEARCH, BS_NFALL, BS_NSETTLE, BS_FINISH, BS_STORE };
+enum BStage : int { BS_CSEARCH = 0, BS_CFALL, BS_CSETTLE, BS_NSEARCH, BS_NFALL, BS_NSETTLE, BS_FINISH };
 
 struct BatchArgs {
   const BPart* parts;
@@ -1511,12 +1511,11 @@ __global__ void kb_store(BatchArgs a) {
     store_brick(g - a.brick_off[p], P.band, P.sgn, P.map, P.counters);
   }
 }
+// Every band brick costs the same here, so the blocks stride over them (a queue's
+// round trip per brick would cost more than its copy).
 __global__ __launch_bounds__(128) void kb_store_band(BatchArgs a) {
-  __shared__ uint32_t wslot;
   const uint32_t total = a.band_off[a.n];
-  for (;;) {
-    const uint32_t w = next_item(a, BS_STORE, &wslot);
-    if (w >= total) break;
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
     const int p = __builtin_amdgcn_readfirstlane(part_of<uint32_t>(a.band_off, a.n, w));
     const ICAM_K BPart& P = bpart(a, p);
     store_band_brick(long(w - __builtin_amdgcn_readfirstlane(a.band_off[p])), P.list, P.cls, P.slot, P.staging, P.map, P.atlas);
@@ -2244,7 +2243,7 @@ int batch_enqueue(icam_batch* B, hipStream_t s, Stages* st = nullptr) {
   HIP_OK(launch(st, "bricks_finish", s, dim3(B->gf), dim3(WG), kb_finish, a));
   HIP_OK(launch(st, "slots", s, dim3(B->n), dim3(SCAN_T), kb_slots, a));
   HIP_OK(launch(st, "store", s, dim3(pts_c), dim3(256), kb_store, a));
-  HIP_OK(launch(st, "store_band", s, dim3(B->gf), dim3(128), kb_store_band, a));
+  HIP_OK(launch(st, "store_band", s, dim3(unsigned(std::min<size_t>(std::max<size_t>(B->total_cap, 1), 65535))), dim3(128), kb_store_band, a));
   return 0;
 }
 

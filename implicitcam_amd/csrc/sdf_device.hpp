@@ -1392,6 +1392,31 @@ template <class T> __device__ inline void visit(const Tab<T>& tb, uint32_t code,
 
 // The last correction in double precision of the answer a single-precision search
 // chose: the winner's foot and sign again from the float64 tables.
+// The correction of a sub-patch winner: the single-precision winner's parameters are
+// within rounding of the foot, where the distance is stationary: one Newton step (more
+// only if it moved far) leaves a distance error of the order of the step squared; they
+// are first put inside the sub-patch's double-precision bounds (a foot on its edge then
+// lies on it exactly).
+__device__ inline void refine64_leaf(const Tab<double>& tb, uint32_t id, double pa, double pb, v3<double> p, double& d, double& sg) {
+  const Layout& L = tb.L;
+  const ICAM_K double* lf = tb.F + L.leaf_f + LEAF_F * id;
+  const ICAM_K uint32_t* lu = tb.U + L.leaf_u + LEAF_U * id;
+  const ICAM_K double* sp = tb.F + L.span_f + SPAN_F * lu[0];
+  double s = mclamp(pa, lf[1], lf[2]), t = mclamp(pb, lf[3], lf[4]);
+  newton2(sp, p, s, t, lf[1], lf[2], lf[3], lf[4], 4, 1e-6);
+  SEval<double> e = peval1(sp, s, t);
+  v3<double> r = e.p - p;
+  d = len(r);
+  const ICAM_K double* f = tb.F + L.face_f + FACE_F * lu[1];
+  sg = sgn_of(dot(r, cross(e.ps, e.pt)) * -f[14]);
+}
+__device__ inline void refine64_vert(const Tab<double>& tb, uint32_t id, v3<double> p, double& d, double& sg) {
+  const ICAM_K double* r = tb.F + tb.L.vert_f + VERT_F * id;
+  v3<double> x = ld3(r);
+  d = len(p - x);
+  sg = sgn_of(dot(p - x, ld3(r + 3)));
+}
+
 __device__ inline void refine64(const Tab<double>& tb, uint32_t code, double pa, double pb, v3<double> p, double& d, double& sg) {
   const Layout& L = tb.L;
   uint32_t kind = code >> 28, id = code & 0x0fffffffu;
@@ -1407,25 +1432,9 @@ __device__ inline void refine64(const Tab<double>& tb, uint32_t code, double pa,
     try_edge(tb, id, p, b);
     if (b.code != ~0u) d = b.d, sg = b.sgn;
   } else if (kind == P_VERT) {
-    const ICAM_K double* r = tb.F + L.vert_f + VERT_F * id;
-    v3<double> x = ld3(r);
-    d = len(p - x);
-    sg = sgn_of(dot(p - x, ld3(r + 3)));
+    refine64_vert(tb, id, p, d, sg);
   } else if (kind == P_LEAF) {
-    const ICAM_K double* lf = tb.F + L.leaf_f + LEAF_F * id;
-    const ICAM_K uint32_t* lu = tb.U + L.leaf_u + LEAF_U * id;
-    const ICAM_K double* sp = tb.F + L.span_f + SPAN_F * lu[0];
-    // the single-precision winner's parameters are within rounding of the foot, where
-    // the distance is stationary: one Newton step (more only if it moved far) leaves a
-    // distance error of the order of the step squared; first put them inside the
-    // sub-patch's double-precision bounds (a foot on its edge then lies on it exactly)
-    double s = mclamp(pa, lf[1], lf[2]), t = mclamp(pb, lf[3], lf[4]);
-    newton2(sp, p, s, t, lf[1], lf[2], lf[3], lf[4], 4, 1e-6);
-    SEval<double> e = peval1(sp, s, t);
-    v3<double> r = e.p - p;
-    d = len(r);
-    const ICAM_K double* f = tb.F + L.face_f + FACE_F * lu[1];
-    sg = sgn_of(dot(r, cross(e.ps, e.pt)) * -f[14]);
+    refine64_leaf(tb, id, pa, pb, p, d, sg);
   } else {
     const ICAM_K double* cf = tb.F + L.cleaf_f + CLEAF_F * id;
     const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * id;
