@@ -120,6 +120,10 @@ struct Job {
   // candidates lie within that reach of its centre)
   uint32_t* tile_cand;    // TCAP per tile
   uint32_t* tile_count;   // per tile (more than TCAP: not kept)
+  // the primitives within reach of a super-tile's centres (SUPER^3 tiles), in list
+  // order, which its tiles cull instead of the whole list
+  uint32_t* super_cand;   // SCAP per super-tile
+  uint32_t* super_count;  // per super-tile (more than SCAP: not kept)
 };
 
 // The first node and the cell side of brick bi of a nodes job, and its n-th node.
@@ -220,22 +224,40 @@ __device__ int cull(const Dev& dv, const float* gb, float R, uint32_t* cand, int
 }
 
 // The kept candidates ordered by their box's distance from the block's box centre,
-// nearest first (ties by list order), so a lane's bound drops early. tmp and key
-// hold SORTC entries.
+// nearest first (ties by list order), so a lane's bound drops early: a bitonic sort of
+// 64-bit keys (the distance's bits above the list position; distances are not
+// negative, so their bits order as the distances do). tmp holds SORTC entries, key
+// 2 SORTC words (8-byte aligned).
 template <int NT>
 __device__ void sort_cand(const Dev& dv, const float* gb, uint32_t* cand, uint32_t* tmp, float* key, int nc) {
   if (nc <= 1 || nc > SORTC) return;
   const int tid = threadIdx.x;
   float c[6] = {0.5f * (gb[0] + gb[3]), 0.5f * (gb[1] + gb[4]), 0.5f * (gb[2] + gb[5]), 0, 0, 0};
   c[3] = c[0], c[4] = c[1], c[5] = c[2];
-  for (int k = tid; k < nc; k += NT) key[k] = box_gap(dv.pbox + 6 * cand[k], c), tmp[k] = cand[k];
-  __syncthreads();
-  for (int k = tid; k < nc; k += NT) {
-    float kk = key[k];
-    int rank = 0;
-    for (int j = 0; j < nc; ++j) rank += (key[j] < kk) || (key[j] == kk && j < k);
-    cand[rank] = tmp[k];
+  uint64_t* k64 = reinterpret_cast<uint64_t*>(key);
+  int n = 2;
+  while (n < nc) n <<= 1;
+  for (int k = tid; k < n; k += NT) {
+    if (k < nc) {
+      k64[k] = (uint64_t(__float_as_uint(box_gap(dv.pbox + 6 * cand[k], c))) << 32) | uint32_t(k);
+      tmp[k] = cand[k];
+    } else {
+      k64[k] = ~0ull;
+    }
   }
+  __syncthreads();
+  for (int size = 2; size <= n; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < n; i += NT) {
+        const int l = i ^ stride;
+        if (l > i) {
+          const uint64_t a = k64[i], b = k64[l];
+          if ((a > b) == ((i & size) == 0)) k64[i] = b, k64[l] = a;
+        }
+      }
+      __syncthreads();
+    }
+  for (int k = tid; k < nc; k += NT) cand[k] = tmp[uint32_t(k64[k])];
   __syncthreads();
 }
 
@@ -508,6 +530,18 @@ template <int S> struct Shape {
 constexpr int TILE_X = 4, TILE_Y = 2, TILE_Z = 2;
 static_assert(TILE_X * TILE_Y * TILE_Z * Shape<SRC_CENTRES>::K == Shape<SRC_CENTRES>::NT, "a centres block covers its tile");
 
+// Super-tiles of SUPER^3 tiles: the level's count, and a tile's super-tile.
+constexpr int SUPER = 2, SCAP = 2048;
+__host__ __device__ inline long supers_of(const int nb[3]) {
+  const long tx = (nb[0] + TILE_X - 1) / TILE_X, ty = (nb[1] + TILE_Y - 1) / TILE_Y, tz = (nb[2] + TILE_Z - 1) / TILE_Z;
+  return ((tx + SUPER - 1) / SUPER) * ((ty + SUPER - 1) / SUPER) * ((tz + SUPER - 1) / SUPER);
+}
+__device__ inline long super_of_tile(const Lv& lv, long tile) {
+  const long tx = (lv.nb[0] + TILE_X - 1) / TILE_X, ty = (lv.nb[1] + TILE_Y - 1) / TILE_Y;
+  const long sx = (tx + SUPER - 1) / SUPER, sy = (ty + SUPER - 1) / SUPER;
+  return (tile % tx) / SUPER + sx * (((tile / tx) % ty) / SUPER + sy * ((tile / (tx * ty)) / SUPER));
+}
+
 // The tiles of a level, the tile a brick lies in, and the g-th brick of a tile (-1
 // past the level).
 __host__ __device__ inline long tiles_of(const int nb[3]) {
@@ -600,6 +634,12 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
     long tile = tile_of(j.lv, long(j.list[blk]));
     uint32_t cnt = j.tile_count[tile];
     nc = cnt <= uint32_t(TCAP) ? cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot, j.tile_cand + size_t(tile) * TCAP, cnt) : cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot);
+  } else if (S == SRC_CENTRES && j.super_cand) {
+    // the tile's super-tile's list (a superset in the same order), when it was kept
+    const long sp = super_of_tile(j.lv, blk);
+    const uint32_t cnt = j.super_count[sp];
+    nc = cnt <= uint32_t(SCAP) ? cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot, j.super_cand + size_t(sp) * SCAP, cnt)
+                               : cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot);
   } else {
     nc = cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot);
   }
@@ -609,6 +649,29 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
     if (threadIdx.x == 0) j.tile_count[blk] = uint32_t(nc);
   }
   return nc;
+}
+
+// Super-tile sp's list: the primitives within reach of its centres, culled from the
+// whole list (in list order) by a block of 256.
+__device__ __forceinline__ void super_block(const Dev& dv, const Job& j, long sp, uint32_t* cand, int* wave_tot) {
+  const long tx = (j.lv.nb[0] + TILE_X - 1) / TILE_X, ty = (j.lv.nb[1] + TILE_Y - 1) / TILE_Y;
+  const long sx = (tx + SUPER - 1) / SUPER, sy = (ty + SUPER - 1) / SUPER;
+  const long x0 = TILE_X * SUPER * (sp % sx), y0 = TILE_Y * SUPER * ((sp / sx) % sy), z0 = TILE_Z * SUPER * (sp / (sx * sy));
+  const long x1 = min(x0 + TILE_X * SUPER, long(j.lv.nb[0])) - 1, y1 = min(y0 + TILE_Y * SUPER, long(j.lv.nb[1])) - 1,
+             z1 = min(z0 + TILE_Z * SUPER, long(j.lv.nb[2])) - 1;
+  double bound;
+  const v3<double> a = job_point<SRC_CENTRES>(j, dv, x0 + j.lv.nb[0] * (y0 + long(j.lv.nb[1]) * z0), bound);
+  const v3<double> b = job_point<SRC_CENTRES>(j, dv, x1 + j.lv.nb[0] * (y1 + long(j.lv.nb[1]) * z1), bound);
+  const float r = float(fmin(bound, 3.0e38)) * (1.0f + 1e-5f) + 1e-4f;
+  const float gb[6] = {float(a.x) - 1e-4f, float(a.y) - 1e-4f, float(a.z) - 1e-4f, float(b.x) + 1e-4f, float(b.y) + 1e-4f, float(b.z) + 1e-4f};
+  const int nc = cull<256, SCAP>(dv, gb, r, cand, wave_tot);
+  for (int k = threadIdx.x; k < nc && k < SCAP; k += 256) j.super_cand[size_t(sp) * SCAP + k] = cand[k];
+  if (threadIdx.x == 0) j.super_count[sp] = uint32_t(nc);
+}
+__global__ __launch_bounds__(256) void k_super(Dev dv, Job j) {
+  __shared__ uint32_t cand[SCAP];
+  __shared__ int wave_tot[4];
+  super_block(dv, j, blockIdx.x, cand, wave_tot);
 }
 
 // ------------------------------------------------------------------ search kernels
@@ -647,7 +710,7 @@ template <int S, class T> struct SearchLds {
   uint32_t cand[Shape<S>::C];
   float red[NT];
   int wave_tot[NT / 64];
-  alignas(16) unsigned char scratch[cmax(SORTC * 8, Q * NT * (int(sizeof(T)) + 2))];
+  alignas(16) unsigned char scratch[cmax(SORTC * 12, Q * NT * (int(sizeof(T)) + 2))];
   __device__ uint32_t* tmp() { return (uint32_t*)scratch; }
   __device__ float* key() { return (float*)(scratch + SORTC * 4); }
   __device__ T* qlb() { return (T*)scratch; }
@@ -1268,6 +1331,7 @@ struct BatchArgs {
   int n;
   const long* brick_off;  // n + 1: bricks of the parts before p
   const long* tile_off;   // n + 1: tiles
+  const long* super_off;  // n + 1: super-tiles
   uint32_t* band_off;     // n + 1: band bricks (written by kb_plan)
   uint32_t* btotal;       // per part: band bricks its scan found
   uint32_t* stotal;       // per part: slots
@@ -1308,8 +1372,15 @@ __device__ inline uint32_t next_item(const BatchArgs& a, int st, uint32_t* slot)
   return w;
 }
 
-__global__ void kb_fill(uint32_t* p, size_t n, uint32_t v) {
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+// Up to six buffers cleared by one launch (a fill kernel, not hipMemsetAsync: the
+// batch's launches are captured in a graph).
+struct FillList {
+  uint32_t* p[6];
+  size_t n[6];
+};
+__global__ void kb_fill(FillList f) {
+  for (int k = 0; k < 6; ++k)
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < f.n[k]; i += size_t(gridDim.x) * blockDim.x) f.p[k][i] = 0u;
 }
 
 // The searches, persistent: centres over every part's tiles, nodes over every part's
@@ -1338,6 +1409,21 @@ __global__ __launch_bounds__(Shape<S>::NT) void kb_search(BatchArgs a) {
     const Job j = S == SRC_CENTRES ? ldk(P.jc) : ldk(P.jn);
     const Res r = S == SRC_CENTRES ? ldk(P.rc) : ldk(P.rn);
     search_block<S, T>(dv, j, r, blk, sm);
+    __syncthreads();
+  }
+}
+
+// Every part's super-tile lists, a block each.
+__global__ __launch_bounds__(256) void kb_super(BatchArgs a) {
+  __shared__ uint32_t cand[SCAP];
+  __shared__ int wave_tot[4];
+  const long total = a.super_off[a.n];
+  for (long w = blockIdx.x; w < total; w += gridDim.x) {
+    const int p = __builtin_amdgcn_readfirstlane(part_of<long>(a.super_off, a.n, w));
+    const ICAM_K BPart& P = bpart(a, p);
+    const Dev dv = ldk(P.dv);
+    const Job j = ldk(P.jc);
+    super_block(dv, j, w - a.super_off[p], cand, wave_tot);
     __syncthreads();
   }
 }
@@ -1565,7 +1651,7 @@ struct icam_hip {
   double box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};
   Buf o_count, o_cell, o_rank, o_idx, o_xyz, o_d, o_flag, o_feat;
   // the level
-  Buf dc, band, sgn, total, list, staging, cls, slot, stotal, map, atlas, counters, changed, firsts, tile_cand, tile_count;
+  Buf dc, band, sgn, total, list, staging, cls, slot, stotal, map, atlas, counters, changed, firsts, tile_cand, tile_count, super_cand, super_count;
   std::vector<std::pair<const char*, double>> stages;
   std::string name;
 };
@@ -1819,7 +1905,7 @@ void icam_hip_close(icam_hip* h) {
   h->nodes.free();
   for (Buf* b : {&h->F32, &h->F64, &h->U, &h->pbox, &h->pcode, &h->q_xyz, &h->q_bound, &h->q_d, &h->q_flag, &h->q_feat, &h->dc, &h->band, &h->sgn, &h->total,
                  &h->list, &h->staging, &h->cls, &h->slot, &h->stotal, &h->map, &h->atlas, &h->counters, &h->changed, &h->firsts,
-                 &h->tile_cand, &h->tile_count, &h->o_count, &h->o_cell, &h->o_rank, &h->o_idx, &h->o_xyz, &h->o_d, &h->o_flag, &h->o_feat})
+                 &h->tile_cand, &h->tile_count, &h->super_cand, &h->super_count, &h->o_count, &h->o_cell, &h->o_rank, &h->o_idx, &h->o_xyz, &h->o_d, &h->o_flag, &h->o_feat})
     b->free();
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -1999,6 +2085,11 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   HIP_OK(h->tile_cand.need(size_t(tiles) * TCAP * 4));
   HIP_OK(h->tile_count.need(size_t(tiles) * 4));
   jc.tile_cand = h->tile_cand.as<uint32_t>(), jc.tile_count = h->tile_count.as<uint32_t>();
+  const long supers = supers_of(lv.nb);
+  HIP_OK(h->super_cand.need(size_t(supers) * SCAP * 4));
+  HIP_OK(h->super_count.need(size_t(supers) * 4));
+  jc.super_cand = h->super_cand.as<uint32_t>(), jc.super_count = h->super_count.as<uint32_t>();
+  HIP_OK(launch(&st, "centres_lists", s, dim3(unsigned(supers)), dim3(256), k_super, h->dv, jc));
   if (run_search<SRC_CENTRES>(h, mixed, jc, r, s, &st, tiles)) return 1;
   HIP_OK(launch(&st, "decide", s, gb, dim3(WG), k_decide_finish, h->dv, lv, nb, (const double*)r.val, (const uint32_t*)r.flag, h->dc.as<double>(),
                 h->band.as<uint8_t>(), h->sgn.as<int8_t>()));
@@ -2136,16 +2227,16 @@ struct icam_batch {
   std::vector<int> caller;       // batch position -> the caller's index
   std::vector<int> pos;          // the caller's index -> batch position
   std::vector<BPart> host;       // the records uploaded
-  std::vector<long> brick_off, tile_off;
+  std::vector<long> brick_off, tile_off, super_off;
   std::vector<size_t> cap_off;   // band room before each part
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  Buf bparts, brick_off_d, tile_off_d, band_off, btotal, stotal, ctl, counters;
+  Buf bparts, brick_off_d, tile_off_d, super_off_d, band_off, btotal, stotal, ctl, counters, super_cand, super_count;
   Buf c_code, c_pa, c_pb, c_d32, c_info, c_val, c_flag, c_qmark, c_smark, c_queue, c_squeue;
   Buf n_code, n_pa, n_pb, n_d32, n_info, n_val, n_flag, n_qmark, n_smark, n_queue, n_squeue, items;
   Buf tile_cand, tile_count, dc, band, sgn, list, staging, cls, slot, map, atlas;
-  long total_bricks = 0, total_tiles = 0;
+  long total_bricks = 0, total_tiles = 0, total_supers = 0;
   size_t total_cap = 0;
   // the launch shapes: persistent grids of the centres' search, the nodes' search
   // and the band bricks' finish (blocks)
@@ -2155,7 +2246,7 @@ struct icam_batch {
   ~icam_batch() {
     if (exec) hipGraphExecDestroy(exec);
     if (graph) hipGraphDestroy(graph);
-    for (Buf* b : {&bparts, &brick_off_d, &tile_off_d, &band_off, &btotal, &stotal, &ctl, &counters, &c_code, &c_pa, &c_pb, &c_d32, &c_info, &c_val,
+    for (Buf* b : {&bparts, &brick_off_d, &tile_off_d, &super_off_d, &super_cand, &super_count, &band_off, &btotal, &stotal, &ctl, &counters, &c_code, &c_pa, &c_pb, &c_d32, &c_info, &c_val,
                    &c_flag, &c_qmark, &c_smark, &c_queue, &c_squeue, &n_code, &n_pa, &n_pb, &n_d32, &n_info, &n_val, &n_flag, &n_qmark, &n_smark,
                    &n_queue, &n_squeue, &items, &tile_cand, &tile_count, &dc, &band, &sgn, &list, &staging, &cls, &slot, &map, &atlas})
       b->free();
@@ -2199,18 +2290,20 @@ uint32_t band_room(const icam_hip* h, const Lv& lv) {
 int batch_enqueue(icam_batch* B, hipStream_t s, Stages* st = nullptr) {
   const BatchArgs& a = B->args;
   const bool mixed = B->precision == ICAM_MIXED;
-  auto fill = [&](Buf& b, size_t words) -> int {
-    if (!words) return 0;
-    HIP_OK(launch(st, "zero", s, dim3(unsigned(std::min<size_t>((words + 255) / 256, 4096))), dim3(256), kb_fill, b.as<uint32_t>(), words, 0u));
-    return 0;
-  };
-  if (fill(B->ctl, BC_WORDS) || fill(B->counters, size_t(B->n) * 16) || fill(B->c_qmark, size_t(B->total_tiles)) ||
-      fill(B->c_smark, size_t(B->total_tiles)) || fill(B->n_qmark, B->total_cap) || fill(B->n_smark, B->total_cap))
-    return 1;
+  {
+    // control words, counters and the queues' marks
+    FillList f{{B->ctl.as<uint32_t>(), (uint32_t*)B->counters.as<unsigned long long>(), B->c_qmark.as<uint32_t>(), B->c_smark.as<uint32_t>(),
+                B->n_qmark.as<uint32_t>(), B->n_smark.as<uint32_t>()},
+               {BC_WORDS, size_t(B->n) * 16, size_t(B->total_tiles), size_t(B->total_tiles), B->total_cap, B->total_cap}};
+    size_t most = 0;
+    for (size_t n : f.n) most = std::max(most, n);
+    HIP_OK(launch(st, "zero", s, dim3(unsigned(std::min<size_t>((most + 255) / 256, 4096))), dim3(256), kb_fill, f));
+  }
   const unsigned pts_c = unsigned(std::min<long>((B->total_bricks + 255) / 256, 8192));
   const unsigned pts_n = unsigned(std::min<size_t>((B->total_cap * 729 + 255) / 256, 16384));
   const dim3 ntc(Shape<SRC_CENTRES>::NT), ntn(Shape<SRC_NODES>::NT);
-  // the brick centres: the level's decision
+  // the brick centres: the level's decision (the super-tiles' lists first)
+  HIP_OK(launch(st, "centres_lists", s, dim3(unsigned(std::min<long>(std::max<long>(B->total_supers, 1), 4096))), dim3(256), kb_super, a));
   if (mixed) {
     HIP_OK(launch(st, "centres_search32", s, dim3(B->gc), ntc, kb_search<SRC_CENTRES, float>, a));
     HIP_OK(launch(st, "centres_refine64", s, dim3(pts_c), dim3(256), kb_refine<SRC_CENTRES>, a));
@@ -2331,16 +2424,17 @@ int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t
   B->pos.resize(n);
   for (size_t k = 0; k < n; ++k) B->pos[B->caller[k]] = int(k);
   // offsets
-  B->brick_off.assign(n + 1, 0), B->tile_off.assign(n + 1, 0), B->cap_off.assign(n + 1, 0);
+  B->brick_off.assign(n + 1, 0), B->tile_off.assign(n + 1, 0), B->super_off.assign(n + 1, 0), B->cap_off.assign(n + 1, 0);
   for (size_t k = 0; k < n; ++k) {
     const int i = B->caller[k];
     B->parts.push_back(parts[i]);
     const Lv& lv = lvs[i];
     B->brick_off[k + 1] = B->brick_off[k] + long(lv.nb[0]) * lv.nb[1] * lv.nb[2];
     B->tile_off[k + 1] = B->tile_off[k] + tiles_of(lv.nb);
+    B->super_off[k + 1] = B->super_off[k] + supers_of(lv.nb);
     B->cap_off[k + 1] = B->cap_off[k] + caps[i];
   }
-  B->total_bricks = B->brick_off[n], B->total_tiles = B->tile_off[n], B->total_cap = B->cap_off[n];
+  B->total_bricks = B->brick_off[n], B->total_tiles = B->tile_off[n], B->total_supers = B->super_off[n], B->total_cap = B->cap_off[n];
   const size_t nbk = size_t(std::max<long>(B->total_bricks, 1)), ntl = size_t(std::max<long>(B->total_tiles, 1));
   const size_t ncap = std::max<size_t>(B->total_cap, 1), nnode = ncap * 729;
   const bool mixed = precision == ICAM_MIXED;
@@ -2349,7 +2443,9 @@ int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t
   HIP_OK(hipStreamCreate(&B->stream));
   // device memory
   for (auto [buf, bytes] : std::initializer_list<std::pair<Buf*, size_t>>{
-           {&B->bparts, n * sizeof(BPart)}, {&B->brick_off_d, (n + 1) * 8}, {&B->tile_off_d, (n + 1) * 8}, {&B->band_off, (n + 1) * 4},
+           {&B->bparts, n * sizeof(BPart)}, {&B->brick_off_d, (n + 1) * 8}, {&B->tile_off_d, (n + 1) * 8}, {&B->super_off_d, (n + 1) * 8},
+           {&B->super_cand, size_t(std::max<long>(B->total_supers, 1)) * SCAP * 4}, {&B->super_count, size_t(std::max<long>(B->total_supers, 1)) * 4},
+           {&B->band_off, (n + 1) * 4},
            {&B->btotal, n * 4}, {&B->stotal, n * 4}, {&B->ctl, BC_WORDS * 4}, {&B->counters, n * 64},
            {&B->c_val, nbk * 8}, {&B->c_flag, nbk * 4}, {&B->c_qmark, ntl * 4}, {&B->c_smark, ntl * 4}, {&B->c_queue, ntl * 4},
            {&B->c_squeue, ntl * 4}, {&B->n_val, nnode * 8}, {&B->n_flag, nnode * 4}, {&B->n_qmark, ncap * 4}, {&B->n_smark, ncap * 4},
@@ -2369,7 +2465,7 @@ int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t
     const int i = B->caller[k];
     icam_hip* h = parts[i];
     BPart P{};
-    const long bo = B->brick_off[k], to = B->tile_off[k];
+    const long bo = B->brick_off[k], to = B->tile_off[k], so = B->super_off[k];
     const size_t co = B->cap_off[k];
     P.dv = h->dv, P.lv = lvs[i], P.nb = B->brick_off[k + 1] - bo, P.cap = caps[i];
     P.dc = B->dc.as<double>() + bo, P.band = B->band.as<uint8_t>() + bo, P.sgn = B->sgn.as<int8_t>() + bo, P.list = B->list.as<uint32_t>() + bo;
@@ -2377,6 +2473,7 @@ int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t
     P.map = B->map.as<uint32_t>() + bo, P.atlas = B->atlas.as<uint32_t>() + co * ICAM_WORDS;
     P.counters = B->counters.as<unsigned long long>() + 8 * k;
     P.jc.n = P.nb, P.jc.lv = P.lv, P.jc.tile_cand = B->tile_cand.as<uint32_t>() + size_t(to) * TCAP, P.jc.tile_count = B->tile_count.as<uint32_t>() + to;
+    P.jc.super_cand = B->super_cand.as<uint32_t>() + size_t(so) * SCAP, P.jc.super_count = B->super_count.as<uint32_t>() + so;
     P.jn = P.jc;
     P.jn.n = long(P.cap) * 729, P.jn.list = P.list;
     Res& rc = P.rc;
@@ -2402,8 +2499,9 @@ int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t
   HIP_OK(hipMemcpy(B->bparts.p, B->host.data(), n * sizeof(BPart), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(B->brick_off_d.p, B->brick_off.data(), (n + 1) * 8, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(B->tile_off_d.p, B->tile_off.data(), (n + 1) * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(B->super_off_d.p, B->super_off.data(), (n + 1) * 8, hipMemcpyHostToDevice));
   BatchArgs& a = B->args;
-  a.parts = B->bparts.as<BPart>(), a.n = int(n), a.brick_off = B->brick_off_d.as<long>(), a.tile_off = B->tile_off_d.as<long>();
+  a.parts = B->bparts.as<BPart>(), a.n = int(n), a.brick_off = B->brick_off_d.as<long>(), a.tile_off = B->tile_off_d.as<long>(), a.super_off = B->super_off_d.as<long>();
   a.band_off = B->band_off.as<uint32_t>(), a.btotal = B->btotal.as<uint32_t>(), a.stotal = B->stotal.as<uint32_t>(), a.ctl = ctl;
   // launch shapes: defaults from the device's compute units, then the autotuner
   hipDeviceProp_t prop;
