@@ -754,13 +754,37 @@ void revolve(Builder& b, double cx, double cy, std::vector<Prof> prof, bool hole
         if (lo > 1e-12) pcs.push_back(line_piece(0, lo, 4, lo));
         if (hi < 2 - 1e-12) pcs.push_back(line_piece(0, hi, 4, hi));
         fid[i] = b.analytic(ICAM_SPHERE, V{cx, cy, e.zc}, Z, X, e.rad, 0, sense, 0, -1, 4, 3, 4, 0, pcs, 2.0, 0.5 * (lo + hi), true);
-      } else {
+      } else if (std::fabs(e.rc) > e.rad) {
         // ring torus: tube arc parameter over the whole meridian circle, periodic
         double pm = 0.5 * (e.psi0 + e.psi1);
         double v0 = arc_param_over(e.psi0, 0, 2 * M_PI, 4), v1 = arc_param_over(e.psi1, 0, 2 * M_PI, 4);
         double vm = arc_param_over(pm, 0, 2 * M_PI, 4);
         std::vector<Piece> pcs = {line_piece(0, v0, 4, v0), line_piece(0, v1, 4, v1)};
         fid[i] = b.analytic(ICAM_TORUS, V{cx, cy, e.zc}, Z, X, e.rc, e.rad, sense, 0, 0, 4, 4, 4, 4, pcs, 2.0, vm, true);
+      } else {
+        // spindle torus (the arc crosses the axis: a lemon when its centre lies across
+        // the axis, an apple when on this side): the tube angle over the sheet on the
+        // axis's own side, [-pp, pp] with pp = acos(-major / minor), in narc quarter arcs
+        // (trim2.rs:1807-1818), not periodic; no trim line where the arc meets the axis
+        const double pp = std::acos(std::min(std::max(-e.rc / e.rad, -1.0), 1.0));
+        const int narc = int(std::max(std::ceil(2 * pp / (0.5 * M_PI) - 1e-9), 1.0));
+        double v0 = arc_param_over(e.psi0, -pp, 2 * pp, narc), v1 = arc_param_over(e.psi1, -pp, 2 * pp, narc);
+        double vm = arc_param_over(0.5 * (e.psi0 + e.psi1), -pp, 2 * pp, narc);
+        std::vector<Piece> pcs;
+        if (e.r0 > 0) pcs.push_back(line_piece(0, v0, 4, v0));
+        if (e.r1 > 0) pcs.push_back(line_piece(0, v1, 4, v1));
+        fid[i] = b.analytic(ICAM_TORUS, V{cx, cy, e.zc}, Z, X, e.rc, e.rad, sense, 0, 0, 4, double(narc), 4, 0, pcs, 2.0, vm, true);
+        // the tip on the axis: a vertex loop of the face, as STEP writes it, and a vertex
+        // whose pseudo-normal is the face's normal there weighted by the full turn
+        for (int end = 0; end < 2; ++end) {
+          if ((end ? e.r1 : e.r0) > 0) continue;
+          const double a = end ? e.psi1 : e.psi0, s = e.psi1 > e.psi0 ? 1.0 : -1.0;
+          V tip{cx, cy, end ? e.z1 : e.z0};
+          icam_face_desc& f = b.o->faces.back();
+          f.n_points = 1;
+          f.points = b.o->keep({tip.x, tip.y, tip.z});
+          b.vert(tip, Z * (s * std::sin(a) * flip * 2.0 * M_PI));
+        }
       }
       continue;
     }
@@ -871,6 +895,10 @@ Owned* make_revolved(const std::vector<Prof>& prof) {
 //   2 top: cone from its apex on the axis (a cone tip), cylinder, convex torus
 //     round, top disc
 //   3 washer: annulus, outer cylinder with rounded top edge, inner cylinder
+//   4 bullet: disc, cylinder, a lemon nose (a spindle torus whose tube circle's centre
+//     lies across the axis) meeting the axis in a tip
+//   5 dome: disc, cylinder, an apple top (a spindle torus centred on this side of the
+//     axis, its arc running over the top and down into a dimple on the axis)
 std::vector<Prof> turned_profile(int type, int grooves, Rng& rng) {
   const double R = 6 + 14 * rng.next(), L = 2 * R + 40 * rng.next();
   std::vector<Prof> p;
@@ -910,6 +938,18 @@ std::vector<Prof> turned_profile(int type, int grooves, Rng& rng) {
     p.push_back(arc(R - f, zt, f, 0, 0.5 * M_PI));
     p.push_back(seg(R - f, zt + f, 0, zt + f));
     p.push_back(seg(0, zt + f, 0, 0));
+  } else if (type == 4) {
+    const double d = (0.2 + 0.6 * rng.next()) * R, rad = R + d, pe = std::acos(d / rad);
+    p.push_back(seg(0, 0, R, 0));
+    wall(0, L);
+    p.push_back(arc(-d, L, rad, 0, pe));
+    p.push_back(seg(0, L + rad * std::sin(pe), 0, 0));
+  } else if (type == 5) {
+    const double d = (0.15 + 0.3 * rng.next()) * R, rad = R - d, pe = std::acos(-d / rad);
+    p.push_back(seg(0, 0, R, 0));
+    wall(0, L);
+    p.push_back(arc(d, L, rad, 0, pe));
+    p.push_back(seg(0, L + rad * std::sin(pe), 0, 0));
   } else {
     double ri = (0.3 + 0.3 * rng.next()) * R, H = 0.3 * L + 4, f = std::min(0.3 * (R - ri), 0.3 * H);
     p.push_back(seg(ri, 0, R, 0));
@@ -1273,7 +1313,7 @@ const icam_part_desc* icam_gen_part(const char* name, uint64_t seed) {
   if (s.rfind("turned:", 0) == 0) {
     Rng rng{seed | 1};
     size_t c = s.find(':', 7);
-    o = make_revolved(turned_profile(arg(7) % 4, c == std::string::npos ? 0 : arg(c + 1), rng));
+    o = make_revolved(turned_profile(arg(7) % 6, c == std::string::npos ? 0 : arg(c + 1), rng));
   }
   if (s.rfind("plate:", 0) == 0) {
     Rng rng{seed | 1};

@@ -565,6 +565,23 @@ double piece_radius(const H* pw, double* cert = nullptr) {
   return h > 0 ? 0.99 * l / h : INF;
 }
 
+// trim2.rs:1789-1807 arc_param_over: an angle's parameter in the exact rational form of
+// n equal arcs covering span from a0 (each arc's half-angle tangent, rescaled to [k, k+1])
+double arc_param_over(double a, double a0, double span, int n) {
+  const double q = span / n;
+  double x = a - a0;
+  if (std::fabs(span - TAU) < 1e-12) {
+    x = x - TAU * std::floor(x / TAU);
+  } else {
+    const double y = x - TAU * std::floor((x - 0.5 * span) / TAU + 0.5);
+    x = std::min(std::max(y, 0.0), span);
+  }
+  const double k = std::min(std::max(std::floor(x / q), 0.0), double(n - 1));
+  const double alpha = x - q * k - 0.5 * q;
+  const double t = 0.5 * (std::tan(0.5 * alpha) / std::tan(0.25 * q) + 1.0);
+  return k + std::min(std::max(t, 0.0), 1.0);
+}
+
 // gpu.rs:110-112 the split rule
 bool split_rule(double rho, double diag, double need, int depth) { return depth < 8 && rho < 2.0 * diag + need && diag > 0.25 * need; }
 
@@ -912,26 +929,39 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
   // vertices answers as a point with the pseudo-normal z (-sin a) sense
   // (solid3.rs:1991-2000); it is packed after the vertices as one more point record,
   // word 6 holding its face + 1 (the feature it reports)
+  // A spindle torus's tips (|major| <= minor: lemon or apple) likewise, on the axis at
+  // +-h, h = sqrt(minor^2 - major^2), pseudo-normals +-z sense.
   uint32_t n_tips = 0;
-  for (uint32_t fi = 0; fi < part->n_faces; ++fi) {
+  auto tip = [&](uint32_t fi, V at, V pn, double u, double v) {
     const icam_face_desc& fd = part->faces[fi];
-    if (fd.kind != ICAM_CONE) continue;
-    V z = of(fd.z);
-    const double ha = -fd.c[0] / std::tan(fd.c[1]);
-    V apex = of(fd.o) + z * ha;
     bool near_vertex = false;
-    for (uint32_t v = 0; v < part->n_verts; ++v) near_vertex |= len(of(part->verts[v].p) - apex) <= part->tol;
-    // the apex's parameters: angle 0 about the axis (arc parameter 0), height ha
+    for (uint32_t k = 0; k < part->n_verts; ++k) near_vertex |= len(of(part->verts[k].p) - at) <= part->tol;
     TrimBuild tb;
     tb.build(fd);
-    if (near_vertex || !tb.inside(0.0, ha)) continue;
-    V p = apex - o, pn = z * (-std::sin(fd.c[1]) * fd.sense);
+    if (near_vertex || !tb.inside(u, v)) return;
+    V p = at - o;
     vert_f.insert(vert_f.end(), {p.x, p.y, p.z, pn.x, pn.y, pn.z, double(fi + 1), 0});
     Box b;
     b.add(p);
     pbox.push_back(b);
     pcode.push_back((P_VERT << 28) | (part->n_verts + n_tips));
     ++n_tips;
+  };
+  for (uint32_t fi = 0; fi < part->n_faces; ++fi) {
+    const icam_face_desc& fd = part->faces[fi];
+    V z = of(fd.z);
+    if (fd.kind == ICAM_CONE) {
+      // the apex's parameters: angle 0 about the axis (arc parameter 0), height ha
+      const double ha = -fd.c[0] / std::tan(fd.c[1]);
+      tip(fi, of(fd.o) + z * ha, z * (-std::sin(fd.c[1]) * fd.sense), 0.0, ha);
+    } else if (fd.kind == ICAM_TORUS && std::fabs(fd.c[0]) <= fd.c[1]) {
+      // the tips' parameters: angle 0, tube angle atan2(+-h, -major) (arc parameter)
+      const double h = std::sqrt(std::max(fd.c[1] * fd.c[1] - fd.c[0] * fd.c[0], 0.0));
+      const double pp = std::acos(std::min(std::max(-fd.c[0] / fd.c[1], -1.0), 1.0));
+      const int narc = int(std::max(std::ceil(2 * pp / (0.5 * PI) - 1e-9), 1.0));
+      for (int sgn = 1; sgn >= -1; sgn -= 2)
+        tip(fi, of(fd.o) + z * (sgn * h), z * (sgn * fd.sense), 0.0, arc_param_over(std::atan2(sgn * h, -fd.c[0]), -pp, 2 * pp, narc));
+    }
   }
 
   // ---- concatenate

@@ -592,12 +592,19 @@ template <class T> __device__ void try_face(const Tab<T>& tb, uint32_t fi, v3<T>
   v3<T> q = foot(kind, f, p);
   T d = len(p - q);
   if (!(d < b.d)) return;
-  // a cone's apex is singular: the edges, the vertices and the packed tip (a point
-  // record) answer there (solid3.rs:1915-1918)
+  // a cone's apex and a spindle torus's tips (lemon, apple: |major| <= minor, on the
+  // axis at +-sqrt(minor^2 - major^2)) are singular: the edges, the vertices and the
+  // packed tips (point records) answer there (solid3.rs:1915-1918, 1991-2005)
+  const T stol = sizeof(T) == 4 ? T(1e-6) : T(1e-12);
   if (kind == ICAM_CONE) {
     T ha = -f[12] / mtan(f[13]);
     v3<T> apex = ld3(f) + ld3(f + 9) * ha;
-    if (len(apex - q) <= (sizeof(T) == 4 ? T(1e-6) : T(1e-12)) * (T(1) + len(apex))) return;
+    if (len(apex - q) <= stol * (T(1) + len(apex))) return;
+  }
+  if (kind == ICAM_TORUS && mabs(f[12]) <= f[13]) {
+    const T h = msqrt(mmax(f[13] * f[13] - f[12] * f[12], T(0)));
+    const v3<T> o = ld3(f), z = ld3(f + 9), t0 = o + z * h, t1 = o - z * h;
+    if (len(t0 - q) <= stol * (T(1) + len(t0)) || len(t1 - q) <= stol * (T(1) + len(t1))) return;
   }
   bool inside = true;
   if (!fu[2]) {

@@ -238,9 +238,10 @@ def test_golden_vectors(prec):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-@pytest.mark.parametrize("name", ["turned:0:2", "turned:1:1", "turned:2:1", "turned:3:2", "plate:12"])
+@pytest.mark.parametrize("name", ["turned:0:2", "turned:1:1", "turned:2:1", "turned:3:2", "turned:4:1", "turned:5:2", "plate:12"])
 def test_revolved_and_plates(name, prec):
-    """Cone tips, sphere poles, concave grooves, convex rounds, fillets and countersinks:
+    """Cone tips, sphere poles, lemon and apple tips, concave grooves, convex rounds,
+    fillets and countersinks:
     queries against the closed form (uniform points and points on and around the axis)
     and the oracle's nearest features, and a small level against the oracle's bake."""
     from closed_forms import described_sdf

@@ -282,10 +282,12 @@ def test_bake_map_and_atlas(c1):
     assert (d[m == 0xFFFFFFFE] < 0).all()
 
 
-@pytest.mark.parametrize("name", ["turned:0:0", "turned:0:3", "turned:1:2", "turned:2:0", "turned:2:2", "turned:3:1"])
+@pytest.mark.parametrize("name", ["turned:0:0", "turned:0:3", "turned:1:2", "turned:2:0", "turned:2:2", "turned:3:1", "turned:4:0", "turned:4:2",
+                                  "turned:5:0", "turned:5:1"])
 def test_turned_closed_form(built, name):
     """Solids of revolution (planes, cylinders, cones with their apex on the axis, spheres
-    with their poles on it, ring tori as convex rounds and concave grooves) against the
+    with their poles on it, ring tori as convex rounds and concave grooves, spindle tori
+    as a lemon nose and an apple top meeting the axis in a tip) against the
     planar distance to their profile in the meridian half-plane, on uniform points and
     points crowded on the axis (the tips and poles)."""
     from closed_forms import described_sdf
