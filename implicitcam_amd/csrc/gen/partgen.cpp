@@ -888,6 +888,57 @@ Owned* make_revolved(const std::vector<Prof>& prof) {
   return o;
 }
 
+// A cylinder of radius R and height L on a disc, capped by a hemisphere written as a
+// rational biquadratic B-spline face (the nine-point circle along u, a quarter circle
+// along v, weights 1 and 1/sqrt 2): its v = 1 side collapses to the pole on the axis,
+// which no vertex marks, so the pole answers as a point with the fan of the surface's
+// normals around it (solid3.rs:1609-1625, 1955-1986). The equator is a spline edge of
+// four quadratic pieces mapped onto the dome's u. The closed form is the capsule
+// profile's (a quarter circle about a point of the axis).
+Owned* make_dome(double R, double L) {
+  Owned* o = new Owned();
+  Builder b{o};
+  Prof cap = arc(0, L, R, 0, 0.5 * M_PI);
+  cap.face = false;  // made below as the B-spline face
+  revolve(b, 0, 0, {seg(0, 0, R, 0), seg(R, 0, R, L), cap, seg(0, L + R, 0, 0)}, false);
+  const uint32_t f_cyl = 1;  // the profile's faces in order: the disc, the cylinder
+  const double s = std::sqrt(0.5);
+  const double ex[9] = {1, 1, 0, -1, -1, -1, 0, 1, 1}, ey[9] = {0, 1, 1, 1, 0, -1, -1, -1, 0}, wu[9] = {1, s, 1, s, 1, s, 1, s, 1};
+  const double rj[3] = {1, 1, 0}, zj[3] = {L, L + R, L + R}, wv[3] = {1, s, 1};
+  std::vector<double> pw(9 * 3 * 4);
+  for (int i = 0; i < 9; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const double w = wu[i] * wv[j];
+      double* q = &pw[(i * 3 + j) * 4];
+      q[0] = R * ex[i] * rj[j] * w, q[1] = R * ey[i] * rj[j] * w, q[2] = zj[j] * w, q[3] = w;
+    }
+  const std::vector<double> ku = {0, 0, 0, 0.25, 0.25, 0.5, 0.5, 0.75, 0.75, 1, 1, 1};
+  icam_face_desc f;
+  std::memset(&f, 0, sizeof(f));
+  f.kind = ICAM_NURBS;
+  f.sense = 1.0;  // S_u x S_v points out (along +x at u = 0 on the equator)
+  f.du = f.dv = 2;
+  f.nu = 9, f.nv = 3;
+  f.pw = o->keep(pw);
+  f.ku = o->keep(ku);
+  f.kv = o->keep({0, 0, 0, 1, 1, 1});
+  b.set_trim(f, 0, 0, 1, 1, 0, 0, {}, 0.5, 0.5, true);
+  o->faces.push_back(f);
+  const uint32_t f_dome = uint32_t(o->faces.size() - 1);
+  // the equator: the v = 0 row as four quadratic pieces, u spans of a quarter each
+  std::vector<double> row;
+  for (int i = 0; i < 9; ++i) row.insert(row.end(), {R * ex[i] * wu[i], R * ey[i] * wu[i], L * wu[i], wu[i]});
+  std::vector<double> uv;
+  for (int m = 0; m < 4; ++m) uv.insert(uv.end(), {0.25 * m, 0.0, 0.25 * (m + 1), 0.0, NaN, NaN, NaN, NaN});
+  b.spline(2, decompose(2, ku, row, 4), f_dome, f_cyl, uv);
+  // its seam vertex, both faces' normals (+x) weighted by pi
+  b.vert(V{R, 0, L}, V{2 * M_PI, 0, 0});
+  o->info = {1.0};
+  describe(o->info, {seg(0, 0, R, 0), seg(R, 0, R, L), arc(0, L, R, 0, 0.5 * M_PI), seg(0, L + R, 0, 0)});
+  o->finish({-R, -R, 0}, {R, R, L + R}, 1e-9);
+  return o;
+}
+
 // Profiles of turned parts, each with g grooves (concave tori) cut into its
 // cylinder; sizes in mm from the seed.
 //   0 capsule: hemisphere, cylinder, hemisphere (sphere poles on the axis)
@@ -1310,6 +1361,11 @@ const icam_part_desc* icam_gen_part(const char* name, uint64_t seed) {
   if (s == "c2flat") o = make_c2(seed, true);
   if (s == "c3") o = make_c3(seed);
   if (s.rfind("c4:", 0) == 0) o = make_c4(seed, arg(3));
+  if (s.rfind("dome:", 0) == 0) {
+    Rng r2{(seed ^ (0x51ED270B27FF3A15ull * uint64_t(arg(5) + 1))) | 1};
+    const double R = 8 + 12 * r2.next(), L = 10 + 30 * r2.next();
+    o = make_dome(R, L);
+  }
   if (s.rfind("turned:", 0) == 0) {
     Rng rng{seed | 1};
     size_t c = s.find(':', 7);

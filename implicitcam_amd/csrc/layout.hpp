@@ -33,8 +33,10 @@ enum : uint32_t { VERT_F = 8 };
 // (the sub-patch lies between the planes n.x = lo and n.x = hi); from LEAF_CERT its bounds
 // for the per-point convexity test: lambda_min(G) from below, then the intervals (x lo, x hi,
 // y lo, y hi, z lo, z hi) of S_s, S_t, S_ss, S_st, S_tt in the parameters scaled by the
-// middle's tangent lengths; U: span 0, face 1, known inside 2
-enum : uint32_t { LEAF_CERT = 12, LEAF_F = 44, LEAF_U = 4 };
+// middle's tangent lengths; from LEAF_POLE: a pole the sub-patch touches (x, y, z, then the
+// collapsed side in its span's parameters: 1 s = 0, 2 s = 1, 3 t = 0, 4 t = 1; all zero
+// when none); U: span 0, face 1, known inside 2
+enum : uint32_t { LEAF_CERT = 12, LEAF_POLE = 44, LEAF_F = 48, LEAF_U = 4 };
 // SPAN: 16 homogeneous points (i along u, 4 words each) 0-63, u0 64, u1 65, v0 66, v1 67;
 // U: face 0, trimmed 1 (0 whole surface), root cell 2
 enum : uint32_t { SPAN_F = 72, SPAN_U = 4 };

@@ -392,6 +392,31 @@ void eval_patch(const Patch4& pa, double s, double t, V& p, V& ps, V& pt) {
   pt = (mk(a[2][0], a[2][1], a[2][2]) - p * a[2][3]) * (1 / w);
 }
 
+// The same with the mixed second derivative (the fan of normals at a pole).
+void eval_patch2(const Patch4& pa, double s, double t, V& p, V& ps, V& pt, V& pst) {
+  auto basis = [](double x, double b[4], double d[4]) {
+    double y = 1 - x;
+    b[0] = y * y * y, b[1] = 3 * x * y * y, b[2] = 3 * x * x * y, b[3] = x * x * x;
+    d[0] = -3 * y * y, d[1] = 3 * y * y - 6 * x * y, d[2] = 6 * x * y - 3 * x * x, d[3] = 3 * x * x;
+  };
+  double bu[4], du[4], bv[4], dv[4];
+  basis(s, bu, du), basis(t, bv, dv);
+  double a[4][4] = {{0}};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 4; ++k) {
+        a[0][k] += bu[i] * bv[j] * pa.p[i][j][k];
+        a[1][k] += du[i] * bv[j] * pa.p[i][j][k];
+        a[2][k] += bu[i] * dv[j] * pa.p[i][j][k];
+        a[3][k] += du[i] * dv[j] * pa.p[i][j][k];
+      }
+  const double w = a[0][3];
+  p = mk(a[0][0], a[0][1], a[0][2]) * (1 / w);
+  ps = (mk(a[1][0], a[1][1], a[1][2]) - p * a[1][3]) * (1 / w);
+  pt = (mk(a[2][0], a[2][1], a[2][2]) - p * a[2][3]) * (1 / w);
+  pst = (mk(a[3][0], a[3][1], a[3][2]) - ps * a[2][3] - pt * a[1][3] - p * a[3][3]) * (1 / w);
+}
+
 // What a leaf keeps of its bounds for the per-point convexity test (gpu.wgsl:1581-1614
 // reads the same quantities): lambda_min(G) from below, then the intervals of S_s, S_t,
 // S_ss, S_st and S_tt (x lo, x hi, y lo, y hi, z lo, z hi each), in the parameters
@@ -614,6 +639,7 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
 
   // sections are built into separate vectors, then concatenated
   std::vector<double> face_f, edge_f, vert_f, leaf_f, span_f, cleaf_f, cpiece_f, cell_f, inst_f, tpiece_f, uvmap_f;
+  std::vector<double> pole_f;  // B-spline poles, point records placed after the tips
   std::vector<uint32_t> face_u, edge_u, leaf_u, span_u, cleaf_u, cpiece_u, cell_u, inst_u, among_u;
   std::vector<Box> pbox;
   std::vector<uint32_t> pcode;
@@ -759,9 +785,10 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
         }
         nsv = int(vr.size());
       }
+      // the spans' patches, raised to bicubic (gpu.rs:149-171)
+      std::vector<Patch4> fp(size_t(nsu) * nsv);
       for (int a = 0; a < nsu; ++a)
         for (int b = 0; b < nsv; ++b) {
-          // the span's patch, raised to bicubic (gpu.rs:149-171)
           std::vector<std::vector<H>> pw(fd.du + 1);
           for (uint32_t i = 0; i <= fd.du; ++i)
             for (uint32_t j = 0; j <= fd.dv; ++j) pw[i].push_back(relh(&rows[a][i][b][j * 4]));
@@ -777,9 +804,84 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
             }
             pw = np;
           }
-          Patch4 pa;
+          Patch4& pa = fp[size_t(a) * nsv + b];
           for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j) pa.p[i][j] = pw[i][j];
+        }
+      // solid3.rs:1609-1625 poles: a side of the domain along which the surface
+      // collapses to a point (surface3.rs:1392-1411), away from the vertices and
+      // inside the loops, answers as a point record (after the tips) with the fan of
+      // the surface's normals around it times the face's sense (solid3.rs:1955-1986)
+      struct FacePole {
+        int k;
+        double value;
+        V x;
+      };
+      std::vector<FacePole> fpoles;
+      {
+        auto feval = [&](double u, double v, V& p, V& ps, V& pt, V& pst) {
+          int a = 0, b = 0;
+          for (int k = 1; k < nsu; ++k)
+            if (ur[k].first <= u) a = k;
+          for (int k = 1; k < nsv; ++k)
+            if (vr[k].first <= v) b = k;
+          const double su = ur[a].second - ur[a].first, sv = vr[b].second - vr[b].first;
+          const double s = std::min(std::max((u - ur[a].first) / su, 0.0), 1.0), t = std::min(std::max((v - vr[b].first) / sv, 0.0), 1.0);
+          eval_patch2(fp[size_t(a) * nsv + b], s, t, p, ps, pt, pst);
+          ps = ps * (1 / su), pt = pt * (1 / sv), pst = pst * (1 / (su * sv));
+        };
+        auto inside = [&](double u, double v) { return whole || tb.inside(u, v); };
+        Box cb;
+        for (auto& pa : fp)
+          for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) cb.add(point(pa.p[i][j]));
+        const double scale = std::max(cb.diag(), 1e-9);
+        const double range[2][2] = {{ur.front().first, ur.back().second}, {vr.front().first, vr.back().second}};
+        for (int k = 0; k < 2; ++k)
+          for (double side : {range[k][0], range[k][1]}) {
+            const double* other = range[1 - k];
+            auto at = [&](double t) {
+              V p, ps, pt, pst;
+              const double w = other[0] + (other[1] - other[0]) * t;
+              feval(k == 0 ? side : w, k == 0 ? w : side, p, ps, pt, pst);
+              return p;
+            };
+            const V p0 = at(0.0);
+            bool all = true;
+            for (int i = 1; i <= 16; ++i) all &= len(at(i / 16.0) - p0) <= 1e-12 * scale;
+            if (!all) continue;
+            bool near_vertex = false;
+            for (uint32_t vi = 0; vi < part->n_verts; ++vi) near_vertex |= len(of(part->verts[vi].p) - (p0 + o)) <= part->tol;
+            const double mid = 0.5 * (other[0] + other[1]);
+            if (near_vertex || !(k == 0 ? inside(side, mid) : inside(mid, side))) continue;
+            // the fan: limiting tangents S_k(w) and normals S_k x S_kw (u held) or
+            // S_kw x S_v (v held), oriented into the domain, each weighted by the angle
+            // its tangent turns through where the loops cover the side
+            const double h = side == range[k][0] ? 1.0 : -1.0, step = 1e-9 * (range[k][1] - range[k][0]) * h;
+            V out = mk(0, 0, 0), ptan = mk(0, 0, 0), pnrm = mk(0, 0, 0);
+            bool have = false, pcov = false;
+            for (int i = 0; i <= 512; ++i) {
+              const double w = other[0] + (other[1] - other[0]) * i / 512.0;
+              V p, ps, pt, pst;
+              feval(k == 0 ? side : w, k == 0 ? w : side, p, ps, pt, pst);
+              V tan = (k == 0 ? ps : pt) * h, nrm = (k == 0 ? cross(ps, pst) : cross(pst, pt)) * h;
+              const bool cov = k == 0 ? inside(side + step, w) : inside(w, side + step);
+              if (have && pcov && cov && len(tan) > 0 && len(ptan) > 0) {
+                const V ua = ptan * (1 / len(ptan)), ub = tan * (1 / len(tan));
+                const double ang = std::atan2(len(cross(ua, ub)), dot(ua, ub));
+                const V m = pnrm * (1 / len(pnrm)) + nrm * (1 / len(nrm));
+                if (len(m) > 0) out = out + m * (ang / len(m));
+              }
+              ptan = tan, pnrm = nrm, pcov = cov, have = true;
+            }
+            const V pn = out * fd.sense;
+            pole_f.insert(pole_f.end(), {p0.x, p0.y, p0.z, pn.x, pn.y, pn.z, double(fi + 1), 0});
+            fpoles.push_back({k, side, p0});
+          }
+      }
+      for (int a = 0; a < nsu; ++a)
+        for (int b = 0; b < nsv; ++b) {
+          const Patch4& pa = fp[size_t(a) * nsv + b];
           uint32_t sid = uint32_t(span_f.size() / SPAN_F);
           for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j) span_f.insert(span_f.end(), pa.p[i][j].begin(), pa.p[i][j].end());
@@ -849,6 +951,18 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
             double cert[PATCH_CERT + 1] = {0};
             patch_radius(sb.p, cert);
             leaf_f.insert(leaf_f.end(), cert, cert + PATCH_CERT + 1);
+            // a pole whose side the sub-patch touches, and that side in the span's
+            // parameters (1 s = 0, 2 s = 1, 3 t = 0, 4 t = 1)
+            double pole[4] = {0, 0, 0, 0};
+            for (const FacePole& fpo : fpoles) {
+              int side = 0;
+              if (fpo.k == 0 && fpo.value == ur[a].first && sb.s0 == 0.0) side = 1;
+              if (fpo.k == 0 && fpo.value == ur[a].second && sb.s1 == 1.0) side = 2;
+              if (fpo.k == 1 && fpo.value == vr[b].first && sb.t0 == 0.0) side = 3;
+              if (fpo.k == 1 && fpo.value == vr[b].second && sb.t1 == 1.0) side = 4;
+              if (side) pole[0] = fpo.x.x, pole[1] = fpo.x.y, pole[2] = fpo.x.z, pole[3] = side;
+            }
+            leaf_f.insert(leaf_f.end(), pole, pole + 4);
             leaf_u.insert(leaf_u.end(), {sid, fi, sb.inside == 1 ? 1u : 0u, 0});
             pbox.push_back(bb);
             pcode.push_back((P_LEAF << 28) | id);
@@ -962,6 +1076,16 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
       for (int sgn = 1; sgn >= -1; sgn -= 2)
         tip(fi, of(fd.o) + z * (sgn * h), z * (sgn * fd.sense), 0.0, arc_param_over(std::atan2(sgn * h, -fd.c[0]), -pp, 2 * pp, narc));
     }
+  }
+
+  // the poles, after the tips: point records reporting their face
+  for (size_t k = 0; k + VERT_F <= pole_f.size(); k += VERT_F) {
+    vert_f.insert(vert_f.end(), pole_f.begin() + k, pole_f.begin() + k + VERT_F);
+    Box b;
+    b.add(mk(pole_f[k], pole_f[k + 1], pole_f[k + 2]));
+    pbox.push_back(b);
+    pcode.push_back((P_VERT << 28) | (part->n_verts + n_tips));
+    ++n_tips;
   }
 
   // ---- concatenate

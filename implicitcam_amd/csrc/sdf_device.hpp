@@ -1161,8 +1161,9 @@ __device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb
     auto inside = [&](double uu, double vv) { return known_in || fu[2] != 0 || trim_inside(tb, f + 17, fu[1], uu, vv, 0.0, near); };
     if (d < bd && inside(u, v)) {
       bd = d;
+      const bool at_pole = lf[LEAF_POLE + 3] != 0.0 && len(e.p - ld3(lf + LEAF_POLE)) <= 1e-9 * (1.0 + len(ld3(lf + LEAF_POLE)));
       if (critical64(p, e.p, r, e.ps) && critical64(p, e.p, r, e.pt)) {
-        if (len(nrm) > 0.0) offer(b, d, (P_LEAF << 28) | li, s, t, sgn_of(dot(r, nrm) * -f[14]));
+        if (len(nrm) > 0.0 && !at_pole) offer(b, d, (P_LEAF << 28) | li, s, t, sgn_of(dot(r, nrm) * -f[14]));
       } else {
         double hd, hu, hv, hs;
         if (polish64(tb, fi, p, u, v, hd, hu, hv, hs) && hd <= d && inside(hu, hv)) {
@@ -1177,6 +1178,35 @@ __device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb
     double box, slab;
     hull_bounds(&net[0][0], 16, p, dir, box, slab);
     if (slab >= bd - SETTLE_EPS || it.depth >= SETTLE_DEPTH) continue;
+    // a quarter touching a collapsed side (a pole, answered by its own point record):
+    // never cleared by the critical-point test (surface3.rs:1209) and split across the
+    // side only, so the quarters at the pole stay one chain instead of doubling
+    const int pside = int(lf[LEAF_POLE + 3]);
+    const bool touch = (pside == 1 && it.a0 == 0.0) || (pside == 2 && it.a1 == 1.0) || (pside == 3 && it.b0 == 0.0) || (pside == 4 && it.b1 == 1.0);
+    if (touch) {
+      const bool along_t = pside >= 3;
+      const double am = 0.5 * (it.a0 + it.a1), bm = 0.5 * (it.b0 + it.b1);
+      for (int c = 0; c < 2; ++c) {
+        SettleItem ch = it;
+        ch.depth = it.depth + 1;
+        if (along_t)
+          (c ? ch.b0 : ch.b1) = bm;
+        else
+          (c ? ch.a0 : ch.a1) = am;
+        double cn[4][4][4], cbx, cs;
+        sub_net(sp, ch.a0, ch.a1, ch.b0, ch.b1, cn);
+        hull_bounds(&cn[0][0], 16, p, dir, cbx, cs);
+        ch.lb = fmax(cbx, slab);
+        if (ch.lb < bd - SETTLE_EPS) {
+          if (nq == SETTLE_Q) {
+            complete = false;
+            continue;
+          }
+          q[nq++] = ch;
+        }
+      }
+      continue;
+    }
     {
       // the quarter's own interval test (gpu.wgsl:1581-1600 on its net): no interior
       // minimum drops it; strictly convex for p, its minimum is the Newton point above
@@ -1243,6 +1273,12 @@ __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box,
   T d = len(r);
   if (!certified) b.lmin = mmin(b.lmin, lb);
   if (!(d < b.d)) return;
+  // a foot at a pole the sub-patch touches: the pole's point record answers there
+  // (its normal comes from the fan; the surface's own degenerates)
+  if (lf[LEAF_POLE + 3] != T(0)) {
+    const v3<T> x = ld3(lf + LEAF_POLE);
+    if (len(e.p - x) <= (sizeof(T) == 4 ? T(1e-6) : T(1e-9)) * (T(1) + len(x))) return;
+  }
   T tol = single ? T(1e-4) * d + eps : T(1e-9) * (T(1) + d);
   if (!(is_critical(r, e.ps, tol) && is_critical(r, e.pt, tol))) return;
   const uint32_t fi = lu[1];

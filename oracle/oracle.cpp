@@ -1281,10 +1281,75 @@ struct Sd {
   bool cusp;
 };
 
+// surface3.rs:1392-1411 collapsed_sides: the sides of a B-spline surface's domain along
+// which it collapses to a point (a pole): per side, the direction held (0: u), its
+// value and the point
+struct Collapsed {
+  int k;
+  double value;
+  V3 x;
+};
+std::vector<Collapsed> collapsed_sides(const Nurbs& ns) {
+  const double scale = std::max(len(ns.box.hi - ns.box.lo), 1e-9);
+  const double r[2][2] = {{ns.su0.front(), ns.su0.back()}, {ns.sv0.front(), ns.sv0.back()}};
+  std::vector<Collapsed> out;
+  for (int k = 0; k < 2; ++k)
+    for (double side : {r[k][0], r[k][1]}) {
+      const double* other = r[1 - k];
+      auto at = [&](double t) {
+        const double w = other[0] + (other[1] - other[0]) * t;
+        return k == 0 ? ns.eval(side, w).p : ns.eval(w, side).p;
+      };
+      const V3 p0 = at(0.0);
+      bool all = true;
+      for (int i = 1; i <= 16; ++i) all &= len(at(i / 16.0) - p0) <= 1e-12 * scale;
+      if (all) out.push_back({k, side, p0});
+    }
+  return out;
+}
+
+// solid3.rs:1955-1986 fan_normal: the pseudo-normal at a pole, the surface there being
+// the cone of its limiting tangents S_k(w) with limiting normals S_k x S_kw (u held) or
+// S_kw x S_v (v held), each weighted by the angle its tangent turns through, over the
+// part of the side the trim covers; in the surface's own orientation
+V3 fan_normal(const Nurbs& ns, const Trim& trim, int k, double value) {
+  const double r[2][2] = {{ns.su0.front(), ns.su0.back()}, {ns.sv0.front(), ns.sv0.back()}};
+  const double* range = r[k];
+  const double* other = r[1 - k];
+  const double h = value == range[0] ? 1.0 : -1.0;
+  const double step = 1e-9 * (range[1] - range[0]) * h;
+  const int n = 512;
+  V3 out(0, 0, 0), pt(0, 0, 0), pn(0, 0, 0);
+  bool have = false, pc = false;
+  for (int i = 0; i <= n; ++i) {
+    const double w = other[0] + (other[1] - other[0]) * i / n;
+    const double u = k == 0 ? value : w, v = k == 0 ? w : value;
+    const SEval e = ns.eval(u, v);
+    V3 tan = k == 0 ? e.ps : e.pt, nrm = k == 0 ? cross(e.ps, e.pst) : cross(e.pst, e.pt);
+    tan = tan * h, nrm = nrm * h;
+    const bool covered = k == 0 ? trim.inside(value + step, w) : trim.inside(w, value + step);
+    if (have && pc && covered && len(tan) > 0 && len(pt) > 0) {
+      const V3 a = pt * (1.0 / len(pt)), b = tan * (1.0 / len(tan));
+      const double ang = std::atan2(len(cross(a, b)), dot(a, b));
+      const V3 mid = pn * (1.0 / len(pn)) + nrm * (1.0 / len(nrm));
+      if (len(mid) > 0) out = out + mid * (ang / len(mid));
+    }
+    pt = tan, pn = nrm, pc = covered, have = true;
+  }
+  return out;
+}
+
 struct Exact {
   std::vector<Face> faces;
   std::vector<Edge> edges;
   std::vector<std::pair<V3, V3>> verts;
+  // solid3.rs:1553 poles: a B-spline face's collapsed side away from the vertices and
+  // inside its loops, answering as a point with its fan normal times the face's sense
+  struct Pole {
+    int face;
+    V3 x, n;
+  };
+  std::vector<Pole> poles;
   double tol;
   BoxTree tree;
   Box part;
@@ -1423,12 +1488,31 @@ struct Exact {
         }
       }
     }
+    // solid3.rs:1609-1625 the poles
+    for (size_t fi = 0; fi < faces.size(); ++fi) {
+      const Face& f = faces[fi];
+      if (f.kind != ICAM_NURBS) continue;
+      for (const Collapsed& c : collapsed_sides(*f.ns)) {
+        bool near_vertex = false;
+        for (auto& v : verts) near_vertex |= len(v.first - c.x) <= tol;
+        if (near_vertex) continue;
+        const double o0 = c.k == 0 ? f.ns->sv0.front() : f.ns->su0.front(), o1 = c.k == 0 ? f.ns->sv0.back() : f.ns->su0.back();
+        const double mid = 0.5 * (o0 + o1);
+        if (!(c.k == 0 ? f.trim.inside(c.value, mid) : f.trim.inside(mid, c.value))) continue;
+        poles.push_back({int(fi), c.x, fan_normal(*f.ns, f.trim, c.k, c.value) * f.sense});
+      }
+    }
     std::vector<Box> boxes;
     for (auto& f : faces) boxes.push_back(f.box);
     for (auto& e : edges) boxes.push_back(e.box);
     for (auto& v : verts) {
       Box b;
       b.add(v.first);
+      boxes.push_back(b);
+    }
+    for (auto& pl : poles) {
+      Box b;
+      b.add(pl.x);
       boxes.push_back(b);
     }
     tree.build(boxes);
@@ -1496,10 +1580,15 @@ struct Exact {
           best = {c.dist * sgn(dot(p - c.q, pn)), c.q, (1u << 28) | uint32_t(ei), false, false};
           cusp = len(pn) < 1e-9;
         }
-      } else {
+      } else if (id < nf + ne + int(verts.size())) {
         int vi = id - nf - ne;
         double d = len(p - verts[vi].first);
         if (d < cur) best = {d * sgn(dot(p - verts[vi].first, verts[vi].second)), verts[vi].first, (2u << 28) | uint32_t(vi), false, false};
+      } else {
+        // solid3.rs:1794-1799 a pole: its face's answer
+        const Pole& pl = poles[id - nf - ne - int(verts.size())];
+        double d = len(p - pl.x);
+        if (d < cur) best = {d * sgn(dot(p - pl.x, pl.n)), pl.x, uint32_t(pl.face), false, false};
       }
       if (best.d != before) {
         found = true;

@@ -268,6 +268,42 @@ def test_revolved_and_plates(name, prec):
 
 
 @pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("k", [0, 1])
+def test_dome_pole(k, prec):
+    """A B-spline hemisphere whose collapsed side is a pole no vertex marks: the pole's
+    point record (its fan normal) answers at and above the pole; queries against the
+    capsule closed form away from the hemisphere's centre, the pole's own answer, and a
+    small level against the oracle's bake."""
+    from closed_forms import described_sdf
+
+    part = Part(f"dome:{k}")
+    o = Oracle(part)
+    g = PartGpu(part, 4.0)
+    R = part.hi[0]
+    centre = np.array([0.0, 0.0, part.hi[2] - R])
+    rng = np.random.default_rng(17)
+    pts = uniform_points(SEED + k, 1 << 14, part.lo - 3, part.hi + 3)
+    axis = np.stack([rng.normal(0, 0.02, 512), rng.normal(0, 0.02, 512), rng.uniform(part.lo[2] - 3, part.hi[2] + 3, 512)], 1)
+    pole = np.stack([rng.normal(0, 1e-3, 256), rng.normal(0, 1e-3, 256), part.hi[2] + rng.normal(0, 0.5, 256)], 1)
+    pts = np.concatenate([pts, axis, pole])
+    pts = pts[np.linalg.norm(pts - centre, axis=1) > 0.5 * R]
+    d, flags, feat = g.query(pts, prec)
+    ref = described_sdf(part, pts)
+    assert (flags == 0).all()
+    err = np.abs(d - ref)
+    near_axis = np.hypot(pts[:, 0], pts[:, 1]) < 0.01
+    assert err[~near_axis].max() <= max(tol(prec, part), 1e-8)
+    assert err[near_axis].max() <= 1e-6 * part.diagonal
+    assert ((d < 0) == (ref < 0))[np.abs(ref) > 1e-8].all()
+    top, tflag, tfeat = g.query(np.array([[0.0, 0.0, part.hi[2] + 2.0]]), prec)
+    assert abs(top[0] - 2.0) < 1e-9 and tflag[0] == 0 and tfeat[0] == 2
+    lv = level_for(part, part.diagonal / 48, 8)
+    m_ref, a_ref, _ = o.bake(lv)
+    m, a, st = g.bake(lv, prec)
+    assert (m == m_ref).all() and (a == a_ref).all()
+
+
+@pytest.mark.parametrize("prec", PRECS)
 def test_c4_parts_bake_like_oracle(prec):
     """Every part of the batched workload (turned parts, NURBS blocks with 4 x 4 to
     32 x 32 nets, plates with up to ~200 holes) baked on a 64^3 node grid: map and

@@ -310,6 +310,43 @@ def test_turned_closed_form(built, name):
     assert np.abs(db[near] - ref[near]).max() < 1e-9
 
 
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_dome_pole_closed_form(built, k):
+    """A cylinder capped by a rational B-spline hemisphere whose v = 1 side collapses to
+    a pole no vertex marks: the pole answers as a point signed by its fan normal
+    (solid3.rs:1609-1625, 1955-1986). Against the capsule profile's closed form on
+    uniform points and points on and around the axis through the pole, away from the
+    hemisphere's centre (within half its radius every point of the dome is nearly as
+    near, and the certified search, as the reference's, may end at its read cap)."""
+    from closed_forms import described_sdf
+
+    part = Part(f"dome:{k}")
+    o = Oracle(part)
+    R = part.hi[0]
+    centre = np.array([0.0, 0.0, part.hi[2] - R])
+    rng = np.random.default_rng(13)
+    pts = uniform_points(SEED + k, 12000, part.lo - 3, part.hi + 3)
+    axis = np.stack([rng.normal(0, 0.05, 1500), rng.normal(0, 0.05, 1500), rng.uniform(part.lo[2] - 3, part.hi[2] + 3, 1500)], 1)
+    pole = np.stack([rng.normal(0, 1e-3, 300), rng.normal(0, 1e-3, 300), part.hi[2] + rng.normal(0, 0.5, 300)], 1)
+    pts = np.concatenate([pts, axis, pole, [[0.0, 0.0, part.hi[2] + 1.0], [0.0, 0.0, part.hi[2]]]])
+    pts = pts[np.linalg.norm(pts - centre, axis=1) > 0.5 * R]
+    d, feat, _ = o.sdf(pts)
+    ref = described_sdf(part, pts)
+    # a point whose foot lies within a micrometre of the pole may take the pole itself
+    # (the certified search needs a critical point, and the surface degenerates there):
+    # off by the foot's distance squared over twice the radius, within the metric's
+    # 1e-6 of the diagonal
+    err = np.abs(d - ref)
+    near_axis = np.hypot(pts[:, 0], pts[:, 1]) < 0.01
+    assert err[~near_axis].max() < 1e-9
+    assert err[near_axis].max() < 1e-6 * part.diagonal
+    assert ((d < 0) == (ref < 0))[np.abs(ref) > 1e-9].all()
+    # above the pole the pole itself is nearest, reported as the dome's face
+    top = np.array([[0.0, 0.0, part.hi[2] + 2.0]])
+    dt, ft, _ = o.sdf(top)
+    assert abs(dt[0] - 2.0) < 1e-12 and ft[0] == 2
+
+
 @pytest.mark.parametrize("holes", [1, 12, 40])
 def test_plate_closed_form(built, holes):
     """Plates with plain, filleted (torus) and countersunk (cone) holes against per-face
