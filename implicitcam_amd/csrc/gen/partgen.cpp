@@ -24,6 +24,9 @@
 //           interior weights uniform in [0.8, 1.2]
 //   "c2flat" the same with every control point at the top's base height: its top
 //           is the plane z = 20 and the part the box [-60, 60]^2 x [-20, 20]
+//   "crease:<k>" the c2 block on a 17 x 17 net with a knot of multiplicity three at
+//           0.5 along u (k even) or v (k odd): the top folds along that knot line, a
+//           ridge of height 12 (a valley for k = 2, 3) fading to zero at the boundary
 #include <algorithm>
 #include <algorithm>
 #include <array>
@@ -532,18 +535,46 @@ double value_noise(uint64_t seed, double x, double y) {
   return sum / norm;
 }
 
-Owned* make_c2(uint64_t seed, bool flat, int n = 16, double half = 60, double amp = 5) {
+// crease >= 0: a net of 17 x 17 whose knots along direction crease & 1 (0: u) hold 0.5
+// three times (the degree), so the top is only C0 along that knot line, and a tent of
+// height 12 (a ridge; a valley when crease & 2) across it, faded to zero at the two
+// boundary rows it meets: the top folds along the line and stays smooth at its edges.
+Owned* make_c2(uint64_t seed, bool flat, int n = 16, double half = 60, double amp = 5, int crease = -1) {
   Owned* o = new Owned();
   Builder b{o};
-  const int p = 3, spans = n - p;
+  const int p = 3;
+  if (crease >= 0) n = 17;
   const double zb = -half / 3, ztop = half / 3;
-  std::vector<double> kn;
-  for (int i = 0; i <= p; ++i) kn.push_back(0);
-  for (int i = 1; i < spans; ++i) kn.push_back(double(i) / spans);
-  for (int i = 0; i <= p; ++i) kn.push_back(1);
+  auto knots = [&](bool folded) {
+    std::vector<double> kn;
+    for (int i = 0; i <= p; ++i) kn.push_back(0);
+    if (folded) {
+      for (int i = 1; i < 12; ++i) {
+        if (i == 6)
+          kn.insert(kn.end(), {0.5, 0.5, 0.5});
+        else
+          kn.push_back(i / 12.0);
+      }
+    } else {
+      const int spans = n - p;
+      for (int i = 1; i < spans; ++i) kn.push_back(double(i) / spans);
+    }
+    for (int i = 0; i <= p; ++i) kn.push_back(1);
+    return kn;
+  };
+  const std::vector<double> ku = knots(crease >= 0 && (crease & 1) == 0), kv = knots(crease >= 0 && (crease & 1) == 1);
+  // the pieces' knot intervals: the distinct knots
+  auto breaks = [](const std::vector<double>& kn) {
+    std::vector<double> r;
+    for (double x : kn)
+      if (r.empty() || x > r.back()) r.push_back(x);
+    return r;
+  };
+  const std::vector<double> bu = breaks(ku), bv = breaks(kv);
   // Greville abscissae: the net's x (and y) are then linear in u (and v)
-  std::vector<double> gx(n);
-  for (int i = 0; i < n; ++i) gx[i] = -half + 2 * half * (kn[i + 1] + kn[i + 2] + kn[i + 3]) / 3.0;
+  auto knot_sum = [&](const std::vector<double>& kn, int i) { return kn[i + 1] + kn[i + 2] + kn[i + 3]; };
+  std::vector<double> gx(n), gy(n);
+  for (int i = 0; i < n; ++i) gx[i] = -half + 2 * half * knot_sum(ku, i) / 3.0, gy[i] = -half + 2 * half * knot_sum(kv, i) / 3.0;
   Rng rng{seed ? seed : 1};
   std::vector<double> pw(n * n * 4);
   std::vector<double> zz(n * n), ww(n * n);
@@ -551,14 +582,21 @@ Owned* make_c2(uint64_t seed, bool flat, int n = 16, double half = 60, double am
   std::vector<double> raw(n * n);
   double peak = 0;
   for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) raw[i * n + j] = value_noise(seed, gx[i], gx[j]), peak = std::max(peak, std::fabs(raw[i * n + j]));
+    for (int j = 0; j < n; ++j) raw[i * n + j] = value_noise(seed, gx[i], gy[j]), peak = std::max(peak, std::fabs(raw[i * n + j]));
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
       double z = flat ? ztop : ztop + amp * raw[i * n + j] / peak;
+      if (crease >= 0) {
+        // the tent across the line times a bump along it that is zero on the boundary
+        const int a = (crease & 1) == 0 ? i : j, c = (crease & 1) == 0 ? j : i;
+        const double g = ((crease & 1) == 0 ? knot_sum(ku, a) : knot_sum(kv, a)) / 3.0;
+        const double tent = 1.0 - 2.0 * std::fabs(g - 0.5), bump = (c == 0 || c == n - 1) ? 0.0 : 1.0;
+        z += ((crease & 2) ? -12.0 : 12.0) * tent * bump;
+      }
       double w = (i == 0 || j == 0 || i == n - 1 || j == n - 1) ? 1.0 : 0.8 + 0.4 * rng.next();
       zz[i * n + j] = z, ww[i * n + j] = w;
       double* q = &pw[(i * n + j) * 4];
-      q[0] = gx[i] * w, q[1] = gx[j] * w, q[2] = z * w, q[3] = w;
+      q[0] = gx[i] * w, q[1] = gy[j] * w, q[2] = z * w, q[3] = w;
     }
   icam_face_desc top;
   std::memset(&top, 0, sizeof(top));
@@ -567,8 +605,8 @@ Owned* make_c2(uint64_t seed, bool flat, int n = 16, double half = 60, double am
   top.du = top.dv = p;
   top.nu = top.nv = n;
   top.pw = o->keep(pw);
-  top.ku = o->keep(kn);
-  top.kv = o->keep(kn);
+  top.ku = o->keep(ku);
+  top.kv = o->keep(kv);
   b.set_trim(top, 0, 0, 1, 1, 0, 0, {}, 0.5, 0.5, true);
   o->faces.push_back(top);
   const uint32_t ft = 0;
@@ -583,12 +621,12 @@ Owned* make_c2(uint64_t seed, bool flat, int n = 16, double half = 60, double am
       if (side == 1) i = n - 1, j = k;
       if (side == 2) i = k, j = n - 1;
       if (side == 3) i = 0, j = k;
-      pts.insert(pts.end(), {gx[i], gx[j], zz[i * n + j], 1.0});
+      pts.insert(pts.end(), {gx[i], gy[j], zz[i * n + j], 1.0});
     }
-    return decompose(p, kn, pts, 4);
+    return decompose(p, side % 2 == 0 ? ku : kv, pts, 4);
   };
   V X{1, 0, 0}, Y{0, 1, 0}, Z{0, 0, 1};
-  double top_max = ztop + amp + 1;
+  double top_max = ztop + amp + (crease >= 0 ? 13 : 1);
   uint32_t side_face[4];
   for (int side = 0; side < 4; ++side) {
     auto pcs = boundary(side);
@@ -625,8 +663,9 @@ Owned* make_c2(uint64_t seed, bool flat, int n = 16, double half = 60, double am
   for (int side = 0; side < 4; ++side) {
     auto pcs = boundary(side);
     std::vector<double> uv;
+    const std::vector<double>& br = side % 2 == 0 ? bu : bv;
     for (int k = 0; k < int(pcs.size()); ++k) {
-      double a = double(k) / spans, c = double(k + 1) / spans;
+      double a = br[k], c = br[k + 1];
       if (side == 0) uv.insert(uv.end(), {a, 0, c, 0});
       if (side == 1) uv.insert(uv.end(), {1, a, 1, c});
       if (side == 2) uv.insert(uv.end(), {a, 1, c, 1});
@@ -656,7 +695,7 @@ Owned* make_c2(uint64_t seed, bool flat, int n = 16, double half = 60, double am
     int ci = (k == 1 || k == 2) ? n - 1 : 0, cj = (k >= 2) ? n - 1 : 0;
     int di = ci == 0 ? 1 : -1, dj = cj == 0 ? 1 : -1;
     // tangents into the surface along its two boundaries (unit weights there)
-    auto P = [&](int i, int j) { return V{gx[i], gx[j], zat(i, j)}; };
+    auto P = [&](int i, int j) { return V{gx[i], gy[j], zat(i, j)}; };
     V tu = P(ci + di, cj) - P(ci, cj), tv = P(ci, cj + dj) - P(ci, cj);
     V n0 = unit(cross(tu, tv));
     if (n0.z < 0) n0 = n0 * -1.0;
@@ -1376,6 +1415,7 @@ const icam_part_desc* icam_gen_part(const char* name, uint64_t seed) {
     o = make_plate(120, 80, 16, plate_holes(120, 80, 16, std::max(1, arg(6)), rng));
   }
   if (s.rfind("nurbs:", 0) == 0) o = make_c2(seed, false, std::min(std::max(arg(6), 4), 32), 40, 3);
+  if (s.rfind("crease:", 0) == 0) o = make_c2(seed, false, 17, 60, 5, arg(7) & 3);
   return o ? &o->desc : nullptr;
 }
 

@@ -316,7 +316,7 @@ __device__ inline void try_part(const Tab<T>& tb, uint32_t i, T lb, v3<T> p, Bes
   if ((code >> 28) == P_LEAF)
     try_leaf<T, SETTLE>(tb, id, tb.pbox + 6 * i, lb, p, b, tm, eps);
   else
-    try_cleaf<T, SETTLE>(tb, id, tb.pbox + 6 * i, lb, p, b, eps);
+    try_cleaf<T, SETTLE>(tb, id, tb.pbox + 6 * i, lb, p, b, tm, eps);
 }
 template <int K, class V> __device__ inline V group_min(V v) {
   for (int o = 1; o < K; o <<= 1) v = mmin(v, V(__shfl_xor(v, o)));
@@ -497,7 +497,10 @@ __device__ inline uint32_t feat_of(const Dev& dv, uint32_t code) {
     return tip > 0.0 ? uint32_t(tip) - 1u : (2u << 28) | id;
   }
   if (kind == P_LEAF) return dv.U[dv.L.leaf_u + LEAF_U * id + 1];
-  return (1u << 28) | dv.U[dv.L.cleaf_u + CLEAF_U * id + 1];
+  // a C0 knot line reports its face (solid3.rs:1790-1794)
+  const uint32_t e = dv.U[dv.L.cleaf_u + CLEAF_U * id + 1];
+  const uint32_t* eu = dv.U + dv.L.edge_u + EDGE_U * e;
+  return eu[0] == EDGE_CREASE ? eu[1] : (1u << 28) | e;
 }
 
 // A finished double-precision search as (value, flag): exact when a feature answered

@@ -640,11 +640,59 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
   // sections are built into separate vectors, then concatenated
   std::vector<double> face_f, edge_f, vert_f, leaf_f, span_f, cleaf_f, cpiece_f, cell_f, inst_f, tpiece_f, uvmap_f;
   std::vector<double> pole_f;  // B-spline poles, point records placed after the tips
+  // B-spline faces' interior C0 knot lines: the face, the direction held (0: u), the
+  // knot, and per piece its four points and the span before the line
+  struct Crease {
+    uint32_t face, k;
+    double value;
+    std::vector<std::vector<H>> pieces;
+    std::vector<uint32_t> before;
+  };
+  std::vector<Crease> creases;
   std::vector<uint32_t> face_u, edge_u, leaf_u, span_u, cleaf_u, cpiece_u, cell_u, inst_u, among_u;
   std::vector<Box> pbox;
   std::vector<uint32_t> pcode;
   std::vector<Box> face_box(part->n_faces);
   std::vector<Box> edge_box(part->n_edges);
+
+  // a curve piece's tree of parts, breadth first; its leaves are primitives
+  auto curve_leaves = [&](const std::vector<H>& pts, uint32_t cp, uint32_t e) {
+    struct Part_ {
+      H p[4];
+      double t0, t1;
+      int depth;
+    };
+    std::vector<Part_> todo;
+    Part_ root;
+    for (int i = 0; i < 4; ++i) root.p[i] = pts[i];
+    root.t0 = 0, root.t1 = 1, root.depth = 0;
+    todo.push_back(root);
+    while (!todo.empty()) {
+      Part_ pp = todo.back();
+      todo.pop_back();
+      Box b;
+      for (int i = 0; i < 4; ++i) b.add(point(pp.p[i]));
+      double rho = piece_radius(pp.p);
+      ++P.nodes;
+      if (split_rule(rho, b.diag(), need, pp.depth)) {
+        Part_ l, r;
+        split_curve(pp.p, 0.5, l.p, r.p);
+        double tm = 0.5 * (pp.t0 + pp.t1);
+        l.t0 = pp.t0, l.t1 = tm, r.t0 = tm, r.t1 = pp.t1;
+        l.depth = r.depth = pp.depth + 1;
+        todo.push_back(r), todo.push_back(l);
+        continue;
+      }
+      uint32_t id = uint32_t(cleaf_f.size() / CLEAF_F);
+      cleaf_f.insert(cleaf_f.end(), {rho, pp.t0, pp.t1, 0.0});
+      double cert[PIECE_CERT];
+      piece_radius(pp.p, cert);
+      cleaf_f.insert(cleaf_f.end(), cert, cert + PIECE_CERT);
+      cleaf_u.insert(cleaf_u.end(), {cp, e});
+      pbox.push_back(b);
+      pcode.push_back((P_CLEAF << 28) | id);
+    }
+  };
 
   // ---- edges first (face boxes use them); their records and pieces
   for (uint32_t e = 0; e < part->n_edges; ++e) {
@@ -684,42 +732,7 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
         uint32_t cp = uint32_t(cpiece_f.size() / CPIECE_F);
         for (auto& h : pts) cpiece_f.insert(cpiece_f.end(), h.begin(), h.end());
         cpiece_u.insert(cpiece_u.end(), {e, k, ed.n_pieces, 0});
-        // the piece's tree of parts, breadth first; leaves are primitives
-        struct Part_ {
-          H p[4];
-          double t0, t1;
-          int depth;
-        };
-        std::vector<Part_> todo;
-        Part_ root;
-        for (int i = 0; i < 4; ++i) root.p[i] = pts[i];
-        root.t0 = 0, root.t1 = 1, root.depth = 0;
-        todo.push_back(root);
-        while (!todo.empty()) {
-          Part_ pp = todo.back();
-          todo.pop_back();
-          Box b;
-          for (int i = 0; i < 4; ++i) b.add(point(pp.p[i]));
-          double rho = piece_radius(pp.p);
-          ++P.nodes;
-          if (split_rule(rho, b.diag(), need, pp.depth)) {
-            Part_ l, r;
-            split_curve(pp.p, 0.5, l.p, r.p);
-            double tm = 0.5 * (pp.t0 + pp.t1);
-            l.t0 = pp.t0, l.t1 = tm, r.t0 = tm, r.t1 = pp.t1;
-            l.depth = r.depth = pp.depth + 1;
-            todo.push_back(r), todo.push_back(l);
-            continue;
-          }
-          uint32_t id = uint32_t(cleaf_f.size() / CLEAF_F);
-          cleaf_f.insert(cleaf_f.end(), {rho, pp.t0, pp.t1, 0.0});
-          double cert[PIECE_CERT];
-          piece_radius(pp.p, cert);
-          cleaf_f.insert(cleaf_f.end(), cert, cert + PIECE_CERT);
-          cleaf_u.insert(cleaf_u.end(), {cp, e});
-          pbox.push_back(b);
-          pcode.push_back((P_CLEAF << 28) | id);
-        }
+        curve_leaves(pts, cp, e);
         for (auto& h : pts) bx.add(point(h));
       }
     }
@@ -878,6 +891,27 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
             pole_f.insert(pole_f.end(), {p0.x, p0.y, p0.z, pn.x, pn.y, pn.z, double(fi + 1), 0});
             fpoles.push_back({k, side, p0});
           }
+      }
+      // surface3.rs:1413-1443 c0_lines: an interior knot of multiplicity at least the
+      // degree leaves the surface only C0 along its line, where it may fold; the line's
+      // pieces are the near row (u held) or column (v held) of each span after it
+      for (int k = 0; k < 2; ++k) {
+        const std::vector<std::pair<double, double>>& rr = k == 0 ? ur : vr;
+        const std::vector<double>& kn = k == 0 ? ku : kv;
+        const int deg = int(k == 0 ? fd.du : fd.dv), nspan = k == 0 ? nsu : nsv, others = k == 0 ? nsv : nsu;
+        for (int a = 1; a < nspan; ++a) {
+          const double x = rr[a].first;
+          if (int(std::count(kn.begin(), kn.end(), x)) < deg) continue;
+          Crease c{fi, uint32_t(k), x, {}, {}};
+          for (int b = 0; b < others; ++b) {
+            const int sa = k == 0 ? a * nsv + b : b * nsv + a, sb = k == 0 ? (a - 1) * nsv + b : b * nsv + a - 1;
+            std::vector<H> pts;
+            for (int i = 0; i < 4; ++i) pts.push_back(k == 0 ? fp[sa].p[0][i] : fp[sa].p[i][0]);
+            c.pieces.push_back(pts);
+            c.before.push_back(span0 + uint32_t(sb));
+          }
+          creases.push_back(c);
+        }
       }
       for (int a = 0; a < nsu; ++a)
         for (int b = 0; b < nsv; ++b) {
@@ -1078,6 +1112,26 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
     }
   }
 
+  // solid3.rs:1642-1650 the C0 lines, after the tips as the reference's tree holds
+  // them after the vertices: an edge record each after the part's edges, their pieces,
+  // and their trees of curve leaves
+  for (const Crease& c : creases) {
+    const uint32_t e = uint32_t(edge_u.size() / EDGE_U);
+    double rec[EDGE_F] = {0};
+    uint32_t ru[EDGE_U] = {0};
+    rec[0] = c.value;
+    ru[0] = EDGE_CREASE, ru[1] = ru[2] = c.face, ru[3] = 0xffffffffu;
+    ru[4] = uint32_t(cpiece_f.size() / CPIECE_F), ru[5] = uint32_t(c.pieces.size()), ru[6] = c.k;
+    for (size_t k = 0; k < c.pieces.size(); ++k) {
+      const uint32_t cp = uint32_t(cpiece_f.size() / CPIECE_F);
+      for (const H& h : c.pieces[k]) cpiece_f.insert(cpiece_f.end(), h.begin(), h.end());
+      cpiece_u.insert(cpiece_u.end(), {e, uint32_t(k), uint32_t(c.pieces.size()), c.before[k]});
+      curve_leaves(c.pieces[k], cp, e);
+    }
+    edge_f.insert(edge_f.end(), rec, rec + EDGE_F);
+    edge_u.insert(edge_u.end(), ru, ru + EDGE_U);
+  }
+
   // the poles, after the tips: point records reporting their face
   for (size_t k = 0; k + VERT_F <= pole_f.size(); k += VERT_F) {
     vert_f.insert(vert_f.end(), pole_f.begin() + k, pole_f.begin() + k + VERT_F);
@@ -1109,7 +1163,7 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
   L.among_u = putu(P.U, among_u);
   P.U.push_back(0);
   // uvmap offsets are relative to the uvmap section
-  L.n_faces = part->n_faces, L.n_edges = part->n_edges, L.n_verts = part->n_verts + n_tips;
+  L.n_faces = part->n_faces, L.n_edges = part->n_edges + uint32_t(creases.size()), L.n_verts = part->n_verts + n_tips;
   L.n_spans = uint32_t(span_u.size() / SPAN_U), L.n_leaves = uint32_t(leaf_u.size() / LEAF_U), L.n_cleaves = uint32_t(cleaf_u.size() / CLEAF_U);
   L.n_cpieces = uint32_t(cpiece_u.size() / CPIECE_U);
   L.n_prims = uint32_t(pcode.size());

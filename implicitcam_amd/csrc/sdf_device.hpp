@@ -562,10 +562,55 @@ template <class T> __device__ v3<T> face_normal_at(const Tab<T>& tb, uint32_t fi
   return l > T(0) ? n * (f[14] / l) : n;
 }
 
+// A C0 knot line's pieces (solid3.rs:1818-1886): piece cp of the line of edge record eu
+// runs along the span before the line (its far side) and the span after (its near side);
+// both spans share the piece's parameter range along the line.
+__device__ inline uint32_t crease_before(const Layout& L, const ICAM_K uint32_t* U, uint32_t cp) { return U[L.cpiece_u + CPIECE_U * cp + 3]; }
+
+// The line's point at piece parameter s in the face's parameters: the knot held, the
+// other parameter over the span's range.
+template <class T> __device__ inline void crease_uv(const ICAM_K T* sb, uint32_t k, T s, T& u, T& v) {
+  if (k == 0)
+    u = sb[65], v = sb[66] + s * (sb[67] - sb[66]);
+  else
+    u = sb[64] + s * (sb[65] - sb[64]), v = sb[67];
+}
+
+// Whether the line's point at piece parameter s lies inside its face's loops (m: the
+// margin counted as near).
+template <class T>
+__device__ inline bool crease_inside(const Tab<T>& tb, const ICAM_K uint32_t* eu, uint32_t cp, T s, T m, bool& near) {
+  const Layout& L = tb.L;
+  const uint32_t fi = eu[1];
+  const ICAM_K uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
+  if (fu[2]) return true;
+  const ICAM_K T* sb = tb.F + L.span_f + SPAN_F * crease_before(L, tb.U, cp);
+  T u, v;
+  crease_uv(sb, eu[6], s, u, v);
+  return trim_inside(tb, tb.F + L.face_f + FACE_F * fi + 17, fu[1], u, v, m, near);
+}
+
+// solid3.rs:1878-1885: the line's sign vector at piece parameter s: the unit normals of
+// the span before it at its far side and of the span after it at its near side, summed,
+// times the face's sense.
+template <class T> __device__ v3<T> crease_pn(const Tab<T>& tb, const ICAM_K uint32_t* eu, uint32_t cp, T s) {
+  const Layout& L = tb.L;
+  const uint32_t fi = eu[1], k = eu[6];
+  const ICAM_K uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
+  const uint32_t before = crease_before(L, tb.U, cp), after = before + (k == 0 ? fu[5] : 1u);
+  const SEval<T> eb = k == 0 ? peval(tb.F + L.span_f + SPAN_F * before, T(1), s) : peval(tb.F + L.span_f + SPAN_F * before, s, T(1));
+  const SEval<T> ea = k == 0 ? peval(tb.F + L.span_f + SPAN_F * after, T(0), s) : peval(tb.F + L.span_f + SPAN_F * after, s, T(0));
+  const v3<T> nb = cross(eb.ps, eb.pt), na = cross(ea.ps, ea.pt);
+  const T lb = len(nb), la = len(na);
+  const v3<T> zero = mk3(T(0), T(0), T(0));
+  return ((lb > T(0) ? nb * (T(1) / lb) : zero) + (la > T(0) ? na * (T(1) / la) : zero)) * tb.F[L.face_f + FACE_F * fi + 14];
+}
+
 // An edge's sign vector at its point q (the two faces' outward normals summed).
 template <class T> __device__ v3<T> edge_pn(const Tab<T>& tb, uint32_t e, v3<T> q, uint32_t piece, T tt) {
   const Layout& L = tb.L;
   const ICAM_K uint32_t* eu = tb.U + L.edge_u + EDGE_U * e;
+  if (eu[0] == EDGE_CREASE) return crease_pn(tb, eu, eu[4] + piece, tt);
   const ICAM_K T* uvm = eu[3] != 0xffffffffu ? tb.F + L.uvmap_f + eu[3] + 8 * piece : nullptr;
   v3<T> n0 = face_normal_at(tb, eu[1], q, uvm, tt);
   v3<T> n1 = face_normal_at(tb, eu[2], q, uvm ? uvm + 4 : nullptr, tt);
@@ -1307,6 +1352,8 @@ __device__ inline bool settle_cleaf(const Tab<double>& tb, uint32_t ci, double l
   const uint32_t cp = cu[0];
   const ICAM_K double* pw = tb.F + L.cpiece_f + CPIECE_F * cp;
   const ICAM_K uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cp;
+  const ICAM_K uint32_t* eu = tb.U + L.edge_u + EDGE_U * pu[0];
+  const bool crease = eu[0] == EDGE_CREASE;
   SettleItem q[SETTLE_Q];
   int nq = 1, pops = 0;
   q[0] = SettleItem{cf[1], cf[2], 0.0, 0.0, lb, 0};
@@ -1329,8 +1376,12 @@ __device__ inline bool settle_cleaf(const Tab<double>& tb, uint32_t ci, double l
     const double d = len(r);
     if (d < bd) {
       bd = d;
-      const bool end = (pu[1] == 0 && s <= 0.0) || (pu[1] + 1 == pu[2] && s >= 1.0);
-      if (!end && critical64(p, e.p, r, e.d1)) {
+      // an edge's ends are its vertices', a junction of two of its pieces answers critical
+      // or not; a C0 line's point answers inside its face's loops
+      const bool end = !crease && ((pu[1] == 0 && s <= 0.0) || (pu[1] + 1 == pu[2] && s >= 1.0));
+      const bool junction = !crease && ((pu[1] > 0 && s <= 0.0) || (pu[1] + 1 < pu[2] && s >= 1.0));
+      bool near = false;
+      if (!end && (junction || critical64(p, e.p, r, e.d1)) && (!crease || crease_inside(tb, eu, cp, s, 0.0, near))) {
         v3<double> pn = edge_pn(tb, pu[0], e.p, pu[1], s);
         if (len(pn) < 1e-6) b.unsure = true;
         offer(b, d, (P_CLEAF << 28) | ci, s, 0.0, sgn_of(dot(p - e.p, pn)));
@@ -1371,16 +1422,38 @@ __device__ inline bool settle_cleaf(const Tab<double>& tb, uint32_t ci, double l
 // gpu.wgsl:764-815: a part of a curve piece read by Newton when certified; the minimum
 // answers when it is critical and interior to its edge. With `settle` (double
 // precision), a part past its certificate is settled by the certified search.
+// An edge's nearest point may also be where two of its pieces meet in a corner, where
+// neither piece is critical (curve3.rs:408-476 takes any point interior to the edge): a
+// part ending at such a junction answers with its nearest point there, critical or not,
+// also when it holds no critical point at all. Such an answer is a point of the edge, so
+// it never undercuts the true nearest point, which is read wherever it lies. A C0 knot
+// line's points answer only where critical (solid3.rs:1866-1872).
 template <class T, bool SETTLE = false>
-__device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T eps) {
+__device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps) {
   const Layout& L = tb.L;
   const ICAM_K T* cf = tb.F + L.cleaf_f + CLEAF_F * ci;
   const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * ci;
+  const uint32_t cp = cu[0];
+  const ICAM_K T* pw = tb.F + L.cpiece_f + CPIECE_F * cp;
+  const ICAM_K uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cp;
+  const ICAM_K uint32_t* eu = tb.U + L.edge_u + EDGE_U * pu[0];
+  const bool crease = eu[0] == EDGE_CREASE;
+  T t0 = cf[1], t1 = cf[2];
+  const bool jlo = !crease && pu[1] > 0 && t0 == T(0), jhi = !crease && pu[1] + 1 < pu[2] && t1 == T(1);
   T rho = cf[0];
   bool certified = far_dist(p, box) < T(0.999) * rho - eps;
   if (!certified) {
     const int kind = cleaf_kind(cf + CLEAF_CERT, box, p);
-    if (kind == 1) return;
+    if (kind == 1) {
+      if (!(jlo || jhi)) return;
+      // no critical point: the junction end (the nearer if both are)
+      T s = jlo ? T(0) : T(1);
+      if (jlo && jhi && len(ceval(pw, T(1)).p - p) < len(ceval(pw, T(0)).p - p)) s = T(1);
+      const CEval<T> e = ceval(pw, s);
+      const T d = len(e.p - p);
+      if (d < b.d) offer(b, d, (P_CLEAF << 28) | ci, s, T(0), sgn_of(dot(p - e.p, edge_pn(tb, pu[0], e.p, pu[1], s))));
+      return;
+    }
     certified = kind == 2;
   }
   if constexpr (SETTLE && sizeof(T) == 8) {
@@ -1389,10 +1462,6 @@ __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box
       return;
     }
   }
-  const uint32_t cp = cu[0];
-  const ICAM_K T* pw = tb.F + L.cpiece_f + CPIECE_F * cp;
-  const ICAM_K uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cp;
-  T t0 = cf[1], t1 = cf[2];
   const bool single = sizeof(T) == 4;
   T s = newton1(pw, p, T(0.5) * (t0 + t1), t0, t1, single ? 10 : 40, single ? T(1e-5) : T(1e-15));
   CEval<T> e = ceval(pw, s);
@@ -1400,10 +1469,18 @@ __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box
   T d = len(r);
   if (!certified) b.lmin = mmin(b.lmin, lb);
   if (!(d < b.d)) return;
-  // an end of the edge is its vertex's
-  if ((pu[1] == 0 && s <= T(0)) || (pu[1] + 1 == pu[2] && s >= T(1))) return;
+  // an end of the edge is its vertex's (a C0 line's ends are in its face's boundary,
+  // whose edges answer there)
+  if (!crease && ((pu[1] == 0 && s <= T(0)) || (pu[1] + 1 == pu[2] && s >= T(1)))) return;
   T tol = single ? T(1e-4) * d + eps : T(1e-9) * (T(1) + d);
-  if (!is_critical(r, e.d1, tol)) return;
+  const bool junction = (jlo && s <= T(0)) || (jhi && s >= T(1));
+  if (!junction && !is_critical(r, e.d1, tol)) return;
+  if (crease) {
+    bool near = false;
+    const bool in = crease_inside(tb, eu, cp, s, tm, near);
+    if (near) b.unsure = true;
+    if (!in) return;
+  }
   v3<T> pn = edge_pn(tb, pu[0], e.p, pu[1], s);
   if (len(pn) < T(1e-6)) b.unsure = true;
   offer(b, d, (P_CLEAF << 28) | ci, s, T(0), sgn_of(dot(p - e.p, pn)));
@@ -1429,7 +1506,7 @@ template <class T> __device__ inline void visit(const Tab<T>& tb, uint32_t code,
     case P_EDGE: try_edge(tb, id, p, b); break;
     case P_VERT: try_vert(tb, id, p, b); break;
     case P_LEAF: try_leaf(tb, id, box, lb, p, b, tm, eps); break;
-    default: try_cleaf(tb, id, box, lb, p, b, eps); break;
+    default: try_cleaf(tb, id, box, lb, p, b, tm, eps); break;
   }
 }
 

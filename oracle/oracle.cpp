@@ -751,6 +751,7 @@ struct Nurbs {
   int du, dv, nsu, nsv;
   std::vector<Patch> spans;  // span (a, b) at a * nsv + b
   std::vector<double> su0, sv0;  // span starts, plus the end
+  std::vector<double> knots_u, knots_v;
   Box box;
 
   void build(const icam_face_desc& f) {
@@ -793,6 +794,7 @@ struct Nurbs {
         pa.u0 = ur[a].first, pa.u1 = ur[a].second, pa.v0 = vr[b].first, pa.v1 = vr[b].second;
         box.add(pa.bbox());
       }
+    knots_u = ku, knots_v = kv;
     for (auto& r : ur) su0.push_back(r.first);
     su0.push_back(ur.back().second);
     for (auto& r : vr) sv0.push_back(r.first);
@@ -1339,6 +1341,77 @@ V3 fan_normal(const Nurbs& ns, const Trim& trim, int k, double value) {
   return out;
 }
 
+// surface3.rs:734-760 curve_no_critical_point: |C - p|^2 has no critical point on a
+// rational Bezier curve (positive weights) when w Q.Q' - w' Q.Q, Q = P - p w, has
+// Bernstein coefficients of one strict sign past a rounding allowance
+bool curve_no_critical_point(const Bez3& z, V3 p) {
+  const int m = z.deg;
+  auto mk = [&](int c) {
+    Bern2 b{m, 0, std::vector<double>(m + 1)};
+    for (int i = 0; i <= m; ++i) b.c[i] = c == 3 ? z.pw[i][3] : z.pw[i][c] - (c == 0 ? p.x : c == 1 ? p.y : p.z) * z.pw[i][3];
+    return b;
+  };
+  const Bern2 w = mk(3);
+  for (double x : w.c)
+    if (!(x > 0)) return false;
+  const Bern2 q[3] = {mk(0), mk(1), mk(2)};
+  const Bern2 qu[3] = {d_u(q[0]), d_u(q[1]), d_u(q[2])};
+  auto dotb = [](const Bern2* x, const Bern2* y) {
+    Bern2 s = bmul(x[0], y[0]);
+    for (int k = 1; k < 3; ++k) s = axpy(s, 1.0, bmul(x[k], y[k]), 1.0);
+    return s;
+  };
+  const Bern2 wu = d_u(w);
+  const Bern2 f = axpy(bmul(w, dotb(q, qu)), 1.0, bmul(wu, dotb(q, q)), -1.0);
+  double q0 = 0, q1 = 0;
+  for (int k = 0; k < 3; ++k) q0 = std::max(q0, max_abs(q[k])), q1 = std::max(q1, max_abs(qu[k]));
+  const double size = max_abs(w) * 3.0 * q0 * q1 + max_abs(wu) * 3.0 * q0 * q0;
+  double lo = INF, hi = -INF;
+  for (double x : f.c) lo = std::min(lo, x), hi = std::max(hi, x);
+  return lo > 1e-11 * size || hi < -1e-11 * size;
+}
+
+// surface3.rs:1413-1443 c0_lines: the interior knot lines of multiplicity at least the
+// degree, where the surface is only C0 and may fold: per line the direction held (0: u),
+// its value, and one piece per span of the other direction (the near row or column of
+// the patch after the line, parameters along the line) with the spans on either side
+struct C0Piece {
+  Bez3 z;
+  int before, after;
+};
+struct C0Line {
+  int face = 0, k = 0;
+  double value = 0;
+  std::vector<C0Piece> pieces;
+};
+std::vector<C0Line> c0_lines(const Nurbs& ns, int face) {
+  std::vector<C0Line> out;
+  for (int k = 0; k < 2; ++k) {
+    const std::vector<double>& s0 = k == 0 ? ns.su0 : ns.sv0;
+    const std::vector<double>& kn = k == 0 ? ns.knots_u : ns.knots_v;
+    const int deg = k == 0 ? ns.du : ns.dv, nspan = k == 0 ? ns.nsu : ns.nsv, others = k == 0 ? ns.nsv : ns.nsu;
+    for (int a = 1; a < nspan; ++a) {
+      const double x = s0[a];
+      if (!(x > s0.front() && x < s0.back()) || std::count(kn.begin(), kn.end(), x) < deg) continue;
+      C0Line l;
+      l.face = face, l.k = k, l.value = x;
+      for (int b = 0; b < others; ++b) {
+        C0Piece pc;
+        pc.before = k == 0 ? (a - 1) * ns.nsv + b : b * ns.nsv + a - 1;
+        pc.after = k == 0 ? a * ns.nsv + b : b * ns.nsv + a;
+        const Patch& af = ns.spans[pc.after];
+        pc.z.deg = k == 0 ? af.n : af.m;
+        for (int i = 0; i <= pc.z.deg; ++i)
+          for (int c = 0; c < 4; ++c) pc.z.pw[i][c] = k == 0 ? af.pw[0][i][c] : af.pw[i][0][c];
+        pc.z.t0 = k == 0 ? af.v0 : af.u0, pc.z.t1 = k == 0 ? af.v1 : af.u1;
+        l.pieces.push_back(pc);
+      }
+      out.push_back(l);
+    }
+  }
+  return out;
+}
+
 struct Exact {
   std::vector<Face> faces;
   std::vector<Edge> edges;
@@ -1350,6 +1423,8 @@ struct Exact {
     V3 x, n;
   };
   std::vector<Pole> poles;
+  // solid3.rs:1549-1568 the interior C0 knot lines of B-spline faces, searched like edges
+  std::vector<C0Line> lines;
   double tol;
   BoxTree tree;
   Box part;
@@ -1488,10 +1563,11 @@ struct Exact {
         }
       }
     }
-    // solid3.rs:1609-1625 the poles
+    // solid3.rs:1609-1625 the C0 lines and the poles
     for (size_t fi = 0; fi < faces.size(); ++fi) {
       const Face& f = faces[fi];
       if (f.kind != ICAM_NURBS) continue;
+      for (C0Line& l : c0_lines(*f.ns, int(fi))) lines.push_back(l);
       for (const Collapsed& c : collapsed_sides(*f.ns)) {
         bool near_vertex = false;
         for (auto& v : verts) near_vertex |= len(v.first - c.x) <= tol;
@@ -1508,6 +1584,12 @@ struct Exact {
     for (auto& v : verts) {
       Box b;
       b.add(v.first);
+      boxes.push_back(b);
+    }
+    // solid3.rs:1642-1650 a line's box holds its pieces' control points
+    for (auto& l : lines) {
+      Box b;
+      for (auto& pc : l.pieces) b.add(pc.z.box());
       boxes.push_back(b);
     }
     for (auto& pl : poles) {
@@ -1559,6 +1641,76 @@ struct Exact {
     return true;
   }
 
+  // solid3.rs:1818-1886 c0_line: the nearest point of a C0 line inside its face below
+  // cutoff where the distance along the line is critical (where the line meets the
+  // face's boundary the edges answer), best first over halves of its pieces, and the
+  // two sides' outward normals summed there
+  bool c0_line(const C0Line& l, V3 p, double cutoff, double& dist, V3& q, V3& n, bool& capped) const {
+    const Face& f = faces[l.face];
+    const Trim* t = f.trim.pieces.empty() ? nullptr : &f.trim;
+    auto at = [&](double w, double uv[2]) { uv[0] = l.k == 0 ? l.value : w, uv[1] = l.k == 0 ? w : l.value; };
+    struct Item {
+      double lb;
+      int i;
+      Bez3 z;
+      int depth;
+    };
+    std::vector<Item> heap;
+    for (size_t i = 0; i < l.pieces.size(); ++i) {
+      const double lb = box_dist(p, l.pieces[i].z.box());
+      if (lb < cutoff) heap.push_back({lb, int(i), l.pieces[i].z, 0});
+    }
+    double bound = cutoff, bw = 0;
+    int bi = -1, pops = 0;
+    auto big = [](V3 x) { return std::max(std::fabs(x.x), std::max(std::fabs(x.y), std::fabs(x.z))); };
+    while (!heap.empty()) {
+      size_t k = 0;
+      for (size_t j = 1; j < heap.size(); ++j)
+        if (heap[j].lb < heap[k].lb) k = j;
+      const Item it = heap[k];
+      heap[k] = heap.back();
+      heap.pop_back();
+      if (it.lb >= bound - DIST_EPS) break;
+      if (++pops > 4096) {
+        capped = true;
+        break;
+      }
+      double a[2], b[2];
+      at(it.z.t0, a), at(it.z.t1, b);
+      const double lo[2] = {std::min(a[0], b[0]), std::min(a[1], b[1])}, hi[2] = {std::max(a[0], b[0]), std::max(a[1], b[1])};
+      const int state = t ? t->box_class(lo, hi) : 1;
+      if (state == 0) continue;
+      const double s = it.z.newton(p, 0.5);
+      const Bez3::E e = it.z.eval(s);
+      const double d = len(e.p - p), w = it.z.t0 + s * (it.z.t1 - it.z.t0);
+      double uv[2];
+      at(w, uv);
+      if (d < bound && (state == 1 || t->inside(uv[0], uv[1]))) {
+        bound = d;
+        const V3 r = e.p - p;
+        const double round = 1024.0 * std::numeric_limits<double>::epsilon() * (big(e.p) + big(p) + d);
+        if (std::fabs(dot(r, e.d1)) <= (round + 1e-12 * len(e.d1)) * len(e.d1)) dist = d, q = e.p, bi = it.i, bw = w;
+      }
+      if (it.depth >= 40 || curve_no_critical_point(it.z, p)) continue;
+      Bez3 c[2];
+      it.z.split(0.5, c[0], c[1]);
+      for (const Bez3& ch : c) {
+        const double lbc = box_dist(p, ch.box());
+        if (lbc < bound - DIST_EPS) heap.push_back({lbc, it.i, ch, it.depth + 1});
+      }
+    }
+    if (bi < 0) return false;
+    // the patch before the line at its far side, the patch after at its near side
+    const C0Piece& pc = l.pieces[bi];
+    auto side_normal = [&](const Patch& pa, double side) {
+      const double s = l.k == 0 ? side : (bw - pa.u0) / (pa.u1 - pa.u0), tt = l.k == 0 ? (bw - pa.v0) / (pa.v1 - pa.v0) : side;
+      const SEval e = pa.eval(s, tt);
+      return unit(cross(e.ps, e.pt));
+    };
+    n = (side_normal(f.ns->spans[pc.before], 1.0) + side_normal(f.ns->spans[pc.after], 0.0)) * f.sense;
+    return true;
+  }
+
   // solid3.rs:1750-1816 sdf_within
   Sd sdf_within(V3 p, double bound) const {
     int nf = int(faces.size()), ne = int(edges.size());
@@ -1584,9 +1736,15 @@ struct Exact {
         int vi = id - nf - ne;
         double d = len(p - verts[vi].first);
         if (d < cur) best = {d * sgn(dot(p - verts[vi].first, verts[vi].second)), verts[vi].first, (2u << 28) | uint32_t(vi), false, false};
+      } else if (id < nf + ne + int(verts.size() + lines.size())) {
+        // solid3.rs:1790-1794 a C0 line: its face's answer
+        const C0Line& l = lines[id - nf - ne - int(verts.size())];
+        double d;
+        V3 q, n;
+        if (c0_line(l, p, cur, d, q, n, capped)) best = {d * sgn(dot(p - q, n)), q, uint32_t(l.face), false, false};
       } else {
         // solid3.rs:1794-1799 a pole: its face's answer
-        const Pole& pl = poles[id - nf - ne - int(verts.size())];
+        const Pole& pl = poles[id - nf - ne - int(verts.size() + lines.size())];
         double d = len(p - pl.x);
         if (d < cur) best = {d * sgn(dot(p - pl.x, pl.n)), pl.x, uint32_t(pl.face), false, false};
       }

@@ -340,3 +340,32 @@ def c3_near_trim_points(info, n, radius, seed):
         out[m, :2] = q + v[:, :2]
         out[m, 2] = T / 2 + v[:, 2]
     return out
+
+
+def crease_points(o, k, h, ws):
+    """Points at distance h from the C0 knot line of a "crease:<k>" part's top (face 0),
+    each on the bisector of the two sides' normals at a point of the line, so that the
+    distance along the line is critical there: outside over a ridge (k = 0, 1, expected
+    +h), inside under a valley (k = 2, 3, expected -h); the other side sees the faces.
+    Normals from one-sided differences of the surface across the line (knot 0.5 along u
+    for even k, along v for odd k), at line parameters ws. Returns (points, expected)."""
+    held = k & 1
+
+    def at(a, w):
+        return np.array(o.nurbs_eval(0, a, w) if held == 0 else o.nurbs_eval(0, w, a))
+
+    pts, exp = [], []
+    e = 1e-6
+    for w in ws:
+        q = at(0.5, w)
+        t = at(0.5, w + e) - at(0.5, w - e)
+        a, b = q - at(0.5 - e, w), at(0.5 + e, w) - q
+        na, nb = (np.cross(a, t), np.cross(b, t)) if held == 0 else (np.cross(t, a), np.cross(t, b))
+        na, nb = na / np.linalg.norm(na), nb / np.linalg.norm(nb)
+        if na[2] < 0:
+            na, nb = -na, -nb
+        m = (na + nb) / np.linalg.norm(na + nb)
+        s = 1.0 if k < 2 else -1.0
+        pts.append(q + s * h * m)
+        exp.append(s * h)
+    return np.array(pts), np.array(exp)

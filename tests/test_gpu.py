@@ -304,6 +304,37 @@ def test_dome_pole(k, prec):
 
 
 @pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("k", [0, 1, 2, 3])
+def test_crease_lines(k, prec):
+    """A top folded along a C0 knot line (multiplicity three; a ridge for k = 0, 1, a
+    valley for k = 2, 3; u held for even k): points on the fold's bisectors answer at the
+    line's exact distance, reported as the top's face; uniform points and points around
+    the line against the oracle; a level bit-exact with the oracle's bake."""
+    from closed_forms import crease_points
+
+    part = Part(f"crease:{k}")
+    o = Oracle(part)
+    g = PartGpu(part, 12 * 0.5)
+    pts, exp = crease_points(o, k, 0.3, np.linspace(0.15, 0.85, 29))
+    d, flags, feat = g.query(pts, prec)
+    assert (flags == 0).all() and (feat == 0).all()
+    assert np.abs(d - exp).max() <= max(tol(prec, part), 1e-9)
+    rng = np.random.default_rng(23 + k)
+    near, _ = crease_points(o, k, 1.0, rng.uniform(0.02, 0.98, 512))
+    near = near + rng.normal(0, 0.7, near.shape)
+    pts = np.concatenate([uniform_points(SEED + k, 1 << 14, part.lo - 5, part.hi + 5), near])
+    d, flags, _ = g.query(pts, prec)
+    dref, _, _ = o.sdf(pts)
+    assert (flags == 0).all()
+    assert np.abs(d - dref).max() <= tol(prec, part)
+    assert ((d < 0) == (dref < 0))[np.abs(dref) > 1e-8].all()
+    lv = level_for(part, part.diagonal / 64, 8)
+    m_ref, a_ref, _ = o.bake(lv)
+    m, a, _ = g.bake(lv, prec)
+    assert (m == m_ref).all() and (a == a_ref).all()
+
+
+@pytest.mark.parametrize("prec", PRECS)
 def test_c4_parts_bake_like_oracle(prec):
     """Every part of the batched workload (turned parts, NURBS blocks with 4 x 4 to
     32 x 32 nets, plates with up to ~200 holes) baked on a 64^3 node grid: map and

@@ -347,6 +347,49 @@ def test_dome_pole_closed_form(built, k):
     assert abs(dt[0] - 2.0) < 1e-12 and ft[0] == 2
 
 
+@pytest.mark.parametrize("k", [0, 1, 2, 3])
+def test_crease_closed_form(built, k):
+    """A rational bicubic top with a knot of multiplicity three (the degree) at 0.5: the
+    surface folds along that C0 line (a ridge for k = 0, 1, a valley for k = 2, 3; u held
+    for even k). Points on the bisector of the two sides' normals at distance h from the
+    line: the line answers, at exactly h, reported as the top's face (solid3.rs:1790-1794,
+    1818-1886); without the line search no face or edge would (each side's nearest point is
+    on its boundary and not critical). Then the c2 block's brute-force bounds and the height
+    field's sign on uniform points."""
+    from closed_forms import crease_points
+
+    part = Part(f"crease:{k}")
+    o = Oracle(part)
+    pts, exp = crease_points(o, k, 0.3, np.linspace(0.15, 0.85, 29))
+    d, feat, flags = o.sdf(pts)
+    assert np.abs(d - exp).max() < 1e-9
+    assert (feat == 0).all() and not flags.any()
+    top = _c2_surface_samples(o, 240)
+    side = []
+    for t in np.linspace(0, 1, 241):
+        for q in (o.nurbs_eval(0, t, 0), o.nurbs_eval(0, t, 1), o.nurbs_eval(0, 0, t), o.nurbs_eval(0, 1, t)):
+            for f in np.linspace(0, 1, 61):
+                side.append([q[0], q[1], -20 + (q[2] + 20) * f])
+    g = np.linspace(-60, 60, 241)
+    gx, gy = np.meshgrid(g, g)
+    bottom = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, -20.0)], 1)
+    samples = np.concatenate([top, np.array(side), bottom])
+    pts = uniform_points(11 + k, 2000, part.lo - 5, part.hi + 5)
+    d, _, flags = o.sdf(pts)
+    dense, _ = cKDTree(samples).query(pts)
+    assert not (flags & 1).any()
+    assert (np.abs(d) <= dense + 1e-9).all()
+    assert (np.abs(d) >= dense - 1.0).all()
+    from scipy.interpolate import LinearNDInterpolator
+
+    height = LinearNDInterpolator(top[:, :2], top[:, 2])
+    inner = (np.abs(pts[:, 0]) < 59) & (np.abs(pts[:, 1]) < 59)
+    zt = height(pts[inner, 0], pts[inner, 1])
+    inside = (pts[inner, 2] > -20) & (pts[inner, 2] < zt)
+    clear = np.abs(pts[inner, 2] - zt) > 0.1
+    assert ((d[inner] < 0) == inside)[clear].all()
+
+
 @pytest.mark.parametrize("holes", [1, 12, 40])
 def test_plate_closed_form(built, holes):
     """Plates with plain, filleted (torus) and countersunk (cone) holes against per-face
