@@ -24,6 +24,8 @@
 //           interior weights uniform in [0.8, 1.2]
 //   "c2flat" the same with every control point at the top's base height: its top
 //           is the plane z = 20 and the part the box [-60, 60]^2 x [-20, 20]
+//   "oblique:<k>" a cylinder on a disc cut at the top by an oblique plane: an ellipse
+//           edge (radius, height and angle from the seed and k)
 //   "crease:<k>" the c2 block on a 17 x 17 net with a knot of multiplicity three at
 //           0.5 along u (k even) or v (k odd): the top folds along that knot line, a
 //           ridge of height 12 (a valley for k = 2, 3) fading to zero at the boundary
@@ -978,6 +980,63 @@ Owned* make_dome(double R, double L) {
   return o;
 }
 
+// A cylinder of radius R on a disc at z = 0, cut at the top by the plane
+// z = L + tan(a) x: the cut is an ellipse edge (Edge3::Ellipse, curve3.rs:494) of
+// semi-axes R / cos(a) along the plane's x and R along y, between the plane and the
+// cylinder. The cylinder's loops in its parameters (quarter-arc parameter, height) are
+// the bottom line and the cut's curve as 256 cubic pieces (Hermite in the arc
+// parameter, within 1e-8 mm); the top plane's loop is the ellipse as 64 cubic pieces.
+// Closed forms: the solid is convex (info: 4, R, L, tan a).
+Owned* make_oblique(double R, double L, double ta) {
+  Owned* o = new Owned();
+  Builder b{o};
+  const V X{1, 0, 0}, Z{0, 0, 1};
+  const double ca = 1.0 / std::sqrt(1.0 + ta * ta), sa = ta * ca, ea = R / ca;
+  const V xt{ca, 0, sa}, zt{-sa, 0, ca};  // the cut plane's x axis and normal (its y is Y)
+  // the cylinder: the cut's height over the quarter-arc parameter (trim2.rs:1789-1807):
+  // on quarter k, u = k + (tan(b / 2) / tan(pi / 8) + 1) / 2 with b the angle from the
+  // quarter's middle
+  std::vector<Piece> band = {line_piece(0, 0, 4, 0)};
+  const double t8 = std::tan(M_PI / 8);
+  auto beta = [&](int k, double u) { return 2.0 * std::atan((2.0 * (u - k) - 1.0) * t8); };
+  auto theta = [&](int k, double u) { return 0.5 * M_PI * k + 0.25 * M_PI + beta(k, u); };
+  auto height = [&](double th) { return L + R * ta * std::cos(th); };
+  auto slope = [&](int k, double u) {  // dv / du
+    const double c = std::cos(0.5 * beta(k, u));
+    return -R * ta * std::sin(theta(k, u)) * 4.0 * t8 * c * c;
+  };
+  for (int k = 0; k < 4; ++k)
+    for (int j = 0; j < 64; ++j) {
+      const double ua = k + j / 64.0, ub = k + (j + 1) / 64.0, du = ub - ua;
+      const double va = height(theta(k, ua)), vb = height(theta(k, ub));
+      add_monotone(band, {ua, va, ua + du / 3, va + du / 3 * slope(k, ua), ub - du / 3, vb - du / 3 * slope(k, ub), ub, vb});
+    }
+  const double top = L + R * std::fabs(ta);
+  const uint32_t cyl = b.analytic(ICAM_CYLINDER, {0, 0, 0}, Z, X, R, 0, 1.0, 0, -1, 4, top + 1, 4, 0, band, 2.0, 0.5 * (L - R * std::fabs(ta)), true);
+  std::vector<Piece> disc;
+  add_circle(disc, 0, 0, R);
+  const uint32_t bot = b.plane({0, 0, 0}, Z, X, -1.0, -R - 1, -R - 1, R + 1, R + 1, disc, -R - 0.5, 0.0123, false);
+  std::vector<Piece> unit_circle, ell;
+  add_circle(unit_circle, 0, 0, 1.0);
+  for (const Piece& c : unit_circle) ell.push_back({ea * c[0], R * c[1], ea * c[2], R * c[3], ea * c[4], R * c[5], ea * c[6], R * c[7]});
+  const uint32_t cut = b.plane({0, 0, L}, zt, xt, 1.0, -ea - 1, -R - 1, ea + 1, R + 1, ell, -ea - 0.5, 0.0123, false);
+  b.circle({0, 0, 0}, Z, X, R, bot, cyl);
+  icam_edge_desc e;
+  std::memset(&e, 0, sizeof(e));
+  e.kind = ICAM_ELLIPSE;
+  put3(e.o, V{0, 0, L}), put3(e.x, xt), put3(e.z, zt);
+  e.r = ea, e.r2 = R, e.t0 = 0.0, e.sweep = 2.0 * M_PI;
+  e.n_pieces = 1;
+  e.faces[0] = cut, e.faces[1] = cyl;
+  o->edges.push_back(e);
+  // the seam vertices (angle 0), each between two faces at angle pi
+  b.vert({R, 0, 0}, V{1, 0, -1} * M_PI);
+  b.vert({R, 0, L + R * ta}, (X + zt) * M_PI);
+  o->info = {4.0, R, L, ta};
+  o->finish({-R, -R, 0}, {R, R, top}, 1e-9);
+  return o;
+}
+
 // Profiles of turned parts, each with g grooves (concave tori) cut into its
 // cylinder; sizes in mm from the seed.
 //   0 capsule: hemisphere, cylinder, hemisphere (sphere poles on the axis)
@@ -1416,6 +1475,11 @@ const icam_part_desc* icam_gen_part(const char* name, uint64_t seed) {
   }
   if (s.rfind("nurbs:", 0) == 0) o = make_c2(seed, false, std::min(std::max(arg(6), 4), 32), 40, 3);
   if (s.rfind("crease:", 0) == 0) o = make_c2(seed, false, 17, 60, 5, arg(7) & 3);
+  if (s.rfind("oblique:", 0) == 0) {
+    Rng r2{(seed ^ (0x9E3779B97F4A7C15ull * uint64_t(arg(8) + 1))) | 1};
+    const double R = 5 + 10 * r2.next(), L = 2 * R + 20 * r2.next(), a = (15 + 45 * r2.next()) * M_PI / 180;
+    o = make_oblique(R, L, std::tan(a));
+  }
   return o ? &o->desc : nullptr;
 }
 

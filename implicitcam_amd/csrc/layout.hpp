@@ -26,11 +26,13 @@ enum : uint32_t {
   FACE_U = 8,    // kind 0, root cell 1, whole 2 (no pieces), narc 3
 };
 // EDGE: line (a 0-2, b 3-5) or arc (o 0-2, x 3-5, y 6-8, r 9, t0 10, sweep 11, full 12);
-// U: kind 0, face0 1, face1 2, uvmap offset 3 (F), first cpiece 4, n 5. After the part's
+// U: kind 0, face0 1, face1 2, uvmap offset 3 (F), first cpiece 4, n 5. Spline and
+// ellipse edges are searched through their pieces (an ellipse as its rational quadratic
+// arcs of at most a quarter turn, curve3.rs:986-1008). After the part's
 // edges, one record of kind EDGE_CREASE per interior C0 knot line of a B-spline face:
 // knot 0; U: both faces the line's face, its pieces (4, 5) as a spline edge's, the
 // direction held 6 (0: u); each of its pieces holds the span before the line in U word 3
-enum : uint32_t { EDGE_F = 16, EDGE_U = 8, EDGE_CREASE = 3 };
+enum : uint32_t { EDGE_F = 16, EDGE_U = 8, EDGE_CREASE = 4 };
 // VERT: p 0-2, pn 3-5
 enum : uint32_t { VERT_F = 8 };
 // LEAF (sub-patch of a span): rho 0, s0 1, s1 2, t0 3, t1 4, slab normal 5-7, slab lo 8, hi 9

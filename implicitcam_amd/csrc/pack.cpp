@@ -719,6 +719,30 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
         double ex = std::sqrt(std::max(1 - dot(z, ax) * dot(z, ax), 0.0)) * ed.r;
         bx.add(c + ax * ex), bx.add(c - ax * ex);
       }
+    } else if (ed.kind == ICAM_ELLIPSE) {
+      // curve3.rs:986-1008 bez_pieces: k arcs of at most a quarter turn, each the rational
+      // quadratic through its ends with its middle point pushed out by 1 / cos(half its
+      // turn) and weighted cos(half its turn)
+      if (ed.uvmap) {
+        P.error = "ellipse edge on a B-spline face (its pieces carry no parameter map)";
+        return false;
+      }
+      V c = rel(ed.o), x = of(ed.x), z = of(ed.z), y = cross(z, x);
+      const int k = std::max(1, int(std::ceil(std::fabs(ed.sweep) / (0.5 * PI))));
+      const double dt = ed.sweep / k, phi = 0.5 * dt, w = std::cos(phi);
+      auto at = [&](double t, double sc) { return c + x * (ed.r * std::cos(t) * sc) + y * (ed.r2 * std::sin(t) * sc); };
+      ru[4] = uint32_t(cpiece_f.size() / CPIECE_F), ru[5] = uint32_t(k);
+      for (int i = 0; i < k; ++i) {
+        const double ta = ed.t0 + dt * i;
+        const V a = at(ta, 1.0), m = at(ta + phi, 1.0 / w), b = at(ta + dt, 1.0);
+        std::vector<H> pts = {H{a.x, a.y, a.z, 1.0}, H{m.x * w, m.y * w, m.z * w, w}, H{b.x, b.y, b.z, 1.0}};
+        pts = raise(pts);
+        const uint32_t cp = uint32_t(cpiece_f.size() / CPIECE_F);
+        for (auto& h : pts) cpiece_f.insert(cpiece_f.end(), h.begin(), h.end());
+        cpiece_u.insert(cpiece_u.end(), {e, uint32_t(i), uint32_t(k), 0});
+        curve_leaves(pts, cp, e);
+        for (auto& h : pts) bx.add(point(h));
+      }
     } else {
       if (ed.degree > 3 || ed.degree < 1) {
         P.error = "spline edge of degree " + std::to_string(ed.degree) + " (at most 3 are packed)";
@@ -1061,7 +1085,7 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
   // ---- edge and vertex primitives (after the faces: their order in the list
   // follows the reference's tree ids, faces, edges, vertices)
   for (uint32_t e = 0; e < part->n_edges; ++e)
-    if (part->edges[e].kind != ICAM_SPLINE) {
+    if (part->edges[e].kind == ICAM_LINE || part->edges[e].kind == ICAM_ARC) {
       pbox.push_back(edge_box[e]);
       pcode.push_back((P_EDGE << 28) | e);
     }

@@ -52,7 +52,7 @@ enum icam_surf_kind {
   ICAM_NURBS = 5
 };
 
-enum icam_edge_kind { ICAM_LINE = 0, ICAM_ARC = 1, ICAM_SPLINE = 2 };
+enum icam_edge_kind { ICAM_LINE = 0, ICAM_ARC = 1, ICAM_SPLINE = 2, ICAM_ELLIPSE = 3 };
 
 typedef struct icam_face_desc {
   uint32_t kind;      /* icam_surf_kind */
@@ -96,6 +96,9 @@ typedef struct icam_edge_desc {
    * edge point; the point itself is on the surface). NULL when no face is NURBS.
    * Line and arc edges count as one piece. */
   const double* uvmap;
+  /* ellipse (Edge3::Ellipse, curve3.rs:494): o + r cos t x + r2 sin t y in the arc
+   * frame, t from t0 through sweep (r along x, r2 along y = z x x) */
+  double r2;
 } icam_edge_desc;
 
 typedef struct icam_vert_desc {

@@ -1159,7 +1159,7 @@ struct Edge {
   int kind;
   V3 a, b;
   Frame f;
-  double r, t0, sweep;
+  double r, r2 = 0, t0, sweep;
   Spline spline;
   int faces[2];
   const double* uvmap = nullptr;  // per piece, per face (u0 v0 u1 v1)
@@ -1174,6 +1174,36 @@ struct Edge {
   // curve3.rs:525-562 (line, arc) and :605-610 closest_below
   bool closest_below(V3 p, double cutoff, Foot& out) const {
     if (kind == ICAM_SPLINE) return spline.closest(p, cutoff, out) && out.dist < cutoff;
+    if (kind == ICAM_ELLIPSE) {
+      // curve3.rs:557-590 in the ellipse's plane: the critical points of |E(t) - p|^2 (at
+      // most four) by Newton from 64 seeds over the arc, and the two ends
+      const V3 l = f.local(p);
+      const double a = r, b = r2;
+      auto e = [&](double t) { return V3(a * std::cos(t), b * std::sin(t), 0); };
+      auto dist = [&](double t) { return len(e(t) - l); };
+      double bd = dist(t0), bt = t0;
+      Near near = START;
+      const double te = t0 + sweep;
+      if (dist(te) < bd) bd = dist(te), bt = te, near = END;
+      for (int k = 0; k < 64; ++k) {
+        double t = t0 + sweep * (k + 0.5) / 64.0;
+        for (int it = 0; it < 30; ++it) {
+          const double sn = std::sin(t), cs = std::cos(t);
+          const double g = (b * b - a * a) * sn * cs + a * l.x * sn - b * l.y * cs;
+          const double gd = (b * b - a * a) * (cs * cs - sn * sn) + a * l.x * cs + b * l.y * sn;
+          if (std::fabs(gd) < 1e-300) break;
+          const double nt = t - g / gd;
+          if (std::fabs(nt - t) < 1e-15) {
+            t = nt;
+            break;
+          }
+          t = nt;
+        }
+        if (swept(t0, t, sweep) <= std::fabs(sweep) && dist(t) < bd) bd = dist(t), bt = t, near = INTERIOR;
+      }
+      out = {f.world(e(bt)), bd, bt, near};
+      return out.dist < cutoff;
+    }
     if (kind == ICAM_LINE) {
       V3 ab = b - a;
       double l2 = dot(ab, ab);
@@ -1477,6 +1507,16 @@ struct Exact {
       if (ed.kind == ICAM_LINE) {
         e.a = V3::of(ed.a), e.b = V3::of(ed.b);
         e.box.add(e.a), e.box.add(e.b);
+      } else if (ed.kind == ICAM_ELLIPSE) {
+        e.f.o = V3::of(ed.o), e.f.x = V3::of(ed.x), e.f.z = V3::of(ed.z);
+        e.f.y = cross(e.f.z, e.f.x);
+        e.r = ed.r, e.r2 = ed.r2, e.t0 = ed.t0, e.sweep = ed.sweep;
+        // curve3.rs:698-704 the box of the whole ellipse
+        V3 axes[3] = {V3(1, 0, 0), V3(0, 1, 0), V3(0, 0, 1)};
+        for (V3 ax : axes) {
+          double ex = std::hypot(e.r * dot(e.f.x, ax), e.r2 * dot(e.f.y, ax));
+          e.box.add(e.f.o + ax * ex), e.box.add(e.f.o - ax * ex);
+        }
       } else if (ed.kind == ICAM_ARC) {
         e.f.o = V3::of(ed.o), e.f.x = V3::of(ed.x), e.f.z = V3::of(ed.z);
         e.f.y = cross(e.f.z, e.f.x);

@@ -369,3 +369,60 @@ def crease_points(o, k, h, ws):
         pts.append(q + s * h * m)
         exp.append(s * h)
     return np.array(pts), np.array(exp)
+
+
+# ---------------------------------------------------------------- the oblique cut
+#
+# A cylinder of radius R on a disc at z = 0, cut by the plane z = L + t x: a convex
+# solid, so an inside point's distance is its nearest face's (plane or cylinder); the
+# cut is an ellipse edge.
+
+
+def oblique_frame(t):
+    c = 1.0 / np.sqrt(1.0 + t * t)
+    return c, t * c
+
+
+def oblique_inside(p, R, L, t):
+    """(inside, depth) of the oblique cut: the depth is the nearest face's distance."""
+    c, _ = oblique_frame(t)
+    rho = np.hypot(p[:, 0], p[:, 1])
+    cut = (L + t * p[:, 0] - p[:, 2]) * c
+    inside = (p[:, 2] > 0) & (cut > 0) & (rho < R)
+    return inside, np.minimum(np.minimum(p[:, 2], cut), R - rho)
+
+
+def oblique_edge_points(R, L, t, h, thetas):
+    """Points at distance h outside the ellipse edge, each on the bisector of the
+    cylinder's and the cut plane's outward normals at a point of the edge (the edge's
+    tangent is normal to both, and the solid is convex there): expected +h."""
+    c, s = oblique_frame(t)
+    pts = []
+    for th in thetas:
+        q = np.array([R * np.cos(th), R * np.sin(th), L + R * t * np.cos(th)])
+        m = np.array([np.cos(th), np.sin(th), 0.0]) + np.array([-s, 0.0, c])
+        pts.append(q + h * m / np.linalg.norm(m))
+    return np.array(pts), np.full(len(thetas), h)
+
+
+def oblique_boundary_samples(R, L, t, step):
+    """Points of the oblique cut's boundary at most `step` apart."""
+    c, _ = oblique_frame(t)
+    n = int(np.ceil(2 * np.pi * R / step))
+    out = []
+    for th in np.linspace(0, 2 * np.pi, n, endpoint=False):
+        x, y = R * np.cos(th), R * np.sin(th)
+        ztop = L + t * x
+        for z in np.linspace(0, ztop, int(np.ceil(ztop / step)) + 1):
+            out.append([x, y, z])
+    g = np.arange(-R, R + step, step)
+    gx, gy = np.meshgrid(g, g)
+    keep = np.hypot(gx, gy) <= R
+    disc = np.stack([gx[keep], gy[keep]], 1)
+    out.extend([[x, y, 0.0] for x, y in disc])
+    # the cut: the disc seen along z, lifted on to the plane (spacing grows by 1 / c)
+    g2 = np.arange(-R, R + step * c, step * c)
+    gx, gy = np.meshgrid(g2, g2)
+    keep = np.hypot(gx, gy) <= R
+    out.extend([[x, y, L + t * x] for x, y in zip(gx[keep], gy[keep])])
+    return np.array(out)

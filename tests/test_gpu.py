@@ -335,6 +335,40 @@ def test_crease_lines(k, prec):
 
 
 @pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_oblique_ellipse(k, prec):
+    """A cylinder cut by an oblique plane, its ellipse edge searched through its rational
+    quadratic arcs: points on the bisector of the two faces' normals answer at the edge's
+    exact distance as the edge; inside points at the nearest face's (convex solid);
+    uniform and near-edge points against the oracle; a level bit-exact with the oracle's."""
+    from closed_forms import oblique_edge_points, oblique_inside
+
+    part = Part(f"oblique:{k}")
+    o = Oracle(part)
+    g = PartGpu(part, 2.0)
+    _, R, L, t = part.describe()
+    pts, exp = oblique_edge_points(R, L, t, 0.4, np.linspace(0.05, 2 * np.pi - 0.05, 41))
+    d, flags, feat = g.query(pts, prec)
+    assert (flags == 0).all() and (feat == ((1 << 28) | 1)).all()
+    assert np.abs(d - exp).max() <= max(tol(prec, part), 1e-9)
+    rng = np.random.default_rng(41 + k)
+    near, _ = oblique_edge_points(R, L, t, 0.5, rng.uniform(0, 2 * np.pi, 512))
+    near = near + rng.normal(0, 0.6, near.shape)
+    pts = np.concatenate([uniform_points(SEED + k, 1 << 14, part.lo - 3, part.hi + 3), near])
+    d, flags, _ = g.query(pts, prec)
+    dref, _, _ = o.sdf(pts)
+    assert (flags == 0).all()
+    assert np.abs(d - dref).max() <= tol(prec, part)
+    inside, depth = oblique_inside(pts, R, L, t)
+    assert (d[inside] < 0).all() and (d[~inside] > 0).all()
+    assert np.abs(d[inside] + depth[inside]).max() <= max(tol(prec, part), 1e-9)
+    lv = level_for(part, part.diagonal / 64, 8)
+    m_ref, a_ref, _ = o.bake(lv)
+    m, a, _ = g.bake(lv, prec)
+    assert (m == m_ref).all() and (a == a_ref).all()
+
+
+@pytest.mark.parametrize("prec", PRECS)
 def test_c4_parts_bake_like_oracle(prec):
     """Every part of the batched workload (turned parts, NURBS blocks with 4 x 4 to
     32 x 32 nets, plates with up to ~200 holes) baked on a 64^3 node grid: map and

@@ -390,6 +390,41 @@ def test_crease_closed_form(built, k):
     assert ((d[inner] < 0) == inside)[clear].all()
 
 
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_oblique_ellipse_closed_form(built, k):
+    """A cylinder cut by an oblique plane: an ellipse edge (curve3.rs:557-590). The
+    edge's closest point against 20,001 samples of the ellipse (curve3.rs:935-968); points
+    on the bisector of the two faces' normals at distance h from the edge answer at h as
+    the edge; inside points at the nearest face's distance (the solid is convex); outside
+    points within the dense boundary samples' bounds."""
+    from closed_forms import oblique_boundary_samples, oblique_edge_points, oblique_inside
+
+    part = Part(f"oblique:{k}")
+    o = Oracle(part)
+    _, R, L, t = part.describe()
+    th = np.linspace(0, 2 * np.pi, 20001)
+    s = np.stack([R * np.cos(th), R * np.sin(th), L + R * t * np.cos(th)], 1)
+    rng = np.random.default_rng(31 + k)
+    for p in rng.uniform(part.lo - 5, part.hi + 5, (40, 3)):
+        d, _, _ = o.edge_closest(1, p)
+        dense = np.linalg.norm(s - p, axis=1).min()
+        assert dense - 0.01 <= d <= dense + 1e-12
+    pts, exp = oblique_edge_points(R, L, t, 0.4, np.linspace(0.05, 2 * np.pi - 0.05, 41))
+    d, feat, flags = o.sdf(pts)
+    assert np.abs(d - exp).max() < 1e-9
+    assert (feat == ((1 << 28) | 1)).all() and not flags.any()
+    pts = uniform_points(SEED + k, 4000, part.lo - 3, part.hi + 3)
+    d, _, flags = o.sdf(pts)
+    assert not flags.any()
+    inside, depth = oblique_inside(pts, R, L, t)
+    assert (d[inside] < 0).all() and (d[~inside] > 0).all()
+    assert np.abs(d[inside] + depth[inside]).max() < 1e-9
+    step = 0.25
+    dense, _ = cKDTree(oblique_boundary_samples(R, L, t, step)).query(pts[~inside])
+    assert (d[~inside] <= dense + 1e-9).all()
+    assert (d[~inside] >= dense - step).all()
+
+
 @pytest.mark.parametrize("holes", [1, 12, 40])
 def test_plate_closed_form(built, holes):
     """Plates with plain, filleted (torus) and countersunk (cone) holes against per-face
