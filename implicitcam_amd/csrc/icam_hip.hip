@@ -285,6 +285,46 @@ __device__ void block_box(float lo[3], float hi[3], float& r, float* red) {
   }
 }
 
+// An upper bound on the distance from every point of box gb to the part's boundary:
+// the least, over primitives whose boxes hold points of the boundary, of the farthest
+// distance between the two boxes (rounded up). Faces, edges, vertices and edge pieces
+// hold boundary points; a sub-patch only when it lies wholly inside its face's loops
+// (one the packer could not place may hold none), a C0 line's piece not at all (it
+// may run outside the loops).
+__device__ inline float box_far(const float* a, const float* b) {
+  float s = 0.f;
+  for (int k = 0; k < 3; ++k) {
+    const float g = fmaxf(a[3 + k] - b[k], b[3 + k] - a[k]);
+    s += g * g;
+  }
+  return sqrtf(s);
+}
+template <int NT>
+__device__ float block_reach(const Dev& dv, const float* gb, float* red) {
+  float m = 3.0e38f;
+  for (uint32_t i = threadIdx.x; i < dv.L.n_prims; i += NT) {
+    const uint32_t code = dv.pcode[i], kind = code >> 28, id = code & 0x0fffffffu;
+    bool holds = true;
+    if (kind == P_LEAF) {
+      holds = dv.U[dv.L.leaf_u + LEAF_U * id + 2] != 0u;
+    } else if (kind == P_CLEAF) {
+      const uint32_t e = dv.U[dv.L.cleaf_u + CLEAF_U * id + 1];
+      holds = dv.U[dv.L.edge_u + EDGE_U * e] != EDGE_CREASE;
+    }
+    if (holds) m = fminf(m, box_far(dv.pbox + 6 * i, gb));
+  }
+  const int tid = threadIdx.x;
+  red[tid] = m;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] = fminf(red[tid], red[tid + s]);
+    __syncthreads();
+  }
+  m = red[0];
+  __syncthreads();
+  return m < 3.0e38f ? m * (1.0f + 1e-5f) + 1e-4f : m;
+}
+
 // ------------------------------------------------------------------ one lane's search
 
 // A point's search over the block's candidates (or the whole list when they did not
@@ -629,6 +669,12 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
     r = fmaxf(r, ub * (1.0f + 1e-5f) + 1e-4f);
   }
   block_box<NT>(lo, hi, r, red);
+  if (S == SRC_POINTS) {
+    // scattered points: the boundary's own bound over the block's box, usually far
+    // tighter than the nearest vertices'
+    const float gp[6] = {lo[0] - 1e-4f, lo[1] - 1e-4f, lo[2] - 1e-4f, hi[0] + 1e-4f, hi[1] + 1e-4f, hi[2] + 1e-4f};
+    r = fminf(r, block_reach<NT>(dv, gp, red));
+  }
   }
   float gb[6] = {lo[0] - 1e-4f, lo[1] - 1e-4f, lo[2] - 1e-4f, hi[0] + 1e-4f, hi[1] + 1e-4f, hi[2] + 1e-4f};
   int nc;
