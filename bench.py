@@ -63,21 +63,21 @@ def pmc_valu(path, kernel):
 
 
 def trimmed_plate_queries(args, log, threads):
-    """The trimmed plate: 2^20 points, half uniform over its box (2 mm margin) and half
+    """The trimmed plate: 2^20 points (2^c3_log2), half uniform over its box (2 mm margin) and half
     within 0.05 mm of its planar faces' trim curves (where the mixed form takes its trim
     decisions in double precision), inputs resident in HBM, in both precisions; the two
     precisions' signs compared on every point and both against the FP64 oracle on a
-    sample of 2^14. Scattered points are searched in the order given (a block's
-    points may span the part), and points past their sub-patches' convexity
-    certificates are settled in double precision: this leg is far from the bake's
-    rates."""
+    sample of 2^14. Scattered points are searched in cells of a 64^3 grid over their
+    box (results in the caller's order), each block of them within the boundary's own
+    bound over its box, and points past their sub-patches' convexity certificates are
+    settled in double precision: this leg is far from the bake's rates."""
     import torch
 
     from implicitcam_amd import FP64, MIXED, Part, PartGpu, near_trim_points, uniform_points
 
     part = Part("c3")
     g = PartGpu(part, 20.0)
-    n = 1 << 20
+    n = 1 << args.c3_log2
     pts = np.concatenate([uniform_points(0x2545F4914F6CDD1D, n // 2, part.lo - 2, part.hi + 2), near_trim_points(part, 0x9E3779B97F4A7C15, n // 2, 0.05)])
     dev = torch.device("cuda:0")
     d_pts = torch.from_numpy(pts).to(dev)
@@ -179,7 +179,8 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the parity check against the oracle")
     ap.add_argument("--profile", action="store_true", help="a short run for a profiler: bake steps only")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-part batched workload")
-    ap.add_argument("--c3", action="store_true", help="add 2^20 scattered queries on the trimmed plate (half at its trim curves)")
+    ap.add_argument("--c3", action="store_true", help="add scattered queries on the trimmed plate (half at its trim curves)")
+    ap.add_argument("--c3-log2", type=int, default=20, help="log2 of the trimmed plate's query count (24: the configuration's 2^24)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the path shards by brick ranges across GPUs (see the design notes); "
