@@ -1440,8 +1440,11 @@ __global__ void kb_fill(FillList f) {
 
 // The searches, persistent: centres over every part's tiles, nodes over every part's
 // band bricks.
+// The single-precision node search at three waves a SIMD (168 registers, a few spilled):
+// 0.41 -> 0.37 ms on the NURBS-top block's 6,889 band bricks; the centres' search and
+// four waves a SIMD lose.
 template <int S, class T>
-__global__ __launch_bounds__(Shape<S>::NT) void kb_search(BatchArgs a) {
+__global__ __launch_bounds__(Shape<S>::NT) __attribute__((amdgpu_waves_per_eu((S == SRC_NODES && sizeof(T) == 4) ? 3 : 1, 8))) void kb_search(BatchArgs a) {
   __shared__ SearchLds<S, T> sm;
   __shared__ uint32_t wslot;
   const uint32_t total = S == SRC_CENTRES ? uint32_t(a.tile_off[a.n]) : a.band_off[a.n];
