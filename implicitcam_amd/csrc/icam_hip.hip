@@ -1178,7 +1178,31 @@ struct FinishLds {
   uint8_t st[729];  // 0 exact, 1 far, 2 undecided, 3 filled, 4 bridged
   int changed;
   int16_t q[730];
+  // the fill's 729-bit masks, node n at bit n % 64 of word n / 64: the nodes known
+  // positive and negative, the far ones, two dilation stages of each sign, the nodes
+  // filled of each sign, and the sides (x = 0, x = 8, y = 0, y = 8, inside the brick)
+  uint64_t sgn[2][12], far[12], ta[2][12], tb[2][12], filled[2][12], side[5][12];
 };
+
+// One sign's 3 x 3 x 3 dilation of a 729-bit mask, a word a lane, in the three passes the
+// caller separates by barriers: along x (+-1 bit, not across the x = 0 / 8 sides), along y
+// (+-9 bits, not across y = 0 / 8), along z (+-81 = 64 + 17 bits), the last kept inside.
+__device__ inline uint64_t mword(const uint64_t* x, int w) { return w >= 0 && w < 12 ? x[w] : 0ull; }
+__device__ inline uint64_t dilate_x(const uint64_t* x, const uint64_t* lo, const uint64_t* hi, int w) {
+  const uint64_t up = ((x[w] & ~hi[w]) << 1) | ((mword(x, w - 1) & ~mword(hi, w - 1)) >> 63);
+  const uint64_t dn = ((x[w] & ~lo[w]) >> 1) | ((mword(x, w + 1) & ~mword(lo, w + 1)) << 63);
+  return x[w] | up | dn;
+}
+__device__ inline uint64_t dilate_y(const uint64_t* x, const uint64_t* lo, const uint64_t* hi, int w) {
+  const uint64_t up = ((x[w] & ~hi[w]) << 9) | ((mword(x, w - 1) & ~mword(hi, w - 1)) >> 55);
+  const uint64_t dn = ((x[w] & ~lo[w]) >> 9) | ((mword(x, w + 1) & ~mword(lo, w + 1)) << 55);
+  return x[w] | up | dn;
+}
+__device__ inline uint64_t dilate_z(const uint64_t* x, const uint64_t* inside, int w) {
+  const uint64_t up = (mword(x, w - 1) << 17) | (mword(x, w - 2) >> 47);
+  const uint64_t dn = (mword(x, w + 1) >> 17) | (mword(x, w + 2) << 47);
+  return (x[w] | up | dn) & inside[w];
+}
 __device__ __forceinline__ void finish_brick(const Lv& lv, const uint32_t* list, long bi, const double* dcs, const int8_t* sgns, const double* val_in,
                                     const uint32_t* flag_in, uint32_t* staging, uint32_t* cls, unsigned long long* counters, FinishLds& sm) {
   double* val = sm.val;
@@ -1193,30 +1217,59 @@ __device__ __forceinline__ void finish_brick(const Lv& lv, const uint32_t* list,
     st[n] = uint8_t(f > 2 ? 2 : f);
   }
   __syncthreads();
-  // partcache.rs:825-855 far nodes take the sign of any neighbour known, by flood fill
+  // partcache.rs:825-855 far nodes take the sign of any neighbour known, by flood fill. A
+  // far node is farther from the boundary than from any of its neighbours, so all its
+  // known neighbours share its sign, and the fill is the dilation of the nodes known
+  // positive (negative) within the far ones, repeated until nothing changes: on 729-bit
+  // masks, a lane a word (a node with known neighbours of both signs, which consistent
+  // answers never give, is left to the bridging).
+  static_assert(WG == 256, "the fill's masks take a wave's ballot as a word");
+  const int lane = tid & 63, wv = tid >> 6;
+  for (int k = 0; k < 3; ++k) {
+    const int n = tid + 256 * k;  // wave wv of pass k: nodes of word 4 k + wv
+    const bool in = n < 729;
+    const uint64_t b0 = __ballot(in && n % 9 == 0), b1 = __ballot(in && n % 9 == 8);
+    const uint64_t b2 = __ballot(in && (n / 9) % 9 == 0), b3 = __ballot(in && (n / 9) % 9 == 8), b4 = __ballot(in);
+    if (lane == 0) sm.side[0][4 * k + wv] = b0, sm.side[1][4 * k + wv] = b1, sm.side[2][4 * k + wv] = b2, sm.side[3][4 * k + wv] = b3, sm.side[4][4 * k + wv] = b4;
+  }
   auto fill = [&]() {
+    for (int k = 0; k < 3; ++k) {
+      const int n = tid + 256 * k;
+      const bool in = n < 729;
+      const uint8_t sn = in ? st[n] : uint8_t(255);
+      const double v = in ? val[n] : 0.0;
+      const bool known = (sn == 0 || sn == 3) && v != 0.0;
+      const uint64_t bp = __ballot(known && v > 0.0), bn = __ballot(known && v < 0.0), bf = __ballot(sn == 1);
+      if (lane == 0) {
+        const int w = 4 * k + wv;
+        sm.sgn[0][w] = bp, sm.sgn[1][w] = bn, sm.far[w] = bf, sm.filled[0][w] = 0ull, sm.filled[1][w] = 0ull;
+      }
+    }
+    __syncthreads();
+    const int sg = tid / 12, w = tid % 12;
     for (;;) {
+      if (tid < 24) sm.ta[sg][w] = dilate_x(sm.sgn[sg], sm.side[0], sm.side[1], w);
+      __syncthreads();
+      if (tid < 24) sm.tb[sg][w] = dilate_y(sm.ta[sg], sm.side[2], sm.side[3], w);
+      __syncthreads();
+      if (tid < 24) sm.ta[sg][w] = dilate_z(sm.tb[sg], sm.side[4], w) & sm.far[w];
       if (tid == 0) changed = 0;
       __syncthreads();
-      for (int n = tid; n < 729; n += WG) {
-        if (st[n] != 1) continue;
-        for (int k = 0; k < 26; ++k) {
-          int m = nb26(n, k);
-          if (m < 0) continue;
-          uint8_t sm = st[m];
-          if ((sm == 0 || sm == 3) && val[m] != 0.0) {
-            val[n] = copysign(clamp, val[m]);
-            st[n] = 3;
-            changed = 1;
-            break;
-          }
-        }
+      if (tid < 12) {
+        const uint64_t np = sm.ta[0][tid] & ~sm.ta[1][tid], nn = sm.ta[1][tid] & ~sm.ta[0][tid];
+        sm.sgn[0][tid] |= np, sm.sgn[1][tid] |= nn, sm.far[tid] &= ~(np | nn);
+        sm.filled[0][tid] |= np, sm.filled[1][tid] |= nn;
+        if (np | nn) changed = 1;
       }
       __syncthreads();
-      int c = changed;
-      __syncthreads();
-      if (!c) break;
+      if (!changed) break;
     }
+    for (int n = tid; n < 729; n += WG) {
+      const uint64_t b = 1ull << (n & 63);
+      if (sm.filled[0][n >> 6] & b) val[n] = clamp, st[n] = 3;
+      if (sm.filled[1][n >> 6] & b) val[n] = -clamp, st[n] = 3;
+    }
+    __syncthreads();
   };
   fill();
   // partcache.rs:856-871 far nodes within the empty ball of the centre share its sign
