@@ -1681,11 +1681,9 @@ __global__ __launch_bounds__(256) void kb_centre_sign(BatchArgs a) {
 
 __global__ __launch_bounds__(WG) void kb_finish(BatchArgs a) {
   __shared__ FinishLds sm;
-  __shared__ uint32_t wslot;
   const uint32_t total = a.band_off[a.n];
-  for (;;) {
-    const uint32_t w = next_item(a, BS_FINISH, &wslot);
-    if (w >= total) break;
+  // bricks in strides (their costs are alike; a queue item each cost an atomic a brick)
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
     const int p = __builtin_amdgcn_readfirstlane(part_of<uint32_t>(a.band_off, a.n, w));
     const ICAM_K BPart& P = bpart(a, p);
     const Lv lv = ldk(P.lv);
