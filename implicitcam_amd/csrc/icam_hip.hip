@@ -766,6 +766,8 @@ template <int S, class T> struct SearchLds {
   float red[NT];
   int wave_tot[NT / 64];
   alignas(16) unsigned char scratch[cmax(SORTC * 12, Q * NT * (int(sizeof(T)) + 2))];
+  uint16_t list[S == SRC_NODES ? 736 : 2];  // a brick's unsure nodes, compacted
+  int count;
   __device__ uint32_t* tmp() { return (uint32_t*)scratch; }
   __device__ float* key() { return (float*)(scratch + SORTC * 4); }
   __device__ T* qlb() { return (T*)scratch; }
@@ -894,6 +896,35 @@ __device__ __forceinline__ void redo_block(const Dev& dv, const Job& j, const Re
   constexpr int NT = Shape<S>::NT;
   const int nc = block_candidates<S>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot);
   const Tab<double> tb = tab_of<double>(dv);
+  if constexpr (S == SRC_NODES && !SETTLE) {
+    // the few unsure nodes of a brick (near trim curves, near ties), compacted and
+    // searched RK lanes a node: one lane a node left nearly every lane of the block idle
+    constexpr int RK = 8, RQ = 4;
+    if (threadIdx.x == 0) sm.count = 0;
+    __syncthreads();
+    for (int n = threadIdx.x; n < 729; n += NT)
+      if (res.flag[blk * 729 + n] == want) sm.list[atomicAdd(&sm.count, 1)] = uint16_t(n);
+    __syncthreads();
+    const int cnt = sm.count;
+    const bool lead = (threadIdx.x & (RK - 1)) == 0;
+    for (int base = 0; base < cnt; base += NT / RK) {
+      const int idx = base + int(threadIdx.x) / RK;
+      const bool valid = idx < cnt;
+      const long i = valid ? blk * 729 + long(sm.list[idx]) : -1;
+      double bound = 0.0;
+      const v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
+      const Best<double> b = search<double, NT, RQ, RK, Shape<S>::C, false>(tb, p, valid, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, sm.cand, nc,
+                                                                            0.0, dv.L.eps64, sm.qlb(), sm.qk());
+      if (!valid || !lead) continue;
+      double val;
+      uint32_t flag, feat;
+      classify64(b, bound, dv, val, flag, feat);
+      res.val[i] = val;
+      res.flag[i] = flag;
+      if (res.feat) res.feat[i] = feat;
+    }
+    return;
+  }
   constexpr int K = Shape<S>::K;
   const bool owner = (threadIdx.x & (K - 1)) == 0;
   for (int m = 0; m < rounds<S>(); ++m) {
