@@ -841,11 +841,10 @@ __global__ __launch_bounds__(256) void k_leaves(Dev dv, Job j, Res res) {
 // The last correction in double precision of point i's single-precision winner; a
 // point the search was unsure of is marked (flag 3) and its block queued (entries are
 // part << 24 | block) for the fallback.
+// The first half, from the point's record (info): an unsure point is queued, a point
+// with no winner is written; returns whether the point waits for refine_found.
 template <int S>
-__device__ __forceinline__ void refine_point(const Dev& dv, const Job& j, const Res& res, long i) {
-  uint32_t info = res.info[i];
-  double bound;
-  v3<double> p = job_point<S>(j, dv, i, bound);
+__device__ __forceinline__ bool refine_head(const Dev& dv, const Job& j, const Res& res, long i, uint32_t info) {
   const bool unsure = info & 4u;
   wave_count(&res.counters[1], unsure);
   wave_count(&res.counters[3], unsure && (info & 16u));
@@ -854,22 +853,16 @@ __device__ __forceinline__ void refine_point(const Dev& dv, const Job& j, const 
     res.flag[i] = 3;
     long blk = block_of<S>(j, i);
     if (res.qmark[blk] == 0u && atomicOr(&res.qmark[blk], 1u) == 0u) res.queue[atomicAdd(res.qcount, 1u)] = (res.part << 24) | uint32_t(blk);
-    return;
+    return false;
   }
-  uint32_t flag = 0, feat = NONE;
+  const bool found = info & 1u, undecided = info & 8u;
+  if (found && !undecided) return true;
+  double bound;
+  job_point<S>(j, dv, i, bound);
+  const double d = res.d32[i];
   double val;
-  bool found = info & 1u, undecided = info & 8u;
-  double d = res.d32[i];
-  if (found && !undecided) {
-    uint32_t code = res.code[i];
-    double sg = (info & 2u) ? -1.0 : 1.0;
-    refine64(tab_of<double>(dv), code, double(res.pa[i]), double(res.pb[i]), p, d, sg);
-    if (d <= bound) {
-      val = d * sg, feat = feat_of(dv, code);
-    } else {
-      val = bound, flag = 1;
-    }
-  } else if (!undecided) {
+  uint32_t flag;
+  if (!undecided) {
     val = bound, flag = 1;
   } else {
     val = d >= bound ? bound : d;
@@ -877,13 +870,74 @@ __device__ __forceinline__ void refine_point(const Dev& dv, const Job& j, const 
   }
   res.val[i] = val;
   res.flag[i] = flag;
+  if (res.feat) res.feat[i] = NONE;
+  return false;
+}
+// The second half: the winner's foot and sign again in double precision.
+template <int S>
+__device__ __forceinline__ void refine_found(const Dev& dv, const Job& j, const Res& res, long i, uint32_t info) {
+  double bound;
+  v3<double> p = job_point<S>(j, dv, i, bound);
+  uint32_t flag = 0, feat = NONE;
+  double val, d = res.d32[i];
+  const uint32_t code = res.code[i];
+  double sg = (info & 2u) ? -1.0 : 1.0;
+  refine64(tab_of<double>(dv), code, double(res.pa[i]), double(res.pb[i]), p, d, sg);
+  if (d <= bound) {
+    val = d * sg, feat = feat_of(dv, code);
+  } else {
+    val = bound, flag = 1;
+  }
+  res.val[i] = val;
+  res.flag[i] = flag;
   if (res.feat) res.feat[i] = feat;
+}
+template <int S>
+__device__ __forceinline__ void refine_point(const Dev& dv, const Job& j, const Res& res, long i) {
+  const uint32_t info = res.info[i];
+  if (refine_head<S>(dv, j, res, i, info)) refine_found<S>(dv, j, res, i, info);
+}
+
+// A band brick's nodes corrected by one block. The nodes that wait for the correction
+// are listed in LDS, sub-patch winners from the front and the other kinds from the
+// back, so each wave runs one kind's correction on full lanes: one lane a node left
+// the far nodes' lanes idle beside the Newton solves, and mixed the kinds' branches in
+// a wave.
+struct RefineLds {
+  uint16_t list[729];
+  uint32_t na, nb;
+};
+template <int NT>
+__device__ __forceinline__ void refine_brick(const Dev& dv, const Job& j, const Res& res, long blk, RefineLds& sm) {
+  if (threadIdx.x == 0) sm.na = 0u, sm.nb = 0u;
+  __syncthreads();
+  for (int n = threadIdx.x; n < 729; n += NT) {
+    const long i = blk * 729 + n;
+    const uint32_t info = res.info[i];
+    const bool wait = refine_head<SRC_NODES>(dv, j, res, i, info);
+    const bool leaf = wait && (res.code[i] >> 28) == P_LEAF;
+    const uint32_t ka = wave_slot(&sm.na, leaf), kb = wave_slot(&sm.nb, wait && !leaf);
+    if (wait) sm.list[leaf ? ka : 728u - kb] = uint16_t(n);
+  }
+  __syncthreads();
+  const int na = int(sm.na), nw = na + int(sm.nb);
+  for (int k = threadIdx.x; k < nw; k += NT) {
+    const long i = blk * 729 + sm.list[k < na ? k : 728 - (k - na)];
+    refine_found<SRC_NODES>(dv, j, res, i, res.info[i]);
+  }
 }
 
 template <int S>
 __global__ __launch_bounds__(256) void k_refine(Dev dv, Job j, Res res) {
   long i = long(blockIdx.x) * 256 + threadIdx.x;
   if (i < j.n) refine_point<S>(dv, j, res, i);
+}
+__global__ __launch_bounds__(256) void k_refine_bricks(Dev dv, Job j, Res res, long nbricks) {
+  __shared__ RefineLds sm;
+  for (long blk = blockIdx.x; blk < nbricks; blk += gridDim.x) {
+    refine_brick<256>(dv, j, res, blk, sm);
+    __syncthreads();
+  }
 }
 
 // A queued block searched again in double precision for its points flagged `want`:
@@ -1597,6 +1651,21 @@ __global__ __launch_bounds__(256) void kb_refine(BatchArgs a) {
     refine_point<S>(dv, j, r, i);
   }
 }
+// The band bricks' nodes of every part, a block a brick (the part is the same on every
+// lane, so its records are read by scalar loads).
+__global__ __launch_bounds__(256) void kb_refine_bricks(BatchArgs a) {
+  __shared__ RefineLds sm;
+  const uint32_t total = a.band_off[a.n];
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+    const int p = __builtin_amdgcn_readfirstlane(part_of<uint32_t>(a.band_off, a.n, w));
+    const ICAM_K BPart& P = bpart(a, p);
+    const Dev dv = ldk(P.dv);
+    const Job j = ldk(P.jn);
+    const Res r = ldk(P.rn);
+    refine_brick<256>(dv, j, r, long(w - __builtin_amdgcn_readfirstlane(a.band_off[p])), sm);
+    __syncthreads();
+  }
+}
 
 template <int S>
 __global__ __launch_bounds__(256) void kb_mark(BatchArgs a) {
@@ -1908,7 +1977,10 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
     HIP_OK(hipMemsetAsync(r.qmark, 0, size_t(blocks) * 4, s));
     HIP_OK(launch(st, names[S][0], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, float>, h->dv, j, r));
     if (lb) HIP_OK(launch(st, names[S][4], s, dim3(lb), dim3(256), k_leaves<S, float>, h->dv, j, r));
-    HIP_OK(launch(st, names[S][1], s, dim3(unsigned((j.n + 255) / 256)), dim3(256), k_refine<S>, h->dv, j, r));
+    if constexpr (S == SRC_NODES)
+      HIP_OK(launch(st, names[S][1], s, dim3(unsigned(std::min<long>(blocks, 8192))), dim3(256), k_refine_bricks, h->dv, j, r, blocks));
+    else
+      HIP_OK(launch(st, names[S][1], s, dim3(unsigned((j.n + 255) / 256)), dim3(256), k_refine<S>, h->dv, j, r));
     unsigned fb = unsigned(std::min<long>(blocks, 2048));
     HIP_OK(launch(st, names[S][2], s, dim3(fb), dim3(NT), k_fallback<S>, h->dv, j, r));
   } else {
@@ -2467,7 +2539,7 @@ int batch_enqueue(icam_batch* B, hipStream_t s, Stages* st = nullptr) {
   if (mixed) {
     HIP_OK(launch(st, "nodes_search32", s, dim3(B->gn), ntn, kb_search<SRC_NODES, float>, a));
     HIP_OK(launch(st, "nodes_leaves32", s, dim3(gl), dim3(256), kb_leaves<float>, a));
-    HIP_OK(launch(st, "nodes_refine64", s, dim3(pts_n), dim3(256), kb_refine<SRC_NODES>, a));
+    HIP_OK(launch(st, "nodes_refine64", s, dim3(unsigned(std::min<size_t>(std::max<size_t>(B->total_cap, 1), 8192))), dim3(256), kb_refine_bricks, a));
     HIP_OK(launch(st, "nodes_fallback64", s, dim3(B->gn), ntn, kb_redo<SRC_NODES, false>, a));
   } else {
     HIP_OK(launch(st, "nodes_search64", s, dim3(B->gn), ntn, kb_search<SRC_NODES, double>, a));
