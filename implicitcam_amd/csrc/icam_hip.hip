@@ -43,9 +43,15 @@ using namespace icam;
 namespace {
 
 constexpr int WG = 256;
-// The settle kernels' grid: their lanes hold the certified search's queue in private
-// memory, so few blocks run at once (the scratch they need is per block in flight).
-constexpr int SETTLE_GRID = 256;
+// The settle kernels' grid: as many blocks as the device runs at once (settle_grid,
+// from the occupancy the compiled kernel allows: two blocks a CU on MI355X). Their
+// lanes hold the certified search's queue in private memory and run long, so blocks
+// past that would wait for a slot and take their share of the queue late.
+template <class K> int settle_grid(K kernel, int nt, int cus) {
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, nt, 0) != hipSuccess || per < 1) per = 1;
+  return per * std::max(cus, 1);
+}
 constexpr int SORTC = 1024;  // lists up to this long are sorted nearest first
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int TCAP = 512;    // a tile's list kept for its bricks' nodes, at most
@@ -1027,11 +1033,19 @@ __global__ __launch_bounds__(256) void k_mark_settle(Job j, Res res) {
   if (i < j.n) mark_point<S>(j, res, i);
 }
 
+// Blocks take the next queued block from a counter (scount[1], cleared with the queue's
+// length): a queued block's settle costs from one read to thousands, so a fixed share
+// each left a few blocks running long after the rest.
 template <int S>
 __global__ __launch_bounds__(Shape<S>::NT) void k_settle(Dev dv, Job j, Res res) {
   __shared__ SearchLds<S, double> sm;
+  __shared__ uint32_t wslot;
   const uint32_t total = *res.scount;
-  for (uint32_t qb = blockIdx.x; qb < total; qb += gridDim.x) {
+  for (;;) {
+    if (threadIdx.x == 0) wslot = atomicAdd(res.scount + 1, 1u);
+    __syncthreads();
+    const uint32_t qb = wslot;
+    if (qb >= total) break;
     redo_block<S, true>(dv, j, res, long(res.squeue[qb] & 0xffffffu), sm);
     __syncthreads();
   }
@@ -1667,16 +1681,16 @@ __global__ __launch_bounds__(256) void kb_refine_bricks(BatchArgs a) {
   }
 }
 
-template <int S>
-__global__ __launch_bounds__(256) void kb_mark(BatchArgs a) {
-  const long total = batch_points<S>(a);
-  for (long g = long(blockIdx.x) * 256 + threadIdx.x; g < total; g += long(gridDim.x) * 256) {
-    long i;
-    const int p = point_part<S>(a, g, i);
+// The band bricks' nodes left undecided, marked for the settle a block a brick.
+__global__ __launch_bounds__(256) void kb_mark_bricks(BatchArgs a) {
+  const uint32_t total = a.band_off[a.n];
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+    const int p = __builtin_amdgcn_readfirstlane(part_of<uint32_t>(a.band_off, a.n, w));
     const ICAM_K BPart& P = bpart(a, p);
-    const Job j = S == SRC_CENTRES ? ldk(P.jc) : ldk(P.jn);
-    const Res r = S == SRC_CENTRES ? ldk(P.rc) : ldk(P.rn);
-    mark_point<S>(j, r, i);
+    const Job j = ldk(P.jn);
+    const Res r = ldk(P.rn);
+    const long blk = long(w - __builtin_amdgcn_readfirstlane(a.band_off[p]));
+    for (int n = threadIdx.x; n < 729; n += 256) mark_point<SRC_NODES>(j, r, blk * 729 + n);
   }
 }
 
@@ -1843,6 +1857,7 @@ struct Buf {
 struct icam_hip {
   std::mutex mu;
   int device = 0;
+  int settle_grid = 256;
   hipStream_t stream = nullptr;
   Packed pk;
   Dev dv{};
@@ -1990,10 +2005,10 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
   // what is still undecided: settled by the certified search (not the brick centres:
   // an undecided centre under its reach keeps its brick, whose nodes are read)
   if (S == SRC_CENTRES) return 0;
-  HIP_OK(hipMemsetAsync(r.scount, 0, 4, s));
+  HIP_OK(hipMemsetAsync(r.scount, 0, 8, s));
   HIP_OK(hipMemsetAsync(r.smark, 0, size_t(blocks) * 4, s));
   HIP_OK(launch(st, names[S][6], s, dim3(unsigned((j.n + 255) / 256)), dim3(256), k_mark_settle<S>, j, r));
-  HIP_OK(launch(st, names[S][7], s, dim3(unsigned(std::min<long>(blocks, SETTLE_GRID))), dim3(NT), k_settle<S>, h->dv, j, r));
+  HIP_OK(launch(st, names[S][7], s, dim3(unsigned(std::min<long>(blocks, h->settle_grid))), dim3(NT), k_settle<S>, h->dv, j, r));
   return 0;
 }
 
@@ -2081,6 +2096,7 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
   hipDeviceProp_t prop;
   HIP_OK(hipGetDeviceProperties(&prop, h->device));
   h->name = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ", " + std::to_string(prop.multiProcessorCount) + " CUs)";
+  h->settle_grid = settle_grid(k_settle<SRC_WIDE>, Shape<SRC_WIDE>::NT, prop.multiProcessorCount);
   if (!pack_part(part, need, h->pk)) {
     std::string e = h->pk.error;
     delete h;
@@ -2346,11 +2362,11 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     jn.tile_cand = h->tile_cand.as<uint32_t>(), jn.tile_count = h->tile_count.as<uint32_t>();
     if (run_search<SRC_NODES>(h, mixed, jn, rn, s, &st, long(nband))) return 1;
     // the centres of all-far band bricks with no sign: settled
-    HIP_OK(hipMemsetAsync(r.scount, 0, 4, s));
+    HIP_OK(hipMemsetAsync(r.scount, 0, 8, s));
     HIP_OK(hipMemsetAsync(r.smark, 0, size_t(tiles) * 4, s));
     HIP_OK(launch(&st, "centres_need", s, dim3(unsigned(std::min<long>(nband, 4096))), dim3(256), k_need_centre, jc, r, (const uint32_t*)h->list.as<uint32_t>(),
                   long(nband), (const uint32_t*)rn.flag, (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>()));
-    HIP_OK(launch(&st, "centres_settle64", s, dim3(unsigned(std::min<long>(tiles, SETTLE_GRID))), dim3(Shape<SRC_CENTRES>::NT), k_settle<SRC_CENTRES>,
+    HIP_OK(launch(&st, "centres_settle64", s, dim3(unsigned(std::min<long>(tiles, h->settle_grid))), dim3(Shape<SRC_CENTRES>::NT), k_settle<SRC_CENTRES>,
                   h->dv, jc, r));
     HIP_OK(launch(&st, "centres_sign", s, dim3(unsigned(std::min<long>((long(nband) + 255) / 256, 4096))), dim3(256), k_centre_sign, r,
                   (const uint32_t*)h->list.as<uint32_t>(), long(nband), h->dc.as<double>(), h->sgn.as<int8_t>()));
@@ -2460,6 +2476,7 @@ struct icam_batch {
   // the launch shapes: persistent grids of the centres' search, the nodes' search
   // and the band bricks' finish (blocks)
   int gc = 2304, gn = 1024, gf = 2048;
+  int gs = 256;  // the settle stages' grid
   std::string tuning;
   BatchArgs args{};
   ~icam_batch() {
@@ -2519,7 +2536,6 @@ int batch_enqueue(icam_batch* B, hipStream_t s, Stages* st = nullptr) {
     HIP_OK(launch(st, "zero", s, dim3(unsigned(std::min<size_t>((most + 255) / 256, 4096))), dim3(256), kb_fill, f));
   }
   const unsigned pts_c = unsigned(std::min<long>((B->total_bricks + 255) / 256, 8192));
-  const unsigned pts_n = unsigned(std::min<size_t>((B->total_cap * 729 + 255) / 256, 16384));
   const dim3 ntc(Shape<SRC_CENTRES>::NT), ntn(Shape<SRC_NODES>::NT);
   // the brick centres: the level's decision (the super-tiles' lists first)
   HIP_OK(launch(st, "centres_lists", s, dim3(unsigned(std::min<long>(std::max<long>(B->total_supers, 1), 4096))), dim3(256), kb_super, a));
@@ -2545,11 +2561,11 @@ int batch_enqueue(icam_batch* B, hipStream_t s, Stages* st = nullptr) {
     HIP_OK(launch(st, "nodes_search64", s, dim3(B->gn), ntn, kb_search<SRC_NODES, double>, a));
     HIP_OK(launch(st, "nodes_leaves64", s, dim3(gl), dim3(256), kb_leaves<double>, a));
   }
-  HIP_OK(launch(st, "nodes_mark", s, dim3(pts_n), dim3(256), kb_mark<SRC_NODES>, a));
-  HIP_OK(launch(st, "nodes_settle64", s, dim3(SETTLE_GRID), ntn, kb_redo<SRC_NODES, true>, a));
+  HIP_OK(launch(st, "nodes_mark", s, dim3(unsigned(std::min<size_t>(std::max<size_t>(B->total_cap, 1), 8192))), dim3(256), kb_mark_bricks, a));
+  HIP_OK(launch(st, "nodes_settle64", s, dim3(unsigned(B->gs)), ntn, kb_redo<SRC_NODES, true>, a));
   // the centres of all-far band bricks with no sign: settled
   HIP_OK(launch(st, "centres_need", s, dim3(B->gf), dim3(256), kb_need_centre, a));
-  HIP_OK(launch(st, "centres_settle64", s, dim3(SETTLE_GRID), ntc, kb_redo<SRC_CENTRES, true>, a));
+  HIP_OK(launch(st, "centres_settle64", s, dim3(unsigned(B->gs)), ntc, kb_redo<SRC_CENTRES, true>, a));
   HIP_OK(launch(st, "centres_sign", s, dim3(unsigned(std::min<size_t>((B->total_cap + 255) / 256, 4096))), dim3(256), kb_centre_sign, a));
   // fill, quantize, pack, number and store
   HIP_OK(launch(st, "bricks_finish", s, dim3(B->gf), dim3(WG), kb_finish, a));
@@ -2727,6 +2743,7 @@ int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t
   HIP_OK(hipGetDeviceProperties(&prop, dev));
   const int cus = std::max(prop.multiProcessorCount, 1);
   B->gc = 9 * cus, B->gn = 4 * cus, B->gf = 8 * cus;
+  B->gs = settle_grid(kb_redo<SRC_NODES, true>, Shape<SRC_NODES>::NT, cus);
   if (tune && batch_tune(B.get(), cus)) return 1;
   // the captured graph
   HIP_OK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeRelaxed));
