@@ -2,7 +2,9 @@
 
 - implicitcam_amd/_lib/libicam_hip.so: the product (HIP kernels for gfx950 and the
   C ABI of include/icam_hip.h), compiled by hipcc;
-- implicitcam_amd/_lib/libicam_gen.so: the synthetic part generator (g++).
+- implicitcam_amd/_lib/libicam_gen.so: the synthetic part generator (g++);
+- implicitcam_amd/_lib/libicam_rules.so: the host build of the brick rules the device
+  runs (csrc/brick_rules.hpp), which the tests check on the CPU (g++).
 
 Every call compiles again (no cache), so a changed source is always picked up.
 """
@@ -41,6 +43,14 @@ def build_gen(verbose=False):
     return out
 
 
+def build_rules(verbose=False):
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libicam_rules.so")
+    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", out + ".tmp", os.path.join(CSRC, "brick_rules.cpp")], verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_hip(verbose=False):
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libicam_hip.so")
@@ -52,6 +62,7 @@ def build_hip(verbose=False):
 
 def build_all(verbose=False):
     build_gen(verbose)
+    build_rules(verbose)
     build_hip(verbose)
 
 

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -35,6 +36,7 @@
 #include <vector>
 
 #include "../../include/icam_hip.h"
+#include "brick_rules.hpp"
 #include "pack.hpp"
 #include "sdf_device.hpp"
 
@@ -487,7 +489,10 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const 
   }
   T last_lb = T(-1);
   uint32_t last_k = 0;
-  for (int round = 0; round < 4096; ++round) {
+  // At most tb.L.max_rounds rounds: past them the parts not read keep their lower bounds
+  // (the least becomes lmin), so the point is undecided rather than taken as exact.
+  const int max_rounds = int(tb.L.max_rounds);
+  for (int round = 0;; ++round) {
     T next_lb = K > 1 ? group_min<K>(b.d) : b.d;
     int next = -1;
     if (parts && !scan) {
@@ -507,6 +512,10 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const 
       }
     }
     if (K > 1 ? !__any(next >= 0) : next < 0) break;
+    if (round >= max_rounds) {
+      if (next >= 0) b.lmin = mmin(b.lmin, next_lb);
+      break;
+    }
     if (next >= 0) {
       uint32_t k;
       if (!scan) {
@@ -1256,26 +1265,12 @@ __global__ __launch_bounds__(SCAN_T) void k_scan(const uint8_t* band, const uint
   scan_block(band, cls, n, out, list, total, s);
 }
 
-__device__ inline int nb26(int n, int k) {
-  // the k-th of the 26 neighbours of node n (x fastest), or -1 outside the brick
-  int d = k < 13 ? k : k + 1;  // skip the centre (13)
-  int di = d % 3 - 1, dj = (d / 3) % 3 - 1, dk = d / 9 - 1;
-  int i = n % 9 + di, j = (n / 9) % 9 + dj, kk = n / 81 + dk;
-  if (i < 0 || i > 8 || j < 0 || j > 8 || kk < 0 || kk > 8) return -1;
-  return i + 9 * j + 81 * kk;
-}
-__device__ inline double nb26_len(int k) {
-  int d = k < 13 ? k : k + 1;
-  int di = d % 3 - 1, dj = (d / 3) % 3 - 1, dk = d / 9 - 1;
-  return sqrt(double(di * di + dj * dj + dk * dk));
-}
-
 // A band brick from its nodes' answers: far nodes signed by flood fill from the nodes
 // known, then from the centre's empty ball, what is left bridged; quantized, classed
 // and packed.
 struct FinishLds {
   double val[729];
-  uint8_t st[729];  // 0 exact, 1 far, 2 undecided, 3 filled, 4 bridged
+  uint8_t st[729];  // 0 exact, 1 far, 2 undecided, 3 filled, 4 bridged (5 during a pass)
   int changed;
   int16_t q[730];
   // the fill's 729-bit masks, node n at bit n % 64 of word n / 64: the nodes known
@@ -1386,10 +1381,11 @@ __device__ __forceinline__ void finish_brick(const Lv& lv, const uint32_t* list,
     }
     if (__syncthreads_or(any)) fill();
   }
-  // partcache.rs:872-909 what is left is bridged (the mean of the known neighbours held
-  // within the exact nodes' Lipschitz brackets), one node after another; a brick all
-  // past the clamp takes its centre's sign (the centre's own, or as carried along the
-  // grid when the decision left it undecided)
+  // partcache.rs:872-909 what is left is bridged (bridge_nodes: in passes, the mean of
+  // the neighbours known when the pass began, held within the Lipschitz brackets of the
+  // nodes known exactly or by sign); a brick all past the clamp takes its centre's sign
+  // first (the centre's own, or as carried along the grid when the decision left it
+  // undecided)
   int open = 0;
   for (int n = tid; n < 729; n += WG) open |= (st[n] == 1 || st[n] == 2);
   if (__syncthreads_or(open)) {
@@ -1401,35 +1397,7 @@ __device__ __forceinline__ void finish_brick(const Lv& lv, const uint32_t* list,
         double s = (dc == dc && dc != 0.0) ? dc : csgn < 0 ? -1.0 : 1.0;
         for (int n = 0; n < 729; ++n) val[n] = copysign(clamp, s), st[n] = 3;
       }
-      for (int pass = 0; pass < 729; ++pass) {
-        bool left = false, moved = false;
-        for (int n = 0; n < 729; ++n) {
-          if (st[n] != 1 && st[n] != 2) continue;
-          double sum = 0, lo_ = -clamp, hi_ = clamp;
-          int cnt = 0;
-          for (int k = 0; k < 26; ++k) {
-            int m = nb26(n, k);
-            if (m < 0) continue;
-            if (st[m] == 0 || st[m] == 3 || st[m] == 4) sum += val[m], ++cnt;
-            if (st[m] == 0) lo_ = fmax(lo_, val[m] - nb26_len(k) * h), hi_ = fmin(hi_, val[m] + nb26_len(k) * h);
-          }
-          if (!cnt) {
-            left = true;
-            continue;
-          }
-          double v = sum / cnt;
-          val[n] = lo_ <= hi_ ? fmin(fmax(v, lo_), hi_) : 0.5 * (lo_ + hi_);
-          st[n] = 4;
-          moved = true;
-          ++bridged;
-        }
-        if (!left) break;
-        if (!moved) {
-          for (int n = 0; n < 729; ++n)
-            if (st[n] == 1 || st[n] == 2) val[n] = clamp, st[n] = 4, ++bridged;
-          break;
-        }
-      }
+      bridged += (unsigned long long)bridge_nodes(val, st, h, clamp);
       if (bridged) atomicAdd(&counters[0], bridged);
     }
   }
@@ -1879,6 +1847,11 @@ struct icam_hip {
   Buf dc, band, sgn, total, list, staging, cls, slot, stotal, map, atlas, counters, changed, firsts, tile_cand, tile_count, super_cand, super_count;
   std::vector<std::pair<const char*, double>> stages;
   std::string name;
+  // the end of the last call's work: calls share the scratch above, and a call that
+  // returns before its work is done (query_device, bake_device) may have run it on
+  // another stream, so each call's work waits for this event first
+  hipEvent_t done = nullptr;
+  bool pending = false;
 };
 
 namespace {
@@ -1913,6 +1886,18 @@ hipError_t launch(Stages* st, const char* name, hipStream_t s, dim3 g, dim3 b, K
   hipLaunchKernelGGL(k, g, b, 0, s, args...);
   if (st) st->tm.back().stop(s), st->names.push_back(name);
   return hipGetLastError();
+}
+
+// A call's work after the previous call's (whatever stream that ran on), and its end
+// recorded for the next.
+int after_previous(icam_hip* h, hipStream_t s) {
+  if (h->pending) HIP_OK(hipStreamWaitEvent(s, h->done, 0));
+  return 0;
+}
+int mark_end(icam_hip* h, hipStream_t s) {
+  HIP_OK(hipEventRecord(h->done, s));
+  h->pending = true;
+  return 0;
 }
 
 void collect(icam_hip* h, Stages& st) {
@@ -1987,10 +1972,14 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
                                     {"points_search32", "points_refine64", "points_fallback64", "points_search64", "points_leaves32", "points_leaves64",
                                      "points_mark", "points_settle64"}};
   const unsigned lb = r.items ? unsigned(std::min<long>((long(r.icap) + 255) / 256, 2048)) : 0;
-  if (r.items) HIP_OK(hipMemsetAsync(r.icount, 0, 4, s));
+  {
+    // the queues' counts and marks, cleared by one fill launch (no hipMemsetAsync: a
+    // caller may capture these launches in a graph)
+    FillList f{{r.items ? r.icount : nullptr, mixed ? r.qcount : nullptr, mixed ? r.qmark : nullptr, r.scount, r.smark, nullptr},
+               {r.items ? 1u : 0u, mixed ? 1u : 0u, mixed ? size_t(blocks) : 0u, 2u, size_t(blocks), 0u}};
+    HIP_OK(launch(st, "zero", s, dim3(unsigned(std::min<long>((blocks + 255) / 256, 4096))), dim3(256), kb_fill, f));
+  }
   if (mixed) {
-    HIP_OK(hipMemsetAsync(r.qcount, 0, 4, s));
-    HIP_OK(hipMemsetAsync(r.qmark, 0, size_t(blocks) * 4, s));
     HIP_OK(launch(st, names[S][0], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, float>, h->dv, j, r));
     if (lb) HIP_OK(launch(st, names[S][4], s, dim3(lb), dim3(256), k_leaves<S, float>, h->dv, j, r));
     if constexpr (S == SRC_NODES)
@@ -2006,8 +1995,6 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
   // what is still undecided: settled by the certified search (not the brick centres:
   // an undecided centre under its reach keeps its brick, whose nodes are read)
   if (S == SRC_CENTRES) return 0;
-  HIP_OK(hipMemsetAsync(r.scount, 0, 8, s));
-  HIP_OK(hipMemsetAsync(r.smark, 0, size_t(blocks) * 4, s));
   HIP_OK(launch(st, names[S][6], s, dim3(unsigned((j.n + 255) / 256)), dim3(256), k_mark_settle<S>, j, r));
   HIP_OK(launch(st, names[S][7], s, dim3(unsigned(std::min<long>(blocks, h->settle_grid))), dim3(NT), k_settle<S>, h->dv, j, r));
   return 0;
@@ -2112,6 +2099,12 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
     return fail(e);
   }
   HIP_OK(hipStreamCreate(&h->stream));
+  HIP_OK(hipEventCreateWithFlags(&h->done, hipEventDisableTiming));
+  // ICAM_SEARCH_ROUNDS: the searches' best-first rounds over parts, at most (4096)
+  if (const char* e = std::getenv("ICAM_SEARCH_ROUNDS")) {
+    const long r = std::atol(e);
+    if (r > 0) h->pk.L.max_rounds = uint32_t(r);
+  }
   h->dv.F32 = h->F32.as<float>();
   h->dv.F64 = h->F64.as<double>();
   h->dv.U = h->U.as<uint32_t>();
@@ -2131,6 +2124,7 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
 
 void icam_hip_close(icam_hip* h) {
   if (!h) return;
+  if (h->pending) hipEventSynchronize(h->done);
   if (h->stream) hipStreamSynchronize(h->stream);
   h->centres.free();
   h->nodes.free();
@@ -2139,6 +2133,7 @@ void icam_hip_close(icam_hip* h) {
                  &h->tile_cand, &h->tile_count, &h->super_cand, &h->super_count, &h->o_count, &h->o_cell, &h->o_rank, &h->o_idx, &h->o_xyz, &h->o_d, &h->o_flag, &h->o_feat})
     b->free();
   if (h->stream) hipStreamDestroy(h->stream);
+  if (h->done) hipEventDestroy(h->done);
   delete h;
 }
 
@@ -2153,6 +2148,7 @@ size_t icam_hip_nodes(icam_hip* h) { return h ? h->pk.nodes : 0; }
 int icam_hip_warm(icam_hip* h, double secs) {
   if (!h) return fail("no handle");
   std::lock_guard<std::mutex> g(h->mu);
+  if (after_previous(h, h->stream)) return 1;
   Buf x;
   HIP_OK(x.need(4096));
   HIP_OK(hipMemsetAsync(x.p, 0, 4096, h->stream));
@@ -2165,44 +2161,116 @@ int icam_hip_warm(icam_hip* h, double secs) {
   return 0;
 }
 
-static int query_impl(icam_hip* h, int precision, const double* d_xyz, const double* d_bounds, size_t n, double* d_d, uint32_t* d_flags,
-                      uint32_t* d_feat, hipStream_t s, Stages* st) {
+// The scratch of a query of n points: the cell order's counts, cells, ranks, order,
+// points and results in that order; the single-precision records; the fallback's and
+// the settle's block queues, their counts and marks; the statistics.
+struct QBufs {
+  uint32_t *count, *cell, *rank, *idx;
+  double* xyz;
+  double* d;
+  uint32_t *flag, *feat;
+  uint32_t *code, *info;
+  float *pa, *pb, *d32;
+  uint32_t *queue, *qcount, *qmark, *squeue, *scount, *smark;
+  unsigned long long* counters;
+};
+static bool query_order(size_t n, bool bounds) { return n >= ORDER_MIN && !bounds && n < (size_t(1) << 31); }
+
+// From the handle's buffers, grown as needed.
+static int query_bufs(icam_hip* h, size_t n, bool mixed, bool order, bool feat, QBufs& q) {
+  q = QBufs{};
+  const size_t m = std::max<size_t>(n, 1);
+  if (order) {
+    HIP_OK(h->o_count.need(ORDER_CELLS * 4));
+    HIP_OK(h->o_cell.need(m * 4));
+    HIP_OK(h->o_rank.need(m * 4));
+    HIP_OK(h->o_idx.need(m * 4));
+    HIP_OK(h->o_xyz.need(m * 24));
+    HIP_OK(h->o_d.need(m * 8));
+    HIP_OK(h->o_flag.need(m * 4));
+    if (feat) HIP_OK(h->o_feat.need(m * 4));
+    q.count = h->o_count.as<uint32_t>(), q.cell = h->o_cell.as<uint32_t>(), q.rank = h->o_rank.as<uint32_t>(), q.idx = h->o_idx.as<uint32_t>();
+    q.xyz = h->o_xyz.as<double>(), q.d = h->o_d.as<double>(), q.flag = h->o_flag.as<uint32_t>(), q.feat = feat ? h->o_feat.as<uint32_t>() : nullptr;
+  }
+  icam_hip::ResBufs& B = h->nodes;
+  if (mixed) {
+    HIP_OK(B.code.need(m * 4));
+    HIP_OK(B.pa.need(m * 4));
+    HIP_OK(B.pb.need(m * 4));
+    HIP_OK(B.d32.need(m * 4));
+    HIP_OK(B.info.need(m * 4));
+    HIP_OK(B.queue.need(m * 4));
+    HIP_OK(B.qmark.need(m * 4));
+    q.code = B.code.as<uint32_t>(), q.pa = B.pa.as<float>(), q.pb = B.pb.as<float>(), q.d32 = B.d32.as<float>(), q.info = B.info.as<uint32_t>();
+    q.queue = B.queue.as<uint32_t>(), q.qmark = B.qmark.as<uint32_t>();
+  }
+  HIP_OK(B.qcount.need(16));
+  HIP_OK(B.squeue.need(m * 4));
+  HIP_OK(B.smark.need(m * 4));
+  HIP_OK(B.scount.need(16));
+  HIP_OK(h->counters.need(64));
+  q.qcount = B.qcount.as<uint32_t>(), q.squeue = B.squeue.as<uint32_t>(), q.smark = B.smark.as<uint32_t>(), q.scount = B.scount.as<uint32_t>();
+  q.counters = h->counters.as<unsigned long long>();
+  return 0;
+}
+
+// The same carved from device memory at `base` (cap bytes); the bytes it takes (more
+// than cap: it does not fit, and q is not usable).
+static size_t query_carve(char* base, size_t cap, size_t n, bool mixed, bool order, bool feat, QBufs& q) {
+  q = QBufs{};
+  size_t at = 0;
+  auto take = [&](size_t bytes) -> char* {
+    char* p = base + at;
+    at = (at + bytes + 255) / 256 * 256;
+    return at <= cap ? p : nullptr;
+  };
+  const size_t m = std::max<size_t>(n, 1), blk = m / 64 + 1;  // blocks of 64 points or more
+  if (order) {
+    q.count = (uint32_t*)take(ORDER_CELLS * 4), q.cell = (uint32_t*)take(m * 4), q.rank = (uint32_t*)take(m * 4), q.idx = (uint32_t*)take(m * 4);
+    q.xyz = (double*)take(m * 24), q.d = (double*)take(m * 8), q.flag = (uint32_t*)take(m * 4);
+    q.feat = feat ? (uint32_t*)take(m * 4) : nullptr;
+  }
+  if (mixed) {
+    q.code = (uint32_t*)take(m * 4), q.pa = (float*)take(m * 4), q.pb = (float*)take(m * 4), q.d32 = (float*)take(m * 4), q.info = (uint32_t*)take(m * 4);
+    q.queue = (uint32_t*)take(blk * 4), q.qmark = (uint32_t*)take(blk * 4);
+  }
+  q.qcount = (uint32_t*)take(16), q.squeue = (uint32_t*)take(blk * 4), q.smark = (uint32_t*)take(blk * 4), q.scount = (uint32_t*)take(16);
+  q.counters = (unsigned long long*)take(64);
+  return at;
+}
+
+static int query_impl(icam_hip* h, int precision, const double* d_xyz, const double* d_bounds, size_t n, double* d_d, uint32_t* d_flags, uint32_t* d_feat,
+               hipStream_t s, Stages* st, const QBufs& q) {
   if (!n) return 0;
-  if (precision != ICAM_FP64 && precision != ICAM_MIXED) return fail("unknown precision");
   const bool mixed = precision == ICAM_MIXED;
   const unsigned blocks = unsigned((n + 255) / 256);
-  Res r{};
   Job j{};
   j.n = long(n), j.xyz = d_xyz, j.bounds = d_bounds;
   // scattered points without bounds of their own: searched in cell order
-  const bool order = n >= ORDER_MIN && !d_bounds && n < (size_t(1) << 31);
+  const bool order = query_order(n, d_bounds != nullptr);
+  {
+    FillList f{{q.counters ? (uint32_t*)q.counters : nullptr, order ? q.count : nullptr, nullptr, nullptr, nullptr, nullptr},
+               {16u, order ? size_t(ORDER_CELLS) : 0u, 0u, 0u, 0u, 0u}};
+    HIP_OK(launch(st, "zero", s, dim3(order ? 1024u : 1u), dim3(256), kb_fill, f));
+  }
   if (order) {
-    HIP_OK(h->o_count.need(ORDER_CELLS * 4));
-    HIP_OK(h->o_cell.need(n * 4));
-    HIP_OK(h->o_rank.need(n * 4));
-    HIP_OK(h->o_idx.need(n * 4));
-    HIP_OK(h->o_xyz.need(n * 24));
-    HIP_OK(h->o_d.need(n * 8));
-    HIP_OK(h->o_flag.need(n * 4));
-    if (d_feat) HIP_OK(h->o_feat.need(n * 4));
     double sc[3];
     for (int k = 0; k < 3; ++k) {
       const double side = h->box_hi[k] - h->box_lo[k];
       sc[k] = side > 0.0 ? double(1 << ORDER_BITS) / side : 0.0;
     }
-    uint32_t* count = h->o_count.as<uint32_t>();
-    HIP_OK(hipMemsetAsync(count, 0, ORDER_CELLS * 4, s));
     HIP_OK(launch(st, "order", s, dim3(blocks), dim3(256), k_order_cells, d_xyz, long(n), h->box_lo[0], h->box_lo[1], h->box_lo[2], sc[0], sc[1], sc[2],
-                  count, h->o_cell.as<uint32_t>(), h->o_rank.as<uint32_t>()));
-    HIP_OK(launch(st, "order", s, dim3(1), dim3(1024), k_order_scan, count));
-    HIP_OK(launch(st, "order", s, dim3(blocks), dim3(256), k_order_place, d_xyz, long(n), (const uint32_t*)h->o_cell.as<uint32_t>(),
-                  (const uint32_t*)h->o_rank.as<uint32_t>(), (const uint32_t*)count, h->o_idx.as<uint32_t>(), h->o_xyz.as<double>()));
-    j.xyz = h->o_xyz.as<double>();
-    if (results(h, true, long(n), r, h->o_d.as<double>(), h->o_flag.as<uint32_t>(), d_feat ? h->o_feat.as<uint32_t>() : nullptr, mixed)) return 1;
-  } else if (results(h, true, long(n), r, d_d, d_flags, d_feat, mixed)) {
-    return 1;
+                  q.count, q.cell, q.rank));
+    HIP_OK(launch(st, "order", s, dim3(1), dim3(1024), k_order_scan, q.count));
+    HIP_OK(launch(st, "order", s, dim3(blocks), dim3(256), k_order_place, d_xyz, long(n), (const uint32_t*)q.cell, (const uint32_t*)q.rank,
+                  (const uint32_t*)q.count, q.idx, q.xyz));
+    j.xyz = q.xyz;
   }
-  HIP_OK(hipMemsetAsync(h->counters.p, 0, 64, s));
+  Res r{};
+  r.code = q.code, r.pa = q.pa, r.pb = q.pb, r.d32 = q.d32, r.info = q.info;
+  r.queue = q.queue, r.qcount = q.qcount, r.qmark = q.qmark, r.squeue = q.squeue, r.scount = q.scount, r.smark = q.smark;
+  r.counters = q.counters;
+  r.val = order ? q.d : d_d, r.flag = order ? q.flag : d_flags, r.feat = order ? (d_feat ? q.feat : nullptr) : d_feat;
   // a part with B-spline faces or spline edges: four lanes a point (on the trimmed plate's
   // uniform points 2.2x faster than one; its points at the trim curves 1.2x slower)
   const bool wide = h->dv.L.n_leaves + h->dv.L.n_cleaves > 0;
@@ -2210,16 +2278,27 @@ static int query_impl(icam_hip* h, int precision, const double* d_xyz, const dou
   const long sblocks = (long(n) + ppb - 1) / ppb;
   if (wide ? run_search<SRC_WIDE>(h, mixed, j, r, s, st, sblocks) : run_search<SRC_POINTS>(h, mixed, j, r, s, st, sblocks)) return 1;
   if (order)
-    HIP_OK(launch(st, "order", s, dim3(blocks), dim3(256), k_order_back, long(n), (const uint32_t*)h->o_idx.as<uint32_t>(), (const double*)h->o_d.as<double>(),
-                  (const uint32_t*)h->o_flag.as<uint32_t>(), (const uint32_t*)(d_feat ? h->o_feat.as<uint32_t>() : nullptr), d_d, d_flags, d_feat));
+    HIP_OK(launch(st, "order", s, dim3(blocks), dim3(256), k_order_back, long(n), (const uint32_t*)q.idx, (const double*)q.d, (const uint32_t*)q.flag,
+                  (const uint32_t*)(d_feat ? q.feat : nullptr), d_d, d_flags, d_feat));
   return 0;
+}
+
+// A query on the handle's own scratch.
+static int query_handle(icam_hip* h, int precision, const double* d_xyz, const double* d_bounds, size_t n, double* d_d, uint32_t* d_flags, uint32_t* d_feat,
+                 hipStream_t s, Stages* st) {
+  if (!n) return 0;
+  if (precision != ICAM_FP64 && precision != ICAM_MIXED) return fail("unknown precision");
+  QBufs q;
+  if (query_bufs(h, n, precision == ICAM_MIXED, query_order(n, d_bounds != nullptr), d_feat != nullptr, q)) return 1;
+  return query_impl(h, precision, d_xyz, d_bounds, n, d_d, d_flags, d_feat, s, st, q);
 }
 
 int icam_hip_query_device(icam_hip* h, int precision, const double* d_xyz, size_t n, double* d_d, uint32_t* d_flags, uint32_t* d_feat, void* stream) {
   if (!h) return fail("no handle");
   std::lock_guard<std::mutex> g(h->mu);
   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-  return query_impl(h, precision, d_xyz, nullptr, n, d_d, d_flags, d_feat, s, nullptr);
+  if (after_previous(h, s) || query_handle(h, precision, d_xyz, nullptr, n, d_d, d_flags, d_feat, s, nullptr)) return 1;
+  return mark_end(h, s);
 }
 
 int icam_hip_query(icam_hip* h, int precision, const double* xyz, size_t n, double* d, uint32_t* flags, uint32_t* feat) {
@@ -2230,15 +2309,17 @@ int icam_hip_query(icam_hip* h, int precision, const double* xyz, size_t n, doub
   HIP_OK(h->q_d.need(n * 8));
   HIP_OK(h->q_flag.need(n * 4));
   HIP_OK(h->q_feat.need(n * 4));
+  if (after_previous(h, h->stream)) return 1;
   HIP_OK(hipMemcpyAsync(h->q_xyz.p, xyz, n * 24, hipMemcpyHostToDevice, h->stream));
   Stages st;
-  if (query_impl(h, precision, h->q_xyz.as<double>(), nullptr, n, h->q_d.as<double>(), h->q_flag.as<uint32_t>(), h->q_feat.as<uint32_t>(), h->stream,
-                 &st))
+  if (query_handle(h, precision, h->q_xyz.as<double>(), nullptr, n, h->q_d.as<double>(), h->q_flag.as<uint32_t>(), h->q_feat.as<uint32_t>(), h->stream,
+                   &st))
     return 1;
   HIP_OK(hipMemcpyAsync(d, h->q_d.p, n * 8, hipMemcpyDeviceToHost, h->stream));
   if (flags) HIP_OK(hipMemcpyAsync(flags, h->q_flag.p, n * 4, hipMemcpyDeviceToHost, h->stream));
   if (feat) HIP_OK(hipMemcpyAsync(feat, h->q_feat.p, n * 4, hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
+  h->pending = false;
   collect(h, st);
   return 0;
 }
@@ -2256,16 +2337,18 @@ int icam_hip_points(icam_hip* h, int precision, const icam_point_rec* recs, size
   HIP_OK(h->q_bound.need(n * 8));
   HIP_OK(h->q_d.need(n * 8));
   HIP_OK(h->q_flag.need(n * 4));
+  if (after_previous(h, h->stream)) return 1;
   HIP_OK(hipMemcpyAsync(h->q_xyz.p, xyz.data(), n * 24, hipMemcpyHostToDevice, h->stream));
   HIP_OK(hipMemcpyAsync(h->q_bound.p, bd.data(), n * 8, hipMemcpyHostToDevice, h->stream));
-  if (query_impl(h, precision, h->q_xyz.as<double>(), h->q_bound.as<double>(), n, h->q_d.as<double>(), h->q_flag.as<uint32_t>(), nullptr,
-                 h->stream, nullptr))
+  if (query_handle(h, precision, h->q_xyz.as<double>(), h->q_bound.as<double>(), n, h->q_d.as<double>(), h->q_flag.as<uint32_t>(), nullptr,
+                   h->stream, nullptr))
     return 1;
   std::vector<double> d(n);
   std::vector<uint32_t> f(n);
   HIP_OK(hipMemcpyAsync(d.data(), h->q_d.p, n * 8, hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipMemcpyAsync(f.data(), h->q_flag.p, n * 4, hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
+  h->pending = false;
   for (size_t i = 0; i < n; ++i) out[2 * i] = d[i], out[2 * i + 1] = double(f[i]);
   return 0;
 }
@@ -2276,6 +2359,7 @@ int icam_hip_bricks(icam_hip* h, int precision, const icam_brick_rec* recs, size
   if (!n) return 0;
   if (precision != ICAM_FP64 && precision != ICAM_MIXED) return fail("unknown precision");
   HIP_OK(h->firsts.need(n * 32));
+  if (after_previous(h, h->stream)) return 1;
   HIP_OK(hipMemcpyAsync(h->firsts.p, recs, n * 32, hipMemcpyHostToDevice, h->stream));
   Res r{};
   long np = long(n) * 729;
@@ -2288,6 +2372,7 @@ int icam_hip_bricks(icam_hip* h, int precision, const icam_brick_rec* recs, size
   HIP_OK(hipMemcpyAsync(v.data(), r.val, np * 8, hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipMemcpyAsync(f.data(), r.flag, np * 4, hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
+  h->pending = false;
   for (long i = 0; i < np; ++i) out[2 * i] = v[i], out[2 * i + 1] = double(f[i]);
   return 0;
 }
@@ -2407,7 +2492,7 @@ int icam_hip_bake_device(icam_hip* h, int precision, const icam_level* lv, uint3
   std::lock_guard<std::mutex> g(h->mu);
   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
   icam_bake_stats tmp{};
-  if (bake_impl(h, precision, lv, s, stats ? stats : &tmp)) return 1;
+  if (after_previous(h, s) || bake_impl(h, precision, lv, s, stats ? stats : &tmp) || mark_end(h, s)) return 1;
   if (d_map) *d_map = h->map.as<uint32_t>();
   if (d_atlas) *d_atlas = h->atlas.as<uint32_t>();
   return 0;
@@ -2417,7 +2502,7 @@ int icam_hip_bake(icam_hip* h, int precision, const icam_level* lv, uint32_t* ma
   if (!h || !lv) return fail("bad arguments");
   std::lock_guard<std::mutex> g(h->mu);
   icam_bake_stats st{};
-  if (bake_impl(h, precision, lv, h->stream, &st)) return 1;
+  if (after_previous(h, h->stream) || bake_impl(h, precision, lv, h->stream, &st) || mark_end(h, h->stream)) return 1;
   if (stats) *stats = st;
   long nb = long(lv->nb[0]) * lv->nb[1] * lv->nb[2];
   HIP_OK(hipMemcpyAsync(map, h->map.p, nb * 4, hipMemcpyDeviceToHost, h->stream));
@@ -2652,6 +2737,12 @@ int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t
     if (double(lv.nb[0]) * lv.nb[1] * lv.nb[2] >= double(1 << 24)) return fail("level " + std::to_string(i) + " has 2^24 bricks or more");
     if (sign_pitch(lv.nb[0]) * lv.nb[1] * lv.nb[2] > SIGN_LDS) return fail("level " + std::to_string(i) + " is too large for the batched sign pass");
     caps[i] = band_room(parts[i], lv);
+    // ICAM_BATCH_ROOM: band bricks a part has room for, at most (a smaller room makes a
+    // band outgrow it, which the results then report)
+    if (const char* e = std::getenv("ICAM_BATCH_ROOM")) {
+      const long r = std::atol(e);
+      if (r > 0) caps[i] = std::min<uint32_t>(caps[i], uint32_t(r));
+    }
     work[i] = double(caps[i]) * (1.0 + double(parts[i]->pk.L.n_prims));
   }
   B->caller.resize(n);
@@ -2776,17 +2867,20 @@ int icam_hip_batch_result(icam_batch* b, size_t i, const uint32_t** d_map, const
   if (!b || i >= size_t(b->n)) return fail("bad arguments");
   const int k = b->pos[i];
   const BPart& P = b->host[k];
+  // the failures are read on every call, before any result is handed out: a part
+  // whose band outgrew its room, a brick kept whole with no sign
+  uint32_t over = 0;
+  unsigned long long cnt[8];
+  HIP_OK(hipMemcpy(&over, b->ctl.as<uint32_t>() + BC_OVER, 4, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(cnt, P.counters, 64, hipMemcpyDeviceToHost));
+  if (over) return fail("the band of the batch's part " + std::to_string(b->caller[over - 1]) + " outgrew its room");
+  if (cnt[2]) return fail(std::to_string(cnt[2]) + " bricks kept whole with no sign found (no surface near them and none outside the part's box)");
   if (d_map) *d_map = P.map;
   if (d_atlas) *d_atlas = P.atlas;
   if (stats) {
-    uint32_t over = 0, bt = 0, sl = 0;
-    unsigned long long cnt[8];
-    HIP_OK(hipMemcpy(&over, b->ctl.as<uint32_t>() + BC_OVER, 4, hipMemcpyDeviceToHost));
+    uint32_t bt = 0, sl = 0;
     HIP_OK(hipMemcpy(&bt, b->btotal.as<uint32_t>() + k, 4, hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy(&sl, b->stotal.as<uint32_t>() + k, 4, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(cnt, P.counters, 64, hipMemcpyDeviceToHost));
-    if (over) return fail("the band of the batch's part " + std::to_string(b->caller[over - 1]) + " outgrew its room");
-    if (cnt[2]) return fail(std::to_string(cnt[2]) + " bricks kept whole with no sign found (no surface near them and none outside the part's box)");
     stats->bricks = P.nb, stats->band = bt, stats->slots = sl;
     stats->undecided = int64_t(cnt[0]), stats->refined = int64_t(cnt[1]);
     stats->unsure_trim = int64_t(cnt[3]), stats->unsure_tie = int64_t(cnt[4]), stats->settled = int64_t(cnt[5]);

@@ -74,6 +74,7 @@ struct Layout {
   float eps32;   // rounding allowance on single-precision lengths
   double eps64;  // on double-precision lengths
   double diag;   // the part's box diagonal
+  uint32_t max_rounds;  // a search's best-first rounds over parts, at most
 };
 
 }  // namespace icam

@@ -1205,6 +1205,7 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
   double ext = std::max(0.5 * L.diag, 1.0);
   L.eps32 = float(8.0 * 1.1920929e-7 * ext);  // partgpu.rs:139
   L.eps64 = 8.0 * 2.220446049250313e-16 * ext;
+  L.max_rounds = 4096;
   return true;
 }
 
