@@ -1823,6 +1823,36 @@ struct Buf {
 
 }  // namespace
 
+// The scratch of icam_hip_query_device_noalloc: one static array a device (the
+// runtime gives every device that loads the library its own copy, held until the process
+// ends). Host code uses the device address hipGetSymbolAddress gives for it
+// (noalloc_scratch), never the array's name.
+constexpr size_t NOALLOC_POINTS = size_t(1) << 18;
+constexpr size_t NOALLOC_BYTES = size_t(24) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+std::mutex g_noalloc_mu;
+constexpr int MAX_DEVICES = 64;
+void* g_noalloc[MAX_DEVICES] = {};  // per device: the scratch array's device address
+
+// Looks the array's address up once per device, with that device current.
+int noalloc_setup(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return fail("device number out of range");
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  if (g_noalloc[device]) return 0;
+  void* ptr = nullptr;
+  HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return fail("no device address for the scratch array");
+  g_noalloc[device] = ptr;
+  return 0;
+}
+void* noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  return device >= 0 && device < MAX_DEVICES ? g_noalloc[device] : nullptr;
+}
+}  // namespace
+
 struct icam_hip {
   std::mutex mu;
   int device = 0;
@@ -2085,6 +2115,11 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
   HIP_OK(hipGetDeviceProperties(&prop, h->device));
   h->name = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ", " + std::to_string(prop.multiProcessorCount) + " CUs)";
   h->settle_grid = settle_grid(k_settle<SRC_WIDE>, Shape<SRC_WIDE>::NT, prop.multiProcessorCount);
+  if (noalloc_setup(h->device)) {
+    std::string e = g_error;
+    delete h;
+    return fail(e);
+  }
   if (!pack_part(part, need, h->pk)) {
     std::string e = h->pk.error;
     delete h;
@@ -2298,6 +2333,23 @@ int icam_hip_query_device(icam_hip* h, int precision, const double* d_xyz, size_
   std::lock_guard<std::mutex> g(h->mu);
   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
   if (after_previous(h, s) || query_handle(h, precision, d_xyz, nullptr, n, d_d, d_flags, d_feat, s, nullptr)) return 1;
+  return mark_end(h, s);
+}
+
+int icam_hip_query_device_noalloc(icam_hip* h, int precision, const double* d_xyz, size_t n, double* d_d, uint32_t* d_flags, uint32_t* d_feat,
+                                  void* stream) {
+  if (!h) return fail("no handle");
+  if (precision != ICAM_FP64 && precision != ICAM_MIXED) return fail("unknown precision");
+  if (n > NOALLOC_POINTS) return fail("icam_hip_query_device_noalloc takes at most " + std::to_string(NOALLOC_POINTS) + " points a call, not " + std::to_string(n));
+  if (!n) return 0;
+  std::lock_guard<std::mutex> g(h->mu);
+  char* base = static_cast<char*>(noalloc_scratch(h->device));
+  if (!base) return fail("the scratch array of device " + std::to_string(h->device) + " is not set up");
+  QBufs q;
+  const size_t bytes = query_carve(base, NOALLOC_BYTES, n, precision == ICAM_MIXED, query_order(n, false), d_feat != nullptr, q);
+  if (bytes > NOALLOC_BYTES) return fail("the query needs " + std::to_string(bytes) + " bytes of scratch, the array holds " + std::to_string(NOALLOC_BYTES));
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  if (after_previous(h, s) || query_impl(h, precision, d_xyz, nullptr, n, d_d, d_flags, d_feat, s, nullptr, q)) return 1;
   return mark_end(h, s);
 }
 

@@ -47,6 +47,8 @@ def lib():
         L.icam_hip_query.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_size_t, _dp, _u32p, _u32p]
         L.icam_hip_query_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p]
+        L.icam_hip_query_device_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_void_p]
         L.icam_hip_bake.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, _u32p, _u32p, ctypes.c_size_t, ctypes.POINTER(BakeStats)]
         L.icam_hip_bake_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
                                            ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(BakeStats), ctypes.c_void_p]
@@ -131,6 +133,11 @@ class PartGpu:
     def query_device(self, d_xyz, n, d_d, d_flags, d_feat=None, precision=FP64, stream=None):
         """The same on device pointers (ints), enqueued on `stream` (a hipStream_t as int)."""
         _check(lib().icam_hip_query_device(self.h, precision, d_xyz, n, d_d, d_flags, d_feat, stream))
+
+    def query_device_noalloc(self, d_xyz, n, d_d, d_flags, d_feat=None, precision=FP64, stream=None):
+        """query_device with no allocation (scratch from the library's static array, at
+        most 2^18 points a call)."""
+        _check(lib().icam_hip_query_device_noalloc(self.h, precision, d_xyz, n, d_d, d_flags, d_feat, stream))
 
     def query_stats(self):
         """Statistics of the last query: points searched again in double precision (for

@@ -99,6 +99,17 @@ int icam_hip_query(icam_hip* h, int precision, const double* xyz, size_t n, doub
  * (a hipStream_t; NULL: the default stream); returns without waiting. */
 int icam_hip_query_device(icam_hip* h, int precision, const double* d_xyz, size_t n, double* d_d, uint32_t* d_flags, uint32_t* d_feat, void* stream);
 
+/* No counterpart in the reference: icam_hip_query_device for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). It allocates nothing: its scratch (cell order, single-precision
+ * records, queues, counters) is carved from one static device array of 24 MiB a GPU,
+ * held until the process ends, so a call takes at most 2^18 points (more fail). The
+ * arguments and results are icam_hip_query_device's; it returns without waiting. Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (from different
+ * handles, or on streams not ordered with each other) race. Calls on one handle are
+ * ordered after each other like the handle's other calls. */
+int icam_hip_query_device_noalloc(icam_hip* h, int precision, const double* d_xyz, size_t n, double* d_d, uint32_t* d_flags, uint32_t* d_feat, void* stream);
+
 /* Statistics of a bake (or of a query: icam_hip_query_stats). */
 typedef struct icam_bake_stats {
   int64_t bricks;      /* bricks in the level */

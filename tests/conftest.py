@@ -26,3 +26,16 @@ def built():
         build.build_gen()
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, capture_output=True)
     return True
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _torch_gpu_first(request):
+    """Modules of GPU tests (some hand torch's device memory to the library) start
+    torch's HIP runtime before any fixture opens a handle: torch finds no GPU when the
+    library's own HIP calls came first in the process."""
+    if request.node.get_closest_marker("gpu") is not None:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    yield
