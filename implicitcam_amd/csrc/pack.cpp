@@ -16,6 +16,8 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -436,157 +438,287 @@ void put_iv(double* out, const Iv* iv) {
   }
 }
 
-// surface3.rs:819-830, 866-926: the radius of convexity of a rational bicubic, from
-// interval bounds on its derivatives over its Bernstein hulls (relative to the
-// points' middle), with lambda_min(G) bounded two ways; `cert` (when not NULL)
-// receives the bounds themselves (PATCH_CERT words)
-double patch_radius(const Patch4& pw, double* cert = nullptr) {
-  if (cert) open_cert(cert, PATCH_CERT);
-  const int du = 3, dv = 3;
+// ---- bounds over a rational Bezier patch or piece, from exact Bernstein forms
+//
+// A rational bicubic S = P / w (P the homogeneous x, y, z part, w the weights; polynomials
+// of bidegree (3, 3) in Bernstein form) has derivatives whose denominators are powers of w:
+//   S_s  = N_s / w^2,                       N_s = P_s w - P w_s        bidegree (5, 6)
+//   S_t  = N_t / w^2,                       N_t = P_t w - P w_t        (6, 5)
+//   S_ss = (N_s,s w - 2 N_s w_s) / w^3                                 (7, 9)
+//   S_st = (N_s,t w - 2 N_s w_t) / w^3                                 (8, 8)
+//   S_tt = (N_t,t w - 2 N_t w_t) / w^3                                 (9, 7)
+// and its Gram matrix G = [S_s.S_s, S_s.S_t; S_t.S_s, S_t.S_t] has entries (N_s.N_s,
+// N_s.N_t, N_t.N_t) / w^4, numerators of bidegree (10, 12), (11, 11), (12, 10). Products
+// and derivatives of Bernstein forms are exact operations on their coefficients, and a
+// Bernstein form lies within the hull of its coefficients over the unit square, so each
+// quantity lies in the quotient of its numerator's hull by its (positive) denominator's
+// hull. The forms are exact in the coefficients of P - c w (c the control points' middle,
+// which leaves every derivative unchanged and keeps the coefficients small).
+//
+// The forms are kept scaled, coefficient (i, j) times binom(m, i) binom(n, j): a product
+// is then a plain convolution of the scaled coefficients, and a derivative along s has
+// scaled coefficients (i + 1) a_(i+1) j - (m - i) a_ij. Each form carries one bound E on
+// the rounding error of every scaled coefficient: a product's is (k + 2) u |A|_1 |B|_max
+// + |A|_1 E_B + |B|_1 E_A + k E_A E_B (k terms a coefficient at most, u = 2^-53), and a
+// hull widens each coefficient by E over its binomials. The least eigenvalue of G is
+// bounded below from the entries' enclosures: it grows with either diagonal entry and
+// shrinks as |G_12| grows, so it is at least the least eigenvalue of [a_lo, b_mag; b_mag,
+// c_lo], less the rounding of that formula. A piece (a rational cubic curve) is the same
+// with one parameter.
+constexpr double U53 = 1.1102230246251565e-16;
+constexpr int BMAX = 13;  // coefficients a side, at most (bidegree 12)
+struct BForm {
+  int m = 0, n = 0;        // bidegree
+  double c[BMAX * BMAX];   // scaled coefficients, (i, j) at i (n + 1) + j
+  double E = 0.0;          // rounding error bound of every scaled coefficient
+};
+const double* binom_row(int n) {
+  static const struct Table {
+    double t[BMAX][BMAX];
+    Table() {
+      for (int a = 0; a < BMAX; ++a) {
+        t[a][0] = 1.0;
+        for (int b = 1; b < BMAX; ++b) t[a][b] = b > a ? 0.0 : t[a - 1][b - 1] + (b <= a - 1 ? t[a - 1][b] : 0.0);
+      }
+    }
+  } T;
+  return T.t[n];
+}
+double l1(const BForm& A) {
+  double s = 0;
+  for (int q = 0; q < (A.m + 1) * (A.n + 1); ++q) s += std::fabs(A.c[q]);
+  return s;
+}
+double linf(const BForm& A) {
+  double s = 0;
+  for (int q = 0; q < (A.m + 1) * (A.n + 1); ++q) s = std::max(s, std::fabs(A.c[q]));
+  return s;
+}
+// the derivative along the first (s) or second (t) parameter
+BForm bf_d(const BForm& A, bool along_s) {
+  BForm R;
+  R.m = along_s ? A.m - 1 : A.m, R.n = along_s ? A.n : A.n - 1;
+  double big = 0;
+  for (int i = 0; i <= R.m; ++i)
+    for (int j = 0; j <= R.n; ++j) {
+      double x, y;
+      if (along_s)
+        x = (i + 1) * A.c[(i + 1) * (A.n + 1) + j], y = (A.m - i) * A.c[i * (A.n + 1) + j];
+      else
+        x = (j + 1) * A.c[i * (A.n + 1) + j + 1], y = (A.n - j) * A.c[i * (A.n + 1) + j];
+      R.c[i * (R.n + 1) + j] = x - y;
+      big = std::max(big, std::fabs(x) + std::fabs(y));
+    }
+  R.E = double(along_s ? A.m + 1 : A.n + 1) * A.E + 3.0 * U53 * big;
+  return R;
+}
+// the product
+BForm bf_mul(const BForm& A, const BForm& B) {
+  BForm R;
+  R.m = A.m + B.m, R.n = A.n + B.n;
+  const int N = (R.m + 1) * (R.n + 1), rn = R.n + 1;
+  for (int q = 0; q < N; ++q) R.c[q] = 0.0;
+  for (int i1 = 0; i1 <= A.m; ++i1)
+    for (int j1 = 0; j1 <= A.n; ++j1) {
+      const double a = A.c[i1 * (A.n + 1) + j1];
+      double* r = R.c + i1 * rn + j1;
+      const double* b = B.c;
+      for (int i2 = 0; i2 <= B.m; ++i2, b += B.n + 1)
+        for (int j2 = 0; j2 <= B.n; ++j2) r[i2 * rn + j2] += a * b[j2];
+    }
+  const double k = double((std::min(A.m, B.m) + 1) * (std::min(A.n, B.n) + 1));
+  const double la = l1(A), lb = l1(B);
+  R.E = ((k + 2.0) * U53 * la * linf(B) + la * B.E + lb * A.E + k * A.E * B.E) * (1.0 + 1e-10);
+  return R;
+}
+// A - k B (the same bidegree), and A + B
+BForm bf_sub(const BForm& A, const BForm& B, double k = 1.0) {
+  BForm R;
+  R.m = A.m, R.n = A.n;
+  double big = 0;
+  for (int q = 0; q < (R.m + 1) * (R.n + 1); ++q) {
+    const double kb = k * B.c[q];
+    R.c[q] = A.c[q] - kb;
+    big = std::max(big, std::fabs(A.c[q]) + std::fabs(kb));
+  }
+  R.E = A.E + std::fabs(k) * B.E + 2.0 * U53 * big;
+  return R;
+}
+BForm bf_add(const BForm& A, const BForm& B) { return bf_sub(A, B, -1.0); }
+// the hull of the coefficients (unscaled), widened by their errors
+Iv bf_hull(const BForm& A) {
+  Iv r{INF, -INF};
+  const double* bm = binom_row(A.m);
+  const double* bn = binom_row(A.n);
+  for (int i = 0; i <= A.m; ++i)
+    for (int j = 0; j <= A.n; ++j) {
+      const double s = bm[i] * bn[j], v = A.c[i * (A.n + 1) + j] / s, e = A.E / s + 2.0 * U53 * std::fabs(v);
+      r.a = std::min(r.a, v - e), r.b = std::max(r.b, v + e);
+    }
+  return r;
+}
+// x / w for w a positive interval, widened by the rounding of the reciprocals and products
+Iv div_hull(Iv x, Iv w) {
+  Iv r = divpos(x, w);
+  const double pad = 4.0 * U53 * std::max(std::fabs(r.a), std::fabs(r.b));
+  return {r.a - pad, r.b + pad};
+}
+
+// The scaled forms of a patch: component k of P - c w (k < 3), or w (k = 3).
+BForm patch_form(const Patch4& pw, V c, int k) {
+  BForm R;
+  R.m = R.n = 3;
+  const double* b3 = binom_row(3);
+  const double ck = k == 0 ? c.x : k == 1 ? c.y : c.z;
+  double big = 0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const H& h = pw.p[i][j];
+      const double sc = b3[i] * b3[j], cw = k == 3 ? 0.0 : ck * h[3], v = k == 3 ? h[3] : h[k] - cw;
+      R.c[i * 4 + j] = v * sc;
+      big = std::max(big, (std::fabs(k == 3 ? h[3] : h[k]) + std::fabs(cw)) * sc);
+    }
+  R.E = (k == 3 ? 1.0 : 3.0) * U53 * big;
+  return R;
+}
+
+// Enclosures over a patch: lambda_min(G) from below and S_s, S_t, S_ss, S_st, S_tt, all
+// in parameters scaled by the tangents' largest lengths (any scaling D of the parameters
+// turns G into D G D and the Hessian of the squared distance into D H D, which keeps
+// every convexity test made with them); false when a weight is not positive. G's
+// entries are first enclosed from the tangents' enclosures (products of intervals); when
+// the radius that gives falls short of `enough`, also from G's own numerators (exact,
+// which keeps a tangent turning over the patch from driving a component's interval
+// through zero), and the larger bound is kept.
+struct PatchBounds {
+  double lambda;
+  Iv d[5][3];
+};
+double lambda2(double alo, double bm, double clo) {
+  const double lam = 0.5 * ((alo + clo) - std::sqrt((alo - clo) * (alo - clo) + 4.0 * bm * bm));
+  return lam - 16.0 * U53 * (alo + clo + 2.0 * bm);
+}
+double frob(const Iv d[5][3]) {
+  auto mag2 = [](const Iv* v) { return mag(v[0]) * mag(v[0]) + mag(v[1]) * mag(v[1]) + mag(v[2]) * mag(v[2]); };
+  return std::sqrt(mag2(d[2]) + 2 * mag2(d[3]) + mag2(d[4])) * (1 + 1e-12);
+}
+bool patch_bounds(const Patch4& pw, PatchBounds& out, double enough) {
   V c{0, 0, 0};
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) c = c + point(pw.p[i][j]);
   c = c * (1.0 / 16);
-  auto q = [&](int i, int j, int k) {
-    const H& h = pw.p[i][j];
-    double cc[3] = {c.x, c.y, c.z};
-    return k < 3 ? h[k] - cc[k] * h[3] : h[3];
-  };
-  auto hull = [&](int which, int k) {
-    Iv r{INF, -INF};
-    for (int i = 0; i <= du; ++i)
-      for (int j = 0; j <= dv; ++j) {
-        double x;
-        if (which == 0) x = q(i, j, k);
-        else if (which == 1) { if (i >= du) continue; x = du * (q(i + 1, j, k) - q(i, j, k)); }
-        else if (which == 2) { if (j >= dv) continue; x = dv * (q(i, j + 1, k) - q(i, j, k)); }
-        else if (which == 3) { if (i + 1 >= du) continue; x = du * (du - 1.0) * (q(i + 2, j, k) - 2 * q(i + 1, j, k) + q(i, j, k)); }
-        else if (which == 4) { if (i >= du || j >= dv) continue; x = du * dv * (q(i + 1, j + 1, k) - q(i + 1, j, k) - q(i, j + 1, k) + q(i, j, k)); }
-        else { if (j + 1 >= dv) continue; x = dv * (dv - 1.0) * (q(i, j + 2, k) - 2 * q(i, j + 1, k) + q(i, j, k)); }
-        r.a = std::min(r.a, x), r.b = std::max(r.b, x);
-      }
-    return r;
-  };
-  Iv A[6][4];
-  for (int w = 0; w < 6; ++w)
-    for (int k = 0; k < 4; ++k) A[w][k] = hull(w, k);
-  Iv wt = A[0][3];
-  if (!(wt.a > 0)) return 0.0;
-  Iv s[3];
+  const BForm w = patch_form(pw, c, 3);
+  if (!(bf_hull(w).a > 0)) return false;
+  const BForm ws = bf_d(w, true), wt = bf_d(w, false);
+  const BForm w2 = bf_mul(w, w), w3 = bf_mul(w2, w);
+  const Iv h2 = bf_hull(w2), h3 = bf_hull(w3);
+  if (!(h2.a > 0 && h3.a > 0)) return false;
+  BForm Ns[3], Nt[3];
+  Iv S[5][3];
   for (int k = 0; k < 3; ++k) {
-    Iv r{INF, -INF};
-    for (int i = 0; i < 4; ++i)
-      for (int j = 0; j < 4; ++j) {
-        V pp = point(pw.p[i][j]) - c;
-        double x = k == 0 ? pp.x : k == 1 ? pp.y : pp.z;
-        r.a = std::min(r.a, x), r.b = std::max(r.b, x);
-      }
-    s[k] = r;
+    const BForm q = patch_form(pw, c, k);
+    Ns[k] = bf_sub(bf_mul(bf_d(q, true), w), bf_mul(q, ws));
+    Nt[k] = bf_sub(bf_mul(bf_d(q, false), w), bf_mul(q, wt));
+    S[0][k] = div_hull(bf_hull(Ns[k]), h2);
+    S[1][k] = div_hull(bf_hull(Nt[k]), h2);
+    S[2][k] = div_hull(bf_hull(bf_sub(bf_mul(bf_d(Ns[k], true), w), bf_mul(Ns[k], ws), 2.0)), h3);
+    S[3][k] = div_hull(bf_hull(bf_sub(bf_mul(bf_d(Ns[k], false), w), bf_mul(Ns[k], wt), 2.0)), h3);
+    S[4][k] = div_hull(bf_hull(bf_sub(bf_mul(bf_d(Nt[k], false), w), bf_mul(Nt[k], wt), 2.0)), h3);
   }
-  Iv ss[3], st[3], sss[3], sst[3], stt[3];
+  // G's entries from the tangents' enclosures: |S_s|^2 in [sum mig^2, sum mag^2], S_s.S_t
+  // in the sum of the interval products
+  double alo = 0, ahi = 0, clo = 0, chi = 0;
+  Iv b{0, 0};
   for (int k = 0; k < 3; ++k) {
-    ss[k] = divpos(sub(A[1][k], mul(s[k], A[1][3])), wt);
-    st[k] = divpos(sub(A[2][k], mul(s[k], A[2][3])), wt);
+    alo += mig(S[0][k]) * mig(S[0][k]), ahi += mag(S[0][k]) * mag(S[0][k]);
+    clo += mig(S[1][k]) * mig(S[1][k]), chi += mag(S[1][k]) * mag(S[1][k]);
+    b = add(b, mul(S[0][k], S[1][k]));
   }
-  for (int k = 0; k < 3; ++k) {
-    sss[k] = divpos(sub(sub(A[3][k], scale(mul(ss[k], A[1][3]), 2.0)), mul(s[k], A[3][3])), wt);
-    sst[k] = divpos(sub(sub(sub(A[4][k], mul(ss[k], A[2][3])), mul(st[k], A[1][3])), mul(s[k], A[4][3])), wt);
-    stt[k] = divpos(sub(sub(A[5][k], scale(mul(st[k], A[2][3]), 2.0)), mul(s[k], A[5][3])), wt);
-  }
-  V p0, ps0, pt0;
-  eval_patch(pw, 0.5, 0.5, p0, ps0, pt0);
-  double sa = len(ps0), sb = len(pt0);
-  if (!(sa > 0)) sa = 1;
-  if (!(sb > 0)) sb = 1;
-  for (int k = 0; k < 3; ++k) {
-    ss[k] = scale(ss[k], 1 / sa), st[k] = scale(st[k], 1 / sb);
-    sss[k] = scale(sss[k], 1 / (sa * sa)), sst[k] = scale(sst[k], 1 / (sa * sb)), stt[k] = scale(stt[k], 1 / (sb * sb));
-  }
-  ps0 = ps0 * (1 / sa), pt0 = pt0 * (1 / sb);
-  auto mig2 = [](const Iv* a) { return mig(a[0]) * mig(a[0]) + mig(a[1]) * mig(a[1]) + mig(a[2]) * mig(a[2]); };
-  double ga = mig2(ss), gc = mig2(st);
-  double gb = mag(add(add(mul(ss[0], st[0]), mul(ss[1], st[1])), mul(ss[2], st[2])));
-  double by_entries = 0.5 * ((ga + gc) - std::sqrt((ga - gc) * (ga - gc) + 4 * gb * gb));
-  double a0 = dot(ps0, ps0), b0 = dot(ps0, pt0), c0 = dot(pt0, pt0);
-  double sigma0 = std::sqrt(std::max(0.5 * ((a0 + c0) - std::sqrt((a0 - c0) * (a0 - c0) + 4 * b0 * b0)), 0.0));
-  auto dev = [](const Iv* iv, V x) {
-    double xs[3] = {x.x, x.y, x.z}, s2 = 0;
-    for (int k = 0; k < 3; ++k) {
-      double m = std::max(std::fabs(iv[k].a - xs[k]), std::fabs(iv[k].b - xs[k]));
-      s2 += m * m;
+  alo *= 1 - 8 * U53, ahi *= 1 + 8 * U53, clo *= 1 - 8 * U53, chi *= 1 + 8 * U53;
+  const double bmag = mag(b) * (1 + 8 * U53);
+  // the scaling: the tangents' largest lengths
+  const double sa = ahi > 0 ? std::sqrt(ahi) : 1.0, sb = chi > 0 ? std::sqrt(chi) : 1.0;
+  const double k_s[5] = {1 / sa, 1 / sb, 1 / (sa * sa), 1 / (sa * sb), 1 / (sb * sb)};
+  for (int q = 0; q < 5; ++q)
+    for (int k = 0; k < 3; ++k) out.d[q][k] = scale(S[q][k], k_s[q]);
+  double lam = lambda2(alo / (sa * sa), bmag / (sa * sb), clo / (sb * sb));
+  const double hh = frob(out.d);
+  if (!(hh > 0 && 0.99 * lam / hh >= enough)) {
+    // G's entries from their own numerators
+    const BForm w4 = bf_mul(w2, w2);
+    const Iv h4 = bf_hull(w4);
+    if (h4.a > 0) {
+      BForm ga = bf_mul(Ns[0], Ns[0]), gb = bf_mul(Ns[0], Nt[0]), gc = bf_mul(Nt[0], Nt[0]);
+      for (int k = 1; k < 3; ++k) ga = bf_add(ga, bf_mul(Ns[k], Ns[k])), gb = bf_add(gb, bf_mul(Ns[k], Nt[k])), gc = bf_add(gc, bf_mul(Nt[k], Nt[k]));
+      const Iv ea = div_hull(bf_hull(ga), h4), eb = div_hull(bf_hull(gb), h4), ec = div_hull(bf_hull(gc), h4);
+      lam = std::max(lam, lambda2(std::max(ea.a, 0.0) / (sa * sa), mag(eb) / (sa * sb), std::max(ec.a, 0.0) / (sb * sb)));
     }
-    return s2;
-  };
-  double spread = std::sqrt(dev(ss, ps0) + dev(st, pt0));
-  double by_middle = std::pow(std::max(sigma0 - spread, 0.0), 2);
-  double lambda = std::max(by_entries, by_middle);
-  // gpu.rs:87-98 rounding allowance on the bounds
-  lambda = lambda - std::fabs(lambda) * 1e-6 - 1e-30;
-  if (cert) {
-    cert[0] = lambda > 0 ? lambda : 0.0;
-    put_iv(cert + 1, ss), put_iv(cert + 7, st), put_iv(cert + 13, sss), put_iv(cert + 19, sst), put_iv(cert + 25, stt);
   }
-  if (!(lambda > 0)) return 0.0;
-  auto mag2 = [](const Iv* a) { return mag(a[0]) * mag(a[0]) + mag(a[1]) * mag(a[1]) + mag(a[2]) * mag(a[2]); };
-  double hh = std::sqrt(mag2(sss) + 2 * mag2(sst) + mag2(stt)) * (1 + 1e-6);
-  return hh > 0 ? 0.99 * lambda / hh : INF;  // RHO_MARGIN, surface3.rs:807
+  out.lambda = lam;
+  return true;
 }
 
-// surface3.rs:931-970 a cubic piece's radius |C'|^2_min / |C''|_max; `cert` (when not
-// NULL) receives the bounds (PIECE_CERT words)
+// surface3.rs:819-830 the radius of convexity, 0.99 lambda_min(G) / sqrt(|S_ss|^2 + 2 |S_st|^2
+// + |S_tt|^2) with both bounded over the patch (patch_bounds; the tighter bound on G is
+// sought when the first gives less than `enough`); `cert` (when not NULL) receives lambda
+// and the derivatives' enclosures (PATCH_CERT words)
+double patch_radius(const Patch4& pw, double* cert = nullptr, double enough = INF) {
+  if (cert) open_cert(cert, PATCH_CERT);
+  PatchBounds pb;
+  if (!patch_bounds(pw, pb, enough)) return 0.0;
+  if (cert) {
+    cert[0] = pb.lambda > 0 ? pb.lambda : 0.0;
+    for (int q = 0; q < 5; ++q) put_iv(cert + 1 + 6 * q, pb.d[q]);
+  }
+  if (!(pb.lambda > 0)) return 0.0;
+  const double hh = frob(pb.d);
+  return hh > 0 ? 0.99 * pb.lambda / hh : INF;  // RHO_MARGIN, surface3.rs:807
+}
+
+// A piece's scaled forms: component k of P - c w (k < 3), or w (k = 3), one parameter.
+BForm piece_form(const H* pw, V c, int k) {
+  BForm R;
+  R.m = 3, R.n = 0;
+  const double* b3 = binom_row(3);
+  const double ck = k == 0 ? c.x : k == 1 ? c.y : c.z;
+  double big = 0;
+  for (int i = 0; i < 4; ++i) {
+    const double cw = k == 3 ? 0.0 : ck * pw[i][3], v = k == 3 ? pw[i][3] : pw[i][k] - cw;
+    R.c[i] = v * b3[i];
+    big = std::max(big, (std::fabs(k == 3 ? pw[i][3] : pw[i][k]) + std::fabs(cw)) * b3[i]);
+  }
+  R.E = (k == 3 ? 1.0 : 3.0) * U53 * big;
+  return R;
+}
+
+// surface3.rs:931-970 a cubic piece's radius |C'|^2_min / |C''|_max: C' = N / w^2 with
+// N = P' w - P w' (degree 5), C'' = (N' w - 2 N w') / w^3 (degree 7), |C'|^2 = N.N / w^4
+// (degree 10), each enclosed as the patch's are; `cert` (when not NULL) receives |C'|^2
+// from below and the enclosures of C' and C'' (PIECE_CERT words)
 double piece_radius(const H* pw, double* cert = nullptr) {
   if (cert) open_cert(cert, PIECE_CERT);
-  const int n = 3;
   V c{0, 0, 0};
   for (int i = 0; i < 4; ++i) c = c + point(pw[i]);
   c = c * 0.25;
-  auto q = [&](int i, int k) {
-    double cc[3] = {c.x, c.y, c.z};
-    return k < 3 ? pw[i][k] - cc[k] * pw[i][3] : pw[i][3];
-  };
-  Iv a[4], a1[4], a2[4];
-  for (int k = 0; k < 4; ++k) {
-    a[k] = iv_of({q(0, k), q(1, k), q(2, k), q(3, k)});
-    a1[k] = iv_of({n * (q(1, k) - q(0, k)), n * (q(2, k) - q(1, k)), n * (q(3, k) - q(2, k))});
-    a2[k] = iv_of({n * (n - 1.0) * (q(2, k) - 2 * q(1, k) + q(0, k)), n * (n - 1.0) * (q(3, k) - 2 * q(2, k) + q(1, k))});
-  }
-  Iv w = a[3];
-  if (!(w.a > 0)) return 0.0;
-  Iv s[3], d1[3], d2[3];
+  const BForm w = piece_form(pw, c, 3);
+  if (!(bf_hull(w).a > 0)) return 0.0;
+  const BForm w1 = bf_d(w, true), w2 = bf_mul(w, w), w3 = bf_mul(w2, w), w4 = bf_mul(w2, w2);
+  const Iv h2 = bf_hull(w2), h3 = bf_hull(w3), h4 = bf_hull(w4);
+  if (!(h2.a > 0 && h3.a > 0 && h4.a > 0)) return 0.0;
+  Iv d1[3], d2[3];
+  BForm len2;
   for (int k = 0; k < 3; ++k) {
-    Iv r{INF, -INF};
-    for (int i = 0; i < 4; ++i) {
-      V pp = point(pw[i]) - c;
-      double x = k == 0 ? pp.x : k == 1 ? pp.y : pp.z;
-      r.a = std::min(r.a, x), r.b = std::max(r.b, x);
-    }
-    s[k] = r;
-    d1[k] = divpos(sub(a1[k], mul(s[k], a1[3])), w);
+    const BForm q = piece_form(pw, c, k);
+    const BForm N = bf_sub(bf_mul(bf_d(q, true), w), bf_mul(q, w1));
+    d1[k] = div_hull(bf_hull(N), h2);
+    d2[k] = div_hull(bf_hull(bf_sub(bf_mul(bf_d(N, true), w), bf_mul(N, w1), 2.0)), h3);
+    const BForm nn = bf_mul(N, N);
+    len2 = k == 0 ? nn : bf_add(len2, nn);
   }
-  for (int k = 0; k < 3; ++k) d2[k] = divpos(sub(sub(a2[k], scale(mul(d1[k], a1[3]), 2.0)), mul(s[k], a2[3])), w);
-  // the middle's derivative (de Casteljau at 1/2)
-  H b[4];
-  for (int i = 0; i < 4; ++i) b[i] = pw[i];
-  H d1h;
-  for (int lev = 1; lev <= 3; ++lev) {
-    if (lev == 3)
-      for (int k = 0; k < 4; ++k) d1h[k] = 3 * (b[1][k] - b[0][k]);
-    for (int i = 0; i <= 3 - lev; ++i)
-      for (int k = 0; k < 4; ++k) b[i][k] = 0.5 * (b[i][k] + b[i + 1][k]);
-  }
-  V p = point(b[0]);
-  V e1 = (mk(d1h[0], d1h[1], d1h[2]) - p * d1h[3]) * (1 / b[0][3]);
-  double ex[3] = {e1.x, e1.y, e1.z}, spread = 0;
-  for (int k = 0; k < 3; ++k) {
-    double m = std::max(std::fabs(d1[k].a - ex[k]), std::fabs(d1[k].b - ex[k]));
-    spread += m * m;
-  }
-  spread = std::sqrt(spread);
-  double l = std::max(mig(d1[0]) * mig(d1[0]) + mig(d1[1]) * mig(d1[1]) + mig(d1[2]) * mig(d1[2]), std::pow(std::max(len(e1) - spread, 0.0), 2));
-  l = l - std::fabs(l) * 1e-6 - 1e-30;
-  if (cert) cert[0] = l > 0 ? l : 0.0, put_iv(cert + 1, d1), put_iv(cert + 7, d2);
+  const double l = std::max(div_hull(bf_hull(len2), h4).a, 0.0);
+  if (cert) cert[0] = l, put_iv(cert + 1, d1), put_iv(cert + 7, d2);
   if (!(l > 0)) return 0.0;
-  double h = std::sqrt(mag(d2[0]) * mag(d2[0]) + mag(d2[1]) * mag(d2[1]) + mag(d2[2]) * mag(d2[2])) * (1 + 1e-6);
+  const double h = std::sqrt(mag(d2[0]) * mag(d2[0]) + mag(d2[1]) * mag(d2[1]) + mag(d2[2]) * mag(d2[2])) * (1 + 1e-12);
   return h > 0 ? 0.99 * l / h : INF;
 }
 
@@ -628,6 +760,26 @@ std::vector<float> to_f32(const std::vector<double>& f) {
   for (size_t i = 0; i < f.size(); ++i) r[i] = float(f[i]);
   return r;
 }
+
+namespace {
+// f(0) .. f(n - 1) on up to 16 host threads (inline for small n); f must only write
+// what its index owns.
+template <class F> void parallel_for(size_t n, F&& f) {
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 8 || hw == 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+  };
+  std::vector<std::thread> ts;
+  for (unsigned t = 1; t < hw; ++t) ts.emplace_back(work);
+  work();
+  for (auto& t : ts) t.join();
+}
+}  // namespace
 
 bool pack_part(const icam_part_desc* part, double need, Packed& out) {
   Packed& P = out;
@@ -672,7 +824,8 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
       todo.pop_back();
       Box b;
       for (int i = 0; i < 4; ++i) b.add(point(pp.p[i]));
-      double rho = piece_radius(pp.p);
+      double cert[PIECE_CERT];
+      const double rho = piece_radius(pp.p, cert);
       ++P.nodes;
       if (split_rule(rho, b.diag(), need, pp.depth)) {
         Part_ l, r;
@@ -685,8 +838,6 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
       }
       uint32_t id = uint32_t(cleaf_f.size() / CLEAF_F);
       cleaf_f.insert(cleaf_f.end(), {rho, pp.t0, pp.t1, 0.0});
-      double cert[PIECE_CERT];
-      piece_radius(pp.p, cert);
       cleaf_f.insert(cleaf_f.end(), cert, cert + PIECE_CERT);
       cleaf_u.insert(cleaf_u.end(), {cp, e});
       pbox.push_back(b);
@@ -937,94 +1088,120 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
           creases.push_back(c);
         }
       }
+      // each span's tree of sub-patches, breadth first (gpu.rs:292-350), built on
+      // several threads, then appended in span order
+      struct SpanLeaf {
+        double f[LEAF_F];
+        uint32_t u[LEAF_U];
+        Box b;
+      };
+      struct SpanTree {
+        std::vector<SpanLeaf> leaves;
+        size_t nodes = 0;
+      };
+      std::vector<SpanTree> trees(size_t(nsu) * nsv);
+      auto span_tree = [&](size_t idx) {
+        const int a = int(idx / size_t(nsv)), b = int(idx % size_t(nsv));
+        const Patch4& pa = fp[idx];
+        const uint32_t sid = span0 + uint32_t(idx);
+        SpanTree& out = trees[idx];
+        struct Sub {
+          Patch4 p;
+          double s0, s1, t0, t1;
+          int depth;
+          int inside;
+        };
+        std::vector<Sub> queue;
+        double klo[2] = {ur[a].first, vr[b].first}, khi[2] = {ur[a].second, vr[b].second};
+        queue.push_back({pa, 0, 1, 0, 1, 0, whole ? 1 : tb.box_class(klo, khi)});
+        for (size_t qi = 0; qi < queue.size(); ++qi) {
+          Sub sb = queue[qi];
+          if (sb.inside == 0) continue;  // wholly outside the face's loops
+          Box bb;
+          for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) bb.add(point(sb.p.p[i][j]));
+          double cert[PATCH_CERT + 1] = {0};
+          const double rho = patch_radius(sb.p, cert, 2.0 * bb.diag() + need);
+          ++out.nodes;
+          if (split_rule(rho, bb.diag(), need, sb.depth)) {
+            Patch4 q[4];
+            quarter(sb.p, q);
+            double sm = 0.5 * (sb.s0 + sb.s1), tm = 0.5 * (sb.t0 + sb.t1);
+            for (int k = 0; k < 4; ++k) {
+              Sub c;
+              c.p = q[k];
+              c.s0 = (k & 1) ? sm : sb.s0, c.s1 = (k & 1) ? sb.s1 : sm;
+              c.t0 = (k & 2) ? tm : sb.t0, c.t1 = (k & 2) ? sb.t1 : tm;
+              c.depth = sb.depth + 1;
+              if (sb.inside == 1) {
+                c.inside = 1;
+              } else {
+                double du_ = ur[a].second - ur[a].first, dv_ = vr[b].second - vr[b].first;
+                double clo[2] = {ur[a].first + c.s0 * du_, vr[b].first + c.t0 * dv_}, chi[2] = {ur[a].first + c.s1 * du_, vr[b].first + c.t1 * dv_};
+                c.inside = tb.box_class(clo, chi);
+              }
+              queue.push_back(c);
+            }
+            continue;
+          }
+          // the slab between two planes normal to the sub-patch's middle normal that
+          // holds its control points (the patch lies in their hull)
+          V p0, ps0, pt0;
+          eval_patch(sb.p, 0.5, 0.5, p0, ps0, pt0);
+          V nn = cross(ps0, pt0);
+          double nl = len(nn);
+          double slo = -INF, shi = INF;
+          if (nl > 0) {
+            nn = nn * (1 / nl);
+            slo = INF, shi = -INF;
+            for (int i = 0; i < 4; ++i)
+              for (int j = 0; j < 4; ++j) {
+                double x = dot(point(sb.p.p[i][j]), nn);
+                slo = std::min(slo, x), shi = std::max(shi, x);
+              }
+            double pad = 1e-9 * (std::fabs(slo) + std::fabs(shi) + bb.diag());
+            slo -= pad, shi += pad;
+          } else {
+            nn = mk(0, 0, 1);
+          }
+          SpanLeaf lf;
+          const double head[12] = {rho, sb.s0, sb.s1, sb.t0, sb.t1, nn.x, nn.y, nn.z, slo, shi, 0, 0};
+          std::copy(head, head + 12, lf.f);
+          std::copy(cert, cert + PATCH_CERT + 1, lf.f + 12);
+          // a pole whose side the sub-patch touches, and that side in the span's
+          // parameters (1 s = 0, 2 s = 1, 3 t = 0, 4 t = 1)
+          double pole[4] = {0, 0, 0, 0};
+          for (const FacePole& fpo : fpoles) {
+            int side = 0;
+            if (fpo.k == 0 && fpo.value == ur[a].first && sb.s0 == 0.0) side = 1;
+            if (fpo.k == 0 && fpo.value == ur[a].second && sb.s1 == 1.0) side = 2;
+            if (fpo.k == 1 && fpo.value == vr[b].first && sb.t0 == 0.0) side = 3;
+            if (fpo.k == 1 && fpo.value == vr[b].second && sb.t1 == 1.0) side = 4;
+            if (side) pole[0] = fpo.x.x, pole[1] = fpo.x.y, pole[2] = fpo.x.z, pole[3] = side;
+          }
+          std::copy(pole, pole + 4, lf.f + 12 + PATCH_CERT + 1);
+          lf.u[0] = sid, lf.u[1] = fi, lf.u[2] = sb.inside == 1 ? 1u : 0u, lf.u[3] = 0;
+          lf.b = bb;
+          out.leaves.push_back(lf);
+        }
+      };
+      parallel_for(trees.size(), span_tree);
       for (int a = 0; a < nsu; ++a)
         for (int b = 0; b < nsv; ++b) {
           const Patch4& pa = fp[size_t(a) * nsv + b];
-          uint32_t sid = uint32_t(span_f.size() / SPAN_F);
           for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j) span_f.insert(span_f.end(), pa.p[i][j].begin(), pa.p[i][j].end());
           span_f.insert(span_f.end(), {ur[a].first, ur[a].second, vr[b].first, vr[b].second, 0, 0, 0, 0});
           span_u.insert(span_u.end(), {fi, whole ? 0u : 1u, cell0, 0});
-          // the span's tree of sub-patches, breadth first (gpu.rs:292-350)
-          struct Sub {
-            Patch4 p;
-            double s0, s1, t0, t1;
-            int depth;
-            int inside;
-          };
-          std::vector<Sub> queue;
-          double klo[2] = {ur[a].first, vr[b].first}, khi[2] = {ur[a].second, vr[b].second};
-          queue.push_back({pa, 0, 1, 0, 1, 0, whole ? 1 : tb.box_class(klo, khi)});
-          for (size_t qi = 0; qi < queue.size(); ++qi) {
-            Sub sb = queue[qi];
-            if (sb.inside == 0) continue;  // wholly outside the face's loops
-            Box bb;
-            for (int i = 0; i < 4; ++i)
-              for (int j = 0; j < 4; ++j) bb.add(point(sb.p.p[i][j]));
-            double rho = patch_radius(sb.p);
-            ++P.nodes;
-            if (split_rule(rho, bb.diag(), need, sb.depth)) {
-              Patch4 q[4];
-              quarter(sb.p, q);
-              double sm = 0.5 * (sb.s0 + sb.s1), tm = 0.5 * (sb.t0 + sb.t1);
-              for (int k = 0; k < 4; ++k) {
-                Sub c;
-                c.p = q[k];
-                c.s0 = (k & 1) ? sm : sb.s0, c.s1 = (k & 1) ? sb.s1 : sm;
-                c.t0 = (k & 2) ? tm : sb.t0, c.t1 = (k & 2) ? sb.t1 : tm;
-                c.depth = sb.depth + 1;
-                if (sb.inside == 1) {
-                  c.inside = 1;
-                } else {
-                  double du_ = ur[a].second - ur[a].first, dv_ = vr[b].second - vr[b].first;
-                  double clo[2] = {ur[a].first + c.s0 * du_, vr[b].first + c.t0 * dv_}, chi[2] = {ur[a].first + c.s1 * du_, vr[b].first + c.t1 * dv_};
-                  c.inside = tb.box_class(clo, chi);
-                }
-                queue.push_back(c);
-              }
-              continue;
-            }
-            // the slab between two planes normal to the sub-patch's middle normal that
-            // holds its control points (the patch lies in their hull)
-            V p0, ps0, pt0;
-            eval_patch(sb.p, 0.5, 0.5, p0, ps0, pt0);
-            V nn = cross(ps0, pt0);
-            double nl = len(nn);
-            double slo = -INF, shi = INF;
-            if (nl > 0) {
-              nn = nn * (1 / nl);
-              slo = INF, shi = -INF;
-              for (int i = 0; i < 4; ++i)
-                for (int j = 0; j < 4; ++j) {
-                  double x = dot(point(sb.p.p[i][j]), nn);
-                  slo = std::min(slo, x), shi = std::max(shi, x);
-                }
-              double pad = 1e-9 * (std::fabs(slo) + std::fabs(shi) + bb.diag());
-              slo -= pad, shi += pad;
-            } else {
-              nn = mk(0, 0, 1);
-            }
-            uint32_t id = uint32_t(leaf_f.size() / LEAF_F);
-            leaf_f.insert(leaf_f.end(), {rho, sb.s0, sb.s1, sb.t0, sb.t1, nn.x, nn.y, nn.z, slo, shi, 0, 0});
-            double cert[PATCH_CERT + 1] = {0};
-            patch_radius(sb.p, cert);
-            leaf_f.insert(leaf_f.end(), cert, cert + PATCH_CERT + 1);
-            // a pole whose side the sub-patch touches, and that side in the span's
-            // parameters (1 s = 0, 2 s = 1, 3 t = 0, 4 t = 1)
-            double pole[4] = {0, 0, 0, 0};
-            for (const FacePole& fpo : fpoles) {
-              int side = 0;
-              if (fpo.k == 0 && fpo.value == ur[a].first && sb.s0 == 0.0) side = 1;
-              if (fpo.k == 0 && fpo.value == ur[a].second && sb.s1 == 1.0) side = 2;
-              if (fpo.k == 1 && fpo.value == vr[b].first && sb.t0 == 0.0) side = 3;
-              if (fpo.k == 1 && fpo.value == vr[b].second && sb.t1 == 1.0) side = 4;
-              if (side) pole[0] = fpo.x.x, pole[1] = fpo.x.y, pole[2] = fpo.x.z, pole[3] = side;
-            }
-            leaf_f.insert(leaf_f.end(), pole, pole + 4);
-            leaf_u.insert(leaf_u.end(), {sid, fi, sb.inside == 1 ? 1u : 0u, 0});
-            pbox.push_back(bb);
+          const SpanTree& tr = trees[size_t(a) * nsv + b];
+          P.nodes += tr.nodes;
+          for (const SpanLeaf& lf : tr.leaves) {
+            const uint32_t id = uint32_t(leaf_f.size() / LEAF_F);
+            leaf_f.insert(leaf_f.end(), lf.f, lf.f + LEAF_F);
+            leaf_u.insert(leaf_u.end(), lf.u, lf.u + LEAF_U);
+            pbox.push_back(lf.b);
             pcode.push_back((P_LEAF << 28) | id);
-            fb.add(bb);
+            fb.add(lf.b);
           }
         }
       ru[3] = span0, ru[4] = uint32_t(nsu), ru[5] = uint32_t(nsv);
