@@ -56,6 +56,7 @@ def gen():
         lib.icam_gen_free.argtypes = [ctypes.POINTER(PartDesc)]
         lib.icam_gen_uniform.argtypes = [ctypes.c_uint64, ctypes.c_size_t, c_double_p, c_double_p, c_double_p]
         lib.icam_gen_near_trim.argtypes = [ctypes.POINTER(PartDesc), ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double, c_double_p]
+        lib.icam_gen_near_surface.argtypes = [ctypes.POINTER(PartDesc), ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double, c_double_p]
         lib.icam_gen_describe.restype = ctypes.c_size_t
         lib.icam_gen_describe.argtypes = [ctypes.POINTER(PartDesc), c_double_p, ctypes.c_size_t]
         _gen = lib

@@ -30,7 +30,6 @@
 //           0.5 along u (k even) or v (k odd): the top folds along that knot line, a
 //           ridge of height 12 (a valley for k = 2, 3) fading to zero at the boundary
 #include <algorithm>
-#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -104,6 +103,12 @@ using Piece = std::array<double, 8>;
 
 Piece line_piece(double u0, double v0, double u1, double v1) {
   return {u0, v0, u0 + (u1 - u0) / 3.0, v0 + (v1 - v0) / 3.0, u0 + 2.0 * (u1 - u0) / 3.0, v0 + 2.0 * (v1 - v0) / 3.0, u1, v1};
+}
+
+// The straight line from (u0, v0) to (u1, v1) as k cubic pieces end to end.
+void add_line(std::vector<Piece>& out, double u0, double v0, double u1, double v1, int k) {
+  for (int i = 0; i < k; ++i)
+    out.push_back(line_piece(u0 + (u1 - u0) * i / k, v0 + (v1 - v0) * i / k, u0 + (u1 - u0) * (i + 1) / k, v0 + (v1 - v0) * (i + 1) / k));
 }
 
 void add_polyline(std::vector<Piece>& out, const std::vector<std::array<double, 2>>& pts, bool closed) {
@@ -772,7 +777,9 @@ Prof arc(double rc, double zc, double rad, double psi0, double psi1) {
 //   arc about a point of the axis: a sphere;  arc about a point off it: a ring torus.
 // Analytic trim domains as in the part model: (quarter-arc parameter of the angle
 // about the axis, height | latitude arc parameter | tube arc parameter).
-void revolve(Builder& b, double cx, double cy, std::vector<Prof> prof, bool hole) {
+void revolve(Builder& b, double cx, double cy, std::vector<Prof> prof, bool hole, int split = 1) {
+  // a trim line u = 0 .. 4 at v, as `split` pieces
+  auto tline = [&](std::vector<Piece>& pcs, double v) { add_line(pcs, 0, v, 4, v, split); };
   V X{1, 0, 0}, Z{0, 0, 1};
   const size_t n = prof.size();
   std::vector<uint32_t> fid(n, 0xffffffffu);
@@ -792,15 +799,16 @@ void revolve(Builder& b, double cx, double cy, std::vector<Prof> prof, bool hole
         double v0 = arc_param_over(e.psi0, -0.5 * M_PI, M_PI, 2), v1 = arc_param_over(e.psi1, -0.5 * M_PI, M_PI, 2);
         double lo = std::min(v0, v1), hi = std::max(v0, v1);
         std::vector<Piece> pcs;
-        if (lo > 1e-12) pcs.push_back(line_piece(0, lo, 4, lo));
-        if (hi < 2 - 1e-12) pcs.push_back(line_piece(0, hi, 4, hi));
+        if (lo > 1e-12) tline(pcs, lo);
+        if (hi < 2 - 1e-12) tline(pcs, hi);
         fid[i] = b.analytic(ICAM_SPHERE, V{cx, cy, e.zc}, Z, X, e.rad, 0, sense, 0, -1, 4, 3, 4, 0, pcs, 2.0, 0.5 * (lo + hi), true);
       } else if (std::fabs(e.rc) > e.rad) {
         // ring torus: tube arc parameter over the whole meridian circle, periodic
         double pm = 0.5 * (e.psi0 + e.psi1);
         double v0 = arc_param_over(e.psi0, 0, 2 * M_PI, 4), v1 = arc_param_over(e.psi1, 0, 2 * M_PI, 4);
         double vm = arc_param_over(pm, 0, 2 * M_PI, 4);
-        std::vector<Piece> pcs = {line_piece(0, v0, 4, v0), line_piece(0, v1, 4, v1)};
+        std::vector<Piece> pcs;
+        tline(pcs, v0), tline(pcs, v1);
         fid[i] = b.analytic(ICAM_TORUS, V{cx, cy, e.zc}, Z, X, e.rc, e.rad, sense, 0, 0, 4, 4, 4, 4, pcs, 2.0, vm, true);
       } else {
         // spindle torus (the arc crosses the axis: a lemon when its centre lies across
@@ -812,8 +820,8 @@ void revolve(Builder& b, double cx, double cy, std::vector<Prof> prof, bool hole
         double v0 = arc_param_over(e.psi0, -pp, 2 * pp, narc), v1 = arc_param_over(e.psi1, -pp, 2 * pp, narc);
         double vm = arc_param_over(0.5 * (e.psi0 + e.psi1), -pp, 2 * pp, narc);
         std::vector<Piece> pcs;
-        if (e.r0 > 0) pcs.push_back(line_piece(0, v0, 4, v0));
-        if (e.r1 > 0) pcs.push_back(line_piece(0, v1, 4, v1));
+        if (e.r0 > 0) tline(pcs, v0);
+        if (e.r1 > 0) tline(pcs, v1);
         fid[i] = b.analytic(ICAM_TORUS, V{cx, cy, e.zc}, Z, X, e.rc, e.rad, sense, 0, 0, 4, double(narc), 4, 0, pcs, 2.0, vm, true);
         // the tip on the axis: a vertex loop of the face, as STEP writes it, and a vertex
         // whose pseudo-normal is the face's normal there weighted by the full turn
@@ -842,7 +850,8 @@ void revolve(Builder& b, double cx, double cy, std::vector<Prof> prof, bool hole
       fid[i] = b.plane(V{cx, cy, e.z0}, zn, X, 1.0, -rb - 1, -rb - 1, rb + 1, rb + 1, pcs, 0.5 * (ra + rb), 0.0123, true);
     } else if (dr == 0) {
       double lo = std::min(e.z0, e.z1), hi = std::max(e.z0, e.z1);
-      std::vector<Piece> pcs = {line_piece(0, lo, 4, lo), line_piece(0, hi, 4, hi)};
+      std::vector<Piece> pcs;
+      tline(pcs, lo), tline(pcs, hi);
       double sense = (on_r > 0 ? 1.0 : -1.0) * flip;
       fid[i] = b.analytic(ICAM_CYLINDER, o, Z, X, e.r0, 0, sense, 0, lo - 1, 4, hi + 1, 4, 0, pcs, 2.0, 0.5 * (lo + hi), true);
     } else {
@@ -855,8 +864,8 @@ void revolve(Builder& b, double cx, double cy, std::vector<Prof> prof, bool hole
       const double own_r = std::cos(ang), own_z = -zdir * std::sin(ang);
       double sense = ((own_r * on_r + own_z * on_z) > 0 ? 1.0 : -1.0) * flip;
       std::vector<Piece> pcs;
-      if (ra > 0) pcs.push_back(line_piece(0, 0, 4, 0));
-      pcs.push_back(line_piece(0, len_h, 4, len_h));
+      if (ra > 0) tline(pcs, 0);
+      tline(pcs, len_h);
       fid[i] = b.analytic(ICAM_CONE, V{cx, cy, za}, Z * zdir, X, ra, ang, sense, 0, -1, 4, len_h + 1, 4, 0, pcs, 2.0, 0.5 * len_h, true);
       if (ra == 0) {
         // the apex: a vertex loop of the face, as STEP writes it, and a vertex whose
@@ -1441,6 +1450,367 @@ Owned* make_c4(uint64_t seed, int index) {
   return make_plate(W, D, T, plate_holes(W, D, T, k, rng));
 }
 
+// ------------------------------------------------------------- C5: the large mixed solid
+
+// A clamped uniform cubic B-spline of n control points (points of dimension N, flat;
+// n - 3 spans): its Bezier pieces.
+std::vector<std::vector<double>> uniform_spans(const std::vector<double>& ctrl, int N) {
+  const int n = int(ctrl.size()) / N, spans = n - 3;
+  std::vector<double> kn(4, 0.0);
+  for (int k = 1; k < spans; ++k) kn.push_back(double(k) / spans);
+  kn.insert(kn.end(), 4, 1.0);
+  return decompose(3, kn, ctrl, N);
+}
+
+// A rectangle's loop, each side as k pieces.
+std::vector<Piece> rect_loop(double u0, double v0, double u1, double v1, int k) {
+  std::vector<Piece> p;
+  add_line(p, u0, v0, u1, v0, k), add_line(p, u1, v0, u1, v1, k), add_line(p, u1, v1, u0, v1, k), add_line(p, u0, v1, u0, v0, k);
+  return p;
+}
+
+// A plate 400 x 300 x 40 (centred) with 400 pockets and 264 through holes on a jittered
+// 29 x 23 grid (three slots empty), 1,998 faces of which 800 are rational bicubic
+// B-splines with 32 x 32 nets and 1,198 analytic, every face bounded by loops of 12 to
+// 64 cubic spans.
+//  - A pocket (3 to 9 mm deep) has an outline of two clamped cubic B-spline segments of
+//    32 control points each (29 spans, 58 in the loop) from a radially perturbed rounded
+//    rectangle, meeting with equal tangents; its floor is a plane trimmed by the outline;
+//    each segment is a wall, a rational B-spline with a 32 x 32 net: the segment's points
+//    along u, heights at the Greville abscissae along v, weights in [0.5, 2] from a
+//    smooth profile along v only (the end rows 1), so the wall is the vertical ruled
+//    surface over the outline, and its loop is its domain's border, three pieces a side.
+//  - A hole of radius r has a 45-degree countersink of depth c at the bottom (a cone), its
+//    wall (a cylinder) and a torus fillet of radius f at the top; each face's loops are
+//    its two bounding circles in its parameters, eight pieces each.
+//  - The plate's top and bottom are trimmed by their rectangle (four pieces a side), the
+//    pockets' outlines and the holes' circles (64 pieces each); its sides by their
+//    rectangles.
+Owned* make_c5(uint64_t seed) {
+  Owned* o = new Owned();
+  Builder b{o};
+  Rng rng{seed ^ 0x5C5C5C5C5C5C5C5Cull};
+  for (int k = 0; k < 4; ++k) rng.next();
+  V X{1, 0, 0}, Y{0, 1, 0}, Z{0, 0, 1};
+  const double ax = 200, ay = 150, az = 20;
+  const int NX = 29, NY = 23, NPOCK = 400, NHOLE = 264, NCTL = 32, F = 2;
+  std::vector<int> kind(NX * NY);
+  for (int i = 0; i < NX * NY; ++i) kind[i] = i < NPOCK ? 0 : i < NPOCK + NHOLE ? 1 : 2;
+  for (int i = NX * NY - 1; i > 0; --i) std::swap(kind[i], kind[int(rng.next() * (i + 1)) % (i + 1)]);
+  struct Pocket {
+    double z0, z1;
+    std::vector<std::vector<double>> seg;  // F segments: NCTL control points (x, y)
+    std::vector<Piece> pieces;             // the outline as monotone trim pieces
+    double lo[2], hi[2];
+  };
+  struct HoleC5 {
+    double x, y, r, f, c;
+  };
+  std::vector<Pocket> pockets;
+  std::vector<HoleC5> holes;
+  const double cw = 2 * ax / NX, cd = 2 * ay / NY, cell = std::min(cw, cd);
+  for (int s = 0; s < NX * NY; ++s) {
+    const double cx = -ax + cw * (s % NX + 0.5) + (rng.next() - 0.5) * 0.1 * cw, cy = -ay + cd * (s / NX + 0.5) + (rng.next() - 0.5) * 0.1 * cd;
+    if (kind[s] == 1) {
+      HoleC5 h;
+      h.x = cx, h.y = cy, h.r = cell * (0.12 + 0.1 * rng.next()), h.f = 0.5 + 0.5 * rng.next(), h.c = 0.6 + 0.6 * rng.next();
+      holes.push_back(h);
+      continue;
+    }
+    if (kind[s] != 0) continue;
+    Pocket L;
+    const double ha = (0.26 + 0.06 * rng.next()) * cw, hb = (0.26 + 0.06 * rng.next()) * cd, rc = 0.8 + 0.8 * rng.next();
+    L.z1 = az, L.z0 = az - (3.0 + 6.0 * rng.next());
+    const int M = (NCTL - 1) * F;
+    const double straight_x = 2 * (ha - rc), straight_y = 2 * (hb - rc), quarter = 0.5 * M_PI * rc;
+    const double per = 2 * straight_x + 2 * straight_y + 4 * quarter;
+    auto rr = [&](double t) {
+      double l = t * per;
+      const double segs[9] = {0.5 * straight_y, quarter, straight_x, quarter, straight_y, quarter, straight_x, quarter, 0.5 * straight_y};
+      int k = 0;
+      while (k < 8 && l > segs[k]) l -= segs[k], ++k;
+      double x = 0, y = 0, nx = 1, ny = 0;
+      const double ex = ha - rc, ey = hb - rc;
+      switch (k) {
+        case 0: x = ha, y = l, nx = 1, ny = 0; break;
+        case 1: { double a = l / rc; x = ex + rc * std::cos(a), y = ey + rc * std::sin(a), nx = std::cos(a), ny = std::sin(a); break; }
+        case 2: x = ex - l, y = hb, nx = 0, ny = 1; break;
+        case 3: { double a = 0.5 * M_PI + l / rc; x = -ex + rc * std::cos(a), y = ey + rc * std::sin(a), nx = std::cos(a), ny = std::sin(a); break; }
+        case 4: x = -ha, y = ey - l, nx = -1, ny = 0; break;
+        case 5: { double a = M_PI + l / rc; x = -ex + rc * std::cos(a), y = -ey + rc * std::sin(a), nx = std::cos(a), ny = std::sin(a); break; }
+        case 6: x = -ex + l, y = -hb, nx = 0, ny = -1; break;
+        case 7: { double a = 1.5 * M_PI + l / rc; x = ex + rc * std::cos(a), y = -ey + rc * std::sin(a), nx = std::cos(a), ny = std::sin(a); break; }
+        default: x = ha, y = -ey + l, nx = 1, ny = 0; break;
+      }
+      return std::array<double, 4>{x, y, nx, ny};
+    };
+    const double a1 = 0.15 + 0.15 * rng.next(), f1 = double(2 + int(rng.next() * 3.0)), p1 = 2 * M_PI * rng.next();
+    const double a2 = 0.1 * rng.next(), f2 = double(5 + int(rng.next() * 4.0)), p2 = 2 * M_PI * rng.next();
+    std::vector<std::array<double, 2>> pts(M);
+    for (int i = 0; i < M; ++i) {
+      const double t = double(i) / M;
+      auto q = rr(t);
+      const double d = a1 * std::sin(2 * M_PI * f1 * t + p1) + a2 * std::sin(2 * M_PI * f2 * t + p2);
+      pts[i] = {cx + q[0] + d * q[2], cy + q[1] + d * q[3]};
+    }
+    auto joint = [&](int k) {
+      const auto& a = pts[((NCTL - 1) * k + M - 1) % M];
+      const auto& c = pts[((NCTL - 1) * k + 1) % M];
+      return std::array<double, 2>{0.5 * (a[0] + c[0]), 0.5 * (a[1] + c[1])};
+    };
+    L.lo[0] = L.lo[1] = 1e300, L.hi[0] = L.hi[1] = -1e300;
+    for (int k = 0; k < F; ++k) {
+      std::vector<double> c;
+      auto j0 = joint(k), j1 = joint((k + 1) % F);
+      c.insert(c.end(), {j0[0], j0[1]});
+      for (int m = 1; m <= NCTL - 2; ++m) c.insert(c.end(), {pts[((NCTL - 1) * k + m) % M][0], pts[((NCTL - 1) * k + m) % M][1]});
+      c.insert(c.end(), {j1[0], j1[1]});
+      for (int m = 0; m < NCTL; ++m)
+        for (int a = 0; a < 2; ++a) L.lo[a] = std::min(L.lo[a], c[2 * m + a]), L.hi[a] = std::max(L.hi[a], c[2 * m + a]);
+      for (auto& pc : uniform_spans(c, 2)) add_monotone(L.pieces, to_cubic2(pc, 3));
+      L.seg.push_back(c);
+    }
+    pockets.push_back(L);
+  }
+  // the plate
+  std::vector<Piece> top = rect_loop(-ax, -ay, ax, ay, 4), bot = rect_loop(-ax, -ay, ax, ay, 4);
+  for (auto& L : pockets) top.insert(top.end(), L.pieces.begin(), L.pieces.end());
+  for (auto& h : holes) add_circle(top, h.x, h.y, h.r + h.f), add_circle(bot, h.x, -h.y, h.r + h.c);
+  uint32_t f_top = b.plane({0, 0, az}, Z, X, 1.0, -ax - 1, -ay - 1, ax + 1, ay + 1, top, -ax - 0.5, 0.0123, false);
+  uint32_t f_bot = b.plane({0, 0, -az}, Z * -1.0, X, 1.0, -ax - 1, -ay - 1, ax + 1, ay + 1, bot, -ax - 0.5, 0.0123, false);
+  uint32_t f_xp = b.plane({ax, 0, 0}, X, Y, 1.0, -ay - 1, -az - 1, ay + 1, az + 1, rect_loop(-ay, -az, ay, az, 4), -ay - 0.5, 0.0123, false);
+  uint32_t f_xm = b.plane({-ax, 0, 0}, X * -1.0, Y, 1.0, -ay - 1, -az - 1, ay + 1, az + 1, rect_loop(-ay, -az, ay, az, 4), -ay - 0.5, 0.0123, false);
+  uint32_t f_yp = b.plane({0, ay, 0}, Y, X, 1.0, -ax - 1, -az - 1, ax + 1, az + 1, rect_loop(-ax, -az, ax, az, 4), -ax - 0.5, 0.0123, false);
+  uint32_t f_ym = b.plane({0, -ay, 0}, Y * -1.0, X, 1.0, -ax - 1, -az - 1, ax + 1, az + 1, rect_loop(-ax, -az, ax, az, 4), -ax - 0.5, 0.0123, false);
+  V c[2][2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) c[i][j][k] = V{i ? ax : -ax, j ? ay : -ay, k ? az : -az};
+  for (int k = 0; k < 2; ++k) {
+    uint32_t f = k ? f_top : f_bot;
+    b.line(c[0][0][k], c[1][0][k], f, f_ym);
+    b.line(c[1][0][k], c[1][1][k], f, f_xp);
+    b.line(c[1][1][k], c[0][1][k], f, f_yp);
+    b.line(c[0][1][k], c[0][0][k], f, f_xm);
+  }
+  b.line(c[0][0][0], c[0][0][1], f_xm, f_ym);
+  b.line(c[1][0][0], c[1][0][1], f_xp, f_ym);
+  b.line(c[1][1][0], c[1][1][1], f_xp, f_yp);
+  b.line(c[0][1][0], c[0][1][1], f_xm, f_yp);
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) b.vert(c[i][j][k], V{i ? 1.0 : -1.0, j ? 1.0 : -1.0, k ? 1.0 : -1.0} * (M_PI / 2));
+  o->info = {5.0, 2 * ax, 2 * ay, 2 * az, double(pockets.size()), double(holes.size())};
+  // the walls' knots (29 spans) and their Greville abscissae along v
+  std::vector<double> kn(4, 0.0);
+  for (int k = 1; k < NCTL - 3; ++k) kn.push_back(double(k) / (NCTL - 3));
+  kn.insert(kn.end(), 4, 1.0);
+  std::vector<double> gv(NCTL);
+  for (int j = 0; j < NCTL; ++j) gv[j] = (kn[j + 1] + kn[j + 2] + kn[j + 3]) / 3.0;
+  const int nsp = NCTL - 3;
+  for (auto& L : pockets) {
+    o->info.insert(o->info.end(), {L.z0, L.z1, double(F)});
+    for (auto& sgm : L.seg) o->info.insert(o->info.end(), sgm.begin(), sgm.end());
+    uint32_t f_cap = b.plane({0, 0, L.z0}, Z, X, 1.0, L.lo[0] - 1, L.lo[1] - 1, L.hi[0] + 1, L.hi[1] + 1, L.pieces, L.lo[0] - 0.5,
+                             0.5 * (L.lo[1] + L.hi[1]) + 0.0123, false);
+    std::vector<uint32_t> walls;
+    for (int k = 0; k < F; ++k) {
+      const auto& sgm = L.seg[k];
+      // the weights along v: 2^sin(2 pi (f v + phase)), a smooth profile through [0.5, 2]
+      // (f from 0.5 to 1.5 turns over the wall), its end rows 1
+      std::vector<double> w(NCTL);
+      const double fq = 0.5 + rng.next(), ph = rng.next();
+      for (int j = 0; j < NCTL; ++j) w[j] = (j == 0 || j == NCTL - 1) ? 1.0 : std::exp2(std::sin(2 * M_PI * (fq * gv[j] + ph)));
+      std::vector<double> pw(size_t(NCTL) * NCTL * 4);
+      for (int i = 0; i < NCTL; ++i)
+        for (int j = 0; j < NCTL; ++j) {
+          double* q = &pw[size_t(i * NCTL + j) * 4];
+          const double z = L.z0 + (L.z1 - L.z0) * gv[j];
+          q[0] = sgm[2 * i] * w[j], q[1] = sgm[2 * i + 1] * w[j], q[2] = z * w[j], q[3] = w[j];
+        }
+      icam_face_desc f;
+      std::memset(&f, 0, sizeof(f));
+      f.kind = ICAM_NURBS;
+      f.sense = -1.0;  // Su x Sv points out of the outline: the pocket's wall faces in
+      f.du = f.dv = 3;
+      f.nu = f.nv = NCTL;
+      f.pw = o->keep(pw);
+      f.ku = o->keep(kn);
+      f.kv = o->keep(kn);
+      b.set_trim(f, -0.01, -0.01, 1.01, 1.01, 0, 0, rect_loop(0, 0, 1, 1, 3), 0.5, 0.5, true);
+      o->faces.push_back(f);
+      walls.push_back(uint32_t(o->faces.size() - 1));
+    }
+    for (int k = 0; k < F; ++k) {
+      const auto& sgm = L.seg[k];
+      for (int end = 0; end < 2; ++end) {
+        // end 0: the rim on the floor (v = 0), end 1: on the plate's top (v = 1)
+        const double z = end ? L.z1 : L.z0, v = end ? 1.0 : 0.0;
+        std::vector<double> c3;
+        for (int i = 0; i < NCTL; ++i) c3.insert(c3.end(), {sgm[2 * i], sgm[2 * i + 1], z, 1.0});
+        auto pcs = uniform_spans(c3, 4);
+        std::vector<double> uv;
+        for (int m = 0; m < nsp; ++m) uv.insert(uv.end(), {double(m) / nsp, v, double(m + 1) / nsp, v, NaN, NaN, NaN, NaN});
+        b.spline(3, pcs, walls[k], end ? f_top : f_cap, uv);
+      }
+      const int k1 = (k + 1) % F;
+      const int e = 2 * (NCTL - 1);
+      V pj{sgm[e], sgm[e + 1], 0};
+      const double tx = sgm[e] - sgm[e - 2], ty = sgm[e + 1] - sgm[e - 1], tl = std::hypot(tx, ty);
+      V nwall = V{ty / tl, -tx / tl, 0} * -1.0;
+      V a{pj.x, pj.y, L.z0}, top_v{pj.x, pj.y, L.z1};
+      b.line(a, top_v, walls[k], walls[k1], o->keep({1, 0, 1, 1, 0, 0, 0, 1}));
+      b.vert(a, (Z + nwall) * M_PI);
+      b.vert(top_v, (Z + nwall) * M_PI);
+    }
+  }
+  for (auto& h : holes) {
+    o->info.insert(o->info.end(), {h.x, h.y, h.r, h.f, h.c});
+    std::vector<Prof> p;
+    Prof cap_b = seg(0, -az, h.r + h.c, -az);
+    cap_b.face = false, cap_b.ext = f_bot, cap_b.ext_nz = -1;
+    p.push_back(cap_b);
+    p.push_back(seg(h.r + h.c, -az, h.r, -az + h.c));
+    p.push_back(seg(h.r, -az + h.c, h.r, az - h.f));
+    p.push_back(arc(h.r + h.f, az - h.f, h.f, M_PI, 0.5 * M_PI));
+    Prof cap_t = seg(h.r + h.f, az, 0, az);
+    cap_t.face = false, cap_t.ext = f_top, cap_t.ext_nz = 1;
+    p.push_back(cap_t);
+    p.push_back(seg(0, az, 0, -az));
+    revolve(b, h.x, h.y, p, true, 8);
+  }
+  o->finish({-ax, -ay, -az}, {ax, ay, az}, 1e-9);
+  return o;
+}
+
+// ------------------------------------------------------------- points near a surface
+
+// The part's surface point at (u, v) of face f (the part model's parameterizations).
+// Quarter-arc parameters (cylinder, cone, sphere and torus angles) are turned back into
+// angles by the inverse of arc_param_over.
+double arc_angle(double u, double a0, double span, int n) {
+  const double q = span / n;
+  const double k = std::min(std::max(std::floor(u), 0.0), double(n - 1)), t = u - k;
+  return a0 + q * k + 0.5 * q + 2.0 * std::atan((2.0 * t - 1.0) * std::tan(0.25 * q));
+}
+V nurbs_point(const icam_face_desc& f, double u, double v) {
+  // de Boor in both directions on the homogeneous net
+  auto span_of = [](const double* kn, int n, int p, double x) {
+    int s = p;
+    while (s < n - 1 && kn[s + 1] <= x) ++s;
+    return s;
+  };
+  const int p = int(f.du), q = int(f.dv), nu = int(f.nu), nv = int(f.nv);
+  u = std::min(std::max(u, f.ku[p]), f.ku[nu]), v = std::min(std::max(v, f.kv[q]), f.kv[nv]);
+  const int su = span_of(f.ku, nu, p, u), sv = span_of(f.kv, nv, q, v);
+  double row[8][4];
+  for (int a = 0; a <= p; ++a) {
+    const int i = su - p + a;
+    double d[8][4];
+    for (int b2 = 0; b2 <= q; ++b2)
+      for (int k = 0; k < 4; ++k) d[b2][k] = f.pw[size_t((i * nv + sv - q + b2) * 4 + k)];
+    for (int r = 1; r <= q; ++r)
+      for (int j = q; j >= r; --j) {
+        const int idx = sv - q + j;
+        const double den = f.kv[idx + q - r + 1] - f.kv[idx], al = den > 0 ? (v - f.kv[idx]) / den : 0.0;
+        for (int k = 0; k < 4; ++k) d[j][k] = (1 - al) * d[j - 1][k] + al * d[j][k];
+      }
+    for (int k = 0; k < 4; ++k) row[a][k] = d[q][k];
+  }
+  for (int r = 1; r <= p; ++r)
+    for (int j = p; j >= r; --j) {
+      const int idx = su - p + j;
+      const double den = f.ku[idx + p - r + 1] - f.ku[idx], al = den > 0 ? (u - f.ku[idx]) / den : 0.0;
+      for (int k = 0; k < 4; ++k) row[j][k] = (1 - al) * row[j - 1][k] + al * row[j][k];
+    }
+  return V{row[p][0] / row[p][3], row[p][1] / row[p][3], row[p][2] / row[p][3]};
+}
+V face_point(const icam_face_desc& f, double u, double v) {
+  if (f.kind == ICAM_NURBS) return nurbs_point(f, u, v);
+  V o{f.o[0], f.o[1], f.o[2]}, x{f.x[0], f.x[1], f.x[2]}, z{f.z[0], f.z[1], f.z[2]}, y = cross(z, x);
+  if (f.kind == ICAM_PLANE) return o + x * u + y * v;
+  const double a = arc_angle(u, 0.0, 2 * M_PI, 4);
+  const V e = x * std::cos(a) + y * std::sin(a);
+  if (f.kind == ICAM_CYLINDER) return o + e * f.c[0] + z * v;
+  if (f.kind == ICAM_CONE) return o + e * (f.c[0] + v * std::tan(f.c[1])) + z * v;
+  if (f.kind == ICAM_SPHERE) {
+    const double lat = arc_angle(v, -0.5 * M_PI, M_PI, 2);
+    return o + e * (f.c[0] * std::cos(lat)) + z * (f.c[0] * std::sin(lat));
+  }
+  // ring torus (spindle tori: their sheet's angle range)
+  double psi;
+  if (f.c[0] > f.c[1]) {
+    psi = arc_angle(v, 0.0, 2 * M_PI, 4);
+  } else {
+    const double pp = std::acos(std::min(std::max(-f.c[0] / f.c[1], -1.0), 1.0));
+    psi = arc_angle(v, -pp, 2 * pp, int(std::max(std::ceil(2 * pp / (0.5 * M_PI) - 1e-9), 1.0)));
+  }
+  return o + e * (f.c[0] + f.c[1] * std::cos(psi)) + z * (f.c[1] * std::sin(psi));
+}
+// Inside a face's loops, for drawing points: crossing parity along the two-leg path from
+// the face's reference point (up or down to the point's v, then across to its u), the
+// pieces flattened to 16 chords each (points near a loop may fall on either side), the
+// chords bucketed by rows of v for the second leg.
+struct FaceTrim {
+  double ru = 0, rv = 0, v0 = 0, dv = 1;
+  bool rin = true, whole = true;
+  std::vector<std::array<double, 4>> chords;
+  std::vector<uint32_t> leg1;            // chords whose u range holds ru
+  std::vector<std::vector<uint32_t>> row;  // chords whose v range meets each row
+};
+FaceTrim face_trim(const icam_face_desc& f) {
+  FaceTrim t;
+  t.ru = f.ref[0], t.rv = f.ref[1], t.rin = f.ref_inside != 0, t.whole = f.n_pieces == 0;
+  if (t.whole) return t;
+  double vlo = 1e300, vhi = -1e300;
+  for (uint32_t k = 0; k < f.n_pieces; ++k) {
+    const double* c = f.pieces + 8 * k;
+    double pu = c[0], pv = c[1];
+    for (int m = 1; m <= 16; ++m) {
+      const double x = m / 16.0, y = 1 - x;
+      const double b0 = y * y * y, b1 = 3 * x * y * y, b2 = 3 * x * x * y, b3 = x * x * x;
+      const double qu = b0 * c[0] + b1 * c[2] + b2 * c[4] + b3 * c[6], qv = b0 * c[1] + b1 * c[3] + b2 * c[5] + b3 * c[7];
+      t.chords.push_back({pu, pv, qu, qv});
+      vlo = std::min(vlo, std::min(pv, qv)), vhi = std::max(vhi, std::max(pv, qv));
+      pu = qu, pv = qv;
+    }
+  }
+  const int R = int(std::min<size_t>(4096, t.chords.size() / 4 + 1));
+  t.v0 = vlo, t.dv = std::max((vhi - vlo) / R, 1e-12);
+  t.row.resize(size_t(R));
+  for (uint32_t i = 0; i < t.chords.size(); ++i) {
+    const auto& c = t.chords[i];
+    if (std::min(c[0], c[2]) <= t.ru && std::max(c[0], c[2]) >= t.ru) t.leg1.push_back(i);
+    const int r0 = std::max(0, int((std::min(c[1], c[3]) - t.v0) / t.dv)), r1 = std::min(R - 1, int((std::max(c[1], c[3]) - t.v0) / t.dv));
+    for (int r = r0; r <= r1; ++r) t.row[size_t(r)].push_back(i);
+  }
+  return t;
+}
+// segments (a, b) and (c, d) crossing properly
+bool seg_cross(double au, double av, double bu, double bv, const std::array<double, 4>& c) {
+  const double d1 = (c[2] - c[0]) * (av - c[1]) - (c[3] - c[1]) * (au - c[0]), d2 = (c[2] - c[0]) * (bv - c[1]) - (c[3] - c[1]) * (bu - c[0]);
+  const double d3 = (bu - au) * (c[1] - av) - (bv - av) * (c[0] - au), d4 = (bu - au) * (c[3] - av) - (bv - av) * (c[2] - au);
+  return (d1 > 0) != (d2 > 0) && (d3 > 0) != (d4 > 0);
+}
+bool face_inside(const FaceTrim& t, double u, double v) {
+  if (t.whole) return true;
+  int n = 0;
+  for (uint32_t i : t.leg1) n += seg_cross(t.ru, t.rv, t.ru, v, t.chords[i]);
+  const int r = int((v - t.v0) / t.dv);
+  if (r >= 0 && r < int(t.row.size()))
+    for (uint32_t i : t.row[size_t(r)]) n += seg_cross(t.ru, v, u, v, t.chords[i]);
+  return t.rin != bool(n & 1);
+}
+// The domain box a face's points are drawn from (a NURBS face: its knot range).
+void face_domain(const icam_face_desc& f, double lo[2], double hi[2]) {
+  lo[0] = f.lo[0], lo[1] = f.lo[1], hi[0] = f.hi[0], hi[1] = f.hi[1];
+  if (f.kind == ICAM_NURBS) {
+    lo[0] = std::max(lo[0], f.ku[f.du]), hi[0] = std::min(hi[0], f.ku[f.nu]);
+    lo[1] = std::max(lo[1], f.kv[f.dv]), hi[1] = std::min(hi[1], f.kv[f.nv]);
+  }
+  if (f.period[0] > 0) lo[0] = 0, hi[0] = f.period[0];
+  if (f.period[1] > 0) lo[1] = 0, hi[1] = f.period[1];
+}
 }  // namespace
 
 extern "C" {
@@ -1459,6 +1829,7 @@ const icam_part_desc* icam_gen_part(const char* name, uint64_t seed) {
   if (s == "c2flat") o = make_c2(seed, true);
   if (s == "c3") o = make_c3(seed);
   if (s.rfind("c4:", 0) == 0) o = make_c4(seed, arg(3));
+  if (s == "c5") o = make_c5(seed);
   if (s.rfind("dome:", 0) == 0) {
     Rng r2{(seed ^ (0x51ED270B27FF3A15ull * uint64_t(arg(5) + 1))) | 1};
     const double R = 8 + 12 * r2.next(), L = 10 + 30 * r2.next();
@@ -1521,6 +1892,55 @@ void icam_gen_near_trim(const icam_part_desc* p, uint64_t seed, size_t n, double
     V x{f.x[0], f.x[1], f.x[2]}, z{f.z[0], f.z[1], f.z[2]}, y = cross(z, x);
     V q = V{f.o[0], f.o[1], f.o[2]} + x * u + y * v;
     V d;
+    do {
+      d = V{2 * r.next() - 1, 2 * r.next() - 1, 2 * r.next() - 1};
+    } while (dot(d, d) > 1.0);
+    q = q + d * radius;
+    out[0] = q.x, out[1] = q.y, out[2] = q.z;
+  }
+}
+
+// n points within `radius` of the part's surface: a face drawn in proportion to its area
+// (estimated on a 24 x 24 grid of its domain box, the cells inside its loops), a point
+// of it drawn uniformly in its domain box until inside its loops, then offset uniformly
+// in the ball of `radius`; from the reference checks' xorshift64.
+void icam_gen_near_surface(const icam_part_desc* p, uint64_t seed, size_t n, double radius, double* xyz) {
+  Rng r{seed ? seed : 1};
+  const uint32_t nf = p->n_faces;
+  std::vector<double> cum(nf + 1, 0.0);
+  std::vector<FaceTrim> trims(nf);
+  for (uint32_t fi = 0; fi < nf; ++fi) {
+    const icam_face_desc& f = p->faces[fi];
+    trims[fi] = face_trim(f);
+    double lo[2], hi[2];
+    face_domain(f, lo, hi);
+    const int G = 24;
+    const double du = (hi[0] - lo[0]) / G, dv = (hi[1] - lo[1]) / G;
+    double area = 0;
+    for (int i = 0; i < G; ++i)
+      for (int j = 0; j < G; ++j) {
+        const double u = lo[0] + (i + 0.5) * du, v = lo[1] + (j + 0.5) * dv;
+        if (!face_inside(trims[fi], u, v)) continue;
+        const V a = face_point(f, u, v), bu = face_point(f, u + 0.5 * du, v) - face_point(f, u - 0.5 * du, v),
+                bv = face_point(f, u, v + 0.5 * dv) - face_point(f, u, v - 0.5 * dv);
+        (void)a;
+        area += len(cross(bu, bv));
+      }
+    cum[fi + 1] = cum[fi] + area;
+  }
+  for (size_t i = 0; i < n; ++i) {
+    double* out = xyz + 3 * i;
+    const double x = r.next() * cum[nf];
+    const uint32_t fi = uint32_t(std::min<size_t>(size_t(std::upper_bound(cum.begin() + 1, cum.end(), x) - (cum.begin() + 1)), nf - 1));
+    const icam_face_desc& f = p->faces[fi];
+    double lo[2], hi[2];
+    face_domain(f, lo, hi);
+    double u = 0.5 * (lo[0] + hi[0]), v = 0.5 * (lo[1] + hi[1]);
+    for (int t = 0; t < 1000; ++t) {
+      u = lo[0] + (hi[0] - lo[0]) * r.next(), v = lo[1] + (hi[1] - lo[1]) * r.next();
+      if (face_inside(trims[fi], u, v)) break;
+    }
+    V q = face_point(f, u, v), d;
     do {
       d = V{2 * r.next() - 1, 2 * r.next() - 1, 2 * r.next() - 1};
     } while (dot(d, d) > 1.0);

@@ -24,6 +24,7 @@
 // Splitting the precisions into their own kernels keeps each kernel's registers to
 // its own precision's needs.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -78,6 +79,10 @@ struct Dev {
   const uint32_t* U;
   const float* pbox;
   const uint32_t* pcode;
+  // the primitives' hierarchy (Layout::bvh_*)
+  const float* bvh_box;
+  const uint32_t* bvh_prim;
+  const uint8_t* bvh_holds;
   Layout L;
   double origin[3];
 };
@@ -193,7 +198,8 @@ struct Res {
   uint32_t part;      // the part's number in a batch (0 alone), in queue entries' top byte
   unsigned long long* counters;  // [0] bridged nodes, [1] points searched again in double,
                                  // [3] of them near a trim curve, [4] near-ties of opposite
-                                 // sign, [5] points settled by the certified search
+                                 // sign, [5] points settled by the certified search, [6] search
+                                 // blocks whose candidates did not fit their list
 };
 
 // ------------------------------------------------------------------ block culling
@@ -297,12 +303,11 @@ __device__ void block_box(float lo[3], float hi[3], float& r, float* red) {
   }
 }
 
-// An upper bound on the distance from every point of box gb to the part's boundary:
-// the least, over primitives whose boxes hold points of the boundary, of the farthest
-// distance between the two boxes (rounded up). Faces, edges, vertices and edge pieces
-// hold boundary points; a sub-patch only when it lies wholly inside its face's loops
-// (one the packer could not place may hold none), a C0 line's piece not at all (it
-// may run outside the loops).
+// The farthest distance between points of two boxes: from a box holding a point of the
+// boundary, an upper bound on the distance from every point of the other box to the
+// boundary. Faces, edges, vertices and edge pieces hold boundary points; a sub-patch
+// only when it lies wholly inside its face's loops (one the packer could not place may
+// hold none), a C0 line's piece not at all (it may run outside the loops).
 __device__ inline float box_far(const float* a, const float* b) {
   float s = 0.f;
   for (int k = 0; k < 3; ++k) {
@@ -311,30 +316,190 @@ __device__ inline float box_far(const float* a, const float* b) {
   }
   return sqrtf(s);
 }
-template <int NT>
-__device__ float block_reach(const Dev& dv, const float* gb, float* red) {
-  float m = 3.0e38f;
-  for (uint32_t i = threadIdx.x; i < dv.L.n_prims; i += NT) {
-    const uint32_t code = dv.pcode[i], kind = code >> 28, id = code & 0x0fffffffu;
-    bool holds = true;
-    if (kind == P_LEAF) {
-      holds = dv.U[dv.L.leaf_u + LEAF_U * id + 2] != 0u;
-    } else if (kind == P_CLEAF) {
-      const uint32_t e = dv.U[dv.L.cleaf_u + CLEAF_U * id + 1];
-      holds = dv.U[dv.L.edge_u + EDGE_U * e] != EDGE_CREASE;
+// The primitives whose box comes within R of box gb, found by the block walking the
+// hierarchy from its top level down: a level's nodes that pass are listed (FCAP at most),
+// and their children are the next level's to test; the primitives that pass go to cand (C
+// entries) in the hierarchy's order. Returns how many (more than C: they did not fit; C +
+// 1 also when a level's list did not fit). With `reach`, R is first lowered at each level
+// to the least farthest distance from gb to a box holding a boundary point (rounded up),
+// an upper bound on the distance from every point of gb to the boundary; *reach_out gets
+// the final R. la and lb hold FCAP entries each. When the primitives (or a level's list)
+// do not fit and `chunks` is not NULL, the level-2 nodes that passed (the grandparents of
+// every primitive within R, 4,096 primitives each at most) are copied there and counted
+// in *nchunk (0 otherwise).
+constexpr int FCAP = 1024;
+template <int NT, int C>
+__device__ int bvh_cull(const Dev& dv, const float* gb, float R, bool reach, float* reach_out, uint32_t* cand, uint32_t* la, uint32_t* lb, int* wave_tot,
+                        float* red, uint32_t* chunks = nullptr, int* nchunk = nullptr) {
+  if (nchunk) *nchunk = 0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (reach) {
+    // a first bound from one path down the hierarchy, wave 0 taking at each level the
+    // child nearest gb among those holding a boundary point: every level's least farthest
+    // distance over its holding children bounds the distance from gb to the boundary
+    if (wave == 0) {
+      uint32_t node = 0;
+      float m = 3.0e38f;
+      for (int lev = int(dv.L.bvh_top); lev >= 0; --lev) {
+        const uint32_t off = dv.L.bvh_off[lev], c = 64u * node + uint32_t(lane);
+        const bool ok = c < dv.L.bvh_cnt[lev] && dv.bvh_holds[off + c];
+        const float* bx = dv.bvh_box + 6 * (size_t(off) + c);
+        float g = ok ? box_gap(bx, gb) : 3.0e38f, f = ok ? box_far(bx, gb) : 3.0e38f;
+        uint32_t pick = ok ? c : 0xffffffffu;
+        for (int o = 1; o < 64; o <<= 1) {
+          f = fminf(f, __shfl_xor(f, o));
+          const float g2 = __shfl_xor(g, o);
+          const uint32_t p2 = uint32_t(__shfl_xor(int(pick), o));
+          if (g2 < g || (g2 == g && p2 < pick)) g = g2, pick = p2;
+        }
+        m = fminf(m, f);
+        if (pick == 0xffffffffu) break;
+        node = pick;
+      }
+      if (lane == 0) red[0] = m;
     }
-    if (holds) m = fminf(m, box_far(dv.pbox + 6 * i, gb));
-  }
-  const int tid = threadIdx.x;
-  red[tid] = m;
-  __syncthreads();
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (tid < s) red[tid] = fminf(red[tid], red[tid + s]);
     __syncthreads();
+    const float m = red[0];
+    __syncthreads();
+    if (m < 3.0e38f) R = fminf(R, m * (1.0f + 1e-5f) + 1e-4f);
   }
-  m = red[0];
+  uint32_t *par = la, *out = lb;
+  if (tid == 0) par[0] = 0u;  // the root, whose children are the top level's nodes
   __syncthreads();
-  return m < 3.0e38f ? m * (1.0f + 1e-5f) + 1e-4f : m;
+  uint32_t npar = 1;
+  int total = 0;
+  const uint32_t* p2 = nullptr;  // the level-2 nodes that passed, and how many
+  uint32_t n2 = 0;
+  for (int lev = int(dv.L.bvh_top); lev >= 0; --lev) {
+    const uint32_t off = dv.L.bvh_off[lev], cnt = dv.L.bvh_cnt[lev], nk = npar * 64u;
+    if (lev == 1) p2 = par, n2 = npar;
+    const float* boxes = dv.bvh_box + 6 * size_t(off);
+    if (reach) {
+      float m = 3.0e38f;
+      for (uint32_t k = tid; k < nk; k += NT) {
+        const uint32_t node = 64u * par[k >> 6] + (k & 63u);
+        if (node < cnt && dv.bvh_holds[off + node]) m = fminf(m, box_far(boxes + 6 * size_t(node), gb));
+      }
+      red[tid] = m;
+      __syncthreads();
+      for (int st = NT / 2; st > 0; st >>= 1) {
+        if (tid < st) red[tid] = fminf(red[tid], red[tid + st]);
+        __syncthreads();
+      }
+      m = red[0];
+      __syncthreads();
+      if (m < 3.0e38f) R = fminf(R, m * (1.0f + 1e-5f) + 1e-4f);
+    }
+    int base = 0;
+    for (uint32_t k0 = 0; k0 < nk; k0 += NT) {
+      const uint32_t k = k0 + tid;
+      uint32_t node = 0;
+      bool keep = false;
+      if (k < nk) {
+        node = 64u * par[k >> 6] + (k & 63u);
+        keep = node < cnt && box_gap(boxes + 6 * size_t(node), gb) <= R;
+      }
+      const unsigned long long m = __ballot(keep);
+      const int before = __popcll(m & ((1ull << lane) - 1ull));
+      if (lane == 0) wave_tot[wave] = __popcll(m);
+      __syncthreads();
+      int at = base;
+      for (int w = 0; w < wave; ++w) at += wave_tot[w];
+      at += before;
+      if (keep) {
+        if (lev == 0) {
+          if (at < C) cand[at] = dv.bvh_prim[node];
+        } else if (at < FCAP) {
+          out[at] = node;
+        }
+      }
+      int tot = 0;
+      for (int w = 0; w < NT / 64; ++w) tot += wave_tot[w];
+      __syncthreads();
+      base += tot;
+    }
+    if (lev == 0 || base > FCAP) {
+      total = lev == 0 ? base : C + 1;
+      if (total > C && chunks && lev <= 1 && dv.L.bvh_top >= 2) {
+        for (uint32_t k = tid; k < n2; k += NT) chunks[k] = p2[k];
+        *nchunk = int(n2);
+        __syncthreads();
+      }
+      break;
+    }
+    npar = uint32_t(base);
+    uint32_t* t = par;
+    par = out, out = t;
+  }
+  if (reach_out) *reach_out = R;
+  return total;
+}
+
+// The primitives under level-2 node `node2` whose box comes within R of gb, into cand in
+// order (4,096 at most); `sub` holds 64 entries. Returns how many.
+template <int NT>
+__device__ int bvh_chunk(const Dev& dv, const float* gb, float R, uint32_t node2, uint32_t* cand, uint32_t* sub, int* wave_tot) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // its level-1 children within R
+  {
+    const uint32_t c = 64u * node2 + uint32_t(lane);
+    const bool keep = wave == 0 && c < dv.L.bvh_cnt[1] && box_gap(dv.bvh_box + 6 * (size_t(dv.L.bvh_off[1]) + c), gb) <= R;
+    const unsigned long long m = __ballot(keep);
+    if (keep) sub[__popcll(m & ((1ull << lane) - 1ull))] = c;
+    if (wave == 0 && lane == 0) wave_tot[0] = __popcll(m);
+  }
+  __syncthreads();
+  const uint32_t nsub = uint32_t(wave_tot[0]), cnt = dv.L.bvh_cnt[0], nk = nsub * 64u;
+  __syncthreads();
+  int base = 0;
+  for (uint32_t k0 = 0; k0 < nk; k0 += NT) {
+    const uint32_t k = k0 + tid;
+    uint32_t node = 0;
+    bool keep = false;
+    if (k < nk) {
+      node = 64u * sub[k >> 6] + (k & 63u);
+      keep = node < cnt && box_gap(dv.bvh_box + 6 * size_t(node), gb) <= R;
+    }
+    const unsigned long long m = __ballot(keep);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_tot[wave] = __popcll(m);
+    __syncthreads();
+    int at = base;
+    for (int w = 0; w < wave; ++w) at += wave_tot[w];
+    if (keep) cand[at + before] = dv.bvh_prim[node];
+    int tot = 0;
+    for (int w = 0; w < NT / 64; ++w) tot += wave_tot[w];
+    __syncthreads();
+    base += tot;
+  }
+  return base;
+}
+
+// Level-2 nodes ordered by their box's distance from gb, nearest first (ties by node): a
+// bitonic sort of 64-bit keys in `key` (n rounded up to a power of two, 8-byte aligned).
+template <int NT>
+__device__ void sort_chunks(const Dev& dv, const float* gb, uint32_t* chunks, int n, unsigned long long* key) {
+  if (n <= 1) return;
+  const int tid = threadIdx.x;
+  const float* boxes = dv.bvh_box + 6 * size_t(dv.L.bvh_off[2]);
+  int m = 2;
+  while (m < n) m <<= 1;
+  for (int k = tid; k < m; k += NT)
+    key[k] = k < n ? (uint64_t(__float_as_uint(box_gap(boxes + 6 * size_t(chunks[k]), gb))) << 32) | uint64_t(chunks[k]) : ~0ull;
+  __syncthreads();
+  for (int size = 2; size <= m; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < m; i += NT) {
+        const int l = i ^ stride;
+        if (l > i) {
+          const uint64_t a = key[i], b = key[l];
+          if ((a > b) == ((i & size) == 0)) key[i] = b, key[l] = a;
+        }
+      }
+      __syncthreads();
+    }
+  for (int k = tid; k < n; k += NT) chunks[k] = uint32_t(key[k] & 0xffffffffu);
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------ one lane's search
@@ -428,11 +593,17 @@ template <class T> __device__ inline Best<T> item_best(const uint32_t* w) {
   return b;
 }
 
-template <class T, int NT, int Q, int K, int C, bool SETTLE = false>
-__device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk,
-                          const Res* defer = nullptr, long point = 0) {
+template <class T> __device__ inline Best<T> best_init(T bound) {
   Best<T> b;
   b.d = bound, b.code = NONE, b.a = b.b = T(0), b.sgn = T(1), b.lmin = TypeEps<T>::big, b.d2 = TypeEps<T>::big, b.sgn2 = T(1), b.unsure = false;
+  return b;
+}
+// One candidate list's share of a point's search, the lane's answer b carried in and
+// out (a block whose candidates come in chunks calls it once a chunk; every part under
+// the final answer is still read, in the chunk that holds it).
+template <class T, int NT, int Q, int K, int C, bool SETTLE = false>
+__device__ inline void search_in(const Tab<T>& tb, v3<T> p, bool valid, Best<T>& b, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk,
+                                 const Res* defer = nullptr, long point = 0) {
   const bool all = nc > C;
   const uint32_t n = all ? tb.L.n_prims : uint32_t(nc);
   const int lane = threadIdx.x, sub = lane & (K - 1);
@@ -449,7 +620,7 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const 
       visit(tb, code, tb.pbox + 6 * i, p, b, tm, eps);
     }
   }
-  if (K == 1 && !parts) return b;
+  if (K == 1 && !parts) return;
   int m = 0;
   if (parts && !all) {
     const T bd = b.d;
@@ -484,7 +655,7 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const 
       item_state(w, b);
       for (int q = 0; q < m; ++q) item_put(w, Item<T>::LB + q * int(sizeof(T) / 4), qlb[q * NT + lane]), w[Item<T>::PR + q] = cand[qk[q * NT + lane]];
       b.code = DEFERRED;
-      return b;
+      return;
     }
   }
   T last_lb = T(-1);
@@ -528,7 +699,13 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const 
       try_part<T, SETTLE>(tb, all ? k : cand[k], next_lb, p, b, tm, eps);
     }
   }
-  if (K > 1) {
+}
+// The K lanes' answers of a group merged (its lanes' least distance, ties: the lowest
+// lane; near-ties of opposite signs among all its lanes' answers and runners-up are
+// reported as a runner-up at the same distance).
+template <class T, int K> __device__ inline void group_merge(Best<T>& b, T eps) {
+  const int lane = threadIdx.x, sub = lane & (K - 1);
+  {
     const T dmin = group_min<K>(b.d);
     const int wsub = group_min<K>(b.d == dmin ? sub : K);
     const int src = (lane & ~(K - 1)) + wsub;
@@ -543,6 +720,14 @@ __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const 
     b.d = dmin, b.sgn = sw;
     b.d2 = tie ? dmin : TypeEps<T>::big, b.sgn2 = tie ? -sw : sw;
   }
+}
+// A point's whole search over one candidate list.
+template <class T, int NT, int Q, int K, int C, bool SETTLE = false>
+__device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk,
+                          const Res* defer = nullptr, long point = 0) {
+  Best<T> b = best_init<T>(bound);
+  search_in<T, NT, Q, K, C, SETTLE>(tb, p, valid, b, cand, nc, tm, eps, qlb, qk, defer, point);
+  if (K > 1) group_merge<T, K>(b, eps);
   return b;
 }
 
@@ -589,7 +774,7 @@ template <int S> struct Shape {
   static constexpr int NT = 256;
   static constexpr int K = S == SRC_CENTRES ? 16 : S == SRC_WIDE ? 4 : 1;  // lanes a point
   static constexpr int Q = S == SRC_CENTRES ? 8 : 16;   // queue entries a lane
-  static constexpr int C = pts<S>() ? 4096 : 1024;  // a block's candidates, at most
+  static constexpr int C = S == SRC_NODES ? 1024 : 4096;  // a block's candidates, at most
 };
 constexpr int TILE_X = 4, TILE_Y = 2, TILE_Z = 2;
 static_assert(TILE_X * TILE_Y * TILE_Z * Shape<SRC_CENTRES>::K == Shape<SRC_CENTRES>::NT, "a centres block covers its tile");
@@ -647,10 +832,12 @@ template <int S> __device__ inline long block_of(const Job& j, long i) {
 template <int S> constexpr int rounds() { return S == SRC_NODES ? 3 : 1; }
 
 // The block's candidates: the primitives within reach of its points' box, where the
-// reach covers every point's answer (its bound, or for unbounded points the nearest
-// of the first vertices, points of the surface).
+// reach covers every point's answer (its bound, or for unbounded points the boundary's
+// upper bound over the block's box, from the hierarchy).
 template <int S>
-__device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t* cand, uint32_t* tmp, float* key, float* red, int* wave_tot) {
+__device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t* cand, uint32_t* tmp, float* key, float* red, int* wave_tot,
+                                uint32_t* chunks = nullptr, int* nchunk = nullptr, float* gb_out = nullptr, float* r_out = nullptr,
+                                const uint32_t* only_flag = nullptr, uint32_t only = 0) {
   constexpr int NT = Shape<S>::NT;
   float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
   float r = 0.f;
@@ -677,25 +864,16 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
   } else {
   for (int m = 0; m < rounds<S>(); ++m) {
     long i = block_point<S>(j, blk, m);
-    if (i < 0) continue;
+    // (scattered points searched again: only those flagged `only` set the box)
+    if (i < 0 || (only_flag && only_flag[i] != only)) continue;
     double bound;
     v3<double> p = job_point<S>(j, dv, i, bound);
-    float ub = float(fmin(bound, 3.0e38));
-    if (pts<S>()) {
-      uint32_t nv = dv.L.n_verts < 16 ? dv.L.n_verts : 16;
-      for (uint32_t q = 0; q < nv; ++q) ub = fminf(ub, float(len(p - ld3((const ICAM_K double*)dv.F64 + dv.L.vert_f + VERT_F * q))) * (1.0f + 1e-5f) + 1e-5f);
-    }
+    const float ub = float(fmin(bound, 3.0e38));
     lo[0] = fminf(lo[0], float(p.x)), lo[1] = fminf(lo[1], float(p.y)), lo[2] = fminf(lo[2], float(p.z));
     hi[0] = fmaxf(hi[0], float(p.x)), hi[1] = fmaxf(hi[1], float(p.y)), hi[2] = fmaxf(hi[2], float(p.z));
     r = fmaxf(r, ub * (1.0f + 1e-5f) + 1e-4f);
   }
   block_box<NT>(lo, hi, r, red);
-  if (pts<S>()) {
-    // scattered points: the boundary's own bound over the block's box, usually far
-    // tighter than the nearest vertices'
-    const float gp[6] = {lo[0] - 1e-4f, lo[1] - 1e-4f, lo[2] - 1e-4f, hi[0] + 1e-4f, hi[1] + 1e-4f, hi[2] + 1e-4f};
-    r = fminf(r, block_reach<NT>(dv, gp, red));
-  }
   }
   float gb[6] = {lo[0] - 1e-4f, lo[1] - 1e-4f, lo[2] - 1e-4f, hi[0] + 1e-4f, hi[1] + 1e-4f, hi[2] + 1e-4f};
   int nc;
@@ -703,16 +881,23 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
     // the brick's tile's list, when it was kept, instead of the whole primitive list
     long tile = tile_of(j.lv, long(j.list[blk]));
     uint32_t cnt = j.tile_count[tile];
-    nc = cnt <= uint32_t(TCAP) ? cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot, j.tile_cand + size_t(tile) * TCAP, cnt) : cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot);
+    nc = cnt <= uint32_t(TCAP) ? cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot, j.tile_cand + size_t(tile) * TCAP, cnt)
+                               : bvh_cull<NT, Shape<S>::C>(dv, gb, r, false, nullptr, cand, tmp, tmp + FCAP, wave_tot, red);
   } else if (S == SRC_CENTRES && j.super_cand) {
     // the tile's super-tile's list (a superset in the same order), when it was kept
     const long sp = super_of_tile(j.lv, blk);
     const uint32_t cnt = j.super_count[sp];
     nc = cnt <= uint32_t(SCAP) ? cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot, j.super_cand + size_t(sp) * SCAP, cnt)
-                               : cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot);
+                               : bvh_cull<NT, Shape<S>::C>(dv, gb, r, false, nullptr, cand, tmp, tmp + FCAP, wave_tot, red);
   } else {
-    nc = cull<NT, Shape<S>::C>(dv, gb, r, cand, wave_tot);
+    // scattered points: the boundary's own bound over the block's box from the hierarchy
+    // (past the list, the level-1 nodes over them, taken nearest first in chunks)
+    nc = bvh_cull<NT, Shape<S>::C>(dv, gb, r, pts<S>(), &r, cand, tmp, tmp + FCAP, wave_tot, red, chunks, nchunk);
+    if (nchunk && *nchunk > 0) sort_chunks<NT>(dv, gb, chunks, *nchunk, reinterpret_cast<unsigned long long*>(tmp));
   }
+  if (gb_out)
+    for (int k = 0; k < 6; ++k) gb_out[k] = gb[k];
+  if (r_out) *r_out = r;
   sort_cand<NT>(dv, gb, cand, tmp, key, nc);
   if (S == SRC_CENTRES && j.tile_cand) {
     for (int k = threadIdx.x; k < nc && k < TCAP; k += NT) j.tile_cand[size_t(blk) * TCAP + k] = cand[k];
@@ -723,7 +908,7 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
 
 // Super-tile sp's list: the primitives within reach of its centres, culled from the
 // whole list (in list order) by a block of 256.
-__device__ __forceinline__ void super_block(const Dev& dv, const Job& j, long sp, uint32_t* cand, int* wave_tot) {
+__device__ __forceinline__ void super_block(const Dev& dv, const Job& j, long sp, uint32_t* cand, int* wave_tot, uint32_t* lists, float* red) {
   const long tx = (j.lv.nb[0] + TILE_X - 1) / TILE_X, ty = (j.lv.nb[1] + TILE_Y - 1) / TILE_Y;
   const long sx = (tx + SUPER - 1) / SUPER, sy = (ty + SUPER - 1) / SUPER;
   const long x0 = TILE_X * SUPER * (sp % sx), y0 = TILE_Y * SUPER * ((sp / sx) % sy), z0 = TILE_Z * SUPER * (sp / (sx * sy));
@@ -734,14 +919,16 @@ __device__ __forceinline__ void super_block(const Dev& dv, const Job& j, long sp
   const v3<double> b = job_point<SRC_CENTRES>(j, dv, x1 + j.lv.nb[0] * (y1 + long(j.lv.nb[1]) * z1), bound);
   const float r = float(fmin(bound, 3.0e38)) * (1.0f + 1e-5f) + 1e-4f;
   const float gb[6] = {float(a.x) - 1e-4f, float(a.y) - 1e-4f, float(a.z) - 1e-4f, float(b.x) + 1e-4f, float(b.y) + 1e-4f, float(b.z) + 1e-4f};
-  const int nc = cull<256, SCAP>(dv, gb, r, cand, wave_tot);
+  const int nc = bvh_cull<256, SCAP>(dv, gb, r, false, nullptr, cand, lists, lists + FCAP, wave_tot, red);
   for (int k = threadIdx.x; k < nc && k < SCAP; k += 256) j.super_cand[size_t(sp) * SCAP + k] = cand[k];
   if (threadIdx.x == 0) j.super_count[sp] = uint32_t(nc);
 }
 __global__ __launch_bounds__(256) void k_super(Dev dv, Job j) {
   __shared__ uint32_t cand[SCAP];
+  __shared__ uint32_t lists[2 * FCAP];
+  __shared__ float red[256];
   __shared__ int wave_tot[4];
-  super_block(dv, j, blockIdx.x, cand, wave_tot);
+  super_block(dv, j, blockIdx.x, cand, wave_tot, lists, red);
 }
 
 // ------------------------------------------------------------------ search kernels
@@ -780,14 +967,44 @@ template <int S, class T> struct SearchLds {
   uint32_t cand[Shape<S>::C];
   float red[NT];
   int wave_tot[NT / 64];
-  alignas(16) unsigned char scratch[cmax(SORTC * 12, Q * NT * (int(sizeof(T)) + 2))];
+  alignas(16) unsigned char scratch[cmax(cmax(SORTC * 12, 2 * FCAP * 4), Q * NT * (int(sizeof(T)) + 2))];
   uint16_t list[S == SRC_NODES ? 736 : 2];  // a brick's unsure nodes, compacted
   int count;
+  uint32_t chunks[pts<S>() ? FCAP : 1];  // scattered points past their list: level-2 nodes
+  uint32_t sub[pts<S>() ? 64 : 1];       // a chunk's level-1 nodes
   __device__ uint32_t* tmp() { return (uint32_t*)scratch; }
   __device__ float* key() { return (float*)(scratch + SORTC * 4); }
   __device__ T* qlb() { return (T*)scratch; }
   __device__ uint16_t* qk() { return (uint16_t*)(scratch + Q * NT * sizeof(T)); }
 };
+
+// A scattered point's search when its block's candidates did not fit the list: chunk by
+// chunk over the hierarchy's level-2 nodes within reach, nearest first, each chunk's
+// primitives within reach culled into the list and the search carried on over them; a
+// chunk no point of the block can find anything nearer in (its box no nearer than any
+// point's answer so far) is passed over.
+template <int S, class T, bool SETTLE>
+__device__ Best<T> search_chunks(const Dev& dv, const Tab<T>& tb, v3<T> p, bool valid, T bound, SearchLds<S, T>& sm, int nchunk, const float* gb, float r,
+                                 T tm, T eps) {
+  constexpr int NT = Shape<S>::NT, Q = Shape<S>::Q, K = Shape<S>::K, C = Shape<S>::C;
+  Best<T> b = best_init<T>(bound);
+  if constexpr (!pts<S>()) {
+    return b;  // (only scattered points are searched in chunks)
+  } else {
+  static_assert(C >= 4096, "a chunk holds a level-2 node's primitives");
+  const float* boxes = dv.bvh_box + 6 * size_t(dv.L.bvh_off[2]);
+  for (int c = 0; c < nchunk; ++c) {
+    const uint32_t node2 = sm.chunks[c];
+    const int need = valid && box_dist(p, (const ICAM_K float*)(boxes + 6 * size_t(node2))) < b.d;
+    if (!__syncthreads_or(need)) continue;
+    const int cnt = bvh_chunk<NT>(dv, gb, r, node2, sm.cand, sm.sub, sm.wave_tot);
+    search_in<T, NT, Q, K, C, SETTLE>(tb, p, valid, b, sm.cand, cnt, tm, eps, sm.qlb(), sm.qk());
+    __syncthreads();
+  }
+  if (K > 1) group_merge<T, K>(b, eps);
+  return b;
+  }
+}
 
 // Block blk of a job searched, in single (T = float: records for k_refine) or double
 // precision (T = double: final answers). A brick's nodes leave their parts' rounds to
@@ -796,7 +1013,11 @@ template <int S, class T>
 __device__ __forceinline__ void search_block(const Dev& dv, const Job& j, const Res& res, long blk, SearchLds<S, T>& sm) {
   constexpr int NT = Shape<S>::NT;
   constexpr int Q = Shape<S>::Q;
-  const int nc = block_candidates<S>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot);
+  int nchunk = 0;
+  float gb[6], rr = 0.f;
+  const int nc = block_candidates<S>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot, pts<S>() ? sm.chunks : nullptr, &nchunk, gb, &rr);
+  const bool chunked = pts<S>() && nchunk > 0;
+  if (threadIdx.x == 0 && nc > Shape<S>::C && !chunked) atomicAdd(&res.counters[6], 1ull);
   const Tab<T> tb = tab_of<T>(dv);
   const T eps = sizeof(T) == 4 ? T(dv.L.eps32) : T(dv.L.eps64);
   const T tm = sizeof(T) == 4 ? T(16) * eps : T(0);
@@ -806,7 +1027,7 @@ __device__ __forceinline__ void search_block(const Dev& dv, const Job& j, const 
   const v3<double> nfirst = S == SRC_NODES ? node_first(j, dv, blk, nh) : mk3(0.0, 0.0, 0.0);
   for (int m = 0; m < rounds<S>(); ++m) {
     long i = block_point<S>(j, blk, m);
-    if (K == 1 && i < 0) continue;
+    if (K == 1 && i < 0 && !chunked) continue;
     const bool valid = i >= 0;
     double bound = 0.0;
     v3<double> p;
@@ -814,8 +1035,9 @@ __device__ __forceinline__ void search_block(const Dev& dv, const Job& j, const 
       p = node_at(nfirst, nh, int(threadIdx.x) + NT * m), bound = nh * 1.7320508075688772;
     else
       p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
-    Best<T> b = search<T, NT, Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, search_bound<T>(bound, eps), sm.cand, nc, tm, eps, sm.qlb(), sm.qk(),
-                                                 S == SRC_NODES && res.items ? &res : nullptr, i);
+    Best<T> b = chunked ? search_chunks<S, T, false>(dv, tb, cv3<T>(p), valid, search_bound<T>(bound, eps), sm, nchunk, gb, rr, tm, eps)
+                        : search<T, NT, Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, search_bound<T>(bound, eps), sm.cand, nc, tm, eps, sm.qlb(), sm.qk(),
+                                                           S == SRC_NODES && res.items ? &res : nullptr, i);
     if (!valid || !owner || b.code == DEFERRED) continue;
     finish_point<T>(dv, res, i, bound, b, eps);
   }
@@ -963,7 +1185,11 @@ template <int S, bool SETTLE>
 __device__ __forceinline__ void redo_block(const Dev& dv, const Job& j, const Res& res, long blk, SearchLds<S, double>& sm) {
   const uint32_t want = SETTLE ? 4u : 3u;
   constexpr int NT = Shape<S>::NT;
-  const int nc = block_candidates<S>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot);
+  int nchunk = 0;
+  float gb[6], rr = 0.f;
+  const int nc = block_candidates<S>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot, pts<S>() ? sm.chunks : nullptr, &nchunk, gb, &rr,
+                                     pts<S>() ? res.flag : nullptr, want);
+  const bool chunked = pts<S>() && nchunk > 0;
   const Tab<double> tb = tab_of<double>(dv);
   if constexpr (S == SRC_NODES && !SETTLE) {
     // the few unsure nodes of a brick (near trim curves, near ties), compacted and
@@ -1000,11 +1226,12 @@ __device__ __forceinline__ void redo_block(const Dev& dv, const Job& j, const Re
   for (int m = 0; m < rounds<S>(); ++m) {
     long i = block_point<S>(j, blk, m);
     const bool valid = i >= 0 && res.flag[i] == want;
-    if (K == 1 && !valid) continue;
+    if (K == 1 && !valid && !chunked) continue;
     double bound = 0.0;
     v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
-    Best<double> b = search<double, NT, Shape<S>::Q, K, Shape<S>::C, SETTLE>(tb, p, valid, bound < 1e300 ? bound * (1.0 + 1e-12) : bound, sm.cand,
-                                                                             nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk());
+    const double sb = bound < 1e300 ? bound * (1.0 + 1e-12) : bound;
+    Best<double> b = chunked ? search_chunks<S, double, SETTLE>(dv, tb, p, valid, sb, sm, nchunk, gb, rr, 0.0, dv.L.eps64)
+                             : search<double, NT, Shape<S>::Q, K, Shape<S>::C, SETTLE>(tb, p, valid, sb, sm.cand, nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk());
     if (!valid || !owner) continue;
     double val;
     uint32_t flag, feat;
@@ -1595,6 +1822,8 @@ __global__ __launch_bounds__(Shape<S>::NT) __attribute__((amdgpu_waves_per_eu((S
 // Every part's super-tile lists, a block each.
 __global__ __launch_bounds__(256) void kb_super(BatchArgs a) {
   __shared__ uint32_t cand[SCAP];
+  __shared__ uint32_t lists[2 * FCAP];
+  __shared__ float red[256];
   __shared__ int wave_tot[4];
   const long total = a.super_off[a.n];
   for (long w = blockIdx.x; w < total; w += gridDim.x) {
@@ -1602,7 +1831,7 @@ __global__ __launch_bounds__(256) void kb_super(BatchArgs a) {
     const ICAM_K BPart& P = bpart(a, p);
     const Dev dv = ldk(P.dv);
     const Job j = ldk(P.jc);
-    super_block(dv, j, w - a.super_off[p], cand, wave_tot);
+    super_block(dv, j, w - a.super_off[p], cand, wave_tot, lists, red);
     __syncthreads();
   }
 }
@@ -1828,7 +2057,7 @@ struct Buf {
 // ends). Host code uses the device address hipGetSymbolAddress gives for it
 // (noalloc_scratch), never the array's name.
 constexpr size_t NOALLOC_POINTS = size_t(1) << 18;
-constexpr size_t NOALLOC_BYTES = size_t(24) << 20;
+constexpr size_t NOALLOC_BYTES = size_t(32) << 20;
 static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 
 namespace {
@@ -1860,7 +2089,7 @@ struct icam_hip {
   hipStream_t stream = nullptr;
   Packed pk;
   Dev dv{};
-  Buf F32, F64, U, pbox, pcode;
+  Buf F32, F64, U, pbox, pcode, bvh_box, bvh_prim, bvh_holds;
   // per-point results of the centres' and the nodes' searches, and host-call staging
   struct ResBufs {
     Buf code, pa, pb, d32, info, val, flag, queue, qcount, qmark, items, icount, squeue, scount, smark;
@@ -1869,10 +2098,11 @@ struct icam_hip {
     }
   } centres, nodes;
   Buf q_xyz, q_bound, q_d, q_flag, q_feat;
-  // scattered queries taken in cell order: the part's box (world), the cells' counts,
-  // each point's cell and rank in it, the order, and the points and results in it
+  // scattered queries taken in Morton order: the part's box (world), the sort's scratch,
+  // the keys before and after the sort, the indices before and after, and the points
+  // and results in that order
   double box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};
-  Buf o_count, o_cell, o_rank, o_idx, o_xyz, o_d, o_flag, o_feat;
+  Buf o_count, o_cell, o_rank, o_idx0, o_idx, o_xyz, o_d, o_flag, o_feat;
   // the level
   Buf dc, band, sgn, total, list, staging, cls, slot, stotal, map, atlas, counters, changed, firsts, tile_cand, tile_count, super_cand, super_count;
   std::vector<std::pair<const char*, double>> stages;
@@ -2032,18 +2262,16 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
 
 // ------------------------------------------------------------------ point order
 //
-// Scattered queries are searched in the Morton order of a 64^3 grid of cells over the
-// part's box (points outside it fall in its border cells): a block's 256 points then
-// lie together and cull to a short candidate list, where points in the caller's order
-// may each lie anywhere. A counting sort (cells and ranks by atomics, one exclusive
-// scan, placement); the order within a cell is the atomics' and does not matter. The
-// results go back to the caller's order.
-constexpr int ORDER_BITS = 6;
-constexpr uint32_t ORDER_CELLS = 1u << (3 * ORDER_BITS);
+// Scattered queries are searched in the Morton order of their positions in the part's box
+// (10 bits an axis, points outside it clamped to its border): a block's points then lie
+// together and cull to a short candidate list, where points in the caller's order may
+// each lie anywhere. The keys are sorted with their indices by rocPRIM's radix sort (a
+// plain library sort), the points gathered in that order, and the results put back in
+// the caller's order.
+constexpr int ORDER_BITS = 10;
 constexpr size_t ORDER_MIN = 8192;  // fewer points are searched as given
 
-__global__ void k_order_cells(const double* xyz, long n, double lx, double ly, double lz, double sx, double sy, double sz, uint32_t* count,
-                              uint32_t* cell, uint32_t* rank) {
+__global__ void k_order_keys(const double* xyz, long n, double lx, double ly, double lz, double sx, double sy, double sz, uint32_t* key, uint32_t* idx) {
   const long i = long(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
   const double lo[3] = {lx, ly, lz}, sc[3] = {sx, sy, sz};
@@ -2053,39 +2281,22 @@ __global__ void k_order_cells(const double* xyz, long n, double lx, double ly, d
     const uint32_t q = f >= 0.0 ? (f < double((1 << ORDER_BITS) - 1) ? uint32_t(f) : uint32_t((1 << ORDER_BITS) - 1)) : 0u;
     for (int b = 0; b < ORDER_BITS; ++b) code |= ((q >> b) & 1u) << (3 * b + k);
   }
-  cell[i] = code;
-  rank[i] = atomicAdd(&count[code], 1u);
+  key[i] = code;
+  idx[i] = uint32_t(i);
 }
 
-// Exclusive scan of the cells' counts in place, by one block.
-__global__ __launch_bounds__(1024) void k_order_scan(uint32_t* count) {
-  __shared__ uint32_t part[1024];
-  constexpr uint32_t per = ORDER_CELLS / 1024;
-  uint32_t* c = count + size_t(threadIdx.x) * per;
-  uint32_t sum = 0;
-  for (uint32_t k = 0; k < per; ++k) sum += c[k];
-  part[threadIdx.x] = sum;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const uint32_t v = int(threadIdx.x) >= o ? part[threadIdx.x - o] : 0u;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  uint32_t base = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-  for (uint32_t k = 0; k < per; ++k) {
-    const uint32_t v = c[k];
-    c[k] = base;
-    base += v;
-  }
+__global__ void k_order_place(const double* xyz, long n, const uint32_t* idx, double* oxyz) {
+  const long k = long(blockIdx.x) * 256 + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = idx[k];
+  for (int c = 0; c < 3; ++c) oxyz[3 * size_t(k) + c] = xyz[3 * size_t(i) + c];
 }
 
-__global__ void k_order_place(const double* xyz, long n, const uint32_t* cell, const uint32_t* rank, const uint32_t* start, uint32_t* idx, double* oxyz) {
-  const long i = long(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t at = start[cell[i]] + rank[i];
-  idx[at] = uint32_t(i);
-  for (int k = 0; k < 3; ++k) oxyz[3 * size_t(at) + k] = xyz[3 * i + k];
+// The radix sort's scratch for n keys with their indices (bytes).
+size_t order_sort_bytes(size_t n) {
+  size_t bytes = 0;
+  rocprim::radix_sort_pairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, 3 * ORDER_BITS);
+  return bytes;
 }
 
 __global__ void k_order_back(long n, const uint32_t* idx, const double* od, const uint32_t* of, const uint32_t* oft, double* d, uint32_t* f, uint32_t* ft) {
@@ -2128,7 +2339,8 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
   std::vector<float> f32 = to_f32(h->pk.F);
   if (upload(h->F32, f32.data(), f32.size() * 4) || upload(h->F64, h->pk.F.data(), h->pk.F.size() * 8) ||
       upload(h->U, h->pk.U.data(), h->pk.U.size() * 4) || upload(h->pbox, h->pk.pbox.data(), h->pk.pbox.size() * 4) ||
-      upload(h->pcode, h->pk.pcode.data(), h->pk.pcode.size() * 4)) {
+      upload(h->pcode, h->pk.pcode.data(), h->pk.pcode.size() * 4) || upload(h->bvh_box, h->pk.bvh_box.data(), h->pk.bvh_box.size() * 4) ||
+      upload(h->bvh_prim, h->pk.bvh_prim.data(), h->pk.bvh_prim.size() * 4) || upload(h->bvh_holds, h->pk.bvh_holds.data(), h->pk.bvh_holds.size())) {
     std::string e = g_error;
     icam_hip_close(h);
     return fail(e);
@@ -2145,6 +2357,7 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
   h->dv.U = h->U.as<uint32_t>();
   h->dv.pbox = h->pbox.as<float>();
   h->dv.pcode = h->pcode.as<uint32_t>();
+  h->dv.bvh_box = h->bvh_box.as<float>(), h->dv.bvh_prim = h->bvh_prim.as<uint32_t>(), h->dv.bvh_holds = h->bvh_holds.as<uint8_t>();
   h->dv.L = h->pk.L;
   for (int k = 0; k < 3; ++k) h->dv.origin[k] = h->pk.origin[k];
   for (int k = 0; k < 3; ++k) h->box_lo[k] = 1e300, h->box_hi[k] = -1e300;
@@ -2163,9 +2376,9 @@ void icam_hip_close(icam_hip* h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   h->centres.free();
   h->nodes.free();
-  for (Buf* b : {&h->F32, &h->F64, &h->U, &h->pbox, &h->pcode, &h->q_xyz, &h->q_bound, &h->q_d, &h->q_flag, &h->q_feat, &h->dc, &h->band, &h->sgn, &h->total,
+  for (Buf* b : {&h->F32, &h->F64, &h->U, &h->pbox, &h->pcode, &h->bvh_box, &h->bvh_prim, &h->bvh_holds, &h->q_xyz, &h->q_bound, &h->q_d, &h->q_flag, &h->q_feat, &h->dc, &h->band, &h->sgn, &h->total,
                  &h->list, &h->staging, &h->cls, &h->slot, &h->stotal, &h->map, &h->atlas, &h->counters, &h->changed, &h->firsts,
-                 &h->tile_cand, &h->tile_count, &h->super_cand, &h->super_count, &h->o_count, &h->o_cell, &h->o_rank, &h->o_idx, &h->o_xyz, &h->o_d, &h->o_flag, &h->o_feat})
+                 &h->tile_cand, &h->tile_count, &h->super_cand, &h->super_count, &h->o_count, &h->o_cell, &h->o_rank, &h->o_idx0, &h->o_idx, &h->o_xyz, &h->o_d, &h->o_flag, &h->o_feat})
     b->free();
   if (h->stream) hipStreamDestroy(h->stream);
   if (h->done) hipEventDestroy(h->done);
@@ -2200,7 +2413,9 @@ int icam_hip_warm(icam_hip* h, double secs) {
 // points and results in that order; the single-precision records; the fallback's and
 // the settle's block queues, their counts and marks; the statistics.
 struct QBufs {
-  uint32_t *count, *cell, *rank, *idx;
+  uint32_t *key, *key2, *idx0, *idx;  // the Morton keys and indices, before and after the sort
+  void* sort;                          // the sort's scratch (sort_bytes)
+  size_t sort_bytes;
   double* xyz;
   double* d;
   uint32_t *flag, *feat;
@@ -2216,15 +2431,17 @@ static int query_bufs(icam_hip* h, size_t n, bool mixed, bool order, bool feat, 
   q = QBufs{};
   const size_t m = std::max<size_t>(n, 1);
   if (order) {
-    HIP_OK(h->o_count.need(ORDER_CELLS * 4));
+    q.sort_bytes = order_sort_bytes(m);
+    HIP_OK(h->o_count.need(std::max<size_t>(q.sort_bytes, 16)));
     HIP_OK(h->o_cell.need(m * 4));
     HIP_OK(h->o_rank.need(m * 4));
+    HIP_OK(h->o_idx0.need(m * 4));
     HIP_OK(h->o_idx.need(m * 4));
     HIP_OK(h->o_xyz.need(m * 24));
     HIP_OK(h->o_d.need(m * 8));
     HIP_OK(h->o_flag.need(m * 4));
     if (feat) HIP_OK(h->o_feat.need(m * 4));
-    q.count = h->o_count.as<uint32_t>(), q.cell = h->o_cell.as<uint32_t>(), q.rank = h->o_rank.as<uint32_t>(), q.idx = h->o_idx.as<uint32_t>();
+    q.sort = h->o_count.p, q.key = h->o_cell.as<uint32_t>(), q.key2 = h->o_rank.as<uint32_t>(), q.idx0 = h->o_idx0.as<uint32_t>(), q.idx = h->o_idx.as<uint32_t>();
     q.xyz = h->o_xyz.as<double>(), q.d = h->o_d.as<double>(), q.flag = h->o_flag.as<uint32_t>(), q.feat = feat ? h->o_feat.as<uint32_t>() : nullptr;
   }
   icam_hip::ResBufs& B = h->nodes;
@@ -2261,7 +2478,9 @@ static size_t query_carve(char* base, size_t cap, size_t n, bool mixed, bool ord
   };
   const size_t m = std::max<size_t>(n, 1), blk = m / 64 + 1;  // blocks of 64 points or more
   if (order) {
-    q.count = (uint32_t*)take(ORDER_CELLS * 4), q.cell = (uint32_t*)take(m * 4), q.rank = (uint32_t*)take(m * 4), q.idx = (uint32_t*)take(m * 4);
+    q.sort_bytes = order_sort_bytes(m);
+    q.sort = take(std::max<size_t>(q.sort_bytes, 16)), q.key = (uint32_t*)take(m * 4), q.key2 = (uint32_t*)take(m * 4), q.idx0 = (uint32_t*)take(m * 4);
+    q.idx = (uint32_t*)take(m * 4);
     q.xyz = (double*)take(m * 24), q.d = (double*)take(m * 8), q.flag = (uint32_t*)take(m * 4);
     q.feat = feat ? (uint32_t*)take(m * 4) : nullptr;
   }
@@ -2284,9 +2503,8 @@ static int query_impl(icam_hip* h, int precision, const double* d_xyz, const dou
   // scattered points without bounds of their own: searched in cell order
   const bool order = query_order(n, d_bounds != nullptr);
   {
-    FillList f{{q.counters ? (uint32_t*)q.counters : nullptr, order ? q.count : nullptr, nullptr, nullptr, nullptr, nullptr},
-               {16u, order ? size_t(ORDER_CELLS) : 0u, 0u, 0u, 0u, 0u}};
-    HIP_OK(launch(st, "zero", s, dim3(order ? 1024u : 1u), dim3(256), kb_fill, f));
+    FillList f{{(uint32_t*)q.counters, nullptr, nullptr, nullptr, nullptr, nullptr}, {16u, 0u, 0u, 0u, 0u, 0u}};
+    HIP_OK(launch(st, "zero", s, dim3(1), dim3(256), kb_fill, f));
   }
   if (order) {
     double sc[3];
@@ -2294,11 +2512,11 @@ static int query_impl(icam_hip* h, int precision, const double* d_xyz, const dou
       const double side = h->box_hi[k] - h->box_lo[k];
       sc[k] = side > 0.0 ? double(1 << ORDER_BITS) / side : 0.0;
     }
-    HIP_OK(launch(st, "order", s, dim3(blocks), dim3(256), k_order_cells, d_xyz, long(n), h->box_lo[0], h->box_lo[1], h->box_lo[2], sc[0], sc[1], sc[2],
-                  q.count, q.cell, q.rank));
-    HIP_OK(launch(st, "order", s, dim3(1), dim3(1024), k_order_scan, q.count));
-    HIP_OK(launch(st, "order", s, dim3(blocks), dim3(256), k_order_place, d_xyz, long(n), (const uint32_t*)q.cell, (const uint32_t*)q.rank,
-                  (const uint32_t*)q.count, q.idx, q.xyz));
+    HIP_OK(launch(st, "order", s, dim3(blocks), dim3(256), k_order_keys, d_xyz, long(n), h->box_lo[0], h->box_lo[1], h->box_lo[2], sc[0], sc[1], sc[2],
+                  q.key, q.idx0));
+    size_t bytes = q.sort_bytes;
+    HIP_OK(rocprim::radix_sort_pairs(q.sort, bytes, q.key, q.key2, q.idx0, q.idx, n, 0, 3 * ORDER_BITS, s));
+    HIP_OK(launch(st, "order", s, dim3(blocks), dim3(256), k_order_place, d_xyz, long(n), (const uint32_t*)q.idx, q.xyz));
     j.xyz = q.xyz;
   }
   Res r{};
@@ -2526,8 +2744,8 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     HIP_OK(launch(&st, "store_band", s, dim3(nband), dim3(128), k_store_band, long(nband), (const uint32_t*)h->list.as<uint32_t>(),
                   (const uint32_t*)h->cls.as<uint32_t>(), (const uint32_t*)h->slot.as<uint32_t>(), (const uint32_t*)h->staging.as<uint32_t>(),
                   h->map.as<uint32_t>(), h->atlas.as<uint32_t>()));
-  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
-  HIP_OK(hipMemcpyAsync(cnt, h->counters.p, 48, hipMemcpyDeviceToHost, s));
+  unsigned long long cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  HIP_OK(hipMemcpyAsync(cnt, h->counters.p, 64, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   collect(h, st);
   if (cnt[2]) return fail(std::to_string(cnt[2]) + " bricks kept whole with no sign found (no surface near them and none outside the part's box)");
@@ -2535,6 +2753,7 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     stats->bricks = nb, stats->band = nband, stats->slots = nslots;
     stats->undecided = int64_t(cnt[0]), stats->refined = int64_t(cnt[1]);
     stats->unsure_trim = int64_t(cnt[3]), stats->unsure_tie = int64_t(cnt[4]), stats->settled = int64_t(cnt[5]);
+    stats->overflowed = int64_t(cnt[6]);
   }
   return 0;
 }
@@ -2571,11 +2790,12 @@ int icam_hip_bake(icam_hip* h, int precision, const icam_level* lv, uint32_t* ma
 int icam_hip_query_stats(icam_hip* h, icam_bake_stats* stats) {
   if (!h || !stats) return fail("bad arguments");
   std::lock_guard<std::mutex> g(h->mu);
-  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
+  unsigned long long cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   HIP_OK(hipDeviceSynchronize());
-  if (h->counters.p) HIP_OK(hipMemcpy(cnt, h->counters.p, 48, hipMemcpyDeviceToHost));
+  if (h->counters.p) HIP_OK(hipMemcpy(cnt, h->counters.p, 64, hipMemcpyDeviceToHost));
   *stats = icam_bake_stats{};
   stats->refined = int64_t(cnt[1]), stats->unsure_trim = int64_t(cnt[3]), stats->unsure_tie = int64_t(cnt[4]), stats->settled = int64_t(cnt[5]);
+  stats->overflowed = int64_t(cnt[6]);
   return 0;
 }
 
@@ -2936,6 +3156,7 @@ int icam_hip_batch_result(icam_batch* b, size_t i, const uint32_t** d_map, const
     stats->bricks = P.nb, stats->band = bt, stats->slots = sl;
     stats->undecided = int64_t(cnt[0]), stats->refined = int64_t(cnt[1]);
     stats->unsure_trim = int64_t(cnt[3]), stats->unsure_tie = int64_t(cnt[4]), stats->settled = int64_t(cnt[5]);
+    stats->overflowed = int64_t(cnt[6]);
   }
   return 0;
 }

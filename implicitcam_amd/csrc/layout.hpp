@@ -75,6 +75,14 @@ struct Layout {
   double eps64;  // on double-precision lengths
   double diag;   // the part's box diagonal
   uint32_t max_rounds;  // a search's best-first rounds over parts, at most
+  // The primitives' hierarchy: an implicit 64-ary tree over the primitives sorted by the
+  // Morton code of their box centres. Level 0 holds the primitives' boxes in that order
+  // (bvh_prim[k] the k-th one's index); node j of level l + 1 bounds nodes 64 j .. 64 j + 63
+  // of level l. Level l's boxes (6 floats, lo then hi) and flags (a boundary point inside)
+  // start at node bvh_off[l] of the hierarchy's arrays; bvh_cnt[l] nodes; level bvh_top
+  // holds at most 64.
+  uint32_t bvh_top;
+  uint32_t bvh_off[8], bvh_cnt[8];
 };
 
 }  // namespace icam

@@ -781,6 +781,67 @@ template <class F> void parallel_for(size_t n, F&& f) {
 }
 }  // namespace
 
+namespace {
+// The primitives' hierarchy (Layout::bvh_*): the primitives sorted by the Morton code of
+// their box centres (10 bits an axis over the primitives' extent), then levels of nodes
+// each bounding 64 of the level below, until a level has at most 64. A node holds a
+// boundary point when one of its primitives does: faces, edges, vertices and edge pieces
+// do; a sub-patch when it lies wholly inside its face's loops; a C0 line's piece never.
+void build_hierarchy(Packed& P) {
+  Layout& L = P.L;
+  const size_t n = P.pcode.size();
+  float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], P.pbox[6 * i + k]), hi[k] = std::max(hi[k], P.pbox[6 * i + 3 + k]);
+  std::vector<std::pair<uint32_t, uint32_t>> key(n);
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t code = 0;
+    for (int k = 0; k < 3; ++k) {
+      const double c = 0.5 * (double(P.pbox[6 * i + k]) + P.pbox[6 * i + 3 + k]), side = double(hi[k]) - lo[k];
+      const double f = side > 0 ? (c - lo[k]) / side * 1024.0 : 0.0;
+      const uint32_t q = f <= 0 ? 0u : f >= 1023.0 ? 1023u : uint32_t(f);
+      for (int b = 0; b < 10; ++b) code |= ((q >> b) & 1u) << (3 * b + k);
+    }
+    key[i] = {code, uint32_t(i)};
+  }
+  std::sort(key.begin(), key.end());
+  P.bvh_prim.resize(n);
+  for (size_t i = 0; i < n; ++i) P.bvh_prim[i] = key[i].second;
+  auto holds = [&](uint32_t code) {
+    const uint32_t kind = code >> 28, id = code & 0x0fffffffu;
+    if (kind == P_LEAF) return P.U[L.leaf_u + LEAF_U * id + 2] != 0u;
+    if (kind == P_CLEAF) return P.U[L.edge_u + EDGE_U * P.U[L.cleaf_u + CLEAF_U * id + 1]] != EDGE_CREASE;
+    return true;
+  };
+  P.bvh_box.clear(), P.bvh_holds.clear();
+  for (size_t i = 0; i < n; ++i) {
+    const uint32_t q = P.bvh_prim[i];
+    P.bvh_box.insert(P.bvh_box.end(), P.pbox.begin() + 6 * q, P.pbox.begin() + 6 * q + 6);
+    P.bvh_holds.push_back(holds(P.pcode[q]) ? 1 : 0);
+  }
+  for (int l = 0; l < 8; ++l) L.bvh_off[l] = 0, L.bvh_cnt[l] = 0;
+  L.bvh_off[0] = 0, L.bvh_cnt[0] = uint32_t(n);
+  int l = 0;
+  while (L.bvh_cnt[l] > 64 && l < 7) {
+    const uint32_t below = L.bvh_off[l], cnt = L.bvh_cnt[l], m = (cnt + 63) / 64;
+    L.bvh_off[l + 1] = uint32_t(P.bvh_holds.size()), L.bvh_cnt[l + 1] = m;
+    for (uint32_t j = 0; j < m; ++j) {
+      float b[6] = {3e38f, 3e38f, 3e38f, -3e38f, -3e38f, -3e38f};
+      uint8_t h = 0;
+      for (uint32_t c = 64 * j; c < std::min(cnt, 64 * j + 64); ++c) {
+        const float* x = &P.bvh_box[6 * size_t(below + c)];
+        for (int k = 0; k < 3; ++k) b[k] = std::min(b[k], x[k]), b[3 + k] = std::max(b[3 + k], x[3 + k]);
+        h |= P.bvh_holds[below + c];
+      }
+      P.bvh_box.insert(P.bvh_box.end(), b, b + 6);
+      P.bvh_holds.push_back(h);
+    }
+    ++l;
+  }
+  L.bvh_top = uint32_t(l);
+}
+}  // namespace
+
 bool pack_part(const icam_part_desc* part, double need, Packed& out) {
   Packed& P = out;
   Layout& L = P.L;
@@ -1383,6 +1444,7 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
   L.eps32 = float(8.0 * 1.1920929e-7 * ext);  // partgpu.rs:139
   L.eps64 = 8.0 * 2.220446049250313e-16 * ext;
   L.max_rounds = 4096;
+  build_hierarchy(P);
   return true;
 }
 

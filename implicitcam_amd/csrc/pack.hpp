@@ -14,6 +14,11 @@ struct Packed {
   std::vector<uint32_t> U; // index records
   std::vector<float> pbox; // 6 per primitive (lo, hi), rounded outward
   std::vector<uint32_t> pcode;
+  // the primitives' hierarchy (Layout::bvh_*): boxes, the sorted primitives, and per node
+  // whether a boundary point lies in it
+  std::vector<float> bvh_box;
+  std::vector<uint32_t> bvh_prim;
+  std::vector<uint8_t> bvh_holds;
   double origin[3] = {0, 0, 0};
   size_t nodes = 0;        // sub-patches and curve pieces made
   std::string error;

@@ -25,7 +25,8 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 
 class BakeStats(ctypes.Structure):
     _fields_ = [("bricks", ctypes.c_int64), ("band", ctypes.c_int64), ("slots", ctypes.c_int64), ("undecided", ctypes.c_int64),
-                ("refined", ctypes.c_int64), ("unsure_trim", ctypes.c_int64), ("unsure_tie", ctypes.c_int64), ("settled", ctypes.c_int64)]
+                ("refined", ctypes.c_int64), ("unsure_trim", ctypes.c_int64), ("unsure_tie", ctypes.c_int64), ("settled", ctypes.c_int64),
+                ("overflowed", ctypes.c_int64)]
 
 
 _lib = None

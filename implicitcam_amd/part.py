@@ -80,6 +80,15 @@ def near_trim_points(part, seed, n, radius):
     return out
 
 
+def near_surface_points(part, seed, n, radius):
+    """n points within `radius` (mm) of the part's surface (a face drawn by area, a point
+    of it inside its loops, an offset uniform in the ball), float64 (n, 3), from the
+    reference checks' xorshift64."""
+    out = np.empty((n, 3), dtype=np.float64)
+    part._lib.icam_gen_near_surface(part.desc, seed, n, radius, out.ctypes.data_as(_native.c_double_p))
+    return out
+
+
 def level_for(part, h, max_bricks=None):
     """The brick level of cell side h over the part, placed as the reference's cache
     places a level (crates/icam-sweep/src/cache.rs:505-521): first node a cell below the

@@ -102,7 +102,7 @@ int icam_hip_query_device(icam_hip* h, int precision, const double* d_xyz, size_
 /* No counterpart in the reference: icam_hip_query_device for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc). It allocates nothing: its scratch (cell order, single-precision
- * records, queues, counters) is carved from one static device array of 24 MiB a GPU,
+ * records, queues, counters) is carved from one static device array of 32 MiB a GPU,
  * held until the process ends, so a call takes at most 2^18 points (more fail). The
  * arguments and results are icam_hip_query_device's; it returns without waiting. Every
  * call on a GPU shares that GPU's array: calls that overlap on one GPU (from different
@@ -121,6 +121,8 @@ typedef struct icam_bake_stats {
   int64_t unsure_tie;  /* of them, for a near-tie of opposite signs */
   int64_t settled;     /* points past a convexity certificate settled by the certified
                           search (left undecided, then bridged, when it does not finish) */
+  int64_t overflowed;  /* search blocks whose candidates did not fit their list (those
+                          scan every primitive each round: slow, not wrong) */
 } icam_bake_stats;
 
 /* Statistics of the last query on this handle (waits for the device): the points

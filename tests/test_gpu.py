@@ -73,13 +73,18 @@ def test_c1_queries_match_oracle(c1, prec):
 @pytest.mark.parametrize("prec", PRECS)
 def test_c1_full_size_closed_form(c1, prec):
     """All 2^20 points of the configuration, against the closed form (size-independent:
-    no oracle run at this size)."""
+    no oracle run at this size). The double-precision path to 1e-8 mm; the mixed path to
+    the metric's 1e-6 of the diagonal (a point on a medial surface of two faces, whose
+    distances tie within single-precision rounding, keeps whichever its search read
+    first and corrects that one in double precision: about 1e-6 mm here)."""
     part, _, g = c1
     pts = uniform_points(SEED, 1 << 20, part.lo - 5, part.hi + 5)
     d, flags, _ = g.query(pts, prec)
     ref = c1_sdf(pts)
     assert (flags == 0).all()
-    assert np.abs(d - ref).max() <= 1e-8
+    assert np.abs(d - ref).max() <= (1e-8 if prec == FP64 else tol(prec, part))
+    if prec == MIXED:
+        assert np.mean(np.abs(d - ref) > 1e-8) < 1e-4
     clear = np.abs(ref) > 1e-8
     assert ((d < 0) == (ref < 0))[clear].all()
 
