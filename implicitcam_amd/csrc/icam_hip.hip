@@ -83,6 +83,10 @@ struct Dev {
   const float* bvh_box;
   const uint32_t* bvh_prim;
   const uint8_t* bvh_holds;
+  // the hierarchy's level table in device memory: bvh_off[0..7] then bvh_cnt[0..7] (a
+  // batch kernel holds Dev in a private copy, where a level indexed at run time would
+  // send the whole copy to scratch memory)
+  const uint32_t* bvh_lv;
   Layout L;
   double origin[3];
 };
@@ -195,6 +199,8 @@ struct Res {
   uint32_t* squeue;   // blocks holding points left undecided, for k_settle
   uint32_t* scount;
   uint32_t* smark;
+  uint32_t* cqueue;   // scattered points' blocks whose candidates did not fit their list,
+  uint32_t* ccount;   // searched in chunks by k_chunks (NULL: searched over the whole list)
   uint32_t part;      // the part's number in a batch (0 alone), in queue entries' top byte
   unsigned long long* counters;  // [0] bridged nodes, [1] points searched again in double,
                                  // [3] of them near a trim curve, [4] near-ties of opposite
@@ -341,8 +347,8 @@ __device__ int bvh_cull(const Dev& dv, const float* gb, float R, bool reach, flo
       uint32_t node = 0;
       float m = 3.0e38f;
       for (int lev = int(dv.L.bvh_top); lev >= 0; --lev) {
-        const uint32_t off = dv.L.bvh_off[lev], c = 64u * node + uint32_t(lane);
-        const bool ok = c < dv.L.bvh_cnt[lev] && dv.bvh_holds[off + c];
+        const uint32_t off = dv.bvh_lv[lev], c = 64u * node + uint32_t(lane);
+        const bool ok = c < dv.bvh_lv[8 + lev] && dv.bvh_holds[off + c];
         const float* bx = dv.bvh_box + 6 * (size_t(off) + c);
         float g = ok ? box_gap(bx, gb) : 3.0e38f, f = ok ? box_far(bx, gb) : 3.0e38f;
         uint32_t pick = ok ? c : 0xffffffffu;
@@ -371,7 +377,7 @@ __device__ int bvh_cull(const Dev& dv, const float* gb, float R, bool reach, flo
   const uint32_t* p2 = nullptr;  // the level-2 nodes that passed, and how many
   uint32_t n2 = 0;
   for (int lev = int(dv.L.bvh_top); lev >= 0; --lev) {
-    const uint32_t off = dv.L.bvh_off[lev], cnt = dv.L.bvh_cnt[lev], nk = npar * 64u;
+    const uint32_t off = dv.bvh_lv[lev], cnt = dv.bvh_lv[8 + lev], nk = npar * 64u;
     if (lev == 1) p2 = par, n2 = npar;
     const float* boxes = dv.bvh_box + 6 * size_t(off);
     if (reach) {
@@ -833,12 +839,13 @@ template <int S> constexpr int rounds() { return S == SRC_NODES ? 3 : 1; }
 
 // The block's candidates: the primitives within reach of its points' box, where the
 // reach covers every point's answer (its bound, or for unbounded points the boundary's
-// upper bound over the block's box, from the hierarchy).
-template <int S>
+// upper bound over the block's box, from the hierarchy). NT threads cull; scattered
+// points set the box from the block's own points (only those flagged `only` when
+// only_flag is given), or from the pcount points listed in plist when it is given.
+template <int S, int NT = Shape<S>::NT>
 __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t* cand, uint32_t* tmp, float* key, float* red, int* wave_tot,
                                 uint32_t* chunks = nullptr, int* nchunk = nullptr, float* gb_out = nullptr, float* r_out = nullptr,
-                                const uint32_t* only_flag = nullptr, uint32_t only = 0) {
-  constexpr int NT = Shape<S>::NT;
+                                const uint32_t* only_flag = nullptr, uint32_t only = 0, const uint32_t* plist = nullptr, int pcount = 0) {
   float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
   float r = 0.f;
   if (!pts<S>()) {
@@ -862,16 +869,23 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
     hi[0] = float(b.x), hi[1] = float(b.y), hi[2] = float(b.z);
     r = float(fmin(bound, 3.0e38)) * (1.0f + 1e-5f) + 1e-4f;
   } else {
-  for (int m = 0; m < rounds<S>(); ++m) {
-    long i = block_point<S>(j, blk, m);
-    // (scattered points searched again: only those flagged `only` set the box)
-    if (i < 0 || (only_flag && only_flag[i] != only)) continue;
+  auto add = [&](long i) {
     double bound;
     v3<double> p = job_point<S>(j, dv, i, bound);
     const float ub = float(fmin(bound, 3.0e38));
     lo[0] = fminf(lo[0], float(p.x)), lo[1] = fminf(lo[1], float(p.y)), lo[2] = fminf(lo[2], float(p.z));
     hi[0] = fmaxf(hi[0], float(p.x)), hi[1] = fmaxf(hi[1], float(p.y)), hi[2] = fmaxf(hi[2], float(p.z));
     r = fmaxf(r, ub * (1.0f + 1e-5f) + 1e-4f);
+  };
+  if (plist) {
+    for (int k = threadIdx.x; k < pcount; k += NT) add(long(plist[k]));
+  } else {
+    for (int m = 0; m < rounds<S>(); ++m) {
+      long i = block_point<S>(j, blk, m);
+      // (scattered points searched again: only those flagged `only` set the box)
+      if (i < 0 || (only_flag && only_flag[i] != only)) continue;
+      add(i);
+    }
   }
   block_box<NT>(lo, hi, r, red);
   }
@@ -1008,8 +1022,10 @@ __device__ Best<T> search_chunks(const Dev& dv, const Tab<T>& tb, v3<T> p, bool 
 
 // Block blk of a job searched, in single (T = float: records for k_refine) or double
 // precision (T = double: final answers). A brick's nodes leave their parts' rounds to
-// the dense pass.
-template <int S, class T>
+// the dense pass. Scattered points whose candidates do not fit the list are searched in
+// chunks: by this call when CH, else their block is queued for k_chunks (the chunked
+// search inlined beside the plain one took the search from two waves a SIMD to one).
+template <int S, class T, bool CH = false>
 __device__ __forceinline__ void search_block(const Dev& dv, const Job& j, const Res& res, long blk, SearchLds<S, T>& sm) {
   constexpr int NT = Shape<S>::NT;
   constexpr int Q = Shape<S>::Q;
@@ -1017,6 +1033,12 @@ __device__ __forceinline__ void search_block(const Dev& dv, const Job& j, const 
   float gb[6], rr = 0.f;
   const int nc = block_candidates<S>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot, pts<S>() ? sm.chunks : nullptr, &nchunk, gb, &rr);
   const bool chunked = pts<S>() && nchunk > 0;
+  if constexpr (pts<S>() && !CH) {
+    if (chunked && res.cqueue) {
+      if (threadIdx.x == 0) res.cqueue[atomicAdd(res.ccount, 1u)] = uint32_t(blk);
+      return;
+    }
+  }
   if (threadIdx.x == 0 && nc > Shape<S>::C && !chunked) atomicAdd(&res.counters[6], 1ull);
   const Tab<T> tb = tab_of<T>(dv);
   const T eps = sizeof(T) == 4 ? T(dv.L.eps32) : T(dv.L.eps64);
@@ -1035,9 +1057,14 @@ __device__ __forceinline__ void search_block(const Dev& dv, const Job& j, const 
       p = node_at(nfirst, nh, int(threadIdx.x) + NT * m), bound = nh * 1.7320508075688772;
     else
       p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
-    Best<T> b = chunked ? search_chunks<S, T, false>(dv, tb, cv3<T>(p), valid, search_bound<T>(bound, eps), sm, nchunk, gb, rr, tm, eps)
-                        : search<T, NT, Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, search_bound<T>(bound, eps), sm.cand, nc, tm, eps, sm.qlb(), sm.qk(),
-                                                           S == SRC_NODES && res.items ? &res : nullptr, i);
+    Best<T> b;
+    if constexpr (CH) {
+      b = chunked ? search_chunks<S, T, false>(dv, tb, cv3<T>(p), valid, search_bound<T>(bound, eps), sm, nchunk, gb, rr, tm, eps)
+                  : search<T, NT, Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, search_bound<T>(bound, eps), sm.cand, nc, tm, eps, sm.qlb(), sm.qk());
+    } else {
+      b = search<T, NT, Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, search_bound<T>(bound, eps), sm.cand, nc, tm, eps, sm.qlb(), sm.qk(),
+                                           S == SRC_NODES && res.items ? &res : nullptr, i);
+    }
     if (!valid || !owner || b.code == DEFERRED) continue;
     finish_point<T>(dv, res, i, bound, b, eps);
   }
@@ -1047,6 +1074,17 @@ template <int S, class T>
 __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res) {
   __shared__ SearchLds<S, T> sm;
   search_block<S, T>(dv, j, res, blockIdx.x, sm);
+}
+// The scattered points' blocks k_search queued (their candidates past the list),
+// searched chunk by chunk.
+template <int S, class T>
+__global__ __launch_bounds__(Shape<S>::NT) void k_chunks(Dev dv, Job j, Res res) {
+  __shared__ SearchLds<S, T> sm;
+  const uint32_t total = *res.ccount;
+  for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+    search_block<S, T, true>(dv, j, res, long(res.cqueue[q]), sm);
+    __syncthreads();
+  }
 }
 
 // One deferred point of the dense pass: its parts in order of their lower bounds
@@ -2089,12 +2127,13 @@ struct icam_hip {
   hipStream_t stream = nullptr;
   Packed pk;
   Dev dv{};
-  Buf F32, F64, U, pbox, pcode, bvh_box, bvh_prim, bvh_holds;
+  Buf F32, F64, U, pbox, pcode, bvh_box, bvh_prim, bvh_holds, bvh_lv;
   // per-point results of the centres' and the nodes' searches, and host-call staging
   struct ResBufs {
-    Buf code, pa, pb, d32, info, val, flag, queue, qcount, qmark, items, icount, squeue, scount, smark;
+    Buf code, pa, pb, d32, info, val, flag, queue, qcount, qmark, items, icount, squeue, scount, smark, cqueue, ccount;
     void free() {
-      for (Buf* b : {&code, &pa, &pb, &d32, &info, &val, &flag, &queue, &qcount, &qmark, &items, &icount, &squeue, &scount, &smark}) b->free();
+      for (Buf* b : {&code, &pa, &pb, &d32, &info, &val, &flag, &queue, &qcount, &qmark, &items, &icount, &squeue, &scount, &smark, &cqueue, &ccount})
+        b->free();
     }
   } centres, nodes;
   Buf q_xyz, q_bound, q_d, q_flag, q_feat;
@@ -2206,6 +2245,13 @@ int results(icam_hip* h, bool nodes, long n, Res& r, double* val, uint32_t* flag
   r.feat = feat;
   r.counters = h->counters.as<unsigned long long>();
   r.squeue = B.squeue.as<uint32_t>(), r.scount = B.scount.as<uint32_t>(), r.smark = B.smark.as<uint32_t>();
+  r.cqueue = nullptr, r.ccount = nullptr;
+  if (!nodes) {
+    // (points and brick centres share these buffers; only points are searched in chunks)
+    HIP_OK(B.cqueue.need(m * 4));
+    HIP_OK(B.ccount.need(16));
+    r.cqueue = B.cqueue.as<uint32_t>(), r.ccount = B.ccount.as<uint32_t>();
+  }
   r.items = nullptr, r.icount = nullptr, r.icap = 0;
   if (items) {
     // room for an eighth of the points; points past it run their rounds in place
@@ -2223,6 +2269,8 @@ template <int S>
 int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Stages* st, long blocks) {
   if (j.n <= 0) return 0;
   constexpr int NT = Shape<S>::NT;
+  static const char* chunk_names[2] = {"points_chunks32", "points_chunks64"};
+  const unsigned cb = unsigned(std::min<long>(blocks, 2048));
   static const char* names[4][8] = {{"points_search32", "points_refine64", "points_fallback64", "points_search64", "points_leaves32", "points_leaves64",
                                      "points_mark", "points_settle64"},
                                     {"nodes_search32", "nodes_refine64", "nodes_fallback64", "nodes_search64", "nodes_leaves32", "nodes_leaves64",
@@ -2235,12 +2283,14 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
   {
     // the queues' counts and marks, cleared by one fill launch (no hipMemsetAsync: a
     // caller may capture these launches in a graph)
-    FillList f{{r.items ? r.icount : nullptr, mixed ? r.qcount : nullptr, mixed ? r.qmark : nullptr, r.scount, r.smark, nullptr},
-               {r.items ? 1u : 0u, mixed ? 1u : 0u, mixed ? size_t(blocks) : 0u, 2u, size_t(blocks), 0u}};
+    FillList f{{r.items ? r.icount : nullptr, mixed ? r.qcount : nullptr, mixed ? r.qmark : nullptr, r.scount, r.smark, r.ccount},
+               {r.items ? 1u : 0u, mixed ? 1u : 0u, mixed ? size_t(blocks) : 0u, 2u, size_t(blocks), r.ccount ? 1u : 0u}};
     HIP_OK(launch(st, "zero", s, dim3(unsigned(std::min<long>((blocks + 255) / 256, 4096))), dim3(256), kb_fill, f));
   }
   if (mixed) {
     HIP_OK(launch(st, names[S][0], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, float>, h->dv, j, r));
+    if constexpr (pts<S>())
+      if (r.cqueue) HIP_OK(launch(st, chunk_names[0], s, dim3(cb), dim3(NT), k_chunks<S, float>, h->dv, j, r));
     if (lb) HIP_OK(launch(st, names[S][4], s, dim3(lb), dim3(256), k_leaves<S, float>, h->dv, j, r));
     if constexpr (S == SRC_NODES)
       HIP_OK(launch(st, names[S][1], s, dim3(unsigned(std::min<long>(blocks, 8192))), dim3(256), k_refine_bricks, h->dv, j, r, blocks));
@@ -2250,6 +2300,8 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
     HIP_OK(launch(st, names[S][2], s, dim3(fb), dim3(NT), k_fallback<S>, h->dv, j, r));
   } else {
     HIP_OK(launch(st, names[S][3], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, double>, h->dv, j, r));
+    if constexpr (pts<S>())
+      if (r.cqueue) HIP_OK(launch(st, chunk_names[1], s, dim3(cb), dim3(NT), k_chunks<S, double>, h->dv, j, r));
     if (lb) HIP_OK(launch(st, names[S][5], s, dim3(lb), dim3(256), k_leaves<S, double>, h->dv, j, r));
   }
   // what is still undecided: settled by the certified search (not the brick centres:
@@ -2337,7 +2389,9 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
     return fail(e);
   }
   std::vector<float> f32 = to_f32(h->pk.F);
-  if (upload(h->F32, f32.data(), f32.size() * 4) || upload(h->F64, h->pk.F.data(), h->pk.F.size() * 8) ||
+  uint32_t lvtab[16];
+  for (int l = 0; l < 8; ++l) lvtab[l] = h->pk.L.bvh_off[l], lvtab[8 + l] = h->pk.L.bvh_cnt[l];
+  if (upload(h->bvh_lv, lvtab, sizeof(lvtab)) || upload(h->F32, f32.data(), f32.size() * 4) || upload(h->F64, h->pk.F.data(), h->pk.F.size() * 8) ||
       upload(h->U, h->pk.U.data(), h->pk.U.size() * 4) || upload(h->pbox, h->pk.pbox.data(), h->pk.pbox.size() * 4) ||
       upload(h->pcode, h->pk.pcode.data(), h->pk.pcode.size() * 4) || upload(h->bvh_box, h->pk.bvh_box.data(), h->pk.bvh_box.size() * 4) ||
       upload(h->bvh_prim, h->pk.bvh_prim.data(), h->pk.bvh_prim.size() * 4) || upload(h->bvh_holds, h->pk.bvh_holds.data(), h->pk.bvh_holds.size())) {
@@ -2358,6 +2412,7 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
   h->dv.pbox = h->pbox.as<float>();
   h->dv.pcode = h->pcode.as<uint32_t>();
   h->dv.bvh_box = h->bvh_box.as<float>(), h->dv.bvh_prim = h->bvh_prim.as<uint32_t>(), h->dv.bvh_holds = h->bvh_holds.as<uint8_t>();
+  h->dv.bvh_lv = h->bvh_lv.as<uint32_t>();
   h->dv.L = h->pk.L;
   for (int k = 0; k < 3; ++k) h->dv.origin[k] = h->pk.origin[k];
   for (int k = 0; k < 3; ++k) h->box_lo[k] = 1e300, h->box_hi[k] = -1e300;
@@ -2376,7 +2431,7 @@ void icam_hip_close(icam_hip* h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   h->centres.free();
   h->nodes.free();
-  for (Buf* b : {&h->F32, &h->F64, &h->U, &h->pbox, &h->pcode, &h->bvh_box, &h->bvh_prim, &h->bvh_holds, &h->q_xyz, &h->q_bound, &h->q_d, &h->q_flag, &h->q_feat, &h->dc, &h->band, &h->sgn, &h->total,
+  for (Buf* b : {&h->F32, &h->F64, &h->U, &h->pbox, &h->pcode, &h->bvh_box, &h->bvh_prim, &h->bvh_holds, &h->bvh_lv, &h->q_xyz, &h->q_bound, &h->q_d, &h->q_flag, &h->q_feat, &h->dc, &h->band, &h->sgn, &h->total,
                  &h->list, &h->staging, &h->cls, &h->slot, &h->stotal, &h->map, &h->atlas, &h->counters, &h->changed, &h->firsts,
                  &h->tile_cand, &h->tile_count, &h->super_cand, &h->super_count, &h->o_count, &h->o_cell, &h->o_rank, &h->o_idx0, &h->o_idx, &h->o_xyz, &h->o_d, &h->o_flag, &h->o_feat})
     b->free();
@@ -2421,7 +2476,7 @@ struct QBufs {
   uint32_t *flag, *feat;
   uint32_t *code, *info;
   float *pa, *pb, *d32;
-  uint32_t *queue, *qcount, *qmark, *squeue, *scount, *smark;
+  uint32_t *queue, *qcount, *qmark, *squeue, *scount, *smark, *cqueue, *ccount;
   unsigned long long* counters;
 };
 static bool query_order(size_t n, bool bounds) { return n >= ORDER_MIN && !bounds && n < (size_t(1) << 31); }
@@ -2460,8 +2515,11 @@ static int query_bufs(icam_hip* h, size_t n, bool mixed, bool order, bool feat, 
   HIP_OK(B.squeue.need(m * 4));
   HIP_OK(B.smark.need(m * 4));
   HIP_OK(B.scount.need(16));
+  HIP_OK(B.cqueue.need(m * 4));
+  HIP_OK(B.ccount.need(16));
   HIP_OK(h->counters.need(64));
   q.qcount = B.qcount.as<uint32_t>(), q.squeue = B.squeue.as<uint32_t>(), q.smark = B.smark.as<uint32_t>(), q.scount = B.scount.as<uint32_t>();
+  q.cqueue = B.cqueue.as<uint32_t>(), q.ccount = B.ccount.as<uint32_t>();
   q.counters = h->counters.as<unsigned long long>();
   return 0;
 }
@@ -2489,6 +2547,7 @@ static size_t query_carve(char* base, size_t cap, size_t n, bool mixed, bool ord
     q.queue = (uint32_t*)take(blk * 4), q.qmark = (uint32_t*)take(blk * 4);
   }
   q.qcount = (uint32_t*)take(16), q.squeue = (uint32_t*)take(blk * 4), q.smark = (uint32_t*)take(blk * 4), q.scount = (uint32_t*)take(16);
+  q.cqueue = (uint32_t*)take(blk * 4), q.ccount = (uint32_t*)take(16);
   q.counters = (unsigned long long*)take(64);
   return at;
 }
@@ -2522,6 +2581,7 @@ static int query_impl(icam_hip* h, int precision, const double* d_xyz, const dou
   Res r{};
   r.code = q.code, r.pa = q.pa, r.pb = q.pb, r.d32 = q.d32, r.info = q.info;
   r.queue = q.queue, r.qcount = q.qcount, r.qmark = q.qmark, r.squeue = q.squeue, r.scount = q.scount, r.smark = q.smark;
+  r.cqueue = q.cqueue, r.ccount = q.ccount;
   r.counters = q.counters;
   r.val = order ? q.d : d_d, r.flag = order ? q.flag : d_flags, r.feat = order ? (d_feat ? q.feat : nullptr) : d_feat;
   // a part with B-spline faces or spline edges: four lanes a point (on the trimmed plate's

@@ -1183,7 +1183,7 @@ __device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb
   int nq = 1, pops = 0;
   q[0] = SettleItem{lf[1], lf[2], lf[3], lf[4], lb, 0};
   double bd = b.d;
-  bool complete = true;
+  bool complete = true, got = false;
   while (nq > 0) {
     int k = 0;
     for (int i = 1; i < nq; ++i)
@@ -1204,16 +1204,21 @@ __device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb
     const double u = sp[64] + s * (sp[65] - sp[64]), v = sp[66] + t * (sp[67] - sp[66]);
     bool near = false;
     auto inside = [&](double uu, double vv) { return known_in || fu[2] != 0 || trim_inside(tb, f + 17, fu[1], uu, vv, 0.0, near); };
-    if (d < bd && inside(u, v)) {
-      bd = d;
+    // (a critical point within SETTLE_EPS of a bound no answer of this part set yet, and
+    // a polished one within SETTLE_EPS of its start, are taken: a foot on the shared
+    // border of two quarters stops the first quarter's Newton at the border, not
+    // critical, which sets the bound that the next quarter's critical point only ties)
+    if (d < bd + (got ? 0.0 : SETTLE_EPS) && inside(u, v)) {
+      bd = fmin(bd, d);
       const bool at_pole = lf[LEAF_POLE + 3] != 0.0 && len(e.p - ld3(lf + LEAF_POLE)) <= 1e-9 * (1.0 + len(ld3(lf + LEAF_POLE)));
       if (critical64(p, e.p, r, e.ps) && critical64(p, e.p, r, e.pt)) {
-        if (len(nrm) > 0.0 && !at_pole) offer(b, d, (P_LEAF << 28) | li, s, t, sgn_of(dot(r, nrm) * -f[14]));
+        if (len(nrm) > 0.0 && !at_pole) offer(b, d, (P_LEAF << 28) | li, s, t, sgn_of(dot(r, nrm) * -f[14])), got = true;
       } else {
         double hd, hu, hv, hs;
-        if (polish64(tb, fi, p, u, v, hd, hu, hv, hs) && hd <= d && inside(hu, hv)) {
+        if (polish64(tb, fi, p, u, v, hd, hu, hv, hs) && hd <= d + SETTLE_EPS && inside(hu, hv)) {
           bd = fmin(bd, hd);
           offer(b, hd, (P_LEAF << 28) | li, s, t, hs * f[14]);
+          got = true;
         }
       }
     }

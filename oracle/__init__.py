@@ -32,6 +32,8 @@ def lib():
         L.orc_brick.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.POINTER(ctypes.c_int16)]
         L.orc_decide.restype = ctypes.c_long
         L.orc_decide.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_size_t, ctypes.c_int]
+        L.orc_counts.restype = ctypes.c_int
+        L.orc_counts.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.c_int]
         L.orc_face_inside.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_double, ctypes.c_double]
         L.orc_face_uv.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _dp, _dp]
         L.orc_face_foot.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _dp, _dp]
@@ -40,6 +42,21 @@ def lib():
         L.orc_edge_closest.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _dp, _dp, _dp]
         _lib = L
     return _lib
+
+
+# The primitives the oracle counts (oracle.h ORC_COUNT_*), in order.
+COUNT_NAMES = ("patch_eval", "surface_newton_step", "curve_eval", "curve_newton_step", "foot_plane", "foot_cylinder", "foot_cone",
+               "foot_sphere", "foot_torus", "uv_plane", "uv_cylinder", "uv_cone", "uv_sphere", "uv_torus", "trim_crossing_step",
+               "edge_line", "edge_arc", "vertex")
+
+
+def counts(reset=True):
+    """The primitives every oracle call executed since the last reset (summed over its
+    threads), as {name: count}; reset them when `reset`."""
+    out = (ctypes.c_uint64 * len(COUNT_NAMES))()
+    n = lib().orc_counts(out, len(COUNT_NAMES), 1 if reset else 0)
+    assert n == len(COUNT_NAMES), "oracle counts out of step with COUNT_NAMES"
+    return dict(zip(COUNT_NAMES, (int(v) for v in out)))
 
 
 def _p(a):

@@ -12,7 +12,14 @@
 //    to a cell's reference (the tree is an acceleration of the same parity);
 //  - a B-spline face's search starts from its span patches instead of the
 //    quadtree over spans (same patches, same bounds);
-//  - patches are limited to degree 3 (the parts here are at most bicubic).
+//  - patches are limited to degree 3 (the parts here are at most bicubic);
+//  - closest_trimmed takes a critical point within DIST_EPS of a bound that no answer
+//    has set, and a polished point within DIST_EPS of the point it started from
+//    (surface3.rs:1276-1283 takes them only when strictly nearer): where the foot lies
+//    on the shared border of two spans, the first span's Newton stops at the border,
+//    not critical, and sets the bound, and the next span's critical point only ties it
+//    in rounding, so the reference's search loses the face's answer there (a pocket
+//    wall's point 0.165 mm nearer than the answer it gives).
 #include "oracle.h"
 
 #include <algorithm>
@@ -31,6 +38,31 @@ namespace orc {
 const double INF = std::numeric_limits<double>::infinity();
 const double PI = 3.14159265358979323846;
 const double TAU = 2.0 * PI;
+
+// ------------------------------------------------------------------ work counts
+// The primitives a run executes, counted per thread and added to the totals when a
+// worker ends (oracle.h: orc_counts). Order as ORC_COUNT_* in oracle.h.
+enum Count : int {
+  C_PEVAL = 0,    // rational tensor Bezier point with first and second derivatives
+  C_NEWTON2,      // a surface Newton step, its evaluation excluded
+  C_CEVAL,        // rational Bezier curve point with first and second derivatives
+  C_NEWTON1,      // a curve Newton step, its evaluation excluded
+  C_FOOT,         // analytic feet: plane, cylinder, cone, sphere, torus (5)
+  C_UV = C_FOOT + 5,  // analytic parameters, same kinds (5)
+  C_CROSS = C_UV + 5, // a trim crossing's Newton iteration
+  C_EDGE_LINE,    // a line edge's foot
+  C_EDGE_ARC,     // an arc edge's foot
+  C_VERT,         // a vertex (or pole, tip) distance
+  NCOUNT
+};
+static_assert(NCOUNT == ORC_COUNTS && C_CROSS == ORC_COUNT_CROSS && C_VERT == ORC_COUNT_VERT, "counts as oracle.h numbers them");
+thread_local uint64_t t_count[NCOUNT];
+std::atomic<uint64_t> g_count[NCOUNT];
+inline void count(int k) { ++t_count[k]; }
+void flush_counts() {
+  for (int k = 0; k < NCOUNT; ++k)
+    if (t_count[k]) g_count[k].fetch_add(t_count[k], std::memory_order_relaxed), t_count[k] = 0;
+}
 
 // ------------------------------------------------------------------ vectors
 // crates/icam-core/src/curve3.rs:14-54
@@ -101,6 +133,7 @@ struct Analytic {
   int narc = 4;
 
   V3 foot(V3 p) const {
+    if (kind >= ICAM_PLANE && kind <= ICAM_TORUS) count(C_FOOT + kind);
     switch (kind) {
       case ICAM_PLANE: return p - f.z * dot(p - f.o, f.z);
       case ICAM_CYLINDER: {
@@ -169,6 +202,7 @@ struct Analytic {
 
   // trim2.rs:1907-1934 Analytic::uv
   void uv(V3 q, double& u, double& v) const {
+    if (kind >= ICAM_PLANE && kind <= ICAM_TORUS) count(C_UV + kind);
     switch (kind) {
       case ICAM_PLANE: {
         V3 l = f.local(q);
@@ -252,6 +286,7 @@ bool crossing(const double a[4], double x, double& out) {
   double t = std::min(std::max((x - p0) / (p1 - p0), 0.0), 1.0);
   const double d3[3] = {a[1] - a[0], a[2] - a[1], a[3] - a[2]};
   for (int it = 0; it < 60; ++it) {
+    count(C_CROSS);
     double ft = f(t);
     if (ft == 0.0) {
       out = t;
@@ -448,6 +483,7 @@ struct Patch {
   }
   // surface3.rs:322-351 eval_small
   SEval eval(double s, double t) const {
+    count(C_PEVAL);
     double bu[MAXP], bu1[MAXP], bu2[MAXP], bv[MAXP], bv1[MAXP], bv2[MAXP];
     bernstein(m, s, bu, bu1, bu2);
     bernstein(n, t, bv, bv1, bv2);
@@ -517,6 +553,7 @@ struct Patch {
     bool hs = false, ht = false;
     for (int it = 0; it < iters; ++it) {
       SEval e = eval(s, t);
+      count(C_NEWTON2);
       V3 r = e.p - p;
       double g0 = dot(e.ps, r), g1 = dot(e.pt, r);
       double jss = dot(e.ps, e.ps), jst = dot(e.ps, e.pt), jtt = dot(e.pt, e.pt);
@@ -899,13 +936,17 @@ struct Nurbs {
       double d = len(e.p - p);
       V3 n = unit(cross(e.ps, e.pt));
       double u = pa.u0 + s * (pa.u1 - pa.u0), v = pa.v0 + t * (pa.v1 - pa.v0);
-      if (d < best_d && (now_in || !trim || trim->inside(u, v))) {
-        best_d = d;
+      // (a critical point within DIST_EPS of a bound no answer set yet, and a polished
+      // one within DIST_EPS of its start, are taken: where the foot lies on the shared
+      // border of two spans, the first span's Newton stops at the border, not critical,
+      // and sets the bound; the next span's critical point then ties it in rounding)
+      if (d < best_d + (found ? 0.0 : DIST_EPS) && (now_in || !trim || trim->inside(u, v))) {
+        best_d = std::min(best_d, d);
         if (critical(p, e)) {
           answer({e.p, d, u, v, n});
         } else {
           SurfHit h;
-          if (polish(p, u, v, h) && h.dist <= d && (!trim || trim->inside(h.u, h.v))) {
+          if (polish(p, u, v, h) && h.dist <= d + DIST_EPS && (!trim || trim->inside(h.u, h.v))) {
             best_d = std::min(best_d, h.dist);
             answer(h);
           }
@@ -988,6 +1029,7 @@ struct Bez3 {
   };
   // curve3.rs:129-156
   E eval(double s) const {
+    count(C_CEVAL);
     int n = deg;
     double b[MAXP][4];
     std::memcpy(b, pw, sizeof(b));
@@ -1048,6 +1090,7 @@ struct Bez3 {
   double newton(V3 p, double s) const {
     for (int it = 0; it < 40; ++it) {
       E e = eval(s);
+      count(C_NEWTON1);
       V3 r = e.p - p;
       double f = dot(e.d1, r);
       double fp = dot(e.d2, r) + dot(e.d1, e.d1);
@@ -1204,6 +1247,7 @@ struct Edge {
       out = {f.world(e(bt)), bd, bt, near};
       return out.dist < cutoff;
     }
+    count(kind == ICAM_LINE ? C_EDGE_LINE : C_EDGE_ARC);
     if (kind == ICAM_LINE) {
       V3 ab = b - a;
       double l2 = dot(ab, ab);
@@ -1774,6 +1818,7 @@ struct Exact {
         }
       } else if (id < nf + ne + int(verts.size())) {
         int vi = id - nf - ne;
+        count(C_VERT);
         double d = len(p - verts[vi].first);
         if (d < cur) best = {d * sgn(dot(p - verts[vi].first, verts[vi].second)), verts[vi].first, (2u << 28) | uint32_t(vi), false, false};
       } else if (id < nf + ne + int(verts.size() + lines.size())) {
@@ -1785,6 +1830,7 @@ struct Exact {
       } else {
         // solid3.rs:1794-1799 a pole: its face's answer
         const Pole& pl = poles[id - nf - ne - int(verts.size() + lines.size())];
+        count(C_VERT);
         double d = len(p - pl.x);
         if (d < cur) best = {d * sgn(dot(p - pl.x, pl.n)), pl.x, uint32_t(pl.face), false, false};
       }
@@ -1857,6 +1903,7 @@ void parallel_for(long n, int threads, F&& f) {
   if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
   if (threads == 1 || n < 2) {
     for (long i = 0; i < n; ++i) f(i);
+    flush_counts();
     return;
   }
   std::atomic<long> next{0};
@@ -1868,6 +1915,7 @@ void parallel_for(long n, int threads, F&& f) {
         if (i >= n) break;
         f(i);
       }
+      flush_counts();
     });
   for (auto& t : pool) t.join();
 }
@@ -1997,6 +2045,14 @@ long orc_bake(orc_exact* o, const icam_level* lv, uint32_t* map_out, uint32_t* a
   }
   std::copy(map.begin(), map.end(), map_out);
   return fits ? slots : -1;
+}
+
+int orc_counts(uint64_t* out, int cap, int reset) {
+  for (int k = 0; k < NCOUNT; ++k) {
+    const uint64_t v = reset ? g_count[k].exchange(0) : g_count[k].load();
+    if (k < cap) out[k] = v;
+  }
+  return NCOUNT;
 }
 
 int orc_face_inside(orc_exact* o, uint32_t face, double u, double v) { return o->ex.faces[face].trim.inside(u, v) ? 1 : 0; }

@@ -51,6 +51,25 @@ int orc_brick(orc_exact*, const double first_node[3], double h, int16_t* q);
  * their indices (map order) to out (capacity cap) and returns how many. */
 long orc_decide(orc_exact*, const icam_level* lv, int64_t* out, size_t cap, int threads);
 
+/* The primitives executed by every call since the last reset, summed over threads
+ * (out[k], k < cap), optionally reset; returns how many counts there are (ORC_COUNTS).
+ * The useful work of a run is these counts times the device code's operation count
+ * per primitive. */
+enum {
+  ORC_COUNT_PEVAL = 0,   /* rational tensor Bezier point, 1st and 2nd derivatives */
+  ORC_COUNT_NEWTON2 = 1, /* surface Newton step (its evaluation excluded) */
+  ORC_COUNT_CEVAL = 2,   /* rational Bezier curve point, 1st and 2nd derivatives */
+  ORC_COUNT_NEWTON1 = 3, /* curve Newton step (its evaluation excluded) */
+  ORC_COUNT_FOOT = 4,    /* analytic foot: plane, cylinder, cone, sphere, torus (4..8) */
+  ORC_COUNT_UV = 9,      /* analytic parameters, same kinds (9..13) */
+  ORC_COUNT_CROSS = 14,  /* trim crossing Newton iteration */
+  ORC_COUNT_EDGE_LINE = 15,
+  ORC_COUNT_EDGE_ARC = 16,
+  ORC_COUNT_VERT = 17,   /* vertex, pole or tip distance */
+  ORC_COUNTS = 18
+};
+int orc_counts(uint64_t* out, int cap, int reset);
+
 /* Pieces of the evaluator, for tests. */
 int orc_face_inside(orc_exact*, uint32_t face, double u, double v);
 void orc_face_uv(orc_exact*, uint32_t face, const double q[3], double uv[2]);
