@@ -1,18 +1,21 @@
-"""Benchmark: the exact signed distance's narrow-band brick bake on one MI355X.
+"""Benchmark: the exact signed distance and its narrow-band brick bake on one MI355X.
 
-Workload (a "step"): the freeform single solid -- a 120 x 120 mm block whose top is
-a rational bicubic NURBS with a 16 x 16 control net (seeded value-noise heights,
-5 mm, weights in [0.8, 1.2]) -- baked on a 256^3 node grid at 0.5 mm into 8^3-cell
-bricks (32^3 bricks) within the narrow band: the decision of every brick, the signs
-of the bricks kept whole, the 729 nodes of every band brick, flood fill, i16
-quantization and packing into the 365-word atlas, all on the device, in the
-FP32 + FP64-refine form, run by the batched evaluator (the bake's launches captured
-in one hipGraph, persistent queue stages with autotuned grids; here one part). The
-plain FP64 form is timed beside it, so is the single-level entry point in both forms,
-and so are 2^20 scattered SDF queries on the analytic box-with-hole part and the
+Headline workload (a "step"): the large mixed solid -- a 400 x 300 x 40 mm plate with
+400 pockets and 264 countersunk, filleted holes, 1,998 faces of which 800 are rational
+bicubic B-splines with 32 x 32 nets (weights in [0.5, 2]), every face trimmed by loops
+of 12-64 cubic spans -- baked on its 1024-node grid (1024 nodes along its longest side,
+cell 0.393 mm: 128 x 96 x 13 bricks of 8^3 cells) within the narrow band: the decision
+of every brick, the signs of the bricks kept whole, the 729 nodes of every band brick,
+the settle of nodes past their certificates, flood fill, i16 quantization and packing
+into the 365-word atlas, all on the device, in the FP32 + FP64-refine form through the
+single-level entry point (device-resident results). The plain FP64 form is timed
+beside it, and so are the same part's scattered queries (30 % uniform, 70 % within 1 mm
+of the surface), the NURBS-top block's 256^3 level through the batched evaluator, 2^20
+queries on the analytic box-with-hole part, the trimmed plate's 2^24 queries and the
 64-part batched workload.
 
-Prints one JSON line (rank 0, N = 1). `value` is band bricks baked per second.
+Prints one JSON line (rank 0, N = 1). `value` is the headline's band bricks baked per
+second (mixed form).
 """
 import argparse
 import ctypes
@@ -33,6 +36,132 @@ PEAK_HBM = 8.0e12  # B/s
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def c5_window(part, side=24):
+    """A side x side x full-height brick window of the large solid's 1024-node level
+    (over pockets, holes and the plate's +x side), on the same node grid: the bounded
+    sample the CPU baseline and the parity check bake."""
+    from implicitcam_amd import c4_level
+
+    full = c4_level(part, 1024)
+    lv = c4_level(part, 1024)
+    x0, y0 = 100, 70
+    lv.lo[0] = full.lo[0] + 8 * full.h * x0
+    lv.lo[1] = full.lo[1] + 8 * full.h * y0
+    lv.nb[:] = [min(side, full.nb[0] - x0), min(side, full.nb[1] - y0), full.nb[2]]
+    return lv
+
+
+def large_solid(args, log, threads):
+    """The headline: the large mixed solid's 1024-node level baked in both precisions
+    (device-resident), its scattered queries in both precisions, and the CPU baseline,
+    the parity check and the useful-work count on a window of the same level."""
+    import torch
+
+    from implicitcam_amd import FP64, MIXED, Part, PartGpu, c4_level, near_surface_points, uniform_points
+    from implicitcam_amd.flops import useful_flops
+
+    part = Part("c5")
+    lv = c4_level(part, 1024)
+    t0 = time.time()
+    g = PartGpu(part, 12 * lv.h)
+    pack_s = time.time() - t0
+    log(f"c5: {part.n_faces} faces packed in {pack_s:.1f}s ({g.nodes} sub-patches and curve parts), level {list(lv.nb)} at {lv.h:.4f} mm")
+    res = {"faces": part.n_faces, "pack_s": pack_s, "level": list(lv.nb), "h": lv.h}
+    for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+        _, _, st = g.bake(lv, prec)
+        for _ in range(args.warmup):
+            g.bake_device(lv, prec)
+        torch.cuda.synchronize()
+        stage_sum = {}
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            g.bake_device(lv, prec)
+            for k, v in g.stage_times().items():
+                stage_sum[k] = stage_sum.get(k, 0.0) + v
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        res[name] = {"ms": dt * 1e3, "bricks": st.bricks, "band": st.band, "slots": st.slots, "settled": st.settled, "refined": st.refined,
+                     "undecided": st.undecided, "bricks_per_s": st.band / dt, "stages": {k: v / args.steps for k, v in stage_sum.items()}}
+        log(f"c5 bake {name}: {dt * 1e3:.2f} ms, band {st.band}, stored {st.slots}, settled {st.settled}: {st.band / dt / 1e6:.3f} M band bricks/s; "
+            f"stages {res[name]['stages']}")
+    res["mixed_over_fp64"] = res["fp64"]["ms"] / res["mixed"]["ms"]
+    if args.profile:
+        return res, None, None, None
+    # scattered queries of the workload's mix, inputs resident in HBM
+    n = 1 << args.c5_log2
+    k = n * 3 // 10
+    pts = np.concatenate([uniform_points(0x2545F4914F6CDD1D, k, part.lo - 5, part.hi + 5), near_surface_points(part, 0x9E3779B97F4A7C15, n - k, 1.0)])
+    dev = torch.device("cuda:0")
+    d_pts = torch.from_numpy(pts).to(dev)
+    qout = {}
+    res["queries"] = {"points": n, "near_surface": n - k}
+    for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+        d_out = torch.empty(n, dtype=torch.float64, device=dev)
+        d_flag = torch.empty(n, dtype=torch.int32, device=dev)
+        g.query_device(d_pts.data_ptr(), n, d_out.data_ptr(), d_flag.data_ptr(), None, prec, None)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        g.query_device(d_pts.data_ptr(), n, d_out.data_ptr(), d_flag.data_ptr(), None, prec, None)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        st = g.query_stats()
+        qout[name] = (d_out.cpu().numpy(), d_flag.cpu().numpy())
+        res["queries"][name] = {"queries_per_s": n / dt, "ms": dt * 1e3, "settled": st.settled, "undecided": int((qout[name][1] != 0).sum())}
+        log(f"c5 queries {name}: {n / dt / 1e6:.2f} M/s, {res['queries'][name]}")
+    dm, dd = qout["mixed"][0], qout["fp64"][0]
+    res["queries"]["mixed_vs_fp64"] = {"sign_mismatch": int(((dm < 0) != (dd < 0)).sum()), "max_rel_err": float(np.abs(dm - dd).max() / part.diagonal)}
+
+    cpu = check = useful = None
+    if not (args.no_cpu and args.no_check):
+        from oracle import Oracle, counts
+
+        o = Oracle(part)
+        win = c5_window(part)
+        # the decision's work and the band nodes' work, counted apart
+        counts()
+        o.decide(win, threads=threads)
+        c_dec = counts()
+        t = time.perf_counter()
+        m_ref, a_ref, nread = o.bake(win, threads=threads)
+        cpu_s = time.perf_counter() - t
+        c_all = counts()
+        c_nodes = {key: c_all[key] - c_dec[key] for key in c_all}
+        nb_win = win.nb[0] * win.nb[1] * win.nb[2]
+        cpu = {"value": nread / cpu_s, "unit": "bricks/s", "cores": threads, "kind": "port",
+               "sample": f"a {win.nb[0]} x {win.nb[1]} x {win.nb[2]}-brick window of the same level ({nread} band bricks read), "
+                         f"FP64 oracle on {threads} host threads, {cpu_s:.1f} s"}
+        # one thread on the window's first band bricks
+        small = c5_window(part, 6)
+        t = time.perf_counter()
+        _, _, nread1 = o.bake(small, threads=1)
+        cpu1_s = time.perf_counter() - t
+        cpu["single_thread"] = {"value": nread1 / cpu1_s, "unit": "bricks/s", "sample": f"a 6 x 6 x {win.nb[2]}-brick window, {nread1} band bricks, {cpu1_s:.1f} s"}
+        log(f"cpu oracle: {nread / cpu_s:.1f} bricks/s on {threads} threads, {nread1 / cpu1_s:.1f} on one")
+        # useful work a band brick (its nodes) and a brick (its decision), from the window
+        useful = {"node_flop_per_band_brick": useful_flops(c_nodes) / max(nread, 1), "decide_flop_per_brick": useful_flops(c_dec) / nb_win,
+                  "counts_window_nodes": c_nodes, "counts_window_decide": c_dec}
+        if not args.no_check:
+            check = {}
+            for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+                m, a, _ = g.bake(win, prec)
+                qa = a.view(np.int16).astype(np.int64)
+                qr = a_ref.view(np.int16).astype(np.int64)
+                same = len(qa) == len(qr)
+                check["window_" + name] = {"map_equal": bool((m == m_ref).all()), "atlas_equal": bool(same and (qa == qr).all()),
+                                           "max_dq": int(np.abs(qa - qr).max(initial=0)) if same else None,
+                                           "nodes_off": int((qa != qr).sum()) if same else None,
+                                           "sign_mismatch": int(((qa > 0) != (qr > 0)).sum()) if same else None}
+            idx = np.concatenate([np.arange(0, k, max(k >> 11, 1)), np.arange(k, n, max((n - k) >> 12, 1))])
+            dref, _, _ = o.sdf(pts[idx], threads=threads)
+            for name in ("mixed", "fp64"):
+                err = np.abs(qout[name][0][idx] - dref) / part.diagonal
+                check["queries_" + name] = {"n": int(len(idx)), "max_rel_err": float(err.max()), "p99_rel_err": float(np.percentile(err, 99)),
+                                            "p50_rel_err": float(np.percentile(err, 50)),
+                                            "sign_mismatch": int(((qout[name][0][idx] < 0) != (dref < 0)).sum())}
+            log(f"c5 check: {check}")
+    return res, cpu, check, useful
 
 
 def pmc_traffic(path, kernel):
@@ -177,10 +306,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the parity check against the oracle")
-    ap.add_argument("--profile", action="store_true", help="a short run for a profiler: bake steps only")
+    ap.add_argument("--profile", action="store_true", help="a short run for a profiler: the headline's bake steps and a copy of known size")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-part batched workload")
-    ap.add_argument("--c3", action="store_true", help="add scattered queries on the trimmed plate (half at its trim curves)")
-    ap.add_argument("--c3-log2", type=int, default=20, help="log2 of the trimmed plate's query count (24: the configuration's 2^24)")
+    ap.add_argument("--no-c3", action="store_true", help="skip the trimmed plate's scattered queries")
+    ap.add_argument("--c3-log2", type=int, default=24, help="log2 of the trimmed plate's query count (the configuration's 2^24)")
+    ap.add_argument("--c5-log2", type=int, default=20, help="log2 of the large solid's query count (the configuration's 2^26)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the path shards by brick ranges across GPUs (see the design notes); "
@@ -195,6 +325,19 @@ def main():
         raise SystemExit("no GPU: the benchmark runs the HIP kernels on an MI355X")
     torch.cuda.init()
     dev = torch.device("cuda:0")
+
+    threads = len(os.sched_getaffinity(0))
+    # the headline: the large mixed solid's 1024-node level, and its queries
+    c5, c5_cpu, c5_check, c5_useful = large_solid(args, log, threads)
+    if args.profile:
+        # a plain copy of known size (256 MiB read, 256 MiB written): the scale for the
+        # profiler's HBM byte counters
+        x = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        return
 
     h = 0.5
     part = Part("c2")
@@ -248,15 +391,6 @@ def main():
         dt = (time.perf_counter() - t) / args.steps
         single[name] = dict(ms=dt * 1e3, bricks_per_s=st.band / dt, stages={k: v / args.steps for k, v in stage_sum.items()})
         log(f"bake {name} (single-level entry point): {dt * 1e3:.3f} ms/step, stages {single[name]['stages']}")
-    if args.profile:
-        # a plain copy of known size (256 MiB read, 256 MiB written): the scale for the
-        # profiler's HBM byte counters
-        x = torch.ones(64 << 20, dtype=torch.float32, device=dev)
-        y = torch.empty_like(x)
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        return
 
     # 2^20 scattered queries on the analytic part (C1), inputs resident in HBM
     c1 = Part("c1")
@@ -280,24 +414,23 @@ def main():
         q[name] = n / dt
         log(f"c1 queries {name}: {n / dt / 1e6:.1f} M/s")
 
-    # the trimmed plate's queries, half of them at its trim curves (on request)
-    c3 = trimmed_plate_queries(args, log, len(os.sched_getaffinity(0))) if args.c3 else None
+    # the trimmed plate's queries, half of them at its trim curves
+    c3 = None if args.no_c3 else trimmed_plate_queries(args, log, threads)
     # the batched workload: 64 parts of uneven face counts and grids in one graph
     c4 = None if args.no_batch else batched_workload(args, log)
 
     check = {}
-    cpu = None
+    c2_cpu = None
     if not args.no_check or not args.no_cpu:
         from oracle import Oracle
 
         o = Oracle(part)
-        threads = len(os.sched_getaffinity(0))
         t = time.perf_counter()
         m_ref, a_ref, nread = o.bake(lv, threads=threads)
         cpu_s = time.perf_counter() - t
-        cpu = {"value": nread / cpu_s, "unit": "bricks/s", "cores": threads, "kind": "port",
-               "sample": f"the same c2 level (all {nread} band bricks of the 32^3), FP64 oracle on {threads} host threads, {cpu_s:.1f} s"}
-        log(f"cpu oracle bake: {cpu_s:.1f}s, {nread / cpu_s:.1f} bricks/s on {threads} threads")
+        c2_cpu = {"value": nread / cpu_s, "unit": "bricks/s", "cores": threads, "kind": "port",
+                  "sample": f"the whole c2 level ({nread} band bricks of the 32^3), FP64 oracle on {threads} host threads, {cpu_s:.1f} s"}
+        log(f"cpu oracle bake (c2): {cpu_s:.1f}s, {nread / cpu_s:.1f} bricks/s on {threads} threads")
         if not args.no_check:
             for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
                 for how, (m, a, st) in (("", batches[name].download(0)), ("single_", gpu.bake(lv, prec))):
@@ -321,44 +454,63 @@ def main():
     for b in batches.values():
         b.close()
     mixed, fp64 = res["mixed"], res["fp64"]
-    bricks_ms = mixed["stages"].get("nodes_search32", float("nan"))
-    # the dominant kernel (the mixed form's node search): its algorithmic bytes per launch
-    # are the single-precision node records it must write (winning feature, two
-    # parameters, distance, flags: 20 B for each of the band bricks' 729 nodes) and the
-    # packed tables it reads (under 2.3 MB: left out); its measured HBM traffic comes from
-    # the committed FETCH_SIZE / WRITE_SIZE profile of the same kernel and workload
-    algo_bytes = mixed["band"] * 729 * 20.0
-    traffic = pmc_traffic(os.path.join(ROOT, "profiles", "bake_c2_256_pmc.md"), "kb_search<1, float>")
+    cm, cf = c5["mixed"], c5["fp64"]
+    # the roofline of the headline's dominant work, the band nodes' search (its launches:
+    # single-precision search, dense pass, last correction, fallback, settle): the useful
+    # floating-point work of reading every band brick's nodes -- the FP64 evaluator's
+    # executed primitives a band brick, counted on a window of the same level, times the
+    # device code's operations a primitive (implicitcam_amd/flops.py), times the level's
+    # band bricks -- over those launches' summed HIP-event time, against the FP32 vector
+    # peak (the bulk of the mixed form is single precision)
+    node_stages = ["nodes_search32", "nodes_leaves32", "nodes_refine64", "nodes_fallback64", "nodes_mark", "nodes_settle64"]
+    node_ms = sum(cm["stages"].get(k, 0.0) for k in node_stages)
+    roof = None
+    if c5_useful:
+        flop = c5_useful["node_flop_per_band_brick"] * cm["band"]
+        achieved = flop / (node_ms / 1e3) / 1e12
+        roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved * 1e12 / PEAK_FP32,
+                "traffic": pmc_traffic(os.path.join(ROOT, "profiles", "bake_c5_1024_pmc.md"), "k_search<1, float>"),
+                "kernel": "the band nodes' search: " + " + ".join(node_stages), "kernel_ms": node_ms, "useful_flop": flop,
+                "useful_flop_per_band_brick": c5_useful["node_flop_per_band_brick"],
+                "hbm": {"algorithmic_bytes": 1464.0 * cm["slots"], "achieved_GBps": 1464.0 * cm["slots"] / (cm["ms"] / 1e3) / 1e9,
+                        "peak_GBps": PEAK_HBM / 1e9, "frac": 1464.0 * cm["slots"] / (cm["ms"] / 1e3) / PEAK_HBM},
+                "valu_measured": pmc_valu(os.path.join(ROOT, "profiles", "bake_c5_1024_valu.md"), "k_search<1, float>")}
     out = {
         "metric": "SDF queries/s + bricks/s, FP32+FP64-refine vs FP64 at <=1e-6 rel err, 1 GPU",
-        "value": mixed["band"] / (mixed["ms"] / 1e3),
+        "value": cm["bricks_per_s"],
         "unit": "bricks/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": mixed["ms"],
+        "ms_per_step": cm["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": "c2 freeform NURBS-top block, 256^3 nodes at 0.5 mm, 8^3-cell bricks, narrow band, FP32+FP64-refine, "
-                               "batched evaluator (one part, captured graph)",
-                   "bricks": mixed["bricks"], "band_bricks": mixed["band"], "stored_bricks": mixed["slots"]},
-        "fp64_bricks_per_s": fp64["band"] / (fp64["ms"] / 1e3),
-        "mixed_over_fp64": fp64["ms"] / mixed["ms"],
-        "single_level_entry_point": {"mixed": single["mixed"], "fp64": single["fp64"]},
-        "tuning": {"mixed": mixed["tuning"], "fp64": fp64["tuning"]},
+        "config": {"workload": "c5 large mixed solid: 400 x 300 x 40 mm plate, 1,998 faces (800 rational bicubic 32 x 32 nets, weights 0.5-2; "
+                               "planes, cylinders, cones, tori), loops of 12-64 cubic spans; narrow-band bake of its 1024-node level "
+                               f"(cell {c5['h']:.4f} mm, {c5['level'][0]} x {c5['level'][1]} x {c5['level'][2]} bricks of 8^3 cells), "
+                               "FP32+FP64-refine, single-level entry point, device-resident",
+                   "bricks": cm["bricks"], "band_bricks": cm["band"], "stored_bricks": cm["slots"]},
+        "fp64_bricks_per_s": cf["bricks_per_s"],
+        "mixed_over_fp64": c5["mixed_over_fp64"],
+        "queries_per_s": {"mixed": c5["queries"]["mixed"]["queries_per_s"], "fp64": c5["queries"]["fp64"]["queries_per_s"],
+                          "points": c5["queries"]["points"]},
+        "roofline": roof,
+        "cpu_baseline": c5_cpu,
+        "check": c5_check,
+        "c5": c5,
+        "c2_bake": {"workload": "c2 freeform NURBS-top block, 256^3 nodes at 0.5 mm (32^3 bricks), batched evaluator (one part, captured graph)",
+                    "bricks": mixed["bricks"], "band_bricks": mixed["band"], "stored_bricks": mixed["slots"],
+                    "mixed_bricks_per_s": mixed["band"] / (mixed["ms"] / 1e3), "fp64_bricks_per_s": fp64["band"] / (fp64["ms"] / 1e3),
+                    "mixed_ms": mixed["ms"], "fp64_ms": fp64["ms"], "mixed_over_fp64": fp64["ms"] / mixed["ms"],
+                    "single_level_entry_point": {"mixed": single["mixed"], "fp64": single["fp64"]},
+                    "tuning": {"mixed": mixed["tuning"], "fp64": fp64["tuning"]},
+                    "stages_ms": {"mixed": mixed["stages"], "fp64": fp64["stages"]},
+                    "undecided_nodes": mixed["undecided"], "refined_nodes": mixed["refined"],
+                    "cpu_baseline": c2_cpu, "check": check},
         "c1_queries_per_s": {"mixed": q["mixed"], "fp64": q["fp64"]},
-        "stages_ms": {"mixed": mixed["stages"], "fp64": fp64["stages"]},
-        "undecided_nodes": mixed["undecided"],
-        "refined_nodes": mixed["refined"],
-        "roofline": {"bound": "hbm", "achieved": algo_bytes / (bricks_ms / 1e3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                     "frac": algo_bytes / (bricks_ms / 1e3) / PEAK_HBM, "traffic": traffic, "kernel": "kb_search<1, float> (nodes_search32)",
-                     "kernel_ms": bricks_ms, "algorithmic_bytes": algo_bytes,
-                     "valu": pmc_valu(os.path.join(ROOT, "profiles", "bake_c2_256_valu.md"), "kb_search<1, float>")},
-        "cpu_baseline": cpu,
-        "check": check,
         "c3_queries": c3,
         "c4_batch": c4,
     }

@@ -47,7 +47,7 @@ namespace {
 
 constexpr int WG = 256;
 // The settle kernels' grid: as many blocks as the device runs at once (settle_grid,
-// from the occupancy the compiled kernel allows: two blocks a CU on MI355X). Their
+// from the occupancy the compiled kernel allows: one wave a SIMD). Their
 // lanes hold the certified search's queue in private memory and run long, so blocks
 // past that would wait for a slot and take their share of the queue late.
 template <class K> int settle_grid(K kernel, int nt, int cus) {
@@ -125,9 +125,10 @@ __host__ __device__ inline double brick_reach(double h) {
 
 // Sources of points: scattered points (SRC_POINTS one lane a point; SRC_WIDE four lanes a
 // point, for parts with B-spline faces or spline edges, whose unbounded searches reach
-// many parts), the 729 nodes of bricks, brick centres.
-enum Src { SRC_POINTS = 0, SRC_NODES = 1, SRC_CENTRES = 2, SRC_WIDE = 3 };
-template <int S> constexpr bool pts() { return S == SRC_POINTS || S == SRC_WIDE; }
+// many parts; SRC_FAR a wave a point, four points a block, for the points of blocks whose
+// candidates did not fit their list), the 729 nodes of bricks, brick centres.
+enum Src { SRC_POINTS = 0, SRC_NODES = 1, SRC_CENTRES = 2, SRC_WIDE = 3, SRC_FAR = 4 };
+template <int S> constexpr bool pts() { return S == SRC_POINTS || S == SRC_WIDE || S == SRC_FAR; }
 
 struct Job {
   long n;                 // points
@@ -778,8 +779,8 @@ __device__ inline void classify64(const Best<double>& b, double bound, const Dev
 // more in double).
 template <int S> struct Shape {
   static constexpr int NT = 256;
-  static constexpr int K = S == SRC_CENTRES ? 16 : S == SRC_WIDE ? 4 : 1;  // lanes a point
-  static constexpr int Q = S == SRC_CENTRES ? 8 : 16;   // queue entries a lane
+  static constexpr int K = S == SRC_CENTRES ? 16 : S == SRC_WIDE ? 4 : S == SRC_FAR ? 64 : 1;  // lanes a point
+  static constexpr int Q = S == SRC_CENTRES || S == SRC_FAR ? 8 : 16;   // queue entries a lane
   static constexpr int C = S == SRC_NODES ? 1024 : 4096;  // a block's candidates, at most
 };
 constexpr int TILE_X = 4, TILE_Y = 2, TILE_Z = 2;
@@ -997,10 +998,10 @@ template <int S, class T> struct SearchLds {
 // primitives within reach culled into the list and the search carried on over them; a
 // chunk no point of the block can find anything nearer in (its box no nearer than any
 // point's answer so far) is passed over.
-template <int S, class T, bool SETTLE>
-__device__ Best<T> search_chunks(const Dev& dv, const Tab<T>& tb, v3<T> p, bool valid, T bound, SearchLds<S, T>& sm, int nchunk, const float* gb, float r,
+template <int S, class T, bool SETTLE, int NT = Shape<S>::NT, int K = Shape<S>::K, class L = SearchLds<S, T>>
+__device__ Best<T> search_chunks(const Dev& dv, const Tab<T>& tb, v3<T> p, bool valid, T bound, L& sm, int nchunk, const float* gb, float r,
                                  T tm, T eps) {
-  constexpr int NT = Shape<S>::NT, Q = Shape<S>::Q, K = Shape<S>::K, C = Shape<S>::C;
+  constexpr int Q = Shape<S>::Q, C = Shape<S>::C;
   Best<T> b = best_init<T>(bound);
   if constexpr (!pts<S>()) {
     return b;  // (only scattered points are searched in chunks)
@@ -1075,14 +1076,27 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res)
   __shared__ SearchLds<S, T> sm;
   search_block<S, T>(dv, j, res, blockIdx.x, sm);
 }
-// The scattered points' blocks k_search queued (their candidates past the list),
-// searched chunk by chunk.
-template <int S, class T>
-__global__ __launch_bounds__(Shape<S>::NT) void k_chunks(Dev dv, Job j, Res res) {
-  __shared__ SearchLds<S, T> sm;
-  const uint32_t total = *res.ccount;
+// The points of the scattered blocks k_search queued (their candidates past the list): a
+// block of four points, a wave a point (four points' box reaches far less than their whole
+// block's, so their list fits where the block's did not; past it, chunks), searched in
+// double precision to their final answers, in the mixed form too, where their records are
+// marked final so the last correction leaves them (unbounded points far from the surface,
+// each reaching many parts: on the large mixed solid's uniform points the block's chunked
+// search took 1.7 s for 2^20 queries).
+constexpr uint32_t INFO_FINAL = 0x80000000u;
+template <int S>
+__global__ __launch_bounds__(256) void k_far(Dev dv, Job j, Res res) {
+  static_assert(pts<S>() && S != SRC_FAR, "scattered blocks");
+  constexpr int FP = Shape<SRC_FAR>::NT / Shape<SRC_FAR>::K;   // points a far block
+  constexpr int SUB = (Shape<S>::NT / Shape<S>::K) / FP;         // far blocks a queued block
+  static_assert(SUB * FP == Shape<S>::NT / Shape<S>::K, "a queued block splits into far blocks");
+  __shared__ SearchLds<SRC_FAR, double> sm;
+  const uint32_t total = *res.ccount * uint32_t(SUB);
   for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
-    search_block<S, T, true>(dv, j, res, long(res.cqueue[q]), sm);
+    const long fb = long(res.cqueue[q / SUB]) * SUB + long(q % SUB);
+    const long i = fb * FP + long(threadIdx.x) / Shape<SRC_FAR>::K;
+    if (res.info && (threadIdx.x & (Shape<SRC_FAR>::K - 1)) == 0 && i < j.n) res.info[i] = INFO_FINAL;
+    search_block<SRC_FAR, double, true>(dv, j, res, fb, sm);
     __syncthreads();
   }
 }
@@ -1120,6 +1134,7 @@ __global__ __launch_bounds__(256) void k_leaves(Dev dv, Job j, Res res) {
 // with no winner is written; returns whether the point waits for refine_found.
 template <int S>
 __device__ __forceinline__ bool refine_head(const Dev& dv, const Job& j, const Res& res, long i, uint32_t info) {
+  if (info & INFO_FINAL) return false;  // searched in double precision already (k_far)
   const bool unsure = info & 4u;
   wave_count(&res.counters[1], unsure);
   wave_count(&res.counters[3], unsure && (info & 16u));
@@ -1308,12 +1323,100 @@ __global__ __launch_bounds__(256) void k_mark_settle(Job j, Res res) {
   if (i < j.n) mark_point<S>(j, res, i);
 }
 
+// The settle, one wave a queued block: the block's marked points (flag 4) are listed in
+// LDS, its candidates culled for their box, and each lane takes one listed point. A
+// queued block's marked points are few (a brick's nodes near a concave corner of a
+// wall, a block's points near a trim curve) and the certified search holds 256
+// registers and as many accumulation registers a lane, one wave a SIMD: a block of 256
+// lanes a queued block, each lane its own point, left most lanes and three SIMDs of
+// four idle while the queue's blocks waited.
+constexpr int SETTLE_NT = 64;
+template <int S> struct SettleLds {
+  static constexpr int NT = SETTLE_NT, Q = Shape<S>::Q, C = Shape<S>::C;
+  // the points a queued block holds: a brick's nodes, a tile's centres, a block's points
+  static constexpr int NP = S == SRC_NODES ? 729 : S == SRC_CENTRES ? TILE_X * TILE_Y * TILE_Z : Shape<S>::NT / Shape<S>::K;
+  uint32_t cand[C];
+  float red[NT];
+  int wave_tot[1];
+  alignas(16) unsigned char scratch[cmax(cmax(SORTC * 12, 2 * FCAP * 4), Q * NT * 10)];
+  uint32_t list[NP];
+  int count;
+  uint32_t chunks[pts<S>() ? FCAP : 1];
+  uint32_t sub[pts<S>() ? 64 : 1];
+  __device__ uint32_t* tmp() { return (uint32_t*)scratch; }
+  __device__ float* key() { return (float*)(scratch + SORTC * 4); }
+  __device__ double* qlb() { return (double*)scratch; }
+  __device__ uint16_t* qk() { return (uint16_t*)(scratch + Q * NT * 8); }
+};
+template <int S>
+__device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const Res& res, long blk, SettleLds<S>& sm) {
+  constexpr int NT = SETTLE_NT, NP = SettleLds<S>::NP, Q = Shape<S>::Q, C = Shape<S>::C;
+  if (threadIdx.x == 0) sm.count = 0;
+  __syncthreads();
+  for (int g = threadIdx.x; g < NP; g += NT) {
+    long i;
+    if (S == SRC_NODES)
+      i = blk * 729 + g;
+    else if (S == SRC_CENTRES)
+      i = tile_point(j.lv, blk, g);
+    else
+      i = blk * NP + g;
+    if (i >= 0 && i < j.n && res.flag[i] == 4u) sm.list[atomicAdd(&sm.count, 1)] = uint32_t(i);
+  }
+  __syncthreads();
+  const int cnt = sm.count;
+  if (cnt == 0) return;
+  // brick centres and scattered points take their first search's K lanes a point (a
+  // centre's search reaches far over many parts, which its lanes split), nodes a lane each
+  constexpr int K = S == SRC_NODES ? 1 : Shape<S>::K;
+  int nchunk = 0, nc = 0;
+  float gb[6], rr = 0.f;
+  bool chunked = false;
+  // the candidates: a brick's or a tile's once; scattered points' for each round's
+  // points alone (a queued block's marked points may lie far apart, far from the
+  // surface: the box of all of them could reach past the list)
+  auto cull_for = [&](const uint32_t* pl, int pc) {
+    nchunk = 0;
+    nc = block_candidates<S, NT>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot, pts<S>() ? sm.chunks : nullptr, &nchunk, gb, &rr,
+                                 nullptr, 0u, pl, pc);
+    chunked = pts<S>() && nchunk > 0;
+  };
+  if (!pts<S>()) cull_for(nullptr, 0);
+  const Tab<double> tb = tab_of<double>(dv);
+  for (int base = 0; base < cnt; base += NT / K) {
+    if (pts<S>()) {
+      __syncthreads();
+      cull_for(sm.list + base, min(NT / K, cnt - base));
+    }
+    const int idx = base + int(threadIdx.x) / K;
+    const bool valid = idx < cnt;
+    const long i = valid ? long(sm.list[idx]) : -1;
+    double bound = 0.0;
+    const v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
+    const double sb = bound < 1e300 ? bound * (1.0 + 1e-12) : bound;
+    Best<double> b;
+    if constexpr (pts<S>()) {
+      b = chunked ? search_chunks<S, double, true, NT, K>(dv, tb, p, valid, sb, sm, nchunk, gb, rr, 0.0, dv.L.eps64)
+                  : search<double, NT, Q, K, C, true>(tb, p, valid, sb, sm.cand, nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk());
+    } else {
+      b = search<double, NT, Q, K, C, true>(tb, p, valid, sb, sm.cand, nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk());
+    }
+    if (!valid || (threadIdx.x & (K - 1)) != 0) continue;
+    double val;
+    uint32_t flag, feat;
+    classify64(b, bound, dv, val, flag, feat);
+    res.val[i] = val;
+    res.flag[i] = flag;
+    if (res.feat) res.feat[i] = feat;
+  }
+}
+
 // Blocks take the next queued block from a counter (scount[1], cleared with the queue's
 // length): a queued block's settle costs from one read to thousands, so a fixed share
 // each left a few blocks running long after the rest.
 template <int S>
-__global__ __launch_bounds__(Shape<S>::NT) void k_settle(Dev dv, Job j, Res res) {
-  __shared__ SearchLds<S, double> sm;
+__global__ __launch_bounds__(SETTLE_NT) void k_settle(Dev dv, Job j, Res res) {
+  __shared__ SettleLds<S> sm;
   __shared__ uint32_t wslot;
   const uint32_t total = *res.scount;
   for (;;) {
@@ -1321,7 +1424,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_settle(Dev dv, Job j, Res res)
     __syncthreads();
     const uint32_t qb = wslot;
     if (qb >= total) break;
-    redo_block<S, true>(dv, j, res, long(res.squeue[qb] & 0xffffffu), sm);
+    settle_block<S>(dv, j, res, long(res.squeue[qb] & 0xffffffu), sm);
     __syncthreads();
   }
 }
@@ -1934,6 +2037,7 @@ __global__ __launch_bounds__(256) void kb_mark_bricks(BatchArgs a) {
 // entries are part << 24 | block.
 template <int S, bool SETTLE>
 __global__ __launch_bounds__(Shape<S>::NT) void kb_redo(BatchArgs a) {
+  static_assert(!SETTLE, "the batch's settle is kb_settle");
   __shared__ SearchLds<S, double> sm;
   __shared__ uint32_t wslot;
   const bool settle = SETTLE;
@@ -1950,6 +2054,28 @@ __global__ __launch_bounds__(Shape<S>::NT) void kb_redo(BatchArgs a) {
     const Job j = S == SRC_CENTRES ? ldk(P.jc) : ldk(P.jn);
     const Res r = S == SRC_CENTRES ? ldk(P.rc) : ldk(P.rn);
     redo_block<S, SETTLE>(dv, j, r, long(e & 0xffffffu), sm);
+    __syncthreads();
+  }
+}
+
+// Every part's settle queue, one wave a queued block (settle_block).
+template <int S>
+__global__ __launch_bounds__(SETTLE_NT) void kb_settle(BatchArgs a) {
+  __shared__ SettleLds<S> sm;
+  __shared__ uint32_t wslot;
+  const int st = S == SRC_CENTRES ? BS_CSETTLE : BS_NSETTLE;
+  const uint32_t total = a.ctl[S == SRC_CENTRES ? BC_SC : BC_SN];
+  for (;;) {
+    const uint32_t w = next_item(a, st, &wslot);
+    if (w >= total) break;
+    const ICAM_K BPart& P0 = bpart(a, 0);
+    const Res r0 = S == SRC_CENTRES ? ldk(P0.rc) : ldk(P0.rn);
+    const uint32_t e = __builtin_amdgcn_readfirstlane(r0.squeue[w]);
+    const ICAM_K BPart& P = bpart(a, __builtin_amdgcn_readfirstlane(int(e >> 24)));
+    const Dev dv = ldk(P.dv);
+    const Job j = S == SRC_CENTRES ? ldk(P.jc) : ldk(P.jn);
+    const Res r = S == SRC_CENTRES ? ldk(P.rc) : ldk(P.rn);
+    settle_block<S>(dv, j, r, long(e & 0xffffffu), sm);
     __syncthreads();
   }
 }
@@ -2269,8 +2395,7 @@ template <int S>
 int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Stages* st, long blocks) {
   if (j.n <= 0) return 0;
   constexpr int NT = Shape<S>::NT;
-  static const char* chunk_names[2] = {"points_chunks32", "points_chunks64"};
-  const unsigned cb = unsigned(std::min<long>(blocks, 2048));
+  const unsigned gfar = unsigned(std::min<long>(blocks * 16, 4096));
   static const char* names[4][8] = {{"points_search32", "points_refine64", "points_fallback64", "points_search64", "points_leaves32", "points_leaves64",
                                      "points_mark", "points_settle64"},
                                     {"nodes_search32", "nodes_refine64", "nodes_fallback64", "nodes_search64", "nodes_leaves32", "nodes_leaves64",
@@ -2290,7 +2415,7 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
   if (mixed) {
     HIP_OK(launch(st, names[S][0], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, float>, h->dv, j, r));
     if constexpr (pts<S>())
-      if (r.cqueue) HIP_OK(launch(st, chunk_names[0], s, dim3(cb), dim3(NT), k_chunks<S, float>, h->dv, j, r));
+      if (r.cqueue) HIP_OK(launch(st, "points_far64", s, dim3(gfar), dim3(256), k_far<S>, h->dv, j, r));
     if (lb) HIP_OK(launch(st, names[S][4], s, dim3(lb), dim3(256), k_leaves<S, float>, h->dv, j, r));
     if constexpr (S == SRC_NODES)
       HIP_OK(launch(st, names[S][1], s, dim3(unsigned(std::min<long>(blocks, 8192))), dim3(256), k_refine_bricks, h->dv, j, r, blocks));
@@ -2301,14 +2426,14 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
   } else {
     HIP_OK(launch(st, names[S][3], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, double>, h->dv, j, r));
     if constexpr (pts<S>())
-      if (r.cqueue) HIP_OK(launch(st, chunk_names[1], s, dim3(cb), dim3(NT), k_chunks<S, double>, h->dv, j, r));
+      if (r.cqueue) HIP_OK(launch(st, "points_far64", s, dim3(gfar), dim3(256), k_far<S>, h->dv, j, r));
     if (lb) HIP_OK(launch(st, names[S][5], s, dim3(lb), dim3(256), k_leaves<S, double>, h->dv, j, r));
   }
   // what is still undecided: settled by the certified search (not the brick centres:
   // an undecided centre under its reach keeps its brick, whose nodes are read)
   if (S == SRC_CENTRES) return 0;
   HIP_OK(launch(st, names[S][6], s, dim3(unsigned((j.n + 255) / 256)), dim3(256), k_mark_settle<S>, j, r));
-  HIP_OK(launch(st, names[S][7], s, dim3(unsigned(std::min<long>(blocks, h->settle_grid))), dim3(NT), k_settle<S>, h->dv, j, r));
+  HIP_OK(launch(st, names[S][7], s, dim3(unsigned(std::min<long>(blocks, h->settle_grid))), dim3(SETTLE_NT), k_settle<S>, h->dv, j, r));
   return 0;
 }
 
@@ -2377,7 +2502,7 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
   hipDeviceProp_t prop;
   HIP_OK(hipGetDeviceProperties(&prop, h->device));
   h->name = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ", " + std::to_string(prop.multiProcessorCount) + " CUs)";
-  h->settle_grid = settle_grid(k_settle<SRC_WIDE>, Shape<SRC_WIDE>::NT, prop.multiProcessorCount);
+  h->settle_grid = settle_grid(k_settle<SRC_WIDE>, SETTLE_NT, prop.multiProcessorCount);
   if (noalloc_setup(h->device)) {
     std::string e = g_error;
     delete h;
@@ -2782,7 +2907,7 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     HIP_OK(hipMemsetAsync(r.smark, 0, size_t(tiles) * 4, s));
     HIP_OK(launch(&st, "centres_need", s, dim3(unsigned(std::min<long>(nband, 4096))), dim3(256), k_need_centre, jc, r, (const uint32_t*)h->list.as<uint32_t>(),
                   long(nband), (const uint32_t*)rn.flag, (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>()));
-    HIP_OK(launch(&st, "centres_settle64", s, dim3(unsigned(std::min<long>(tiles, h->settle_grid))), dim3(Shape<SRC_CENTRES>::NT), k_settle<SRC_CENTRES>,
+    HIP_OK(launch(&st, "centres_settle64", s, dim3(unsigned(std::min<long>(tiles, h->settle_grid))), dim3(SETTLE_NT), k_settle<SRC_CENTRES>,
                   h->dv, jc, r));
     HIP_OK(launch(&st, "centres_sign", s, dim3(unsigned(std::min<long>((long(nband) + 255) / 256, 4096))), dim3(256), k_centre_sign, r,
                   (const uint32_t*)h->list.as<uint32_t>(), long(nband), h->dc.as<double>(), h->sgn.as<int8_t>()));
@@ -2980,10 +3105,10 @@ int batch_enqueue(icam_batch* B, hipStream_t s, Stages* st = nullptr) {
     HIP_OK(launch(st, "nodes_leaves64", s, dim3(gl), dim3(256), kb_leaves<double>, a));
   }
   HIP_OK(launch(st, "nodes_mark", s, dim3(unsigned(std::min<size_t>(std::max<size_t>(B->total_cap, 1), 8192))), dim3(256), kb_mark_bricks, a));
-  HIP_OK(launch(st, "nodes_settle64", s, dim3(unsigned(B->gs)), ntn, kb_redo<SRC_NODES, true>, a));
+  HIP_OK(launch(st, "nodes_settle64", s, dim3(unsigned(B->gs)), dim3(SETTLE_NT), kb_settle<SRC_NODES>, a));
   // the centres of all-far band bricks with no sign: settled
   HIP_OK(launch(st, "centres_need", s, dim3(B->gf), dim3(256), kb_need_centre, a));
-  HIP_OK(launch(st, "centres_settle64", s, dim3(unsigned(B->gs)), ntc, kb_redo<SRC_CENTRES, true>, a));
+  HIP_OK(launch(st, "centres_settle64", s, dim3(unsigned(B->gs)), dim3(SETTLE_NT), kb_settle<SRC_CENTRES>, a));
   HIP_OK(launch(st, "centres_sign", s, dim3(unsigned(std::min<size_t>((B->total_cap + 255) / 256, 4096))), dim3(256), kb_centre_sign, a));
   // fill, quantize, pack, number and store
   HIP_OK(launch(st, "bricks_finish", s, dim3(B->gf), dim3(WG), kb_finish, a));
@@ -3167,7 +3292,7 @@ int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t
   HIP_OK(hipGetDeviceProperties(&prop, dev));
   const int cus = std::max(prop.multiProcessorCount, 1);
   B->gc = 9 * cus, B->gn = 4 * cus, B->gf = 8 * cus;
-  B->gs = settle_grid(kb_redo<SRC_NODES, true>, Shape<SRC_NODES>::NT, cus);
+  B->gs = settle_grid(kb_settle<SRC_NODES>, SETTLE_NT, cus);
   if (tune && batch_tune(B.get(), cus)) return 1;
   // the captured graph
   HIP_OK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeRelaxed));

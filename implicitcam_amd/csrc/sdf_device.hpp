@@ -448,8 +448,9 @@ template <class T> __device__ SEval<T> peval1(const ICAM_K T* pw, T s, T t) {
 // its bound is held there while the gradient points out of the rectangle, and Newton
 // runs in the other alone. Over a part its radius of convexity certifies, the
 // squared distance is strictly convex, so this reaches the part's one minimum,
-// inside it or on its border (where two parts share a border both reach it).
-template <class T> __device__ void newton2(const ICAM_K T* pw, v3<T> p, T& s, T& t, T s0, T s1, T t0, T t1, int iters, T stop) {
+// inside it or on its border (where two parts share a border both reach it). Returns
+// whether it settled (a step under `stop`, or held on both bounds) within `iters`.
+template <class T> __device__ bool newton2(const ICAM_K T* pw, v3<T> p, T& s, T& t, T s0, T s1, T t0, T t1, int iters, T stop) {
   for (int it = 0; it < iters; ++it) {
     SEval<T> e = peval(pw, s, t);
     v3<T> r = e.p - p;
@@ -459,7 +460,7 @@ template <class T> __device__ void newton2(const ICAM_K T* pw, v3<T> p, T& s, T&
     bool hs = (s <= s0 && g0 > T(0)) || (s >= s1 && g0 < T(0));
     bool ht = (t <= t0 && g1 > T(0)) || (t >= t1 && g1 < T(0));
     T ds, dt;
-    if (hs && ht) break;
+    if (hs && ht) return true;
     if (hs) {
       ds = 0;
       dt = -g1 / (h2 > T(0) ? h2 : mmax(jtt, TypeEps<T>::ulp));
@@ -469,15 +470,16 @@ template <class T> __device__ void newton2(const ICAM_K T* pw, v3<T> p, T& s, T&
     } else {
       if (!(h0 > T(0) && h0 * h2 - h1 * h1 > T(0))) h0 = jss, h1 = jst, h2 = jtt;
       T det = h0 * h2 - h1 * h1;
-      if (!(mabs(det) > T(0))) break;
+      if (!(mabs(det) > T(0))) return false;
       ds = -(h2 * g0 - h1 * g1) / det;
       dt = -(h0 * g1 - h1 * g0) / det;
     }
     T ns = mclamp(s + ds, s0, s1), nt = mclamp(t + dt, t0, t1);
     T moved = mabs(ns - s) + mabs(nt - t);
     s = ns, t = nt;
-    if (moved < stop) break;
+    if (moved < stop) return true;
   }
+  return false;
 }
 
 // A cubic rational curve piece's point and derivatives (curve3.rs:129-156).
@@ -502,8 +504,10 @@ template <class T> __device__ CEval<T> ceval(const ICAM_K T* pw, T s) {
   return r;
 }
 
-// curve3.rs:192-213 Newton on |C - p|^2 over [t0, t1]
-template <class T> __device__ T newton1(const ICAM_K T* pw, v3<T> p, T s, T t0, T t1, int iters, T stop) {
+// curve3.rs:192-213 Newton on |C - p|^2 over [t0, t1]; *conv (when given) tells whether
+// it settled within `iters`.
+template <class T> __device__ T newton1(const ICAM_K T* pw, v3<T> p, T s, T t0, T t1, int iters, T stop, bool* conv = nullptr) {
+  if (conv) *conv = false;
   for (int it = 0; it < iters; ++it) {
     CEval<T> e = ceval(pw, s);
     v3<T> r = e.p - p;
@@ -514,6 +518,7 @@ template <class T> __device__ T newton1(const ICAM_K T* pw, v3<T> p, T s, T t0, 
     T ns = mclamp(s - f / fp, t0, t1);
     if (mabs(ns - s) < stop) {
       s = ns;
+      if (conv) *conv = true;
       break;
     }
     s = ns;
@@ -914,8 +919,19 @@ template <class T, class P> __device__ inline int patch_kind(P cb, const T el[3]
   const T lm = T(0.5) * ((al + cl) - sqrt((al - cl) * (al - cl) + T(4) * bm * bm));
   return lg > T(0) && lg + lm > IvTol<T>::guard * lg ? 2 : 0;
 }
-template <class T> __device__ inline int leaf_kind(const ICAM_K T* cb, const ICAM_K float* box, v3<T> p) {
-  const T el[3] = {T(box[0]) - p.x, T(box[1]) - p.y, T(box[2]) - p.z}, eh[3] = {T(box[3]) - p.x, T(box[4]) - p.y, T(box[5]) - p.z};
+// The box of S - p over a part for the interval tests: in single precision grown by the
+// point's own rounding (p is the query rounded to single precision, up to half a unit in
+// its last place from the point the answer is for; a test decided within that margin
+// passed over the part holding the large mixed solid's nearest point, 0.6 mm nearer than
+// the answer) and by the search's rounding allowance eps / 8; exact in double precision.
+template <class T> __device__ inline void rel_box(const ICAM_K float* box, v3<T> p, T eps, T el[3], T eh[3]) {
+  const T w = sizeof(T) == 4 ? T(0.125) * eps + (mabs(p.x) + mabs(p.y) + mabs(p.z)) * T(1.2e-7) : T(0);
+  el[0] = T(box[0]) - p.x - w, el[1] = T(box[1]) - p.y - w, el[2] = T(box[2]) - p.z - w;
+  eh[0] = T(box[3]) - p.x + w, eh[1] = T(box[4]) - p.y + w, eh[2] = T(box[5]) - p.z + w;
+}
+template <class T> __device__ inline int leaf_kind(const ICAM_K T* cb, const ICAM_K float* box, v3<T> p, T eps) {
+  T el[3], eh[3];
+  rel_box(box, p, eps, el, eh);
   return patch_kind<T>(cb, el, eh);
 }
 // The same for a curve part (bounds cb: |C'|^2 from below, then C', C''; gpu.wgsl:1602-1614).
@@ -929,8 +945,9 @@ template <class T, class P> __device__ inline int piece_kind(P cb, const T el[3]
   const T l = cb[0];
   return l > T(0) && l + kl > IvTol<T>::guard * l ? 2 : 0;
 }
-template <class T> __device__ inline int cleaf_kind(const ICAM_K T* cb, const ICAM_K float* box, v3<T> p) {
-  const T el[3] = {T(box[0]) - p.x, T(box[1]) - p.y, T(box[2]) - p.z}, eh[3] = {T(box[3]) - p.x, T(box[4]) - p.y, T(box[5]) - p.z};
+template <class T> __device__ inline int cleaf_kind(const ICAM_K T* cb, const ICAM_K float* box, v3<T> p, T eps) {
+  T el[3], eh[3];
+  rel_box(box, p, eps, el, eh);
   return piece_kind<T>(cb, el, eh);
 }
 
@@ -1303,7 +1320,7 @@ __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box,
   bool certified = fd < T(0.999) * rho - eps;
   if (!certified) {
     // past its radius: the interval test for this point (gpu.wgsl:1677-1683)
-    const int kind = leaf_kind(lf + LEAF_CERT, box, p);
+    const int kind = leaf_kind(lf + LEAF_CERT, box, p, eps);
     if (kind == 1) return;
     certified = kind == 2;
   }
@@ -1317,11 +1334,16 @@ __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box,
   T s0 = lf[1], s1 = lf[2], t0 = lf[3], t1 = lf[4];
   T s = T(0.5) * (s0 + s1), t = T(0.5) * (t0 + t1);
   const bool single = sizeof(T) == 4;
-  newton2(sp, p, s, t, s0, s1, t0, t1, single ? 8 : 32, single ? T(1e-5) : T(1e-14));
+  const bool conv = newton2(sp, p, s, t, s0, s1, t0, t1, single ? 8 : 32, single ? T(1e-5) : T(1e-14));
   SEval<T> e = peval1(sp, s, t);
   v3<T> r = e.p - p;
   T d = len(r);
   if (!certified) b.lmin = mmin(b.lmin, lb);
+  // a certified part whose single-precision Newton did not settle within its 8 steps:
+  // its one minimum is not read, and another part's answer must not stand for it, so
+  // the point is searched again in double precision (32 steps; there a step under 1e-14
+  // is not always reached in rounding, and the point it rests on is the minimum)
+  if (single && certified && !conv && lb < b.d) b.unsure = true;
   if (!(d < b.d)) return;
   // a foot at a pole the sub-patch touches: the pole's point record answers there
   // (its normal comes from the fan; the surface's own degenerates)
@@ -1448,7 +1470,7 @@ __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box
   T rho = cf[0];
   bool certified = far_dist(p, box) < T(0.999) * rho - eps;
   if (!certified) {
-    const int kind = cleaf_kind(cf + CLEAF_CERT, box, p);
+    const int kind = cleaf_kind(cf + CLEAF_CERT, box, p, eps);
     if (kind == 1) {
       if (!(jlo || jhi)) return;
       // no critical point: the junction end (the nearer if both are)
@@ -1468,11 +1490,13 @@ __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box
     }
   }
   const bool single = sizeof(T) == 4;
-  T s = newton1(pw, p, T(0.5) * (t0 + t1), t0, t1, single ? 10 : 40, single ? T(1e-5) : T(1e-15));
+  bool conv = false;
+  T s = newton1(pw, p, T(0.5) * (t0 + t1), t0, t1, single ? 10 : 40, single ? T(1e-5) : T(1e-15), &conv);
   CEval<T> e = ceval(pw, s);
   v3<T> r = e.p - p;
   T d = len(r);
   if (!certified) b.lmin = mmin(b.lmin, lb);
+  if (single && certified && !conv && lb < b.d) b.unsure = true;
   if (!(d < b.d)) return;
   // an end of the edge is its vertex's (a C0 line's ends are in its face's boundary,
   // whose edges answer there)
