@@ -88,6 +88,7 @@ def large_solid(args, log, threads):
             f"stages {res[name]['stages']}")
     res["mixed_over_fp64"] = res["fp64"]["ms"] / res["mixed"]["ms"]
     if args.profile:
+        g.close()
         return res, None, None, None
     # scattered queries of the workload's mix, inputs resident in HBM
     n = 1 << args.c5_log2
@@ -161,6 +162,7 @@ def large_solid(args, log, threads):
                                             "p50_rel_err": float(np.percentile(err, 50)),
                                             "sign_mismatch": int(((qout[name][0][idx] < 0) != (dref < 0)).sum())}
             log(f"c5 check: {check}")
+    g.close()
     return res, cpu, check, useful
 
 
@@ -242,6 +244,7 @@ def trimmed_plate_queries(args, log, threads):
             res[name]["vs_oracle"] = {"n": int(len(idx)), "max_rel_err": float(err.max()), "p99_rel_err": float(np.percentile(err, 99)),
                                       "p50_rel_err": float(np.percentile(err, 50)), "sign_mismatch": int(((outs[name][0][idx] < 0) != (dref < 0)).sum())}
     log(f"c3 check: {res['mixed_vs_fp64']} {res.get('mixed', {}).get('vs_oracle')}")
+    g.close()
     return res
 
 
@@ -414,11 +417,6 @@ def main():
         q[name] = n / dt
         log(f"c1 queries {name}: {n / dt / 1e6:.1f} M/s")
 
-    # the trimmed plate's queries, half of them at its trim curves
-    c3 = None if args.no_c3 else trimmed_plate_queries(args, log, threads)
-    # the batched workload: 64 parts of uneven face counts and grids in one graph
-    c4 = None if args.no_batch else batched_workload(args, log)
-
     check = {}
     c2_cpu = None
     if not args.no_check or not args.no_cpu:
@@ -451,8 +449,15 @@ def main():
                                        "sign_mismatch": int(((dq < 0) != (dref < 0)).sum()), "n": int(len(sample))}
             log(f"check: {check}")
 
+    # each leg's handles closed before the next opens its own
     for b in batches.values():
         b.close()
+    gpu.close()
+    g1.close()
+    # the trimmed plate's queries, half of them at its trim curves
+    c3 = None if args.no_c3 else trimmed_plate_queries(args, log, threads)
+    # the batched workload: 64 parts of uneven face counts and grids in one graph
+    c4 = None if args.no_batch else batched_workload(args, log)
     mixed, fp64 = res["mixed"], res["fp64"]
     cm, cf = c5["mixed"], c5["fp64"]
     # the roofline of the headline's dominant work, the band nodes' search (its launches:
@@ -464,17 +469,22 @@ def main():
     # peak (the bulk of the mixed form is single precision)
     node_stages = ["nodes_search32", "nodes_leaves32", "nodes_refine64", "nodes_fallback64", "nodes_mark", "nodes_settle64"]
     node_ms = sum(cm["stages"].get(k, 0.0) for k in node_stages)
+    # HBM bytes a launch of those kernels, from the committed counter profile of the
+    # same workload (profiles/bake_c5_1024_pmc.md)
+    pmc = os.path.join(ROOT, "profiles", "bake_c5_1024_pmc.md")
+    parts = [pmc_traffic(pmc, k) for k in ("k_search<1, float>", "k_leaves<1, float>", "k_refine_bricks", "k_fallback<1>", "k_mark_settle<1>", "k_settle<1>")]
+    node_traffic = sum(parts) if all(v is not None for v in parts) else None
     roof = None
     if c5_useful:
         flop = c5_useful["node_flop_per_band_brick"] * cm["band"]
         achieved = flop / (node_ms / 1e3) / 1e12
         roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved * 1e12 / PEAK_FP32,
-                "traffic": pmc_traffic(os.path.join(ROOT, "profiles", "bake_c5_1024_pmc.md"), "k_search<1, float>"),
+                "traffic": node_traffic,
                 "kernel": "the band nodes' search: " + " + ".join(node_stages), "kernel_ms": node_ms, "useful_flop": flop,
                 "useful_flop_per_band_brick": c5_useful["node_flop_per_band_brick"],
                 "hbm": {"algorithmic_bytes": 1464.0 * cm["slots"], "achieved_GBps": 1464.0 * cm["slots"] / (cm["ms"] / 1e3) / 1e9,
                         "peak_GBps": PEAK_HBM / 1e9, "frac": 1464.0 * cm["slots"] / (cm["ms"] / 1e3) / PEAK_HBM},
-                "valu_measured": pmc_valu(os.path.join(ROOT, "profiles", "bake_c5_1024_valu.md"), "k_search<1, float>")}
+                "valu_measured": {k: pmc_valu(os.path.join(ROOT, "profiles", "bake_c5_1024_valu.md"), k) for k in ("k_search<1, float>", "k_settle<1>")}}
     out = {
         "metric": "SDF queries/s + bricks/s, FP32+FP64-refine vs FP64 at <=1e-6 rel err, 1 GPU",
         "value": cm["bricks_per_s"],

@@ -47,12 +47,15 @@ namespace {
 
 constexpr int WG = 256;
 // The settle kernels' grid: as many blocks as the device runs at once (settle_grid,
-// from the occupancy the compiled kernel allows: one wave a SIMD). Their
-// lanes hold the certified search's queue in private memory and run long, so blocks
-// past that would wait for a slot and take their share of the queue late.
+// from the occupancy the compiled kernel allows, at most one wave a SIMD: the compiler
+// gives the settle 256 registers and 256 accumulation registers a lane). Their lanes
+// hold the certified search's queue in private memory (10 KB a lane) and run long, so
+// blocks past that would wait for a slot and take their share of the queue late, and
+// every wave the grid holds is scratch the runtime reserves for the launch.
 template <class K> int settle_grid(K kernel, int nt, int cus) {
   int per = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, nt, 0) != hipSuccess || per < 1) per = 1;
+  per = std::min(per, std::max(1, 4 * 64 / nt));
   return per * std::max(cus, 1);
 }
 constexpr int SORTC = 1024;  // lists up to this long are sorted nearest first
@@ -2395,7 +2398,9 @@ template <int S>
 int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Stages* st, long blocks) {
   if (j.n <= 0) return 0;
   constexpr int NT = Shape<S>::NT;
-  const unsigned gfar = unsigned(std::min<long>(blocks * 16, 4096));
+  // (the far points' kernel runs one block a CU at a time; a larger grid only reserves
+  // more scratch for its spills)
+  const unsigned gfar = unsigned(std::min<long>(blocks * 16, 1024));
   static const char* names[4][8] = {{"points_search32", "points_refine64", "points_fallback64", "points_search64", "points_leaves32", "points_leaves64",
                                      "points_mark", "points_settle64"},
                                     {"nodes_search32", "nodes_refine64", "nodes_fallback64", "nodes_search64", "nodes_leaves32", "nodes_leaves64",
