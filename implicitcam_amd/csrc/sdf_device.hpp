@@ -921,9 +921,9 @@ template <class T, class P> __device__ inline int patch_kind(P cb, const T el[3]
 }
 // The box of S - p over a part for the interval tests: in single precision grown by the
 // point's own rounding (p is the query rounded to single precision, up to half a unit in
-// its last place from the point the answer is for; a test decided within that margin
-// passed over the part holding the large mixed solid's nearest point, 0.6 mm nearer than
-// the answer) and by the search's rounding allowance eps / 8; exact in double precision.
+// its last place from the point the answer is for, so a test must hold for every point
+// that near) and by an eighth of the search's rounding allowance; exact in double
+// precision.
 template <class T> __device__ inline void rel_box(const ICAM_K float* box, v3<T> p, T eps, T el[3], T eh[3]) {
   const T w = sizeof(T) == 4 ? T(0.125) * eps + (mabs(p.x) + mabs(p.y) + mabs(p.z)) * T(1.2e-7) : T(0);
   el[0] = T(box[0]) - p.x - w, el[1] = T(box[1]) - p.y - w, el[2] = T(box[2]) - p.z - w;
@@ -1334,16 +1334,19 @@ __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box,
   T s0 = lf[1], s1 = lf[2], t0 = lf[3], t1 = lf[4];
   T s = T(0.5) * (s0 + s1), t = T(0.5) * (t0 + t1);
   const bool single = sizeof(T) == 4;
-  const bool conv = newton2(sp, p, s, t, s0, s1, t0, t1, single ? 8 : 32, single ? T(1e-5) : T(1e-14));
+  const bool conv = newton2(sp, p, s, t, s0, s1, t0, t1, single ? 16 : 32, single ? T(1e-5) : T(1e-14));
   SEval<T> e = peval1(sp, s, t);
   v3<T> r = e.p - p;
   T d = len(r);
   if (!certified) b.lmin = mmin(b.lmin, lb);
-  // a certified part whose single-precision Newton did not settle within its 8 steps:
-  // its one minimum is not read, and another part's answer must not stand for it, so
-  // the point is searched again in double precision (32 steps; there a step under 1e-14
-  // is not always reached in rounding, and the point it rests on is the minimum)
-  if (single && certified && !conv && lb < b.d) b.unsure = true;
+  T tol = single ? T(1e-4) * d + eps : T(1e-9) * (T(1) + d);
+  const bool crit = is_critical(r, e.ps, tol) && is_critical(r, e.pt, tol);
+  // a certified part whose single-precision Newton neither settled within its 16 steps
+  // nor rests on a critical point: its one minimum is not read, and another part's answer
+  // must not stand for it, so the point is searched again in double precision (32 steps;
+  // there a step under 1e-14 is not always reached in rounding, and the point it rests on
+  // is the minimum)
+  if (single && certified && !conv && !crit && lb < b.d) b.unsure = true;
   if (!(d < b.d)) return;
   // a foot at a pole the sub-patch touches: the pole's point record answers there
   // (its normal comes from the fan; the surface's own degenerates)
@@ -1351,8 +1354,7 @@ __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box,
     const v3<T> x = ld3(lf + LEAF_POLE);
     if (len(e.p - x) <= (sizeof(T) == 4 ? T(1e-6) : T(1e-9)) * (T(1) + len(x))) return;
   }
-  T tol = single ? T(1e-4) * d + eps : T(1e-9) * (T(1) + d);
-  if (!(is_critical(r, e.ps, tol) && is_critical(r, e.pt, tol))) return;
+  if (!crit) return;
   const uint32_t fi = lu[1];
   const ICAM_K T* f = tb.F + L.face_f + FACE_F * fi;
   if (!lu[2]) {
@@ -1496,14 +1498,15 @@ __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box
   v3<T> r = e.p - p;
   T d = len(r);
   if (!certified) b.lmin = mmin(b.lmin, lb);
-  if (single && certified && !conv && lb < b.d) b.unsure = true;
+  T tol = single ? T(1e-4) * d + eps : T(1e-9) * (T(1) + d);
+  const bool crit = is_critical(r, e.d1, tol);
+  if (single && certified && !conv && !crit && lb < b.d) b.unsure = true;
   if (!(d < b.d)) return;
   // an end of the edge is its vertex's (a C0 line's ends are in its face's boundary,
   // whose edges answer there)
   if (!crease && ((pu[1] == 0 && s <= T(0)) || (pu[1] + 1 == pu[2] && s >= T(1)))) return;
-  T tol = single ? T(1e-4) * d + eps : T(1e-9) * (T(1) + d);
   const bool junction = (jlo && s <= T(0)) || (jhi && s >= T(1));
-  if (!junction && !is_critical(r, e.d1, tol)) return;
+  if (!junction && !crit) return;
   if (crease) {
     bool near = false;
     const bool in = crease_inside(tb, eu, cp, s, tm, near);

@@ -790,6 +790,17 @@ struct Nurbs {
   std::vector<double> su0, sv0;  // span starts, plus the end
   std::vector<double> knots_u, knots_v;
   Box box;
+  // surface3.rs:579-591, 1378-1388 Creases: the parameter values of the sides the surface
+  // collapses on (a patch touching one is never cleared by the critical-point test)
+  std::vector<double> crease_u, crease_v;
+  bool touches(const Patch& pa) const {
+    auto on = [](double x, const std::vector<double>& lines, double a, double b) {
+      for (double c : lines)
+        if (std::fabs(x - c) <= 1e-12 * std::max(std::fabs(b - a), std::fabs(c))) return true;
+      return false;
+    };
+    return on(pa.u0, crease_u, pa.u0, pa.u1) || on(pa.u1, crease_u, pa.u0, pa.u1) || on(pa.v0, crease_v, pa.v0, pa.v1) || on(pa.v1, crease_v, pa.v0, pa.v1);
+  }
 
   void build(const icam_face_desc& f) {
     du = f.du, dv = f.dv;
@@ -954,7 +965,7 @@ struct Nurbs {
       }
       double slab = slab_bound(pa, p, d > 1e-12 ? (p - e.p) * (1.0 / d) : n);
       if (slab >= best_d - DIST_EPS || it.depth >= 40) continue;
-      if (no_critical_point(pa, p)) continue;
+      if (!touches(pa) && no_critical_point(pa, p)) continue;
       Patch q[4];
       pa.quarters(q);
       for (int k = 0; k < 4; ++k) {
@@ -1515,6 +1526,7 @@ struct Exact {
       if (fd.kind == ICAM_NURBS) {
         f.ns.reset(new Nurbs());
         f.ns->build(fd);
+        for (const Collapsed& c : collapsed_sides(*f.ns)) (c.k == 0 ? f.ns->crease_u : f.ns->crease_v).push_back(c.value);
       } else {
         Analytic& a = f.an;
         a.kind = int(fd.kind);
