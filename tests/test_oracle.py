@@ -479,3 +479,18 @@ def test_c3_closed_form(built):
         assert not (flags & 1).any()
         assert np.abs(d - ref).max() < 1e-9
         assert ((d < 0) == (ref < 0))[np.abs(ref) > 1e-9].all()
+
+
+def test_c5_span_border_tie_keeps_the_face(built):
+    """A node of the large solid's 1024-node level whose foot on a pocket wall lies within
+    rounding of a knot line: the first span's Newton stops at the border (not critical)
+    and sets the search's bound, the next span's critical point ties it. The oracle takes
+    the tie (the reference's search, surface3.rs:1276-1283, would answer with the floor
+    0.165 mm farther), so its answer is the wall's, as the wall's untrimmed search finds."""
+    part = Part("c5")
+    o = Oracle(part)
+    x = np.array([[135.13111545988255, -113.02544031311155, 19.334637964774952]])
+    d, feat, flags = o.sdf(x)
+    wall = o.nurbs_closest(163, x[0])
+    assert wall is not None and abs(abs(d[0]) - wall[0]) < 1e-12
+    assert feat[0] == 163 and d[0] < 0 and abs(d[0]) < 0.6
