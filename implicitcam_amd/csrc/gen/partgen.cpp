@@ -29,6 +29,9 @@
 //   "crease:<k>" the c2 block on a 17 x 17 net with a knot of multiplicity three at
 //           0.5 along u (k even) or v (k odd): the top folds along that knot line, a
 //           ridge of height 12 (a valley for k = 2, 3) fading to zero at the boundary
+//   "quintic:<k>" an 80 x 80 block with a rational degree-(5, 5) top whose outer ring
+//           of control points is flat (so its boundary curves are straight): top
+//           edges as lines (k = 0) or as the boundary rows' degree-5 pieces (k = 1)
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -713,6 +716,126 @@ Owned* make_c2(uint64_t seed, bool flat, int n = 16, double half = 60, double am
   }
   double zmax = -1e300;
   for (double z : zz) zmax = std::max(zmax, z);
+  o->finish({-half, -half, zb}, {half, half, zmax}, 1e-9);
+  return o;
+}
+
+// ------------------------------------------------------------- quintic top
+
+// An 80 x 80 block whose top is a rational B-spline of degree (5, 5) on a 10 x 10 net
+// (clamped uniform knots, five spans a direction, control points at the Greville
+// abscissae), interior heights from the seeded noise scaled to 4 mm and interior
+// weights in [0.8, 1.2]. The net's outer ring lies at the top's base height with unit
+// weights, so each boundary curve is the straight segment between two top corners
+// (a B-spline reproduces a linear function from its Greville abscissae). The four top
+// edges are lines (spline_edges false) or the boundary rows' degree-5 Bezier pieces
+// (true); the sides and the bottom are planes.
+Owned* make_quintic(uint64_t seed, bool spline_edges) {
+  Owned* o = new Owned();
+  Builder b{o};
+  const int p = 5, n = 10;
+  const double half = 40, zb = -half / 3, ztop = half / 3, amp = 4;
+  std::vector<double> kn;
+  for (int i = 0; i <= p; ++i) kn.push_back(0);
+  for (int i = 1; i < n - p; ++i) kn.push_back(double(i) / (n - p));
+  for (int i = 0; i <= p; ++i) kn.push_back(1);
+  std::vector<double> br;  // the distinct knots
+  for (double x : kn)
+    if (br.empty() || x > br.back()) br.push_back(x);
+  std::vector<double> g(n);
+  for (int i = 0; i < n; ++i) {
+    double sum = 0;
+    for (int k = 1; k <= p; ++k) sum += kn[i + k];
+    g[i] = -half + 2 * half * sum / p;
+  }
+  Rng rng{seed ? seed : 1};
+  std::vector<double> raw(n * n), zz(n * n), pw(n * n * 4);
+  double peak = 1e-300;
+  for (int i = 1; i < n - 1; ++i)
+    for (int j = 1; j < n - 1; ++j) raw[i * n + j] = value_noise(seed, g[i], g[j]), peak = std::max(peak, std::fabs(raw[i * n + j]));
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const bool ring = i == 0 || j == 0 || i == n - 1 || j == n - 1;
+      const double z = ring ? ztop : ztop + amp * raw[i * n + j] / peak;
+      const double w = ring ? 1.0 : 0.8 + 0.4 * rng.next();
+      zz[i * n + j] = z;
+      double* q = &pw[(i * n + j) * 4];
+      q[0] = g[i] * w, q[1] = g[j] * w, q[2] = z * w, q[3] = w;
+    }
+  icam_face_desc top;
+  std::memset(&top, 0, sizeof(top));
+  top.kind = ICAM_NURBS;
+  top.sense = 1.0;
+  top.du = top.dv = p;
+  top.nu = top.nv = n;
+  top.pw = o->keep(pw);
+  top.ku = o->keep(kn);
+  top.kv = o->keep(kn);
+  b.set_trim(top, 0, 0, 1, 1, 0, 0, {}, 0.5, 0.5, true);
+  o->faces.push_back(top);
+  const uint32_t ft = 0;
+  V X{1, 0, 0}, Y{0, 1, 0}, Z{0, 0, 1};
+  // side k: 0 y = -half, 1 x = +half, 2 y = +half, 3 x = -half; the plane's
+  // parameters (along its x axis, z), each side a rectangle
+  uint32_t side_face[4];
+  for (int k = 0; k < 4; ++k) {
+    V nrm, xa;
+    if (k == 0) nrm = Y * -1.0, xa = X;
+    if (k == 1) nrm = X, xa = Y;
+    if (k == 2) nrm = Y, xa = X * -1.0;
+    if (k == 3) nrm = X * -1.0, xa = Y * -1.0;
+    std::vector<Piece> loop;
+    add_polyline(loop, {{-half, zb}, {half, zb}, {half, ztop}, {-half, ztop}}, true);
+    side_face[k] = b.plane(nrm * half, nrm, xa, 1.0, -half - 1, zb - 1, half + 1, ztop + 1, loop, 0.0123, zb - 0.5, false);
+  }
+  std::vector<Piece> bot;
+  add_polyline(bot, {{-half, -half}, {half, -half}, {half, half}, {-half, half}}, true);
+  const uint32_t fb = b.plane({0, 0, zb}, Z * -1.0, X, 1.0, -half - 1, -half - 1, half + 1, half + 1, bot, -half - 0.5, 0.0123, false);
+  V bc[4] = {{-half, -half, zb}, {half, -half, zb}, {half, half, zb}, {-half, half, zb}};
+  V tc[4] = {{-half, -half, ztop}, {half, -half, ztop}, {half, half, ztop}, {-half, half, ztop}};
+  // top edge k from tc[k] to tc[k + 1], and where it runs on the top: side 0 v = 0
+  // (u up), 1 u = 1 (v up), 2 v = 1 (u down), 3 u = 0 (v down)
+  const double ua[4][4] = {{0, 0, 1, 0}, {1, 0, 1, 1}, {1, 1, 0, 1}, {0, 1, 0, 0}};
+  for (int k = 0; k < 4; ++k) {
+    if (!spline_edges) {
+      const double uv[8] = {ua[k][0], ua[k][1], ua[k][2], ua[k][3], NaN, NaN, NaN, NaN};
+      b.line(tc[k], tc[(k + 1) % 4], ft, side_face[k], o->keep(std::vector<double>(uv, uv + 8)));
+      continue;
+    }
+    // the boundary row in the edge's direction, cut into its Bezier pieces
+    std::vector<double> pts;
+    for (int m = 0; m < n; ++m) {
+      const int t = (k >= 2) ? n - 1 - m : m;
+      const int i = k == 0 ? t : k == 1 ? n - 1 : k == 2 ? t : 0;
+      const int j = k == 0 ? 0 : k == 1 ? t : k == 2 ? n - 1 : t;
+      pts.insert(pts.end(), {g[i], g[j], zz[i * n + j], 1.0});
+    }
+    // (the clamped uniform knots are symmetric: a reversed row has the same)
+    auto pcs = decompose(p, kn, pts, 4);
+    std::vector<double> uv;
+    for (int c = 0; c < int(pcs.size()); ++c) {
+      const double a = double(c) / pcs.size(), e = double(c + 1) / pcs.size();
+      auto lerp = [&](double t, int w) { return ua[k][w] + t * (ua[k][w + 2] - ua[k][w]); };
+      uv.insert(uv.end(), {lerp(a, 0), lerp(a, 1), lerp(e, 0), lerp(e, 1), NaN, NaN, NaN, NaN});
+    }
+    b.spline(p, pcs, ft, side_face[k], uv);
+  }
+  for (int k = 0; k < 4; ++k) {
+    b.line(bc[k], bc[(k + 1) % 4], fb, side_face[k]);
+    b.line(bc[k], tc[k], side_face[(k + 3) % 4], side_face[k]);
+  }
+  // vertices: every corner meets three faces at right angles
+  const double q = M_PI / 2;
+  for (int k = 0; k < 4; ++k) {
+    const double sx = (k == 1 || k == 2) ? 1.0 : -1.0, sy = k >= 2 ? 1.0 : -1.0;
+    b.vert(bc[k], V{sx, sy, -1.0} * q);
+    b.vert(tc[k], V{sx, sy, 1.0} * q);
+  }
+  double zmax = ztop;
+  for (double z : zz) zmax = std::max(zmax, z);
+  o->info = {5.0, double(p), double(n)};
+  o->info.insert(o->info.end(), kn.begin(), kn.end());
+  o->info.insert(o->info.end(), pw.begin(), pw.end());
   o->finish({-half, -half, zb}, {half, half, zmax}, 1e-9);
   return o;
 }
@@ -1846,6 +1969,7 @@ const icam_part_desc* icam_gen_part(const char* name, uint64_t seed) {
   }
   if (s.rfind("nurbs:", 0) == 0) o = make_c2(seed, false, std::min(std::max(arg(6), 4), 32), 40, 3);
   if (s.rfind("crease:", 0) == 0) o = make_c2(seed, false, 17, 60, 5, arg(7) & 3);
+  if (s.rfind("quintic:", 0) == 0) o = make_quintic(seed, arg(8) != 0);
   if (s.rfind("oblique:", 0) == 0) {
     Rng r2{(seed ^ (0x9E3779B97F4A7C15ull * uint64_t(arg(8) + 1))) | 1};
     const double R = 5 + 10 * r2.next(), L = 2 * R + 20 * r2.next(), a = (15 + 45 * r2.next()) * M_PI / 180;
@@ -1859,7 +1983,9 @@ void icam_gen_free(const icam_part_desc* p) { delete reinterpret_cast<const Owne
 // What a part was made from, for closed-form checks: [1, n, n elements] for a solid of
 // revolution about z, [2, W, D, T, k, then per hole x, y, n, n elements] for a plate,
 // [3, W, D, T, loops, then per loop kind (0 pocket, 1 boss), z0, z1, F, then F x 16
-// control points (x, y)] for the trimmed plate, [] otherwise; an element is (arc, r0,
+// control points (x, y)] for the trimmed plate, [4, R, L, tan a] for the oblique cut,
+// [5, p, n, the n + p + 1 knots (both directions), the n x n homogeneous net] for the
+// quintic top, [] otherwise; an element is (arc, r0,
 // z0, r1, z1, rc, zc, rad, psi0, psi1, face).
 // Writes at most cap values and returns how many there are.
 size_t icam_gen_describe(const icam_part_desc* p, double* out, size_t cap) {

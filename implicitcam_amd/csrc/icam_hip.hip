@@ -590,14 +590,14 @@ template <class T> __device__ inline void item_put(uint32_t* w, int off, T v) { 
 template <class T> __device__ inline T item_get(const uint32_t* w, int off) { return *reinterpret_cast<const T*>(w + off); }
 template <class T> __device__ inline void item_state(uint32_t* w, const Best<T>& b) {
   constexpr int u = int(sizeof(T) / 4);
-  w[2] = b.code, w[3] = b.unsure ? 1u : 0u;
+  w[2] = b.code, w[3] = (b.unsure ? 1u : 0u) | (b.host ? 2u : 0u);
   item_put(w, 4, b.d), item_put(w, 4 + u, b.a), item_put(w, 4 + 2 * u, b.b), item_put(w, 4 + 3 * u, b.sgn);
   item_put(w, 4 + 4 * u, b.lmin), item_put(w, 4 + 5 * u, b.d2), item_put(w, 4 + 6 * u, b.sgn2);
 }
 template <class T> __device__ inline Best<T> item_best(const uint32_t* w) {
   constexpr int u = int(sizeof(T) / 4);
   Best<T> b;
-  b.code = w[2], b.unsure = w[3] != 0;
+  b.code = w[2], b.unsure = (w[3] & 1u) != 0, b.host = (w[3] & 2u) != 0;
   b.d = item_get<T>(w, 4), b.a = item_get<T>(w, 4 + u), b.b = item_get<T>(w, 4 + 2 * u), b.sgn = item_get<T>(w, 4 + 3 * u);
   b.lmin = item_get<T>(w, 4 + 4 * u), b.d2 = item_get<T>(w, 4 + 5 * u), b.sgn2 = item_get<T>(w, 4 + 6 * u);
   return b;
@@ -605,7 +605,7 @@ template <class T> __device__ inline Best<T> item_best(const uint32_t* w) {
 
 template <class T> __device__ inline Best<T> best_init(T bound) {
   Best<T> b;
-  b.d = bound, b.code = NONE, b.a = b.b = T(0), b.sgn = T(1), b.lmin = TypeEps<T>::big, b.d2 = TypeEps<T>::big, b.sgn2 = T(1), b.unsure = false;
+  b.d = bound, b.code = NONE, b.a = b.b = T(0), b.sgn = T(1), b.lmin = TypeEps<T>::big, b.d2 = TypeEps<T>::big, b.sgn2 = T(1), b.unsure = false, b.host = false;
   return b;
 }
 // One candidate list's share of a point's search, the lane's answer b carried in and
@@ -726,6 +726,7 @@ template <class T, int K> __device__ inline void group_merge(Best<T>& b, T eps) 
     tie = group_or<K>(tie);
     b.lmin = group_min<K>(b.lmin);
     b.unsure = group_or<K>(b.unsure ? 1 : 0) != 0;
+    b.host = group_or<K>(b.host ? 1 : 0) != 0;
     b.code = __shfl(b.code, src), b.a = __shfl(b.a, src), b.b = __shfl(b.b, src);
     b.d = dmin, b.sgn = sw;
     b.d2 = tie ? dmin : TypeEps<T>::big, b.sgn2 = tie ? -sw : sw;
@@ -759,7 +760,8 @@ __device__ inline uint32_t feat_of(const Dev& dv, uint32_t code) {
 
 // A finished double-precision search as (value, flag): exact when a feature answered
 // and no undecided part had a lower bound under it; a bound when nothing came within
-// the bound; undecided (the least lower bound) otherwise.
+// the bound; left to the caller (flag 3, the least lower bound) when a host feature
+// came under the answer; else undecided (flag 2, the least lower bound).
 __device__ inline void classify64(const Best<double>& b, double bound, const Dev& dv, double& val, uint32_t& flag, uint32_t& feat) {
   feat = NONE;
   if (b.code != NONE && !(b.lmin < b.d) && b.d <= bound) {
@@ -771,7 +773,7 @@ __device__ inline void classify64(const Best<double>& b, double bound, const Dev
     val = bound, flag = 1;
     return;
   }
-  val = l, flag = 2;
+  val = l, flag = b.host ? 3 : 2;
 }
 
 // Block shapes: explicit points 256 a block, one lane each; a brick's 729 nodes a
@@ -1479,7 +1481,9 @@ __device__ __forceinline__ void decide_brick(const Dev& dv, const Lv& lv, long b
     s = v < 0 ? -1 : 1;
   } else {
     dc[b] = __builtin_nan("");
-    band[b] = (f == 2 && v <= reach) ? 1 : 0;
+    // undecided under the reach: kept; its centre left to the host (flag 3): kept, and
+    // the brick goes to the host whole (band 2)
+    band[b] = ((f == 2 || f == 3) && v <= reach) ? (f == 3 ? 2 : 1) : 0;
     // a centre outside the part's box is outside the part
     v3<double> c = brick_first(lv, dv.origin, b) + mk3(4.0 * lv.h, 4.0 * lv.h, 4.0 * lv.h);
     const float* L0 = dv.L.lo;
@@ -1669,20 +1673,28 @@ __device__ inline uint64_t dilate_z(const uint64_t* x, const uint64_t* inside, i
   const uint64_t dn = (mword(x, w + 1) >> 17) | (mword(x, w + 2) << 47);
   return (x[w] | up | dn) & inside[w];
 }
-__device__ __forceinline__ void finish_brick(const Lv& lv, const uint32_t* list, long bi, const double* dcs, const int8_t* sgns, const double* val_in,
-                                    const uint32_t* flag_in, uint32_t* staging, uint32_t* cls, unsigned long long* counters, FinishLds& sm) {
+__device__ __forceinline__ void finish_brick(const Lv& lv, const uint32_t* list, long bi, const double* dcs, const int8_t* sgns, const uint8_t* bands,
+                                    const double* val_in, const uint32_t* flag_in, uint32_t* staging, uint32_t* cls, unsigned long long* counters,
+                                    FinishLds& sm) {
   double* val = sm.val;
   uint8_t* st = sm.st;
   int& changed = sm.changed;
   int16_t* q = sm.q;
   const int tid = threadIdx.x;
   const double h = lv.h, clamp = h * 1.7320508075688772, half = 4.0 * h;
+  int host = 0;
   for (int n = tid; n < 729; n += WG) {
     val[n] = val_in[729 * bi + n];
     uint32_t f = flag_in[729 * bi + n];
     st[n] = uint8_t(f > 2 ? 2 : f);
+    host |= f == 3u;
   }
-  __syncthreads();
+  // gpu.rs:714-716, partcache.rs:794-797 a brick with a node (or its centre) left to
+  // the host is read there whole: class 3, no words
+  if (__syncthreads_or(host | (bands[list[bi]] == 2))) {
+    if (tid == 0) cls[bi] = 3u, atomicAdd(&counters[7], 1ull);
+    return;
+  }
   // partcache.rs:825-855 far nodes take the sign of any neighbour known, by flood fill. A
   // far node is farther from the boundary than from any of its neighbours, so all its
   // known neighbours share its sign, and the fill is the dilation of the nodes known
@@ -1784,16 +1796,16 @@ __device__ __forceinline__ void finish_brick(const Lv& lv, const uint32_t* list,
   if (tid == 0) q[729] = 0;
   any_nonpos = __syncthreads_or(any_nonpos);
   any_pos = __syncthreads_or(any_pos);
-  uint32_t c = !any_nonpos ? 1u : !any_pos ? 2u : 0u;  // 1 AIR, 2 SOLID, 0 words
+  uint32_t c = !any_nonpos ? 1u : !any_pos ? 2u : 0u;  // 1 AIR, 2 SOLID, 0 words (3 host)
   if (tid == 0) cls[bi] = c;
   if (c == 0)
     for (int w = tid; w < ICAM_WORDS; w += WG) staging[size_t(bi) * ICAM_WORDS + w] = uint32_t(uint16_t(q[2 * w])) | (uint32_t(uint16_t(q[2 * w + 1])) << 16);
 }
 __global__ __launch_bounds__(WG) void k_bricks_finish(Lv lv, const uint32_t* list, long nbricks, const double* dcs, const int8_t* sgns,
-                                                      const double* val_in, const uint32_t* flag_in, uint32_t* staging, uint32_t* cls,
+                                                      const uint8_t* bands, const double* val_in, const uint32_t* flag_in, uint32_t* staging, uint32_t* cls,
                                                       unsigned long long* counters) {
   __shared__ FinishLds sm;
-  if (long(blockIdx.x) < nbricks) finish_brick(lv, list, blockIdx.x, dcs, sgns, val_in, flag_in, staging, cls, counters, sm);
+  if (long(blockIdx.x) < nbricks) finish_brick(lv, list, blockIdx.x, dcs, sgns, bands, val_in, flag_in, staging, cls, counters, sm);
 }
 
 
@@ -1814,7 +1826,7 @@ __global__ void k_store(long nbricks, const uint8_t* band, const int8_t* sgn, ui
 __device__ __forceinline__ void store_band_brick(long bi, const uint32_t* list, const uint32_t* cls, const uint32_t* slot, const uint32_t* staging, uint32_t* map,
                                         uint32_t* atlas) {
   uint32_t c = cls[bi];
-  if (threadIdx.x == 0) map[list[bi]] = c == 1 ? ICAM_AIR : c == 2 ? ICAM_SOLID : slot[bi];
+  if (threadIdx.x == 0) map[list[bi]] = c == 1 ? ICAM_AIR : c == 2 ? ICAM_SOLID : c == 3 ? ICAM_HOST : slot[bi];
   if (c == 0)
     for (int w = threadIdx.x; w < ICAM_WORDS; w += blockDim.x) atlas[size_t(slot[bi]) * ICAM_WORDS + w] = staging[size_t(bi) * ICAM_WORDS + w];
 }
@@ -2166,8 +2178,8 @@ __global__ __launch_bounds__(WG) void kb_finish(BatchArgs a) {
     const int p = __builtin_amdgcn_readfirstlane(part_of<uint32_t>(a.band_off, a.n, w));
     const ICAM_K BPart& P = bpart(a, p);
     const Lv lv = ldk(P.lv);
-    finish_brick(lv, P.list, long(w - __builtin_amdgcn_readfirstlane(a.band_off[p])), P.dc, P.sgn, P.rn.val, P.rn.flag, P.staging, P.cls, P.counters,
-                 sm);
+    finish_brick(lv, P.list, long(w - __builtin_amdgcn_readfirstlane(a.band_off[p])), P.dc, P.sgn, P.band, P.rn.val, P.rn.flag, P.staging, P.cls,
+                 P.counters, sm);
     __syncthreads();
   }
 }
@@ -2917,7 +2929,8 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     HIP_OK(launch(&st, "centres_sign", s, dim3(unsigned(std::min<long>((long(nband) + 255) / 256, 4096))), dim3(256), k_centre_sign, r,
                   (const uint32_t*)h->list.as<uint32_t>(), long(nband), h->dc.as<double>(), h->sgn.as<int8_t>()));
     HIP_OK(launch(&st, "bricks_finish", s, dim3(nband), dim3(WG), k_bricks_finish, lv, (const uint32_t*)h->list.as<uint32_t>(), long(nband),
-                  (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>(), (const double*)rn.val, (const uint32_t*)rn.flag,
+                  (const double*)h->dc.as<double>(), (const int8_t*)h->sgn.as<int8_t>(), (const uint8_t*)h->band.as<uint8_t>(),
+                  (const double*)rn.val, (const uint32_t*)rn.flag,
                   h->staging.as<uint32_t>(), h->cls.as<uint32_t>(), h->counters.as<unsigned long long>()));
     HIP_OK(launch(&st, "slots", s, dim3(1), dim3(SCAN_T), k_scan, (const uint8_t*)nullptr, (const uint32_t*)h->cls.as<uint32_t>(), long(nband),
                   h->slot.as<uint32_t>(), (uint32_t*)nullptr, h->stotal.as<uint32_t>()));
@@ -2943,7 +2956,7 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
     stats->bricks = nb, stats->band = nband, stats->slots = nslots;
     stats->undecided = int64_t(cnt[0]), stats->refined = int64_t(cnt[1]);
     stats->unsure_trim = int64_t(cnt[3]), stats->unsure_tie = int64_t(cnt[4]), stats->settled = int64_t(cnt[5]);
-    stats->overflowed = int64_t(cnt[6]);
+    stats->overflowed = int64_t(cnt[6]), stats->host = int64_t(cnt[7]);
   }
   return 0;
 }
@@ -3346,7 +3359,7 @@ int icam_hip_batch_result(icam_batch* b, size_t i, const uint32_t** d_map, const
     stats->bricks = P.nb, stats->band = bt, stats->slots = sl;
     stats->undecided = int64_t(cnt[0]), stats->refined = int64_t(cnt[1]);
     stats->unsure_trim = int64_t(cnt[3]), stats->unsure_tie = int64_t(cnt[4]), stats->settled = int64_t(cnt[5]);
-    stats->overflowed = int64_t(cnt[6]);
+    stats->overflowed = int64_t(cnt[6]), stats->host = int64_t(cnt[7]);
   }
   return 0;
 }

@@ -11,13 +11,19 @@
 // outward) and a code kind << 28 | record, is what the kernels cull against a
 // brick or a tile of points. Kinds follow the reference's candidate codes
 // (crates/icam-core/src/solid3/gpu.rs:47-51): analytic face, line or arc edge,
-// vertex, B-spline sub-patch (leaf of its span's tree), curve-piece leaf.
+// vertex, B-spline sub-patch (leaf of its span's tree), curve-piece leaf; and a
+// feature the device does not read (a host feature, the reference's Code::Cpu,
+// gpu.rs:55-58, 714-716): a B-spline face past bicubic, a spline edge past cubic, an
+// ellipse edge on a B-spline face, or an edge of a host face. A host feature's box
+// within a point's answer leaves the point to the caller (flag 3).
 #pragma once
 #include <stdint.h>
 
 namespace icam {
 
-enum PrimKind : uint32_t { P_FACE = 0, P_EDGE = 1, P_VERT = 2, P_LEAF = 3, P_CLEAF = 4 };
+enum PrimKind : uint32_t { P_FACE = 0, P_EDGE = 1, P_VERT = 2, P_HOST = 3, P_LEAF = 4, P_CLEAF = 5 };
+// a host feature's code: P_HOST << 28 | the face, or HOST_EDGE | the edge
+constexpr uint32_t HOST_EDGE = 1u << 27;
 
 // FACE: analytic face (F words)
 enum : uint32_t {

@@ -906,9 +906,28 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
     }
   };
 
+  // Host features, which the device does not read and a point within reach of which is
+  // left to the caller (the reference's Code::Cpu, crates/icam-core/src/solid3/gpu.rs:
+  // 271, 287-289, 714-716): a B-spline face past bicubic, a spline edge past cubic, an
+  // ellipse edge on a B-spline face (its pieces carry no parameter map), and every edge
+  // of a host face (an edge's sign takes its faces' normals). Each is one primitive with
+  // its box (a B-spline's from its control points: positive weights keep the surface in
+  // their hull).
+  if (part->n_faces >= HOST_EDGE || part->n_edges >= HOST_EDGE) {
+    P.error = "too many faces or edges";
+    return false;
+  }
+  auto host_face = [&](uint32_t f) {
+    return f < part->n_faces && part->faces[f].kind == ICAM_NURBS && (part->faces[f].du > 3 || part->faces[f].dv > 3);
+  };
+  std::vector<uint8_t> host_edge(part->n_edges, 0);
+
   // ---- edges first (face boxes use them); their records and pieces
   for (uint32_t e = 0; e < part->n_edges; ++e) {
     const icam_edge_desc& ed = part->edges[e];
+    const bool host = (ed.kind == ICAM_SPLINE && ed.degree > 3) || (ed.kind == ICAM_ELLIPSE && ed.uvmap) || host_face(ed.faces[0]) ||
+                      host_face(ed.faces[1]);
+    host_edge[e] = host ? 1 : 0;
     double rec[EDGE_F] = {0};
     uint32_t ru[EDGE_U] = {0};
     ru[0] = ed.kind, ru[1] = ed.faces[0], ru[2] = ed.faces[1], ru[3] = 0xffffffffu;
@@ -935,41 +954,45 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
       // curve3.rs:986-1008 bez_pieces: k arcs of at most a quarter turn, each the rational
       // quadratic through its ends with its middle point pushed out by 1 / cos(half its
       // turn) and weighted cos(half its turn)
-      if (ed.uvmap) {
-        P.error = "ellipse edge on a B-spline face (its pieces carry no parameter map)";
-        return false;
-      }
       V c = rel(ed.o), x = of(ed.x), z = of(ed.z), y = cross(z, x);
       const int k = std::max(1, int(std::ceil(std::fabs(ed.sweep) / (0.5 * PI))));
       const double dt = ed.sweep / k, phi = 0.5 * dt, w = std::cos(phi);
       auto at = [&](double t, double sc) { return c + x * (ed.r * std::cos(t) * sc) + y * (ed.r2 * std::sin(t) * sc); };
-      ru[4] = uint32_t(cpiece_f.size() / CPIECE_F), ru[5] = uint32_t(k);
+      ru[4] = uint32_t(cpiece_f.size() / CPIECE_F), ru[5] = host ? 0u : uint32_t(k);
       for (int i = 0; i < k; ++i) {
         const double ta = ed.t0 + dt * i;
         const V a = at(ta, 1.0), m = at(ta + phi, 1.0 / w), b = at(ta + dt, 1.0);
         std::vector<H> pts = {H{a.x, a.y, a.z, 1.0}, H{m.x * w, m.y * w, m.z * w, w}, H{b.x, b.y, b.z, 1.0}};
         pts = raise(pts);
+        for (auto& h : pts) bx.add(point(h));
+        if (host) continue;
         const uint32_t cp = uint32_t(cpiece_f.size() / CPIECE_F);
         for (auto& h : pts) cpiece_f.insert(cpiece_f.end(), h.begin(), h.end());
         cpiece_u.insert(cpiece_u.end(), {e, uint32_t(i), uint32_t(k), 0});
         curve_leaves(pts, cp, e);
-        for (auto& h : pts) bx.add(point(h));
       }
     } else {
-      if (ed.degree > 3 || ed.degree < 1) {
-        P.error = "spline edge of degree " + std::to_string(ed.degree) + " (at most 3 are packed)";
+      if (ed.degree < 1) {
+        P.error = "spline edge of degree " + std::to_string(ed.degree);
         return false;
       }
-      ru[4] = uint32_t(cpiece_f.size() / CPIECE_F), ru[5] = ed.n_pieces;
+      ru[4] = uint32_t(cpiece_f.size() / CPIECE_F), ru[5] = host ? 0u : ed.n_pieces;
       for (uint32_t k = 0; k < ed.n_pieces; ++k) {
         std::vector<H> pts;
         for (uint32_t i = 0; i <= ed.degree; ++i) pts.push_back(relh(ed.pw + (k * (ed.degree + 1) + i) * 4));
+        for (auto& h : pts) {
+          if (!(h[3] > 0)) {
+            P.error = "spline edge with a weight not positive";
+            return false;
+          }
+          bx.add(point(h));
+        }
+        if (host) continue;
         while (pts.size() < 4) pts = raise(pts);
         uint32_t cp = uint32_t(cpiece_f.size() / CPIECE_F);
         for (auto& h : pts) cpiece_f.insert(cpiece_f.end(), h.begin(), h.end());
         cpiece_u.insert(cpiece_u.end(), {e, k, ed.n_pieces, 0});
         curve_leaves(pts, cp, e);
-        for (auto& h : pts) bx.add(point(h));
       }
     }
     edge_f.insert(edge_f.end(), rec, rec + EDGE_F);
@@ -1004,11 +1027,18 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
     rec[14] = fd.sense;
     rec[17] = fd.lo[0], rec[18] = fd.lo[1], rec[19] = fd.hi[0], rec[20] = fd.hi[1], rec[21] = fd.period[0], rec[22] = fd.period[1];
     Box fb;
-    if (fd.kind == ICAM_NURBS) {
-      if (fd.du > 3 || fd.dv > 3) {
-        P.error = "B-spline face of degree (" + std::to_string(fd.du) + ", " + std::to_string(fd.dv) + ") (at most bicubic is packed)";
-        return false;
+    if (host_face(fi)) {
+      for (uint32_t k = 0; k < fd.nu * fd.nv; ++k) {
+        const H h = relh(fd.pw + 4 * k);
+        if (!(h[3] > 0)) {
+          P.error = "B-spline face with a weight not positive";
+          return false;
+        }
+        fb.add(point(h));
       }
+      pbox.push_back(fb);
+      pcode.push_back((P_HOST << 28) | fi);
+    } else if (fd.kind == ICAM_NURBS) {
       // spans: along u per column, then along v per Bezier row (surface3.rs:1150-1165)
       std::vector<double> ku(fd.ku, fd.ku + fd.nu + fd.du + 1), kv(fd.kv, fd.kv + fd.nv + fd.dv + 1);
       std::vector<std::vector<std::vector<double>>> cols;
@@ -1323,7 +1353,10 @@ bool pack_part(const icam_part_desc* part, double need, Packed& out) {
   // ---- edge and vertex primitives (after the faces: their order in the list
   // follows the reference's tree ids, faces, edges, vertices)
   for (uint32_t e = 0; e < part->n_edges; ++e)
-    if (part->edges[e].kind == ICAM_LINE || part->edges[e].kind == ICAM_ARC) {
+    if (host_edge[e]) {
+      pbox.push_back(edge_box[e]);
+      pcode.push_back((P_HOST << 28) | HOST_EDGE | e);
+    } else if (part->edges[e].kind == ICAM_LINE || part->edges[e].kind == ICAM_ARC) {
       pbox.push_back(edge_box[e]);
       pcode.push_back((P_EDGE << 28) | e);
     }

@@ -538,6 +538,7 @@ template <class T> struct Best {
   T d2;           // the runner-up's distance and sign (near-ties)
   T sgn2;
   bool unsure;    // a decision was taken within rounding (mixed precision redoes it)
+  bool host;      // a host feature's box came under the answer (its bound is in lmin)
 };
 
 // The outward normal of face fi at a point x on (or within rounding of) it. A
@@ -1537,6 +1538,7 @@ template <class T> __device__ inline void visit(const Tab<T>& tb, uint32_t code,
     case P_FACE: try_face(tb, id, p, b, tm); break;
     case P_EDGE: try_edge(tb, id, p, b); break;
     case P_VERT: try_vert(tb, id, p, b); break;
+    case P_HOST: b.lmin = mmin(b.lmin, lb), b.host = true; break;
     case P_LEAF: try_leaf(tb, id, box, lb, p, b, tm, eps); break;
     default: try_cleaf(tb, id, box, lb, p, b, tm, eps); break;
   }
@@ -1580,7 +1582,7 @@ __device__ inline void refine64(const Tab<double>& tb, uint32_t code, double pa,
     sg = sgn_of(dot(p - q, snormal(k, f, q) * f[14]));
   } else if (kind == P_EDGE) {
     Best<double> b;
-    b.d = TypeEps<double>::big, b.d2 = b.d, b.code = ~0u, b.lmin = b.d, b.unsure = false, b.sgn = 1, b.sgn2 = 1, b.a = b.b = 0;
+    b.d = TypeEps<double>::big, b.d2 = b.d, b.code = ~0u, b.lmin = b.d, b.unsure = false, b.host = false, b.sgn = 1, b.sgn2 = 1, b.a = b.b = 0;
     try_edge(tb, id, p, b);
     if (b.code != ~0u) d = b.d, sg = b.sgn;
   } else if (kind == P_VERT) {

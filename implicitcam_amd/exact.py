@@ -5,10 +5,13 @@ over the C ABI of libicam_hip.so (include/icam_hip.h).
 Method names and meanings follow `PartGpu`: `points` / `points_refined` take
 (point, clamp) records and give (value, flag) pairs, `bricks` takes (first node, h)
 records and gives 729 (value, flag) pairs each; flags 0 exact, 1 a bound past the
-clamp, 2 undecided (partgpu.rs:227-264, gpu.wgsl:1913-1922). Errors raise
+clamp, 2 undecided (partgpu.rs:227-264, gpu.wgsl:1913-1922), 3 left to the caller (a
+host feature, one the device does not read, came within the answer: the reference's
+Code::Cpu, crates/icam-core/src/solid3/gpu.rs:714-716). Errors raise
 `RuntimeError` with the library's message, as `PartGpu` returns Err(String).
 `query` and `bake` are the additive entry points: unbounded signed distances, and
-a whole brick level (map and atlas in the layout of crates/icam-sweep/src/cache.rs:74-101).
+a whole brick level (map and atlas in the layout of crates/icam-sweep/src/cache.rs:74-101;
+bricks left to the caller are HOST entries, read on the CPU as partcache.rs:794-797 does).
 """
 import ctypes
 
@@ -26,7 +29,10 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 class BakeStats(ctypes.Structure):
     _fields_ = [("bricks", ctypes.c_int64), ("band", ctypes.c_int64), ("slots", ctypes.c_int64), ("undecided", ctypes.c_int64),
                 ("refined", ctypes.c_int64), ("unsure_trim", ctypes.c_int64), ("unsure_tie", ctypes.c_int64), ("settled", ctypes.c_int64),
-                ("overflowed", ctypes.c_int64)]
+                ("overflowed", ctypes.c_int64), ("host", ctypes.c_int64)]
+
+
+AIR, SOLID, HOST = 0xFFFFFFFF, 0xFFFFFFFE, 0xFFFFFFFD  # map entries that are not slots
 
 
 _lib = None

@@ -25,7 +25,12 @@
  *   0  the value is the exact signed distance;
  *   1  only a lower bound past the record's bound is known (the value is that
  *      bound); the sign is not known;
- *   2  undecided (a convexity certificate was missing within the bound).
+ *   2  undecided (a convexity certificate was missing within the bound);
+ *   3  left to the caller: a host feature (one the device does not read: a B-spline
+ *      face past bicubic, a spline edge past cubic, an ellipse edge on a B-spline
+ *      face, an edge of such a face) came within the answer, so the point is read
+ *      on the CPU, as a brick holding the reference's Code::Cpu is
+ *      (crates/icam-core/src/solid3/gpu.rs:714-716); the value is a lower bound.
  */
 #ifndef ICAM_HIP_H
 #define ICAM_HIP_H
@@ -47,7 +52,8 @@ enum icam_precision { ICAM_FP64 = 0, ICAM_MIXED = 1 };
  * need) (crates/icam-core/src/solid3/gpu.rs:259-701): packs the part relative to
  * its centre, builds the trim cell trees, the B-spline sub-patch trees and the
  * curve-piece trees, certified for points within `need` of each part (mm), and
- * uploads them. Fails when no device is present or a surface is past degree 3. */
+ * uploads them. Fails when no device is present. Features past the degrees the
+ * device reads are packed as host features (flag 3 above; ICAM_HOST bricks). */
 int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out);
 
 /* Drop(PartGpu): frees the device tables. */
@@ -123,6 +129,8 @@ typedef struct icam_bake_stats {
                           search (left undecided, then bridged, when it does not finish) */
   int64_t overflowed;  /* search blocks whose candidates did not fit their list (those
                           scan every primitive each round: slow, not wrong) */
+  int64_t host;        /* bricks left to the caller (ICAM_HOST map entries: a node or the
+                          centre flagged 3), read on the CPU as partcache.rs:794-797 does */
 } icam_bake_stats;
 
 /* Statistics of the last query on this handle (waits for the device): the points
@@ -134,7 +142,10 @@ int icam_hip_query_stats(icam_hip* h, icam_bake_stats* stats);
  * (nb[0] nb[1] nb[2] entries, x fastest: ICAM_AIR, ICAM_SOLID or a slot) and its
  * atlas (ICAM_WORDS words a slot, slots in map order), bit-compatible with
  * crates/icam-sweep/src/cache.rs:74-101. Host arrays; atlas_cap in words.
- * Returns nonzero when the atlas does not fit (stats->slots says how many). */
+ * Returns nonzero when the atlas does not fit (stats->slots says how many).
+ * A part with host features may leave bricks as ICAM_HOST (stats->host of them,
+ * no slot): the caller reads those on the CPU, as PartCache does the bricks whose
+ * candidates hold Code::Cpu (partcache.rs:794-797). */
 int icam_hip_bake(icam_hip* h, int precision, const icam_level* lv, uint32_t* map, uint32_t* atlas, size_t atlas_cap, icam_bake_stats* stats);
 
 /* The same with the map and atlas left in device memory the library owns until

@@ -127,6 +127,10 @@ typedef struct icam_level {
 /* Brick map entries (cache.rs:84-85) and words per brick (cache.rs:92). */
 #define ICAM_AIR 0xffffffffu
 #define ICAM_SOLID 0xfffffffeu
+/* Not a reference map value: a brick the device left to the caller (a host feature
+ * within its reach); the caller reads it on the CPU and stores it as the reference's
+ * PartCache does a brick whose candidates hold Code::Cpu (partcache.rs:794-797). */
+#define ICAM_HOST 0xfffffffdu
 #define ICAM_WORDS 365
 #define ICAM_NODES 729
 

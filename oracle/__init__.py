@@ -69,6 +69,8 @@ class Oracle:
     def __init__(self, part):
         self.part = part
         self.h = lib().orc_open(ctypes.cast(part.desc, ctypes.c_void_p))
+        if not self.h:
+            raise ValueError("the oracle takes surfaces and spline edges up to degree 7")
 
     def sdf(self, xyz, bound=None, threads=0):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)

@@ -12,7 +12,8 @@
 //    to a cell's reference (the tree is an acceleration of the same parity);
 //  - a B-spline face's search starts from its span patches instead of the
 //    quadtree over spans (same patches, same bounds);
-//  - patches are limited to degree 3 (the parts here are at most bicubic);
+//  - patches and spline edges are limited to degree 7 (the reference kernel's
+//    MAX_DEGREE, crates/icam-core/src/solid3/gpu.rs:41-42); orc_open refuses more;
 //  - closest_trimmed takes a critical point within DIST_EPS of a bound that no answer
 //    has set, and a polished point within DIST_EPS of the point it started from
 //    (surface3.rs:1276-1283 takes them only when strictly nearer): where the foot lies
@@ -55,7 +56,7 @@ enum Count : int {
   C_VERT,         // a vertex (or pole, tip) distance
   NCOUNT
 };
-static_assert(NCOUNT == ORC_COUNTS && C_CROSS == ORC_COUNT_CROSS && C_VERT == ORC_COUNT_VERT, "counts as oracle.h numbers them");
+static_assert(int(NCOUNT) == int(ORC_COUNTS) && int(C_CROSS) == int(ORC_COUNT_CROSS) && int(C_VERT) == int(ORC_COUNT_VERT), "counts as oracle.h numbers them");
 thread_local uint64_t t_count[NCOUNT];
 std::atomic<uint64_t> g_count[NCOUNT];
 inline void count(int k) { ++t_count[k]; }
@@ -439,7 +440,7 @@ struct Trim {
 };
 
 // ------------------------------------------------------------------ rational Bezier patches
-const int MAXP = 4;  // at most bicubic
+const int MAXP = 8;  // control points a direction: degree 7 at most
 
 // surface3.rs:246-253 SEval
 struct SEval {
@@ -507,8 +508,10 @@ struct Patch {
     return e;
   }
   // surface3.rs:265-283, 352-380 de Casteljau splits
+  // the parts' headers (every control point they use is written by the split)
+  void head(Patch& o) const { o.m = m, o.n = n, o.u0 = u0, o.u1 = u1, o.v0 = v0, o.v1 = v1; }
   void split_u(double s, Patch& l, Patch& r) const {
-    l = *this, r = *this;
+    head(l), head(r);
     for (int j = 0; j <= n; ++j) {
       double b[MAXP][4];
       for (int i = 0; i <= m; ++i)
@@ -524,7 +527,7 @@ struct Patch {
     l.u1 = um, r.u0 = um;
   }
   void split_v(double t, Patch& l, Patch& r) const {
-    l = *this, r = *this;
+    head(l), head(r);
     for (int i = 0; i <= m; ++i) {
       double b[MAXP][4];
       for (int j = 0; j <= n; ++j)
@@ -915,8 +918,9 @@ struct Nurbs {
   bool closest_trimmed(V3 p, double cutoff, const Trim* trim, SurfHit& best, bool& found, int& pops_out) const {
     found = false;
     double best_d = cutoff;
-    std::vector<Patch> pool;
-    pool.reserve(256);
+    // the thread's pool, kept between calls (a patch of degree 7 takes 2 KB)
+    thread_local std::vector<Patch> pool;
+    pool.clear();
     std::priority_queue<Item> heap;
     for (int k = 0; k < nsu * nsv; ++k) {
       double lb = box_dist(p, spans[k].bbox());
@@ -937,7 +941,7 @@ struct Nurbs {
         pops_out = pops;
         return false;
       }
-      Patch pa = pool[it.idx];
+      const Patch& pa = pool[it.idx];  // read before the pool grows
       int state = it.inside ? 1 : (trim ? trim->box_class(std::array<double, 2>{pa.u0, pa.v0}.data(), std::array<double, 2>{pa.u1, pa.v1}.data()) : 1);
       if (state == 0) continue;
       bool now_in = state == 1;
@@ -984,7 +988,8 @@ struct Nurbs {
   bool closest(V3 p, double cutoff, SurfHit& best) const {
     bool found = false;
     double best_d = cutoff;
-    std::vector<Patch> pool;
+    thread_local std::vector<Patch> pool;
+    pool.clear();
     std::priority_queue<Item> heap;
     for (int k = 0; k < nsu * nsv; ++k) {
       double lb = box_dist(p, spans[k].bbox());
@@ -999,7 +1004,7 @@ struct Nurbs {
       heap.pop();
       if (it.lb >= best_d - DIST_EPS || best_d <= DIST_EPS) break;
       if (++pops > 20000) break;
-      Patch pa = pool[it.idx];
+      const Patch& pa = pool[it.idx];  // read before the pool grows
       double s = 0.5, t = 0.5;
       pa.newton(p, s, t, SEARCH_NEWTON);
       SEval e = pa.eval(s, t);
@@ -1043,7 +1048,7 @@ struct Bez3 {
     count(C_CEVAL);
     int n = deg;
     double b[MAXP][4];
-    std::memcpy(b, pw, sizeof(b));
+    std::memcpy(b, pw, sizeof(b[0]) * (deg + 1));
     double d1h[4] = {0}, d2h[4] = {0};
     for (int lev = 1; lev <= n; ++lev) {
       if (n >= 2 && lev == n - 1)
@@ -1064,7 +1069,7 @@ struct Bez3 {
   void split(double s, Bez3& l, Bez3& r) const {
     l = *this, r = *this;
     double b[MAXP][4];
-    std::memcpy(b, pw, sizeof(b));
+    std::memcpy(b, pw, sizeof(b[0]) * (deg + 1));
     for (int c = 0; c < 4; ++c) l.pw[0][c] = b[0][c], r.pw[deg][c] = b[deg][c];
     for (int lev = 1; lev <= deg; ++lev) {
       for (int i = 0; i <= deg - lev; ++i)
@@ -1999,6 +2004,10 @@ struct orc_exact {
 extern "C" {
 
 orc_exact* orc_open(const icam_part_desc* part) {
+  for (uint32_t f = 0; f < part->n_faces; ++f)
+    if (part->faces[f].kind == ICAM_NURBS && (part->faces[f].du >= uint32_t(MAXP) || part->faces[f].dv >= uint32_t(MAXP))) return nullptr;
+  for (uint32_t e = 0; e < part->n_edges; ++e)
+    if (part->edges[e].kind == ICAM_SPLINE && part->edges[e].degree >= uint32_t(MAXP)) return nullptr;
   orc_exact* o = new orc_exact();
   o->ex.open(part);
   return o;

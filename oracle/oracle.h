@@ -25,7 +25,8 @@ extern "C" {
 
 typedef struct orc_exact orc_exact;
 
-/* Builds the evaluator over a part (Exact::new, solid3.rs:1586-1652). */
+/* Builds the evaluator over a part (Exact::new, solid3.rs:1586-1652); NULL when a
+ * surface or spline edge is past degree 7. */
 orc_exact* orc_open(const icam_part_desc* part);
 void orc_close(orc_exact*);
 

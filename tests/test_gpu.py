@@ -475,3 +475,47 @@ def test_c3_trimmed_plate(prec):
     assert (m == m_ref).all()
     qa, qr = a.view(np.int16).astype(int), a_ref.view(np.int16).astype(int)
     assert len(qa) == len(qr) and np.abs(qa - qr).max(initial=0) <= 1 and ((qa > 0) == (qr > 0)).all()
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("k", [0, 1])
+def test_host_features_quintic(k, prec):
+    """A degree-(5, 5) top, past what the device reads, and its edges (lines for k = 0,
+    degree-5 spline pieces for k = 1) packed as host features: the part opens; a point
+    whose answer a host feature's box comes under is left to the caller (flag 3), every
+    other point is exact against the oracle, and every point the oracle finds nearest
+    the top or one of its edges is left to the caller; a level's bricks are the oracle's
+    except those left to the caller (HOST entries, counted in stats.host), which hold
+    the top."""
+    from implicitcam_amd.exact import HOST
+    from implicitcam_amd.part import near_surface_points
+
+    part = Part(f"quintic:{k}")
+    o = Oracle(part)
+    g = PartGpu(part, 12 * 0.5)
+    pts = np.concatenate([uniform_points(SEED + k, 1 << 14, part.lo - 5, part.hi + 5), near_surface_points(part, SEED + 7, 1 << 13, 1.0)])
+    print(f"quintic:{k} precision {prec}: {len(pts)} queries", flush=True)
+    d, flags, feat = g.query(pts, prec)
+    dref, fref, _ = o.sdf(pts)
+    assert set(np.unique(flags).tolist()) <= {0, 3}
+    ok = flags == 0
+    assert 0.2 < ok.mean() < 0.9
+    assert np.abs(d[ok] - dref[ok]).max() <= tol(prec, part)
+    assert ((d < 0) == (dref < 0))[ok & (np.abs(dref) > 1e-8)].all()
+    assert (feat[ok] == fref[ok]).mean() > 0.999
+    # the top is face 0, its edges 0-3: an answer there is the host's, unless a device
+    # feature ties it (exact then, checked above)
+    on_top = (fref == 0) | ((fref >> 28 == 1) & ((fref & 0xFFFFFFF) < 4))
+    assert on_top.sum() > 1000 and (ok & on_top).sum() <= 0.001 * on_top.sum()
+    lv = level_for(part, part.diagonal / 64, 8)
+    m_ref, a_ref, _ = o.bake(lv)
+    m, a, st = g.bake(lv, prec)
+    host = m == HOST
+    assert st.host == host.sum() and 0 < host.sum() < len(m)
+    kept = ~host
+    slot = m < HOST - 1
+    slot_ref = m_ref < HOST - 1
+    assert (m[kept & ~slot] == m_ref[kept & ~slot]).all()
+    assert (slot_ref[kept] == slot[kept]).all()
+    for b in np.flatnonzero(kept & slot):
+        assert (a[m[b] * 365:(m[b] + 1) * 365] == a_ref[m_ref[b] * 365:(m_ref[b] + 1) * 365]).all(), f"brick {b}"

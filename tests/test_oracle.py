@@ -494,3 +494,74 @@ def test_c5_span_border_tie_keeps_the_face(built):
     wall = o.nurbs_closest(163, x[0])
     assert wall is not None and abs(abs(d[0]) - wall[0]) < 1e-12
     assert feat[0] == 163 and d[0] < 0 and abs(d[0]) < 0.6
+
+
+def _bspline_basis(kn, p, u):
+    """Cox-de Boor recursion for the clamped knots kn, degree p, at u (u = the last knot
+    takes the last span): the n basis values."""
+    n = len(kn) - p - 1
+    b = np.array([1.0 if (kn[i] <= u < kn[i + 1] or (u == kn[-1] and kn[i] < u <= kn[i + 1])) else 0.0 for i in range(n + p)])
+    for k in range(1, p + 1):
+        nb = np.zeros(len(b) - 1)
+        for i in range(len(nb)):
+            if kn[i + k] > kn[i]:
+                nb[i] += (u - kn[i]) / (kn[i + k] - kn[i]) * b[i]
+            if kn[i + k + 1] > kn[i + 1]:
+                nb[i] += (kn[i + k + 1] - u) / (kn[i + k + 1] - kn[i + 1]) * b[i + 1]
+        b = nb
+    return b[:n]
+
+
+@pytest.fixture(scope="module")
+def quintic(built):
+    parts = [Part(f"quintic:{k}") for k in (0, 1)]
+    return parts, [Oracle(p) for p in parts]
+
+
+def test_quintic_top_is_the_de_boor_surface(quintic):
+    """The degree-(5, 5) top (past bicubic: the device leaves it to the host) as the
+    oracle evaluates it through its Bezier spans equals the rational B-spline of the
+    generator's net and knots by the Cox-de Boor recursion."""
+    (part, _), (o, _) = quintic
+    info = part.describe()
+    assert info[0] == 5
+    p, n = int(info[1]), int(info[2])
+    kn = info[3:3 + n + p + 1]
+    pw = info[3 + n + p + 1:].reshape(n, n, 4)
+    rng = np.random.default_rng(5)
+    for u, v in list(rng.random((300, 2))) + [(0.0, 0.0), (1.0, 1.0), (0.4, 0.6), (0.2, 1.0)]:
+        h = np.einsum("i,j,ijc->c", _bspline_basis(kn, p, u), _bspline_basis(kn, p, v), pw)
+        assert np.abs(h[:3] / h[3] - o.nurbs_eval(0, u, v)).max() < 1e-12
+
+
+def test_quintic_offsets_and_edge_forms(quintic):
+    """Points offset from the degree-5 top along its normal answer at the offset, as the
+    top (face 0); the top's edges as lines and as degree-5 spline pieces (the same
+    straight segments) answer alike around them."""
+    (pa, pb), (oa, ob) = quintic
+    rng = np.random.default_rng(6)
+    pts, want = [], []
+    for _ in range(400):
+        u, v = 0.1 + 0.8 * rng.random(2)
+        e = 1e-6
+        x = oa.nurbs_eval(0, u, v)
+        xu = (oa.nurbs_eval(0, u + e, v) - oa.nurbs_eval(0, u - e, v)) / (2 * e)
+        xv = (oa.nurbs_eval(0, u, v + e) - oa.nurbs_eval(0, u, v - e)) / (2 * e)
+        nrm = np.cross(xu, xv)
+        nrm /= np.linalg.norm(nrm)
+        off = rng.uniform(-0.3, 0.3)
+        pts.append(x + off * nrm)
+        want.append(off)
+    d, feat, flags = oa.sdf(np.array(pts))
+    assert np.abs(d - np.array(want)).max() < 1e-9
+    assert (feat == 0).all() and not flags.any()
+    t = rng.uniform(-45, 45, 4000)
+    off = rng.uniform(-3, 3, (4000, 2))
+    side = rng.integers(0, 4, 4000)
+    x = np.where(side == 0, t, np.where(side == 1, 40 + off[:, 0], np.where(side == 2, t, -40 + off[:, 0])))
+    y = np.where(side == 0, -40 + off[:, 0], np.where(side == 1, t, np.where(side == 2, 40 + off[:, 0], t)))
+    pts = np.stack([x, y, 40 / 3 + off[:, 1]], 1)
+    da, fa, _ = oa.sdf(pts)
+    db, fb, _ = ob.sdf(pts)
+    assert np.abs(da - db).max() < 1e-12
+    assert (fa == fb).all() and (fa >> 28 == 1).sum() > 500
