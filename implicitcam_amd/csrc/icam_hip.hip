@@ -538,12 +538,12 @@ __device__ inline T part_lb(const Tab<T>& tb, uint32_t i, uint32_t code, v3<T> p
   return lb;
 }
 template <class T, bool SETTLE = false>
-__device__ inline void try_part(const Tab<T>& tb, uint32_t i, T lb, v3<T> p, Best<T>& b, T tm, T eps) {
+__device__ inline void try_part(const Tab<T>& tb, uint32_t i, T lb, v3<T> p, Best<T>& b, T tm, T eps, const SSink* sk = nullptr) {
   const uint32_t code = tb.pcode[i], id = code & 0x0fffffffu;
   if ((code >> 28) == P_LEAF)
-    try_leaf<T, SETTLE>(tb, id, tb.pbox + 6 * i, lb, p, b, tm, eps);
+    try_leaf<T, SETTLE>(tb, id, tb.pbox + 6 * i, lb, p, b, tm, eps, sk);
   else
-    try_cleaf<T, SETTLE>(tb, id, tb.pbox + 6 * i, lb, p, b, tm, eps);
+    try_cleaf<T, SETTLE>(tb, id, tb.pbox + 6 * i, lb, p, b, tm, eps, sk);
 }
 template <int K, class V> __device__ inline V group_min(V v) {
   for (int o = 1; o < K; o <<= 1) v = mmin(v, V(__shfl_xor(v, o)));
@@ -613,7 +613,7 @@ template <class T> __device__ inline Best<T> best_init(T bound) {
 // the final answer is still read, in the chunk that holds it).
 template <class T, int NT, int Q, int K, int C, bool SETTLE = false>
 __device__ inline void search_in(const Tab<T>& tb, v3<T> p, bool valid, Best<T>& b, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk,
-                                 const Res* defer = nullptr, long point = 0) {
+                                 const Res* defer = nullptr, long point = 0, const SSink* sk = nullptr) {
   const bool all = nc > C;
   const uint32_t n = all ? tb.L.n_prims : uint32_t(nc);
   const int lane = threadIdx.x, sub = lane & (K - 1);
@@ -706,7 +706,7 @@ __device__ inline void search_in(const Tab<T>& tb, v3<T> p, bool valid, Best<T>&
         k = uint32_t(next);
         last_lb = next_lb, last_k = k;
       }
-      try_part<T, SETTLE>(tb, all ? k : cand[k], next_lb, p, b, tm, eps);
+      try_part<T, SETTLE>(tb, all ? k : cand[k], next_lb, p, b, tm, eps, sk);
     }
   }
 }
@@ -735,9 +735,9 @@ template <class T, int K> __device__ inline void group_merge(Best<T>& b, T eps) 
 // A point's whole search over one candidate list.
 template <class T, int NT, int Q, int K, int C, bool SETTLE = false>
 __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk,
-                          const Res* defer = nullptr, long point = 0) {
+                          const Res* defer = nullptr, long point = 0, const SSink* sk = nullptr) {
   Best<T> b = best_init<T>(bound);
-  search_in<T, NT, Q, K, C, SETTLE>(tb, p, valid, b, cand, nc, tm, eps, qlb, qk, defer, point);
+  search_in<T, NT, Q, K, C, SETTLE>(tb, p, valid, b, cand, nc, tm, eps, qlb, qk, defer, point, sk);
   if (K > 1) group_merge<T, K>(b, eps);
   return b;
 }
@@ -847,11 +847,12 @@ template <int S> constexpr int rounds() { return S == SRC_NODES ? 3 : 1; }
 // reach covers every point's answer (its bound, or for unbounded points the boundary's
 // upper bound over the block's box, from the hierarchy). NT threads cull; scattered
 // points set the box from the block's own points (only those flagged `only` when
-// only_flag is given), or from the pcount points listed in plist when it is given.
+// only_flag is given), or from the pcount points listed in plist (by their place in
+// the block) when it is given.
 template <int S, int NT = Shape<S>::NT>
 __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t* cand, uint32_t* tmp, float* key, float* red, int* wave_tot,
                                 uint32_t* chunks = nullptr, int* nchunk = nullptr, float* gb_out = nullptr, float* r_out = nullptr,
-                                const uint32_t* only_flag = nullptr, uint32_t only = 0, const uint32_t* plist = nullptr, int pcount = 0) {
+                                const uint32_t* only_flag = nullptr, uint32_t only = 0, const uint16_t* plist = nullptr, int pcount = 0) {
   float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
   float r = 0.f;
   if (!pts<S>()) {
@@ -884,7 +885,7 @@ __device__ int block_candidates(const Dev& dv, const Job& j, long blk, uint32_t*
     r = fmaxf(r, ub * (1.0f + 1e-5f) + 1e-4f);
   };
   if (plist) {
-    for (int k = threadIdx.x; k < pcount; k += NT) add(long(plist[k]));
+    for (int k = threadIdx.x; k < pcount; k += NT) add(blk * long(Shape<S>::NT / Shape<S>::K) + long(plist[k]));
   } else {
     for (int m = 0; m < rounds<S>(); ++m) {
       long i = block_point<S>(j, blk, m);
@@ -1005,7 +1006,7 @@ template <int S, class T> struct SearchLds {
 // point's answer so far) is passed over.
 template <int S, class T, bool SETTLE, int NT = Shape<S>::NT, int K = Shape<S>::K, class L = SearchLds<S, T>>
 __device__ Best<T> search_chunks(const Dev& dv, const Tab<T>& tb, v3<T> p, bool valid, T bound, L& sm, int nchunk, const float* gb, float r,
-                                 T tm, T eps) {
+                                 T tm, T eps, const SSink* sk = nullptr) {
   constexpr int Q = Shape<S>::Q, C = Shape<S>::C;
   Best<T> b = best_init<T>(bound);
   if constexpr (!pts<S>()) {
@@ -1018,7 +1019,7 @@ __device__ Best<T> search_chunks(const Dev& dv, const Tab<T>& tb, v3<T> p, bool 
     const int need = valid && box_dist(p, (const ICAM_K float*)(boxes + 6 * size_t(node2))) < b.d;
     if (!__syncthreads_or(need)) continue;
     const int cnt = bvh_chunk<NT>(dv, gb, r, node2, sm.cand, sm.sub, sm.wave_tot);
-    search_in<T, NT, Q, K, C, SETTLE>(tb, p, valid, b, sm.cand, cnt, tm, eps, sm.qlb(), sm.qk());
+    search_in<T, NT, Q, K, C, SETTLE>(tb, p, valid, b, sm.cand, cnt, tm, eps, sm.qlb(), sm.qk(), nullptr, 0, sk);
     __syncthreads();
   }
   if (K > 1) group_merge<T, K>(b, eps);
@@ -1241,6 +1242,7 @@ __global__ __launch_bounds__(256) void k_refine_bricks(Dev dv, Job j, Res res, l
 // them on the CPU, crates/icam-sim/src/partcache.rs:563-751).
 template <int S, bool SETTLE>
 __device__ __forceinline__ void redo_block(const Dev& dv, const Job& j, const Res& res, long blk, SearchLds<S, double>& sm) {
+  static_assert(!SETTLE, "the settle is settle_block's");
   const uint32_t want = SETTLE ? 4u : 3u;
   constexpr int NT = Shape<S>::NT;
   int nchunk = 0;
@@ -1328,26 +1330,65 @@ __global__ __launch_bounds__(256) void k_mark_settle(Job j, Res res) {
   if (i < j.n) mark_point<S>(j, res, i);
 }
 
-// The settle, one wave a queued block: the block's marked points (flag 4) are listed in
-// LDS, its candidates culled for their box, and each lane takes one listed point. A
-// queued block's marked points are few (a brick's nodes near a concave corner of a
-// wall, a block's points near a trim curve) and the certified search holds 256
-// registers and as many accumulation registers a lane, one wave a SIMD: a block of 256
-// lanes a queued block, each lane its own point, left most lanes and three SIMDs of
-// four idle while the queue's blocks waited.
+// The settle, one wave a queued block. The block's marked points (flag 4) are listed in
+// LDS and taken NT / K at a time (a round): their candidates are culled for their box
+// and each point is searched again in double precision by its K lanes, every part past
+// its certificate handed to the round's settle as a root piece instead of being read;
+// then the wave works through the round's pool of pieces together, each lane taking one
+// piece (a quarter of a sub-patch or a half of a curve part: settle_quarter,
+// settle_half) from those nearest their points' bounds, and the lane of each point
+// merging its pieces' steps, in lane order, into the point's answer and bound. A
+// queued block's marked points are few (a brick's nodes near a concave corner of a wall,
+// a block's points near a trim curve) and their searches are long and uneven: with a
+// point to a lane, most lanes waited on the longest search, whose queue of pieces lived
+// in private memory. Points whose parts or pieces the round could not hold (points on a
+// pole's axis see every sub-patch around the pole at nearly one distance) are searched
+// again in rounds of a quarter as many points, down to one point a round.
 constexpr int SETTLE_NT = 64;
+constexpr int SPOOL = 704;  // pieces a round's pool holds
+struct SRec {               // a lane's step, for its point's lane to merge
+  double sd, lost, d, pa, pb, sgn;
+  uint32_t code, flags;     // flags: the point's slot (bits 0-5), SR_*
+};
+constexpr uint32_t SR_VALID = 1u << 8, SR_ANS = 1u << 9, SR_UNSURE = 1u << 10, SR_OVER = 1u << 11;
 template <int S> struct SettleLds {
   static constexpr int NT = SETTLE_NT, Q = Shape<S>::Q, C = Shape<S>::C;
+  static constexpr int K = S == SRC_NODES ? 1 : Shape<S>::K;  // lanes a point in the search
+  static constexpr int NR = NT / K;                            // points a round
+  // root pieces a round (what workgroup memory leaves for them: a point under a pole
+  // has a root in each of a hundred sub-patches around it)
+  static constexpr int ROOTS = S == SRC_NODES ? 896 : S == SRC_POINTS ? 448 : S == SRC_CENTRES ? 1024 : 512;
   // the points a queued block holds: a brick's nodes, a tile's centres, a block's points
   static constexpr int NP = S == SRC_NODES ? 729 : S == SRC_CENTRES ? TILE_X * TILE_Y * TILE_Z : Shape<S>::NT / Shape<S>::K;
-  uint32_t cand[C];
+  union {
+    struct {  // a round's search
+      uint32_t cand[C];
+      alignas(16) unsigned char scratch[cmax(cmax(SORTC * 12, 2 * FCAP * 4), Q * NT * 10)];
+      uint32_t chunks[pts<S>() ? FCAP : 1];
+      uint32_t sub[pts<S>() ? 64 : 1];
+    };
+    struct {  // its settle
+      SPiece items[SPOOL];
+      SRec rec[NT];
+      int sel[NT];  // the pieces the lanes take
+      int hist[NT];
+    };
+  };
   float red[NT];
   int wave_tot[1];
-  alignas(16) unsigned char scratch[cmax(cmax(SORTC * 12, 2 * FCAP * 4), Q * NT * 10)];
-  uint32_t list[NP];
+  uint16_t list[NP];  // the marked points, by their place in the block
   int count;
-  uint32_t chunks[pts<S>() ? FCAP : 1];
-  uint32_t sub[pts<S>() ? 64 : 1];
+  // the round's root pieces
+  float rlb[ROOTS];
+  uint32_t rpart[ROOTS];
+  uint8_t rslot[ROOTS];
+  int rcount;
+  int pcount;  // the pool's length (may run past SPOOL)
+  // the round's points: pruning bound, reads and root pieces, whether the settle
+  // answered for it, whether the round could not hold its parts or pieces
+  double pr[NR];
+  int pops[NR], roots[NR];
+  uint8_t got[NR], over[NR];
   __device__ uint32_t* tmp() { return (uint32_t*)scratch; }
   __device__ float* key() { return (float*)(scratch + SORTC * 4); }
   __device__ double* qlb() { return (double*)scratch; }
@@ -1356,63 +1397,232 @@ template <int S> struct SettleLds {
 template <int S>
 __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const Res& res, long blk, SettleLds<S>& sm) {
   constexpr int NT = SETTLE_NT, NP = SettleLds<S>::NP, Q = Shape<S>::Q, C = Shape<S>::C;
-  if (threadIdx.x == 0) sm.count = 0;
+  constexpr int K = SettleLds<S>::K, NR = SettleLds<S>::NR;
+  const int lane = threadIdx.x;
+  // the block's g-th point
+  auto point_of = [&](int g) -> long {
+    if (S == SRC_NODES) return blk * 729 + g;
+    if (S == SRC_CENTRES) return tile_point(j.lv, blk, g);
+    return blk * NP + g;
+  };
+  if (lane == 0) sm.count = 0;
   __syncthreads();
-  for (int g = threadIdx.x; g < NP; g += NT) {
-    long i;
-    if (S == SRC_NODES)
-      i = blk * 729 + g;
-    else if (S == SRC_CENTRES)
-      i = tile_point(j.lv, blk, g);
-    else
-      i = blk * NP + g;
-    if (i >= 0 && i < j.n && res.flag[i] == 4u) sm.list[atomicAdd(&sm.count, 1)] = uint32_t(i);
+  for (int g = lane; g < NP; g += NT) {
+    const long i = point_of(g);
+    if (i >= 0 && i < j.n && res.flag[i] == 4u) sm.list[atomicAdd(&sm.count, 1)] = uint16_t(g);
   }
   __syncthreads();
-  const int cnt = sm.count;
-  if (cnt == 0) return;
-  // brick centres and scattered points take their first search's K lanes a point (a
-  // centre's search reaches far over many parts, which its lanes split), nodes a lane each
-  constexpr int K = S == SRC_NODES ? 1 : Shape<S>::K;
+  if (sm.count == 0) return;
   int nchunk = 0, nc = 0;
   float gb[6], rr = 0.f;
   bool chunked = false;
-  // the candidates: a brick's or a tile's once; scattered points' for each round's
-  // points alone (a queued block's marked points may lie far apart, far from the
-  // surface: the box of all of them could reach past the list)
-  auto cull_for = [&](const uint32_t* pl, int pc) {
+  const Tab<double> tb = tab_of<double>(dv);
+  const SPool pool{sm.items, &sm.pcount, SPOOL};
+  for (int R = NR, cnt = sm.count;; R = max(1, R / 4)) {
+  int again = 0;  // points to search again in smaller rounds, listed in place
+  for (int base = 0; base < cnt; base += R) {
+    const int npt = min(R, cnt - base);
+    // the candidates, every round (the pool shares their memory): a brick's or a tile's;
+    // scattered points' for the round's points alone (a queued block's marked points may
+    // lie far apart, far from the surface: the box of all of them could reach past the list)
+    __syncthreads();
     nchunk = 0;
     nc = block_candidates<S, NT>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot, pts<S>() ? sm.chunks : nullptr, &nchunk, gb, &rr,
-                                 nullptr, 0u, pl, pc);
+                                 nullptr, 0u, pts<S>() ? sm.list + base : nullptr, pts<S>() ? npt : 0);
     chunked = pts<S>() && nchunk > 0;
-  };
-  if (!pts<S>()) cull_for(nullptr, 0);
-  const Tab<double> tb = tab_of<double>(dv);
-  for (int base = 0; base < cnt; base += NT / K) {
-    if (pts<S>()) {
-      __syncthreads();
-      cull_for(sm.list + base, min(NT / K, cnt - base));
-    }
-    const int idx = base + int(threadIdx.x) / K;
-    const bool valid = idx < cnt;
-    const long i = valid ? long(sm.list[idx]) : -1;
+    if (lane == 0) sm.rcount = 0;
+    if (lane < NR) sm.roots[lane] = 0, sm.pops[lane] = 0, sm.got[lane] = 0, sm.over[lane] = 0;
+    __syncthreads();
+    // the search, with the parts past their certificates handed over as root pieces
+    const int g = lane / K;
+    const bool valid = g < npt;
+    const long i = valid ? point_of(sm.list[base + g]) : -1;
     double bound = 0.0;
     const v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
     const double sb = bound < 1e300 ? bound * (1.0 + 1e-12) : bound;
+    const SSink sink{sm.rlb, sm.rpart, sm.rslot, &sm.rcount, SettleLds<S>::ROOTS, g, sm.over};
     Best<double> b;
     if constexpr (pts<S>()) {
-      b = chunked ? search_chunks<S, double, true, NT, K>(dv, tb, p, valid, sb, sm, nchunk, gb, rr, 0.0, dv.L.eps64)
-                  : search<double, NT, Q, K, C, true>(tb, p, valid, sb, sm.cand, nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk());
+      b = chunked ? search_chunks<S, double, true, NT, K>(dv, tb, p, valid, sb, sm, nchunk, gb, rr, 0.0, dv.L.eps64, &sink)
+                  : search<double, NT, Q, K, C, true>(tb, p, valid, sb, sm.cand, nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk(), nullptr, 0, &sink);
     } else {
-      b = search<double, NT, Q, K, C, true>(tb, p, valid, sb, sm.cand, nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk());
+      b = search<double, NT, Q, K, C, true>(tb, p, valid, sb, sm.cand, nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk(), nullptr, 0, &sink);
     }
-    if (!valid || (threadIdx.x & (K - 1)) != 0) continue;
-    double val;
-    uint32_t flag, feat;
-    classify64(b, bound, dv, val, flag, feat);
-    res.val[i] = val;
-    res.flag[i] = flag;
-    if (res.feat) res.feat[i] = feat;
+    // lane g < npt holds point g's answer from here on (its group's first lane has it)
+    Best<double> ob = b;
+    if constexpr (K > 1) {
+      const int src = (lane * K) & (NT - 1);
+      ob.d = __shfl(b.d, src), ob.code = uint32_t(__shfl(int(b.code), src)), ob.a = __shfl(b.a, src), ob.b = __shfl(b.b, src);
+      ob.sgn = __shfl(b.sgn, src), ob.lmin = __shfl(b.lmin, src), ob.d2 = __shfl(b.d2, src), ob.sgn2 = __shfl(b.sgn2, src);
+      ob.unsure = __shfl(int(b.unsure), src) != 0, ob.host = __shfl(int(b.host), src) != 0;
+    }
+    const bool own = lane < npt;
+    long oi = -1;
+    double obound = 0.0, olost = 1e300;
+    if (own) {
+      oi = point_of(sm.list[base + lane]);
+      job_point<S>(j, dv, oi, obound);
+      sm.pr[lane] = ob.d;
+    }
+    __syncthreads();
+    // the root pieces: counted by point, then let into the pool (whose memory the search
+    // has left) while it is less than a quarter full, in the order the searches met them
+    // (nearest first for each lane); in rounds of one point, one at a time once fewer
+    // than NT pieces are left: a point just under a pole has a chain of pieces down to
+    // the pole in each of the sub-patches around it, about a hundred pieces each, more
+    // together than the pool holds
+    const int nroots = min(sm.rcount, SettleLds<S>::ROOTS);
+    for (int k = lane; k < nroots; k += NT) atomicAdd(&sm.roots[sm.rslot[k]], 1);
+    int n = 0, rnext = 0;
+    __syncthreads();
+    // a piece's lower bound over its point's bound: the lanes take the lowest
+    auto key_of = [&](int k) { return sm.items[k].lb / sm.pr[sp_slot(sm.items[k].meta)]; };
+    for (;;) {
+      // the pieces that can still come under their point's bound, kept in order
+      int m = 0;
+      for (int c0 = 0; c0 < n; c0 += NT) {
+        const int k = c0 + lane;
+        SPiece x{};
+        bool keep = false;
+        if (k < n) {
+          x = sm.items[k];
+          const double prb = sm.pr[sp_slot(x.meta)];
+          keep = x.lb < prb - SETTLE_EPS && prb > SETTLE_EPS;
+        }
+        const unsigned long long mask = __ballot(keep);
+        __syncthreads();
+        if (keep) sm.items[m + int(__popcll(mask & __lanemask_lt()))] = x;
+        m += int(__popcll(mask));
+        __syncthreads();
+      }
+      n = m;
+      if (rnext < nroots && n < (R > 1 ? SPOOL / 4 : NT)) {
+        const int a = R > 1 ? min(nroots - rnext, SPOOL / 2 - n) : 1;
+        for (int k = lane; k < a; k += NT)
+          sm.items[n + k] = SPiece{double(sm.rlb[rnext + k]), 0ull, 0ull, sm.rpart[rnext + k], sp_meta(sm.rslot[rnext + k], 0, 0, 0)};
+        n += a, rnext += a;
+        __syncthreads();
+      }
+      if (n == 0) break;
+      // the NT pieces lowest under their bounds, or all: by a histogram of the ratio in NT
+      // bins over the pieces' own range of it (near a pole every piece lies within a
+      // thousandth of its point's bound: bins over [0, 1) took them in pool order, breadth
+      // first, and the pool overflowed)
+      const int take = min(n, NT);
+      if (n > NT) {
+        double kmin = 1e300, kmax = -1e300;
+        for (int k = lane; k < n; k += NT) {
+          const double x = key_of(k);
+          kmin = fmin(kmin, x), kmax = fmax(kmax, x);
+        }
+        for (int o = 1; o < NT; o <<= 1) kmin = fmin(kmin, __shfl_xor(kmin, o)), kmax = fmax(kmax, __shfl_xor(kmax, o));
+        const double scale = kmax > kmin ? double(NT) / (kmax - kmin) : 0.0;
+        auto bin_of = [&](int k) {
+          const double x = (key_of(k) - kmin) * scale;
+          if (!(x >= 0.0)) return 0;
+          return x < double(NT - 1) ? int(x) : NT - 1;
+        };
+        sm.hist[lane] = 0;
+        __syncthreads();
+        for (int k = lane; k < n; k += NT) atomicAdd(&sm.hist[bin_of(k)], 1);
+        __syncthreads();
+        int c = sm.hist[lane];
+        for (int o = 1; o < NT; o <<= 1) {
+          const int t = __shfl_up(c, o);
+          if (lane >= o) c += t;
+        }
+        const int cut = __ffsll((long long)__ballot(c >= NT)) - 1;  // the bin that fills the lanes
+        const int below = __shfl(c, cut) - sm.hist[cut];
+        int lo = 0, at = 0;
+        for (int c0 = 0; c0 < n; c0 += NT) {
+          const int k = c0 + lane;
+          const int bin = k < n ? bin_of(k) : NT;
+          const unsigned long long ml = __ballot(bin < cut), me = __ballot(bin == cut);
+          if (bin < cut) sm.sel[lo + int(__popcll(ml & __lanemask_lt()))] = k;
+          if (bin == cut) {
+            const int q = at + int(__popcll(me & __lanemask_lt()));
+            if (q < NT - below) sm.sel[below + q] = k;
+          }
+          lo += int(__popcll(ml)), at += int(__popcll(me));
+        }
+      } else {
+        sm.sel[lane] = lane;
+      }
+      __syncthreads();
+      const int mine = lane < take ? sm.sel[lane] : -1;
+      SPiece it{};
+      if (mine >= 0) it = sm.items[mine];
+      if (lane == 0) sm.pcount = n;
+      __syncthreads();
+      // each lane's step (its piece leaves the pool: dropped at the next pass)
+      SRec r;
+      r.flags = 0, r.sd = 1e300, r.lost = 1e300, r.d = 0.0, r.pa = 0.0, r.pb = 0.0, r.sgn = 1.0, r.code = NONE;
+      if (mine >= 0) {
+        sm.items[mine].lb = 1e300;
+        const int slot = sp_slot(it.meta);
+        r.flags = uint32_t(slot) | SR_VALID;
+        const double prb = sm.pr[slot];
+        if (it.lb < prb - SETTLE_EPS && prb > SETTLE_EPS) {
+          if (sm.pops[slot] >= SETTLE_POPS * max(sm.roots[slot], 1)) {
+            r.lost = it.lb;  // out of reads: the point is left undecided
+          } else {
+            SOut o;
+            o.sd = 1e300, o.lost = 1e300, o.ans = false, o.unsure = false, o.over = false, o.d = 0.0, o.pa = 0.0, o.pb = 0.0, o.sgn = 1.0, o.code = NONE;
+            double pb_;
+            const v3<double> pp = job_point<S>(j, dv, point_of(sm.list[base + slot]), pb_);
+            if (it.part & SP_CURVE)
+              settle_half(tb, it, pp, prb, pool, o);
+            else
+              settle_quarter(tb, it, pp, prb, sm.got[slot] != 0, pool, o);
+            r.sd = o.sd, r.lost = o.lost;
+            if (o.ans) r.flags |= SR_ANS, r.d = o.d, r.pa = o.pa, r.pb = o.pb, r.sgn = o.sgn, r.code = o.code;
+            if (o.unsure) r.flags |= SR_UNSURE;
+            if (o.over) r.flags |= SR_OVER;
+          }
+        }
+      }
+      sm.rec[lane] = r;
+      __syncthreads();
+      // each point's lane merges its pieces' steps in lane order
+      if (own) {
+        double prl = sm.pr[lane];
+        bool gl = sm.got[lane] != 0;
+        int reads = 0;
+        for (int l = 0; l < NT; ++l) {
+          const uint32_t fl = sm.rec[l].flags;
+          if (!(fl & SR_VALID) || int(fl & 63u) != lane) continue;
+          ++reads;
+          prl = fmin(prl, sm.rec[l].sd);
+          olost = fmin(olost, sm.rec[l].lost);
+          if (fl & SR_ANS) offer(ob, sm.rec[l].d, sm.rec[l].code, sm.rec[l].pa, sm.rec[l].pb, sm.rec[l].sgn), gl = true;
+          if (fl & SR_UNSURE) ob.unsure = true;
+          if (fl & SR_OVER) sm.over[lane] = 1;
+        }
+        sm.pr[lane] = prl, sm.got[lane] = gl ? 1 : 0, sm.pops[lane] += reads;
+      }
+      __syncthreads();
+      n = min(sm.pcount, SPOOL);
+    }
+    // a point the round could not hold goes back on the list (its slot there is read);
+    // otherwise, or in rounds of one, its answer is final
+    const bool retry = own && R > 1 && sm.over[lane] != 0;
+    const unsigned long long rm = __ballot(retry);
+    const uint16_t og = own ? sm.list[base + lane] : uint16_t(0);
+    __syncthreads();
+    if (retry) sm.list[again + int(__popcll(rm & __lanemask_lt()))] = og;
+    again += int(__popcll(rm));
+    if (own && !retry) {
+      ob.lmin = fmin(ob.lmin, olost);
+      double val;
+      uint32_t flag, feat;
+      classify64(ob, obound, dv, val, flag, feat);
+      res.val[oi] = val;
+      res.flag[oi] = flag;
+      if (res.feat) res.feat[oi] = feat;
+    }
+  }
+  cnt = again;
+  if (cnt == 0) break;
   }
 }
 
@@ -1421,6 +1631,7 @@ __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const 
 // each left a few blocks running long after the rest.
 template <int S>
 __global__ __launch_bounds__(SETTLE_NT) void k_settle(Dev dv, Job j, Res res) {
+  static_assert(sizeof(SettleLds<S>) <= 40 * 1024 - 64, "four settle blocks a CU");
   __shared__ SettleLds<S> sm;
   __shared__ uint32_t wslot;
   const uint32_t total = *res.scount;

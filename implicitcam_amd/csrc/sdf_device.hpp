@@ -732,13 +732,13 @@ template <class T> __device__ inline bool is_critical(v3<T> r, v3<T> d, T tol) {
 // a curve piece's search, curve3.rs:408-476): best first over quarters (halves) of
 // the part, each read by SETTLE_NEWTON Newton steps from its middle; a critical
 // point inside the face answers, a non-critical one is carried across spans by Newton
-// first; any point of the face found lowers the part's pruning bound; a quarter is
+// first; any point of the face found lowers the point's pruning bound; a quarter is
 // dropped when its control points' box, or their slab along the direction from its
 // Newton point, keeps it at or past that bound, or at the depth limit, or when the
-// squared distance has no critical point on it (surface3.rs:694-731). A part whose
-// queue overflows, or that takes more than SETTLE_POPS reads, is left undecided with
-// its lower bound.
-constexpr int SETTLE_Q = 128;
+// squared distance has no critical point on it (surface3.rs:694-731). The pieces of a
+// block's points are held in one pool a wave works through together, a piece a lane
+// (settle_quarter, settle_half); a point whose pieces do not fit the pool, or that takes
+// more than SETTLE_POPS reads a part, is left undecided with its lower bound.
 constexpr int SETTLE_DEPTH = 40;     // surface3.rs:1291
 constexpr int SETTLE_NEWTON = 12;    // surface3.rs:552
 constexpr int SETTLE_POPS = 20000;  // surface3.rs:1230
@@ -952,10 +952,76 @@ template <class T> __device__ inline int cleaf_kind(const ICAM_K T* cb, const IC
   return piece_kind<T>(cb, el, eh);
 }
 
-struct SettleItem {
-  double a0, a1, b0, b1, lb;
-  int depth;
+// The settle's work: pieces of the parts past their certificates, a quarter of a
+// sub-patch or a half of a curve part, each with its parameter range as a dyadic step
+// of its part's own range (steps ia / 2^da .. (ia + 1) / 2^da of the range along s,
+// likewise along t). A part's range is itself a dyadic piece of its span's [0, 1], so
+// the bounds are exact and equal the repeated midpoints of the certified search.
+struct SPiece {
+  double lb;       // lower bound on the point's distance over the piece
+  uint64_t ia, ib;
+  uint32_t part;   // sub-patch, or curve part with SP_CURVE set
+  uint32_t meta;   // bits 0-5 the point's slot, 6-11 depth, 12-17 da, 18-23 db
 };
+constexpr uint32_t SP_CURVE = 0x80000000u;
+__device__ inline int sp_slot(uint32_t m) { return int(m & 63u); }
+__device__ inline int sp_depth(uint32_t m) { return int((m >> 6) & 63u); }
+__device__ inline int sp_da(uint32_t m) { return int((m >> 12) & 63u); }
+__device__ inline int sp_db(uint32_t m) { return int((m >> 18) & 63u); }
+__device__ inline uint32_t sp_meta(int slot, int depth, int da, int db) {
+  return uint32_t(slot) | (uint32_t(depth) << 6) | (uint32_t(da) << 12) | (uint32_t(db) << 18);
+}
+__device__ inline void sp_range(double lo, double hi, uint64_t i, int d, double& a0, double& a1) {
+  const double w = ldexp(hi - lo, -d);
+  a0 = lo + double(i) * w, a1 = lo + double(i + 1) * w;
+}
+// The pool pieces go to (workgroup memory) and its length, which may run past `cap`:
+// a piece past it is not kept, and the caller keeps its lower bound instead.
+struct SPool {
+  SPiece* items;
+  int* count;
+  int cap;
+};
+__device__ inline bool sp_push(const SPool& pl, const SPiece& x) {
+  const int k = atomicAdd(pl.count, 1);
+  if (k >= pl.cap) return false;
+  pl.items[k] = x;
+  return true;
+}
+// A search that meets a part past its certificate (double precision) hands it to the
+// settle as a root piece of point `slot`: its lower bound rounded down to single
+// precision, the part and the slot, in workgroup memory (`count` may run past `cap`).
+struct SSink {
+  float* lb;
+  uint32_t* part;
+  uint8_t* slot_of;
+  int* count;
+  int cap;
+  int slot;
+  uint8_t* over;  // per slot: a part was not taken
+};
+// What one piece's step found for its point.
+struct SOut {
+  double sd;    // least distance of a point of the face (edge) found: a bound on the answer
+  double lost;  // least lower bound of the pieces the pool could not keep
+  double d, pa, pb, sgn;  // the answer, when `ans`
+  uint32_t code;
+  bool ans, unsure;
+  bool over;    // a piece was not kept
+};
+// Part `part` (past its certificate, lower bound lb) handed to the settle; kept in the
+// point's undecided bound when the settle cannot take it.
+__device__ inline void settle_root(const SSink* sk, double lb, uint32_t part, Best<double>& b) {
+  if (sk) {
+    const int k = atomicAdd(sk->count, 1);
+    if (k < sk->cap) {
+      sk->lb[k] = __double2float_rd(lb), sk->part[k] = part, sk->slot_of[k] = uint8_t(sk->slot);
+      return;
+    }
+    sk->over[sk->slot] = 1;
+  }
+  b.lmin = fmin(b.lmin, lb);
+}
 
 // Interval arithmetic for the settle's per-part tests (the bounds the packer keeps for
 // a leaf, made here for a quarter from its own net; surface3.rs:866-970).
@@ -1187,9 +1253,22 @@ __device__ inline bool no_critical64(const double n[4][4][4], v3<double> p, doub
   return true;
 }
 
-// Sub-patch li settled for p (its lower bound lb); false when left incomplete.
-__device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb, v3<double> p, Best<double>& b) {
+// One quarter of sub-patch it.part read for p, a step of the certified search: Newton
+// from its middle; a critical point inside the face answers (`o.ans`), a non-critical
+// one is carried across spans by Newton first; any point of the face found lowers the
+// pruning bound (`o.sd`); the quarter is dropped when its control points' box, or their
+// slab along the direction from its Newton point, keeps it at or past that bound, at
+// the depth limit, when its own interval test finds no interior minimum or a single one
+// (its Newton point), or when the squared distance has no critical point on it;
+// otherwise its four children go to the pool (two, split across the side, at a pole).
+// `prune` is the point's bound when the step began, `got` whether the settle answered
+// for the point already (a critical point within SETTLE_EPS of a bound no answer set
+// yet is taken: a foot on the shared border of two quarters stops the first quarter's
+// Newton at the border, not critical, which sets the bound that the next quarter's
+// critical point only ties).
+__device__ inline void settle_quarter(const Tab<double>& tb, const SPiece& it, v3<double> p, double prune, bool got, const SPool& pool, SOut& o) {
   const Layout& L = tb.L;
+  const uint32_t li = it.part;
   const ICAM_K double* lf = tb.F + L.leaf_f + LEAF_F * li;
   const ICAM_K uint32_t* lu = tb.U + L.leaf_u + LEAF_U * li;
   const ICAM_K double* sp = tb.F + L.span_f + SPAN_F * lu[0];
@@ -1197,122 +1276,86 @@ __device__ inline bool settle_leaf(const Tab<double>& tb, uint32_t li, double lb
   const ICAM_K double* f = tb.F + L.face_f + FACE_F * fi;
   const ICAM_K uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
   const bool known_in = lu[2] != 0;
-  SettleItem q[SETTLE_Q];
-  int nq = 1, pops = 0;
-  q[0] = SettleItem{lf[1], lf[2], lf[3], lf[4], lb, 0};
-  double bd = b.d;
-  bool complete = true, got = false;
-  while (nq > 0) {
-    int k = 0;
-    for (int i = 1; i < nq; ++i)
-      if (q[i].lb < q[k].lb) k = i;
-    const SettleItem it = q[k];
-    q[k] = q[--nq];
-    if (it.lb >= bd - SETTLE_EPS || bd <= SETTLE_EPS) break;
-    if (++pops > SETTLE_POPS) {
-      complete = false;
-      break;
-    }
-    double s = 0.5 * (it.a0 + it.a1), t = 0.5 * (it.b0 + it.b1);
-    newton2(sp, p, s, t, it.a0, it.a1, it.b0, it.b1, SETTLE_NEWTON, 1e-13 * (it.a1 - it.a0));
-    SEval<double> e = peval(sp, s, t);
-    v3<double> r = e.p - p;
-    const double d = len(r);
-    v3<double> nrm = cross(e.ps, e.pt);
-    const double u = sp[64] + s * (sp[65] - sp[64]), v = sp[66] + t * (sp[67] - sp[66]);
-    bool near = false;
-    auto inside = [&](double uu, double vv) { return known_in || fu[2] != 0 || trim_inside(tb, f + 17, fu[1], uu, vv, 0.0, near); };
-    // (a critical point within SETTLE_EPS of a bound no answer of this part set yet, and
-    // a polished one within SETTLE_EPS of its start, are taken: a foot on the shared
-    // border of two quarters stops the first quarter's Newton at the border, not
-    // critical, which sets the bound that the next quarter's critical point only ties)
-    if (d < bd + (got ? 0.0 : SETTLE_EPS) && inside(u, v)) {
-      bd = fmin(bd, d);
-      const bool at_pole = lf[LEAF_POLE + 3] != 0.0 && len(e.p - ld3(lf + LEAF_POLE)) <= 1e-9 * (1.0 + len(ld3(lf + LEAF_POLE)));
-      if (critical64(p, e.p, r, e.ps) && critical64(p, e.p, r, e.pt)) {
-        if (len(nrm) > 0.0 && !at_pole) offer(b, d, (P_LEAF << 28) | li, s, t, sgn_of(dot(r, nrm) * -f[14])), got = true;
-      } else {
-        double hd, hu, hv, hs;
-        if (polish64(tb, fi, p, u, v, hd, hu, hv, hs) && hd <= d + SETTLE_EPS && inside(hu, hv)) {
-          bd = fmin(bd, hd);
-          offer(b, hd, (P_LEAF << 28) | li, s, t, hs * f[14]);
-          got = true;
-        }
-      }
-    }
-    double net[4][4][4];
-    sub_net(sp, it.a0, it.a1, it.b0, it.b1, net);
-    const v3<double> dir = d > 1e-12 ? (p - e.p) * (1.0 / d) : (len(nrm) > 0.0 ? nrm * (1.0 / len(nrm)) : mk3(0.0, 0.0, 1.0));
-    double box, slab;
-    hull_bounds(&net[0][0], 16, p, dir, box, slab);
-    if (slab >= bd - SETTLE_EPS || it.depth >= SETTLE_DEPTH) continue;
-    // a quarter touching a collapsed side (a pole, answered by its own point record):
-    // never cleared by the critical-point test (surface3.rs:1209) and split across the
-    // side only, so the quarters at the pole stay one chain instead of doubling
-    const int pside = int(lf[LEAF_POLE + 3]);
-    const bool touch = (pside == 1 && it.a0 == 0.0) || (pside == 2 && it.a1 == 1.0) || (pside == 3 && it.b0 == 0.0) || (pside == 4 && it.b1 == 1.0);
-    if (touch) {
-      const bool along_t = pside >= 3;
-      const double am = 0.5 * (it.a0 + it.a1), bm = 0.5 * (it.b0 + it.b1);
-      for (int c = 0; c < 2; ++c) {
-        SettleItem ch = it;
-        ch.depth = it.depth + 1;
-        if (along_t)
-          (c ? ch.b0 : ch.b1) = bm;
-        else
-          (c ? ch.a0 : ch.a1) = am;
-        double cn[4][4][4], cbx, cs;
-        sub_net(sp, ch.a0, ch.a1, ch.b0, ch.b1, cn);
-        hull_bounds(&cn[0][0], 16, p, dir, cbx, cs);
-        ch.lb = fmax(cbx, slab);
-        if (ch.lb < bd - SETTLE_EPS) {
-          if (nq == SETTLE_Q) {
-            complete = false;
-            continue;
-          }
-          q[nq++] = ch;
-        }
-      }
-      continue;
-    }
-    {
-      // the quarter's own interval test (gpu.wgsl:1581-1600 on its net): no interior
-      // minimum drops it; strictly convex for p, its minimum is the Newton point above
-      double cb[31], el[3], eh[3];
-      if (net_bounds(net, p, cb, el, eh) && patch_kind<double>((const double*)cb, el, eh) != 0) continue;
-    }
-    const double am = 0.5 * (it.a0 + it.a1), bm = 0.5 * (it.b0 + it.b1);
-    {
-      // no critical point on the quarter: no minimum of its own to find there
-      SEval<double> em = peval(sp, am, bm);
-      const v3<double> rm = em.p - p;
-      const double ga = dot(em.ps, rm) * (it.a1 - it.a0), gb = dot(em.pt, rm) * (it.b1 - it.b0), gl = sqrt(ga * ga + gb * gb);
-      if (gl > 0.0 && no_critical64(net, p, ga / gl, gb / gl)) continue;
-    }
-    for (int c = 0; c < 4; ++c) {
-      SettleItem ch{(c & 1) ? am : it.a0, (c & 1) ? it.a1 : am, (c & 2) ? bm : it.b0, (c & 2) ? it.b1 : bm, 0.0, it.depth + 1};
-      double cn[4][4][4], cb, cs;
-      sub_net(sp, ch.a0, ch.a1, ch.b0, ch.b1, cn);
-      hull_bounds(&cn[0][0], 16, p, dir, cb, cs);
-      ch.lb = fmax(cb, slab);
-      if (ch.lb < bd - SETTLE_EPS) {
-        if (nq == SETTLE_Q) {
-          complete = false;
-          continue;
-        }
-        q[nq++] = ch;
+  const int slot = sp_slot(it.meta), depth = sp_depth(it.meta), da = sp_da(it.meta), db = sp_db(it.meta);
+  double a0, a1, b0, b1;
+  sp_range(lf[1], lf[2], it.ia, da, a0, a1);
+  sp_range(lf[3], lf[4], it.ib, db, b0, b1);
+  double bd = prune;
+  double s = 0.5 * (a0 + a1), t = 0.5 * (b0 + b1);
+  newton2(sp, p, s, t, a0, a1, b0, b1, SETTLE_NEWTON, 1e-13 * (a1 - a0));
+  SEval<double> e = peval(sp, s, t);
+  v3<double> r = e.p - p;
+  const double d = len(r);
+  v3<double> nrm = cross(e.ps, e.pt);
+  const double u = sp[64] + s * (sp[65] - sp[64]), v = sp[66] + t * (sp[67] - sp[66]);
+  bool near = false;
+  auto inside = [&](double uu, double vv) { return known_in || fu[2] != 0 || trim_inside(tb, f + 17, fu[1], uu, vv, 0.0, near); };
+  if (d < bd + (got ? 0.0 : SETTLE_EPS) && inside(u, v)) {
+    bd = fmin(bd, d);
+    o.sd = fmin(o.sd, d);
+    const bool at_pole = lf[LEAF_POLE + 3] != 0.0 && len(e.p - ld3(lf + LEAF_POLE)) <= 1e-9 * (1.0 + len(ld3(lf + LEAF_POLE)));
+    if (critical64(p, e.p, r, e.ps) && critical64(p, e.p, r, e.pt)) {
+      if (len(nrm) > 0.0 && !at_pole) o.ans = true, o.d = d, o.code = (P_LEAF << 28) | li, o.pa = s, o.pb = t, o.sgn = sgn_of(dot(r, nrm) * -f[14]);
+    } else {
+      double hd, hu, hv, hs;
+      if (polish64(tb, fi, p, u, v, hd, hu, hv, hs) && hd <= d + SETTLE_EPS && inside(hu, hv)) {
+        bd = fmin(bd, hd);
+        o.sd = fmin(o.sd, hd);
+        o.ans = true, o.d = hd, o.code = (P_LEAF << 28) | li, o.pa = s, o.pb = t, o.sgn = hs * f[14];
       }
     }
   }
-  return complete;
+  double net[4][4][4];
+  sub_net(sp, a0, a1, b0, b1, net);
+  const v3<double> dir = d > 1e-12 ? (p - e.p) * (1.0 / d) : (len(nrm) > 0.0 ? nrm * (1.0 / len(nrm)) : mk3(0.0, 0.0, 1.0));
+  double box, slab;
+  hull_bounds(&net[0][0], 16, p, dir, box, slab);
+  if (slab >= bd - SETTLE_EPS || depth >= SETTLE_DEPTH) return;
+  auto child = [&](uint64_t ia, int cda, uint64_t ib, int cdb) {
+    double c0, c1, e0, e1, cn[4][4][4], cb, cs;
+    sp_range(lf[1], lf[2], ia, cda, c0, c1);
+    sp_range(lf[3], lf[4], ib, cdb, e0, e1);
+    sub_net(sp, c0, c1, e0, e1, cn);
+    hull_bounds(&cn[0][0], 16, p, dir, cb, cs);
+    const double lb = fmax(cb, slab);
+    if (lb < bd - SETTLE_EPS && !sp_push(pool, SPiece{lb, ia, ib, li, sp_meta(slot, depth + 1, cda, cdb)})) o.lost = fmin(o.lost, lb), o.over = true;
+  };
+  // a quarter touching a collapsed side (a pole, answered by its own point record):
+  // never cleared by the critical-point test (surface3.rs:1209) and split across the
+  // side only, so the quarters at the pole stay one chain instead of doubling
+  const int pside = int(lf[LEAF_POLE + 3]);
+  const bool touch = (pside == 1 && a0 == 0.0) || (pside == 2 && a1 == 1.0) || (pside == 3 && b0 == 0.0) || (pside == 4 && b1 == 1.0);
+  if (touch) {
+    for (int c = 0; c < 2; ++c) {
+      if (pside >= 3)
+        child(it.ia, da, 2 * it.ib + uint64_t(c), db + 1);
+      else
+        child(2 * it.ia + uint64_t(c), da + 1, it.ib, db);
+    }
+    return;
+  }
+  {
+    // the quarter's own interval test (gpu.wgsl:1581-1600 on its net): no interior
+    // minimum drops it; strictly convex for p, its minimum is the Newton point above
+    double cb[31], el[3], eh[3];
+    if (net_bounds(net, p, cb, el, eh) && patch_kind<double>((const double*)cb, el, eh) != 0) return;
+  }
+  {
+    // no critical point on the quarter: no minimum of its own to find there
+    SEval<double> em = peval(sp, 0.5 * (a0 + a1), 0.5 * (b0 + b1));
+    const v3<double> rm = em.p - p;
+    const double ga = dot(em.ps, rm) * (a1 - a0), gb = dot(em.pt, rm) * (b1 - b0), gl = sqrt(ga * ga + gb * gb);
+    if (gl > 0.0 && no_critical64(net, p, ga / gl, gb / gl)) return;
+  }
+  for (int c = 0; c < 4; ++c) child(2 * it.ia + uint64_t(c & 1), da + 1, 2 * it.ib + uint64_t((c >> 1) & 1), db + 1);
 }
 
 // gpu.wgsl:679-756 / solid3.rs:1904-1910: a sub-patch read by Newton when its radius
 // certifies a single minimum for p; the minimum answers when it is a critical point
 // inside the face. With `settle` (double precision), a sub-patch past its certificate
-// is settled by the certified search instead of leaving its lower bound.
+// is handed to the settle (sk) instead of leaving its lower bound.
 template <class T, bool SETTLE = false>
-__device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps) {
+__device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps, const SSink* sk = nullptr) {
   const Layout& L = tb.L;
   const ICAM_K T* lf = tb.F + L.leaf_f + LEAF_F * li;
   const ICAM_K uint32_t* lu = tb.U + L.leaf_u + LEAF_U * li;
@@ -1327,7 +1370,7 @@ __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box,
   }
   if constexpr (SETTLE && sizeof(T) == 8) {
     if (!certified) {
-      if (!settle_leaf(tb, li, lb, p, b)) b.lmin = mmin(b.lmin, lb);
+      settle_root(sk, lb, li, b);
       return;
     }
   }
@@ -1372,11 +1415,15 @@ __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box,
   offer(b, d, (P_LEAF << 28) | li, s, t, sgn_of(dot(r, n) * -f[14]));
 }
 
-// Curve part ci settled for p (its lower bound lb): best first over halves of the
-// part, each read by Newton from its middle; a critical point interior to its edge
-// answers; false when left incomplete.
-__device__ inline bool settle_cleaf(const Tab<double>& tb, uint32_t ci, double lb, v3<double> p, Best<double>& b) {
+// One half of curve part it.part read for p, a step of a curve piece's certified
+// search: Newton from its middle; a critical point interior to its edge answers (a
+// point where two pieces of the edge meet answers critical or not; a C0 line's point
+// only inside its face's loops); the half is dropped at the depth limit or when its own
+// interval test finds no interior minimum or a single one; otherwise its two halves go
+// to the pool when their hull can come under the bound.
+__device__ inline void settle_half(const Tab<double>& tb, const SPiece& it, v3<double> p, double prune, const SPool& pool, SOut& o) {
   const Layout& L = tb.L;
+  const uint32_t ci = it.part & ~SP_CURVE;
   const ICAM_K double* cf = tb.F + L.cleaf_f + CLEAF_F * ci;
   const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * ci;
   const uint32_t cp = cu[0];
@@ -1384,74 +1431,54 @@ __device__ inline bool settle_cleaf(const Tab<double>& tb, uint32_t ci, double l
   const ICAM_K uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cp;
   const ICAM_K uint32_t* eu = tb.U + L.edge_u + EDGE_U * pu[0];
   const bool crease = eu[0] == EDGE_CREASE;
-  SettleItem q[SETTLE_Q];
-  int nq = 1, pops = 0;
-  q[0] = SettleItem{cf[1], cf[2], 0.0, 0.0, lb, 0};
-  double bd = b.d;
-  bool complete = true;
-  while (nq > 0) {
-    int k = 0;
-    for (int i = 1; i < nq; ++i)
-      if (q[i].lb < q[k].lb) k = i;
-    const SettleItem it = q[k];
-    q[k] = q[--nq];
-    if (it.lb >= bd - SETTLE_EPS || bd <= SETTLE_EPS) break;
-    if (++pops > SETTLE_POPS) {
-      complete = false;
-      break;
-    }
-    const double s = newton1(pw, p, 0.5 * (it.a0 + it.a1), it.a0, it.a1, 40, 1e-15 * (it.a1 - it.a0));
-    CEval<double> e = ceval(pw, s);
-    v3<double> r = e.p - p;
-    const double d = len(r);
-    if (d < bd) {
-      bd = d;
-      // an edge's ends are its vertices', a junction of two of its pieces answers critical
-      // or not; a C0 line's point answers inside its face's loops
-      const bool end = !crease && ((pu[1] == 0 && s <= 0.0) || (pu[1] + 1 == pu[2] && s >= 1.0));
-      const bool junction = !crease && ((pu[1] > 0 && s <= 0.0) || (pu[1] + 1 < pu[2] && s >= 1.0));
-      bool near = false;
-      if (!end && (junction || critical64(p, e.p, r, e.d1)) && (!crease || crease_inside(tb, eu, cp, s, 0.0, near))) {
-        v3<double> pn = edge_pn(tb, pu[0], e.p, pu[1], s);
-        if (len(pn) < 1e-6) b.unsure = true;
-        offer(b, d, (P_CLEAF << 28) | ci, s, 0.0, sgn_of(dot(p - e.p, pn)));
-      }
-    }
-    if (it.depth >= SETTLE_DEPTH) continue;
-    {
-      // the half's own interval test (gpu.wgsl:1602-1614 on its net)
-      double cc[4][4], cb[13], el[3], eh[3];
-      for (int i = 0; i < 4; ++i)
-        for (int kk = 0; kk < 4; ++kk) cc[i][kk] = pw[4 * i + kk];
-      sub_cubic4(cc, it.a0, it.a1);
-      if (cnet_bounds(cc, p, cb, el, eh) && piece_kind<double>((const double*)cb, el, eh) != 0) continue;
-    }
-    const double m = 0.5 * (it.a0 + it.a1);
-    const v3<double> dir = d > 1e-12 ? (p - e.p) * (1.0 / d) : mk3(0.0, 0.0, 1.0);
-    for (int c = 0; c < 2; ++c) {
-      SettleItem ch{c ? m : it.a0, c ? it.a1 : m, 0.0, 0.0, 0.0, it.depth + 1};
-      double cc[4][4];
-      for (int i = 0; i < 4; ++i)
-        for (int kk = 0; kk < 4; ++kk) cc[i][kk] = pw[4 * i + kk];
-      sub_cubic4(cc, ch.a0, ch.a1);
-      double cb, cs;
-      hull_bounds(cc, 4, p, dir, cb, cs);
-      ch.lb = fmax(cb, cs);
-      if (ch.lb < bd - SETTLE_EPS) {
-        if (nq == SETTLE_Q) {
-          complete = false;
-          continue;
-        }
-        q[nq++] = ch;
-      }
+  const int slot = sp_slot(it.meta), depth = sp_depth(it.meta), da = sp_da(it.meta);
+  double a0, a1;
+  sp_range(cf[1], cf[2], it.ia, da, a0, a1);
+  double bd = prune;
+  const double s = newton1(pw, p, 0.5 * (a0 + a1), a0, a1, 40, 1e-15 * (a1 - a0));
+  CEval<double> e = ceval(pw, s);
+  v3<double> r = e.p - p;
+  const double d = len(r);
+  if (d < bd) {
+    bd = d;
+    o.sd = fmin(o.sd, d);
+    // an edge's ends are its vertices', a junction of two of its pieces answers critical
+    // or not; a C0 line's point answers inside its face's loops
+    const bool end = !crease && ((pu[1] == 0 && s <= 0.0) || (pu[1] + 1 == pu[2] && s >= 1.0));
+    const bool junction = !crease && ((pu[1] > 0 && s <= 0.0) || (pu[1] + 1 < pu[2] && s >= 1.0));
+    bool near = false;
+    if (!end && (junction || critical64(p, e.p, r, e.d1)) && (!crease || crease_inside(tb, eu, cp, s, 0.0, near))) {
+      v3<double> pn = edge_pn(tb, pu[0], e.p, pu[1], s);
+      if (len(pn) < 1e-6) o.unsure = true;
+      o.ans = true, o.d = d, o.code = (P_CLEAF << 28) | ci, o.pa = s, o.pb = 0.0, o.sgn = sgn_of(dot(p - e.p, pn));
     }
   }
-  return complete;
+  if (depth >= SETTLE_DEPTH) return;
+  {
+    // the half's own interval test (gpu.wgsl:1602-1614 on its net)
+    double cc[4][4], cb[13], el[3], eh[3];
+    for (int i = 0; i < 4; ++i)
+      for (int kk = 0; kk < 4; ++kk) cc[i][kk] = pw[4 * i + kk];
+    sub_cubic4(cc, a0, a1);
+    if (cnet_bounds(cc, p, cb, el, eh) && piece_kind<double>((const double*)cb, el, eh) != 0) return;
+  }
+  const v3<double> dir = d > 1e-12 ? (p - e.p) * (1.0 / d) : mk3(0.0, 0.0, 1.0);
+  for (int c = 0; c < 2; ++c) {
+    const uint64_t ia = 2 * it.ia + uint64_t(c);
+    double c0, c1, cc[4][4], cb, cs;
+    sp_range(cf[1], cf[2], ia, da + 1, c0, c1);
+    for (int i = 0; i < 4; ++i)
+      for (int kk = 0; kk < 4; ++kk) cc[i][kk] = pw[4 * i + kk];
+    sub_cubic4(cc, c0, c1);
+    hull_bounds(cc, 4, p, dir, cb, cs);
+    const double lb = fmax(cb, cs);
+    if (lb < bd - SETTLE_EPS && !sp_push(pool, SPiece{lb, ia, 0, it.part, sp_meta(slot, depth + 1, da + 1, 0)})) o.lost = fmin(o.lost, lb), o.over = true;
+  }
 }
 
 // gpu.wgsl:764-815: a part of a curve piece read by Newton when certified; the minimum
 // answers when it is critical and interior to its edge. With `settle` (double
-// precision), a part past its certificate is settled by the certified search.
+// precision), a part past its certificate is handed to the settle (sk).
 // An edge's nearest point may also be where two of its pieces meet in a corner, where
 // neither piece is critical (curve3.rs:408-476 takes any point interior to the edge): a
 // part ending at such a junction answers with its nearest point there, critical or not,
@@ -1459,7 +1486,7 @@ __device__ inline bool settle_cleaf(const Tab<double>& tb, uint32_t ci, double l
 // it never undercuts the true nearest point, which is read wherever it lies. A C0 knot
 // line's points answer only where critical (solid3.rs:1866-1872).
 template <class T, bool SETTLE = false>
-__device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps) {
+__device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps, const SSink* sk = nullptr) {
   const Layout& L = tb.L;
   const ICAM_K T* cf = tb.F + L.cleaf_f + CLEAF_F * ci;
   const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * ci;
@@ -1488,7 +1515,7 @@ __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box
   }
   if constexpr (SETTLE && sizeof(T) == 8) {
     if (!certified) {
-      if (!settle_cleaf(tb, ci, lb, p, b)) b.lmin = mmin(b.lmin, lb);
+      settle_root(sk, lb, ci | SP_CURVE, b);
       return;
     }
   }
