@@ -538,7 +538,7 @@ __device__ inline T part_lb(const Tab<T>& tb, uint32_t i, uint32_t code, v3<T> p
   return lb;
 }
 template <class T, bool SETTLE = false>
-__device__ inline void try_part(const Tab<T>& tb, uint32_t i, T lb, v3<T> p, Best<T>& b, T tm, T eps, const SSink* sk = nullptr) {
+__device__ inline void try_part(const Tab<T>& tb, uint32_t i, T lb, v3<T> p, Best<T>& b, T tm, T eps, SSink* sk = nullptr) {
   const uint32_t code = tb.pcode[i], id = code & 0x0fffffffu;
   if ((code >> 28) == P_LEAF)
     try_leaf<T, SETTLE>(tb, id, tb.pbox + 6 * i, lb, p, b, tm, eps, sk);
@@ -613,7 +613,7 @@ template <class T> __device__ inline Best<T> best_init(T bound) {
 // the final answer is still read, in the chunk that holds it).
 template <class T, int NT, int Q, int K, int C, bool SETTLE = false>
 __device__ inline void search_in(const Tab<T>& tb, v3<T> p, bool valid, Best<T>& b, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk,
-                                 const Res* defer = nullptr, long point = 0, const SSink* sk = nullptr) {
+                                 const Res* defer = nullptr, long point = 0, SSink* sk = nullptr) {
   const bool all = nc > C;
   const uint32_t n = all ? tb.L.n_prims : uint32_t(nc);
   const int lane = threadIdx.x, sub = lane & (K - 1);
@@ -674,7 +674,9 @@ __device__ inline void search_in(const Tab<T>& tb, v3<T> p, bool valid, Best<T>&
   // (the least becomes lmin), so the point is undecided rather than taken as exact.
   const int max_rounds = int(tb.L.max_rounds);
   for (int round = 0;; ++round) {
-    T next_lb = K > 1 ? group_min<K>(b.d) : b.d;
+    T bd = b.d;
+    if constexpr (SETTLE && sizeof(T) == 8) bd = sk ? mmin(bd, T(sk->ub)) : bd;
+    T next_lb = K > 1 ? group_min<K>(bd) : bd;
     int next = -1;
     if (parts && !scan) {
       for (int q = 0; q < m; ++q) {
@@ -735,7 +737,7 @@ template <class T, int K> __device__ inline void group_merge(Best<T>& b, T eps) 
 // A point's whole search over one candidate list.
 template <class T, int NT, int Q, int K, int C, bool SETTLE = false>
 __device__ Best<T> search(const Tab<T>& tb, v3<T> p, bool valid, T bound, const uint32_t* cand, int nc, T tm, T eps, T* qlb, uint16_t* qk,
-                          const Res* defer = nullptr, long point = 0, const SSink* sk = nullptr) {
+                          const Res* defer = nullptr, long point = 0, SSink* sk = nullptr) {
   Best<T> b = best_init<T>(bound);
   search_in<T, NT, Q, K, C, SETTLE>(tb, p, valid, b, cand, nc, tm, eps, qlb, qk, defer, point, sk);
   if (K > 1) group_merge<T, K>(b, eps);
@@ -1006,7 +1008,7 @@ template <int S, class T> struct SearchLds {
 // point's answer so far) is passed over.
 template <int S, class T, bool SETTLE, int NT = Shape<S>::NT, int K = Shape<S>::K, class L = SearchLds<S, T>>
 __device__ Best<T> search_chunks(const Dev& dv, const Tab<T>& tb, v3<T> p, bool valid, T bound, L& sm, int nchunk, const float* gb, float r,
-                                 T tm, T eps, const SSink* sk = nullptr) {
+                                 T tm, T eps, SSink* sk = nullptr) {
   constexpr int Q = Shape<S>::Q, C = Shape<S>::C;
   Best<T> b = best_init<T>(bound);
   if constexpr (!pts<S>()) {
@@ -1440,7 +1442,7 @@ __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const 
     double bound = 0.0;
     const v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
     const double sb = bound < 1e300 ? bound * (1.0 + 1e-12) : bound;
-    const SSink sink{sm.rlb, sm.rpart, sm.rslot, &sm.rcount, SettleLds<S>::ROOTS, g, sm.over};
+    SSink sink{sm.rlb, sm.rpart, sm.rslot, &sm.rcount, SettleLds<S>::ROOTS, g, sm.over, 1e300};
     Best<double> b;
     if constexpr (pts<S>()) {
       b = chunked ? search_chunks<S, double, true, NT, K>(dv, tb, p, valid, sb, sm, nchunk, gb, rr, 0.0, dv.L.eps64, &sink)
@@ -1448,13 +1450,16 @@ __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const 
     } else {
       b = search<double, NT, Q, K, C, true>(tb, p, valid, sb, sm.cand, nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk(), nullptr, 0, &sink);
     }
-    // lane g < npt holds point g's answer from here on (its group's first lane has it)
+    // lane g < npt holds point g's answer and its probes' bound from here on (its
+    // group's first lane has them)
     Best<double> ob = b;
+    double oub = K > 1 ? group_min<K>(sink.ub) : sink.ub;
     if constexpr (K > 1) {
       const int src = (lane * K) & (NT - 1);
       ob.d = __shfl(b.d, src), ob.code = uint32_t(__shfl(int(b.code), src)), ob.a = __shfl(b.a, src), ob.b = __shfl(b.b, src);
       ob.sgn = __shfl(b.sgn, src), ob.lmin = __shfl(b.lmin, src), ob.d2 = __shfl(b.d2, src), ob.sgn2 = __shfl(b.sgn2, src);
       ob.unsure = __shfl(int(b.unsure), src) != 0, ob.host = __shfl(int(b.host), src) != 0;
+      oub = __shfl(oub, src);
     }
     const bool own = lane < npt;
     long oi = -1;
@@ -1462,7 +1467,7 @@ __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const 
     if (own) {
       oi = point_of(sm.list[base + lane]);
       job_point<S>(j, dv, oi, obound);
-      sm.pr[lane] = ob.d;
+      sm.pr[lane] = fmin(ob.d, oub);
     }
     __syncthreads();
     // the root pieces: counted by point, then let into the pool (whose memory the search
@@ -1603,9 +1608,10 @@ __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const 
       __syncthreads();
       n = min(sm.pcount, SPOOL);
     }
-    // a point the round could not hold goes back on the list (its slot there is read);
-    // otherwise, or in rounds of one, its answer is final
-    const bool retry = own && R > 1 && sm.over[lane] != 0;
+    // a point the round could not hold, whose dropped parts or pieces could still come
+    // under its answer, goes back on the list (its slot there is read); otherwise, or in
+    // rounds of one, its answer is final
+    const bool retry = own && R > 1 && sm.over[lane] != 0 && fmin(ob.lmin, olost) < ob.d;
     const unsigned long long rm = __ballot(retry);
     const uint16_t og = own ? sm.list[base + lane] : uint16_t(0);
     __syncthreads();

@@ -999,6 +999,7 @@ struct SSink {
   int cap;
   int slot;
   uint8_t* over;  // per slot: a part was not taken
+  double ub;      // the least distance of a point of the part's face (edge) its probe found
 };
 // What one piece's step found for its point.
 struct SOut {
@@ -1011,7 +1012,7 @@ struct SOut {
 };
 // Part `part` (past its certificate, lower bound lb) handed to the settle; kept in the
 // point's undecided bound when the settle cannot take it.
-__device__ inline void settle_root(const SSink* sk, double lb, uint32_t part, Best<double>& b) {
+__device__ inline void settle_root(SSink* sk, double lb, uint32_t part, Best<double>& b) {
   if (sk) {
     const int k = atomicAdd(sk->count, 1);
     if (k < sk->cap) {
@@ -1021,6 +1022,63 @@ __device__ inline void settle_root(const SSink* sk, double lb, uint32_t part, Be
     sk->over[sk->slot] = 1;
   }
   b.lmin = fmin(b.lmin, lb);
+}
+// The first step of a part's settle taken when the search meets it: Newton from its
+// middle. A point of the face found there bounds the point's answer (sk->ub: the search
+// passes over parts at or past it, as the settle prunes its pieces), and a critical one
+// answers. Without it the search, past parts it could not read, would read every part
+// under the answers of the parts it could.
+__device__ inline void settle_probe_leaf(const Tab<double>& tb, uint32_t li, v3<double> p, Best<double>& b, SSink* sk) {
+  if (!sk) return;
+  const Layout& L = tb.L;
+  const ICAM_K double* lf = tb.F + L.leaf_f + LEAF_F * li;
+  const ICAM_K uint32_t* lu = tb.U + L.leaf_u + LEAF_U * li;
+  const ICAM_K double* sp = tb.F + L.span_f + SPAN_F * lu[0];
+  const uint32_t fi = lu[1];
+  const ICAM_K double* f = tb.F + L.face_f + FACE_F * fi;
+  const ICAM_K uint32_t* fu = tb.U + L.face_u + FACE_U * fi;
+  double s = 0.5 * (lf[1] + lf[2]), t = 0.5 * (lf[3] + lf[4]);
+  newton2(sp, p, s, t, lf[1], lf[2], lf[3], lf[4], SETTLE_NEWTON, 1e-13 * (lf[2] - lf[1]));
+  const SEval<double> e = peval(sp, s, t);
+  const v3<double> r = e.p - p;
+  const double d = len(r);
+  if (!(d < fmin(b.d, sk->ub))) return;
+  const double u = sp[64] + s * (sp[65] - sp[64]), v = sp[66] + t * (sp[67] - sp[66]);
+  bool near = false;
+  if (!(lu[2] != 0 || fu[2] != 0 || trim_inside(tb, f + 17, fu[1], u, v, 0.0, near))) return;
+  sk->ub = d;
+  const v3<double> nrm = cross(e.ps, e.pt);
+  const bool at_pole = lf[LEAF_POLE + 3] != 0.0 && len(e.p - ld3(lf + LEAF_POLE)) <= 1e-9 * (1.0 + len(ld3(lf + LEAF_POLE)));
+  if (critical64(p, e.p, r, e.ps) && critical64(p, e.p, r, e.pt) && len(nrm) > 0.0 && !at_pole)
+    offer(b, d, (P_LEAF << 28) | li, s, t, sgn_of(dot(r, nrm) * -f[14]));
+}
+// The same for curve part ci: a point of an edge is a point of the boundary (of a C0
+// line, when inside its face's loops).
+__device__ inline void settle_probe_curve(const Tab<double>& tb, uint32_t ci, v3<double> p, Best<double>& b, SSink* sk) {
+  if (!sk) return;
+  const Layout& L = tb.L;
+  const ICAM_K double* cf = tb.F + L.cleaf_f + CLEAF_F * ci;
+  const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * ci;
+  const uint32_t cp = cu[0];
+  const ICAM_K double* pw = tb.F + L.cpiece_f + CPIECE_F * cp;
+  const ICAM_K uint32_t* pu = tb.U + L.cpiece_u + CPIECE_U * cp;
+  const ICAM_K uint32_t* eu = tb.U + L.edge_u + EDGE_U * pu[0];
+  const bool crease = eu[0] == EDGE_CREASE;
+  const double s = newton1(pw, p, 0.5 * (cf[1] + cf[2]), cf[1], cf[2], 40, 1e-15 * (cf[2] - cf[1]));
+  const CEval<double> e = ceval(pw, s);
+  const v3<double> r = e.p - p;
+  const double d = len(r);
+  if (!(d < fmin(b.d, sk->ub))) return;
+  bool near = false;
+  if (crease && !crease_inside(tb, eu, cp, s, 0.0, near)) return;
+  sk->ub = d;
+  const bool end = !crease && ((pu[1] == 0 && s <= 0.0) || (pu[1] + 1 == pu[2] && s >= 1.0));
+  const bool junction = !crease && ((pu[1] > 0 && s <= 0.0) || (pu[1] + 1 < pu[2] && s >= 1.0));
+  if (!end && (junction || critical64(p, e.p, r, e.d1))) {
+    const v3<double> pn = edge_pn(tb, pu[0], e.p, pu[1], s);
+    if (len(pn) < 1e-6) b.unsure = true;
+    offer(b, d, (P_CLEAF << 28) | ci, s, 0.0, sgn_of(dot(p - e.p, pn)));
+  }
 }
 
 // Interval arithmetic for the settle's per-part tests (the bounds the packer keeps for
@@ -1355,7 +1413,7 @@ __device__ inline void settle_quarter(const Tab<double>& tb, const SPiece& it, v
 // inside the face. With `settle` (double precision), a sub-patch past its certificate
 // is handed to the settle (sk) instead of leaving its lower bound.
 template <class T, bool SETTLE = false>
-__device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps, const SSink* sk = nullptr) {
+__device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps, SSink* sk = nullptr) {
   const Layout& L = tb.L;
   const ICAM_K T* lf = tb.F + L.leaf_f + LEAF_F * li;
   const ICAM_K uint32_t* lu = tb.U + L.leaf_u + LEAF_U * li;
@@ -1370,6 +1428,7 @@ __device__ void try_leaf(const Tab<T>& tb, uint32_t li, const ICAM_K float* box,
   }
   if constexpr (SETTLE && sizeof(T) == 8) {
     if (!certified) {
+      settle_probe_leaf(tb, li, p, b, sk);
       settle_root(sk, lb, li, b);
       return;
     }
@@ -1486,7 +1545,7 @@ __device__ inline void settle_half(const Tab<double>& tb, const SPiece& it, v3<d
 // it never undercuts the true nearest point, which is read wherever it lies. A C0 knot
 // line's points answer only where critical (solid3.rs:1866-1872).
 template <class T, bool SETTLE = false>
-__device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps, const SSink* sk = nullptr) {
+__device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box, T lb, v3<T> p, Best<T>& b, T tm, T eps, SSink* sk = nullptr) {
   const Layout& L = tb.L;
   const ICAM_K T* cf = tb.F + L.cleaf_f + CLEAF_F * ci;
   const ICAM_K uint32_t* cu = tb.U + L.cleaf_u + CLEAF_U * ci;
@@ -1515,6 +1574,7 @@ __device__ void try_cleaf(const Tab<T>& tb, uint32_t ci, const ICAM_K float* box
   }
   if constexpr (SETTLE && sizeof(T) == 8) {
     if (!certified) {
+      settle_probe_curve(tb, ci, p, b, sk);
       settle_root(sk, lb, ci | SP_CURVE, b);
       return;
     }
