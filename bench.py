@@ -70,6 +70,11 @@ def large_solid(args, log, threads):
     log(f"c5: {part.n_faces} faces packed in {pack_s:.1f}s ({g.nodes} sub-patches and curve parts), level {list(lv.nb)} at {lv.h:.4f} mm")
     res = {"faces": part.n_faces, "pack_s": pack_s, "level": list(lv.nb), "h": lv.h}
     for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+        # the single-level autotuner (lanes a node of the nodes' search), outside the
+        # timed steps; a profile run keeps the default shape
+        tuning = None if args.profile else g.tune(lv, prec)
+        if tuning:
+            log(f"c5 tune {name}: {tuning}")
         _, _, st = g.bake(lv, prec)
         for _ in range(args.warmup):
             g.bake_device(lv, prec)
@@ -83,7 +88,8 @@ def large_solid(args, log, threads):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / args.steps
         res[name] = {"ms": dt * 1e3, "bricks": st.bricks, "band": st.band, "slots": st.slots, "settled": st.settled, "refined": st.refined,
-                     "undecided": st.undecided, "bricks_per_s": st.band / dt, "stages": {k: v / args.steps for k, v in stage_sum.items()}}
+                     "undecided": st.undecided, "bricks_per_s": st.band / dt, "stages": {k: v / args.steps for k, v in stage_sum.items()},
+                     "tuning": tuning}
         log(f"c5 bake {name}: {dt * 1e3:.2f} ms, band {st.band}, stored {st.slots}, settled {st.settled}: {st.band / dt / 1e6:.3f} M band bricks/s; "
             f"stages {res[name]['stages']}")
     res["mixed_over_fp64"] = res["fp64"]["ms"] / res["mixed"]["ms"]
@@ -166,13 +172,21 @@ def large_solid(args, log, threads):
     return res, cpu, check, useful
 
 
+def same_kernel(cell, kernel):
+    """Whether a profile table's kernel cell names `kernel`, whose trailing template
+    arguments (the nodes' search's lanes a node) may be left out: `k_search<1, float>`
+    names `k_search<1, float, 4>`."""
+    name = cell.strip("`")
+    return name == kernel or (kernel.endswith(">") and name.startswith(kernel[:-1] + ", "))
+
+
 def pmc_traffic(path, kernel):
     """HBM bytes per launch of `kernel` from a committed counter profile (its table row's
     scaled column), or None."""
     try:
         for line in open(path):
             cells = [c.strip() for c in line.split("|")]
-            if len(cells) > 5 and cells[1] == f"`{kernel}`":
+            if len(cells) > 5 and same_kernel(cells[1], kernel):
                 return float(cells[5].replace(",", ""))
     except OSError:
         return None
@@ -185,7 +199,7 @@ def pmc_valu(path, kernel):
     try:
         for line in open(path):
             cells = [c.strip() for c in line.split("|")]
-            if len(cells) > 8 and cells[1] == f"`{kernel}`":
+            if len(cells) > 8 and same_kernel(cells[1], kernel):
                 return {"fp32_tflops": float(cells[4]), "fp64_tflops": float(cells[6]), "lane_use": float(cells[7]),
                         "frac_fp32_peak": float(cells[4]) * 1e12 / PEAK_FP32}
     except OSError:

@@ -1034,7 +1034,7 @@ __device__ Best<T> search_chunks(const Dev& dv, const Tab<T>& tb, v3<T> p, bool 
 // the dense pass. Scattered points whose candidates do not fit the list are searched in
 // chunks: by this call when CH, else their block is queued for k_chunks (the chunked
 // search inlined beside the plain one took the search from two waves a SIMD to one).
-template <int S, class T, bool CH = false>
+template <int S, class T, bool CH = false, int KN = Shape<S>::K>
 __device__ __forceinline__ void search_block(const Dev& dv, const Job& j, const Res& res, long blk, SearchLds<S, T>& sm) {
   constexpr int NT = Shape<S>::NT;
   constexpr int Q = Shape<S>::Q;
@@ -1052,18 +1052,25 @@ __device__ __forceinline__ void search_block(const Dev& dv, const Job& j, const 
   const Tab<T> tb = tab_of<T>(dv);
   const T eps = sizeof(T) == 4 ? T(dv.L.eps32) : T(dv.L.eps64);
   const T tm = sizeof(T) == 4 ? T(16) * eps : T(0);
-  constexpr int K = Shape<S>::K;
+  // lanes a point: a brick's nodes take KN (the tuned lanes a node; a node's parts are
+  // split over its lanes, and only nodes searched a lane each leave their parts to the
+  // dense pass), every other source its shape's
+  static_assert(S == SRC_NODES || KN == Shape<S>::K, "only the nodes' lanes are tuned");
+  constexpr int K = S == SRC_NODES ? KN : Shape<S>::K;
+  constexpr int PPR = NT / K;  // nodes a round
+  constexpr int NRD = S == SRC_NODES ? (729 + PPR - 1) / PPR : rounds<S>();
   const bool owner = (threadIdx.x & (K - 1)) == 0;
   double nh = 0.0;
   const v3<double> nfirst = S == SRC_NODES ? node_first(j, dv, blk, nh) : mk3(0.0, 0.0, 0.0);
-  for (int m = 0; m < rounds<S>(); ++m) {
-    long i = block_point<S>(j, blk, m);
+  for (int m = 0; m < NRD; ++m) {
+    const int nn = int(threadIdx.x) / K + PPR * m;  // the node, for a brick's nodes
+    long i = S == SRC_NODES ? (nn < 729 ? blk * 729 + nn : -1) : block_point<S>(j, blk, m);
     if (K == 1 && i < 0 && !chunked) continue;
     const bool valid = i >= 0;
     double bound = 0.0;
     v3<double> p;
     if (S == SRC_NODES)
-      p = node_at(nfirst, nh, int(threadIdx.x) + NT * m), bound = nh * 1.7320508075688772;
+      p = node_at(nfirst, nh, nn), bound = nh * 1.7320508075688772;
     else
       p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
     Best<T> b;
@@ -1072,17 +1079,17 @@ __device__ __forceinline__ void search_block(const Dev& dv, const Job& j, const 
                   : search<T, NT, Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, search_bound<T>(bound, eps), sm.cand, nc, tm, eps, sm.qlb(), sm.qk());
     } else {
       b = search<T, NT, Q, K, Shape<S>::C>(tb, cv3<T>(p), valid, search_bound<T>(bound, eps), sm.cand, nc, tm, eps, sm.qlb(), sm.qk(),
-                                           S == SRC_NODES && res.items ? &res : nullptr, i);
+                                           S == SRC_NODES && K == 1 && res.items ? &res : nullptr, i);
     }
     if (!valid || !owner || b.code == DEFERRED) continue;
     finish_point<T>(dv, res, i, bound, b, eps);
   }
 }
 
-template <int S, class T>
+template <int S, class T, int KN = Shape<S>::K>
 __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res) {
   __shared__ SearchLds<S, T> sm;
-  search_block<S, T>(dv, j, res, blockIdx.x, sm);
+  search_block<S, T, false, KN>(dv, j, res, blockIdx.x, sm);
 }
 // The points of the scattered blocks k_search queued (their candidates past the list): a
 // block of four points, a wave a point (four points' box reaches far less than their whole
@@ -2164,7 +2171,7 @@ __global__ void kb_fill(FillList f) {
 // The single-precision node search at three waves a SIMD (168 registers, a few spilled):
 // 0.41 -> 0.37 ms on the NURBS-top block's 6,889 band bricks; the centres' search and
 // four waves a SIMD lose.
-template <int S, class T>
+template <int S, class T, int KN = Shape<S>::K>
 __global__ __launch_bounds__(Shape<S>::NT) __attribute__((amdgpu_waves_per_eu((S == SRC_NODES && sizeof(T) == 4) ? 3 : 1, 8))) void kb_search(BatchArgs a) {
   __shared__ SearchLds<S, T> sm;
   __shared__ uint32_t wslot;
@@ -2187,7 +2194,7 @@ __global__ __launch_bounds__(Shape<S>::NT) __attribute__((amdgpu_waves_per_eu((S
     const Dev dv = ldk(P.dv);
     const Job j = S == SRC_CENTRES ? ldk(P.jc) : ldk(P.jn);
     const Res r = S == SRC_CENTRES ? ldk(P.rc) : ldk(P.rn);
-    search_block<S, T>(dv, j, r, blk, sm);
+    search_block<S, T, false, KN>(dv, j, r, blk, sm);
     __syncthreads();
   }
 }
@@ -2482,6 +2489,12 @@ struct icam_hip {
   std::mutex mu;
   int device = 0;
   int settle_grid = 256;
+  // lanes a node in a brick's nodes' search, double and mixed precision (1, 2, 4 or 8:
+  // icam_hip_tune, ICAM_NODE_LANES). Untuned: 4 for parts of more than 65,536 sub-patches
+  // and curve parts, whose nodes each reach many (the large mixed solid's level: 1, 2, 4
+  // and 8 took 70, 53, 38 and 48 ms single-precision, 172, 128, 104 and 131 double), else
+  // 1 (the 64 parts' nodes, few parts each, lose at 4: 67 -> 92 ms baked one by one)
+  int kn[2] = {1, 1};
   hipStream_t stream = nullptr;
   Packed pk;
   Dev dv{};
@@ -2646,8 +2659,18 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
                {r.items ? 1u : 0u, mixed ? 1u : 0u, mixed ? size_t(blocks) : 0u, 2u, size_t(blocks), r.ccount ? 1u : 0u}};
     HIP_OK(launch(st, "zero", s, dim3(unsigned(std::min<long>((blocks + 255) / 256, 4096))), dim3(256), kb_fill, f));
   }
+  // a brick's nodes searched kn lanes a node (1, 2, 4 or 8)
+  auto search_kernel = [&](auto t) {
+    using T = decltype(t);
+    if constexpr (S == SRC_NODES) {
+      const int kn = h->kn[mixed ? 1 : 0];
+      return kn == 8 ? k_search<S, T, 8> : kn == 4 ? k_search<S, T, 4> : kn == 2 ? k_search<S, T, 2> : k_search<S, T, 1>;
+    } else {
+      return k_search<S, T>;
+    }
+  };
   if (mixed) {
-    HIP_OK(launch(st, names[S][0], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, float>, h->dv, j, r));
+    HIP_OK(launch(st, names[S][0], s, dim3(unsigned(blocks)), dim3(NT), search_kernel(0.f), h->dv, j, r));
     if constexpr (pts<S>())
       if (r.cqueue) HIP_OK(launch(st, "points_far64", s, dim3(gfar), dim3(256), k_far<S>, h->dv, j, r));
     if (lb) HIP_OK(launch(st, names[S][4], s, dim3(lb), dim3(256), k_leaves<S, float>, h->dv, j, r));
@@ -2658,7 +2681,7 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
     unsigned fb = unsigned(std::min<long>(blocks, 2048));
     HIP_OK(launch(st, names[S][2], s, dim3(fb), dim3(NT), k_fallback<S>, h->dv, j, r));
   } else {
-    HIP_OK(launch(st, names[S][3], s, dim3(unsigned(blocks)), dim3(NT), k_search<S, double>, h->dv, j, r));
+    HIP_OK(launch(st, names[S][3], s, dim3(unsigned(blocks)), dim3(NT), search_kernel(0.0), h->dv, j, r));
     if constexpr (pts<S>())
       if (r.cqueue) HIP_OK(launch(st, "points_far64", s, dim3(gfar), dim3(256), k_far<S>, h->dv, j, r));
     if (lb) HIP_OK(launch(st, names[S][5], s, dim3(lb), dim3(256), k_leaves<S, double>, h->dv, j, r));
@@ -2760,6 +2783,12 @@ int icam_hip_open(const icam_part_desc* part, double need, icam_hip** out) {
   }
   HIP_OK(hipStreamCreate(&h->stream));
   HIP_OK(hipEventCreateWithFlags(&h->done, hipEventDisableTiming));
+  // ICAM_NODE_LANES: lanes a node in a brick's nodes' search (1, 2, 4 or 8)
+  if (h->pk.L.n_leaves + h->pk.L.n_cleaves > 65536) h->kn[0] = h->kn[1] = 4;
+  if (const char* e = std::getenv("ICAM_NODE_LANES")) {
+    const int v = std::atoi(e);
+    if (v == 1 || v == 2 || v == 4 || v == 8) h->kn[0] = h->kn[1] = v;
+  }
   // ICAM_SEARCH_ROUNDS: the searches' best-first rounds over parts, at most (4096)
   if (const char* e = std::getenv("ICAM_SEARCH_ROUNDS")) {
     const long r = std::atol(e);
@@ -3207,6 +3236,44 @@ int icam_hip_bake(icam_hip* h, int precision, const icam_level* lv, uint32_t* ma
   return 0;
 }
 
+int icam_hip_tune(icam_hip* h, int precision, const icam_level* lv, char* report, size_t cap) {
+  if (!h || !lv || (precision != ICAM_FP64 && precision != ICAM_MIXED)) return fail("bad arguments");
+  std::lock_guard<std::mutex> g(h->mu);
+  int& kn = h->kn[precision == ICAM_MIXED ? 1 : 0];
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  std::string rep;
+  int best = kn;
+  double best_ms = 1e300;
+  for (int c : {1, 2, 4, 8}) {
+    kn = c;
+    double t = 1e300;
+    for (int r = 0; r < 3; ++r) {  // the first a warm-up
+      icam_bake_stats st{};
+      if (after_previous(h, h->stream)) return 1;
+      HIP_OK(hipEventRecord(e0, h->stream));
+      if (bake_impl(h, precision, lv, h->stream, &st)) return 1;
+      HIP_OK(hipEventRecord(e1, h->stream));
+      if (mark_end(h, h->stream)) return 1;
+      HIP_OK(hipEventSynchronize(e1));
+      float m = 0.f;
+      HIP_OK(hipEventElapsedTime(&m, e0, e1));
+      if (r > 0) t = std::min(t, double(m));
+    }
+    char line[64];
+    std::snprintf(line, sizeof line, "%slanes a node %d: %.3f ms", rep.empty() ? "" : "; ", c, t);
+    rep += line;
+    if (t < best_ms) best_ms = t, best = c;
+  }
+  kn = best;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  rep += "; chosen " + std::to_string(best);
+  if (report && cap) std::snprintf(report, cap, "%s", rep.c_str());
+  return 0;
+}
+
 int icam_hip_query_stats(icam_hip* h, icam_bake_stats* stats) {
   if (!h || !stats) return fail("bad arguments");
   std::lock_guard<std::mutex> g(h->mu);
@@ -3254,6 +3321,7 @@ struct icam_batch {
   // the launch shapes: persistent grids of the centres' search, the nodes' search
   // and the band bricks' finish (blocks)
   int gc = 2304, gn = 1024, gf = 2048;
+  int kn = 1;    // lanes a node in the nodes' search
   int gs = 256;  // the settle stages' grid
   std::string tuning;
   BatchArgs args{};
@@ -3331,12 +3399,12 @@ int batch_enqueue(icam_batch* B, hipStream_t s, Stages* st = nullptr) {
   // the band bricks' nodes
   const unsigned gl = unsigned(std::min<size_t>((B->total_cap * 729 / 8 + 1024 + 255) / 256, 2048));
   if (mixed) {
-    HIP_OK(launch(st, "nodes_search32", s, dim3(B->gn), ntn, kb_search<SRC_NODES, float>, a));
+    HIP_OK(launch(st, "nodes_search32", s, dim3(B->gn), ntn, B->kn == 8 ? kb_search<SRC_NODES, float, 8> : B->kn == 4 ? kb_search<SRC_NODES, float, 4> : B->kn == 2 ? kb_search<SRC_NODES, float, 2> : kb_search<SRC_NODES, float, 1>, a));
     HIP_OK(launch(st, "nodes_leaves32", s, dim3(gl), dim3(256), kb_leaves<float>, a));
     HIP_OK(launch(st, "nodes_refine64", s, dim3(unsigned(std::min<size_t>(std::max<size_t>(B->total_cap, 1), 8192))), dim3(256), kb_refine_bricks, a));
     HIP_OK(launch(st, "nodes_fallback64", s, dim3(B->gn), ntn, kb_redo<SRC_NODES, false>, a));
   } else {
-    HIP_OK(launch(st, "nodes_search64", s, dim3(B->gn), ntn, kb_search<SRC_NODES, double>, a));
+    HIP_OK(launch(st, "nodes_search64", s, dim3(B->gn), ntn, B->kn == 8 ? kb_search<SRC_NODES, double, 8> : B->kn == 4 ? kb_search<SRC_NODES, double, 4> : B->kn == 2 ? kb_search<SRC_NODES, double, 2> : kb_search<SRC_NODES, double, 1>, a));
     HIP_OK(launch(st, "nodes_leaves64", s, dim3(gl), dim3(256), kb_leaves<double>, a));
   }
   HIP_OK(launch(st, "nodes_mark", s, dim3(unsigned(std::min<size_t>(std::max<size_t>(B->total_cap, 1), 8192))), dim3(256), kb_mark_bricks, a));
@@ -3369,22 +3437,24 @@ int batch_time(icam_batch* B, double& ms) {
   return 0;
 }
 
-// The autotuner: the whole batch timed with each candidate grid of one persistent
-// stage at a time (the others held), the fastest kept, stage after stage.
+// The autotuner: the whole batch timed with each candidate of one knob at a time (the
+// others held), the fastest kept, knob after knob: the persistent grids of the centres'
+// search, the nodes' search and the finish (blocks a CU), then the lanes a node of the
+// nodes' search (points a wave: 64, 32, 16 or 8).
 int batch_tune(icam_batch* B, int cus) {
   double ms;
   if (batch_time(B, ms)) return 1;  // warm-up
   char line[160];
-  std::snprintf(line, sizeof line, "start gc %d gn %d gf %d: %.3f ms;", B->gc, B->gn, B->gf, ms);
+  std::snprintf(line, sizeof line, "start gc %d gn %d gf %d kn %d: %.3f ms;", B->gc, B->gn, B->gf, B->kn, ms);
   B->tuning = line;
-  int* knobs[3] = {&B->gc, &B->gn, &B->gf};
-  const char* names[3] = {"gc", "gn", "gf"};
-  const int per_cu[3][4] = {{2, 4, 9, 18}, {1, 2, 4, 8}, {2, 4, 8, 16}};
-  for (int k = 0; k < 3; ++k) {
+  int* knobs[4] = {&B->gc, &B->gn, &B->gf, &B->kn};
+  const char* names[4] = {"gc", "gn", "gf", "kn"};
+  const int cand[4][4] = {{2 * cus, 4 * cus, 9 * cus, 18 * cus}, {cus, 2 * cus, 4 * cus, 8 * cus}, {2 * cus, 4 * cus, 8 * cus, 16 * cus}, {1, 2, 4, 8}};
+  for (int k = 0; k < 4; ++k) {
     int best = *knobs[k];
     double best_ms = 1e300;
     for (int c = 0; c < 4; ++c) {
-      *knobs[k] = per_cu[k][c] * cus;
+      *knobs[k] = cand[k][c];
       double t0, t1;
       if (batch_time(B, t0) || batch_time(B, t1)) return 1;
       const double t = std::min(t0, t1);
@@ -3394,7 +3464,7 @@ int batch_tune(icam_batch* B, int cus) {
     }
     *knobs[k] = best;
   }
-  std::snprintf(line, sizeof line, " chosen gc %d gn %d gf %d", B->gc, B->gn, B->gf);
+  std::snprintf(line, sizeof line, " chosen gc %d gn %d gf %d kn %d", B->gc, B->gn, B->gf, B->kn);
   B->tuning += line;
   return 0;
 }

@@ -60,6 +60,7 @@ def lib():
         L.icam_hip_bake_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
                                            ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(BakeStats), ctypes.c_void_p]
         L.icam_hip_stage_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), _dp, ctypes.c_int]
+        L.icam_hip_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
         L.icam_hip_query_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(BakeStats)]
         L.icam_hip_batch_open.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_void_p)]
@@ -172,6 +173,14 @@ class PartGpu:
         _check(lib().icam_hip_bake_device(self.h, precision, ctypes.byref(level), ctypes.byref(dm), ctypes.byref(da),
                                           ctypes.byref(st) if st is not None else None, stream))
         return dm.value, da.value, st
+
+    def tune(self, level, precision=FP64):
+        """The single-level bake's autotuner (icam_hip_tune): lanes a node of the nodes'
+        search timed on this level, the fastest kept for later bakes in `precision`;
+        returns its report."""
+        buf = ctypes.create_string_buffer(512)
+        _check(lib().icam_hip_tune(self.h, precision, ctypes.byref(level), buf, 512))
+        return buf.value.decode()
 
     def stage_times(self):
         """Device time (ms) of each kernel stage of the last bake or query."""

@@ -194,6 +194,14 @@ int icam_hip_batch_tuning(icam_batch* b, char* buf, size_t cap);
 
 void icam_hip_batch_close(icam_batch* b);
 
+/* The autotuner of the single-level bake (no counterpart in the reference, which
+ * bakes with one fixed kernel shape): the level is baked with a brick's nodes searched
+ * 1, 2, 4 and 8 lanes a node (a wave's 64 lanes on 64, 32, 16 or 8 nodes), each timed
+ * twice after a warm-up on the handle's stream; the fastest is kept for later bakes in
+ * `precision` on this handle. report (cap bytes, may be NULL) receives each
+ * candidate's time and the choice. The level's results are left as a bake's. */
+int icam_hip_tune(icam_hip* h, int precision, const icam_level* level, char* report, size_t cap);
+
 /* Average device time (ms) of each kernel stage of the last bake or query on
  * this handle, measured with HIP events on the stream the kernels ran on:
  * names[i] (static strings) and ms[i] for i < returned count (at most cap). */

@@ -19,7 +19,7 @@ def test_header_declares_entry_points():
     names = declared("icam_hip.h", "icam_hip_")
     for n in ["icam_hip_open", "icam_hip_close", "icam_hip_last_error", "icam_hip_device_name", "icam_hip_warm", "icam_hip_points",
               "icam_hip_bricks", "icam_hip_query", "icam_hip_query_device", "icam_hip_bake", "icam_hip_bake_device", "icam_hip_stage_times",
-              "icam_hip_nodes"]:
+              "icam_hip_nodes", "icam_hip_tune"]:
         assert n in names
 
 

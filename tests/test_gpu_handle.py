@@ -128,3 +128,39 @@ def test_search_out_of_rounds_is_undecided(monkeypatch):
     assert np.abs(d[ok] - dref[ok]).max() <= 1e-9
     # an undecided value is a lower bound on the distance
     assert (d[~ok] <= np.abs(dref[~ok]) + 1e-9).all()
+
+
+@pytest.mark.parametrize("lanes", [2, 4, 8])
+@pytest.mark.parametrize("prec", PRECS)
+def test_nodes_searched_several_lanes_a_node(monkeypatch, lanes, prec):
+    """A brick's nodes searched two, four or eight lanes a node (the tuned knob; one by
+    default on these parts; ICAM_NODE_LANES sets it for a handle): the NURBS-top block's
+    coarse level and the trimmed plate's bake bit-identical to the oracle's."""
+    monkeypatch.setenv("ICAM_NODE_LANES", str(lanes))
+    for name, h in (("c2", 1.0), ("c3", 1.0)):
+        part = Part(name)
+        g = PartGpu(part, 12 * h)
+        lv = level_for(part, h, 8)
+        m_ref, a_ref, _ = Oracle(part).bake(lv)
+        m, a, st = g.bake(lv, prec)
+        assert st.undecided == 0
+        assert (m == m_ref).all() and len(a) == len(a_ref) and (a == a_ref).all()
+        g.close()
+
+
+def test_tune_keeps_the_fastest_and_bakes_alike():
+    """icam_hip_tune times the nodes' search at 1, 2, 4 and 8 lanes a node on a level and
+    keeps the fastest: its report names every candidate and the choice, and the bakes
+    after it stay bit-identical to the oracle's."""
+    part = Part("c2")
+    lv = level_for(part, 1.0, 8)
+    g = PartGpu(part, 12 * lv.h)
+    m_ref, a_ref, _ = Oracle(part).bake(lv)
+    for prec in PRECS:
+        rep = g.tune(lv, prec)
+        for k in (1, 2, 4, 8):
+            assert f"lanes a node {k}:" in rep
+        assert "chosen" in rep
+        m, a, _ = g.bake(lv, prec)
+        assert (m == m_ref).all() and len(a) == len(a_ref) and (a == a_ref).all()
+    g.close()
