@@ -240,9 +240,11 @@ def trimmed_plate_queries(args, log, threads):
             g.query_device(d_pts.data_ptr(), n, d_out.data_ptr(), d_flag.data_ptr(), None, prec, None)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / args.steps
+        stages = g.stage_times()
         outs[name] = (d_out.cpu().numpy(), d_flag.cpu().numpy())
         res[name] = {"queries_per_s": n / dt, "ms": dt * 1e3, "searched_again_fp64": st.refined, "near_trim_curve": st.unsure_trim,
-                     "near_tie": st.unsure_tie, "settled": st.settled, "undecided": int((outs[name][1] != 0).sum())}
+                     "near_tie": st.unsure_tie, "settled": st.settled, "undecided": int((outs[name][1] != 0).sum()), "stages": stages,
+                     "settle_share": stages.get("points_settle64", 0.0) / max(sum(stages.values()), 1e-9)}
         log(f"c3 queries {name}: {n / dt / 1e6:.1f} M/s, {res[name]}")
     dm, dd = outs["mixed"][0], outs["fp64"][0]
     res["mixed_vs_fp64"] = {"sign_mismatch": int(((dm < 0) != (dd < 0)).sum()), "max_rel_err": float(np.abs(dm - dd).max() / part.diagonal),
@@ -323,7 +325,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the parity check against the oracle")
-    ap.add_argument("--profile", action="store_true", help="a short run for a profiler: the headline's bake steps and a copy of known size")
+    ap.add_argument("--profile", action="store_true", help="a short run for a profiler: one leg's timed steps (--leg) and a copy of known size")
+    ap.add_argument("--leg", choices=["c5", "c2", "c3"], default="c5",
+                    help="with --profile: c5 the headline's bake, c2 the NURBS-top block's 256^3 bake (batched and single-level), "
+                         "c3 the trimmed plate's queries")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-part batched workload")
     ap.add_argument("--no-c3", action="store_true", help="skip the trimmed plate's scattered queries")
     ap.add_argument("--c3-log2", type=int, default=24, help="log2 of the trimmed plate's query count (the configuration's 2^24)")
@@ -344,9 +349,10 @@ def main():
     dev = torch.device("cuda:0")
 
     threads = len(os.sched_getaffinity(0))
-    # the headline: the large mixed solid's 1024-node level, and its queries
-    c5, c5_cpu, c5_check, c5_useful = large_solid(args, log, threads)
     if args.profile:
+        args.no_check = args.no_cpu = True
+
+    def profile_end():
         # a plain copy of known size (256 MiB read, 256 MiB written): the scale for the
         # profiler's HBM byte counters
         x = torch.ones(64 << 20, dtype=torch.float32, device=dev)
@@ -354,7 +360,17 @@ def main():
         for _ in range(3):
             y.copy_(x)
         torch.cuda.synchronize()
+
+    if args.profile and args.leg == "c3":
+        trimmed_plate_queries(args, log, threads)
+        profile_end()
         return
+    # the headline: the large mixed solid's 1024-node level, and its queries
+    if not args.profile or args.leg == "c5":
+        c5, c5_cpu, c5_check, c5_useful = large_solid(args, log, threads)
+        if args.profile:
+            profile_end()
+            return
 
     h = 0.5
     part = Part("c2")
@@ -409,6 +425,9 @@ def main():
         single[name] = dict(ms=dt * 1e3, bricks_per_s=st.band / dt, stages={k: v / args.steps for k, v in stage_sum.items()})
         log(f"bake {name} (single-level entry point): {dt * 1e3:.3f} ms/step, stages {single[name]['stages']}")
 
+    if args.profile:
+        profile_end()
+        return
     # 2^20 scattered queries on the analytic part (C1), inputs resident in HBM
     c1 = Part("c1")
     g1 = PartGpu(c1, 12 * 1.6)

@@ -1249,7 +1249,7 @@ __global__ __launch_bounds__(256) void k_refine_bricks(Dev dv, Job j, Res res, l
 // flag 3, the fallback of the mixed form; flag 4, the settle (parts past their
 // convexity certificate settled by the certified search, as the reference settles
 // them on the CPU, crates/icam-sim/src/partcache.rs:563-751).
-template <int S, bool SETTLE>
+template <int S, bool SETTLE, int RK = 4>
 __device__ __forceinline__ void redo_block(const Dev& dv, const Job& j, const Res& res, long blk, SearchLds<S, double>& sm) {
   static_assert(!SETTLE, "the settle is settle_block's");
   const uint32_t want = SETTLE ? 4u : 3u;
@@ -1262,9 +1262,11 @@ __device__ __forceinline__ void redo_block(const Dev& dv, const Job& j, const Re
   const Tab<double> tb = tab_of<double>(dv);
   if constexpr (S == SRC_NODES && !SETTLE) {
     // the few unsure nodes of a brick (near trim curves, near ties), compacted and
-    // searched RK lanes a node: one lane a node left nearly every lane of the block idle
-    // (measured on the NURBS-top block and the 64-part batch: 4 lanes beat 8 and 16)
-    constexpr int RK = 4, RQ = 8;
+    // searched RK lanes a node: one lane a node left nearly every lane of the block idle.
+    // RK follows the nodes' search: 4 where it runs 1 or 2 lanes a node (the NURBS-top
+    // block and the 64-part batch: 4 beat 8 and 16), 8 where it runs 4 or 8 (the large
+    // mixed solid's fallback 20.8 -> 15.1 ms)
+    constexpr int RQ = 8;
     if (threadIdx.x == 0) sm.count = 0;
     __syncthreads();
     for (int n = threadIdx.x; n < 729; n += NT)
@@ -1311,12 +1313,12 @@ __device__ __forceinline__ void redo_block(const Dev& dv, const Job& j, const Re
   }
 }
 
-template <int S>
+template <int S, int RK = 4>
 __global__ __launch_bounds__(Shape<S>::NT) void k_fallback(Dev dv, Job j, Res res) {
   __shared__ SearchLds<S, double> sm;
   const uint32_t total = *res.qcount;
   for (uint32_t qb = blockIdx.x; qb < total; qb += gridDim.x) {
-    redo_block<S, false>(dv, j, res, long(res.queue[qb] & 0xffffffu), sm);
+    redo_block<S, false, RK>(dv, j, res, long(res.queue[qb] & 0xffffffu), sm);
     __syncthreads();
   }
 }
@@ -2274,7 +2276,7 @@ __global__ __launch_bounds__(256) void kb_mark_bricks(BatchArgs a) {
 
 // The fallback (want 3) or the settle (want 4) over the batch's shared queue, whose
 // entries are part << 24 | block.
-template <int S, bool SETTLE>
+template <int S, bool SETTLE, int RK = 4>
 __global__ __launch_bounds__(Shape<S>::NT) void kb_redo(BatchArgs a) {
   static_assert(!SETTLE, "the batch's settle is kb_settle");
   __shared__ SearchLds<S, double> sm;
@@ -2292,7 +2294,7 @@ __global__ __launch_bounds__(Shape<S>::NT) void kb_redo(BatchArgs a) {
     const Dev dv = ldk(P.dv);
     const Job j = S == SRC_CENTRES ? ldk(P.jc) : ldk(P.jn);
     const Res r = S == SRC_CENTRES ? ldk(P.rc) : ldk(P.rn);
-    redo_block<S, SETTLE>(dv, j, r, long(e & 0xffffffu), sm);
+    redo_block<S, SETTLE, RK>(dv, j, r, long(e & 0xffffffu), sm);
     __syncthreads();
   }
 }
@@ -2679,7 +2681,8 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
     else
       HIP_OK(launch(st, names[S][1], s, dim3(unsigned((j.n + 255) / 256)), dim3(256), k_refine<S>, h->dv, j, r));
     unsigned fb = unsigned(std::min<long>(blocks, 2048));
-    HIP_OK(launch(st, names[S][2], s, dim3(fb), dim3(NT), k_fallback<S>, h->dv, j, r));
+    // (a brick's unsure nodes 8 lanes a node where its search runs 4 or more)
+    HIP_OK(launch(st, names[S][2], s, dim3(fb), dim3(NT), S == SRC_NODES && h->kn[1] >= 4 ? k_fallback<S, 8> : k_fallback<S, 4>, h->dv, j, r));
   } else {
     HIP_OK(launch(st, names[S][3], s, dim3(unsigned(blocks)), dim3(NT), search_kernel(0.0), h->dv, j, r));
     if constexpr (pts<S>())
@@ -3402,7 +3405,7 @@ int batch_enqueue(icam_batch* B, hipStream_t s, Stages* st = nullptr) {
     HIP_OK(launch(st, "nodes_search32", s, dim3(B->gn), ntn, B->kn == 8 ? kb_search<SRC_NODES, float, 8> : B->kn == 4 ? kb_search<SRC_NODES, float, 4> : B->kn == 2 ? kb_search<SRC_NODES, float, 2> : kb_search<SRC_NODES, float, 1>, a));
     HIP_OK(launch(st, "nodes_leaves32", s, dim3(gl), dim3(256), kb_leaves<float>, a));
     HIP_OK(launch(st, "nodes_refine64", s, dim3(unsigned(std::min<size_t>(std::max<size_t>(B->total_cap, 1), 8192))), dim3(256), kb_refine_bricks, a));
-    HIP_OK(launch(st, "nodes_fallback64", s, dim3(B->gn), ntn, kb_redo<SRC_NODES, false>, a));
+    HIP_OK(launch(st, "nodes_fallback64", s, dim3(B->gn), ntn, B->kn >= 4 ? kb_redo<SRC_NODES, false, 8> : kb_redo<SRC_NODES, false, 4>, a));
   } else {
     HIP_OK(launch(st, "nodes_search64", s, dim3(B->gn), ntn, B->kn == 8 ? kb_search<SRC_NODES, double, 8> : B->kn == 4 ? kb_search<SRC_NODES, double, 4> : B->kn == 2 ? kb_search<SRC_NODES, double, 2> : kb_search<SRC_NODES, double, 1>, a));
     HIP_OK(launch(st, "nodes_leaves64", s, dim3(gl), dim3(256), kb_leaves<double>, a));
