@@ -505,8 +505,10 @@ def main():
     # HBM bytes a launch of those kernels, from the committed counter profile of the
     # same workload (profiles/bake_c5_1024_pmc.md)
     pmc = os.path.join(ROOT, "profiles", "bake_c5_1024_pmc.md")
+    # (a kernel under the table's cut, such as the dense pass when the search runs several
+    # lanes a node and defers nothing, counts as none; the search and the settle must be there)
     parts = [pmc_traffic(pmc, k) for k in ("k_search<1, float>", "k_leaves<1, float>", "k_refine_bricks", "k_fallback<1>", "k_mark_settle<1>", "k_settle<1>")]
-    node_traffic = sum(parts) if all(v is not None for v in parts) else None
+    node_traffic = sum(v for v in parts if v is not None) if parts[0] is not None and parts[-1] is not None else None
     roof = None
     if c5_useful:
         flop = c5_useful["node_flop_per_band_brick"] * cm["band"]
