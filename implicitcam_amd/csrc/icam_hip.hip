@@ -1364,7 +1364,13 @@ struct SRec {               // a lane's step, for its point's lane to merge
 constexpr uint32_t SR_VALID = 1u << 8, SR_ANS = 1u << 9, SR_UNSURE = 1u << 10, SR_OVER = 1u << 11;
 template <int S> struct SettleLds {
   static constexpr int NT = SETTLE_NT, Q = Shape<S>::Q, C = Shape<S>::C;
-  static constexpr int K = S == SRC_NODES ? 1 : Shape<S>::K;  // lanes a point in the search
+  // lanes a point in the search: a centre its first search's 16 (its reach holds many
+  // parts), a brick's node or a scattered point 1 (rounds of 64 points: at 4 lanes a
+  // scattered point the pool's pieces came from a quarter as many points and three lanes
+  // of four idle, the trimmed plate's 2^22 queries' settle 72.9 against 45.6 ms; a brick's
+  // marked nodes, about 30, at 2 lanes a node took two rounds as often as not: 50.8
+  // against 44.8 ms)
+  static constexpr int K = S == SRC_CENTRES ? Shape<S>::K : 1;
   static constexpr int NR = NT / K;                            // points a round
   // root pieces a round (what workgroup memory leaves for them: a point under a pole
   // has a root in each of a hundred sub-patches around it)
