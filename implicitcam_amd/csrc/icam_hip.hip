@@ -34,6 +34,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/icam_hip.h"
@@ -1369,7 +1370,9 @@ template <int S> struct SettleLds {
   // scattered point the pool's pieces came from a quarter as many points and three lanes
   // of four idle, the trimmed plate's 2^22 queries' settle 72.9 against 45.6 ms; a brick's
   // marked nodes, about 30, at 2 lanes a node took two rounds as often as not: 50.8
-  // against 44.8 ms)
+  // against 44.8 ms) -- but scattered points whose candidates come in chunks (far from
+  // the surface, each reaching many parts) 4 (settle_block; one lane each took the large
+  // mixed solid's 2^20 queries from 0.58 to 4.2 s)
   static constexpr int K = S == SRC_CENTRES ? Shape<S>::K : 1;
   static constexpr int NR = NT / K;                            // points a round
   // root pieces a round (what workgroup memory leaves for them: a point under a pole
@@ -1414,7 +1417,6 @@ template <int S> struct SettleLds {
 template <int S>
 __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const Res& res, long blk, SettleLds<S>& sm) {
   constexpr int NT = SETTLE_NT, NP = SettleLds<S>::NP, Q = Shape<S>::Q, C = Shape<S>::C;
-  constexpr int K = SettleLds<S>::K, NR = SettleLds<S>::NR;
   const int lane = threadIdx.x;
   // the block's g-th point
   auto point_of = [&](int g) -> long {
@@ -1435,6 +1437,9 @@ __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const 
   bool chunked = false;
   const Tab<double> tb = tab_of<double>(dv);
   const SPool pool{sm.items, &sm.pcount, SPOOL};
+  // the rounds, K lanes a point in the search
+  auto settle_rounds = [&](auto kc) {
+  constexpr int K = decltype(kc)::value, NR = NT / K;
   for (int R = NR, cnt = sm.count;; R = max(1, R / 4)) {
   int again = 0;  // points to search again in smaller rounds, listed in place
   for (int base = 0; base < cnt; base += R) {
@@ -1644,6 +1649,21 @@ __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const 
   }
   cnt = again;
   if (cnt == 0) break;
+  }
+  };
+  if constexpr (pts<S>()) {
+    // scattered points far from the surface (their candidates in chunks) take 4 lanes a
+    // point, the others 1
+    __syncthreads();
+    nchunk = 0;
+    block_candidates<S, NT>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot, sm.chunks, &nchunk, gb, &rr, nullptr, 0u, sm.list,
+                            sm.count);
+    if (nchunk > 0)
+      settle_rounds(std::integral_constant<int, 4>{});
+    else
+      settle_rounds(std::integral_constant<int, 1>{});
+  } else {
+    settle_rounds(std::integral_constant<int, SettleLds<S>::K>{});
   }
 }
 
