@@ -2523,6 +2523,10 @@ struct icam_hip {
   // and 8 took 70, 53, 38 and 48 ms single-precision, 172, 128, 104 and 131 double), else
   // 1 (the 64 parts' nodes, few parts each, lose at 4: 67 -> 92 ms baked one by one)
   int kn[2] = {1, 1};
+  // the brick centres' decision in double precision in the mixed form too (icam_hip_tune):
+  // the large mixed solid's single-precision centres' search and its fallback took 4.0 +
+  // 5.1 ms against 7.7 in double, the NURBS-top block's 0.14 against 0.12 ms
+  bool c64 = true;
   hipStream_t stream = nullptr;
   Packed pk;
   Dev dv{};
@@ -3145,7 +3149,8 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   dim3 gb(unsigned((nb + WG - 1) / WG));
   // the brick centres
   Res r{};
-  if (results(h, false, nb, r, nullptr, nullptr, nullptr, mixed)) return 1;
+  const bool cmixed = mixed && !h->c64;
+  if (results(h, false, nb, r, nullptr, nullptr, nullptr, cmixed)) return 1;
   HIP_OK(hipMemsetAsync(h->counters.p, 0, 64, s));
   Job jc{};
   jc.n = nb, jc.lv = lv;
@@ -3158,7 +3163,7 @@ static int bake_impl(icam_hip* h, int precision, const icam_level* lvp, hipStrea
   HIP_OK(h->super_count.need(size_t(supers) * 4));
   jc.super_cand = h->super_cand.as<uint32_t>(), jc.super_count = h->super_count.as<uint32_t>();
   HIP_OK(launch(&st, "centres_lists", s, dim3(unsigned(supers)), dim3(256), k_super, h->dv, jc));
-  if (run_search<SRC_CENTRES>(h, mixed, jc, r, s, &st, tiles)) return 1;
+  if (run_search<SRC_CENTRES>(h, cmixed, jc, r, s, &st, tiles)) return 1;
   HIP_OK(launch(&st, "decide", s, gb, dim3(WG), k_decide_finish, h->dv, lv, nb, (const double*)r.val, (const uint32_t*)r.flag, h->dc.as<double>(),
                 h->band.as<uint8_t>(), h->sgn.as<int8_t>()));
   // signs of the bricks kept whole: along x, y, z lines, until nothing changes
@@ -3268,17 +3273,15 @@ int icam_hip_bake(icam_hip* h, int precision, const icam_level* lv, uint32_t* ma
 int icam_hip_tune(icam_hip* h, int precision, const icam_level* lv, char* report, size_t cap) {
   if (!h || !lv || (precision != ICAM_FP64 && precision != ICAM_MIXED)) return fail("bad arguments");
   std::lock_guard<std::mutex> g(h->mu);
-  int& kn = h->kn[precision == ICAM_MIXED ? 1 : 0];
+  const bool mixed = precision == ICAM_MIXED;
+  int& kn = h->kn[mixed ? 1 : 0];
   hipEvent_t e0, e1;
   HIP_OK(hipEventCreate(&e0));
   HIP_OK(hipEventCreate(&e1));
-  std::string rep;
-  int best = kn;
-  double best_ms = 1e300;
-  for (int c : {1, 2, 4, 8}) {
-    kn = c;
-    double t = 1e300;
-    for (int r = 0; r < 3; ++r) {  // the first a warm-up
+  // the level's bake timed: the least of two runs after a warm-up
+  auto time_bake = [&](double& t) -> int {
+    t = 1e300;
+    for (int r = 0; r < 3; ++r) {
       icam_bake_stats st{};
       if (after_previous(h, h->stream)) return 1;
       HIP_OK(hipEventRecord(e0, h->stream));
@@ -3290,15 +3293,36 @@ int icam_hip_tune(icam_hip* h, int precision, const icam_level* lv, char* report
       HIP_OK(hipEventElapsedTime(&m, e0, e1));
       if (r > 0) t = std::min(t, double(m));
     }
-    char line[64];
+    return 0;
+  };
+  std::string rep;
+  char line[80];
+  int best = kn;
+  double best_ms = 1e300;
+  for (int c : {1, 2, 4, 8}) {
+    kn = c;
+    double t;
+    if (time_bake(t)) return 1;
     std::snprintf(line, sizeof line, "%slanes a node %d: %.3f ms", rep.empty() ? "" : "; ", c, t);
     rep += line;
     if (t < best_ms) best_ms = t, best = c;
   }
   kn = best;
+  rep += "; chosen " + std::to_string(best);
+  if (mixed) {
+    // the centres' decision in single precision (with its double-precision fallback) or
+    // in double
+    const bool keep = h->c64;
+    h->c64 = !keep;
+    double t;
+    if (time_bake(t)) return 1;
+    std::snprintf(line, sizeof line, "; centres in %s: %.3f ms", h->c64 ? "double" : "single", t);
+    rep += line;
+    if (!(t < best_ms)) h->c64 = keep;
+    rep += h->c64 ? "; centres chosen double" : "; centres chosen single";
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  rep += "; chosen " + std::to_string(best);
   if (report && cap) std::snprintf(report, cap, "%s", rep.c_str());
   return 0;
 }
@@ -3351,6 +3375,7 @@ struct icam_batch {
   // and the band bricks' finish (blocks)
   int gc = 2304, gn = 1024, gf = 2048;
   int kn = 1;    // lanes a node in the nodes' search
+  int c64 = 0;   // the mixed form's centres' decision in double precision
   int gs = 256;  // the settle stages' grid
   std::string tuning;
   BatchArgs args{};
@@ -3414,7 +3439,7 @@ int batch_enqueue(icam_batch* B, hipStream_t s, Stages* st = nullptr) {
   const dim3 ntc(Shape<SRC_CENTRES>::NT), ntn(Shape<SRC_NODES>::NT);
   // the brick centres: the level's decision (the super-tiles' lists first)
   HIP_OK(launch(st, "centres_lists", s, dim3(unsigned(std::min<long>(std::max<long>(B->total_supers, 1), 4096))), dim3(256), kb_super, a));
-  if (mixed) {
+  if (mixed && !B->c64) {
     HIP_OK(launch(st, "centres_search32", s, dim3(B->gc), ntc, kb_search<SRC_CENTRES, float>, a));
     HIP_OK(launch(st, "centres_refine64", s, dim3(pts_c), dim3(256), kb_refine<SRC_CENTRES>, a));
     HIP_OK(launch(st, "centres_fallback64", s, dim3(B->gc), ntc, kb_redo<SRC_CENTRES, false>, a));
@@ -3469,20 +3494,23 @@ int batch_time(icam_batch* B, double& ms) {
 // The autotuner: the whole batch timed with each candidate of one knob at a time (the
 // others held), the fastest kept, knob after knob: the persistent grids of the centres'
 // search, the nodes' search and the finish (blocks a CU), then the lanes a node of the
-// nodes' search (points a wave: 64, 32, 16 or 8).
+// nodes' search (points a wave: 64, 32, 16 or 8), then, in the mixed form, the precision of the
+// centres' decision (single with its fallback, or double).
 int batch_tune(icam_batch* B, int cus) {
   double ms;
   if (batch_time(B, ms)) return 1;  // warm-up
   char line[160];
-  std::snprintf(line, sizeof line, "start gc %d gn %d gf %d kn %d: %.3f ms;", B->gc, B->gn, B->gf, B->kn, ms);
+  std::snprintf(line, sizeof line, "start gc %d gn %d gf %d kn %d c64 %d: %.3f ms;", B->gc, B->gn, B->gf, B->kn, B->c64, ms);
   B->tuning = line;
-  int* knobs[4] = {&B->gc, &B->gn, &B->gf, &B->kn};
-  const char* names[4] = {"gc", "gn", "gf", "kn"};
-  const int cand[4][4] = {{2 * cus, 4 * cus, 9 * cus, 18 * cus}, {cus, 2 * cus, 4 * cus, 8 * cus}, {2 * cus, 4 * cus, 8 * cus, 16 * cus}, {1, 2, 4, 8}};
-  for (int k = 0; k < 4; ++k) {
+  int* knobs[5] = {&B->gc, &B->gn, &B->gf, &B->kn, &B->c64};
+  const char* names[5] = {"gc", "gn", "gf", "kn", "c64"};
+  const int cand[5][4] = {{2 * cus, 4 * cus, 9 * cus, 18 * cus}, {cus, 2 * cus, 4 * cus, 8 * cus}, {2 * cus, 4 * cus, 8 * cus, 16 * cus}, {1, 2, 4, 8},
+                          {0, 1, -1, -1}};
+  const int knob_count = B->precision == ICAM_MIXED ? 5 : 4;
+  for (int k = 0; k < knob_count; ++k) {
     int best = *knobs[k];
     double best_ms = 1e300;
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < 4 && cand[k][c] >= 0; ++c) {
       *knobs[k] = cand[k][c];
       double t0, t1;
       if (batch_time(B, t0) || batch_time(B, t1)) return 1;
@@ -3493,7 +3521,7 @@ int batch_tune(icam_batch* B, int cus) {
     }
     *knobs[k] = best;
   }
-  std::snprintf(line, sizeof line, " chosen gc %d gn %d gf %d kn %d", B->gc, B->gn, B->gf, B->kn);
+  std::snprintf(line, sizeof line, " chosen gc %d gn %d gf %d kn %d c64 %d", B->gc, B->gn, B->gf, B->kn, B->c64);
   B->tuning += line;
   return 0;
 }

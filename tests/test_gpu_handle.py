@@ -149,9 +149,10 @@ def test_nodes_searched_several_lanes_a_node(monkeypatch, lanes, prec):
 
 
 def test_tune_keeps_the_fastest_and_bakes_alike():
-    """icam_hip_tune times the nodes' search at 1, 2, 4 and 8 lanes a node on a level and
-    keeps the fastest: its report names every candidate and the choice, and the bakes
-    after it stay bit-identical to the oracle's."""
+    """icam_hip_tune times the nodes' search at 1, 2, 4 and 8 lanes a node on a level, and in
+    the mixed form the centres' decision in single and in double precision, and keeps the
+    fastest: its report names every candidate and the choices, and the bakes after it stay
+    bit-identical to the oracle's."""
     part = Part("c2")
     lv = level_for(part, 1.0, 8)
     g = PartGpu(part, 12 * lv.h)
@@ -161,6 +162,7 @@ def test_tune_keeps_the_fastest_and_bakes_alike():
         for k in (1, 2, 4, 8):
             assert f"lanes a node {k}:" in rep
         assert "chosen" in rep
+        assert ("centres chosen" in rep) == (prec == MIXED)
         m, a, _ = g.bake(lv, prec)
         assert (m == m_ref).all() and len(a) == len(a_ref) and (a == a_ref).all()
     g.close()
