@@ -8,11 +8,13 @@ cell 0.393 mm: 128 x 96 x 13 bricks of 8^3 cells) within the narrow band: the de
 of every brick, the signs of the bricks kept whole, the 729 nodes of every band brick,
 the settle of nodes past their certificates, flood fill, i16 quantization and packing
 into the 365-word atlas, all on the device, in the FP32 + FP64-refine form through the
-single-level entry point (device-resident results). The plain FP64 form is timed
-beside it, and so are the same part's scattered queries (30 % uniform, 70 % within 1 mm
-of the surface), the NURBS-top block's 256^3 level through the batched evaluator, 2^20
-queries on the analytic box-with-hole part, the trimmed plate's 2^24 queries and the
-64-part batched workload.
+batched evaluator (the part's level as a batch of one: one launch of the captured graph
+a step, its shape chosen by the autotuner; device-resident results, checked equal to the
+single-level bake's). The plain FP64 form is timed beside it, and so are the
+single-level entry point's bakes of the same level, the same part's scattered queries (30
+% uniform, 70 % within 1 mm of the surface), the NURBS-top block's 256^3 level through
+the batched evaluator, 2^20 queries on the analytic box-with-hole part, the trimmed
+plate's 2^24 queries and the 64-part batched workload.
 
 Prints one JSON line (rank 0, N = 1). `value` is the headline's band bricks baked per
 second (mixed form).
@@ -59,7 +61,7 @@ def large_solid(args, log, threads):
     the parity check and the useful-work count on a window of the same level."""
     import torch
 
-    from implicitcam_amd import FP64, MIXED, Part, PartGpu, c4_level, near_surface_points, uniform_points
+    from implicitcam_amd import FP64, MIXED, Batch, Part, PartGpu, c4_level, near_surface_points, uniform_points
     from implicitcam_amd.flops import useful_flops
 
     part = Part("c5")
@@ -93,6 +95,39 @@ def large_solid(args, log, threads):
         log(f"c5 bake {name}: {dt * 1e3:.2f} ms, band {st.band}, stored {st.slots}, settled {st.settled}: {st.band / dt / 1e6:.3f} M band bricks/s; "
             f"stages {res[name]['stages']}")
     res["mixed_over_fp64"] = res["fp64"]["ms"] / res["mixed"]["ms"]
+    # the headline: the same level through the batched evaluator (one part: the captured
+    # graph of the bake's launches, persistent queue stages, autotuned shape; a profile run
+    # keeps the untuned shape), one graph launch a step; its map and atlas checked equal to
+    # the single-level bake's
+    res["batched"] = {}
+    for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+        t0 = time.time()
+        b = Batch([g], [lv], prec, tune=not args.profile)
+        open_s = time.time() - t0
+        for _ in range(max(args.warmup, 1)):
+            b.run()
+        b.sync()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            b.run()
+        b.sync()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        st = b.stats(0)
+        stages = b.stage_times()
+        same = None
+        if not args.profile:
+            mb, ab, _ = b.download(0)
+            ms_, as_, _ = g.bake(lv, prec)
+            same = bool((mb == ms_).all() and len(ab) == len(as_) and (ab == as_).all())
+        res["batched"][name] = {"ms": dt * 1e3, "bricks": st.bricks, "band": st.band, "slots": st.slots, "settled": st.settled,
+                                "refined": st.refined, "undecided": st.undecided, "bricks_per_s": st.band / dt, "stages": stages,
+                                "tuning": b.tuning, "open_and_tune_s": open_s, "equal_to_single_level": same}
+        log(f"c5 batched {name}: {dt * 1e3:.2f} ms, {st.band / dt / 1e6:.3f} M band bricks/s, equal to the single-level bake: {same}; "
+            f"{b.tuning}; stages {stages}")
+        b.close()
+    res["batched"]["mixed_over_fp64"] = res["batched"]["fp64"]["ms"] / res["batched"]["mixed"]["ms"]
     if args.profile:
         g.close()
         return res, None, None, None
@@ -495,7 +530,7 @@ def main():
     # the batched workload: 64 parts of uneven face counts and grids in one graph
     c4 = None if args.no_batch else batched_workload(args, log)
     mixed, fp64 = res["mixed"], res["fp64"]
-    cm, cf = c5["mixed"], c5["fp64"]
+    cm, cf = c5["batched"]["mixed"], c5["batched"]["fp64"]
     # the roofline of the headline's dominant work, the band nodes' search (its launches:
     # single-precision search, dense pass, last correction, fallback, settle): the useful
     # floating-point work of reading every band brick's nodes -- the FP64 evaluator's
@@ -510,7 +545,8 @@ def main():
     pmc = os.path.join(ROOT, "profiles", "bake_c5_1024_pmc.md")
     # (a kernel under the table's cut, such as the dense pass when the search runs several
     # lanes a node and defers nothing, counts as none; the search and the settle must be there)
-    parts = [pmc_traffic(pmc, k) for k in ("k_search<1, float>", "k_leaves<1, float>", "k_refine_bricks", "k_fallback<1>", "k_mark_settle<1>", "k_settle<1>")]
+    parts = [pmc_traffic(pmc, k) for k in ("kb_search<1, float>", "kb_leaves<float>", "kb_refine_bricks", "kb_redo<1, false>", "kb_mark_bricks",
+                                           "kb_settle<1>")]
     node_traffic = sum(v for v in parts if v is not None) if parts[0] is not None and parts[-1] is not None else None
     roof = None
     if c5_useful:
@@ -522,7 +558,7 @@ def main():
                 "useful_flop_per_band_brick": c5_useful["node_flop_per_band_brick"],
                 "hbm": {"algorithmic_bytes": 1464.0 * cm["slots"], "achieved_GBps": 1464.0 * cm["slots"] / (cm["ms"] / 1e3) / 1e9,
                         "peak_GBps": PEAK_HBM / 1e9, "frac": 1464.0 * cm["slots"] / (cm["ms"] / 1e3) / PEAK_HBM},
-                "valu_measured": {k: pmc_valu(os.path.join(ROOT, "profiles", "bake_c5_1024_valu.md"), k) for k in ("k_search<1, float>", "k_settle<1>")}}
+                "valu_measured": {k: pmc_valu(os.path.join(ROOT, "profiles", "bake_c5_1024_valu.md"), k) for k in ("kb_search<1, float>", "kb_settle<1>")}}
     out = {
         "metric": "SDF queries/s + bricks/s, FP32+FP64-refine vs FP64 at <=1e-6 rel err, 1 GPU",
         "value": cm["bricks_per_s"],
@@ -539,10 +575,13 @@ def main():
         "config": {"workload": "c5 large mixed solid: 400 x 300 x 40 mm plate, 1,998 faces (800 rational bicubic 32 x 32 nets, weights 0.5-2; "
                                "planes, cylinders, cones, tori), loops of 12-64 cubic spans; narrow-band bake of its 1024-node level "
                                f"(cell {c5['h']:.4f} mm, {c5['level'][0]} x {c5['level'][1]} x {c5['level'][2]} bricks of 8^3 cells), "
-                               "FP32+FP64-refine, single-level entry point, device-resident",
+                               "FP32+FP64-refine, batched evaluator (one part: one captured-graph launch a step, autotuned), "
+                               "device-resident",
                    "bricks": cm["bricks"], "band_bricks": cm["band"], "stored_bricks": cm["slots"]},
         "fp64_bricks_per_s": cf["bricks_per_s"],
-        "mixed_over_fp64": c5["mixed_over_fp64"],
+        "mixed_over_fp64": c5["batched"]["mixed_over_fp64"],
+        "single_level_entry_point": {"mixed_bricks_per_s": c5["mixed"]["bricks_per_s"], "fp64_bricks_per_s": c5["fp64"]["bricks_per_s"],
+                                     "mixed_ms": c5["mixed"]["ms"], "fp64_ms": c5["fp64"]["ms"], "mixed_over_fp64": c5["mixed_over_fp64"]},
         "queries_per_s": {"mixed": c5["queries"]["mixed"]["queries_per_s"], "fp64": c5["queries"]["fp64"]["queries_per_s"],
                           "points": c5["queries"]["points"]},
         "roofline": roof,

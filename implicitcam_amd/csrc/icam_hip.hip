@@ -2147,6 +2147,8 @@ struct BatchArgs {
   uint32_t* btotal;       // per part: band bricks its scan found
   uint32_t* stotal;       // per part: slots
   uint32_t* ctl;
+  const int* wide;        // the parts whose level is too large for kb_sign's workgroup memory
+  int nwide;
 };
 
 // The part whose range [off[p], off[p + 1]) holds w.
@@ -2380,7 +2382,36 @@ __global__ __launch_bounds__(SIGN_T) void kb_sign(BatchArgs a) {
   __shared__ int changed;
   const ICAM_K BPart& P = bpart(a, blockIdx.x);
   const Lv lv = ldk(P.lv);
+  if (sign_pitch(lv.nb[0]) * lv.nb[1] * lv.nb[2] > SIGN_LDS) return;  // kb_sign_wide's
   sign_block(lv, P.band, P.sgn, g, changed);
+}
+// The levels too large for kb_sign's workgroup memory, grid-wide: one round's sweeps along
+// axis ax of every such part's lines (blockIdx.y the part among them), as k_sign. The graph
+// runs SIGN_ROUNDS rounds with no check in between; a brick still without a sign after them
+// is counted by kb_store and fails the part's results.
+__global__ __launch_bounds__(256) void kb_sign_wide(BatchArgs a, int ax) {
+  const ICAM_K BPart& P = bpart(a, a.wide[blockIdx.y]);
+  const Lv lv = ldk(P.lv);
+  const long nb[3] = {lv.nb[0], lv.nb[1], lv.nb[2]};
+  const long a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
+  const long line = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (line >= nb[a1] * nb[a2]) return;
+  const long stride[3] = {1, nb[0], nb[0] * nb[1]};
+  const long base = (line % nb[a1]) * stride[a1] + (line / nb[a1]) * stride[a2];
+  const uint8_t* band = P.band;
+  int8_t* sgn = P.sgn;
+  for (int pass = 0; pass < 2; ++pass) {
+    int8_t carry = 0;
+    bool carry_ok = false;  // the previous brick may hand its sign on
+    for (long k = 0; k < nb[ax]; ++k) {
+      const long id = base + (pass == 0 ? k : nb[ax] - 1 - k) * stride[ax];
+      int8_t sg = sgn[id];
+      const bool nonband = band[id] == 0;
+      if (sg == 0 && carry != 0 && (carry_ok || nonband)) sgn[id] = sg = carry;
+      carry = sg;
+      carry_ok = nonband;
+    }
+  }
 }
 __global__ __launch_bounds__(SCAN_T) void kb_band_list(BatchArgs a) {
   __shared__ uint32_t s[SCAN_T];
@@ -3365,7 +3396,8 @@ struct icam_batch {
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  Buf bparts, brick_off_d, tile_off_d, super_off_d, band_off, btotal, stotal, ctl, counters, super_cand, super_count;
+  Buf bparts, brick_off_d, tile_off_d, super_off_d, band_off, btotal, stotal, ctl, counters, super_cand, super_count, wide_d;
+  long wide_lines = 0;  // the most lines along one axis of a level signed grid-wide
   Buf c_code, c_pa, c_pb, c_d32, c_info, c_val, c_flag, c_qmark, c_smark, c_queue, c_squeue;
   Buf n_code, n_pa, n_pb, n_d32, n_info, n_val, n_flag, n_qmark, n_smark, n_queue, n_squeue, items;
   Buf tile_cand, tile_count, dc, band, sgn, list, staging, cls, slot, map, atlas;
@@ -3382,7 +3414,7 @@ struct icam_batch {
   ~icam_batch() {
     if (exec) hipGraphExecDestroy(exec);
     if (graph) hipGraphDestroy(graph);
-    for (Buf* b : {&bparts, &brick_off_d, &tile_off_d, &super_off_d, &super_cand, &super_count, &band_off, &btotal, &stotal, &ctl, &counters, &c_code, &c_pa, &c_pb, &c_d32, &c_info, &c_val,
+    for (Buf* b : {&wide_d, &bparts, &brick_off_d, &tile_off_d, &super_off_d, &super_cand, &super_count, &band_off, &btotal, &stotal, &ctl, &counters, &c_code, &c_pa, &c_pb, &c_d32, &c_info, &c_val,
                    &c_flag, &c_qmark, &c_smark, &c_queue, &c_squeue, &n_code, &n_pa, &n_pb, &n_d32, &n_info, &n_val, &n_flag, &n_qmark, &n_smark,
                    &n_queue, &n_squeue, &items, &tile_cand, &tile_count, &dc, &band, &sgn, &list, &staging, &cls, &slot, &map, &atlas})
       b->free();
@@ -3448,6 +3480,10 @@ int batch_enqueue(icam_batch* B, hipStream_t s, Stages* st = nullptr) {
   }
   HIP_OK(launch(st, "decide", s, dim3(pts_c), dim3(256), kb_decide, a));
   HIP_OK(launch(st, "sign", s, dim3(B->n), dim3(SIGN_T), kb_sign, a));
+  if (a.nwide)
+    for (int round = 0; round < SIGN_ROUNDS; ++round)
+      for (int ax = 0; ax < 3; ++ax)
+        HIP_OK(launch(st, "sign", s, dim3(unsigned((B->wide_lines + 255) / 256), unsigned(a.nwide)), dim3(256), kb_sign_wide, a, ax));
   HIP_OK(launch(st, "band_list", s, dim3(B->n), dim3(SCAN_T), kb_band_list, a));
   HIP_OK(launch(st, "plan", s, dim3(1), dim3(64), kb_plan, a));
   // the band bricks' nodes
@@ -3554,7 +3590,6 @@ int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t
     lv.h = levels[i].h;
     if (!(lv.h > 0) || lv.nb[0] <= 0 || lv.nb[1] <= 0 || lv.nb[2] <= 0) return fail("bad level " + std::to_string(i));
     if (double(lv.nb[0]) * lv.nb[1] * lv.nb[2] >= double(1 << 24)) return fail("level " + std::to_string(i) + " has 2^24 bricks or more");
-    if (sign_pitch(lv.nb[0]) * lv.nb[1] * lv.nb[2] > SIGN_LDS) return fail("level " + std::to_string(i) + " is too large for the batched sign pass");
     caps[i] = band_room(parts[i], lv);
     // ICAM_BATCH_ROOM: band bricks a part has room for, at most (a smaller room makes a
     // band outgrow it, which the results then report)
@@ -3647,6 +3682,19 @@ int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t
   HIP_OK(hipMemcpy(B->tile_off_d.p, B->tile_off.data(), (n + 1) * 8, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(B->super_off_d.p, B->super_off.data(), (n + 1) * 8, hipMemcpyHostToDevice));
   BatchArgs& a = B->args;
+  // the levels too large for the sign pass in workgroup memory, signed grid-wide
+  std::vector<int> wide;
+  for (size_t k = 0; k < n; ++k) {
+    const Lv& lv = B->host[k].lv;
+    if (sign_pitch(lv.nb[0]) * lv.nb[1] * lv.nb[2] <= SIGN_LDS) continue;
+    wide.push_back(int(k));
+    for (int ax = 0; ax < 3; ++ax) B->wide_lines = std::max(B->wide_lines, long(lv.nb[(ax + 1) % 3]) * lv.nb[(ax + 2) % 3]);
+  }
+  if (!wide.empty()) {
+    HIP_OK(B->wide_d.need(wide.size() * 4));
+    HIP_OK(hipMemcpy(B->wide_d.p, wide.data(), wide.size() * 4, hipMemcpyHostToDevice));
+  }
+  a.wide = B->wide_d.as<int>(), a.nwide = int(wide.size());
   a.parts = B->bparts.as<BPart>(), a.n = int(n), a.brick_off = B->brick_off_d.as<long>(), a.tile_off = B->tile_off_d.as<long>(), a.super_off = B->super_off_d.as<long>();
   a.band_off = B->band_off.as<uint32_t>(), a.btotal = B->btotal.as<uint32_t>(), a.stotal = B->stotal.as<uint32_t>(), a.ctl = ctl;
   // launch shapes: defaults from the device's compute units, then the autotuner
@@ -3654,6 +3702,10 @@ int icam_hip_batch_open(icam_hip* const* parts, const icam_level* levels, size_t
   HIP_OK(hipGetDeviceProperties(&prop, dev));
   const int cus = std::max(prop.multiProcessorCount, 1);
   B->gc = 9 * cus, B->gn = 4 * cus, B->gf = 8 * cus;
+  // untuned lanes a node as a single handle's: 4 when a part holds more than 65,536
+  // sub-patches and curve parts (the large mixed solid alone: 120.9 ms at 1, 100.7 at 4)
+  for (const icam_hip* hp : B->parts)
+    if (hp->pk.L.n_leaves + hp->pk.L.n_cleaves > 65536) B->kn = 4;
   B->gs = settle_grid(kb_settle<SRC_NODES>, SETTLE_NT, cus);
   if (tune && batch_tune(B.get(), cus)) return 1;
   // the captured graph

@@ -192,3 +192,26 @@ def test_c5_level_window_matches_oracle(c5, prec):
     assert st.undecided == 0
     assert st.slots > 500
     assert_level(m, a, m_ref, a_ref, prec == FP64, f"c5 window precision {prec}")
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED])
+def test_batch_with_a_level_past_the_workgroup_sign_pass(prec):
+    """A batch holding the large mixed solid's 1024-node level (128 x 96 x 13 bricks, past
+    the sign pass in workgroup memory, signed grid-wide in the graph) beside the box with a
+    hole: each part's map and atlas bit-identical to its single-level bake."""
+    c5 = Part("c5")
+    lv5 = c4_level(c5, 1024)
+    g5 = PartGpu(c5, 12 * lv5.h)
+    c1 = Part("c1")
+    lv1 = level_for(c1, 1.6, 16)
+    g1 = PartGpu(c1, 12 * 1.6)
+    b = Batch([g5, g1], [lv5, lv1], prec, tune=False)
+    b.run()
+    b.sync()
+    for k, (g, lv) in enumerate(((g5, lv5), (g1, lv1))):
+        m, a, st = b.download(k)
+        m_ref, a_ref, _ = g.bake(lv, prec)
+        assert (m == m_ref).all() and len(a) == len(a_ref) and (a == a_ref).all()
+    b.close()
+    g5.close()
+    g1.close()
