@@ -128,6 +128,38 @@ def large_solid(args, log, threads):
             f"{b.tuning}; stages {stages}")
         b.close()
     res["batched"]["mixed_over_fp64"] = res["batched"]["fp64"]["ms"] / res["batched"]["mixed"]["ms"]
+    if not args.profile:
+        # the brick cache's consumer: the viewer's ray cast (icam_hip_cast) of the baked level,
+        # a 1024 x 1024 camera looking down at the plate from above one corner
+        from implicitcam_amd import cast_device
+
+        dm, da, _ = g.bake_device(lv, MIXED)
+        c = 0.5 * (part.lo + part.hi)
+        eye = c + np.array([-0.45, -0.55, 0.9]) * part.diagonal
+        fwd = (c - eye) / np.linalg.norm(c - eye)
+        right = np.cross(fwd, [0.0, 0.0, 1.0])
+        right /= np.linalg.norm(right)
+        up = np.cross(right, fwd)
+        px = (np.arange(1024) + 0.5) / 512.0 - 1.0
+        u, v = np.meshgrid(px * 0.45, px * 0.45)
+        dirs = fwd + u.reshape(-1, 1) * right + v.reshape(-1, 1) * up
+        dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+        rays = np.concatenate([np.broadcast_to(eye, dirs.shape), dirs], axis=1)
+        nr = len(rays)
+        d_rays = torch.from_numpy(np.ascontiguousarray(rays)).cuda()
+        d_out = torch.empty((nr, 4), dtype=torch.float32, device="cuda")
+        d_hit = torch.empty((nr, 2), dtype=torch.int32, device="cuda")
+        for _ in range(max(args.warmup, 1)):
+            cast_device([lv], [dm], [da], d_rays.data_ptr(), nr, d_out.data_ptr(), d_hit.data_ptr())
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            cast_device([lv], [dm], [da], d_rays.data_ptr(), nr, d_out.data_ptr(), d_hit.data_ptr())
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        hit_frac = float((d_out[:, 0] >= 0).float().mean().item())
+        res["cast"] = {"rays": nr, "ms": dt * 1e3, "rays_per_s": nr / dt, "hit_fraction": hit_frac}
+        log(f"c5 cast: {nr} rays in {dt * 1e3:.3f} ms, {nr / dt / 1e6:.1f} M rays/s, {hit_frac:.2f} of them meet the part")
     if args.profile:
         g.close()
         return res, None, None, None
@@ -580,6 +612,7 @@ def main():
                    "bricks": cm["bricks"], "band_bricks": cm["band"], "stored_bricks": cm["slots"]},
         "fp64_bricks_per_s": cf["bricks_per_s"],
         "mixed_over_fp64": c5["batched"]["mixed_over_fp64"],
+        "cast_rays_per_s": c5.get("cast", {}).get("rays_per_s"),
         "single_level_entry_point": {"mixed_bricks_per_s": c5["mixed"]["bricks_per_s"], "fp64_bricks_per_s": c5["fp64"]["bricks_per_s"],
                                      "mixed_ms": c5["mixed"]["ms"], "fp64_ms": c5["fp64"]["ms"], "mixed_over_fp64": c5["mixed_over_fp64"]},
         "queries_per_s": {"mixed": c5["queries"]["mixed"]["queries_per_s"], "fp64": c5["queries"]["fp64"]["queries_per_s"],

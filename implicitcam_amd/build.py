@@ -54,7 +54,7 @@ def build_rules(verbose=False):
 def build_hip(verbose=False):
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libicam_hip.so")
-    srcs = [os.path.join(CSRC, "icam_hip.hip"), os.path.join(CSRC, "pack.cpp")]
+    srcs = [os.path.join(CSRC, "icam_hip.hip"), os.path.join(CSRC, "cast.hip"), os.path.join(CSRC, "pack.cpp")]
     _run([HIPCC] + HIP_FLAGS + ["-o", out + ".tmp"] + srcs, verbose)
     os.replace(out + ".tmp", out)
     return out

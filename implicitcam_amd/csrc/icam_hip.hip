@@ -2200,9 +2200,11 @@ __global__ void kb_fill(FillList f) {
 // band bricks.
 // The single-precision node search at three waves a SIMD (168 registers, a few spilled):
 // 0.41 -> 0.37 ms on the NURBS-top block's 6,889 band bricks; the centres' search and
-// four waves a SIMD lose.
+// four waves a SIMD lose. At several lanes a node the registers are the compiler's choice
+// (at three waves a SIMD the large mixed solid's search at 4 lanes a node spilled 20 GB a
+// launch against the single-level kernel's 2.7 GB).
 template <int S, class T, int KN = Shape<S>::K>
-__global__ __launch_bounds__(Shape<S>::NT) __attribute__((amdgpu_waves_per_eu((S == SRC_NODES && sizeof(T) == 4) ? 3 : 1, 8))) void kb_search(BatchArgs a) {
+__global__ __launch_bounds__(Shape<S>::NT) __attribute__((amdgpu_waves_per_eu((S == SRC_NODES && sizeof(T) == 4 && KN == 1) ? 3 : 1, 8))) void kb_search(BatchArgs a) {
   __shared__ SearchLds<S, T> sm;
   __shared__ uint32_t wslot;
   const uint32_t total = S == SRC_CENTRES ? uint32_t(a.tile_off[a.n]) : a.band_off[a.n];
@@ -3808,3 +3810,6 @@ void icam_hip_batch_close(icam_batch* b) {
 }
 
 }  // extern "C"
+
+// The error channel for the library's other translation units (cast.hip).
+int icam_fail(const std::string& s) { return fail(s); }

@@ -61,6 +61,8 @@ def lib():
                                            ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(BakeStats), ctypes.c_void_p]
         L.icam_hip_stage_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), _dp, ctypes.c_int]
         L.icam_hip_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+        L.icam_hip_cast.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.icam_hip_query_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(BakeStats)]
         L.icam_hip_batch_open.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_void_p)]
@@ -79,6 +81,18 @@ def lib():
 def _check(rc):
     if rc != 0:
         raise RuntimeError(lib().icam_hip_last_error().decode())
+
+
+def cast_device(levels, d_maps, d_atlases, d_rays, n, d_out, d_hit, stream=None):
+    """The viewer's ray cast over baked levels (icam_hip_cast): `levels` icam_level records
+    finest first, their device maps and atlases (ints, as bake_device or a batch's result
+    gives them), n rays at d_rays (6 doubles each: origin, unit direction); d_out gets 4
+    floats a ray (t, -1 for none; the gradient there), d_hit 2 words (level, slot)."""
+    k = len(levels)
+    lv = (_native.Level * k)(*levels)
+    maps = (ctypes.c_void_p * k)(*d_maps)
+    atl = (ctypes.c_void_p * k)(*d_atlases)
+    _check(lib().icam_hip_cast(lv, maps, atl, k, d_rays, n, d_out, d_hit, stream))
 
 
 class PartGpu:

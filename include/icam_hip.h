@@ -202,6 +202,20 @@ void icam_hip_batch_close(icam_batch* b);
  * candidate's time and the choice. The level's results are left as a bake's. */
 int icam_hip_tune(icam_hip* h, int precision, const icam_level* level, char* report, size_t cap);
 
+/* The viewer's ray cast over baked brick levels (crates/icam-sim/src/cache.wgsl:441-503
+ * trace_ray, :335-420 walk, :270-318 march, :191-268 cell_hit), the consumer of the
+ * brick cache: n rays (origin xyz, unit direction xyz: 6 doubles a ray, device memory)
+ * against `count` levels (1 to 16, finest first), each with the device map and atlas a
+ * bake left (icam_hip_bake_device, icam_hip_batch_result). For each ray, d_out gets 4
+ * floats (t along the ray, -1 when it meets nothing; the gradient of the trilinear
+ * interpolant of the level's nodes where it meets it, or -direction at a solid brick)
+ * and d_hit 2 words (the level met, ~0 for none; the brick's slot, or ICAM_SOLID, or
+ * ICAM_HOST for a brick left to the caller, met where the ray enters it). Single
+ * precision, as the shader reads the cache. Enqueued on `stream` (NULL: the default
+ * stream); needs no handle. */
+int icam_hip_cast(const icam_level* levels, const uint32_t* const* d_maps, const uint32_t* const* d_atlases, int count, const double* d_rays, size_t n,
+                  float* d_out, uint32_t* d_hit, void* stream);
+
 /* Average device time (ms) of each kernel stage of the last bake or query on
  * this handle, measured with HIP events on the stream the kernels ran on:
  * names[i] (static strings) and ms[i] for i < returned count (at most cap). */
