@@ -1365,16 +1365,15 @@ struct SRec {               // a lane's step, for its point's lane to merge
 constexpr uint32_t SR_VALID = 1u << 8, SR_ANS = 1u << 9, SR_UNSURE = 1u << 10, SR_OVER = 1u << 11;
 template <int S> struct SettleLds {
   static constexpr int NT = SETTLE_NT, Q = Shape<S>::Q, C = Shape<S>::C;
-  // lanes a point in the search: a centre its first search's 16 (its reach holds many
-  // parts), a brick's node or a scattered point 1 (rounds of 64 points: at 4 lanes a
-  // scattered point the pool's pieces came from a quarter as many points and three lanes
-  // of four idle, the trimmed plate's 2^22 queries' settle 72.9 against 45.6 ms; a brick's
-  // marked nodes, about 30, at 2 lanes a node took two rounds as often as not: 50.8
-  // against 44.8 ms) -- but scattered points whose candidates come in chunks (far from
-  // the surface, each reaching many parts) 4 (settle_block; one lane each took the large
-  // mixed solid's 2^20 queries from 0.58 to 4.2 s)
-  static constexpr int K = S == SRC_CENTRES ? Shape<S>::K : 1;
-  static constexpr int NR = NT / K;                            // points a round
+  // lanes a point in the search, which takes a round's points NT / K at a time: a centre
+  // its first search's 16 (its reach holds many parts), a brick's node 4, a scattered point
+  // 1 -- but scattered points whose candidates come in chunks (far from the surface, each
+  // reaching many parts) 4 (settle_block; one lane each took the large mixed solid's 2^20
+  // queries from 0.58 to 4.2 s). A round holds 64 points whatever K is: rounds of 16 at 4
+  // lanes a point left the pool a quarter of the pieces and three lanes of four idle (the
+  // trimmed plate's 2^22 queries' settle 72.9 against 45.6 ms at 1 lane a point).
+  static constexpr int K = S == SRC_CENTRES ? Shape<S>::K : S == SRC_NODES ? 4 : 1;
+  static constexpr int NR = NT;                                // points a round
   // root pieces a round (what workgroup memory leaves for them: a point under a pole
   // has a root in each of a hundred sub-patches around it)
   static constexpr int ROOTS = S == SRC_NODES ? 896 : S == SRC_POINTS ? 448 : S == SRC_CENTRES ? 1024 : 512;
@@ -1439,7 +1438,7 @@ __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const 
   const SPool pool{sm.items, &sm.pcount, SPOOL};
   // the rounds, K lanes a point in the search
   auto settle_rounds = [&](auto kc) {
-  constexpr int K = decltype(kc)::value, NR = NT / K;
+  constexpr int K = decltype(kc)::value, NR = NT;
   for (int R = NR, cnt = sm.count;; R = max(1, R / 4)) {
   int again = 0;  // points to search again in smaller rounds, listed in place
   for (int base = 0; base < cnt; base += R) {
@@ -1455,31 +1454,38 @@ __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const 
     if (lane == 0) sm.rcount = 0;
     if (lane < NR) sm.roots[lane] = 0, sm.pops[lane] = 0, sm.got[lane] = 0, sm.over[lane] = 0;
     __syncthreads();
-    // the search, with the parts past their certificates handed over as root pieces
-    const int g = lane / K;
-    const bool valid = g < npt;
-    const long i = valid ? point_of(sm.list[base + g]) : -1;
-    double bound = 0.0;
-    const v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
-    const double sb = bound < 1e300 ? bound * (1.0 + 1e-12) : bound;
-    SSink sink{sm.rlb, sm.rpart, sm.rslot, &sm.rcount, SettleLds<S>::ROOTS, g, sm.over, 1e300};
-    Best<double> b;
-    if constexpr (pts<S>()) {
-      b = chunked ? search_chunks<S, double, true, NT, K>(dv, tb, p, valid, sb, sm, nchunk, gb, rr, 0.0, dv.L.eps64, &sink)
-                  : search<double, NT, Q, K, C, true>(tb, p, valid, sb, sm.cand, nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk(), nullptr, 0, &sink);
-    } else {
-      b = search<double, NT, Q, K, C, true>(tb, p, valid, sb, sm.cand, nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk(), nullptr, 0, &sink);
-    }
-    // lane g < npt holds point g's answer and its probes' bound from here on (its
-    // group's first lane has them)
-    Best<double> ob = b;
-    double oub = K > 1 ? group_min<K>(sink.ub) : sink.ub;
-    if constexpr (K > 1) {
-      const int src = (lane * K) & (NT - 1);
-      ob.d = __shfl(b.d, src), ob.code = uint32_t(__shfl(int(b.code), src)), ob.a = __shfl(b.a, src), ob.b = __shfl(b.b, src);
-      ob.sgn = __shfl(b.sgn, src), ob.lmin = __shfl(b.lmin, src), ob.d2 = __shfl(b.d2, src), ob.sgn2 = __shfl(b.sgn2, src);
-      ob.unsure = __shfl(int(b.unsure), src) != 0, ob.host = __shfl(int(b.host), src) != 0;
-      oub = __shfl(oub, src);
+    // the search, with the parts past their certificates handed over as root pieces: NT / K
+    // points at a time (a sub-round), K lanes a point; lane g < npt then holds point g's
+    // answer and its probes' bound (its group's first lane had them)
+    Best<double> ob = best_init<double>(0.0);
+    double oub = 1e300;
+    for (int sub = 0; sub < npt; sub += NT / K) {
+      const int g = sub + lane / K;
+      const bool valid = g < npt;
+      const long i = valid ? point_of(sm.list[base + g]) : -1;
+      double bound = 0.0;
+      const v3<double> p = valid ? job_point<S>(j, dv, i, bound) : mk3(0.0, 0.0, 0.0);
+      const double sb = bound < 1e300 ? bound * (1.0 + 1e-12) : bound;
+      SSink sink{sm.rlb, sm.rpart, sm.rslot, &sm.rcount, SettleLds<S>::ROOTS, valid ? g : 0, sm.over, 1e300};
+      Best<double> b;
+      if constexpr (pts<S>()) {
+        b = chunked ? search_chunks<S, double, true, NT, K>(dv, tb, p, valid, sb, sm, nchunk, gb, rr, 0.0, dv.L.eps64, &sink)
+                    : search<double, NT, Q, K, C, true>(tb, p, valid, sb, sm.cand, nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk(), nullptr, 0, &sink);
+      } else {
+        b = search<double, NT, Q, K, C, true>(tb, p, valid, sb, sm.cand, nc, 0.0, dv.L.eps64, sm.qlb(), sm.qk(), nullptr, 0, &sink);
+      }
+      const double ub = K > 1 ? group_min<K>(sink.ub) : sink.ub;
+      if constexpr (K > 1) {
+        const int src = ((lane - sub) * K) & (NT - 1);
+        Best<double> x;
+        x.d = __shfl(b.d, src), x.code = uint32_t(__shfl(int(b.code), src)), x.a = __shfl(b.a, src), x.b = __shfl(b.b, src);
+        x.sgn = __shfl(b.sgn, src), x.lmin = __shfl(b.lmin, src), x.d2 = __shfl(b.d2, src), x.sgn2 = __shfl(b.sgn2, src);
+        x.unsure = __shfl(int(b.unsure), src) != 0, x.host = __shfl(int(b.host), src) != 0;
+        const double xu = __shfl(ub, src);
+        if (lane >= sub && lane < sub + NT / K) ob = x, oub = xu;
+      } else {
+        ob = b, oub = ub;
+      }
     }
     const bool own = lane < npt;
     long oi = -1;
