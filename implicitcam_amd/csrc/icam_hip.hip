@@ -205,7 +205,8 @@ struct Res {
   uint32_t* scount;
   uint32_t* smark;
   uint32_t* cqueue;   // scattered points' blocks whose candidates did not fit their list,
-  uint32_t* ccount;   // searched in chunks by k_chunks (NULL: searched over the whole list)
+  uint32_t* ccount;   // searched again by k_far (NULL: searched over the whole list); [1] the
+                      // next of them k_far takes
   uint32_t part;      // the part's number in a batch (0 alone), in queue entries' top byte
   unsigned long long* counters;  // [0] bridged nodes, [1] points searched again in double,
                                  // [3] of them near a trim curve, [4] near-ties of opposite
@@ -1098,8 +1099,19 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res)
 // double precision to their final answers, in the mixed form too, where their records are
 // marked final so the last correction leaves them (unbounded points far from the surface,
 // each reaching many parts: on the large mixed solid's uniform points the block's chunked
-// search took 1.7 s for 2^20 queries).
+// search took 1.7 s for 2^20 queries). Blocks take the next four points from a counter:
+// their searches range from a few parts to thousands.
 constexpr uint32_t INFO_FINAL = 0x80000000u;
+// A settle mark (smark) per block: SM_QUEUED once the block is queued for the settle,
+// SM_FAR when its points were searched in chunks (k_far). A far block's undecided
+// points are settled in SETTLE_SLICES queue entries (the slice in the entry's top byte),
+// each taking the points g of the block with g % SETTLE_SLICES == slice: the search
+// before a settle round costs a far point thousands of parts, and one wave holding a
+// far block's round was the whole settle's critical path (on the large mixed solid's
+// uniform points, one round took 456 of the settle's 459 ms while the wave-time summed
+// over all rounds would have filled the GPU for about 40 ms).
+constexpr uint32_t SM_QUEUED = 1u, SM_FAR = 2u;
+constexpr int SETTLE_SLICES = 16;
 template <int S>
 __global__ __launch_bounds__(256) void k_far(Dev dv, Job j, Res res) {
   static_assert(pts<S>() && S != SRC_FAR, "scattered blocks");
@@ -1107,10 +1119,17 @@ __global__ __launch_bounds__(256) void k_far(Dev dv, Job j, Res res) {
   constexpr int SUB = (Shape<S>::NT / Shape<S>::K) / FP;         // far blocks a queued block
   static_assert(SUB * FP == Shape<S>::NT / Shape<S>::K, "a queued block splits into far blocks");
   __shared__ SearchLds<SRC_FAR, double> sm;
+  __shared__ uint32_t wq;
   const uint32_t total = *res.ccount * uint32_t(SUB);
-  for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+  for (;;) {
+    if (threadIdx.x == 0) wq = atomicAdd(res.ccount + 1, 1u);
+    __syncthreads();
+    const uint32_t q = wq;
+    if (q >= total) break;
     const long fb = long(res.cqueue[q / SUB]) * SUB + long(q % SUB);
     const long i = fb * FP + long(threadIdx.x) / Shape<SRC_FAR>::K;
+    // (the block's undecided points are settled in slices: SM_FAR)
+    if (threadIdx.x == 0 && q % SUB == 0) atomicOr(&res.smark[res.cqueue[q / SUB]], SM_FAR);
     if (res.info && (threadIdx.x & (Shape<SRC_FAR>::K - 1)) == 0 && i < j.n) res.info[i] = INFO_FINAL;
     search_block<SRC_FAR, double, true>(dv, j, res, fb, sm);
     __syncthreads();
@@ -1333,7 +1352,14 @@ __device__ __forceinline__ void mark_point(const Job& j, const Res& res, long i)
   if (!undecided) return;
   res.flag[i] = 4;
   long blk = block_of<S>(j, i);
-  if (res.smark[blk] == 0u && atomicOr(&res.smark[blk], 1u) == 0u) res.squeue[atomicAdd(res.scount, 1u)] = (res.part << 24) | uint32_t(blk);
+  if ((res.smark[blk] & SM_QUEUED) == 0u && (atomicOr(&res.smark[blk], SM_QUEUED) & SM_QUEUED) == 0u) {
+    if (pts<S>() && (res.smark[blk] & SM_FAR)) {
+      const uint32_t at = atomicAdd(res.scount, uint32_t(SETTLE_SLICES));
+      for (int k = 0; k < SETTLE_SLICES; ++k) res.squeue[at + k] = (uint32_t(k) << 24) | uint32_t(blk);
+    } else {
+      res.squeue[atomicAdd(res.scount, 1u)] = (res.part << 24) | uint32_t(blk);
+    }
+  }
 }
 
 template <int S>
@@ -1370,7 +1396,7 @@ template <int S> struct SettleLds {
   // solid's batched settle 39.6 ms at 1, 31.4 at 4, 35.9 at 8), a scattered point
   // 1 -- but scattered points whose candidates come in chunks (far from the surface, each
   // reaching many parts) 4 (settle_block; one lane each took the large mixed solid's 2^20
-  // queries from 0.58 to 4.2 s). A round holds 64 points whatever K is: rounds of 16 at 4
+  // queries from 0.58 to 4.2 s), 16 in a slice of a far block (a few points). A round holds 64 points whatever K is: rounds of 16 at 4
   // lanes a point left the pool a quarter of the pieces and three lanes of four idle (the
   // trimmed plate's 2^22 queries' settle 72.9 against 45.6 ms at 1 lane a point).
   static constexpr int K = S == SRC_CENTRES ? Shape<S>::K : S == SRC_NODES ? 4 : 1;
@@ -1415,7 +1441,7 @@ template <int S> struct SettleLds {
   __device__ uint16_t* qk() { return (uint16_t*)(scratch + Q * NT * 8); }
 };
 template <int S>
-__device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const Res& res, long blk, SettleLds<S>& sm) {
+__device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const Res& res, long blk, SettleLds<S>& sm, int slice = 0, int ns = 1) {
   constexpr int NT = SETTLE_NT, NP = SettleLds<S>::NP, Q = Shape<S>::Q, C = Shape<S>::C;
   const int lane = threadIdx.x;
   // the block's g-th point
@@ -1428,7 +1454,7 @@ __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const 
   __syncthreads();
   for (int g = lane; g < NP; g += NT) {
     const long i = point_of(g);
-    if (i >= 0 && i < j.n && res.flag[i] == 4u) sm.list[atomicAdd(&sm.count, 1)] = uint16_t(g);
+    if (i >= 0 && i < j.n && res.flag[i] == 4u && g % ns == slice) sm.list[atomicAdd(&sm.count, 1)] = uint16_t(g);
   }
   __syncthreads();
   if (sm.count == 0) return;
@@ -1660,12 +1686,14 @@ __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const 
   };
   if constexpr (pts<S>()) {
     // scattered points far from the surface (their candidates in chunks) take 4 lanes a
-    // point, the others 1
+    // point, 16 in a far block's slice (a few points), the others 1
     __syncthreads();
     nchunk = 0;
     block_candidates<S, NT>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot, sm.chunks, &nchunk, gb, &rr, nullptr, 0u, sm.list,
                             sm.count);
-    if (nchunk > 0)
+    if (nchunk > 0 && ns > 1)
+      settle_rounds(std::integral_constant<int, 16>{});
+    else if (nchunk > 0)
       settle_rounds(std::integral_constant<int, 4>{});
     else
       settle_rounds(std::integral_constant<int, 1>{});
@@ -1688,7 +1716,12 @@ __global__ __launch_bounds__(SETTLE_NT) void k_settle(Dev dv, Job j, Res res) {
     __syncthreads();
     const uint32_t qb = wslot;
     if (qb >= total) break;
-    settle_block<S>(dv, j, res, long(res.squeue[qb] & 0xffffffu), sm);
+    const uint32_t e = res.squeue[qb];
+    const long blk = long(e & 0xffffffu);
+    if (pts<S>() && (res.smark[blk] & SM_FAR))
+      settle_block<S>(dv, j, res, blk, sm, int(e >> 24), SETTLE_SLICES);
+    else
+      settle_block<S>(dv, j, res, blk, sm);
     __syncthreads();
   }
 }
@@ -2675,7 +2708,7 @@ int results(icam_hip* h, bool nodes, long n, Res& r, double* val, uint32_t* flag
     HIP_OK(B.qmark.need(m * 4));
   }
   HIP_OK(qcount.need(16));
-  HIP_OK(B.squeue.need(m * 4));
+  HIP_OK(B.squeue.need((m + SETTLE_SLICES) * 4));
   HIP_OK(B.smark.need(m * 4));
   HIP_OK(B.scount.need(16));
   HIP_OK(h->counters.need(64));
@@ -2728,7 +2761,7 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
     // the queues' counts and marks, cleared by one fill launch (no hipMemsetAsync: a
     // caller may capture these launches in a graph)
     FillList f{{r.items ? r.icount : nullptr, mixed ? r.qcount : nullptr, mixed ? r.qmark : nullptr, r.scount, r.smark, r.ccount},
-               {r.items ? 1u : 0u, mixed ? 1u : 0u, mixed ? size_t(blocks) : 0u, 2u, size_t(blocks), r.ccount ? 1u : 0u}};
+               {r.items ? 1u : 0u, mixed ? 1u : 0u, mixed ? size_t(blocks) : 0u, 2u, size_t(blocks), r.ccount ? 2u : 0u}};
     HIP_OK(launch(st, "zero", s, dim3(unsigned(std::min<long>((blocks + 255) / 256, 4096))), dim3(256), kb_fill, f));
   }
   // a brick's nodes searched kn lanes a node (1, 2, 4 or 8)
@@ -2973,7 +3006,7 @@ static int query_bufs(icam_hip* h, size_t n, bool mixed, bool order, bool feat, 
     q.queue = B.queue.as<uint32_t>(), q.qmark = B.qmark.as<uint32_t>();
   }
   HIP_OK(B.qcount.need(16));
-  HIP_OK(B.squeue.need(m * 4));
+  HIP_OK(B.squeue.need((m + SETTLE_SLICES) * 4));
   HIP_OK(B.smark.need(m * 4));
   HIP_OK(B.scount.need(16));
   HIP_OK(B.cqueue.need(m * 4));
@@ -3007,7 +3040,7 @@ static size_t query_carve(char* base, size_t cap, size_t n, bool mixed, bool ord
     q.code = (uint32_t*)take(m * 4), q.pa = (float*)take(m * 4), q.pb = (float*)take(m * 4), q.d32 = (float*)take(m * 4), q.info = (uint32_t*)take(m * 4);
     q.queue = (uint32_t*)take(blk * 4), q.qmark = (uint32_t*)take(blk * 4);
   }
-  q.qcount = (uint32_t*)take(16), q.squeue = (uint32_t*)take(blk * 4), q.smark = (uint32_t*)take(blk * 4), q.scount = (uint32_t*)take(16);
+  q.qcount = (uint32_t*)take(16), q.squeue = (uint32_t*)take(blk * 4 * SETTLE_SLICES), q.smark = (uint32_t*)take(blk * 4), q.scount = (uint32_t*)take(16);
   q.cqueue = (uint32_t*)take(blk * 4), q.ccount = (uint32_t*)take(16);
   q.counters = (unsigned long long*)take(64);
   return at;
