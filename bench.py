@@ -182,7 +182,11 @@ def large_solid(args, log, threads):
         dt = time.perf_counter() - t
         st = g.query_stats()
         qout[name] = (d_out.cpu().numpy(), d_flag.cpu().numpy())
-        res["queries"][name] = {"queries_per_s": n / dt, "ms": dt * 1e3, "settled": st.settled, "undecided": int((qout[name][1] != 0).sum())}
+        # the stages' device times, from one more call through the host-buffer entry point
+        g.query(pts, prec)
+        stages = {key: round(val, 3) for key, val in g.stage_times().items()}
+        res["queries"][name] = {"queries_per_s": n / dt, "ms": dt * 1e3, "settled": st.settled, "undecided": int((qout[name][1] != 0).sum()),
+                                "stages": stages}
         log(f"c5 queries {name}: {n / dt / 1e6:.2f} M/s, {res['queries'][name]}")
     dm, dd = qout["mixed"][0], qout["fp64"][0]
     res["queries"]["mixed_vs_fp64"] = {"sign_mismatch": int(((dm < 0) != (dd < 0)).sum()), "max_rel_err": float(np.abs(dm - dd).max() / part.diagonal)}
@@ -314,7 +318,7 @@ def trimmed_plate_queries(args, log, threads):
         stages = g.stage_times()
         res[name] = {"queries_per_s": n / dt, "ms": dt * 1e3, "searched_again_fp64": st.refined, "near_trim_curve": st.unsure_trim,
                      "near_tie": st.unsure_tie, "settled": st.settled, "undecided": int((outs[name][1] != 0).sum()), "stages": stages,
-                     "settle_share": stages.get("points_settle64", 0.0) / max(sum(stages.values()), 1e-9)}
+                     "settle_share": (stages.get("points_settle64", 0.0) + stages.get("points_settle_far64", 0.0)) / max(sum(stages.values()), 1e-9)}
         log(f"c3 queries {name}: {n / dt / 1e6:.1f} M/s, {res[name]}")
     dm, dd = outs["mixed"][0], outs["fp64"][0]
     res["mixed_vs_fp64"] = {"sign_mismatch": int(((dm < 0) != (dd < 0)).sum()), "max_rel_err": float(np.abs(dm - dd).max() / part.diagonal),

@@ -1104,14 +1104,17 @@ __global__ __launch_bounds__(Shape<S>::NT) void k_search(Dev dv, Job j, Res res)
 constexpr uint32_t INFO_FINAL = 0x80000000u;
 // A settle mark (smark) per block: SM_QUEUED once the block is queued for the settle,
 // SM_FAR when its points were searched in chunks (k_far). A far block's undecided
-// points are settled in SETTLE_SLICES queue entries (the slice in the entry's top byte),
-// each taking the points g of the block with g % SETTLE_SLICES == slice: the search
-// before a settle round costs a far point thousands of parts, and one wave holding a
-// far block's round was the whole settle's critical path (on the large mixed solid's
-// uniform points, one round took 456 of the settle's 459 ms while the wave-time summed
-// over all rounds would have filled the GPU for about 40 ms).
+// points are settled by k_settle_far in SETTLE_SLICES queue entries (the slice in the
+// entry's top byte), each taking the points g of the block with g % SETTLE_SLICES ==
+// slice: the search before a settle round costs a far point thousands of parts, and one
+// wave holding a far block's round was the whole settle's critical path (on the large
+// mixed solid's uniform points, one round took 456 of the settle's 459 ms while the
+// wave-time summed over all rounds would have filled the GPU for about 40 ms). The
+// slices' entries follow the other blocks' in the queue, from entry `blocks` on, counted
+// by scount[2] (scount[0] the others, [1] and [3] the kernels' next entries).
 constexpr uint32_t SM_QUEUED = 1u, SM_FAR = 2u;
 constexpr int SETTLE_SLICES = 16;
+template <int S> __device__ inline long settle_blocks(const Job& j) { return (j.n + Shape<S>::NT / Shape<S>::K - 1) / (Shape<S>::NT / Shape<S>::K); }
 template <int S>
 __global__ __launch_bounds__(256) void k_far(Dev dv, Job j, Res res) {
   static_assert(pts<S>() && S != SRC_FAR, "scattered blocks");
@@ -1354,7 +1357,7 @@ __device__ __forceinline__ void mark_point(const Job& j, const Res& res, long i)
   long blk = block_of<S>(j, i);
   if ((res.smark[blk] & SM_QUEUED) == 0u && (atomicOr(&res.smark[blk], SM_QUEUED) & SM_QUEUED) == 0u) {
     if (pts<S>() && (res.smark[blk] & SM_FAR)) {
-      const uint32_t at = atomicAdd(res.scount, uint32_t(SETTLE_SLICES));
+      const size_t at = size_t(settle_blocks<S>(j)) + atomicAdd(res.scount + 2, uint32_t(SETTLE_SLICES));
       for (int k = 0; k < SETTLE_SLICES; ++k) res.squeue[at + k] = (uint32_t(k) << 24) | uint32_t(blk);
     } else {
       res.squeue[atomicAdd(res.scount, 1u)] = (res.part << 24) | uint32_t(blk);
@@ -1440,7 +1443,7 @@ template <int S> struct SettleLds {
   __device__ double* qlb() { return (double*)scratch; }
   __device__ uint16_t* qk() { return (uint16_t*)(scratch + Q * NT * 8); }
 };
-template <int S>
+template <int S, bool FAR = false>
 __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const Res& res, long blk, SettleLds<S>& sm, int slice = 0, int ns = 1) {
   constexpr int NT = SETTLE_NT, NP = SettleLds<S>::NP, Q = Shape<S>::Q, C = Shape<S>::C;
   const int lane = threadIdx.x;
@@ -1691,12 +1694,17 @@ __device__ __forceinline__ void settle_block(const Dev& dv, const Job& j, const 
     nchunk = 0;
     block_candidates<S, NT>(dv, j, blk, sm.cand, sm.tmp(), sm.key(), sm.red, sm.wave_tot, sm.chunks, &nchunk, gb, &rr, nullptr, 0u, sm.list,
                             sm.count);
-    if (nchunk > 0 && ns > 1)
-      settle_rounds(std::integral_constant<int, 16>{});
-    else if (nchunk > 0)
-      settle_rounds(std::integral_constant<int, 4>{});
-    else
-      settle_rounds(std::integral_constant<int, 1>{});
+    if constexpr (FAR) {
+      if (nchunk > 0)
+        settle_rounds(std::integral_constant<int, 16>{});
+      else
+        settle_rounds(std::integral_constant<int, 1>{});
+    } else {
+      if (nchunk > 0)
+        settle_rounds(std::integral_constant<int, 4>{});
+      else
+        settle_rounds(std::integral_constant<int, 1>{});
+    }
   } else {
     settle_rounds(std::integral_constant<int, SettleLds<S>::K>{});
   }
@@ -1716,12 +1724,27 @@ __global__ __launch_bounds__(SETTLE_NT) void k_settle(Dev dv, Job j, Res res) {
     __syncthreads();
     const uint32_t qb = wslot;
     if (qb >= total) break;
-    const uint32_t e = res.squeue[qb];
-    const long blk = long(e & 0xffffffu);
-    if (pts<S>() && (res.smark[blk] & SM_FAR))
-      settle_block<S>(dv, j, res, blk, sm, int(e >> 24), SETTLE_SLICES);
-    else
-      settle_block<S>(dv, j, res, blk, sm);
+    settle_block<S>(dv, j, res, long(res.squeue[qb] & 0xffffffu), sm);
+    __syncthreads();
+  }
+}
+// The slices of far blocks (SM_FAR), in a kernel of their own: their rounds' 16 lanes a
+// point in the kernel of the other blocks took its registers past what those need (the
+// trimmed plate's 2^24 queries' settle 143 -> 162 ms).
+template <int S>
+__global__ __launch_bounds__(SETTLE_NT) void k_settle_far(Dev dv, Job j, Res res) {
+  static_assert(pts<S>(), "scattered points");
+  __shared__ SettleLds<S> sm;
+  __shared__ uint32_t wslot;
+  const uint32_t total = res.scount[2];
+  const size_t base = size_t(settle_blocks<S>(j));
+  for (;;) {
+    if (threadIdx.x == 0) wslot = atomicAdd(res.scount + 3, 1u);
+    __syncthreads();
+    const uint32_t qb = wslot;
+    if (qb >= total) break;
+    const uint32_t e = res.squeue[base + qb];
+    settle_block<S, true>(dv, j, res, long(e & 0xffffffu), sm, int(e >> 24), SETTLE_SLICES);
     __syncthreads();
   }
 }
@@ -2761,7 +2784,7 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
     // the queues' counts and marks, cleared by one fill launch (no hipMemsetAsync: a
     // caller may capture these launches in a graph)
     FillList f{{r.items ? r.icount : nullptr, mixed ? r.qcount : nullptr, mixed ? r.qmark : nullptr, r.scount, r.smark, r.ccount},
-               {r.items ? 1u : 0u, mixed ? 1u : 0u, mixed ? size_t(blocks) : 0u, 2u, size_t(blocks), r.ccount ? 2u : 0u}};
+               {r.items ? 1u : 0u, mixed ? 1u : 0u, mixed ? size_t(blocks) : 0u, 4u, size_t(blocks), r.ccount ? 2u : 0u}};
     HIP_OK(launch(st, "zero", s, dim3(unsigned(std::min<long>((blocks + 255) / 256, 4096))), dim3(256), kb_fill, f));
   }
   // a brick's nodes searched kn lanes a node (1, 2, 4 or 8)
@@ -2797,6 +2820,8 @@ int run_search(icam_hip* h, bool mixed, const Job& j, Res& r, hipStream_t s, Sta
   if (S == SRC_CENTRES) return 0;
   HIP_OK(launch(st, names[S][6], s, dim3(unsigned((j.n + 255) / 256)), dim3(256), k_mark_settle<S>, j, r));
   HIP_OK(launch(st, names[S][7], s, dim3(unsigned(std::min<long>(blocks, h->settle_grid))), dim3(SETTLE_NT), k_settle<S>, h->dv, j, r));
+  if constexpr (pts<S>())
+    if (r.cqueue) HIP_OK(launch(st, "points_settle_far64", s, dim3(unsigned(h->settle_grid)), dim3(SETTLE_NT), k_settle_far<S>, h->dv, j, r));
   return 0;
 }
 
@@ -3040,7 +3065,7 @@ static size_t query_carve(char* base, size_t cap, size_t n, bool mixed, bool ord
     q.code = (uint32_t*)take(m * 4), q.pa = (float*)take(m * 4), q.pb = (float*)take(m * 4), q.d32 = (float*)take(m * 4), q.info = (uint32_t*)take(m * 4);
     q.queue = (uint32_t*)take(blk * 4), q.qmark = (uint32_t*)take(blk * 4);
   }
-  q.qcount = (uint32_t*)take(16), q.squeue = (uint32_t*)take(blk * 4 * SETTLE_SLICES), q.smark = (uint32_t*)take(blk * 4), q.scount = (uint32_t*)take(16);
+  q.qcount = (uint32_t*)take(16), q.squeue = (uint32_t*)take(blk * 4 * (SETTLE_SLICES + 1)), q.smark = (uint32_t*)take(blk * 4), q.scount = (uint32_t*)take(16);
   q.cqueue = (uint32_t*)take(blk * 4), q.ccount = (uint32_t*)take(16);
   q.counters = (unsigned long long*)take(64);
   return at;
