@@ -1173,7 +1173,12 @@ __global__ __launch_bounds__(256) void k_leaves(Dev dv, Job j, Res res) {
 template <int S>
 __device__ __forceinline__ bool refine_head(const Dev& dv, const Job& j, const Res& res, long i, uint32_t info) {
   if (info & INFO_FINAL) return false;  // searched in double precision already (k_far)
-  const bool unsure = info & 4u;
+  // an unsure point that is undecided too (a part past its certificate lies under its
+  // answer) goes to the settle, whose search in double precision starts afresh: the
+  // fallback's answer would be searched again there (the large mixed solid's bake: 413 k
+  // unsure nodes, most of them beside the pockets' rounded corners the settle reads);
+  // a brick centre is not settled, so it takes the fallback
+  const bool unsure = (info & 4u) && !(S != SRC_CENTRES && (info & 8u));
   wave_count(&res.counters[1], unsure);
   wave_count(&res.counters[3], unsure && (info & 16u));
   wave_count(&res.counters[4], unsure && (info & 32u));
@@ -1192,6 +1197,8 @@ __device__ __forceinline__ bool refine_head(const Dev& dv, const Job& j, const R
   uint32_t flag;
   if (!undecided) {
     val = bound, flag = 1;
+  } else if (info & 4u) {
+    val = d, flag = 2;  // unsure too: the settle decides, past the bound as well
   } else {
     val = d >= bound ? bound : d;
     flag = d >= bound ? 1 : 2;
