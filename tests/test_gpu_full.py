@@ -149,6 +149,22 @@ def test_c5_sample_matches_oracle(c5, prec):
     assert np.abs(d - dref).max() <= (1e-9 if prec == FP64 else 1e-6 * part.diagonal)
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_c5_far_points_match_oracle(c5, prec):
+    """2^13 points uniform in the box grown by 5 mm, most of them millimetres from the
+    surface: their blocks' candidates come in chunks (k_far), and those left undecided
+    past their parts' certificates are settled in slices of their blocks (k_settle_far).
+    Every answer against the oracle's."""
+    part, o, g = c5
+    pts = uniform_points(0x51ED27, 1 << 13, part.lo - 5, part.hi + 5)
+    dref, _, _ = o.sdf(pts)
+    d, flags, _ = g.query(pts, prec)
+    assert (flags == 0).all()
+    assert g.query_stats().settled > 0
+    assert ((d < 0) == (dref < 0)).all()
+    assert np.abs(d - dref).max() <= (1e-9 if prec == FP64 else 1e-6 * part.diagonal)
+
+
 def test_c5_queries_by_properties(c5):
     """2^22 queries of the workload's mix in both precisions: every point decided, equal
     signs, distances within 1e-6 of the diagonal of each other, |d| no larger than the
