@@ -1406,9 +1406,10 @@ template <int S> struct SettleLds {
   // solid's batched settle 39.6 ms at 1, 31.4 at 4, 35.9 at 8), a scattered point
   // 1 -- but scattered points whose candidates come in chunks (far from the surface, each
   // reaching many parts) 4 (settle_block; one lane each took the large mixed solid's 2^20
-  // queries from 0.58 to 4.2 s), 16 in a slice of a far block (a few points). A round holds 64 points whatever K is: rounds of 16 at 4
-  // lanes a point left the pool a quarter of the pieces and three lanes of four idle (the
-  // trimmed plate's 2^22 queries' settle 72.9 against 45.6 ms at 1 lane a point).
+  // queries from 0.58 to 4.2 s), 16 in a slice of a far block (a few points). A round
+  // holds 64 points whatever K is: rounds of 16 at 4 lanes a point left the pool a quarter
+  // of the pieces and three lanes of four idle (the trimmed plate's 2^22 queries' settle
+  // 72.9 against 45.6 ms at 1 lane a point).
   static constexpr int K = S == SRC_CENTRES ? Shape<S>::K : S == SRC_NODES ? 4 : 1;
   static constexpr int NR = NT;                                // points a round
   // root pieces a round (what workgroup memory leaves for them: a point under a pole
@@ -1735,9 +1736,9 @@ __global__ __launch_bounds__(SETTLE_NT) void k_settle(Dev dv, Job j, Res res) {
     __syncthreads();
   }
 }
-// The slices of far blocks (SM_FAR), in a kernel of their own: their rounds' 16 lanes a
-// point in the kernel of the other blocks took its registers past what those need (the
-// trimmed plate's 2^24 queries' settle 143 -> 162 ms).
+// The slices of far blocks (SM_FAR), in a kernel of their own: their rounds at 16 lanes a
+// point, compiled into the kernel of the other blocks, slowed those (the trimmed plate's
+// 2^24 queries' settle 143 -> 162 ms, with a millisecond of far points).
 template <int S>
 __global__ __launch_bounds__(SETTLE_NT) void k_settle_far(Dev dv, Job j, Res res) {
   static_assert(pts<S>(), "scattered points");
